@@ -1,0 +1,249 @@
+// C-ABI entry points of libstokes_hip.so (declared in include/erosion_stokes.h).
+#include <cstring>
+#include <exception>
+
+#include "../../include/erosion_stokes.h"
+#include "common.hpp"
+
+namespace stokes {
+
+Context& ctx() {
+    static Context* c = nullptr;
+    if (!c) {
+        c = new Context();
+        STOKES_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    }
+    return *c;
+}
+
+}  // namespace stokes
+
+using stokes::Context;
+
+namespace {
+
+int record_error(const char* where, const std::string& msg) {
+    std::string m = std::string(where) + ": " + msg;
+    try {
+        Context& c = stokes::ctx();
+        c.status = -1;
+        c.last_error = m;
+    } catch (...) {
+    }
+    std::fprintf(stderr, "libstokes_hip: %s\n", m.c_str());
+    return -1;
+}
+
+void check_sizes(int64_t n, int64_t m, int64_t no) {
+    if (n < 2 || m < 0 || no < 4)
+        throw stokes::Error("invalid sizes npts=" + std::to_string(n) + " nbods=" +
+                            std::to_string(m) + " nouter=" + std::to_string(no));
+    const int64_t nunk = 2 * no + 2 * n * m + 3 * m;
+    if (nunk > (int64_t{1} << 30)) throw stokes::Error("problem too large for 32-bit indexing");
+}
+
+}  // namespace
+
+extern "C" {
+
+void stokessolver_(const int64_t* npts, const int64_t* nbods, const int64_t* nouter,
+                   const int64_t* ifmm, const int64_t* ibary, const int64_t* ibc,
+                   const int64_t* maxl, const double* xx, const double* yy, double* den,
+                   int64_t* nits) {
+    try {
+        if (!npts || !nbods || !nouter || !ifmm || !ibary || !ibc || !maxl || !den || !nits)
+            throw stokes::Error("null argument");
+        *nits = 0;
+        check_sizes(*npts, *nbods, *nouter);
+        if (*nbods > 0 && (!xx || !yy)) throw stokes::Error("null body coordinates");
+        if (*ibary == 1 && *ifmm == 0)
+            throw stokes::Error(
+                "ifmm=0 with ibary=1 (all-pairs barycentric matvec_DLP_bary) is not provided");
+        if (*ibary != 0 && *ibary != 1) throw stokes::Error("ibary must be 0 or 1");
+        Context& c = stokes::ctx();
+        c.status = 0;
+        c.last_error.clear();
+        const int n = static_cast<int>(*npts), m = static_cast<int>(*nbods),
+                  no = static_cast<int>(*nouter);
+        stokes::build_geometry(c, n, m, no, xx, yy);
+        stokes::build_near_list(c, static_cast<int>(*ibary));
+        long long it = stokes::gmres_solve(c, static_cast<int>(*ibary), static_cast<int>(*ibc),
+                                           static_cast<int>(*maxl), c.tol, den);
+        *nits = it;
+        if (c.status == 1)
+            std::fprintf(stderr,
+                         "libstokes_hip: GMRES reached maxl=%lld with relative residual %.3e\n",
+                         static_cast<long long>(*maxl), c.last_implicit_relres);
+        else if (c.status == 2)
+            std::fprintf(stderr, "libstokes_hip: GMRES breakdown after %lld iterations\n", it);
+    } catch (const std::exception& e) {
+        record_error("stokessolver_", e.what());
+    }
+}
+
+void computeshearstress_(const int64_t* npts, const int64_t* nbods, const int64_t* nouter,
+                         const double* xx, const double* yy, const double* den,
+                         const int64_t* ibary, double* tau) {
+    try {
+        if (!npts || !nbods || !nouter || !den || !ibary || !tau)
+            throw stokes::Error("null argument");
+        check_sizes(*npts, *nbods, *nouter);
+        const int n = static_cast<int>(*npts), m = static_cast<int>(*nbods),
+                  no = static_cast<int>(*nouter);
+        std::memset(tau, 0, sizeof(double) * static_cast<size_t>(n) * static_cast<size_t>(m));
+        if (m == 0) return;
+        if (!xx || !yy) throw stokes::Error("null body coordinates");
+        if (*ibary != 1)
+            throw stokes::Error("ibary=0 (deformation_on_boundary_trap) is not provided");
+        Context& c = stokes::ctx();
+        c.status = 0;
+        c.last_error.clear();
+        stokes::build_geometry(c, n, m, no, xx, yy);
+        const size_t nunk = static_cast<size_t>(c.g.nunk);
+        const size_t nbp = static_cast<size_t>(n) * static_cast<size_t>(m);
+        c.rvec.ensure(std::max(nunk, c.rvec.cap));
+        c.tau.ensure(nbp);
+        STOKES_HIP(hipMemcpyAsync(c.rvec.p, den, nunk * sizeof(double), hipMemcpyHostToDevice,
+                                  c.stream));
+        stokes::shear_stress(c, c.rvec.p, c.tau.p);
+        STOKES_HIP(hipMemcpyAsync(tau, c.tau.p, nbp * sizeof(double), hipMemcpyDeviceToHost,
+                                  c.stream));
+        STOKES_HIP(hipStreamSynchronize(c.stream));
+    } catch (const std::exception& e) {
+        record_error("computeshearstress_", e.what());
+    }
+}
+
+int erosion_set_gmres_tol(double tol) {
+    try {
+        if (!(tol > 0.0)) return -1;
+        stokes::ctx().tol = tol;
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_set_gmres_tol", e.what());
+    }
+}
+
+double erosion_get_gmres_tol(void) {
+    try {
+        return stokes::ctx().tol;
+    } catch (...) {
+        return -1.0;
+    }
+}
+
+int erosion_last_status(void) {
+    try {
+        return stokes::ctx().status;
+    } catch (...) {
+        return -1;
+    }
+}
+
+const char* erosion_last_error(void) {
+    try {
+        return stokes::ctx().last_error.c_str();
+    } catch (...) {
+        return "library context unavailable";
+    }
+}
+
+int erosion_last_solve_info(double* info) {
+    if (!info) return -1;
+    try {
+        Context& c = stokes::ctx();
+        info[0] = static_cast<double>(c.last_iters);
+        info[1] = c.last_implicit_relres;
+        info[2] = c.last_true_relres;
+        info[3] = c.last_seconds;
+        info[4] = static_cast<double>(c.last_reorth);
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_last_solve_info", e.what());
+    }
+}
+
+int erosion_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t ibary,
+                   const double* xx, const double* yy, const double* den, double* vel) {
+    try {
+        check_sizes(npts, nbods, nouter);
+        if (!den || !vel || (nbods > 0 && (!xx || !yy))) throw stokes::Error("null argument");
+        if (ibary != 0 && ibary != 1) throw stokes::Error("ibary must be 0 or 1");
+        Context& c = stokes::ctx();
+        c.status = 0;
+        const int n = static_cast<int>(npts), m = static_cast<int>(nbods),
+                  no = static_cast<int>(nouter);
+        stokes::build_geometry(c, n, m, no, xx, yy);
+        stokes::build_near_list(c, static_cast<int>(ibary));
+        const size_t nunk = static_cast<size_t>(c.g.nunk);
+        c.xvec.ensure(std::max(nunk, c.xvec.cap));
+        c.rvec.ensure(std::max(nunk, c.rvec.cap));
+        STOKES_HIP(hipMemcpyAsync(c.xvec.p, den, nunk * sizeof(double), hipMemcpyHostToDevice,
+                                  c.stream));
+        stokes::apply_operator(c, c.xvec.p, c.rvec.p, static_cast<int>(ibary));
+        STOKES_HIP(hipMemcpyAsync(vel, c.rvec.p, nunk * sizeof(double), hipMemcpyDeviceToHost,
+                                  c.stream));
+        STOKES_HIP(hipStreamSynchronize(c.stream));
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_matvec", e.what());
+    }
+}
+
+int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t ibary,
+                        const double* xx, const double* yy, const double* den, int64_t reps,
+                        double* out) {
+    try {
+        check_sizes(npts, nbods, nouter);
+        if (!den || !out || reps < 1 || (nbods > 0 && (!xx || !yy)))
+            throw stokes::Error("invalid argument");
+        Context& c = stokes::ctx();
+        c.status = 0;
+        const int n = static_cast<int>(npts), m = static_cast<int>(nbods),
+                  no = static_cast<int>(nouter);
+        stokes::build_geometry(c, n, m, no, xx, yy);
+        stokes::build_near_list(c, static_cast<int>(ibary));
+        const size_t nunk = static_cast<size_t>(c.g.nunk);
+        c.xvec.ensure(std::max(nunk, c.xvec.cap));
+        c.rvec.ensure(std::max(nunk, c.rvec.cap));
+        STOKES_HIP(hipMemcpyAsync(c.xvec.p, den, nunk * sizeof(double), hipMemcpyHostToDevice,
+                                  c.stream));
+        for (auto& e : c.ev)
+            if (!e) STOKES_HIP(hipEventCreate(&e));
+        stokes::apply_operator(c, c.xvec.p, c.rvec.p, static_cast<int>(ibary));  // warm-up
+        STOKES_HIP(hipStreamSynchronize(c.stream));
+        // whole operator applications, back to back
+        STOKES_HIP(hipEventRecord(c.ev[3], c.stream));
+        for (int64_t r = 0; r < reps; ++r)
+            stokes::apply_operator(c, c.xvec.p, c.rvec.p, static_cast<int>(ibary));
+        hipEvent_t e_end;
+        STOKES_HIP(hipEventCreate(&e_end));
+        STOKES_HIP(hipEventRecord(e_end, c.stream));
+        STOKES_HIP(hipEventSynchronize(e_end));
+        float tot = 0.f;
+        STOKES_HIP(hipEventElapsedTime(&tot, c.ev[3], e_end));
+        STOKES_HIP(hipEventDestroy(e_end));
+        // per-kernel durations
+        c.time_kernels = true;
+        c.ff_ms = c.near_ms = 0.0;
+        c.ff_launches = c.near_launches = 0;
+        for (int64_t r = 0; r < reps; ++r)
+            stokes::apply_operator(c, c.xvec.p, c.rvec.p, static_cast<int>(ibary));
+        STOKES_HIP(hipStreamSynchronize(c.stream));
+        c.time_kernels = false;
+        out[0] = static_cast<double>(tot) / static_cast<double>(reps);
+        out[1] = c.ff_ms / static_cast<double>(std::max<long long>(1, c.ff_launches));
+        const double np = static_cast<double>(c.g.npts);
+        out[2] = np * (np - 1.0);
+        out[3] = static_cast<double>(c.near.nentries);
+        out[4] = c.near_launches ? c.near_ms / static_cast<double>(c.near_launches) : 0.0;
+        out[5] = static_cast<double>(c.ff_splits);
+        return 0;
+    } catch (const std::exception& e) {
+        Context& c = stokes::ctx();
+        c.time_kernels = false;
+        return record_error("erosion_time_matvec", e.what());
+    }
+}
+
+}  // extern "C"

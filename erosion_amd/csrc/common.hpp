@@ -1,0 +1,192 @@
+// Shared declarations of the MI355X Stokes library: device helpers, the
+// persistent per-process context, and the host-side entry points of each
+// kernel family.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace stokes {
+
+// ---------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+    explicit Error(const std::string& s) : std::runtime_error(s) {}
+};
+
+#define STOKES_HIP(expr)                                                                    \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw ::stokes::Error(std::string(#expr) + ": " + hipGetErrorString(e_) + " (" + \
+                                  __FILE__ + ":" + std::to_string(__LINE__) + ")");         \
+    } while (0)
+
+#define STOKES_LAUNCH_CHECK() STOKES_HIP(hipGetLastError())
+
+constexpr double kPi = 3.14159265358979323846;
+constexpr double kTwoPi = 6.28318530717958647692;
+
+// ------------------------------------------------------- device helpers
+// FP64 reciprocal: hardware estimate refined by two Newton steps.  Used for
+// every 1/rho^2 and complex reciprocal on the hot paths; arguments are
+// squared distances between distinct boundary points (finite, > 0).
+__device__ __forceinline__ double rcp64(double a) {
+    double r = __builtin_amdgcn_rcp(a);
+    r = __builtin_fma(r, __builtin_fma(-a, r, 1.0), r);
+    r = __builtin_fma(r, __builtin_fma(-a, r, 1.0), r);
+    return r;
+}
+
+struct cplx {
+    double r, i;
+};
+__device__ __forceinline__ cplx cmk(double r, double i) { return cplx{r, i}; }
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return cplx{a.r + b.r, a.i + b.i}; }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return cplx{a.r - b.r, a.i - b.i}; }
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+    return cplx{__builtin_fma(a.r, b.r, -a.i * b.i), __builtin_fma(a.r, b.i, a.i * b.r)};
+}
+__device__ __forceinline__ cplx cscale(cplx a, double s) { return cplx{a.r * s, a.i * s}; }
+__device__ __forceinline__ cplx cinv(cplx a) {
+    double id = rcp64(__builtin_fma(a.r, a.r, a.i * a.i));
+    return cplx{a.r * id, -a.i * id};
+}
+__device__ __forceinline__ cplx cdiv(cplx a, cplx b) { return cmul(a, cinv(b)); }
+// acc += a * b
+__device__ __forceinline__ void cfma(cplx& acc, cplx a, cplx b) {
+    acc.r = __builtin_fma(a.r, b.r, __builtin_fma(-a.i, b.i, acc.r));
+    acc.i = __builtin_fma(a.r, b.i, __builtin_fma(a.i, b.r, acc.i));
+}
+
+// ------------------------------------------------------------ buffers
+// Owning device allocation; grows only.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    void ensure(size_t count) {
+        if (count <= cap && p) return;
+        release();
+        size_t c = count ? count : 1;
+        STOKES_HIP(hipMalloc(reinterpret_cast<void**>(&p), c * sizeof(T)));
+        cap = c;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    ~DevBuf() { release(); }
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+};
+
+// --------------------------------------------------------- geometry
+// All boundary points of one solve, wall first (index 0..no-1) then bodies
+// body-major (no + b*n + k).  "Curves": curve 0 is the wall, curve 1+b body b.
+struct Geometry {
+    int n = 0, m = 0, no = 0;
+    int npts = 0;  // no + m*n
+    int nunk = 0;  // 2*no + 2*m*n + 3*m  (den layout of stokesSolver.f:631-657)
+    // per point (npts)
+    DevBuf<double> x, y, px, py, cur, speed;
+    DevBuf<double> dzr, dzi;  // z'(theta) * dtheta  (fourierDiff(z*dtheta))
+    DevBuf<double> mx, my;    // normal * speed * 2/nsrc: trapezoid weight over pi
+    DevBuf<double> iar, iai;  // 1/(z - a_curve), a = point mean of the curve
+    // per curve (m+1)
+    DevBuf<double> ax, ay;
+    // per body (m)
+    DevBuf<double> cx, cy;
+    // spectral derivative tables d(k) = (-1)^k cot(pi k / N) / 2
+    DevBuf<double> dtab_n, dtab_no;
+    // host copies needed to build near lists
+    std::vector<double> h_x, h_y, h_speed, h_cx, h_cy;
+};
+
+// Near-pair replacement list of matvec_DLP_fmmbary (stokesSolver.f:687-926):
+// an entry is (source curve, target point); interior rule iff source is wall.
+struct NearList {
+    int nentries = 0;
+    int ntasks = 0;
+    int nactive = 0;                 // source curves with at least one entry
+    int wall_active = 0;             // 1 if the wall is a source (active[0] == 0)
+    int wall_tasks = 0;              // tasks with the wall as source (listed first)
+    DevBuf<int> ent_tgt;             // target point per entry (sorted by source)
+    DevBuf<int> task_src, task_beg, task_cnt;  // <=64 entries of one source
+    DevBuf<int> tgt_ptr, tgt_ent;    // CSR: entries of each target point
+    DevBuf<int> active;              // active source curves
+    DevBuf<double> corr_u, corr_v;   // per-entry correction (bary - trapezoid)
+};
+
+// Per-solve GMRES state kept on the device.
+struct GmresState {
+    double bnrm, r0nrm, prod, rho, snormw, vnrm, inv_snormw;
+    int k;          // iterations done
+    int done;       // 1 converged, 2 stopped (maxl / breakdown)
+    int reorth;     // reorthogonalisation requested for this iteration
+    int nreorth;    // number of reorthogonalised iterations
+};
+
+struct Context {
+    Geometry g;
+    NearList near;
+    hipStream_t stream = nullptr;
+    // operator work buffers
+    DevBuf<double> ex, ey;          // density in point order
+    DevBuf<double> part_u, part_v;  // far-field partial sums [split][npts]
+    DevBuf<double> bd;              // boundary values [5 fields][2][npts]
+    DevBuf<double> fld;             // field densities [5][2][npts]
+    DevBuf<double> scal;            // sigdotn etc.
+    int ff_splits = 0;
+    // GMRES buffers
+    int npad = 0;
+    int maxl_alloc = 0;
+    DevBuf<double> V;       // (maxl+1) x npad
+    DevBuf<double> H;       // (maxl+1) x maxl, column-major
+    DevBuf<double> q;       // Givens (c, s) pairs
+    DevBuf<double> hbuf, hpart, wpart, npart, ybuf;
+    DevBuf<double> rhs, xvec, rvec;
+    DevBuf<GmresState> st;
+    GmresState* h_st = nullptr;  // pinned mirror
+    // stress buffers
+    DevBuf<double> spart;   // stress partials
+    DevBuf<double> tau;
+    double tol = 1e-8;
+    // per-kernel timing with HIP events on `stream` (measurement hooks only)
+    bool time_kernels = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double ff_ms = 0.0, near_ms = 0.0, stress_eval_ms = 0.0;
+    long long ff_launches = 0, near_launches = 0, stress_launches = 0;
+    int status = 0;
+    std::string last_error;
+    // last solve record
+    long long last_iters = 0;
+    double last_true_relres = -1.0, last_implicit_relres = -1.0, last_seconds = 0.0;
+    int last_reorth = 0;
+};
+
+Context& ctx();
+
+// geometry.hip
+void build_geometry(Context& c, int n, int m, int no, const double* xx, const double* yy);
+void build_near_list(Context& c, int ibary);
+
+// operator.hip : y = A x for the completed double-layer operator
+void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary);
+void gather_density(Context& c, const double* den_dev);
+void boundary_values(Context& c, const int* curves_dev, int ncurves, int wall_first);
+void bd_condition(Context& c, int ibc, double* rhs_dev);
+
+// gmres.hip
+long long gmres_solve(Context& c, int ibary, int ibc, int maxl, double tol, double* den_host);
+
+// stress.hip
+void shear_stress(Context& c, const double* den_dev, double* tau_dev);
+
+}  // namespace stokes

@@ -1,0 +1,519 @@
+// Unrestarted GMRES on the device.
+//
+// Restates DGMRES/DPIGMR as solveBIE configures them
+// (ErosionPackage/fortran_code/stokesSolver.f:481-497,
+//  ErosionPackage/fortran_code/gmres_routines.f:380-554, 1524-1963):
+// no preconditioner, no restart, Arnoldi with a Hessenberg QR by Givens
+// rotations (DHEQR), stop on the implicit residual rho = |prod s_i| ||r0||
+// relative to ||b|| (ISDGMR, ITOL=0), solution by DHELS.
+//
+// The reference orthogonalises with modified Gram-Schmidt (DORTH), k
+// dependent dot/axpy pairs per iteration.  Here the Krylov basis V lives in
+// HBM as (maxl+1) rows of npad doubles and each Gram-Schmidt pass is two
+// streaming kernels (all dots at once, then one fused update): classical
+// Gram-Schmidt with a second pass when ||w|| drops below ||w_in||/sqrt(2)
+// (the Daniel-Gragg-Kaufman-Stewart test).  The Givens update applies the k
+// stored rotations with a block-wide scan of the affine carry recurrence
+// c_{i+1} = s_i c_i + c_i' h_{i+1} instead of a k-step serial loop.
+//
+// The host enqueues chunks of iterations and reads the device state once per
+// chunk; kernels of iterations after convergence return immediately.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+
+#include "common.hpp"
+
+namespace stokes {
+
+namespace {
+
+constexpr int kBlk = 256;
+constexpr int kRows = 4;       // Krylov rows per block in the dot kernel
+constexpr int kChunkIters = 16;
+constexpr double kReorthEta = 0.70710678118654752440;
+
+inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
+inline size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// out = a*x + b*y over n (n even, 16-byte aligned rows); per-block sum of out^2
+__global__ __launch_bounds__(kBlk) void k_axpby_norm(double a, const double* __restrict__ x,
+                                                     double b, const double* __restrict__ y,
+                                                     double* __restrict__ out, size_t n,
+                                                     double* __restrict__ npart) {
+    __shared__ double sh[kBlk];
+    double acc = 0.0;
+    for (size_t e = (static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x) * 2; e < n;
+         e += static_cast<size_t>(gridDim.x) * kBlk * 2) {
+        double2 xv = *reinterpret_cast<const double2*>(x + e);
+        double2 yv = y ? *reinterpret_cast<const double2*>(y + e) : make_double2(0.0, 0.0);
+        double2 o = make_double2(a * xv.x + b * yv.x, a * xv.y + b * yv.y);
+        if (out) *reinterpret_cast<double2*>(out + e) = o;
+        acc += o.x * o.x + o.y * o.y;
+    }
+    double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) npart[blockIdx.x] = s;
+}
+
+__global__ void k_zero_tail(double* __restrict__ v, int n, int npad) {
+    int i = n + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < npad) v[i] = 0.0;
+}
+
+// Initial state: ||b||, ||r0|| from per-block partial sums.
+__global__ void k_init_state(const double* __restrict__ bpart, int nb,
+                             const double* __restrict__ rpart, int nr, double tol,
+                             GmresState* __restrict__ st) {
+    if (threadIdx.x != 0) return;
+    double sb = 0.0, sr = 0.0;
+    for (int i = 0; i < nb; ++i) sb += bpart[i];
+    for (int i = 0; i < nr; ++i) sr += rpart[i];
+    GmresState s{};
+    s.bnrm = sqrt(sb);
+    s.r0nrm = sqrt(sr);
+    s.prod = 1.0;
+    s.rho = s.r0nrm;
+    s.k = 0;
+    s.done = (s.r0nrm / s.bnrm <= tol) ? 1 : 0;
+    s.reorth = 0;
+    s.nreorth = 0;
+    *st = s;
+}
+
+__global__ void k_scale_row(double* __restrict__ v, size_t n, const GmresState* __restrict__ st,
+                            int use_r0) {
+    if (st->done) return;
+    const double s = use_r0 ? 1.0 / st->r0nrm : st->inv_snormw;
+    for (size_t e = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < n;
+         e += static_cast<size_t>(gridDim.x) * blockDim.x)
+        v[e] *= s;
+}
+
+// Partial dots of rows [r0, r0+kRows) of V with w over one chunk, plus the
+// chunk's ||w||^2.  pass 1 runs unless done; pass 2 only when reorth is set.
+__global__ __launch_bounds__(kBlk) void k_dots(const double* __restrict__ V, size_t npad,
+                                               const double* __restrict__ w, int nrows,
+                                               size_t chunk, int nchunk,
+                                               const GmresState* __restrict__ st, int pass,
+                                               double* __restrict__ hpart,
+                                               double* __restrict__ npart) {
+    __shared__ double sh[kBlk];
+    if (st->done || (pass == 2 && !st->reorth)) return;
+    const int r0 = blockIdx.x * kRows;
+    const int c = blockIdx.y;
+    const size_t e0 = static_cast<size_t>(c) * chunk;
+    const size_t e1 = std::min(npad, e0 + chunk);
+    double acc[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) acc[r] = 0.0;
+    double nacc = 0.0;
+    for (size_t e = e0 + 2 * threadIdx.x; e < e1; e += 2 * kBlk) {
+        const double2 wv = *reinterpret_cast<const double2*>(w + e);
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            if (r0 + r < nrows) {
+                const double2 vv =
+                    *reinterpret_cast<const double2*>(V + static_cast<size_t>(r0 + r) * npad + e);
+                acc[r] = __builtin_fma(vv.x, wv.x, __builtin_fma(vv.y, wv.y, acc[r]));
+            }
+        }
+        nacc = __builtin_fma(wv.x, wv.x, __builtin_fma(wv.y, wv.y, nacc));
+    }
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+        double s = block_sum(acc[r], sh);
+        if (threadIdx.x == 0 && r0 + r < nrows) hpart[static_cast<size_t>(r0 + r) * nchunk + c] = s;
+    }
+    if (blockIdx.x == 0) {
+        double s = block_sum(nacc, sh);
+        if (threadIdx.x == 0) npart[c] = s;
+    }
+}
+
+// h = sum over chunks; pass 1 stores it as the Hessenberg column and sets
+// ||w_in||; pass 2 adds the correction.
+__global__ void k_dots_reduce(const double* __restrict__ hpart, int nrows, int nchunk,
+                              const double* __restrict__ npart, GmresState* __restrict__ st,
+                              int pass, double* __restrict__ hcur, double* __restrict__ hsum) {
+    if (st->done || (pass == 2 && !st->reorth)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nrows) {
+        double s = 0.0;
+        for (int c = 0; c < nchunk; ++c) s += hpart[static_cast<size_t>(i) * nchunk + c];
+        hcur[i] = s;
+        hsum[i] = pass == 1 ? s : hsum[i] + s;
+    }
+    if (pass == 1 && i == 0) {
+        double s = 0.0;
+        for (int c = 0; c < nchunk; ++c) s += npart[c];
+        st->vnrm = sqrt(s);
+    }
+}
+
+// wpart[split][e] = sum_{rows of split} h_r V[r][e]
+__global__ __launch_bounds__(kBlk) void k_update(const double* __restrict__ V, size_t npad,
+                                                 const double* __restrict__ h, int nrows,
+                                                 int rows_per_split,
+                                                 const GmresState* __restrict__ st, int mode,
+                                                 double* __restrict__ wpart) {
+    // mode 1: pass 1, mode 2: reorth pass, mode 0: final solution (always)
+    if (mode != 0 && (st->done || (mode == 2 && !st->reorth))) return;
+    const size_t e = (static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x) * 2;
+    if (e >= npad) return;
+    const int s = blockIdx.y;
+    const int ra = s * rows_per_split, rb = min(nrows, ra + rows_per_split);
+    double ax = 0.0, ay = 0.0;
+    int r = ra;
+    for (; r + 3 < rb; r += 4) {
+        const double h0 = h[r], h1 = h[r + 1], h2 = h[r + 2], h3 = h[r + 3];
+        const double2 v0 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r) * npad + e);
+        const double2 v1 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r + 1) * npad + e);
+        const double2 v2 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r + 2) * npad + e);
+        const double2 v3 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r + 3) * npad + e);
+        ax = __builtin_fma(h0, v0.x, ax);
+        ay = __builtin_fma(h0, v0.y, ay);
+        ax = __builtin_fma(h1, v1.x, ax);
+        ay = __builtin_fma(h1, v1.y, ay);
+        ax = __builtin_fma(h2, v2.x, ax);
+        ay = __builtin_fma(h2, v2.y, ay);
+        ax = __builtin_fma(h3, v3.x, ax);
+        ay = __builtin_fma(h3, v3.y, ay);
+    }
+    for (; r < rb; ++r) {
+        const double hr = h[r];
+        const double2 v0 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r) * npad + e);
+        ax = __builtin_fma(hr, v0.x, ax);
+        ay = __builtin_fma(hr, v0.y, ay);
+    }
+    *reinterpret_cast<double2*>(wpart + static_cast<size_t>(s) * npad + e) = make_double2(ax, ay);
+}
+
+// w += sign * sum_s wpart[s]; per-block ||w||^2
+__global__ __launch_bounds__(kBlk) void k_combine(double* __restrict__ w, size_t npad,
+                                                  const double* __restrict__ wpart, int nsplit,
+                                                  double sign, const GmresState* __restrict__ st,
+                                                  int mode, double* __restrict__ npart) {
+    __shared__ double sh[kBlk];
+    if (mode != 0 && (st->done || (mode == 2 && !st->reorth))) return;
+    const size_t e = (static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x) * 2;
+    double acc = 0.0;
+    if (e < npad) {
+        double2 wv = *reinterpret_cast<const double2*>(w + e);
+        double sx = 0.0, sy = 0.0;
+        for (int s = 0; s < nsplit; ++s) {
+            const double2 p = *reinterpret_cast<const double2*>(wpart + static_cast<size_t>(s) * npad + e);
+            sx += p.x;
+            sy += p.y;
+        }
+        wv.x = __builtin_fma(sign, sx, wv.x);
+        wv.y = __builtin_fma(sign, sy, wv.y);
+        *reinterpret_cast<double2*>(w + e) = wv;
+        acc = wv.x * wv.x + wv.y * wv.y;
+    }
+    double s = block_sum(acc, sh);
+    if (threadIdx.x == 0) npart[blockIdx.x] = s;
+}
+
+// ||w|| after a Gram-Schmidt pass; after pass 1 decide on a second pass.
+__global__ void k_norm_decide(const double* __restrict__ npart, int nb, GmresState* __restrict__ st,
+                              int pass, double eta) {
+    if (threadIdx.x != 0 || st->done || (pass == 2 && !st->reorth)) return;
+    double s = 0.0;
+    for (int i = 0; i < nb; ++i) s += npart[i];
+    st->snormw = sqrt(s);
+    if (pass == 1) {
+        st->reorth = (st->snormw < eta * st->vnrm) ? 1 : 0;
+        st->nreorth += st->reorth;
+    }
+}
+
+// Hessenberg column k, Givens rotations (DHEQR, IJOB>1), stop test.
+// The k stored rotations act on (col[i], col[i+1]) in order; the value
+// carried into row i obeys carry_{i+1} = s_i carry_i + c_i h_{i+1}, an affine
+// recurrence evaluated with a block-wide scan over segments.
+constexpr int kGivT = 1024;
+__global__ __launch_bounds__(kGivT) void k_givens(double* __restrict__ H, int ldh,
+                                                  double* __restrict__ q,
+                                                  const double* __restrict__ hsum,
+                                                  GmresState* __restrict__ st, double tol,
+                                                  int maxl) {
+    __shared__ double sA[kGivT], sB[kGivT];
+    if (st->done) return;
+    const int k = st->k;
+    double* __restrict__ col = H + static_cast<size_t>(k) * ldh;
+    const int tid = threadIdx.x;
+    // segment of rotations for this thread
+    const int L = (k + kGivT - 1) / kGivT;
+    const int i0 = min(k, tid * L), i1 = min(k, i0 + L);
+    double A = 1.0, B = 0.0;
+    for (int i = i0; i < i1; ++i) {
+        const double c = q[2 * i], s = q[2 * i + 1];
+        A = s * A;
+        B = __builtin_fma(s, B, c * hsum[i + 1]);
+    }
+    sA[tid] = A;
+    sB[tid] = B;
+    __syncthreads();
+    // inclusive scan of maps (composition: later after earlier)
+    for (int off = 1; off < kGivT; off <<= 1) {
+        double a2 = 0, b2 = 0;
+        const bool has = tid >= off;
+        if (has) {
+            a2 = sA[tid - off];
+            b2 = sB[tid - off];
+        }
+        __syncthreads();
+        if (has) {
+            const double a1 = sA[tid], b1 = sB[tid];
+            sA[tid] = a1 * a2;
+            sB[tid] = __builtin_fma(a1, b2, b1);
+        }
+        __syncthreads();
+    }
+    const double h0 = hsum[0];
+    double carry = tid == 0 ? h0 : __builtin_fma(sA[tid - 1], h0, sB[tid - 1]);
+    for (int i = i0; i < i1; ++i) {
+        const double c = q[2 * i], s = q[2 * i + 1];
+        const double hn = hsum[i + 1];
+        col[i] = c * carry - s * hn;
+        carry = __builtin_fma(s, carry, c * hn);
+    }
+    // the thread owning the last rotation (or thread 0 when k == 0) finishes
+    const bool last = (k == 0) ? (tid == 0) : (i0 < i1 && i1 == k);
+    if (!last) return;
+    const double t1 = carry, t2 = st->snormw;
+    double c, s;
+    if (t2 == 0.0) {
+        c = 1.0;
+        s = 0.0;
+    } else if (fabs(t2) >= fabs(t1)) {
+        const double t = t1 / t2;
+        s = -1.0 / sqrt(1.0 + t * t);
+        c = -s * t;
+    } else {
+        const double t = t2 / t1;
+        c = 1.0 / sqrt(1.0 + t * t);
+        s = -c * t;
+    }
+    q[2 * k] = c;
+    q[2 * k + 1] = s;
+    const double rkk = c * t1 - s * t2;
+    col[k] = rkk;
+    col[k + 1] = t2;
+    st->k = k + 1;
+    if (rkk == 0.0) {  // DHEQR INFO = k: breakdown
+        st->done = 3;
+        return;
+    }
+    st->prod = st->prod * s;
+    st->rho = fabs(st->prod * st->r0nrm);
+    st->inv_snormw = 1.0 / t2;
+    if (st->rho / st->bnrm <= tol)
+        st->done = 1;
+    else if (k + 1 >= maxl)
+        st->done = 2;
+}
+
+// DHELS: rotate (r0nrm, 0, ..., 0), back-substitute the k x k triangle.
+__global__ __launch_bounds__(1024) void k_hels(const double* __restrict__ H, int ldh,
+                                               const double* __restrict__ q,
+                                               const GmresState* __restrict__ st,
+                                               double* __restrict__ y) {
+    const int k = st->k;
+    if (threadIdx.x == 0) {
+        double b = st->r0nrm;
+        for (int i = 0; i < k; ++i) {  // (b_i, b_{i+1}) = rot_i (b_i, 0)
+            const double c = q[2 * i], s = q[2 * i + 1];
+            y[i] = c * b;
+            b = s * b;
+        }
+    }
+    __syncthreads();
+    for (int kk = k - 1; kk >= 0; --kk) {
+        const double* colk = H + static_cast<size_t>(kk) * ldh;
+        if (threadIdx.x == 0) y[kk] = y[kk] / colk[kk];
+        __syncthreads();
+        const double yk = y[kk];
+        for (int i = threadIdx.x; i < kk; i += blockDim.x) y[i] = __builtin_fma(-yk, colk[i], y[i]);
+        __syncthreads();
+    }
+}
+
+struct Plan {
+    int nrows, nchunk, bx;
+    size_t chunk;
+};
+
+Plan dots_plan(int nrows, size_t npad) {
+    Plan p{};
+    p.nrows = nrows;
+    p.bx = cdiv_int(nrows, kRows);
+    int want = std::max(1, cdiv_int(1024, p.bx));
+    int maxc = static_cast<int>(std::max<size_t>(1, npad / (2 * kBlk * 4)));
+    p.nchunk = std::min(std::min(want, maxc), 64);
+    p.chunk = round_up(cdiv_int(static_cast<int>(npad), p.nchunk), 2 * kBlk);
+    p.nchunk = static_cast<int>((npad + p.chunk - 1) / p.chunk);
+    return p;
+}
+
+}  // namespace
+
+// One Gram-Schmidt pass over rows [0, nrows) of V against w.
+static void gs_pass(Context& c, double* w, int nrows, int pass, double* hcur) {
+    const size_t npad = static_cast<size_t>(c.npad);
+    hipStream_t s = c.stream;
+    Plan p = dots_plan(nrows, npad);
+    k_dots<<<dim3(p.bx, p.nchunk), kBlk, 0, s>>>(c.V.p, npad, w, nrows, p.chunk, p.nchunk, c.st.p,
+                                                 pass, c.hpart.p, c.npart.p);
+    STOKES_LAUNCH_CHECK();
+    k_dots_reduce<<<cdiv_int(nrows, kBlk), kBlk, 0, s>>>(c.hpart.p, nrows, p.nchunk, c.npart.p,
+                                                         c.st.p, pass, hcur, c.hbuf.p);
+    STOKES_LAUNCH_CHECK();
+    const int ebx = static_cast<int>(npad / (2 * kBlk));
+    int nsplit = std::min(nrows, std::max(1, cdiv_int(1024, ebx)));
+    int rps = cdiv_int(nrows, nsplit);
+    nsplit = cdiv_int(nrows, rps);
+    k_update<<<dim3(ebx, nsplit), kBlk, 0, s>>>(c.V.p, npad, hcur, nrows, rps, c.st.p, pass,
+                                                c.wpart.p);
+    STOKES_LAUNCH_CHECK();
+    k_combine<<<ebx, kBlk, 0, s>>>(w, npad, c.wpart.p, nsplit, -1.0, c.st.p, pass, c.npart.p);
+    STOKES_LAUNCH_CHECK();
+    k_norm_decide<<<1, 64, 0, s>>>(c.npart.p, ebx, c.st.p, pass, kReorthEta);
+    STOKES_LAUNCH_CHECK();
+}
+
+long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, double* den_host) {
+    auto t_start = std::chrono::steady_clock::now();
+    Geometry& g = c.g;
+    hipStream_t s = c.stream;
+    const int N = g.nunk;
+    const size_t npad = round_up(static_cast<size_t>(N), 2 * kBlk);
+    c.npad = static_cast<int>(npad);
+    int maxl = maxl_in == 0 ? 10 : maxl_in;
+    maxl = std::max(1, std::min(maxl, N));
+    const int ldh = maxl + 1;
+    c.V.ensure(static_cast<size_t>(maxl + 1) * npad);
+    c.H.ensure(static_cast<size_t>(ldh) * static_cast<size_t>(maxl));
+    c.q.ensure(2 * static_cast<size_t>(maxl));
+    c.hbuf.ensure(static_cast<size_t>(maxl + 1 + kRows));
+    c.ybuf.ensure(static_cast<size_t>(maxl + 1 + kRows));
+    {
+        Plan p = dots_plan(maxl + 1, npad);
+        c.hpart.ensure(static_cast<size_t>(p.bx * kRows) * static_cast<size_t>(std::max(p.nchunk, 64)));
+    }
+    const int ebx = static_cast<int>(npad / (2 * kBlk));
+    c.npart.ensure(static_cast<size_t>(std::max(ebx, 1024)));
+    c.wpart.ensure(static_cast<size_t>(std::max(1, cdiv_int(1024, ebx)) + 1) * npad);
+    c.rhs.ensure(npad);
+    c.xvec.ensure(npad);
+    c.rvec.ensure(npad);
+    c.st.ensure(1);
+    if (!c.h_st) STOKES_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.h_st), sizeof(GmresState)));
+    DevBuf<double> hcur;
+    hcur.ensure(static_cast<size_t>(maxl + 1 + kRows));
+    DevBuf<double> bpart;
+    bpart.ensure(1024);
+
+    // rhs and initial guess (Julia passes zeros, DensityStress.jl:45)
+    STOKES_HIP(hipMemsetAsync(c.rhs.p, 0, npad * sizeof(double), s));
+    STOKES_HIP(hipMemsetAsync(c.xvec.p, 0, npad * sizeof(double), s));
+    bd_condition(c, ibc, c.rhs.p);
+    STOKES_HIP(hipMemcpyAsync(c.xvec.p, den_host, sizeof(double) * N, hipMemcpyHostToDevice, s));
+
+    const int nb = static_cast<int>(std::min<size_t>(1024, npad / (2 * kBlk)));
+    k_axpby_norm<<<nb, kBlk, 0, s>>>(1.0, c.rhs.p, 0.0, nullptr, nullptr, npad, bpart.p);
+    STOKES_LAUNCH_CHECK();
+    // r0 = b - A x0 into V[0]
+    double* V0 = c.V.p;
+    apply_operator(c, c.xvec.p, c.rvec.p, ibary);
+    k_zero_tail<<<cdiv_int(static_cast<int>(npad) - N + 1, kBlk), kBlk, 0, s>>>(c.rvec.p, N,
+                                                                              static_cast<int>(npad));
+    STOKES_LAUNCH_CHECK();
+    k_axpby_norm<<<nb, kBlk, 0, s>>>(1.0, c.rhs.p, -1.0, c.rvec.p, V0, npad, c.npart.p);
+    STOKES_LAUNCH_CHECK();
+    k_init_state<<<1, 64, 0, s>>>(bpart.p, nb, c.npart.p, nb, tol, c.st.p);
+    STOKES_LAUNCH_CHECK();
+    k_scale_row<<<std::min(1024, ebx * 2), kBlk, 0, s>>>(V0, npad, c.st.p, 1);
+    STOKES_LAUNCH_CHECK();
+
+    int k = 0;
+    while (k < maxl) {
+        const int chunk = std::min(kChunkIters, maxl - k);
+        for (int j = 0; j < chunk; ++j) {
+            const int kk = k + j;
+            double* vk = c.V.p + static_cast<size_t>(kk) * npad;
+            double* w = c.V.p + static_cast<size_t>(kk + 1) * npad;
+            apply_operator(c, vk, w, ibary);
+            k_zero_tail<<<cdiv_int(static_cast<int>(npad) - N + 1, kBlk), kBlk, 0, s>>>(
+                w, N, static_cast<int>(npad));
+            STOKES_LAUNCH_CHECK();
+            gs_pass(c, w, kk + 1, 1, hcur.p);
+            gs_pass(c, w, kk + 1, 2, hcur.p);
+            k_givens<<<1, kGivT, 0, s>>>(c.H.p, ldh, c.q.p, c.hbuf.p, c.st.p, tol, maxl);
+            STOKES_LAUNCH_CHECK();
+            k_scale_row<<<std::min(1024, ebx * 2), kBlk, 0, s>>>(w, npad, c.st.p, 0);
+            STOKES_LAUNCH_CHECK();
+        }
+        k += chunk;
+        STOKES_HIP(hipMemcpyAsync(c.h_st, c.st.p, sizeof(GmresState), hipMemcpyDeviceToHost, s));
+        STOKES_HIP(hipStreamSynchronize(s));
+        if (c.h_st->done) break;
+    }
+    GmresState fin = *c.h_st;
+    const int iters = fin.k;
+    int status = 0;
+    if (fin.done == 3) {
+        status = 2;  // breakdown: solution left at x0 (DPIGMR IFLAG=2)
+    } else if (iters > 0 && (fin.done == 1 || fin.rho < fin.r0nrm)) {
+        // x += V y (DHELS + the sum of DPIGMR label 200)
+        k_hels<<<1, 1024, 0, s>>>(c.H.p, ldh, c.q.p, c.st.p, c.ybuf.p);
+        STOKES_LAUNCH_CHECK();
+        int nsplit = std::min(iters, std::max(1, cdiv_int(1024, ebx)));
+        int rps = cdiv_int(iters, nsplit);
+        nsplit = cdiv_int(iters, rps);
+        k_update<<<dim3(ebx, nsplit), kBlk, 0, s>>>(c.V.p, npad, c.ybuf.p, iters, rps, c.st.p, 0,
+                                                    c.wpart.p);
+        STOKES_LAUNCH_CHECK();
+        k_combine<<<ebx, kBlk, 0, s>>>(c.xvec.p, npad, c.wpart.p, nsplit, 1.0, c.st.p, 0,
+                                       c.npart.p);
+        STOKES_LAUNCH_CHECK();
+        status = fin.done == 1 ? 0 : 1;
+    } else if (fin.done != 1) {
+        status = fin.done == 2 ? 1 : 2;
+    }
+    // true residual of the returned density
+    apply_operator(c, c.xvec.p, c.rvec.p, ibary);
+    k_zero_tail<<<cdiv_int(static_cast<int>(npad) - N + 1, kBlk), kBlk, 0, s>>>(c.rvec.p, N,
+                                                                              static_cast<int>(npad));
+    STOKES_LAUNCH_CHECK();
+    k_axpby_norm<<<nb, kBlk, 0, s>>>(1.0, c.rhs.p, -1.0, c.rvec.p, c.rvec.p, npad, c.npart.p);
+    STOKES_LAUNCH_CHECK();
+    std::vector<double> part(static_cast<size_t>(nb));
+    STOKES_HIP(hipMemcpyAsync(part.data(), c.npart.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    STOKES_HIP(hipMemcpyAsync(den_host, c.xvec.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+    STOKES_HIP(hipStreamSynchronize(s));
+    double rr = 0.0;
+    for (double v : part) rr += v;
+    c.last_iters = iters;
+    c.last_implicit_relres = fin.rho / fin.bnrm;
+    c.last_true_relres = std::sqrt(rr) / fin.bnrm;
+    c.last_reorth = fin.nreorth;
+    c.status = status;
+    c.last_seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    return iters;
+}
+
+}  // namespace stokes

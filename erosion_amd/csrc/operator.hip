@@ -1,0 +1,605 @@
+// The completed double-layer operator y = A x on the device.
+//
+// Restates matvec_DLP_fmmbary (ibary=1) and matvec_DLP (ibary=0),
+// ErosionPackage/fortran_code/stokesSolver.f:570-1124 and 1519-1912:
+//   far field     all-pairs trapezoid double layer, j != i, replacing the FMM
+//                 (stokesSolver.f:636-681, pair form of :1563-1584)
+//   near pairs    subtract the trapezoid, add barycentric StokesExteriorDLP /
+//                 StokesInteriorDLP with nder=1 (:687-926, :5233-5528)
+//   local terms   curvature diagonal and jump -eta/2 (:932-987)
+//   completion    stokeslets + rotlets (:992-1049), constraint rows
+//                 (:1055-1096), rank-one wall term (:1102-1118)
+//
+// Kernel map, one matvec:
+//   k_gather       den -> (etax, etay) in point order
+//   k_farfield     LDS-tiled all-pairs sum, target per lane, partial sums per
+//                  source split                                (VALU bound)
+//   k_fields       the five Cauchy densities per point
+//   k_bdval        boundary values of the Cauchy integrals of the active
+//                  source curves, node-sliced across waves      (VALU bound)
+//   k_near_eval    barycentric value/derivative at near targets minus the
+//                  trapezoid part, one block per <=64 targets of one source
+//   k_reduce_small rank-one integral and constraint rows
+//   k_assemble     partial sums + near corrections + local terms + stokeslets
+#include <algorithm>
+
+#include "bary.hpp"
+#include "common.hpp"
+
+namespace stokes {
+
+namespace {
+
+constexpr int kBlk = 256;
+constexpr int kTile = 256;
+
+inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
+
+__device__ __forceinline__ int row_x(int i, int no, int n) {
+    if (i < no) return i;
+    int b = (i - no) / n, k = (i - no) - b * n;
+    return 2 * no + 2 * b * n + k;
+}
+__device__ __forceinline__ int row_y(int i, int no, int n) {
+    if (i < no) return no + i;
+    int b = (i - no) / n, k = (i - no) - b * n;
+    return 2 * no + 2 * b * n + n + k;
+}
+
+__global__ void k_gather(const double* __restrict__ den, int no, int n, int npts,
+                         double* __restrict__ ex, double* __restrict__ ey) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    ex[i] = den[row_x(i, no, n)];
+    ey[i] = den[row_y(i, no, n)];
+}
+
+// ---------------------------------------------------------------- far field
+// Pair term (1/pi)(r.n)(r.eta) r w / rho^4 with w = speed 2 pi / N folded
+// into (mx, my).  CHECK skips the self pair j == i.
+template <bool CHECK>
+__device__ __forceinline__ void ff_tile(const double2* __restrict__ sxy,
+                                        const double2* __restrict__ sm,
+                                        const double2* __restrict__ se, double xt, double yt,
+                                        int self, double& u, double& v) {
+#pragma unroll 8
+    for (int jj = 0; jj < kTile; ++jj) {
+        const double2 p = sxy[jj];
+        const double2 mm = sm[jj];
+        const double2 e = se[jj];
+        const double rx = xt - p.x, ry = yt - p.y;
+        const double rho2 = __builtin_fma(rx, rx, ry * ry);
+        const double a = __builtin_fma(rx, mm.x, ry * mm.y);
+        const double b = __builtin_fma(rx, e.x, ry * e.y);
+        const double inv = rcp64(rho2);
+        double c = (a * b) * (inv * inv);
+        if (CHECK) c = (jj == self) ? 0.0 : c;
+        u = __builtin_fma(c, rx, u);
+        v = __builtin_fma(c, ry, v);
+    }
+}
+
+__global__ __launch_bounds__(kBlk) void k_farfield(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ mx,
+    const double* __restrict__ my, const double* __restrict__ ex, const double* __restrict__ ey,
+    int npts, int split_len, double* __restrict__ part_u, double* __restrict__ part_v) {
+    __shared__ double2 sxy[kTile], sm[kTile], se[kTile];
+    const int tid = threadIdx.x;
+    const int t = blockIdx.x * kBlk + tid;
+    const int tb0 = blockIdx.x * kBlk;
+    const int tb1 = min(tb0 + kBlk, npts);
+    const bool valid = t < npts;
+    const double xt = valid ? x[t] : 0.0, yt = valid ? y[t] : 0.0;
+    const int s0 = blockIdx.y * split_len;
+    const int s1 = min(s0 + split_len, npts);
+    double u = 0.0, v = 0.0;
+    for (int base = s0; base < s1; base += kTile) {
+        const int j = base + tid;
+        __syncthreads();
+        if (j < s1) {
+            sxy[tid] = make_double2(x[j], y[j]);
+            sm[tid] = make_double2(mx[j], my[j]);
+            se[tid] = make_double2(ex[j], ey[j]);
+        } else {  // padding source: zero weight far away
+            sxy[tid] = make_double2(1.0e100, 1.0e100);
+            sm[tid] = make_double2(0.0, 0.0);
+            se[tid] = make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+        const bool diag = (base < tb1) && (base + kTile > tb0);
+        if (diag)
+            ff_tile<true>(sxy, sm, se, xt, yt, t - base, u, v);
+        else
+            ff_tile<false>(sxy, sm, se, xt, yt, -1, u, v);
+    }
+    if (valid) {
+        part_u[static_cast<size_t>(blockIdx.y) * npts + t] = u;
+        part_v[static_cast<size_t>(blockIdx.y) * npts + t] = v;
+    }
+}
+
+// ------------------------------------------------------- Cauchy densities
+// s.eta, eta_x, eta_y, tau1 = eta n_x/(n_x+i n_y), tau2 = eta n_y/(n_x+i n_y)
+// (stokesSolver.f:5268-5276, 4992-4997); stored as fld[(2f+{0,1})*npts + p].
+__global__ void k_fields(const double* __restrict__ x, const double* __restrict__ y,
+                         const double* __restrict__ px, const double* __restrict__ py,
+                         const double* __restrict__ ex, const double* __restrict__ ey, int npts,
+                         double* __restrict__ fld) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npts) return;
+    const size_t N = static_cast<size_t>(npts);
+    const double dx = ex[i], dy = ey[i], nx = px[i], ny = py[i];
+    const cplx eta = cmk(dx, dy);
+    const cplx inor = cinv(cmk(nx, ny));
+    const cplx t1 = cmul(cscale(eta, nx), inor);
+    const cplx t2 = cmul(cscale(eta, ny), inor);
+    fld[0 * N + i] = x[i] * dx + y[i] * dy;
+    fld[1 * N + i] = 0.0;
+    fld[2 * N + i] = dx;
+    fld[3 * N + i] = 0.0;
+    fld[4 * N + i] = dy;
+    fld[5 * N + i] = 0.0;
+    fld[6 * N + i] = t1.r;
+    fld[7 * N + i] = t1.i;
+    fld[8 * N + i] = t2.r;
+    fld[9 * N + i] = t2.i;
+}
+
+// ------------------------------------------------------ boundary values
+// compute_bdval_ex / compute_bdval_in (stokesSolver.f:5382-5427, 5103-5146)
+// for the five fields of one curve at once:
+//   B_j = sum_{k!=j} -(f_k - f_j) dz_k/(z_k - z_j) - dtheta f'_j
+//   exterior: bd = i B / 2pi       interior: bd = -i B / 2pi - f_j
+// grid: (ceil(len/64), curves); block 64*NSL threads: lane -> node j,
+// wave -> slice of the k range; slices combined through LDS.
+template <int NSL>
+__global__ __launch_bounds__(64 * NSL) void k_bdval(
+    const int* __restrict__ curves, int no, int n, int npts, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ dzr, const double* __restrict__ dzi,
+    const double* __restrict__ dno, const double* __restrict__ dn,
+    const double* __restrict__ fld, double* __restrict__ bd) {
+    __shared__ double red[NSL][64];
+    const int c = curves[blockIdx.y];
+    const int len = c == 0 ? no : n;
+    const int off = c == 0 ? 0 : no + (c - 1) * n;
+    const double* __restrict__ dt = c == 0 ? dno : dn;
+    const int lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
+    const int jl = blockIdx.x * 64 + lane;
+    if (blockIdx.x * 64 >= len) return;  // uniform per block
+    const bool valid = jl < len;
+    const int j = valid ? jl : 0;
+    const size_t N = static_cast<size_t>(npts);
+    const cplx zj = cmk(x[off + j], y[off + j]);
+    cplx fj[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) fj[f] = cmk(fld[(2 * f) * N + off + j], fld[(2 * f + 1) * N + off + j]);
+    cplx B[5], D[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) B[f] = D[f] = cmk(0.0, 0.0);
+    const int k0 = (slice * len) / NSL, k1 = ((slice + 1) * len) / NSL;
+    for (int k = k0; k < k1; ++k) {
+        const int p = off + k;
+        cplx fk[5];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) fk[f] = cmk(fld[(2 * f) * N + p], fld[(2 * f + 1) * N + p]);
+        int idx = j - k;
+        idx += idx < 0 ? len : 0;
+        const double w = dt[idx];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            D[f].r = __builtin_fma(w, fk[f].r, D[f].r);
+            D[f].i = __builtin_fma(w, fk[f].i, D[f].i);
+        }
+        if (k != j) {
+            const cplx g = cmul(cmk(dzr[p], dzi[p]), cinv(csub(cmk(x[p], y[p]), zj)));
+#pragma unroll
+            for (int f = 0; f < 5; ++f) cfma(B[f], csub(fj[f], fk[f]), g);  // -(fk - fj) g
+        }
+    }
+    // combine slices: 20 doubles per lane
+    double vals[20];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+        vals[4 * f + 0] = B[f].r;
+        vals[4 * f + 1] = B[f].i;
+        vals[4 * f + 2] = D[f].r;
+        vals[4 * f + 3] = D[f].i;
+    }
+#pragma unroll
+    for (int q = 0; q < 20; ++q) {
+        red[slice][lane] = vals[q];
+        __syncthreads();
+        if (slice == 0) {
+            double s = 0.0;
+            for (int t = 0; t < NSL; ++t) s += red[t][lane];
+            vals[q] = s;
+        }
+        __syncthreads();
+    }
+    if (slice != 0 || !valid) return;
+    const double dth = kTwoPi / static_cast<double>(len);
+    const bool ext = c != 0;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+        cplx Bf = cmk(vals[4 * f + 0] - dth * vals[4 * f + 2], vals[4 * f + 1] - dth * vals[4 * f + 3]);
+        // i B / 2pi
+        cplx iB = cmk(-Bf.i / kTwoPi, Bf.r / kTwoPi);
+        cplx out = ext ? iB : cmk(-iB.r - fj[f].r, -iB.i - fj[f].i);
+        bd[(2 * f) * N + off + j] = out.r;
+        bd[(2 * f + 1) * N + off + j] = out.i;
+    }
+}
+
+// ------------------------------------------------------- near evaluation
+// Barycentric value (pass 1) accumulators.
+struct Pass1 {
+    cplx S0, Nf[5];
+    double tu, tv;
+};
+
+template <int NS>
+__global__ __launch_bounds__(64 * NS) void k_near_eval(
+    const int* __restrict__ task_src, const int* __restrict__ task_beg,
+    const int* __restrict__ task_cnt, const int* __restrict__ ent_tgt, NodeView nv,
+    const double* __restrict__ ax, const double* __restrict__ ay, int no, int n, int task0,
+    double* __restrict__ corr_u, double* __restrict__ corr_v) {
+    __shared__ double red[NS][64];
+    const int task = task0 + blockIdx.x;
+    const int src = task_src[task], beg = task_beg[task], cnt = task_cnt[task];
+    const int len = src == 0 ? no : n;
+    const int off = src == 0 ? 0 : no + (src - 1) * n;
+    const bool ext = src != 0;
+    const int lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
+    const bool active = lane < cnt;
+    const int tp = ent_tgt[beg + (active ? lane : 0)];
+    const double xt = nv.x[tp], yt = nv.y[tp];
+    const cplx zt = cmk(xt, yt);
+    const cplx a = cmk(ax[src], ay[src]);
+    const size_t N = nv.N;
+    const int k0 = (slice * len) / NS, k1 = ((slice + 1) * len) / NS;
+
+    // pass 1: S0 = sum dz/(z-x), N_f = sum b_f dz/(z-x); trapezoid part
+    Pass1 p1;
+    p1.S0 = cmk(0, 0);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) p1.Nf[f] = cmk(0, 0);
+    p1.tu = p1.tv = 0.0;
+    for (int k = k0; k < k1; ++k) {
+        const int p = off + k;
+        const cplx d = cmk(nv.x[p] - xt, nv.y[p] - yt);
+        const double rho2 = __builtin_fma(d.r, d.r, d.i * d.i);
+        const double irho2 = rcp64(rho2);
+        const cplx id = cmk(d.r * irho2, -d.i * irho2);
+        const cplx w = cmul(cmk(nv.dzr[p], nv.dzi[p]), id);
+        p1.S0 = cadd(p1.S0, w);
+#pragma unroll
+        for (int f = 0; f < 5; ++f) cfma(p1.Nf[f], cmk(nv.bd[(2 * f) * N + p], nv.bd[(2 * f + 1) * N + p]), w);
+        // trapezoid pair with r = x_t - z_k = -d
+        const double rx = -d.r, ry = -d.i;
+        const double aa = __builtin_fma(rx, nv.mx[p], ry * nv.my[p]);
+        const double bb = __builtin_fma(rx, nv.ex[p], ry * nv.ey[p]);
+        const double cc = (aa * bb) * (irho2 * irho2);
+        p1.tu = __builtin_fma(cc, rx, p1.tu);
+        p1.tv = __builtin_fma(cc, ry, p1.tv);
+    }
+    double v1[14];
+    v1[0] = p1.S0.r;
+    v1[1] = p1.S0.i;
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+        v1[2 + 2 * f] = p1.Nf[f].r;
+        v1[3 + 2 * f] = p1.Nf[f].i;
+    }
+    v1[12] = p1.tu;
+    v1[13] = p1.tv;
+#pragma unroll
+    for (int q = 0; q < 14; ++q) {
+        red[slice][lane] = v1[q];
+        __syncthreads();
+        double s = 0.0;
+        for (int t = 0; t < NS; ++t) s += red[t][lane];
+        v1[q] = s;
+        __syncthreads();
+    }
+    const cplx S0 = cmk(v1[0], v1[1]);
+    // exterior: num/(den - 2 pi i) (stokesSolver.f:5481); interior: num/den
+    const cplx dv = ext ? cmk(S0.r, S0.i - kTwoPi) : S0;
+    const cplx idv = cinv(dv);
+    cplx u[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) u[f] = cmul(cmk(v1[2 + 2 * f], v1[3 + 2 * f]), idv);
+
+    // pass 2 (fields 0..2): M_f = sum (b_f - u_f) dz/(z-x)^2;
+    // exterior also D_a = sum dz/((z-a)(z-x))
+    cplx M[3] = {cmk(0, 0), cmk(0, 0), cmk(0, 0)};
+    cplx Da = cmk(0, 0);
+    for (int k = k0; k < k1; ++k) {
+        const int p = off + k;
+        const cplx d = cmk(nv.x[p] - xt, nv.y[p] - yt);
+        const double rho2 = __builtin_fma(d.r, d.r, d.i * d.i);
+        const double irho2 = rcp64(rho2);
+        const cplx id = cmk(d.r * irho2, -d.i * irho2);
+        const cplx w = cmul(cmk(nv.dzr[p], nv.dzi[p]), id);
+        if (ext && !(sqrt(rho2) > 1.0e-10)) {
+#pragma unroll
+            for (int f = 0; f < 3; ++f) M[f] = cadd(M[f], coincident_term(nv, off, len, k, f, zt, a, d));
+        } else {
+            const cplx w2 = cmul(w, id);
+#pragma unroll
+            for (int f = 0; f < 3; ++f)
+                cfma(M[f], csub(cmk(nv.bd[(2 * f) * N + p], nv.bd[(2 * f + 1) * N + p]), u[f]), w2);
+        }
+        if (ext) cfma(Da, w, cmk(nv.iar[p], nv.iai[p]));
+    }
+    double v2[8] = {M[0].r, M[0].i, M[1].r, M[1].i, M[2].r, M[2].i, Da.r, Da.i};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        red[slice][lane] = v2[q];
+        __syncthreads();
+        double s = 0.0;
+        for (int t = 0; t < NS; ++t) s += red[t][lane];
+        v2[q] = s;
+        __syncthreads();
+    }
+    if (slice != 0 || !active) return;
+    cplx du[3];
+    if (ext) {
+        const cplx den = cmul(csub(zt, a), cmk(v2[6], v2[7]));  // (x-a) D_a
+        const cplx iden = cinv(den);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) du[f] = cmul(cmk(v2[2 * f], v2[2 * f + 1]), iden);
+    } else {
+        const cplx iS0 = cinv(S0);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) du[f] = cmul(cmk(v2[2 * f], v2[2 * f + 1]), iS0);
+    }
+    // u = Re v1' - x Re v2' - y Re v3' + Re v4 ; v = -Im v1' + x Im v2' + y Im v3' + Re v5
+    const double ux = du[0].r - xt * du[1].r - yt * du[2].r + u[3].r;
+    const double uy = -du[0].i + xt * du[1].i + yt * du[2].i + u[4].r;
+    corr_u[beg + lane] = ux - v1[12];
+    corr_v[beg + lane] = uy - v1[13];
+}
+
+// ------------------------------------------------- small reductions
+// block 0: rank-one integral sigdotn (stokesSolver.f:1107-1111);
+// block 1+b: constraint rows of body b (stokesSolver.f:1055-1096).
+__global__ __launch_bounds__(kBlk) void k_reduce_small(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ px,
+    const double* __restrict__ py, const double* __restrict__ speed,
+    const double* __restrict__ ex, const double* __restrict__ ey, const double* __restrict__ den,
+    int no, int n, int m, double* __restrict__ scal, double* __restrict__ out) {
+    __shared__ double s[3][kBlk];
+    const int b = blockIdx.x;
+    double a0 = 0, a1 = 0, a2 = 0;
+    if (b == 0) {
+        for (int k = threadIdx.x; k < no; k += kBlk)
+            a0 += (px[k] * ex[k] + py[k] * ey[k]) * speed[k];
+    } else {
+        const int off = no + (b - 1) * n;
+        for (int k = threadIdx.x; k < n; k += kBlk) {
+            const int p = off + k;
+            a0 -= ex[p] * speed[p];
+            a1 -= ey[p] * speed[p];
+            a2 -= (ex[p] * y[p] - ey[p] * x[p]) * speed[p];
+        }
+    }
+    s[0][threadIdx.x] = a0;
+    s[1][threadIdx.x] = a1;
+    s[2][threadIdx.x] = a2;
+    __syncthreads();
+    for (int w = kBlk / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s[0][threadIdx.x] += s[0][threadIdx.x + w];
+            s[1][threadIdx.x] += s[1][threadIdx.x + w];
+            s[2][threadIdx.x] += s[2][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    if (b == 0) {
+        scal[0] = s[0][0] * kTwoPi / static_cast<double>(no);
+    } else {
+        const int base = 2 * no + 2 * n * m;
+        const int r = base + 3 * (b - 1);
+        const double sc = kTwoPi / static_cast<double>(n);
+        out[r + 0] = s[0][0] / kTwoPi * sc + den[r + 0];
+        out[r + 1] = s[1][0] / kTwoPi * sc + den[r + 1];
+        out[r + 2] = s[2][0] / kTwoPi * sc + den[r + 2];
+    }
+}
+
+// ----------------------------------------------------------- assembly
+__global__ __launch_bounds__(kBlk) void k_assemble(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ px,
+    const double* __restrict__ py, const double* __restrict__ cur,
+    const double* __restrict__ speed, const double* __restrict__ ex,
+    const double* __restrict__ ey, const double* __restrict__ cx, const double* __restrict__ cy,
+    const double* __restrict__ den, const double* __restrict__ part_u,
+    const double* __restrict__ part_v, int nsplit, const int* __restrict__ tgt_ptr,
+    const int* __restrict__ tgt_ent, const double* __restrict__ corr_u,
+    const double* __restrict__ corr_v, int has_near, const double* __restrict__ scal, int no,
+    int n, int m, int npts, double* __restrict__ out) {
+    __shared__ double sc[5][kBlk];
+    const int i = blockIdx.x * kBlk + threadIdx.x;
+    const bool valid = i < npts;
+    const int ii = valid ? i : 0;
+    double u = 0.0, v = 0.0;
+    if (valid) {
+        for (int s = 0; s < nsplit; ++s) {
+            u += part_u[static_cast<size_t>(s) * npts + i];
+            v += part_v[static_cast<size_t>(s) * npts + i];
+        }
+    }
+    const double xi = x[ii], yi = y[ii];
+    // near-pair replacement
+    if (valid && has_near) {
+        for (int e = tgt_ptr[i]; e < tgt_ptr[i + 1]; ++e) {
+            u += corr_u[tgt_ent[e]];
+            v += corr_v[tgt_ent[e]];
+        }
+    }
+    // curvature diagonal and jump
+    {
+        const double nsrc = ii < no ? static_cast<double>(no) : static_cast<double>(n);
+        const double dx = ex[ii], dy = ey[ii];
+        const double tdotden = py[ii] * dx - px[ii] * dy;
+        u = u - cur[ii] * tdotden * py[ii] * speed[ii] / nsrc;
+        v = v + cur[ii] * tdotden * px[ii] * speed[ii] / nsrc;
+        u = u - 0.5 * dx;
+        v = v - 0.5 * dy;
+    }
+    // stokeslets and rotlets of every body
+    const int base = 2 * no + 2 * n * m;
+    for (int b0 = 0; b0 < m; b0 += kBlk) {
+        __syncthreads();
+        const int bb = b0 + threadIdx.x;
+        if (bb < m) {
+            sc[0][threadIdx.x] = cx[bb];
+            sc[1][threadIdx.x] = cy[bb];
+            sc[2][threadIdx.x] = den[base + 3 * bb];
+            sc[3][threadIdx.x] = den[base + 3 * bb + 1];
+            sc[4][threadIdx.x] = den[base + 3 * bb + 2];
+        }
+        __syncthreads();
+        const int cnt = min(kBlk, m - b0);
+        for (int q = 0; q < cnt; ++q) {
+            const double rx = xi - sc[0][q], ry = yi - sc[1][q];
+            const double sto1 = sc[2][q], sto2 = sc[3][q], rot = sc[4][q];
+            const double rho2 = rx * rx + ry * ry;
+            const double irho2 = 1.0 / rho2;
+            const double lg = log(rho2);
+            const double rdots = rx * sto1 + ry * sto2;
+            u += 0.5 / kTwoPi * (-0.5 * lg * sto1 + rdots * irho2 * rx) + rot * ry * irho2;
+            v += 0.5 / kTwoPi * (-0.5 * lg * sto2 + rdots * irho2 * ry) - rot * rx * irho2;
+        }
+    }
+    if (!valid) return;
+    if (i < no) {
+        const double sig = scal[0];
+        u += sig * px[i];
+        v += sig * py[i];
+    }
+    out[row_x(i, no, n)] = u;
+    out[row_y(i, no, n)] = v;
+}
+
+__global__ void k_rhs(const double* __restrict__ y, int no, int ibc, int nunk,
+                      double* __restrict__ rhs) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nunk) return;
+    double v = 0.0;
+    if (i < no) v = ibc == 0 ? 1.0 * (1.0 + y[i]) * (1.0 - y[i]) : (ibc == 1 ? 1.0 : 0.0);
+    rhs[i] = v;
+}
+
+}  // namespace
+
+// bd_condition (stokesSolver.f:208-314): background flow on the wall rows.
+void bd_condition(Context& c, int ibc, double* rhs_dev) {
+    const Geometry& g = c.g;
+    k_rhs<<<cdiv_int(g.nunk, kBlk), kBlk, 0, c.stream>>>(g.y.p, g.no, ibc, g.nunk, rhs_dev);
+    STOKES_LAUNCH_CHECK();
+}
+
+void gather_density(Context& c, const double* den_dev) {
+    const Geometry& g = c.g;
+    c.ex.ensure(static_cast<size_t>(g.npts));
+    c.ey.ensure(static_cast<size_t>(g.npts));
+    k_gather<<<cdiv_int(g.npts, kBlk), kBlk, 0, c.stream>>>(den_dev, g.no, g.n, g.npts, c.ex.p,
+                                                            c.ey.p);
+    STOKES_LAUNCH_CHECK();
+}
+
+// Five Cauchy densities of the gathered density and their boundary values on
+// the listed curves; the wall, when listed, must come first and gets 16
+// k-slices per block, bodies get 4.
+void boundary_values(Context& c, const int* curves_dev, int ncurves, int wall_first) {
+    const Geometry& g = c.g;
+    const int np = g.npts;
+    const size_t N = static_cast<size_t>(np);
+    hipStream_t s = c.stream;
+    c.fld.ensure(10 * N);
+    c.bd.ensure(10 * N);
+    k_fields<<<cdiv_int(np, kBlk), kBlk, 0, s>>>(g.x.p, g.y.p, g.px.p, g.py.p, c.ex.p, c.ey.p, np,
+                                                 c.fld.p);
+    STOKES_LAUNCH_CHECK();
+    if (wall_first) {
+        k_bdval<16><<<dim3(cdiv_int(g.no, 64), 1), 1024, 0, s>>>(curves_dev, g.no, g.n, np, g.x.p,
+                                                                 g.y.p, g.dzr.p, g.dzi.p,
+                                                                 g.dtab_no.p, g.dtab_n.p, c.fld.p,
+                                                                 c.bd.p);
+        STOKES_LAUNCH_CHECK();
+    }
+    if (ncurves > wall_first) {
+        k_bdval<4><<<dim3(cdiv_int(g.n, 64), ncurves - wall_first), 256, 0, s>>>(
+            curves_dev + wall_first, g.no, g.n, np, g.x.p, g.y.p, g.dzr.p, g.dzi.p, g.dtab_no.p,
+            g.dtab_n.p, c.fld.p, c.bd.p);
+        STOKES_LAUNCH_CHECK();
+    }
+}
+
+void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary) {
+    Geometry& g = c.g;
+    NearList& L = c.near;
+    hipStream_t s = c.stream;
+    const int np = g.npts;
+    const size_t N = static_cast<size_t>(np);
+    c.scal.ensure(8);
+    // far-field decomposition: enough blocks to fill 256 CUs several times
+    const int tblocks = cdiv_int(np, kBlk);
+    int want = std::max(1, cdiv_int(2048, tblocks));
+    int split_len = cdiv_int(cdiv_int(np, want), kTile) * kTile;
+    int nsplit = cdiv_int(np, split_len);
+    c.ff_splits = nsplit;
+    c.part_u.ensure(N * static_cast<size_t>(nsplit));
+    c.part_v.ensure(N * static_cast<size_t>(nsplit));
+
+    gather_density(c, x_dev);
+    if (c.time_kernels) STOKES_HIP(hipEventRecord(c.ev[0], s));
+    k_farfield<<<dim3(tblocks, nsplit), kBlk, 0, s>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
+                                                      np, split_len, c.part_u.p, c.part_v.p);
+    STOKES_LAUNCH_CHECK();
+    if (c.time_kernels) STOKES_HIP(hipEventRecord(c.ev[1], s));
+    const bool has_near = ibary == 1 && L.nentries > 0;
+    if (has_near) {
+        boundary_values(c, L.active.p, L.nactive, L.wall_active);
+        NodeView nv{g.x.p, g.y.p, g.dzr.p, g.dzi.p, c.bd.p, g.iar.p, g.iai.p, g.mx.p, g.my.p,
+                    c.ex.p, c.ey.p, N};
+        if (L.wall_tasks > 0) {
+            k_near_eval<16><<<L.wall_tasks, 1024, 0, s>>>(L.task_src.p, L.task_beg.p,
+                                                          L.task_cnt.p, L.ent_tgt.p, nv, g.ax.p,
+                                                          g.ay.p, g.no, g.n, 0, L.corr_u.p,
+                                                          L.corr_v.p);
+            STOKES_LAUNCH_CHECK();
+        }
+        if (L.ntasks > L.wall_tasks) {
+            k_near_eval<4><<<L.ntasks - L.wall_tasks, 256, 0, s>>>(
+                L.task_src.p, L.task_beg.p, L.task_cnt.p, L.ent_tgt.p, nv, g.ax.p, g.ay.p, g.no,
+                g.n, L.wall_tasks, L.corr_u.p, L.corr_v.p);
+            STOKES_LAUNCH_CHECK();
+        }
+    }
+    if (c.time_kernels) {
+        STOKES_HIP(hipEventRecord(c.ev[2], s));
+        STOKES_HIP(hipEventSynchronize(c.ev[2]));
+        float ff = 0.f, nr = 0.f;
+        STOKES_HIP(hipEventElapsedTime(&ff, c.ev[0], c.ev[1]));
+        STOKES_HIP(hipEventElapsedTime(&nr, c.ev[1], c.ev[2]));
+        c.ff_ms += ff;
+        c.ff_launches += 1;
+        if (has_near) {
+            c.near_ms += nr;
+            c.near_launches += 1;
+        }
+    }
+    k_reduce_small<<<g.m + 1, kBlk, 0, s>>>(g.x.p, g.y.p, g.px.p, g.py.p, g.speed.p, c.ex.p,
+                                            c.ey.p, x_dev, g.no, g.n, g.m, c.scal.p, y_dev);
+    STOKES_LAUNCH_CHECK();
+    k_assemble<<<cdiv_int(np, kBlk), kBlk, 0, s>>>(
+        g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p, c.ex.p, c.ey.p, g.cx.p, g.cy.p, x_dev,
+        c.part_u.p, c.part_v.p, nsplit, L.tgt_ptr.p, L.tgt_ent.p, L.corr_u.p, L.corr_v.p,
+        has_near ? 1 : 0, c.scal.p, g.no, g.n, g.m, np, y_dev);
+    STOKES_LAUNCH_CHECK();
+}
+
+}  // namespace stokes

@@ -1,0 +1,141 @@
+"""ctypes binding of libstokes_hip.so (C-ABI in include/erosion_stokes.h).
+
+The library is the only compute path: loading fails loudly when it has not
+been built (``python -m erosion_amd.build``) and every call checks the
+library's status channel, raising on error.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libstokes_hip.so")
+
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_dp = ctypes.POINTER(ctypes.c_double)
+_lib = None
+
+
+class StokesError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libstokes_hip.so; raise if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise StokesError(f"{LIB_PATH} not built: run `python -m erosion_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.stokessolver_.argtypes = [_i64p] * 7 + [_dp, _dp, _dp, _i64p]
+        L.stokessolver_.restype = None
+        L.computeshearstress_.argtypes = [_i64p, _i64p, _i64p, _dp, _dp, _dp, _i64p, _dp]
+        L.computeshearstress_.restype = None
+        L.erosion_set_gmres_tol.argtypes = [ctypes.c_double]
+        L.erosion_set_gmres_tol.restype = ctypes.c_int
+        L.erosion_get_gmres_tol.restype = ctypes.c_double
+        L.erosion_last_status.restype = ctypes.c_int
+        L.erosion_last_error.restype = ctypes.c_char_p
+        L.erosion_last_solve_info.argtypes = [_dp]
+        L.erosion_last_solve_info.restype = ctypes.c_int
+        L.erosion_matvec.argtypes = [ctypes.c_int64] * 4 + [_dp, _dp, _dp, _dp]
+        L.erosion_matvec.restype = ctypes.c_int
+        L.erosion_time_matvec.argtypes = [ctypes.c_int64] * 4 + [_dp, _dp, _dp, ctypes.c_int64, _dp]
+        L.erosion_time_matvec.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(_dp)
+
+
+def _ref(v: int):
+    return ctypes.byref(ctypes.c_int64(int(v)))
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def nunk(npts: int, nbods: int, nouter: int) -> int:
+    return 2 * nouter + 2 * npts * nbods + 3 * nbods
+
+
+def last_status() -> int:
+    return int(lib().erosion_last_status())
+
+
+def last_error() -> str:
+    return lib().erosion_last_error().decode()
+
+
+def _check(where: str) -> None:
+    st = last_status()
+    if st < 0:
+        raise StokesError(f"{where}: {last_error()}")
+
+
+def set_gmres_tol(tol: float) -> None:
+    if lib().erosion_set_gmres_tol(float(tol)) != 0:
+        raise StokesError(f"invalid GMRES tolerance {tol}")
+
+
+def get_gmres_tol() -> float:
+    return float(lib().erosion_get_gmres_tol())
+
+
+def last_solve_info() -> dict:
+    info = np.zeros(8)
+    lib().erosion_last_solve_info(_p(info))
+    return {"iters": int(info[0]), "implicit_relres": float(info[1]),
+            "true_relres": float(info[2]), "seconds": float(info[3]), "reorth": int(info[4])}
+
+
+def stokessolver(npts: int, nbods: int, nouter: int, ifmm: int, ibary: int, ibc: int, maxl: int,
+                 xx, yy, den: Optional[np.ndarray] = None) -> Tuple[np.ndarray, int]:
+    """Call stokessolver_ exactly as DensityStress.jl:48-51 does.
+
+    ``den`` is the initial guess (zeros when omitted); returns (den, nits)."""
+    xx, yy = _f64(xx), _f64(yy)
+    out = np.zeros(nunk(npts, nbods, nouter)) if den is None else _f64(den).copy()
+    nits = ctypes.c_int64(0)
+    lib().stokessolver_(_ref(npts), _ref(nbods), _ref(nouter), _ref(ifmm), _ref(ibary), _ref(ibc),
+                        _ref(maxl), _p(xx), _p(yy), _p(out), ctypes.byref(nits))
+    _check("stokessolver_")
+    return out, int(nits.value)
+
+
+def computeshearstress(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: int = 1) -> np.ndarray:
+    """Call computeshearstress_ exactly as DensityStress.jl:195-198 does."""
+    xx, yy, den = _f64(xx), _f64(yy), _f64(den)
+    tau = np.zeros(npts * nbods)
+    lib().computeshearstress_(_ref(npts), _ref(nbods), _ref(nouter), _p(xx), _p(yy), _p(den),
+                              _ref(ibary), _p(tau))
+    _check("computeshearstress_")
+    return tau
+
+
+def time_matvec(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: int = 1,
+                reps: int = 10) -> dict:
+    """Operator and far-field kernel durations measured with HIP events."""
+    xx, yy, den = _f64(xx), _f64(yy), _f64(den)
+    out = np.zeros(8)
+    if lib().erosion_time_matvec(npts, nbods, nouter, ibary, _p(xx), _p(yy), _p(den), int(reps),
+                                 _p(out)) != 0:
+        raise StokesError(f"erosion_time_matvec: {last_error()}")
+    return {"matvec_ms": out[0], "farfield_ms": out[1], "pairs": out[2], "near_entries": int(out[3]),
+            "near_ms": out[4], "splits": int(out[5])}
+
+
+def matvec(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: int = 1) -> np.ndarray:
+    """One application of the completed double-layer operator."""
+    xx, yy, den = _f64(xx), _f64(yy), _f64(den)
+    vel = np.zeros(nunk(npts, nbods, nouter))
+    if lib().erosion_matvec(npts, nbods, nouter, ibary, _p(xx), _p(yy), _p(den), _p(vel)) != 0:
+        raise StokesError(f"erosion_matvec: {last_error()}")
+    return vel

@@ -1,0 +1,104 @@
+/*
+ * erosion_stokes.h -- C-ABI of libstokes_hip.so, the MI355X (gfx950) Stokes
+ * solver of the Erosion package.
+ *
+ * The two Fortran-convention entry points have exactly the signatures the
+ * reference's libstokes.so exports and that ErosionPackage/src/DensityStress.jl
+ * binds with ccall: lower-case name with a trailing underscore, every
+ * argument by pointer, integers as Julia's Ref{Int} (64-bit).  A caller that
+ * swaps the library path (DensityStress.jl:19) needs no other change.
+ *
+ * Ownership: the caller allocates every array; the library keeps no caller
+ * pointer after a call returns.  Device buffers are library-owned and reused
+ * across calls.  Calls are synchronous and not reentrant (the reference keeps
+ * its state in COMMON blocks and is not reentrant either).
+ *
+ * Errors: the reference has no error channel.  These entry points never abort
+ * the host process; a failure prints one line to stderr, leaves the outputs
+ * as described per function and is reported by erosion_last_status() /
+ * erosion_last_error().
+ *
+ * Array layouts (stokesSolver.f:631-657):
+ *   xx, yy [npts*nbods]   body points, body-major
+ *   den  [2*nouter + 2*npts*nbods + 3*nbods]
+ *        [eta_x wall | eta_y wall | per body (eta_x | eta_y) | per body
+ *         (stokeslet_x, stokeslet_y, rotlet)]
+ *   tau  [npts*nbods]     shear stress, body-major
+ */
+#ifndef EROSION_STOKES_H
+#define EROSION_STOKES_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Density of the completed double-layer boundary integral equation.
+ * Replaces subroutine stokesSolver (ErosionPackage/fortran_code/stokesSolver.f:1-131),
+ * bound at ErosionPackage/src/DensityStress.jl:48-51.
+ *   den   in: initial guess (Julia passes zeros); out: solution
+ *   nits  out: GMRES iterations
+ * GMRES stops when the implicit residual estimate is <= tol * ||b||, tol as
+ * set by erosion_set_gmres_tol (default 1e-8, the reference's hard-coded value,
+ * stokesSolver.f:482).  ibary=1 selects the near-pair barycentric operator of
+ * matvec_DLP_fmmbary, ibary=0 the trapezoid operator of matvec_DLP.  The far
+ * field is summed directly on the GPU for either value of ifmm; the
+ * all-pairs-barycentric operator of ifmm=0, ibary=1 (matvec_DLP_bary) is not
+ * provided and that combination is reported as an error (den untouched,
+ * nits = 0). */
+void stokessolver_(const int64_t* npts, const int64_t* nbods, const int64_t* nouter,
+                   const int64_t* ifmm, const int64_t* ibary, const int64_t* ibc,
+                   const int64_t* maxl, const double* xx, const double* yy, double* den,
+                   int64_t* nits);
+
+/* Shear stress on every body point.
+ * Replaces subroutine computeShearStress with ib=1
+ * (ErosionPackage/fortran_code/stokesSolver.f:3002-3096, deformation_on_boundary
+ * :2618-2998), bound at ErosionPackage/src/DensityStress.jl:195-198.
+ * ibary=0 (the trapezoid variant deformation_on_boundary_trap, :2305-2614) is
+ * not provided and is reported as an error (tau zeroed). */
+void computeshearstress_(const int64_t* npts, const int64_t* nbods, const int64_t* nouter,
+                         const double* xx, const double* yy, const double* den,
+                         const int64_t* ibary, double* tau);
+
+/* ---- Additions with no counterpart in the reference ------------------------
+ * The reference fixes the GMRES tolerance in source (stokesSolver.f:482) and
+ * has no status channel; these let a caller run at another tolerance and read
+ * what happened without changing the two signatures above. */
+
+/* GMRES relative tolerance for stokessolver_; returns 0, or -1 if tol <= 0. */
+int erosion_set_gmres_tol(double tol);
+double erosion_get_gmres_tol(void);
+
+/* 0 ok; 1 GMRES stopped at maxl without meeting the tolerance; 2 GMRES
+ * breakdown; -1 error (see erosion_last_error). */
+int erosion_last_status(void);
+const char* erosion_last_error(void);
+
+/* Record of the last stokessolver_ call:
+ * info[0] iterations, info[1] implicit relative residual, info[2] true relative
+ * residual ||b - A x|| / ||b|| of the returned density, info[3] seconds,
+ * info[4] re-orthogonalised iterations.  Returns 0. */
+int erosion_last_solve_info(double* info);
+
+/* One application of the completed operator vel = A den, as matvec_DLP_fmmbary
+ * (ibary=1, stokesSolver.f:570-1124, far field summed directly) or matvec_DLP
+ * (ibary=0, stokesSolver.f:1519-1912).  Returns 0 or -1. */
+int erosion_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t ibary,
+                   const double* xx, const double* yy, const double* den, double* vel);
+
+/* Measurement hook: set up the geometry once, then time `reps` operator
+ * applications with HIP events on the library's stream.
+ * out[0] ms per application, out[1] ms per far-field kernel launch, out[2]
+ * ordered source-target pairs per far-field launch, out[3] near-pair entries,
+ * out[4] ms per near-pair correction (boundary values + evaluation), out[5]
+ * far-field source splits.  Returns 0 or -1. */
+int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t ibary,
+                        const double* xx, const double* yy, const double* den, int64_t reps,
+                        double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,147 @@
+"""Parity of the HIP library against the oracle and the reference goldens.
+
+Every call goes through the C-ABI of libstokes_hip.so.  Tolerances:
+  operator (matvec)  1e-12 relative (max-norm) to the FP64 oracle / reference
+  solve              density 1e-10 relative to the reference at tol 1e-12
+  shear stress       1e-10 relative to the reference on the same density
+"""
+import numpy as np
+import pytest
+
+from conftest import relerr
+from erosion_amd import stokes
+from erosion_amd.geometry import WORKLOADS, make_bodies, workload_xy
+from erosion_amd.theta_len import getnxy, new_thlenden
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+FIXTURES = ["tight8_n64", "bodies10_n128"]
+
+
+def _case(g):
+    return int(g["n"]), int(g["m"]), int(g["no"]), g["xx"], g["yy"]
+
+
+@pytest.fixture(autouse=True)
+def _tol_reset():
+    stokes.set_gmres_tol(1e-8)
+    yield
+    stokes.set_gmres_tol(1e-8)
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+@pytest.mark.parametrize("ibary", [0, 1])
+def test_matvec_matches_reference(golden, name, ibary):
+    g = golden(name)
+    n, m, no, xx, yy = _case(g)
+    v = stokes.matvec(n, m, no, xx, yy, g["eta"], ibary=ibary)
+    ref = g["mv_fmmbary"] if ibary else g["mv_trap"]
+    assert relerr(v, ref) < 1e-12
+    assert relerr(v, O.Geometry(n, m, no, xx, yy).matvec(g["eta"], ibary)) < 1e-12
+
+
+def test_matvec_full_size_vs_oracle():
+    """100 bodies x 256 points, wall 2048 (27,648 boundary points)."""
+    n, m, no, xx, yy = workload_xy(WORKLOADS["bodies100_n256"])
+    N = 2 * no + 2 * n * m + 3 * m
+    eta = np.random.default_rng(7).standard_normal(N)
+    v = stokes.matvec(n, m, no, xx, yy, eta, ibary=1)
+    vo = O.Geometry(n, m, no, xx, yy).matvec(eta, 1)
+    assert relerr(v, vo) < 1e-12
+
+
+def test_matvec_tight_geometry_many_near_pairs():
+    """Bodies 0.002 apart and against the wall: every near branch with many entries."""
+    thl = make_bodies(30, 128, 0.05, 0.12, 0.002, 31)
+    m, xx, yy = getnxy(new_thlenden(thl))
+    n, no = 128, 1024
+    N = 2 * no + 2 * n * m + 3 * m
+    eta = np.random.default_rng(8).standard_normal(N)
+    geo = O.Geometry(n, m, no, xx, yy)
+    vo = geo.matvec(eta, 1)
+    assert relerr(vo, geo.matvec(eta, 0)) > 1e-6  # near corrections are active
+    assert relerr(stokes.matvec(n, m, no, xx, yy, eta, ibary=1), vo) < 1e-12
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_solve_matches_reference(golden, name):
+    g = golden(name)
+    n, m, no, xx, yy = _case(g)
+    den, it = stokes.stokessolver(n, m, no, 1, 1, 0, 2000, xx, yy)
+    assert stokes.last_status() == 0
+    it_ref = int(g["iter_tol8"])
+    assert abs(it - it_ref) <= max(2, it_ref // 50)
+    assert relerr(den, g["den_tol8"]) < 1e-7
+    stokes.set_gmres_tol(1e-12)
+    den12, it12 = stokes.stokessolver(n, m, no, 1, 1, 0, 2000, xx, yy)
+    info = stokes.last_solve_info()
+    assert relerr(den12, g["den_tol12"]) < 1e-10
+    assert info["true_relres"] < 1e-10
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_shear_stress_matches_reference(golden, name):
+    g = golden(name)
+    n, m, no, xx, yy = _case(g)
+    tau = stokes.computeshearstress(n, m, no, xx, yy, g["den_tol12"], 1)
+    assert relerr(tau, g["tau_tol12"]) < 1e-10
+
+
+def test_shear_stress_vs_oracle_20_bodies():
+    n, m, no, xx, yy = workload_xy(WORKLOADS["bodies100_n256"], nbodies=20)
+    N = 2 * no + 2 * n * m + 3 * m
+    den = np.random.default_rng(9).standard_normal(N) * 0.1
+    tau = stokes.computeshearstress(n, m, no, xx, yy, den, 1)
+    assert relerr(tau, O.shear_stress(n, m, no, xx, yy, den)) < 1e-10
+
+
+def test_trapezoid_solve_vs_reference(golden):
+    g = golden("tight8_n64")
+    n, m, no, xx, yy = _case(g)
+    stokes.set_gmres_tol(1e-12)
+    den, it = stokes.stokessolver(n, m, no, 0, 0, 0, 2000, xx, yy)
+    assert relerr(den, g["den_trap_tol12"]) < 1e-10
+
+
+def test_wall_only():
+    """No bodies: the wall alone (rows of the stokeslet/constraint blocks vanish)."""
+    n, m, no = 32, 0, 256
+    xx = yy = np.zeros(0)
+    stokes.set_gmres_tol(1e-12)
+    den, it = stokes.stokessolver(n, m, no, 1, 1, 0, 500, xx, yy)
+    ref, it_o = O.stokessolver(n, m, no, xx, yy, ibary=1, ibc=0, maxl=500, tol=1e-12)
+    assert relerr(den, ref) < 1e-10
+    assert stokes.computeshearstress(n, m, no, xx, yy, den, 1).size == 0
+
+
+def test_initial_guess_is_used(golden):
+    g = golden("tight8_n64")
+    n, m, no, xx, yy = _case(g)
+    den, it = stokes.stokessolver(n, m, no, 1, 1, 0, 2000, xx, yy, den=g["den_tol12"])
+    assert it == 0  # the guess already meets 1e-8
+    assert np.array_equal(den, g["den_tol12"])
+
+
+def test_maxl_exhaustion(golden):
+    g = golden("tight8_n64")
+    n, m, no, xx, yy = _case(g)
+    den, it = stokes.stokessolver(n, m, no, 1, 1, 0, 7, xx, yy)
+    assert it == 7 and stokes.last_status() == 1
+    geo = O.Geometry(n, m, no, xx, yy)
+    ref, it_o, _err, ierr = geo.gmres(geo.rhs(0), ibary=1, maxl=7, tol=1e-8)
+    assert it_o == 7 and ierr == 1
+    assert relerr(den, ref) < 1e-10
+
+
+def test_unsupported_combinations_raise(golden):
+    g = golden("tight8_n64")
+    n, m, no, xx, yy = _case(g)
+    with pytest.raises(stokes.StokesError):
+        stokes.stokessolver(n, m, no, 0, 1, 0, 100, xx, yy)
+    with pytest.raises(stokes.StokesError):
+        stokes.computeshearstress(n, m, no, xx, yy, g["den_tol12"], 0)
+    with pytest.raises(stokes.StokesError):
+        stokes.matvec(0, m, no, xx, yy, g["eta"], 1)
+    # the library stays usable after an error
+    v = stokes.matvec(n, m, no, xx, yy, g["eta"], ibary=1)
+    assert relerr(v, g["mv_fmmbary"]) < 1e-12
