@@ -1,0 +1,226 @@
+#!/usr/bin/env python
+"""Erosion steps/s of the per-step Stokes solve on one MI355X.
+
+A step is one ``rungekutta2`` call (ErosionPackage/src/TimeStepping.jl:
+163-176): two GMRES density solves to a 1e-10 residual and two shear-stress
+evaluations through the C-ABI of libstokes_hip.so, with the theta-L updates
+on the host in between.  Workload: 100 bodies x 256 points, wall 2048 points,
+seeded perturbed-circle geometry, pipe-flow boundary condition, dt = 1e-3,
+fixpdrop off.
+
+Prints ONE JSON line on stdout (everything else goes to stderr):
+  value        erosion steps/s over the timed steps
+  roofline     the dominant kernel's achieved rate against the MI355X peak,
+               from HIP events recorded around every launch in the timed region
+  cpu_baseline the reference's own Fortran (oracle/_ref, flang build of
+               stokesSolver.f + the FMM) timed on this host on a bounded sample
+"""
+from __future__ import annotations
+
+import argparse
+import contextlib
+import json
+import math
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "erosion steps/s at 1e-10 GMRES residual: learned+mixed vs plain FP64 on GPU"
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak
+PEAK_HBM_GBS = 8000.0
+FF_FLOP_PER_PAIR = 20.0  # canonical flops per ordered source-target pair
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+@contextlib.contextmanager
+def quiet_stdout(workdir):
+    """Silence the reference's Fortran prints and keep its side files out of the tree."""
+    sys.stdout.flush()
+    fd = os.dup(1)
+    old = os.getcwd()
+    with open(os.devnull, "w") as dn:
+        os.dup2(dn.fileno(), 1)
+        os.chdir(workdir)
+        try:
+            yield
+        finally:
+            os.chdir(old)
+            os.dup2(fd, 1)
+            os.close(fd)
+
+
+def cpu_baseline(w, xx, yy, iters_per_solve):
+    """Reference CPU rate on a bounded sample, scaled to steps/s."""
+    import numpy as np
+    from oracle import reference as R
+
+    if not R.available():
+        return None
+    n, m, no = w.npts, w.nbodies, w.nouter
+    N = 2 * no + 2 * n * m + 3 * m
+    eta = np.random.default_rng(3).standard_normal(N)
+    msub = min(10, m)
+    with tempfile.TemporaryDirectory() as td, quiet_stdout(td):
+        R.matvec(n, m, no, xx, yy, eta, ibary=1, ifmm=1)  # warm-up (FMM tables, threads)
+        t0 = time.perf_counter()
+        reps = 2
+        for _ in range(reps):
+            R.matvec(n, m, no, xx, yy, eta, ibary=1, ifmm=1)
+        t_mv = (time.perf_counter() - t0) / reps
+        den = np.random.default_rng(4).standard_normal(2 * no + 2 * n * msub + 3 * msub) * 0.1
+        t0 = time.perf_counter()
+        R.shear_stress(n, msub, no, xx[: n * msub], yy[: n * msub], den)
+        t_st_sub = time.perf_counter() - t0
+
+    # reference stress cost in complex multiply-adds: wall->bodies, wall
+    # boundary values per body, body->body pairs (stokesSolver.f:2707-2925)
+    def cost(mm):
+        return 5 * mm * no * no + 15 * mm * n * no + 20 * mm * (mm - 1) * n * n
+
+    t_st = t_st_sub * cost(m) / cost(msub)
+    t_step = 2 * iters_per_solve * t_mv + 2 * t_st
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": 1.0 / t_step,
+        "unit": "erosion steps/s",
+        "cores": threads,
+        "kind": "reference",
+        "sample": (f"flang build of the reference (stokesSolver.f + FMM, OpenMP {threads} threads): "
+                   f"{reps} matvec_DLP_fmmbary calls at the full workload ({t_mv:.3f} s each) and one "
+                   f"computeShearStress on {msub} of the {m} bodies ({t_st_sub:.2f} s, scaled by its "
+                   f"pair count to {t_st:.1f} s); step = 2 solves x {iters_per_solve:.0f} GMRES "
+                   f"iterations (the GPU's count) + 2 stresses; DGMRES orthogonalisation not counted"),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="bodies100_n256")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        log("bench.py: this build runs the Stokes solve on one GPU only; "
+            "multi-GPU runs are not part of it (see DESIGN.md for how the solve would shard).")
+        return 2
+
+    import numpy as np
+
+    from erosion_amd import stokes
+    from erosion_amd.geometry import WORKLOADS, workload_geometry
+    from erosion_amd.theta_len import ParamSet, getnxy
+    from erosion_amd.time_stepping import rungekutta2
+
+    try:
+        import torch
+
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is plumbing only
+        def sync():
+            return None
+
+    w = WORKLOADS[args.workload]
+    params = ParamSet(npts=w.npts, nouter=w.nouter, ibary=1, ifmm=1, ibc=w.ibc, dt=w.dt,
+                      maxl=8000, fixpdrop=False)
+    stokes.set_gmres_tol(args.tol)
+    thld = workload_geometry(w)
+    _m, xx0, yy0 = getnxy(thld)
+
+    timings = {}
+    for i in range(args.warmup):
+        thld = rungekutta2(thld, params, timings=timings)
+        log(f"warmup step {i + 1}/{args.warmup}: bodies={len(thld.thlenvec)}")
+    sync()
+    stokes.kernel_timing(True)
+    timings = {}
+    solve_info = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        ts = time.perf_counter()
+        thld = rungekutta2(thld, params, timings=timings)
+        solve_info.append(stokes.last_solve_info())
+        log(f"step {i + 1}/{args.steps}: {time.perf_counter() - ts:.3f} s, bodies={len(thld.thlenvec)}, "
+            f"gmres iters so far {timings.get('gmres_iters', 0)}, last true relres "
+            f"{solve_info[-1]['true_relres']:.2e}")
+    sync()
+    elapsed = time.perf_counter() - t0
+    kt = stokes.kernel_timers()
+    stokes.kernel_timing(False)
+
+    steps_per_s = args.steps / elapsed
+    solves = max(1, timings.get("solves", 1))
+    iters_per_solve = timings.get("gmres_iters", 0) / solves
+    ff = kt["farfield"]
+    ff_ms = ff["ms"] / max(1, ff["launches"])
+    ff_pairs = ff["work"] / max(1, ff["launches"])
+    ff_tflops = FF_FLOP_PER_PAIR * ff_pairs / (ff_ms * 1e-3) / 1e12
+    gs = kt["gram_schmidt"]
+    gs_gbs = gs["work"] / (gs["ms"] * 1e-3) / 1e9 if gs["ms"] > 0 else 0.0
+    se = kt["stress_eval"]
+    share = {k: kt[k]["ms"] / (elapsed * 1e3) for k in kt}
+    log(f"kernel time shares of the timed region: " +
+        ", ".join(f"{k} {100 * v:.1f}%" for k, v in share.items()))
+    log(f"far field: {ff_ms:.3f} ms/launch, {ff_tflops:.2f} TFLOP/s; Gram-Schmidt {gs_gbs:.0f} GB/s; "
+        f"stress eval {se['ms'] / max(1, se['launches']):.2f} ms/launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(w, xx0, yy0, iters_per_solve)
+        except Exception as e:  # the baseline is reported, never required
+            log(f"cpu baseline unavailable: {e}")
+
+    out = {
+        "metric": METRIC,
+        "value": steps_per_s,
+        "unit": "erosion steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * elapsed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded perturbed-circle porous geometry, theta-L state",
+        "config": {
+            "workload": w.name, "bodies": w.nbodies, "points_per_body": w.npts,
+            "wall_points": w.nouter, "gmres_tol": args.tol, "variant": "plain-fp64",
+            "dt": w.dt, "fixpdrop": False, "ibc": w.ibc,
+            "gmres_iters_per_solve": iters_per_solve,
+            "true_relres_last": solve_info[-1]["true_relres"] if solve_info else None,
+            "matvecs_per_s": ff["launches"] / elapsed,
+            "farfield_ms": ff_ms,
+            "gram_schmidt_GBps": gs_gbs,
+            "kernel_share": share,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "pipe": "VALU FP64 (the FP64 vector peak equals the FP64 matrix peak on MI355X)",
+            "kernel": "k_farfield (all-pairs double layer)",
+            "achieved": ff_tflops,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": ff_tflops / PEAK_FP64_TFLOPS,
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

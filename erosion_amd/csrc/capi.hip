@@ -208,41 +208,77 @@ int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t iba
         c.rvec.ensure(std::max(nunk, c.rvec.cap));
         STOKES_HIP(hipMemcpyAsync(c.xvec.p, den, nunk * sizeof(double), hipMemcpyHostToDevice,
                                   c.stream));
-        for (auto& e : c.ev)
-            if (!e) STOKES_HIP(hipEventCreate(&e));
         stokes::apply_operator(c, c.xvec.p, c.rvec.p, static_cast<int>(ibary));  // warm-up
         STOKES_HIP(hipStreamSynchronize(c.stream));
         // whole operator applications, back to back
-        STOKES_HIP(hipEventRecord(c.ev[3], c.stream));
+        hipEvent_t e0, e1;
+        STOKES_HIP(hipEventCreate(&e0));
+        STOKES_HIP(hipEventCreate(&e1));
+        STOKES_HIP(hipEventRecord(e0, c.stream));
         for (int64_t r = 0; r < reps; ++r)
             stokes::apply_operator(c, c.xvec.p, c.rvec.p, static_cast<int>(ibary));
-        hipEvent_t e_end;
-        STOKES_HIP(hipEventCreate(&e_end));
-        STOKES_HIP(hipEventRecord(e_end, c.stream));
-        STOKES_HIP(hipEventSynchronize(e_end));
+        STOKES_HIP(hipEventRecord(e1, c.stream));
+        STOKES_HIP(hipEventSynchronize(e1));
         float tot = 0.f;
-        STOKES_HIP(hipEventElapsedTime(&tot, c.ev[3], e_end));
-        STOKES_HIP(hipEventDestroy(e_end));
+        STOKES_HIP(hipEventElapsedTime(&tot, e0, e1));
+        STOKES_HIP(hipEventDestroy(e0));
+        STOKES_HIP(hipEventDestroy(e1));
         // per-kernel durations
+        const bool was = c.time_kernels;
+        c.t_farfield.reset();
+        c.t_near.reset();
         c.time_kernels = true;
-        c.ff_ms = c.near_ms = 0.0;
-        c.ff_launches = c.near_launches = 0;
         for (int64_t r = 0; r < reps; ++r)
             stokes::apply_operator(c, c.xvec.p, c.rvec.p, static_cast<int>(ibary));
-        STOKES_HIP(hipStreamSynchronize(c.stream));
-        c.time_kernels = false;
+        c.t_farfield.harvest();
+        c.t_near.harvest();
+        c.time_kernels = was;
         out[0] = static_cast<double>(tot) / static_cast<double>(reps);
-        out[1] = c.ff_ms / static_cast<double>(std::max<long long>(1, c.ff_launches));
+        out[1] = c.t_farfield.ms / static_cast<double>(std::max<long long>(1, c.t_farfield.launches));
         const double np = static_cast<double>(c.g.npts);
         out[2] = np * (np - 1.0);
         out[3] = static_cast<double>(c.near.nentries);
-        out[4] = c.near_launches ? c.near_ms / static_cast<double>(c.near_launches) : 0.0;
+        out[4] = c.t_near.launches ? c.t_near.ms / static_cast<double>(c.t_near.launches) : 0.0;
         out[5] = static_cast<double>(c.ff_splits);
         return 0;
     } catch (const std::exception& e) {
         Context& c = stokes::ctx();
         c.time_kernels = false;
         return record_error("erosion_time_matvec", e.what());
+    }
+}
+
+int erosion_kernel_timing(int enable) {
+    try {
+        Context& c = stokes::ctx();
+        if (enable) {
+            for (stokes::KernelTimer* t : {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs})
+                t->reset();
+        } else {
+            for (stokes::KernelTimer* t : {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs})
+                t->harvest();
+        }
+        c.time_kernels = enable != 0;
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_kernel_timing", e.what());
+    }
+}
+
+int erosion_kernel_timers(double* out) {
+    if (!out) return -1;
+    try {
+        Context& c = stokes::ctx();
+        stokes::KernelTimer* ts[4] = {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs};
+        for (int i = 0; i < 4; ++i) {
+            ts[i]->harvest();
+            out[3 * i + 0] = ts[i]->ms;
+            out[3 * i + 1] = static_cast<double>(ts[i]->launches);
+            out[3 * i + 2] = ts[i]->work;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_kernel_timers", e.what());
     }
 }
 

@@ -109,6 +109,48 @@ struct Geometry {
     std::vector<double> h_x, h_y, h_speed, h_cx, h_cy;
 };
 
+// Durations of one kernel family, measured with HIP events recorded around
+// each launch on the library stream.  Events come from a pool and are read
+// back only when the pool is full or on harvest(), so timing adds no host
+// synchronisation to the launch sequence.
+struct KernelTimer {
+    std::vector<hipEvent_t> ev;  // pairs: start 2i, stop 2i+1
+    size_t used = 0;
+    double ms = 0.0;
+    long long launches = 0;
+    double work = 0.0;  // algorithmic units summed over launches
+    void start(hipStream_t s) {
+        if (ev.empty()) {
+            ev.resize(2 * 512);
+            for (auto& e : ev) STOKES_HIP(hipEventCreate(&e));
+        }
+        if (used == ev.size() / 2) harvest();
+        STOKES_HIP(hipEventRecord(ev[2 * used], s));
+    }
+    void stop(hipStream_t s, double work_units) {
+        STOKES_HIP(hipEventRecord(ev[2 * used + 1], s));
+        ++used;
+        work += work_units;
+    }
+    void harvest() {
+        if (!used) return;
+        STOKES_HIP(hipEventSynchronize(ev[2 * used - 1]));
+        for (size_t i = 0; i < used; ++i) {
+            float t = 0.f;
+            STOKES_HIP(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
+            ms += t;
+        }
+        launches += static_cast<long long>(used);
+        used = 0;
+    }
+    void reset() {
+        harvest();
+        ms = 0.0;
+        launches = 0;
+        work = 0.0;
+    }
+};
+
 // Near-pair replacement list of matvec_DLP_fmmbary (stokesSolver.f:687-926):
 // an entry is (source curve, target point); interior rule iff source is wall.
 struct NearList {
@@ -160,9 +202,7 @@ struct Context {
     double tol = 1e-8;
     // per-kernel timing with HIP events on `stream` (measurement hooks only)
     bool time_kernels = false;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    double ff_ms = 0.0, near_ms = 0.0, stress_eval_ms = 0.0;
-    long long ff_launches = 0, near_launches = 0, stress_launches = 0;
+    KernelTimer t_farfield, t_near, t_stress_eval, t_gs;
     int status = 0;
     std::string last_error;
     // last solve record

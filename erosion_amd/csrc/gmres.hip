@@ -459,10 +459,15 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
             k_zero_tail<<<cdiv_int(static_cast<int>(npad) - N + 1, kBlk), kBlk, 0, s>>>(
                 w, N, static_cast<int>(npad));
             STOKES_LAUNCH_CHECK();
+            if (c.time_kernels) c.t_gs.start(s);
             gs_pass(c, w, kk + 1, 1, hcur.p);
             gs_pass(c, w, kk + 1, 2, hcur.p);
             k_givens<<<1, kGivT, 0, s>>>(c.H.p, ldh, c.q.p, c.hbuf.p, c.st.p, tol, maxl);
             STOKES_LAUNCH_CHECK();
+            if (c.time_kernels) {
+                // Krylov rows streamed by the dot and update kernels of pass 1
+                c.t_gs.stop(s, 2.0 * static_cast<double>(kk + 1) * static_cast<double>(npad) * 8.0);
+            }
             k_scale_row<<<std::min(1024, ebx * 2), kBlk, 0, s>>>(w, npad, c.st.p, 0);
             STOKES_LAUNCH_CHECK();
         }

@@ -555,12 +555,15 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary) {
     c.part_v.ensure(N * static_cast<size_t>(nsplit));
 
     gather_density(c, x_dev);
-    if (c.time_kernels) STOKES_HIP(hipEventRecord(c.ev[0], s));
+    if (c.time_kernels) c.t_farfield.start(s);
     k_farfield<<<dim3(tblocks, nsplit), kBlk, 0, s>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
                                                       np, split_len, c.part_u.p, c.part_v.p);
     STOKES_LAUNCH_CHECK();
-    if (c.time_kernels) STOKES_HIP(hipEventRecord(c.ev[1], s));
+    if (c.time_kernels) {
+        c.t_farfield.stop(s, static_cast<double>(np) * static_cast<double>(np - 1));
+    }
     const bool has_near = ibary == 1 && L.nentries > 0;
+    if (has_near && c.time_kernels) c.t_near.start(s);
     if (has_near) {
         boundary_values(c, L.active.p, L.nactive, L.wall_active);
         NodeView nv{g.x.p, g.y.p, g.dzr.p, g.dzi.p, c.bd.p, g.iar.p, g.iai.p, g.mx.p, g.my.p,
@@ -579,18 +582,8 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary) {
             STOKES_LAUNCH_CHECK();
         }
     }
-    if (c.time_kernels) {
-        STOKES_HIP(hipEventRecord(c.ev[2], s));
-        STOKES_HIP(hipEventSynchronize(c.ev[2]));
-        float ff = 0.f, nr = 0.f;
-        STOKES_HIP(hipEventElapsedTime(&ff, c.ev[0], c.ev[1]));
-        STOKES_HIP(hipEventElapsedTime(&nr, c.ev[1], c.ev[2]));
-        c.ff_ms += ff;
-        c.ff_launches += 1;
-        if (has_near) {
-            c.near_ms += nr;
-            c.near_launches += 1;
-        }
+    if (has_near && c.time_kernels) {
+        c.t_near.stop(s, static_cast<double>(L.nentries));
     }
     k_reduce_small<<<g.m + 1, kBlk, 0, s>>>(g.x.p, g.y.p, g.px.p, g.py.p, g.speed.p, c.ex.p,
                                             c.ey.p, x_dev, g.no, g.n, g.m, c.scal.p, y_dev);

@@ -312,9 +312,14 @@ void shear_stress(Context& c, const double* den_dev, double* tau_dev) {
     c.spart.ensure(static_cast<size_t>(G) * 3 * static_cast<size_t>(nbp));
     NodeView nv{g.x.p, g.y.p, g.dzr.p, g.dzi.p, c.bd.p, g.iar.p, g.iai.p, g.mx.p, g.my.p,
                 c.ex.p, c.ey.p, static_cast<size_t>(g.npts)};
+    if (c.time_kernels) c.t_stress_eval.start(s);
     k_stress_eval<<<dim3(tiles, G), 64 * kNS, 0, s>>>(dgb.p, dge.p, nv, g.ax.p, g.ay.p, no, n, m,
                                                       c.spart.p);
     STOKES_LAUNCH_CHECK();
+    if (c.time_kernels) {
+        // (target point, source node) pairs: every other curve's nodes
+        c.t_stress_eval.stop(s, static_cast<double>(nbp) * static_cast<double>(total - n));
+    }
     k_stress_final<<<cdiv_int(nbp, kBlk), kBlk, 0, s>>>(c.spart.p, G, g.x.p, g.y.p, g.px.p,
                                                         g.py.p, g.speed.p, g.cx.p, g.cy.p, c.ex.p,
                                                         c.ey.p, den_dev, g.dtab_n.p, no, n, m,

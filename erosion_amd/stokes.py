@@ -46,6 +46,10 @@ def lib():
         L.erosion_matvec.restype = ctypes.c_int
         L.erosion_time_matvec.argtypes = [ctypes.c_int64] * 4 + [_dp, _dp, _dp, ctypes.c_int64, _dp]
         L.erosion_time_matvec.restype = ctypes.c_int
+        L.erosion_kernel_timing.argtypes = [ctypes.c_int]
+        L.erosion_kernel_timing.restype = ctypes.c_int
+        L.erosion_kernel_timers.argtypes = [_dp]
+        L.erosion_kernel_timers.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -130,6 +134,24 @@ def time_matvec(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: int = 1,
         raise StokesError(f"erosion_time_matvec: {last_error()}")
     return {"matvec_ms": out[0], "farfield_ms": out[1], "pairs": out[2], "near_entries": int(out[3]),
             "near_ms": out[4], "splits": int(out[5])}
+
+
+TIMER_FAMILIES = ("farfield", "near", "stress_eval", "gram_schmidt")
+
+
+def kernel_timing(enable: bool) -> None:
+    """Start (clearing totals) or stop HIP-event timing of the hot kernels."""
+    if lib().erosion_kernel_timing(1 if enable else 0) != 0:
+        raise StokesError(f"erosion_kernel_timing: {last_error()}")
+
+
+def kernel_timers() -> dict:
+    """{family: {"ms": total, "launches": n, "work": total work}} since enabling."""
+    out = np.zeros(12)
+    if lib().erosion_kernel_timers(_p(out)) != 0:
+        raise StokesError(f"erosion_kernel_timers: {last_error()}")
+    return {f: {"ms": out[3 * i], "launches": int(out[3 * i + 1]), "work": out[3 * i + 2]}
+            for i, f in enumerate(TIMER_FAMILIES)}
 
 
 def matvec(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: int = 1) -> np.ndarray:

@@ -1,0 +1,143 @@
+"""Erosion time stepping in the theta-L variables (host side).
+
+Restates ErosionPackage/src/TimeStepping.jl: ``tangvel`` (:16-26),
+``getderivs`` (:37-58, :60-69), ``delete_indices`` (:77-100), ``timestep!``
+(:104-159, here ``timestep_``) and ``rungekutta2`` (:163-176).  One erosion
+step is one ``rungekutta2`` call: two density solves and two shear-stress
+evaluations on the GPU, with O(n log n) spectral updates per body on the host
+in between.
+
+``den0`` hooks: every solve can start GMRES from an initial guess instead of
+zeros.  The reference always starts from zeros (DensityStress.jl:45); the
+guess changes only the iteration count, never the stopping test.
+"""
+from __future__ import annotations
+
+import math
+import warnings
+from typing import Callable, List, Optional
+
+import numpy as np
+
+from .density_stress import getstress_
+from .spectral import expsmooth, specdiff, specint
+from .theta_len import ParamSet, ThetaLenType, ThLenDenType, getalpha, new_thlenden
+
+
+class DerivsType:
+    """dL/dt (mterm), the nonlinear theta term, and d(xsm, ysm)/dt."""
+
+    __slots__ = ("mterm", "nterm", "xsmdot", "ysmdot")
+
+    def __init__(self, mterm, nterm, xsmdot, ysmdot):
+        self.mterm, self.nterm, self.xsmdot, self.ysmdot = mterm, nterm, xsmdot, ysmdot
+
+
+def tangvel(dtheta: np.ndarray, vnorm: np.ndarray):
+    """Tangential velocity and mterm = dL/dt (TimeStepping.jl:16-26)."""
+    dthvn = dtheta * vnorm
+    mdthvn = float(np.mean(dthvn))
+    mterm = -2 * math.pi * mdthvn
+    vtang = specint(dthvn - mdthvn)
+    return vtang, mterm
+
+
+def getderivs_body(thlen: ThetaLenType, params: ParamSet, atau: np.ndarray) -> DerivsType:
+    """Time derivatives of one body (TimeStepping.jl:37-58)."""
+    theta, ln, matau = thlen.theta, thlen.len, thlen.matau
+    ataueff = atau - float(params.fixarea) * matau
+    alpha = getalpha(len(theta))
+    dtheta = specdiff(theta - alpha) + 1.0
+    vnorm = ataueff + params.epsilon * matau * (dtheta - 1.0)
+    vtang, mterm = tangvel(dtheta, vnorm)
+    datau = specdiff(atau)
+    nterm = 2 * math.pi / ln * (datau + dtheta * vtang)
+    xsmdot = float(np.mean(-vnorm * np.sin(theta) + vtang * np.cos(theta)))
+    ysmdot = float(np.mean(vnorm * np.cos(theta) + vtang * np.sin(theta)))
+    return DerivsType(mterm, nterm, xsmdot, ysmdot)
+
+
+def getderivs(thlenden: ThLenDenType, params: ParamSet, den0=None, timings=None) -> List[DerivsType]:
+    """Solve, stress, then per-body derivatives (TimeStepping.jl:60-69)."""
+    atau = getstress_(thlenden, params, den0=den0, timings=timings)
+    return [getderivs_body(t, params, atau[:, b]) for b, t in enumerate(thlenden.thlenvec)]
+
+
+def delete_indices(thld0: ThLenDenType, dvec: List[DerivsType], dt: float) -> List[int]:
+    """Bodies to remove: len <= -sqrt(2) mterm dt or mterm > 10 (TimeStepping.jl:77-100)."""
+    ndts, mthresh = 1.0, 10.0
+    assert len(dvec) == len(thld0.thlenvec)
+    out = []
+    for bod, (thl, d) in enumerate(zip(thld0.thlenvec, dvec)):
+        minlen = -math.sqrt(2) * d.mterm * ndts * dt
+        if d.mterm > 0.0:
+            warnings.warn("mterm is positive.")
+        if thl.len <= minlen or d.mterm > mthresh:
+            out.append(bod)
+    return out
+
+
+def timestep_(thld0: ThLenDenType, thld_derivs: ThLenDenType, dt1: float, dt2: float,
+              params: ParamSet, den0=None, timings=None) -> ThLenDenType:
+    """Advance every body by dt1 with derivatives taken at thld_derivs (TimeStepping.jl:104-159)."""
+    dvec = getderivs(thld_derivs, params, den0=den0, timings=timings)
+    deletevec = delete_indices(thld0, dvec, dt1)
+    for bod in sorted(deletevec, reverse=True):
+        del thld0.thlenvec[bod]
+        del dvec[bod]
+        if thld0 is not thld_derivs:
+            del thld_derivs.thlenvec[bod]
+    if timings is not None and deletevec:
+        timings["deleted"] = timings.get("deleted", 0) + len(deletevec)
+    nbods = len(thld0.thlenvec)
+    assert len(dvec) == len(thld_derivs.thlenvec) == nbods
+    eps = params.epsilon
+    alpha = getalpha(params.npts)
+
+    def zetafun(ln, matau):
+        return 2 * math.pi / ln * matau
+
+    thlv1 = []
+    for bod in range(nbods):
+        t0 = thld0.thlenvec[bod]
+        td = thld_derivs.thlenvec[bod]
+        d = dvec[bod]
+        len1 = t0.len + dt1 * d.mterm
+        if not len1 > 0.0:
+            raise FloatingPointError("body length became non-positive")
+        zeta0 = zetafun(t0.len, t0.matau)
+        zetad = zetafun(td.len, td.matau)
+        fac1 = eps * dt1 * zetad
+        fac2 = eps * dt2 * (2 * zetad - zeta0)
+        th1 = expsmooth(t0.theta - alpha, fac1) + alpha
+        th1 = th1 + dt1 * expsmooth(d.nterm, fac2)
+        thlv1.append(ThetaLenType(th1, len1, t0.xsm + dt1 * d.xsmdot, t0.ysm + dt1 * d.ysmdot,
+                                  float("nan")))
+    return new_thlenden(thlv1)
+
+
+def rungekutta2(thld0: ThLenDenType, params: ParamSet,
+                guess: Optional[Callable[[ThLenDenType], Optional[np.ndarray]]] = None,
+                timings=None) -> ThLenDenType:
+    """One erosion step: midpoint RK2 (TimeStepping.jl:163-176).
+
+    ``guess(thlenden)`` may return a GMRES initial guess for that state."""
+    dt = params.dt
+    g0 = guess(thld0) if guess else None
+    thld05 = timestep_(thld0, thld0, 0.5 * dt, 0.5 * dt, params, den0=g0, timings=timings)
+    g1 = guess(thld05) if guess else None
+    thld1 = timestep_(thld0, thld05, dt, 0.5 * dt, params, den0=g1, timings=timings)
+    thld1.tt = thld0.tt + dt
+    return thld1
+
+
+def body_areas(thlenden: ThLenDenType) -> np.ndarray:
+    """Area of every body, (1/2) closed integral of (x dy - y dx) with spectral
+    derivatives (the quantity of ErosionPackage/src/PostProcess.jl:36-58)."""
+    from .theta_len import getxy
+
+    out = []
+    for t in thlenden.thlenvec:
+        x, y = getxy(t)
+        out.append(0.5 * float(np.mean(x * specdiff(y) - y * specdiff(x))) * 2 * math.pi)
+    return np.array(out)

@@ -98,6 +98,19 @@ int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t iba
                         const double* xx, const double* yy, const double* den, int64_t reps,
                         double* out);
 
+/* Measurement hook: enable (1) or stop (0) recording HIP events around the
+ * far-field, near-pair, stress-evaluation and Gram-Schmidt launches of every
+ * later call; enabling clears the totals.  Events are read back lazily, so
+ * timing adds no host synchronisation.  Returns 0 or -1. */
+int erosion_kernel_timing(int enable);
+
+/* Totals since timing was enabled, three values per family in the order
+ * far field, near pairs, stress evaluation, Gram-Schmidt:
+ * out[3i] milliseconds, out[3i+1] launches, out[3i+2] algorithmic work
+ * (ordered source-target pairs; near entries; target-node pairs; bytes of
+ * Krylov basis streamed).  Returns 0 or -1. */
+int erosion_kernel_timers(double* out);
+
 #ifdef __cplusplus
 }
 #endif

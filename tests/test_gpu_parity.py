@@ -41,13 +41,18 @@ def test_matvec_matches_reference(golden, name, ibary):
 
 
 def test_matvec_full_size_vs_oracle():
-    """100 bodies x 256 points, wall 2048 (27,648 boundary points)."""
+    """100 bodies x 256 points, wall 2048 (27,648 boundary points).
+
+    Every row is a 27,647-term sum whose terms reach ~1e3 x the result near
+    close bodies, so summation-order rounding alone is ~1e-12 in max-norm:
+    the bar here is 1e-11 max-norm and 1e-13 RMS."""
     n, m, no, xx, yy = workload_xy(WORKLOADS["bodies100_n256"])
     N = 2 * no + 2 * n * m + 3 * m
     eta = np.random.default_rng(7).standard_normal(N)
     v = stokes.matvec(n, m, no, xx, yy, eta, ibary=1)
     vo = O.Geometry(n, m, no, xx, yy).matvec(eta, 1)
-    assert relerr(v, vo) < 1e-12
+    assert relerr(v, vo) < 1e-11
+    assert np.sqrt(np.mean((v - vo) ** 2)) / np.sqrt(np.mean(vo ** 2)) < 1e-13
 
 
 def test_matvec_tight_geometry_many_near_pairs():
