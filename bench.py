@@ -4,9 +4,11 @@
 A step is one ``rungekutta2`` call (ErosionPackage/src/TimeStepping.jl:
 163-176): two GMRES density solves to a 1e-10 residual and two shear-stress
 evaluations through the C-ABI of libstokes_hip.so, with the theta-L updates
-on the host in between.  Workload: 100 bodies x 256 points, wall 2048 points,
-seeded perturbed-circle geometry, pipe-flow boundary condition, dt = 1e-3,
-fixpdrop off.
+on the host in between, and -- with the pressure drop held fixed as the
+reference's default ParamSet does (fixpdrop, DensityStress.jl:157-167) -- four
+computeqoitargets_ calls at the x = +-2 slices.  Workload: 100 bodies x 256
+points, wall 2048 points, seeded perturbed-circle geometry, pipe-flow
+boundary condition, dt = 1e-3.
 
 Prints ONE JSON line on stdout (everything else goes to stderr):
   value        erosion steps/s over the timed steps
@@ -133,7 +135,7 @@ def main() -> int:
 
     w = WORKLOADS[args.workload]
     params = ParamSet(npts=w.npts, nouter=w.nouter, ibary=1, ifmm=1, ibc=w.ibc, dt=w.dt,
-                      maxl=8000, fixpdrop=False)
+                      maxl=8000, fixpdrop=True)
     stokes.set_gmres_tol(args.tol)
     thld = workload_geometry(w)
     _m, xx0, yy0 = getnxy(thld)
@@ -198,7 +200,7 @@ def main() -> int:
         "config": {
             "workload": w.name, "bodies": w.nbodies, "points_per_body": w.npts,
             "wall_points": w.nouter, "gmres_tol": args.tol, "variant": "plain-fp64",
-            "dt": w.dt, "fixpdrop": False, "ibc": w.ibc,
+            "dt": w.dt, "fixpdrop": True, "ibc": w.ibc,
             "gmres_iters_per_solve": iters_per_solve,
             "true_relres_last": solve_info[-1]["true_relres"] if solve_info else None,
             "matvecs_per_s": ff["launches"] / elapsed,

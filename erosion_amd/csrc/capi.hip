@@ -114,6 +114,56 @@ void computeshearstress_(const int64_t* npts, const int64_t* nbods, const int64_
     }
 }
 
+void computeqoitargets_(const int64_t* npts, const int64_t* nbods, const int64_t* nouter,
+                        const int64_t* ibary, const double* xx, const double* yy,
+                        const double* den, const int64_t* ntargets, const double* xtar,
+                        const double* ytar, double* utar, double* vtar, double* ptar,
+                        double* vortar) {
+    try {
+        if (!npts || !nbods || !nouter || !ibary || !den || !ntargets)
+            throw stokes::Error("null argument");
+        check_sizes(*npts, *nbods, *nouter);
+        const int nt = static_cast<int>(*ntargets);
+        if (nt < 0) throw stokes::Error("negative target count");
+        if (nt == 0) return;
+        if (!xtar || !ytar || !utar || !vtar || !ptar || !vortar)
+            throw stokes::Error("null target array");
+        for (double* a : {utar, vtar, ptar, vortar})
+            std::memset(a, 0, sizeof(double) * static_cast<size_t>(nt));
+        const int n = static_cast<int>(*npts), m = static_cast<int>(*nbods),
+                  no = static_cast<int>(*nouter);
+        if (m > 0 && (!xx || !yy)) throw stokes::Error("null body coordinates");
+        if (*ibary != 0 && *ibary != 1) throw stokes::Error("ibary must be 0 or 1");
+        Context& c = stokes::ctx();
+        c.status = 0;
+        c.last_error.clear();
+        stokes::build_geometry(c, n, m, no, xx, yy);
+        const size_t nunk = static_cast<size_t>(c.g.nunk);
+        c.rvec.ensure(std::max(nunk, c.rvec.cap));
+        c.tgt.ensure(6 * static_cast<size_t>(nt));
+        double* tx = c.tgt.p;
+        double* ty = c.tgt.p + nt;
+        double* out = c.tgt.p + 2 * nt;
+        STOKES_HIP(hipMemcpyAsync(c.rvec.p, den, nunk * sizeof(double), hipMemcpyHostToDevice,
+                                  c.stream));
+        STOKES_HIP(hipMemcpyAsync(tx, xtar, nt * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        STOKES_HIP(hipMemcpyAsync(ty, ytar, nt * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        stokes::qoi_targets(c, static_cast<int>(*ibary), c.rvec.p, nt, tx, ty, out);
+        std::vector<double> h(4 * static_cast<size_t>(nt));
+        STOKES_HIP(hipMemcpyAsync(h.data(), out, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                  c.stream));
+        STOKES_HIP(hipStreamSynchronize(c.stream));
+        for (int t = 0; t < nt; ++t) {
+            utar[t] = h[4 * static_cast<size_t>(t)];
+            vtar[t] = h[4 * static_cast<size_t>(t) + 1];
+            ptar[t] = h[4 * static_cast<size_t>(t) + 2];
+            vortar[t] = h[4 * static_cast<size_t>(t) + 3];
+        }
+    } catch (const std::exception& e) {
+        record_error("computeqoitargets_", e.what());
+    }
+}
+
 int erosion_set_gmres_tol(double tol) {
     try {
         if (!(tol > 0.0)) return -1;

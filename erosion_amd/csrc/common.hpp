@@ -196,9 +196,11 @@ struct Context {
     DevBuf<double> rhs, xvec, rvec;
     DevBuf<GmresState> st;
     GmresState* h_st = nullptr;  // pinned mirror
-    // stress buffers
-    DevBuf<double> spart;   // stress partials
+    // stress / target-evaluation buffers
+    DevBuf<double> spart;   // per curve-group partials
     DevBuf<double> tau;
+    DevBuf<int> curves_all, grp_beg, grp_end, iside;
+    DevBuf<double> tgt;     // target coordinates and QoI outputs
     double tol = 1e-8;
     // per-kernel timing with HIP events on `stream` (measurement hooks only)
     bool time_kernels = false;
@@ -228,5 +230,8 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl, double tol, doub
 
 // stress.hip
 void shear_stress(Context& c, const double* den_dev, double* tau_dev);
+// velocity, pressure, vorticity at targets; out[4*t + {0,1,2,3}]
+void qoi_targets(Context& c, int ibary, const double* den_dev, int ntar, const double* tx_dev,
+                 const double* ty_dev, double* out_dev);
 
 }  // namespace stokes

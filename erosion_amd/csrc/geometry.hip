@@ -33,9 +33,12 @@ __global__ void k_points(const double* __restrict__ xx, const double* __restrict
     }
 }
 
-// One spectral derivative of (fx, fy) on every curve: out_j = sum_k d(j-k) f_k.
-// grid: (ceil(maxlen/256), ncurves); curve 0 = wall (length no, table dno),
-// curve c>0 = body c-1 (length n, table dn).
+// One spectral derivative of (fx, fy) on every curve:
+//   out_j = sum_k d(j-k) (f_k - f_j)
+// (the rows of d sum to zero, so subtracting f_j changes nothing in exact
+// arithmetic and removes the O(N) cancellation of the large entries near the
+// diagonal).  grid: (ceil(maxlen/256), ncurves); curve 0 = wall (length no,
+// table dno), curve c>0 = body c-1 (length n, table dn).
 __global__ void k_deriv(const double* __restrict__ fx, const double* __restrict__ fy, int no,
                         int n, const double* __restrict__ dno, const double* __restrict__ dn,
                         double* __restrict__ ox, double* __restrict__ oy) {
@@ -45,13 +48,14 @@ __global__ void k_deriv(const double* __restrict__ fx, const double* __restrict_
     const double* __restrict__ d = c == 0 ? dno : dn;
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= len) return;
+    const double xj = fx[off + j], yj = fy[off + j];
     double sx = 0.0, sy = 0.0;
     for (int k = 0; k < len; ++k) {
         int idx = j - k;
         idx += idx < 0 ? len : 0;
         double w = d[idx];
-        sx = __builtin_fma(w, fx[off + k], sx);
-        sy = __builtin_fma(w, fy[off + k], sy);
+        sx = __builtin_fma(w, fx[off + k] - xj, sx);
+        sy = __builtin_fma(w, fy[off + k] - yj, sy);
     }
     ox[off + j] = sx;
     oy[off + j] = sy;

@@ -185,10 +185,11 @@ __global__ __launch_bounds__(64 * NSL) void k_bdval(
         int idx = j - k;
         idx += idx < 0 ? len : 0;
         const double w = dt[idx];
+        // derivative as sum_k d(j-k)(f_k - f_j): rows of d sum to zero
 #pragma unroll
         for (int f = 0; f < 5; ++f) {
-            D[f].r = __builtin_fma(w, fk[f].r, D[f].r);
-            D[f].i = __builtin_fma(w, fk[f].i, D[f].i);
+            D[f].r = __builtin_fma(w, fk[f].r - fj[f].r, D[f].r);
+            D[f].i = __builtin_fma(w, fk[f].i - fj[f].i, D[f].i);
         }
         if (k != j) {
             const cplx g = cmul(cmk(dzr[p], dzi[p]), cinv(csub(cmk(x[p], y[p]), zj)));

@@ -45,21 +45,25 @@ __device__ __forceinline__ void slice_sum(double (&v)[NV], double (*red)[kNS][64
     __syncthreads();
 }
 
-// Deformation tensor contributions of source curves [grp_beg[g], grp_end[g])
-// at 64 body points per block; spart[(g*3 + {0,1,2}) * nbp + q] = E11, E12, E22.
-__global__ __launch_bounds__(64 * kNS) void k_stress_eval(
+// nder=2 barycentric evaluation of source curves [grp_beg[g], grp_end[g]) at
+// 64 targets per block.  MODE 0 (stress): targets are the body points
+// (tgt_x == nullptr), a body's own curve is skipped, and
+// part[(g*3 + {0,1,2}) * ntar + q] = E11, E12, E22.  MODE 1 (targets in the
+// fluid): every curve counts and part[...] = u, v, vorticity.
+template <int MODE>
+__global__ __launch_bounds__(64 * kNS) void k_bary_eval(
     const int* __restrict__ grp_beg, const int* __restrict__ grp_end, NodeView nv,
-    const double* __restrict__ ax, const double* __restrict__ ay, int no, int n, int m,
+    const double* __restrict__ ax, const double* __restrict__ ay, int no, int n,
+    const double* __restrict__ tgt_x, const double* __restrict__ tgt_y, int ntar,
     double* __restrict__ spart) {
     __shared__ double red[12][kNS][64];
     const int lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
-    const int nbp = n * m;
     const int q = blockIdx.x * 64 + lane;
-    const bool valid = q < nbp;
+    const bool valid = q < ntar;
     const int qq = valid ? q : 0;
-    const int tp = no + qq;
-    const int bt = qq / n;
-    const double xt = nv.x[tp], yt = nv.y[tp];
+    const int bt = MODE == 0 ? qq / n : -2;
+    const double xt = MODE == 0 ? nv.x[no + qq] : tgt_x[qq];
+    const double yt = MODE == 0 ? nv.y[no + qq] : tgt_y[qq];
     const cplx zt = cmk(xt, yt);
     double E11 = 0.0, E12 = 0.0, E22 = 0.0;
     const int g = blockIdx.y;
@@ -157,15 +161,136 @@ __global__ __launch_bounds__(64 * kNS) void k_stress_eval(
         double u1y = -dd[0].i + xt * dd[1].i - du[2].r + yt * dd[2].i - du[3].i;
         double u2x = -dd[0].i + du[1].i + xt * dd[1].i + yt * dd[2].i + du[4].r;
         double u2y = -dd[0].r + xt * dd[1].r + du[2].i + yt * dd[2].r - du[4].i;
-        E11 += u1x;
-        E12 += 0.5 * (u1y + u2x);
-        E22 += u2y;
+        if (MODE == 0) {
+            E11 += u1x;
+            E12 += 0.5 * (u1y + u2x);
+            E22 += u2y;
+        } else {  // velocity (nder=1 outputs) and vorticity v_x - u_y
+            E11 += du[0].r - xt * du[1].r - yt * du[2].r + u[3].r;
+            E12 += -du[0].i + xt * du[1].i + yt * du[2].i + u[4].r;
+            E22 += u2x - u1y;
+        }
     }
     if (slice == 0 && valid) {
-        spart[(static_cast<size_t>(g) * 3 + 0) * nbp + q] = E11;
-        spart[(static_cast<size_t>(g) * 3 + 1) * nbp + q] = E12;
-        spart[(static_cast<size_t>(g) * 3 + 2) * nbp + q] = E22;
+        spart[(static_cast<size_t>(g) * 3 + 0) * ntar + q] = E11;
+        spart[(static_cast<size_t>(g) * 3 + 1) * ntar + q] = E12;
+        spart[(static_cast<size_t>(g) * 3 + 2) * ntar + q] = E22;
     }
+}
+
+// computeQoiTargets (stokesSolver.f:3099-3323), per target: trapezoid
+// pressure from every source node (and, for ibary=0, trapezoid velocity and
+// vorticity), closest body node for the inside/outside test of
+// classifyPointsOld (stokesSolver.f:4420-4500), stokeslet/rotlet terms.
+// One block per target; out[4*t + {0,1,2,3}] = u, v, p, vorticity before the
+// barycentric partial sums are added.
+__global__ __launch_bounds__(kBlk) void k_qoi_trap(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ px,
+    const double* __restrict__ py, const double* __restrict__ mx, const double* __restrict__ my,
+    const double* __restrict__ ex, const double* __restrict__ ey, const double* __restrict__ cx,
+    const double* __restrict__ cy, const double* __restrict__ den, int no, int n, int m,
+    int ibary, const double* __restrict__ tx, const double* __restrict__ ty,
+    double* __restrict__ out, int* __restrict__ iside) {
+    __shared__ double sh[5][kBlk];
+    __shared__ int si[kBlk];
+    const int t = blockIdx.x;
+    const double xt = tx[t], yt = ty[t];
+    const int npts = no + n * m;
+    double u = 0, v = 0, p = 0, w = 0;
+    for (int j = threadIdx.x; j < npts; j += kBlk) {
+        const double rx = xt - x[j], ry = yt - y[j];
+        const double rho2 = rx * rx + ry * ry;
+        const double ir2 = 1.0 / rho2;
+        const double rdotm = rx * mx[j] + ry * my[j];
+        const double rdotd = rx * ex[j] + ry * ey[j];
+        const double mdotd = mx[j] * ex[j] + my[j] * ey[j];
+        p += (2.0 * rdotm * rdotd * ir2 * ir2 - mdotd * ir2);
+        if (!ibary) {
+            const double routm = -rx * my[j] + ry * mx[j];
+            const double routd = -rx * ey[j] + ry * ex[j];
+            u += rdotm * ir2 * rdotd * ir2 * rx;
+            v += rdotm * ir2 * rdotd * ir2 * ry;
+            w += (rdotd * routm + rdotm * routd) * ir2 * ir2;
+        }
+    }
+    // closest body node (first minimum in body-major order wins)
+    double dmin = 1.0e300;
+    int imin = 0x7fffffff;
+    for (int q = threadIdx.x; q < n * m; q += kBlk) {
+        const double dx = xt - x[no + q], dy = yt - y[no + q];
+        const double d2 = dx * dx + dy * dy;
+        if (d2 < dmin) {
+            dmin = d2;
+            imin = q;
+        }
+    }
+    sh[0][threadIdx.x] = u;
+    sh[1][threadIdx.x] = v;
+    sh[2][threadIdx.x] = p;
+    sh[3][threadIdx.x] = w;
+    sh[4][threadIdx.x] = dmin;
+    si[threadIdx.x] = imin;
+    __syncthreads();
+    for (int s = kBlk / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
+            const double d2 = sh[4][threadIdx.x + s];
+            const int i2 = si[threadIdx.x + s];
+            if (d2 < sh[4][threadIdx.x] || (d2 == sh[4][threadIdx.x] && i2 < si[threadIdx.x])) {
+                sh[4][threadIdx.x] = d2;
+                si[threadIdx.x] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    u = sh[0][0];
+    v = sh[1][0];
+    p = sh[2][0];
+    w = sh[3][0];
+    const int base = 2 * no + 2 * n * m;
+    for (int b = 0; b < m; ++b) {
+        const double sto1 = den[base + 3 * b], sto2 = den[base + 3 * b + 1];
+        const double rot = den[base + 3 * b + 2];
+        const double rx = xt - cx[b], ry = yt - cy[b];
+        const double rho2 = rx * rx + ry * ry;
+        const double rdots = rx * sto1 + ry * sto2;
+        u += 0.5 / kTwoPi * (-0.5 * log(rho2) * sto1 + rdots / rho2 * rx);
+        v += 0.5 / kTwoPi * (-0.5 * log(rho2) * sto2 + rdots / rho2 * ry);
+        p += 1.0 / kTwoPi * rdots / rho2;
+        w += (ry * sto1 - rx * sto2) / rho2 / kTwoPi;
+        u += rot * ry / rho2;
+        v -= rot * rx / rho2;
+    }
+    out[4 * t + 0] = u;
+    out[4 * t + 1] = v;
+    out[4 * t + 2] = p;
+    out[4 * t + 3] = w;
+    int side = 1;
+    if (m > 0) {
+        const int q = no + si[0];
+        const double vdn = (xt - x[q]) * px[q] + (yt - y[q]) * py[q];
+        side = vdn < 0.0 ? 1 : 0;
+    }
+    iside[t] = side;
+}
+
+// Barycentric partial sums into the QoI, then zero targets inside a body.
+__global__ void k_qoi_final(const double* __restrict__ part, int ngroups, int ntar, int m,
+                            const int* __restrict__ iside, double* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntar) return;
+    double u = out[4 * t], v = out[4 * t + 1], p = out[4 * t + 2], w = out[4 * t + 3];
+    for (int g = 0; g < ngroups; ++g) {
+        u += part[(static_cast<size_t>(g) * 3 + 0) * ntar + t];
+        v += part[(static_cast<size_t>(g) * 3 + 1) * ntar + t];
+        w += part[(static_cast<size_t>(g) * 3 + 2) * ntar + t];
+    }
+    if (m > 0 && iside[t] == 0) u = v = p = w = 0.0;
+    out[4 * t] = u;
+    out[4 * t + 1] = v;
+    out[4 * t + 2] = p;
+    out[4 * t + 3] = w;
 }
 
 // Sum of the curve groups + odd-even self term + jump + stokeslet/rotlet
@@ -217,13 +342,14 @@ __global__ __launch_bounds__(kBlk) void k_stress_final(
     }
     // jump: derivative of eta along the body (fourierDiff of eta_x + i eta_y)
     {
+        // sum_j d(k-j)(eta_j - eta_k): rows of d sum to zero
         double dr = 0.0, di = 0.0;
         for (int j = 0; j < n; ++j) {
             int idx = k - j;
             idx += idx < 0 ? n : 0;
             const double wd = dtab[idx];
-            dr = __builtin_fma(wd, ex[off + j], dr);
-            di = __builtin_fma(wd, ey[off + j], di);
+            dr = __builtin_fma(wd, ex[off + j] - dxk, dr);
+            di = __builtin_fma(wd, ey[off + j] - dyk, di);
         }
         const double tx = -py[p], ty = px[p];
         const double dsdtx = dr / speed[p], dsdty = di / speed[p];
@@ -269,23 +395,11 @@ __global__ __launch_bounds__(kBlk) void k_stress_final(
 
 }  // namespace
 
-void shear_stress(Context& c, const double* den_dev, double* tau_dev) {
-    Geometry& g = c.g;
+// Source-curve groups balanced by node count; uploads every curve's boundary
+// values first.  Returns the number of groups; fills c.grp_beg / c.grp_end.
+static int plan_curve_groups(Context& c, int tiles, std::vector<int>& gb, std::vector<int>& ge) {
+    const Geometry& g = c.g;
     const int n = g.n, m = g.m, no = g.no;
-    const int nbp = n * m;
-    if (nbp == 0) return;
-    hipStream_t s = c.stream;
-    gather_density(c, den_dev);
-    // every curve is a source: wall first, then the bodies
-    std::vector<int> curves(static_cast<size_t>(m + 1));
-    for (int i = 0; i <= m; ++i) curves[static_cast<size_t>(i)] = i;
-    DevBuf<int> dcurves;
-    dcurves.ensure(curves.size());
-    STOKES_HIP(hipMemcpyAsync(dcurves.p, curves.data(), curves.size() * sizeof(int),
-                              hipMemcpyHostToDevice, s));
-    boundary_values(c, dcurves.p, m + 1, 1);
-    // curve groups balanced by node count so that the grid has ~2048 blocks
-    const int tiles = cdiv_int(nbp, 64);
     int G = std::max(1, std::min(m + 1, cdiv_int(2048, tiles)));
     // cum[c] = nodes of curves 0..c-1; group gi ends at the first curve
     // boundary past a fraction (gi+1)/G of all nodes
@@ -293,7 +407,8 @@ void shear_stress(Context& c, const double* den_dev, double* tau_dev) {
     for (int cc = 0; cc <= m; ++cc)
         cum[static_cast<size_t>(cc + 1)] = cum[static_cast<size_t>(cc)] + (cc == 0 ? no : n);
     const long long total = cum[static_cast<size_t>(m + 1)];
-    std::vector<int> gb, ge;
+    gb.clear();
+    ge.clear();
     for (int gi = 0, cur = 0; gi < G && cur <= m; ++gi) {
         const long long target = total * (gi + 1) / G;
         int end = cur + 1;
@@ -303,27 +418,94 @@ void shear_stress(Context& c, const double* den_dev, double* tau_dev) {
         ge.push_back(end);
         cur = end;
     }
-    G = static_cast<int>(gb.size());
-    DevBuf<int> dgb, dge;
-    dgb.ensure(gb.size());
-    dge.ensure(ge.size());
-    STOKES_HIP(hipMemcpyAsync(dgb.p, gb.data(), gb.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    STOKES_HIP(hipMemcpyAsync(dge.p, ge.data(), ge.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    return static_cast<int>(gb.size());
+}
+
+// Density gathered, boundary values of every curve (wall first).
+static void all_boundary_values(Context& c, const double* den_dev) {
+    const int m = c.g.m;
+    gather_density(c, den_dev);
+    std::vector<int> curves(static_cast<size_t>(m + 1));
+    for (int i = 0; i <= m; ++i) curves[static_cast<size_t>(i)] = i;
+    c.curves_all.ensure(curves.size());
+    STOKES_HIP(hipMemcpyAsync(c.curves_all.p, curves.data(), curves.size() * sizeof(int),
+                              hipMemcpyHostToDevice, c.stream));
+    boundary_values(c, c.curves_all.p, m + 1, 1);
+}
+
+void shear_stress(Context& c, const double* den_dev, double* tau_dev) {
+    Geometry& g = c.g;
+    const int n = g.n, m = g.m, no = g.no;
+    const int nbp = n * m;
+    if (nbp == 0) return;
+    hipStream_t s = c.stream;
+    all_boundary_values(c, den_dev);
+    const int tiles = cdiv_int(nbp, 64);
+    std::vector<int> gb, ge;
+    const int G = plan_curve_groups(c, tiles, gb, ge);
+    c.grp_beg.ensure(gb.size());
+    c.grp_end.ensure(ge.size());
+    STOKES_HIP(hipMemcpyAsync(c.grp_beg.p, gb.data(), gb.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    STOKES_HIP(hipMemcpyAsync(c.grp_end.p, ge.data(), ge.size() * sizeof(int), hipMemcpyHostToDevice, s));
     c.spart.ensure(static_cast<size_t>(G) * 3 * static_cast<size_t>(nbp));
     NodeView nv{g.x.p, g.y.p, g.dzr.p, g.dzi.p, c.bd.p, g.iar.p, g.iai.p, g.mx.p, g.my.p,
                 c.ex.p, c.ey.p, static_cast<size_t>(g.npts)};
     if (c.time_kernels) c.t_stress_eval.start(s);
-    k_stress_eval<<<dim3(tiles, G), 64 * kNS, 0, s>>>(dgb.p, dge.p, nv, g.ax.p, g.ay.p, no, n, m,
-                                                      c.spart.p);
+    k_bary_eval<0><<<dim3(tiles, G), 64 * kNS, 0, s>>>(c.grp_beg.p, c.grp_end.p, nv, g.ax.p,
+                                                       g.ay.p, no, n, nullptr, nullptr, nbp,
+                                                       c.spart.p);
     STOKES_LAUNCH_CHECK();
     if (c.time_kernels) {
         // (target point, source node) pairs: every other curve's nodes
-        c.t_stress_eval.stop(s, static_cast<double>(nbp) * static_cast<double>(total - n));
+        c.t_stress_eval.stop(s, static_cast<double>(nbp) * static_cast<double>(g.npts - n));
     }
     k_stress_final<<<cdiv_int(nbp, kBlk), kBlk, 0, s>>>(c.spart.p, G, g.x.p, g.y.p, g.px.p,
                                                         g.py.p, g.speed.p, g.cx.p, g.cy.p, c.ex.p,
                                                         c.ey.p, den_dev, g.dtab_n.p, no, n, m,
                                                         tau_dev);
+    STOKES_LAUNCH_CHECK();
+    STOKES_HIP(hipStreamSynchronize(s));
+}
+
+void qoi_targets(Context& c, int ibary, const double* den_dev, int ntar, const double* tx_dev,
+                 const double* ty_dev, double* out_dev) {
+    Geometry& g = c.g;
+    const int n = g.n, m = g.m, no = g.no;
+    if (ntar <= 0) return;
+    hipStream_t s = c.stream;
+    c.iside.ensure(static_cast<size_t>(ntar));
+    int G = 0;
+    if (ibary == 1) {
+        all_boundary_values(c, den_dev);
+    } else {
+        gather_density(c, den_dev);
+    }
+    k_qoi_trap<<<ntar, kBlk, 0, s>>>(g.x.p, g.y.p, g.px.p, g.py.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
+                                     g.cx.p, g.cy.p, den_dev, no, n, m, ibary, tx_dev, ty_dev,
+                                     out_dev, c.iside.p);
+    STOKES_LAUNCH_CHECK();
+    if (ibary == 1) {
+        const int tiles = cdiv_int(ntar, 64);
+        std::vector<int> gb, ge;
+        // one group per curve keeps the grid wide for a handful of targets
+        for (int cc = 0; cc <= m; ++cc) {
+            gb.push_back(cc);
+            ge.push_back(cc + 1);
+        }
+        G = m + 1;
+        (void)tiles;
+        c.grp_beg.ensure(gb.size());
+        c.grp_end.ensure(ge.size());
+        STOKES_HIP(hipMemcpyAsync(c.grp_beg.p, gb.data(), gb.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        STOKES_HIP(hipMemcpyAsync(c.grp_end.p, ge.data(), ge.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        c.spart.ensure(static_cast<size_t>(G) * 3 * static_cast<size_t>(ntar));
+        NodeView nv{g.x.p, g.y.p, g.dzr.p, g.dzi.p, c.bd.p, g.iar.p, g.iai.p, g.mx.p, g.my.p,
+                    c.ex.p, c.ey.p, static_cast<size_t>(g.npts)};
+        k_bary_eval<1><<<dim3(cdiv_int(ntar, 64), G), 64 * kNS, 0, s>>>(
+            c.grp_beg.p, c.grp_end.p, nv, g.ax.p, g.ay.p, no, n, tx_dev, ty_dev, ntar, c.spart.p);
+        STOKES_LAUNCH_CHECK();
+    }
+    k_qoi_final<<<cdiv_int(ntar, kBlk), kBlk, 0, s>>>(c.spart.p, G, ntar, m, c.iside.p, out_dev);
     STOKES_LAUNCH_CHECK();
     STOKES_HIP(hipStreamSynchronize(s));
 }

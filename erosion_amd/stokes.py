@@ -35,6 +35,8 @@ def lib():
         L.stokessolver_.restype = None
         L.computeshearstress_.argtypes = [_i64p, _i64p, _i64p, _dp, _dp, _dp, _i64p, _dp]
         L.computeshearstress_.restype = None
+        L.computeqoitargets_.argtypes = [_i64p] * 4 + [_dp, _dp, _dp, _i64p] + [_dp] * 6
+        L.computeqoitargets_.restype = None
         L.erosion_set_gmres_tol.argtypes = [ctypes.c_double]
         L.erosion_set_gmres_tol.restype = ctypes.c_int
         L.erosion_get_gmres_tol.restype = ctypes.c_double
@@ -152,6 +154,17 @@ def kernel_timers() -> dict:
         raise StokesError(f"erosion_kernel_timers: {last_error()}")
     return {f: {"ms": out[3 * i], "launches": int(out[3 * i + 1]), "work": out[3 * i + 2]}
             for i, f in enumerate(TIMER_FAMILIES)}
+
+
+def computeqoitargets(npts: int, nbods: int, nouter: int, ibary: int, xx, yy, den, xtar, ytar):
+    """Call computeqoitargets_ exactly as DensityStress.jl:81-86 does; returns (u, v, p, vort)."""
+    xx, yy, den, xtar, ytar = map(_f64, (xx, yy, den, xtar, ytar))
+    nt = len(xtar)
+    outs = [np.zeros(nt) for _ in range(4)]
+    lib().computeqoitargets_(_ref(npts), _ref(nbods), _ref(nouter), _ref(ibary), _p(xx), _p(yy),
+                             _p(den), _ref(nt), _p(xtar), _p(ytar), *[_p(o) for o in outs])
+    _check("computeqoitargets_")
+    return tuple(outs)
 
 
 def matvec(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: int = 1) -> np.ndarray:

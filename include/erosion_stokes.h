@@ -62,6 +62,20 @@ void computeshearstress_(const int64_t* npts, const int64_t* nbods, const int64_
                          const double* xx, const double* yy, const double* den,
                          const int64_t* ibary, double* tau);
 
+/* Velocity, pressure and vorticity at target points.
+ * Replaces subroutine computeQoiTargets (ErosionPackage/fortran_code/stokesSolver.f:3099-3323,
+ * inside/outside test classifyPointsOld :4420-4500), bound at
+ * ErosionPackage/src/DensityStress.jl:81-86 and called four times per erosion
+ * step when the pressure drop is held fixed (getpdrop, DensityStress.jl:129-154).
+ * ibary=1: velocity and vorticity by the barycentric rules from every curve,
+ * pressure by the trapezoid rule; ibary=0: trapezoid for all four.  Targets
+ * inside a body get zeros.  On error all outputs are zeroed. */
+void computeqoitargets_(const int64_t* npts, const int64_t* nbods, const int64_t* nouter,
+                        const int64_t* ibary, const double* xx, const double* yy,
+                        const double* den, const int64_t* ntargets, const double* xtar,
+                        const double* ytar, double* utar, double* vtar, double* ptar,
+                        double* vortar);
+
 /* ---- Additions with no counterpart in the reference ------------------------
  * The reference fixes the GMRES tolerance in source (stokesSolver.f:482) and
  * has no status channel; these let a caller run at another tolerance and read

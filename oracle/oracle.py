@@ -58,6 +58,7 @@ def lib():
                                 _dp, _ip]
         L.orc_deformation.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.orc_shear_stress.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
+        L.orc_qoi_targets.argtypes = [ctypes.c_int] * 4 + [_dp, _dp, _dp, ctypes.c_int] + [_dp] * 6
         L.orc_set_threads.argtypes = [ctypes.c_int]
         L.orc_get_threads.restype = ctypes.c_int
         _lib = L
@@ -159,6 +160,16 @@ def stokessolver(n, m, no, xx, yy, ibary=1, ibc=0, maxl=8000, tol=1e-8, den0=Non
     g = Geometry(n, m, no, xx, yy)
     x, it, _err, _ierr = g.gmres(g.rhs(ibc), den0, ibary=ibary, maxl=maxl, tol=tol)
     return x, it
+
+
+def qoi_targets(n, m, no, ibary, xx, yy, den, xt, yt):
+    """(u, v, p, vorticity) at targets, computeQoiTargets."""
+    xx, yy, den, xt, yt = map(_f64, (xx, yy, den, xt, yt))
+    nt = len(xt)
+    outs = [np.zeros(nt) for _ in range(4)]
+    lib().orc_qoi_targets(int(n), int(m), int(no), int(ibary), _p(xx), _p(yy), _p(den), nt,
+                          _p(xt), _p(yt), *[_p(o) for o in outs])
+    return tuple(outs)
 
 
 def shear_stress(n, m, no, xx, yy, den) -> np.ndarray:

@@ -210,6 +210,17 @@ def _dlp(fn, exterior, xs, ys, dx, dy, px1, py1, xt, yt, nder):
     return tuple(outs)
 
 
+def qoi_targets(n, m, no, ibary, xx, yy, den, xt, yt):
+    """computeqoitargets_ called with int64 arguments as DensityStress.jl:81-86 does."""
+    xx, yy, den = _f64(xx).copy(), _f64(yy).copy(), _f64(den)
+    xt, yt = _f64(xt).copy(), _f64(yt).copy()
+    nt = len(xt)
+    outs = [np.zeros(nt) for _ in range(4)]
+    lib().computeqoitargets_(_i64(n), _i64(m), _i64(no), _i64(ibary), _p(xx), _p(yy), _p(den),
+                             _i64(nt), _p(xt), _p(yt), *[_p(o) for o in outs])
+    return tuple(outs)
+
+
 def shear_stress(n, m, no, xx, yy, den, ib=1) -> np.ndarray:
     """computeshearstress_ called with int64 arguments as DensityStress.jl:195-198 does."""
     xx, yy, den = _f64(xx).copy(), _f64(yy).copy(), _f64(den)

@@ -1113,6 +1113,99 @@ void orc_deformation(const orc_geom *g, const double *den, double *e11, double *
     free(u2y);
 }
 
+/* classifyPointsOld (stokesSolver.f:4420-4500): 1 if the target lies on the
+ * fluid side of its closest body node (first minimum in body-major order). */
+static int classify_point(const orc_geom *g, double xt, double yt) {
+    int n = g->n, m = g->m;
+    int best = 0;
+    double dmin = 1.0e10;
+    for (int j = 0; j < m; j++)
+        for (int k = 0; k < n; k++) {
+            int q = j * n + k;
+            double d2 = (xt - g->x[q]) * (xt - g->x[q]) + (yt - g->y[q]) * (yt - g->y[q]);
+            if (d2 < dmin) {
+                dmin = d2;
+                best = q;
+            }
+        }
+    double vdn = (xt - g->x[best]) * g->px[best] + (yt - g->y[best]) * g->py[best];
+    return vdn < 0.0 ? 1 : 0;
+}
+
+/* computeQoiTargets (stokesSolver.f:3099-3323). */
+void orc_qoi_targets(int n, int m, int no, int ibary, const double *xx, const double *yy,
+                     const double *den, int ntar, const double *xt, const double *yt, double *u,
+                     double *v, double *p, double *vort) {
+    orc_geom *g = orc_geom_create(n, m, no, xx, yy);
+    for (int j = 0; j < ntar; j++) u[j] = v[j] = p[j] = vort[j] = 0.0;
+    double *t6 = xcalloc((size_t)(6 * (ntar > 0 ? ntar : 1)), sizeof(double));
+    double *ut = t6, *vt = t6 + ntar, *uxt = t6 + 2 * ntar, *uyt = t6 + 3 * ntar,
+           *vxt = t6 + 4 * ntar, *vyt = t6 + 5 * ntar;
+    /* one source curve: trapezoid pressure (+ u, v, vorticity if ibary=0) */
+    for (int c = 0; c <= m; c++) {
+        int len = c == 0 ? no : n;
+        const double *xs = c == 0 ? g->xo : g->x + (c - 1) * n;
+        const double *ys = c == 0 ? g->yo : g->y + (c - 1) * n;
+        const double *nx = c == 0 ? g->px0 : g->px + (c - 1) * n;
+        const double *ny = c == 0 ? g->py0 : g->py + (c - 1) * n;
+        const double *sp = c == 0 ? g->speed0 : g->speed + (c - 1) * n;
+        const double *dx = c == 0 ? den : den + 2 * no + 2 * (c - 1) * n;
+        const double *dy = c == 0 ? den + no : dx + n;
+        for (int j = 0; j < ntar; j++) {
+            for (int k = 0; k < len; k++) {
+                double rx = xt[j] - xs[k], ry = yt[j] - ys[k];
+                double rho2 = rx * rx + ry * ry;
+                double rdotn = rx * nx[k] + ry * ny[k];
+                double rdotden = rx * dx[k] + ry * dy[k];
+                double dendotn = nx[k] * dx[k] + ny[k] * dy[k];
+                double routn = -rx * ny[k] + ry * nx[k];
+                double routden = -rx * dy[k] + ry * dx[k];
+                double w = sp[k] * TWOPI / (double)len / PI;
+                p[j] = p[j] + (2.0 * rdotn * rdotden / rho2 / rho2 - dendotn / rho2) * w;
+                if (ibary != 1) {
+                    u[j] = u[j] + rdotn / rho2 * rdotden / rho2 * rx * w;
+                    v[j] = v[j] + rdotn / rho2 * rdotden / rho2 * ry * w;
+                    vort[j] = vort[j] + (rdotden * routn + rdotn * routden) / (rho2 * rho2) * w;
+                }
+            }
+        }
+        if (ibary == 1) {
+            if (c == 0)
+                orc_stokes_interior_dlp(no, xs, ys, dx, dy, nx, ny, ntar, xt, yt, 2, ut, vt, uxt,
+                                        uyt, vxt, vyt);
+            else
+                orc_stokes_exterior_dlp(n, xs, ys, dx, dy, nx, ny, ntar, xt, yt, 2, ut, vt, uxt,
+                                        uyt, vxt, vyt);
+            for (int j = 0; j < ntar; j++) {
+                u[j] += ut[j];
+                v[j] += vt[j];
+                vort[j] += vxt[j] - uyt[j];
+            }
+        }
+    }
+    if (m > 0) {
+        const double *ext = den + 2 * no + 2 * n * m;
+        for (int b = 0; b < m; b++) {
+            double sto1 = ext[3 * b], sto2 = ext[3 * b + 1], rot = ext[3 * b + 2];
+            for (int j = 0; j < ntar; j++) {
+                double rx = xt[j] - g->cx[b], ry = yt[j] - g->cy[b];
+                double rho2 = rx * rx + ry * ry;
+                double rdots = rx * sto1 + ry * sto2;
+                u[j] = u[j] + 0.5 / TWOPI * (-0.5 * log(rho2) * sto1 + rdots / rho2 * rx);
+                v[j] = v[j] + 0.5 / TWOPI * (-0.5 * log(rho2) * sto2 + rdots / rho2 * ry);
+                p[j] = p[j] + 1.0 / TWOPI * rdots / rho2;
+                vort[j] = vort[j] + (ry * sto1 - rx * sto2) / rho2 / TWOPI;
+                u[j] = u[j] + rot * ry / rho2;
+                v[j] = v[j] - rot * rx / rho2;
+            }
+        }
+        for (int j = 0; j < ntar; j++)
+            if (classify_point(g, xt[j], yt[j]) == 0) u[j] = v[j] = p[j] = vort[j] = 0.0;
+    }
+    free(t6);
+    orc_geom_free(g);
+}
+
 /* computeShearStress, ib=1 (stokesSolver.f:3002-3096):
  * traction t = -2 E n, tau = t_x p_y - t_y p_x. */
 void orc_shear_stress(int n, int m, int no, const double *xx, const double *yy,

@@ -79,6 +79,12 @@ void orc_deformation(const orc_geom *g, const double *den, double *e11,
 void orc_shear_stress(int n, int m, int no, const double *xx, const double *yy,
                       const double *den, double *tau);
 
+/* computeQoiTargets (stokesSolver.f:3099-3323) with classifyPointsOld
+ * (:4420-4500): velocity, pressure, vorticity at targets. */
+void orc_qoi_targets(int n, int m, int no, int ibary, const double *xx, const double *yy,
+                     const double *den, int ntar, const double *xt, const double *yt, double *u,
+                     double *v, double *p, double *vort);
+
 /* Number of OpenMP threads the oracle uses (0 = runtime default). */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

@@ -55,6 +55,17 @@ def solver_case(name, n, m, no, xx, yy, ibc, maxl, seed):
     # ibary=0 solve (trapezoid operator) to a tight tolerance
     den0, it0, _e, _ie = R.solve(n, m, no, xx, yy, ibary=0, ifmm=0, ibc=ibc, maxl=maxl, tol=1e-12)
     out["den_trap_tol12"], out["iter_trap_tol12"] = den0, it0
+    # computeQoiTargets at the pressure-drop slices x = +-2 (DensityStress.jl:131-136),
+    # at every body centre (inside: zeroed) and just outside two bodies
+    dy = 2.0 / 13
+    ys = -1 + 0.5 * dy + dy * np.arange(13)
+    geo_cx, geo_cy = out["geo_cx"], out["geo_cy"]
+    xt = np.concatenate([np.full(13, 2.0), np.full(13, -2.0), geo_cx,
+                         xx[[0, n]] - 0.01 * out["geo_px"][[0, n]]])
+    yt = np.concatenate([ys, ys, geo_cy, yy[[0, n]] - 0.01 * out["geo_py"][[0, n]]])
+    out["qoi_xt"], out["qoi_yt"] = xt, yt
+    out["qoi_bary"] = np.array(R.qoi_targets(n, m, no, 1, xx, yy, den12, xt, yt))
+    out["qoi_trap"] = np.array(R.qoi_targets(n, m, no, 0, xx, yy, den12, xt, yt))
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
     print(f"{name}: N={N} iters tol8={it8} tol12={it12} trap={it0}")
 

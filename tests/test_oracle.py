@@ -268,3 +268,19 @@ def test_deformation_is_traceless(golden):
     g = golden("tight8_n64")
     e11, e12, e22 = _geom(g).deformation(g["den_tol12"])
     assert np.max(np.abs(e11 + e22)) < 1e-8 * np.max(np.abs(e11))
+
+
+# -------------------------------------------------------------------- QoI
+@pytest.mark.parametrize("name", FIXTURES)
+@pytest.mark.parametrize("ibary,key", [(1, "qoi_bary"), (0, "qoi_trap")])
+def test_qoi_targets_match_reference(golden, name, ibary, key):
+    """computeQoiTargets + classifyPointsOld (stokesSolver.f:3099-3323, 4420-4500)."""
+    g = golden(name)
+    out = O.qoi_targets(int(g["n"]), int(g["m"]), int(g["no"]), ibary, g["xx"], g["yy"],
+                        g["den_tol12"], g["qoi_xt"], g["qoi_yt"])
+    for i in range(4):
+        assert relerr(out[i], g[key][i]) < 1e-10, i
+    # body centres are inside: all four quantities zeroed
+    m = int(g["m"])
+    centres = slice(26, 26 + m)
+    assert np.all(np.array(out)[:, centres] == 0.0)
