@@ -207,6 +207,8 @@ int erosion_last_solve_info(double* info) {
         info[2] = c.last_true_relres;
         info[3] = c.last_seconds;
         info[4] = static_cast<double>(c.last_reorth);
+        info[5] = static_cast<double>(c.ff_tpt);
+        info[6] = static_cast<double>(c.ff_splits);
         return 0;
     } catch (const std::exception& e) {
         return record_error("erosion_last_solve_info", e.what());

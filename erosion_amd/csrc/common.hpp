@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -185,7 +186,9 @@ struct Context {
     DevBuf<double> bd;              // boundary values [5 fields][2][npts]
     DevBuf<double> fld;             // field densities [5][2][npts]
     DevBuf<double> scal;            // sigdotn etc.
-    int ff_splits = 0;
+    int ff_splits = 0, ff_tpt = 0;
+    std::map<int, int> ff_choice;      // boundary points -> far-field shape index
+    std::map<int, float> ff_tuned_ms;  // boundary points -> tuned launch time
     // GMRES buffers
     int npad = 0;
     int maxl_alloc = 0;

@@ -99,7 +99,8 @@ def last_solve_info() -> dict:
     info = np.zeros(8)
     lib().erosion_last_solve_info(_p(info))
     return {"iters": int(info[0]), "implicit_relres": float(info[1]),
-            "true_relres": float(info[2]), "seconds": float(info[3]), "reorth": int(info[4])}
+            "true_relres": float(info[2]), "seconds": float(info[3]), "reorth": int(info[4]),
+            "ff_targets_per_lane": int(info[5]), "ff_splits": int(info[6])}
 
 
 def stokessolver(npts: int, nbods: int, nouter: int, ifmm: int, ibary: int, ibc: int, maxl: int,

@@ -116,17 +116,49 @@ def timestep_(thld0: ThLenDenType, thld_derivs: ThLenDenType, dt1: float, dt2: f
     return new_thlenden(thlv1)
 
 
+class PreviousDensity:
+    """GMRES initial guess = the most recently computed density, when the
+    body count still matches (after a deletion the guess falls back to zeros).
+
+    Consecutive solves of an erosion run differ by half a time step, so the
+    previous density is close to the next one; this is the warm start any
+    time stepper can use without a model."""
+
+    def __init__(self):
+        self.last = None
+
+    def __call__(self, thld: ThLenDenType):
+        if self.last is None:
+            return None
+        n = len(thld.thlenvec[0].theta) if thld.thlenvec else 0
+        m = len(thld.thlenvec)
+        # the wall part is 2*nouter long: infer it from the stored vector
+        no2 = len(self.last) - 2 * n * m - 3 * m
+        return self.last if (no2 > 0 and no2 % 2 == 0 and self.last_m == m) else None
+
+    def seen(self, thld: ThLenDenType) -> None:
+        if len(thld.density) > 0:
+            self.last = thld.density
+            self.last_m = len(thld.thlenvec)
+
+
 def rungekutta2(thld0: ThLenDenType, params: ParamSet,
                 guess: Optional[Callable[[ThLenDenType], Optional[np.ndarray]]] = None,
                 timings=None) -> ThLenDenType:
     """One erosion step: midpoint RK2 (TimeStepping.jl:163-176).
 
-    ``guess(thlenden)`` may return a GMRES initial guess for that state."""
+    ``guess(thlenden)`` may return a GMRES initial guess for that state; if it
+    has a ``seen`` method it is told about every state whose density was just
+    computed."""
     dt = params.dt
     g0 = guess(thld0) if guess else None
     thld05 = timestep_(thld0, thld0, 0.5 * dt, 0.5 * dt, params, den0=g0, timings=timings)
+    if guess is not None and hasattr(guess, "seen"):
+        guess.seen(thld0)
     g1 = guess(thld05) if guess else None
     thld1 = timestep_(thld0, thld05, dt, 0.5 * dt, params, den0=g1, timings=timings)
+    if guess is not None and hasattr(guess, "seen"):
+        guess.seen(thld05)
     thld1.tt = thld0.tt + dt
     return thld1
 

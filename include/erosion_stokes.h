@@ -93,7 +93,8 @@ const char* erosion_last_error(void);
 /* Record of the last stokessolver_ call:
  * info[0] iterations, info[1] implicit relative residual, info[2] true relative
  * residual ||b - A x|| / ||b|| of the returned density, info[3] seconds,
- * info[4] re-orthogonalised iterations.  Returns 0. */
+ * info[4] re-orthogonalised iterations, info[5] far-field targets per lane and
+ * info[6] far-field source splits chosen by the autotuner.  Returns 0. */
 int erosion_last_solve_info(double* info);
 
 /* One application of the completed operator vel = A den, as matvec_DLP_fmmbary

@@ -129,14 +129,23 @@ def test_initial_guess_is_used(golden):
 
 
 def test_maxl_exhaustion(golden):
+    """maxl reached: status 1, nits = maxl, the partial solution returned.
+
+    The 7th iterate sits in GMRES's plateau (it moves by ~1e-2 per iteration
+    and its least-squares problem amplifies ~1e-13 geometry rounding to
+    ~1e-8), so it is compared at 1e-6; converged solves are compared at 1e-10
+    elsewhere."""
     g = golden("tight8_n64")
     n, m, no, xx, yy = _case(g)
     den, it = stokes.stokessolver(n, m, no, 1, 1, 0, 7, xx, yy)
     assert it == 7 and stokes.last_status() == 1
     geo = O.Geometry(n, m, no, xx, yy)
     ref, it_o, _err, ierr = geo.gmres(geo.rhs(0), ibary=1, maxl=7, tol=1e-8)
+    ref8, _, _, _ = geo.gmres(geo.rhs(0), ibary=1, maxl=8, tol=1e-8)
     assert it_o == 7 and ierr == 1
-    assert relerr(den, ref) < 1e-10
+    assert relerr(den, ref) < 1e-6 < relerr(ref8, ref)
+    implicit = stokes.last_solve_info()["implicit_relres"]
+    assert abs(implicit - _err) < 1e-8
 
 
 def test_unsupported_combinations_raise(golden):
