@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import argparse
 import contextlib
+import copy
 import json
 import math
 import os
@@ -123,7 +124,8 @@ def main() -> int:
     from erosion_amd import stokes
     from erosion_amd.geometry import WORKLOADS, workload_geometry
     from erosion_amd.theta_len import ParamSet, getnxy
-    from erosion_amd.time_stepping import rungekutta2
+    from erosion_amd.density_stress import getstress_
+    from erosion_amd.time_stepping import PreviousDensity, rungekutta2
 
     try:
         import torch
@@ -137,33 +139,65 @@ def main() -> int:
     params = ParamSet(npts=w.npts, nouter=w.nouter, ibary=1, ifmm=1, ibc=w.ibc, dt=w.dt,
                       maxl=8000, fixpdrop=True)
     stokes.set_gmres_tol(args.tol)
-    thld = workload_geometry(w)
-    _m, xx0, yy0 = getnxy(thld)
+    thld0 = workload_geometry(w)
+    _m, xx0, yy0 = getnxy(thld0)
 
-    timings = {}
-    for i in range(args.warmup):
-        thld = rungekutta2(thld, params, timings=timings)
-        log(f"warmup step {i + 1}/{args.warmup}: bodies={len(thld.thlenvec)}")
-    sync()
-    stokes.kernel_timing(True)
-    timings = {}
-    solve_info = []
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        ts = time.perf_counter()
-        thld = rungekutta2(thld, params, timings=timings)
-        solve_info.append(stokes.last_solve_info())
-        log(f"step {i + 1}/{args.steps}: {time.perf_counter() - ts:.3f} s, bodies={len(thld.thlenvec)}, "
-            f"gmres iters so far {timings.get('gmres_iters', 0)}, last true relres "
-            f"{solve_info[-1]['true_relres']:.2e}")
-    sync()
-    elapsed = time.perf_counter() - t0
-    kt = stokes.kernel_timers()
-    stokes.kernel_timing(False)
+    def run_variant(label, accelerated):
+        """Warm-up steps, then exactly args.steps timed steps from the initial state."""
+        stokes.set_preconditioner(1 if accelerated else 0)
+        guess = PreviousDensity() if accelerated else None
+        th = copy.deepcopy(thld0)
+        for i in range(args.warmup):
+            th = rungekutta2(th, params, guess=guess, timings={})
+            log(f"[{label}] warmup step {i + 1}/{args.warmup}: bodies={len(th.thlenvec)}")
+        sync()
+        stokes.kernel_timing(True)
+        timings, solve_info = {}, []
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            ts = time.perf_counter()
+            th = rungekutta2(th, params, guess=guess, timings=timings)
+            solve_info.append(stokes.last_solve_info())
+            log(f"[{label}] step {i + 1}/{args.steps}: {time.perf_counter() - ts:.3f} s, "
+                f"bodies={len(th.thlenvec)}, gmres iters so far {timings.get('gmres_iters', 0)}, "
+                f"last true relres {solve_info[-1]['true_relres']:.2e}")
+        sync()
+        elapsed = time.perf_counter() - t0
+        kt = stokes.kernel_timers()
+        stokes.kernel_timing(False)
+        stokes.set_preconditioner(0)
+        return {"elapsed": elapsed, "timings": timings, "kt": kt, "solve_info": solve_info,
+                "state": th}
 
-    steps_per_s = args.steps / elapsed
-    solves = max(1, timings.get("solves", 1))
-    iters_per_solve = timings.get("gmres_iters", 0) / solves
+    def first_stress(accelerated):
+        """Shear stress of the initial state through one variant's solve."""
+        stokes.set_preconditioner(1 if accelerated else 0)
+        atau = getstress_(copy.deepcopy(thld0), params)
+        stokes.set_preconditioner(0)
+        return np.asarray(atau)
+
+    tau_plain, tau_acc = first_stress(False), first_stress(True)
+    stress_err = float(np.max(np.abs(tau_acc - tau_plain)) / np.max(np.abs(tau_plain)))
+    log(f"shear stress of the initial state, accelerated vs plain FP64: {stress_err:.2e} relative")
+
+    plain = run_variant("plain-fp64", False)
+    acc = run_variant("accelerated", True)
+    theta_err = max(float(np.max(np.abs(a.theta - b.theta)))
+                    for a, b in zip(acc["state"].thlenvec, plain["state"].thlenvec)) \
+        if len(acc["state"].thlenvec) == len(plain["state"].thlenvec) else float("nan")
+
+    def summary(r):
+        solves = max(1, r["timings"].get("solves", 1))
+        return {"steps_per_s": args.steps / r["elapsed"],
+                "ms_per_step": 1e3 * r["elapsed"] / args.steps,
+                "gmres_iters_per_solve": r["timings"].get("gmres_iters", 0) / solves,
+                "true_relres_max": max(si["true_relres"] for si in r["solve_info"]),
+                "matvecs_per_s": r["kt"]["farfield"]["launches"] / r["elapsed"]}
+
+    sp, sa = summary(plain), summary(acc)
+    elapsed, kt = acc["elapsed"], acc["kt"]
+    steps_per_s = sa["steps_per_s"]
+    iters_per_solve = sp["gmres_iters_per_solve"]
     ff = kt["farfield"]
     ff_ms = ff["ms"] / max(1, ff["launches"])
     ff_pairs = ff["work"] / max(1, ff["launches"])
@@ -172,7 +206,9 @@ def main() -> int:
     gs_gbs = gs["work"] / (gs["ms"] * 1e-3) / 1e9 if gs["ms"] > 0 else 0.0
     se = kt["stress_eval"]
     share = {k: kt[k]["ms"] / (elapsed * 1e3) for k in kt}
-    log(f"kernel time shares of the timed region: " +
+    log(f"plain-fp64: {sp}")
+    log(f"accelerated: {sa}")
+    log(f"kernel time shares of the accelerated timed region: " +
         ", ".join(f"{k} {100 * v:.1f}%" for k, v in share.items()))
     log(f"far field: {ff_ms:.3f} ms/launch, {ff_tflops:.2f} TFLOP/s; Gram-Schmidt {gs_gbs:.0f} GB/s; "
         f"stress eval {se['ms'] / max(1, se['launches']):.2f} ms/launch")
@@ -199,11 +235,16 @@ def main() -> int:
         "data": "synthetic: seeded perturbed-circle porous geometry, theta-L state",
         "config": {
             "workload": w.name, "bodies": w.nbodies, "points_per_body": w.npts,
-            "wall_points": w.nouter, "gmres_tol": args.tol, "variant": "plain-fp64",
+            "wall_points": w.nouter, "gmres_tol": args.tol,
+            "variant": "accelerated: per-curve two-level right preconditioner + previous-density "
+                       "initial guess, FP64",
             "dt": w.dt, "fixpdrop": True, "ibc": w.ibc,
-            "gmres_iters_per_solve": iters_per_solve,
-            "true_relres_last": solve_info[-1]["true_relres"] if solve_info else None,
-            "matvecs_per_s": ff["launches"] / elapsed,
+            "accelerated": sa,
+            "plain_fp64": sp,
+            "speedup_vs_plain_fp64": sa["steps_per_s"] / sp["steps_per_s"],
+            "stress_relerr_vs_plain_fp64": stress_err,
+            "theta_maxdiff_after_steps": theta_err,
+            "matvecs_per_s": sa["matvecs_per_s"],
             "farfield_ms": ff_ms,
             "gram_schmidt_GBps": gs_gbs,
             "kernel_share": share,

@@ -209,6 +209,7 @@ int erosion_last_solve_info(double* info) {
         info[4] = static_cast<double>(c.last_reorth);
         info[5] = static_cast<double>(c.ff_tpt);
         info[6] = static_cast<double>(c.ff_splits);
+        info[7] = static_cast<double>(c.pc_kind);
         return 0;
     } catch (const std::exception& e) {
         return record_error("erosion_last_solve_info", e.what());
@@ -300,14 +301,74 @@ int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t iba
     }
 }
 
+int erosion_set_preconditioner(int kind, int body_modes, int wall_modes) {
+    try {
+        if (kind != 0 && kind != 1) return -1;
+        if (kind == 1 && (body_modes < 1 || body_modes > 16 || wall_modes < 1 || wall_modes > 16))
+            return -1;
+        Context& c = stokes::ctx();
+        c.pc_kind = kind;
+        if (kind == 1) {
+            c.pc_body_modes = body_modes;
+            if (c.pc_wall_modes != wall_modes) c.pc_wall_no = -1;
+            c.pc_wall_modes = wall_modes;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_set_preconditioner", e.what());
+    }
+}
+
+int erosion_get_preconditioner(int* out) {
+    if (!out) return -1;
+    try {
+        Context& c = stokes::ctx();
+        out[0] = c.pc_kind;
+        out[1] = c.pc_body_modes;
+        out[2] = c.pc_wall_modes;
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_get_preconditioner", e.what());
+    }
+}
+
+int erosion_precond_apply(int64_t npts, int64_t nbods, int64_t nouter, const double* xx,
+                          const double* yy, const double* r, double* z) {
+    try {
+        check_sizes(npts, nbods, nouter);
+        if (!r || !z || (nbods > 0 && (!xx || !yy))) throw stokes::Error("null argument");
+        Context& c = stokes::ctx();
+        if (c.pc_kind == 0) throw stokes::Error("no preconditioner selected");
+        c.status = 0;
+        const int n = static_cast<int>(npts), m = static_cast<int>(nbods),
+                  no = static_cast<int>(nouter);
+        stokes::build_geometry(c, n, m, no, xx, yy);
+        stokes::precond_setup(c);
+        const size_t nunk = static_cast<size_t>(c.g.nunk);
+        c.xvec.ensure(std::max(nunk, c.xvec.cap));
+        c.rvec.ensure(std::max(nunk, c.rvec.cap));
+        STOKES_HIP(hipMemcpyAsync(c.xvec.p, r, nunk * sizeof(double), hipMemcpyHostToDevice,
+                                  c.stream));
+        stokes::precond_apply(c, c.xvec.p, c.rvec.p);
+        STOKES_HIP(hipMemcpyAsync(z, c.rvec.p, nunk * sizeof(double), hipMemcpyDeviceToHost,
+                                  c.stream));
+        STOKES_HIP(hipStreamSynchronize(c.stream));
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_precond_apply", e.what());
+    }
+}
+
 int erosion_kernel_timing(int enable) {
     try {
         Context& c = stokes::ctx();
         if (enable) {
-            for (stokes::KernelTimer* t : {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs})
+            for (stokes::KernelTimer* t :
+                 {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs, &c.t_precond})
                 t->reset();
         } else {
-            for (stokes::KernelTimer* t : {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs})
+            for (stokes::KernelTimer* t :
+                 {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs, &c.t_precond})
                 t->harvest();
         }
         c.time_kernels = enable != 0;
@@ -321,8 +382,9 @@ int erosion_kernel_timers(double* out) {
     if (!out) return -1;
     try {
         Context& c = stokes::ctx();
-        stokes::KernelTimer* ts[4] = {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs};
-        for (int i = 0; i < 4; ++i) {
+        stokes::KernelTimer* ts[5] = {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs,
+                                      &c.t_precond};
+        for (int i = 0; i < 5; ++i) {
             ts[i]->harvest();
             out[3 * i + 0] = ts[i]->ms;
             out[3 * i + 1] = static_cast<double>(ts[i]->launches);

@@ -205,9 +205,15 @@ struct Context {
     DevBuf<int> curves_all, grp_beg, grp_end, iside;
     DevBuf<double> tgt;     // target coordinates and QoI outputs
     double tol = 1e-8;
+    // right preconditioner (precond.hip): kind 0 none (the reference's
+    // unpreconditioned GMRES), 1 per-curve two-level Fourier
+    int pc_kind = 0, pc_body_modes = 8, pc_wall_modes = 16;
+    int pc_nfb = 0, pc_nfw = 0;
+    int pc_wall_no = -1, pc_wall_kw = -1;  // key of the kept wall block
+    DevBuf<double> pc_phi_b, pc_phi_w, pc_S, pc_Gb, pc_Gw, pc_z;
     // per-kernel timing with HIP events on `stream` (measurement hooks only)
     bool time_kernels = false;
-    KernelTimer t_farfield, t_near, t_stress_eval, t_gs;
+    KernelTimer t_farfield, t_near, t_stress_eval, t_gs, t_precond;
     int status = 0;
     std::string last_error;
     // last solve record
@@ -230,6 +236,10 @@ void bd_condition(Context& c, int ibc, double* rhs_dev);
 
 // gmres.hip
 long long gmres_solve(Context& c, int ibary, int ibc, int maxl, double tol, double* den_host);
+
+// precond.hip: set up for the current geometry, then z = P^{-1} r
+void precond_setup(Context& c);
+void precond_apply(Context& c, const double* r_dev, double* z_dev);
 
 // stress.hip
 void shear_stress(Context& c, const double* den_dev, double* tau_dev);

@@ -31,6 +31,10 @@ namespace {
 constexpr int kBlk = 256;
 constexpr int kRows = 4;       // Krylov rows per block in the dot kernel
 constexpr int kChunkIters = 16;
+// preconditioned solves: implicit target of a cycle as a fraction of the
+// tolerance, and the most cycles one solve may run
+constexpr double kPcTolFrac = 0.5;
+constexpr int kMaxCycles = 4;
 constexpr double kReorthEta = 0.70710678118654752440;
 
 inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
@@ -65,6 +69,13 @@ __global__ __launch_bounds__(kBlk) void k_axpby_norm(double a, const double* __r
     }
     double s = block_sum(acc, sh);
     if (threadIdx.x == 0) npart[blockIdx.x] = s;
+}
+
+// x += y
+__global__ __launch_bounds__(kBlk) void k_add(double* __restrict__ x, const double* __restrict__ y,
+                                              size_t n) {
+    const size_t e = static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x;
+    if (e < n) x[e] += y[e];
 }
 
 __global__ void k_zero_tail(double* __restrict__ v, int n, int npad) {
@@ -432,93 +443,145 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
     bd_condition(c, ibc, c.rhs.p);
     STOKES_HIP(hipMemcpyAsync(c.xvec.p, den_host, sizeof(double) * N, hipMemcpyHostToDevice, s));
 
+    // right preconditioning: GMRES on A P^{-1}, x = x0 + P^{-1} V y
+    const bool pc = c.pc_kind != 0;
+    if (pc) {
+        precond_setup(c);
+        c.pc_z.ensure(npad);
+        STOKES_HIP(hipMemsetAsync(c.pc_z.p, 0, npad * sizeof(double), s));
+    }
+
     const int nb = static_cast<int>(std::min<size_t>(1024, npad / (2 * kBlk)));
     k_axpby_norm<<<nb, kBlk, 0, s>>>(1.0, c.rhs.p, 0.0, nullptr, nullptr, npad, bpart.p);
     STOKES_LAUNCH_CHECK();
-    // r0 = b - A x0 into V[0]
+    // The reference's iteration: one GMRES cycle stopped by the implicit
+    // residual estimate.  With a preconditioner the cycle aims at half the
+    // tolerance and, should the true residual of the returned density still
+    // exceed the tolerance, GMRES restarts from it (at most kMaxCycles cycles
+    // within maxl iterations in total).
+    const double tol_cycle = pc ? kPcTolFrac * tol : tol;
     double* V0 = c.V.p;
-    apply_operator(c, c.xvec.p, c.rvec.p, ibary);
-    k_zero_tail<<<cdiv_int(static_cast<int>(npad) - N + 1, kBlk), kBlk, 0, s>>>(c.rvec.p, N,
-                                                                              static_cast<int>(npad));
-    STOKES_LAUNCH_CHECK();
-    k_axpby_norm<<<nb, kBlk, 0, s>>>(1.0, c.rhs.p, -1.0, c.rvec.p, V0, npad, c.npart.p);
-    STOKES_LAUNCH_CHECK();
-    k_init_state<<<1, 64, 0, s>>>(bpart.p, nb, c.npart.p, nb, tol, c.st.p);
-    STOKES_LAUNCH_CHECK();
-    k_scale_row<<<std::min(1024, ebx * 2), kBlk, 0, s>>>(V0, npad, c.st.p, 1);
-    STOKES_LAUNCH_CHECK();
-
-    int k = 0;
-    while (k < maxl) {
-        const int chunk = std::min(kChunkIters, maxl - k);
-        for (int j = 0; j < chunk; ++j) {
-            const int kk = k + j;
-            double* vk = c.V.p + static_cast<size_t>(kk) * npad;
-            double* w = c.V.p + static_cast<size_t>(kk + 1) * npad;
-            apply_operator(c, vk, w, ibary);
+    long long total_iters = 0;
+    int status = 0, nreorth = 0;
+    double implicit_relres = 0.0, true_relres = 0.0;
+    for (int cycle = 0;; ++cycle) {
+        const int maxl_cycle = maxl - static_cast<int>(total_iters);
+        if (cycle == 0) {
+            // r0 = b - A x0 into V[0]
+            apply_operator(c, c.xvec.p, c.rvec.p, ibary);
             k_zero_tail<<<cdiv_int(static_cast<int>(npad) - N + 1, kBlk), kBlk, 0, s>>>(
-                w, N, static_cast<int>(npad));
+                c.rvec.p, N, static_cast<int>(npad));
             STOKES_LAUNCH_CHECK();
-            if (c.time_kernels) c.t_gs.start(s);
-            gs_pass(c, w, kk + 1, 1, hcur.p);
-            gs_pass(c, w, kk + 1, 2, hcur.p);
-            k_givens<<<1, kGivT, 0, s>>>(c.H.p, ldh, c.q.p, c.hbuf.p, c.st.p, tol, maxl);
-            STOKES_LAUNCH_CHECK();
-            if (c.time_kernels) {
-                // Krylov rows streamed by the dot and update kernels of pass 1
-                c.t_gs.stop(s, 2.0 * static_cast<double>(kk + 1) * static_cast<double>(npad) * 8.0);
-            }
-            k_scale_row<<<std::min(1024, ebx * 2), kBlk, 0, s>>>(w, npad, c.st.p, 0);
-            STOKES_LAUNCH_CHECK();
+            k_axpby_norm<<<nb, kBlk, 0, s>>>(1.0, c.rhs.p, -1.0, c.rvec.p, V0, npad, c.npart.p);
+        } else {
+            // the previous cycle left r = b - A x in rvec
+            k_axpby_norm<<<nb, kBlk, 0, s>>>(1.0, c.rvec.p, 0.0, nullptr, V0, npad, c.npart.p);
         }
-        k += chunk;
-        STOKES_HIP(hipMemcpyAsync(c.h_st, c.st.p, sizeof(GmresState), hipMemcpyDeviceToHost, s));
+        STOKES_LAUNCH_CHECK();
+        k_init_state<<<1, 64, 0, s>>>(bpart.p, nb, c.npart.p, nb, tol_cycle, c.st.p);
+        STOKES_LAUNCH_CHECK();
+        k_scale_row<<<std::min(1024, ebx * 2), kBlk, 0, s>>>(V0, npad, c.st.p, 1);
+        STOKES_LAUNCH_CHECK();
+
+        int k = 0;
+        while (k < maxl_cycle) {
+            const int chunk = std::min(kChunkIters, maxl_cycle - k);
+            for (int j = 0; j < chunk; ++j) {
+                const int kk = k + j;
+                double* vk = c.V.p + static_cast<size_t>(kk) * npad;
+                double* w = c.V.p + static_cast<size_t>(kk + 1) * npad;
+                if (pc) {
+                    precond_apply(c, vk, c.pc_z.p);
+                    apply_operator(c, c.pc_z.p, w, ibary);
+                } else {
+                    apply_operator(c, vk, w, ibary);
+                }
+                k_zero_tail<<<cdiv_int(static_cast<int>(npad) - N + 1, kBlk), kBlk, 0, s>>>(
+                    w, N, static_cast<int>(npad));
+                STOKES_LAUNCH_CHECK();
+                if (c.time_kernels) c.t_gs.start(s);
+                gs_pass(c, w, kk + 1, 1, hcur.p);
+                gs_pass(c, w, kk + 1, 2, hcur.p);
+                k_givens<<<1, kGivT, 0, s>>>(c.H.p, ldh, c.q.p, c.hbuf.p, c.st.p, tol_cycle,
+                                             maxl_cycle);
+                STOKES_LAUNCH_CHECK();
+                if (c.time_kernels) {
+                    // Krylov rows streamed by the dot and update kernels of pass 1
+                    c.t_gs.stop(s, 2.0 * static_cast<double>(kk + 1) * static_cast<double>(npad) * 8.0);
+                }
+                k_scale_row<<<std::min(1024, ebx * 2), kBlk, 0, s>>>(w, npad, c.st.p, 0);
+                STOKES_LAUNCH_CHECK();
+            }
+            k += chunk;
+            STOKES_HIP(hipMemcpyAsync(c.h_st, c.st.p, sizeof(GmresState), hipMemcpyDeviceToHost, s));
+            STOKES_HIP(hipStreamSynchronize(s));
+            if (c.h_st->done) break;
+        }
+        GmresState fin = *c.h_st;
+        const int iters = fin.k;
+        status = 0;
+        if (fin.done == 3) {
+            status = 2;  // breakdown: solution left at x0 (DPIGMR IFLAG=2)
+        } else if (iters > 0 && (fin.done == 1 || fin.rho < fin.r0nrm)) {
+            // x += V y (DHELS + the sum of DPIGMR label 200)
+            k_hels<<<1, 1024, 0, s>>>(c.H.p, ldh, c.q.p, c.st.p, c.ybuf.p);
+            STOKES_LAUNCH_CHECK();
+            int nsplit = std::min(iters, std::max(1, cdiv_int(1024, ebx)));
+            int rps = cdiv_int(iters, nsplit);
+            nsplit = cdiv_int(iters, rps);
+            k_update<<<dim3(ebx, nsplit), kBlk, 0, s>>>(c.V.p, npad, c.ybuf.p, iters, rps, c.st.p,
+                                                        0, c.wpart.p);
+            STOKES_LAUNCH_CHECK();
+            if (pc) {
+                // u = V y, then x += P^{-1} u
+                STOKES_HIP(hipMemsetAsync(c.rvec.p, 0, npad * sizeof(double), s));
+                k_combine<<<ebx, kBlk, 0, s>>>(c.rvec.p, npad, c.wpart.p, nsplit, 1.0, c.st.p, 0,
+                                               c.npart.p);
+                STOKES_LAUNCH_CHECK();
+                precond_apply(c, c.rvec.p, c.pc_z.p);
+                k_add<<<static_cast<int>(npad / kBlk), kBlk, 0, s>>>(c.xvec.p, c.pc_z.p, npad);
+                STOKES_LAUNCH_CHECK();
+            } else {
+                k_combine<<<ebx, kBlk, 0, s>>>(c.xvec.p, npad, c.wpart.p, nsplit, 1.0, c.st.p, 0,
+                                               c.npart.p);
+                STOKES_LAUNCH_CHECK();
+            }
+            status = fin.done == 1 ? 0 : 1;
+        } else if (fin.done != 1) {
+            status = fin.done == 2 ? 1 : 2;
+        }
+        // true residual of the returned density, left in rvec
+        apply_operator(c, c.xvec.p, c.rvec.p, ibary);
+        k_zero_tail<<<cdiv_int(static_cast<int>(npad) - N + 1, kBlk), kBlk, 0, s>>>(
+            c.rvec.p, N, static_cast<int>(npad));
+        STOKES_LAUNCH_CHECK();
+        k_axpby_norm<<<nb, kBlk, 0, s>>>(1.0, c.rhs.p, -1.0, c.rvec.p, c.rvec.p, npad, c.npart.p);
+        STOKES_LAUNCH_CHECK();
+        std::vector<double> part(static_cast<size_t>(nb));
+        STOKES_HIP(hipMemcpyAsync(part.data(), c.npart.p, sizeof(double) * nb,
+                                  hipMemcpyDeviceToHost, s));
         STOKES_HIP(hipStreamSynchronize(s));
-        if (c.h_st->done) break;
+        double rr = 0.0;
+        for (double v : part) rr += v;
+        total_iters += iters;
+        nreorth += fin.nreorth;
+        implicit_relres = fin.rho / fin.bnrm;
+        true_relres = std::sqrt(rr) / fin.bnrm;
+        if (!pc || status != 0 || true_relres <= tol || total_iters >= maxl ||
+            cycle + 1 >= kMaxCycles)
+            break;
     }
-    GmresState fin = *c.h_st;
-    const int iters = fin.k;
-    int status = 0;
-    if (fin.done == 3) {
-        status = 2;  // breakdown: solution left at x0 (DPIGMR IFLAG=2)
-    } else if (iters > 0 && (fin.done == 1 || fin.rho < fin.r0nrm)) {
-        // x += V y (DHELS + the sum of DPIGMR label 200)
-        k_hels<<<1, 1024, 0, s>>>(c.H.p, ldh, c.q.p, c.st.p, c.ybuf.p);
-        STOKES_LAUNCH_CHECK();
-        int nsplit = std::min(iters, std::max(1, cdiv_int(1024, ebx)));
-        int rps = cdiv_int(iters, nsplit);
-        nsplit = cdiv_int(iters, rps);
-        k_update<<<dim3(ebx, nsplit), kBlk, 0, s>>>(c.V.p, npad, c.ybuf.p, iters, rps, c.st.p, 0,
-                                                    c.wpart.p);
-        STOKES_LAUNCH_CHECK();
-        k_combine<<<ebx, kBlk, 0, s>>>(c.xvec.p, npad, c.wpart.p, nsplit, 1.0, c.st.p, 0,
-                                       c.npart.p);
-        STOKES_LAUNCH_CHECK();
-        status = fin.done == 1 ? 0 : 1;
-    } else if (fin.done != 1) {
-        status = fin.done == 2 ? 1 : 2;
-    }
-    // true residual of the returned density
-    apply_operator(c, c.xvec.p, c.rvec.p, ibary);
-    k_zero_tail<<<cdiv_int(static_cast<int>(npad) - N + 1, kBlk), kBlk, 0, s>>>(c.rvec.p, N,
-                                                                              static_cast<int>(npad));
-    STOKES_LAUNCH_CHECK();
-    k_axpby_norm<<<nb, kBlk, 0, s>>>(1.0, c.rhs.p, -1.0, c.rvec.p, c.rvec.p, npad, c.npart.p);
-    STOKES_LAUNCH_CHECK();
-    std::vector<double> part(static_cast<size_t>(nb));
-    STOKES_HIP(hipMemcpyAsync(part.data(), c.npart.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    if (pc && status == 0 && true_relres > tol) status = 1;
     STOKES_HIP(hipMemcpyAsync(den_host, c.xvec.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
     STOKES_HIP(hipStreamSynchronize(s));
-    double rr = 0.0;
-    for (double v : part) rr += v;
-    c.last_iters = iters;
-    c.last_implicit_relres = fin.rho / fin.bnrm;
-    c.last_true_relres = std::sqrt(rr) / fin.bnrm;
-    c.last_reorth = fin.nreorth;
+    c.last_iters = total_iters;
+    c.last_implicit_relres = implicit_relres;
+    c.last_true_relres = true_relres;
+    c.last_reorth = nreorth;
     c.status = status;
     c.last_seconds =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
-    return iters;
+    return total_iters;
 }
 
 }  // namespace stokes

@@ -1,0 +1,443 @@
+// Per-curve two-level right preconditioner for the completed double-layer
+// operator.  The reference runs GMRES without a preconditioner (solveBIE,
+// ErosionPackage/fortran_code/stokesSolver.f:490; its unused msolve_DLP,
+// :2211-2302, treats only Fourier modes +-1 of a circle); this is an
+// addition with no counterpart there, off unless a caller asks for it.
+//
+// For every curve c (the wall, each body) let B_c be the block of the
+// operator that couples the curve's own unknowns: the trapezoid
+// self-interaction, the curvature diagonal and the jump -eta/2, plus for a
+// body its stokeslet/rotlet columns and constraint rows, for the wall the
+// rank-one term.  With Q_c the orthonormal Fourier modes 0..K of each
+// density component sampled at the curve's points (and, for a body, the
+// three constraint unknowns),
+//     P_c^{-1} r = Q_c (Q_c^T B_c Q_c)^{-1} Q_c^T r - 2 (r - Q_c Q_c^T r),
+// the exact inverse on the coarse space and the inverse -2 of the jump on
+// the rest (the smooth self-interaction damps high modes).  GMRES runs on
+// A P^{-1} (right preconditioning): its residual is the residual of A x = b,
+// so the stopping test and the returned density keep their meaning.
+//
+// Set-up per solve: k_pc_pairs (K-mode projections of the trapezoid
+// self-interaction, O(n^2 K) per body), k_pc_coarse (coarse matrix and its
+// Gauss-Jordan inverse in LDS, one block per curve).  The wall block depends
+// only on the number of wall points and is kept between solves.  Apply:
+// k_pc_apply, one block per curve.
+#include <algorithm>
+#include <cmath>
+
+#include "common.hpp"
+
+namespace stokes {
+
+namespace {
+
+constexpr int kBlk = 256;
+constexpr int kApplyBlk = 1024;
+constexpr int kQC = 17;                        // basis functions per pair pass
+constexpr int kMaxModes = 16;                  // K <= 16 per curve
+constexpr int kMaxNF = 2 * kMaxModes + 1;      // functions per component
+constexpr int kMaxNQ = 2 * kMaxNF + 3;         // coarse unknowns per curve
+
+inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
+
+// phi[f][i], f = 0 (constant), 2k-1 (cos k theta), 2k (sin k theta), orthonormal
+// over the N equispaced parameter points theta_i = 2 pi i / N.
+__global__ void k_pc_basis(int N, int K, double* __restrict__ phi) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double c0 = 1.0 / sqrt(static_cast<double>(N));
+    const double c1 = sqrt(2.0 / static_cast<double>(N));
+    phi[i] = c0;
+    for (int k = 1; k <= K; ++k) {
+        // reduce k i mod N first so the angle is exact in [0, 2 pi)
+        const long long ki = (static_cast<long long>(k) * i) % N;
+        double sn, cs;
+        sincos(kTwoPi * static_cast<double>(ki) / static_cast<double>(N), &sn, &cs);
+        phi[static_cast<size_t>(2 * k - 1) * N + i] = c1 * cs;
+        phi[static_cast<size_t>(2 * k) * N + i] = c1 * sn;
+    }
+}
+
+struct CurveView {
+    int len, beg, nf;
+    const double* phi;
+    size_t soff;
+};
+
+__device__ __forceinline__ CurveView curve_view(int c, int no, int n, int nfw, int nfb,
+                                                const double* phi_w, const double* phi_b) {
+    CurveView v;
+    if (c == 0) {
+        v.len = no;
+        v.beg = 0;
+        v.nf = nfw;
+        v.phi = phi_w;
+        v.soff = 0;
+    } else {
+        v.len = n;
+        v.beg = no + (c - 1) * n;
+        v.nf = nfb;
+        v.phi = phi_b;
+        v.soff = static_cast<size_t>(no) * 3 * nfw + static_cast<size_t>(c - 1) * n * 3 * nfb;
+    }
+    return v;
+}
+
+// S[(t nf + f) len + i] = sum_{j != i} w_ij r_a r_b phi_f(j) over the curve's
+// own points, t = 0 (xx), 1 (xy), 2 (yy), w_ij = (r.n_j) speed_j (2/len) / rho^4:
+// the trapezoid self-interaction of the curve applied to each basis function.
+// grid (curves, ceil(len / 256)); block 256, one target per lane.
+__global__ __launch_bounds__(kBlk) void k_pc_pairs(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ mx,
+    const double* __restrict__ my, const double* __restrict__ phi_w,
+    const double* __restrict__ phi_b, int nfw, int nfb, int no, int n, int first_curve,
+    double* __restrict__ S) {
+    __shared__ double sx[kBlk], sy[kBlk], smx[kBlk], smy[kBlk];
+    __shared__ double sph[kQC][kBlk];
+    const int c = first_curve + static_cast<int>(blockIdx.x);
+    const CurveView cv = curve_view(c, no, n, nfw, nfb, phi_w, phi_b);
+    if (static_cast<int>(blockIdx.y) * kBlk >= cv.len) return;  // uniform per block
+    const int tid = threadIdx.x;
+    const int i = blockIdx.y * kBlk + tid;
+    const bool valid = i < cv.len;
+    // lanes past the curve evaluate a far-away dummy target and store nothing
+    const double xi = valid ? x[cv.beg + i] : 1.0e3;
+    const double yi = valid ? y[cv.beg + i] : 1.0e3;
+    double* Sc = S + cv.soff;
+    for (int q0 = 0; q0 < cv.nf; q0 += kQC) {
+        double axx[kQC], axy[kQC], ayy[kQC];
+#pragma unroll
+        for (int q = 0; q < kQC; ++q) axx[q] = axy[q] = ayy[q] = 0.0;
+        for (int j0 = 0; j0 < cv.len; j0 += kBlk) {
+            __syncthreads();
+            const int j = j0 + tid;
+            if (j < cv.len) {
+                sx[tid] = x[cv.beg + j];
+                sy[tid] = y[cv.beg + j];
+                smx[tid] = mx[cv.beg + j];
+                smy[tid] = my[cv.beg + j];
+            } else {
+                sx[tid] = sy[tid] = smx[tid] = smy[tid] = 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < kQC; ++q)
+                sph[q][tid] = (j < cv.len && q0 + q < cv.nf)
+                                  ? cv.phi[static_cast<size_t>(q0 + q) * cv.len + j]
+                                  : 0.0;
+            __syncthreads();
+            const int jn = min(kBlk, cv.len - j0);
+            for (int jj = 0; jj < jn; ++jj) {
+                const double rx = xi - sx[jj], ry = yi - sy[jj];
+                const bool self = (j0 + jj) == i;
+                const double rho2 = self ? 1.0 : __builtin_fma(rx, rx, ry * ry);
+                const double inv = rcp64(rho2);
+                const double a = __builtin_fma(rx, smx[jj], ry * smy[jj]);
+                const double w = self ? 0.0 : a * inv * inv;
+                const double wxx = w * rx * rx, wxy = w * rx * ry, wyy = w * ry * ry;
+#pragma unroll
+                for (int q = 0; q < kQC; ++q) {
+                    const double ph = sph[q][jj];
+                    axx[q] = __builtin_fma(wxx, ph, axx[q]);
+                    axy[q] = __builtin_fma(wxy, ph, axy[q]);
+                    ayy[q] = __builtin_fma(wyy, ph, ayy[q]);
+                }
+            }
+        }
+        if (valid) {
+#pragma unroll
+            for (int q = 0; q < kQC; ++q) {
+                if (q0 + q >= cv.nf) break;
+                Sc[static_cast<size_t>(0 * cv.nf + q0 + q) * cv.len + i] = axx[q];
+                Sc[static_cast<size_t>(1 * cv.nf + q0 + q) * cv.len + i] = axy[q];
+                Sc[static_cast<size_t>(2 * cv.nf + q0 + q) * cv.len + i] = ayy[q];
+            }
+        }
+    }
+}
+
+// Coarse matrix G = Q^T B Q of one curve, inverted in place in LDS by
+// Gauss-Jordan elimination with partial (row) pivoting; writes G^{-1}
+// row-major to out + blockIdx.x nq^2.  Coarse unknown order: (x, f) for
+// f < nf, then (y, f), then a body's stokeslet x, y and rotlet.
+__global__ __launch_bounds__(kBlk) void k_pc_coarse(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ px,
+    const double* __restrict__ py, const double* __restrict__ cur,
+    const double* __restrict__ speed, const double* __restrict__ cx,
+    const double* __restrict__ cy, const double* __restrict__ phi_w,
+    const double* __restrict__ phi_b, int nfw, int nfb, int no, int n, int first_curve,
+    const double* __restrict__ S, double* __restrict__ out) {
+    __shared__ double G[kMaxNQ * kMaxNQ];
+    __shared__ double fcol[kMaxNQ];
+    __shared__ int perm[kMaxNQ];
+    const int c = first_curve + static_cast<int>(blockIdx.x);
+    const bool wall = c == 0;
+    const CurveView cv = curve_view(c, no, n, nfw, nfb, phi_w, phi_b);
+    const int nf = cv.nf, len = cv.len;
+    const int nc = 2 * nf;
+    const int nq = wall ? nc : nc + 3;
+    const double* Sc = S + cv.soff;
+    const double* phi = cv.phi;
+    const double dlen = static_cast<double>(len);
+    const double cxb = wall ? 0.0 : cx[c - 1], cyb = wall ? 0.0 : cy[c - 1];
+    const double k4pi = 0.5 / kTwoPi;
+    const int tid = threadIdx.x;
+
+    for (int e = tid; e < nq * nq; e += kBlk) {
+        const int r = e / nq, col = e - r * nq;
+        double s = 0.0;
+        if (r < nc && col < nc) {
+            const int a = r / nf, p = r - a * nf, b = col / nf, q = col - b * nf;
+            const double* Srow = Sc + static_cast<size_t>((a + b) * nf + q) * len;
+            const double* pp = phi + static_cast<size_t>(p) * len;
+            const double* pq = phi + static_cast<size_t>(q) * len;
+            for (int i = 0; i < len; ++i) {
+                const int gi = cv.beg + i;
+                const double f = cur[gi] * speed[gi] / dlen;
+                const double nx = px[gi], ny = py[gi];
+                double L;
+                if (a == 0 && b == 0)
+                    L = -f * ny * ny - 0.5;
+                else if (a == 1 && b == 1)
+                    L = -f * nx * nx - 0.5;
+                else
+                    L = f * nx * ny;
+                s = __builtin_fma(pp[i], __builtin_fma(L, pq[i], Srow[i]), s);
+            }
+            if (wall) {
+                // rank-one completion n_i sum_j (n_j . eta_j) speed_j 2 pi / no
+                double s1 = 0.0, s2 = 0.0;
+                for (int i = 0; i < len; ++i) {
+                    const double na = a == 0 ? px[i] : py[i];
+                    const double nb = b == 0 ? px[i] : py[i];
+                    s1 = __builtin_fma(pp[i], na, s1);
+                    s2 = __builtin_fma(speed[i] * kTwoPi / dlen * nb, pq[i], s2);
+                }
+                s = __builtin_fma(s1, s2, s);
+            }
+        } else if (r < nc) {
+            // stokeslet / rotlet column of the body's own centre
+            const int a = r / nf, p = r - a * nf, cc = col - nc;
+            const double* pp = phi + static_cast<size_t>(p) * len;
+            for (int i = 0; i < len; ++i) {
+                const int gi = cv.beg + i;
+                const double qx = x[gi] - cxb, qy = y[gi] - cyb;
+                const double r2 = __builtin_fma(qx, qx, qy * qy);
+                const double ir2 = 1.0 / r2;
+                double t;
+                if (cc == 0)
+                    t = a == 0 ? k4pi * (-0.5 * log(r2) + qx * qx * ir2) : k4pi * qx * qy * ir2;
+                else if (cc == 1)
+                    t = a == 0 ? k4pi * qx * qy * ir2 : k4pi * (-0.5 * log(r2) + qy * qy * ir2);
+                else
+                    t = a == 0 ? qy * ir2 : -qx * ir2;
+                s = __builtin_fma(pp[i], t, s);
+            }
+        } else if (col < nc) {
+            // constraint row of the body
+            const int cr = r - nc, b = col / nf, q = col - b * nf;
+            const double* pq = phi + static_cast<size_t>(q) * len;
+            for (int i = 0; i < len; ++i) {
+                const int gi = cv.beg + i;
+                const double w = speed[gi] / dlen;
+                double t;
+                if (cr == 0)
+                    t = b == 0 ? -w : 0.0;
+                else if (cr == 1)
+                    t = b == 1 ? -w : 0.0;
+                else
+                    t = b == 0 ? -y[gi] * w : x[gi] * w;
+                s = __builtin_fma(t, pq[i], s);
+            }
+        } else {
+            s = r == col ? 1.0 : 0.0;
+        }
+        G[e] = s;
+    }
+    __syncthreads();
+
+    for (int k = 0; k < nq; ++k) {
+        if (tid < 64) {
+            double best = -1.0;
+            int bi = k;
+            for (int r = k + tid; r < nq; r += 64) {
+                const double v = fabs(G[r * nq + k]);
+                if (v > best) {
+                    best = v;
+                    bi = r;
+                }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ob = __shfl_xor(best, off);
+                const int oi = __shfl_xor(bi, off);
+                if (ob > best || (ob == best && oi < bi)) {
+                    best = ob;
+                    bi = oi;
+                }
+            }
+            if (tid == 0) perm[k] = bi;
+        }
+        __syncthreads();
+        const int p = perm[k];
+        if (p != k) {
+            for (int col = tid; col < nq; col += kBlk) {
+                const double t = G[k * nq + col];
+                G[k * nq + col] = G[p * nq + col];
+                G[p * nq + col] = t;
+            }
+        }
+        __syncthreads();
+        const double piv = G[k * nq + k];
+        for (int r = tid; r < nq; r += kBlk) fcol[r] = r == k ? 0.0 : G[r * nq + k];
+        __syncthreads();
+        for (int col = tid; col < nq; col += kBlk)
+            G[k * nq + col] = (col == k ? 1.0 : G[k * nq + col]) / piv;
+        __syncthreads();
+        for (int e = tid; e < nq * nq; e += kBlk) {
+            const int r = e / nq, col = e - r * nq;
+            if (r == k) continue;
+            G[e] = (col == k ? 0.0 : G[e]) - fcol[r] * G[k * nq + col];
+        }
+        __syncthreads();
+    }
+    for (int k = nq - 1; k >= 0; --k) {
+        const int p = perm[k];
+        if (p != k) {
+            for (int r = tid; r < nq; r += kBlk) {
+                const double t = G[r * nq + k];
+                G[r * nq + k] = G[r * nq + p];
+                G[r * nq + p] = t;
+            }
+        }
+        __syncthreads();
+    }
+    double* o = out + static_cast<size_t>(blockIdx.x) * nq * nq;
+    for (int e = tid; e < nq * nq; e += kBlk) o[e] = G[e];
+}
+
+// z = P^{-1} r, one block per curve (wall = block 0, body b = block b + 1).
+__global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
+    const double* __restrict__ phi_w, const double* __restrict__ phi_b, int nfw, int nfb,
+    int no, int n, int m, const double* __restrict__ Gw, const double* __restrict__ Gb,
+    const double* __restrict__ r, double* __restrict__ z) {
+    __shared__ double part[kApplyBlk];
+    __shared__ double cc[kMaxNQ], dd[kMaxNQ];
+    const int c = blockIdx.x;
+    const bool wall = c == 0;
+    const CurveView cv = curve_view(c, no, n, nfw, nfb, phi_w, phi_b);
+    const int nf = cv.nf, len = cv.len, nc = 2 * nf;
+    const int nq = wall ? nc : nc + 3;
+    const int xoff = wall ? 0 : 2 * no + 2 * (c - 1) * n;
+    const int yoff = wall ? no : xoff + n;
+    const int coff = 2 * no + 2 * n * m + 3 * (c - 1);
+    const double* G = wall ? Gw : Gb + static_cast<size_t>(c - 1) * nq * nq;
+    const int tid = threadIdx.x;
+
+    // coarse coefficients Q^T r: lane -> (coefficient tid % nc, slice tid / nc)
+    const int nparts = kApplyBlk / nc;
+    const int idx = tid % nc, pt = tid / nc;
+    double acc = 0.0;
+    if (pt < nparts) {
+        const int a = idx / nf, p = idx - a * nf;
+        const double* rv = r + (a ? yoff : xoff);
+        const double* pp = cv.phi + static_cast<size_t>(p) * len;
+        for (int i = pt; i < len; i += nparts) acc = __builtin_fma(pp[i], rv[i], acc);
+    }
+    part[tid] = acc;
+    __syncthreads();
+    if (tid < nc) {
+        double s = 0.0;
+        for (int q = 0; q < nparts; ++q) s += part[q * nc + tid];
+        cc[tid] = s;
+    } else if (!wall && tid < nq) {
+        cc[tid] = r[coff + tid - nc];
+    }
+    __syncthreads();
+    if (tid < nq) {
+        double d = 0.0;
+        const double* g = G + static_cast<size_t>(tid) * nq;
+        for (int k = 0; k < nq; ++k) d = __builtin_fma(g[k], cc[k], d);
+        if (tid < nc)
+            dd[tid] = d + 2.0 * cc[tid];
+        else
+            z[coff + tid - nc] = d;
+    }
+    __syncthreads();
+    for (int i = tid; i < len; i += kApplyBlk) {
+        double zx = -2.0 * r[xoff + i], zy = -2.0 * r[yoff + i];
+        for (int p = 0; p < nf; ++p) {
+            const double ph = cv.phi[static_cast<size_t>(p) * len + i];
+            zx = __builtin_fma(ph, dd[p], zx);
+            zy = __builtin_fma(ph, dd[nf + p], zy);
+        }
+        z[xoff + i] = zx;
+        z[yoff + i] = zy;
+    }
+}
+
+}  // namespace
+
+void precond_setup(Context& c) {
+    Geometry& g = c.g;
+    hipStream_t s = c.stream;
+    const int n = g.n, m = g.m, no = g.no;
+    const int kb = c.pc_body_modes, kw = c.pc_wall_modes;
+    if (kb < 1 || kb > kMaxModes || kw < 1 || kw > kMaxModes)
+        throw Error("preconditioner modes must lie in [1, 16]");
+    if (m > 0 && 2 * kb + 1 >= n)
+        throw Error("preconditioner body modes " + std::to_string(kb) + " need npts > " +
+                    std::to_string(2 * kb + 1));
+    if (2 * kw + 1 >= no)
+        throw Error("preconditioner wall modes " + std::to_string(kw) + " need nouter > " +
+                    std::to_string(2 * kw + 1));
+    const int nfb = 2 * kb + 1, nfw = 2 * kw + 1;
+    const int nqb = 2 * nfb + 3, nqw = 2 * nfw;
+    c.pc_phi_b.ensure(static_cast<size_t>(nfb) * std::max(n, 1));
+    c.pc_phi_w.ensure(static_cast<size_t>(nfw) * no);
+    c.pc_S.ensure(static_cast<size_t>(no) * 3 * nfw + static_cast<size_t>(m) * n * 3 * nfb);
+    k_pc_basis<<<cdiv_int(no, kBlk), kBlk, 0, s>>>(no, kw, c.pc_phi_w.p);
+    STOKES_LAUNCH_CHECK();
+    if (m > 0) {
+        k_pc_basis<<<cdiv_int(n, kBlk), kBlk, 0, s>>>(n, kb, c.pc_phi_b.p);
+        STOKES_LAUNCH_CHECK();
+    }
+    // the wall is fixed by its point count, so its block is kept between solves
+    if (c.pc_wall_no != no || c.pc_wall_kw != kw) {
+        c.pc_Gw.ensure(static_cast<size_t>(nqw) * nqw);
+        k_pc_pairs<<<dim3(1, cdiv_int(no, kBlk)), kBlk, 0, s>>>(
+            g.x.p, g.y.p, g.mx.p, g.my.p, c.pc_phi_w.p, c.pc_phi_b.p, nfw, nfb, no, n, 0,
+            c.pc_S.p);
+        STOKES_LAUNCH_CHECK();
+        k_pc_coarse<<<1, kBlk, 0, s>>>(g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p,
+                                       g.cx.p, g.cy.p, c.pc_phi_w.p, c.pc_phi_b.p, nfw, nfb,
+                                       no, n, 0, c.pc_S.p, c.pc_Gw.p);
+        STOKES_LAUNCH_CHECK();
+        c.pc_wall_no = no;
+        c.pc_wall_kw = kw;
+    }
+    if (m > 0) {
+        c.pc_Gb.ensure(static_cast<size_t>(m) * nqb * nqb);
+        k_pc_pairs<<<dim3(m, cdiv_int(n, kBlk)), kBlk, 0, s>>>(
+            g.x.p, g.y.p, g.mx.p, g.my.p, c.pc_phi_w.p, c.pc_phi_b.p, nfw, nfb, no, n, 1,
+            c.pc_S.p);
+        STOKES_LAUNCH_CHECK();
+        k_pc_coarse<<<m, kBlk, 0, s>>>(g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p,
+                                       g.cx.p, g.cy.p, c.pc_phi_w.p, c.pc_phi_b.p, nfw, nfb,
+                                       no, n, 1, c.pc_S.p, c.pc_Gb.p);
+        STOKES_LAUNCH_CHECK();
+    }
+    c.pc_nfb = nfb;
+    c.pc_nfw = nfw;
+}
+
+void precond_apply(Context& c, const double* r, double* z) {
+    Geometry& g = c.g;
+    if (c.time_kernels) c.t_precond.start(c.stream);
+    k_pc_apply<<<g.m + 1, kApplyBlk, 0, c.stream>>>(c.pc_phi_w.p, c.pc_phi_b.p, c.pc_nfw,
+                                                   c.pc_nfb, g.no, g.n, g.m, c.pc_Gw.p,
+                                                   c.pc_Gb.p, r, z);
+    STOKES_LAUNCH_CHECK();
+    // bytes: the vector read and written once
+    if (c.time_kernels) c.t_precond.stop(c.stream, 2.0 * 8.0 * static_cast<double>(g.nunk));
+}
+
+}  // namespace stokes

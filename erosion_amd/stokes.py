@@ -52,6 +52,12 @@ def lib():
         L.erosion_kernel_timing.restype = ctypes.c_int
         L.erosion_kernel_timers.argtypes = [_dp]
         L.erosion_kernel_timers.restype = ctypes.c_int
+        L.erosion_set_preconditioner.argtypes = [ctypes.c_int] * 3
+        L.erosion_set_preconditioner.restype = ctypes.c_int
+        L.erosion_get_preconditioner.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        L.erosion_get_preconditioner.restype = ctypes.c_int
+        L.erosion_precond_apply.argtypes = [ctypes.c_int64] * 3 + [_dp] * 4
+        L.erosion_precond_apply.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -100,7 +106,8 @@ def last_solve_info() -> dict:
     lib().erosion_last_solve_info(_p(info))
     return {"iters": int(info[0]), "implicit_relres": float(info[1]),
             "true_relres": float(info[2]), "seconds": float(info[3]), "reorth": int(info[4]),
-            "ff_targets_per_lane": int(info[5]), "ff_splits": int(info[6])}
+            "ff_targets_per_lane": int(info[5]), "ff_splits": int(info[6]),
+            "preconditioner": int(info[7])}
 
 
 def stokessolver(npts: int, nbods: int, nouter: int, ifmm: int, ibary: int, ibc: int, maxl: int,
@@ -139,7 +146,7 @@ def time_matvec(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: int = 1,
             "near_ms": out[4], "splits": int(out[5])}
 
 
-TIMER_FAMILIES = ("farfield", "near", "stress_eval", "gram_schmidt")
+TIMER_FAMILIES = ("farfield", "near", "stress_eval", "gram_schmidt", "precond")
 
 
 def kernel_timing(enable: bool) -> None:
@@ -150,7 +157,7 @@ def kernel_timing(enable: bool) -> None:
 
 def kernel_timers() -> dict:
     """{family: {"ms": total, "launches": n, "work": total work}} since enabling."""
-    out = np.zeros(12)
+    out = np.zeros(3 * len(TIMER_FAMILIES))
     if lib().erosion_kernel_timers(_p(out)) != 0:
         raise StokesError(f"erosion_kernel_timers: {last_error()}")
     return {f: {"ms": out[3 * i], "launches": int(out[3 * i + 1]), "work": out[3 * i + 2]}
@@ -175,3 +182,30 @@ def matvec(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: int = 1) -> n
     if lib().erosion_matvec(npts, nbods, nouter, ibary, _p(xx), _p(yy), _p(den), _p(vel)) != 0:
         raise StokesError(f"erosion_matvec: {last_error()}")
     return vel
+
+
+PRECOND_NONE = 0
+PRECOND_CURVE_FOURIER = 1
+
+
+def set_preconditioner(kind: int, body_modes: int = 8, wall_modes: int = 16) -> None:
+    """Select the right preconditioner of stokessolver (0 none, as the reference;
+    1 per-curve two-level Fourier).  Only the iteration count changes."""
+    if lib().erosion_set_preconditioner(int(kind), int(body_modes), int(wall_modes)) != 0:
+        raise StokesError(f"invalid preconditioner kind={kind} body_modes={body_modes} "
+                          f"wall_modes={wall_modes}")
+
+
+def get_preconditioner() -> Tuple[int, int, int]:
+    out = (ctypes.c_int * 3)()
+    lib().erosion_get_preconditioner(out)
+    return int(out[0]), int(out[1]), int(out[2])
+
+
+def precond_apply(npts: int, nbods: int, nouter: int, xx, yy, r) -> np.ndarray:
+    """z = P^{-1} r for the selected preconditioner on this geometry (test hook)."""
+    xx, yy, r = _f64(xx), _f64(yy), _f64(r)
+    z = np.zeros(nunk(npts, nbods, nouter))
+    if lib().erosion_precond_apply(npts, nbods, nouter, _p(xx), _p(yy), _p(r), _p(z)) != 0:
+        raise StokesError(f"erosion_precond_apply: {last_error()}")
+    return z

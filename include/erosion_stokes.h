@@ -93,8 +93,9 @@ const char* erosion_last_error(void);
 /* Record of the last stokessolver_ call:
  * info[0] iterations, info[1] implicit relative residual, info[2] true relative
  * residual ||b - A x|| / ||b|| of the returned density, info[3] seconds,
- * info[4] re-orthogonalised iterations, info[5] far-field targets per lane and
- * info[6] far-field source splits chosen by the autotuner.  Returns 0. */
+ * info[4] re-orthogonalised iterations, info[5] far-field targets per lane,
+ * info[6] far-field source splits chosen by the autotuner and info[7] the
+ * preconditioner kind.  `info` holds at least 8 doubles.  Returns 0. */
 int erosion_last_solve_info(double* info);
 
 /* One application of the completed operator vel = A den, as matvec_DLP_fmmbary
@@ -114,17 +115,39 @@ int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t iba
                         double* out);
 
 /* Measurement hook: enable (1) or stop (0) recording HIP events around the
- * far-field, near-pair, stress-evaluation and Gram-Schmidt launches of every
- * later call; enabling clears the totals.  Events are read back lazily, so
- * timing adds no host synchronisation.  Returns 0 or -1. */
+ * far-field, near-pair, stress-evaluation, Gram-Schmidt and preconditioner
+ * launches of every later call; enabling clears the totals.  Events are read
+ * back lazily, so timing adds no host synchronisation.  Returns 0 or -1. */
 int erosion_kernel_timing(int enable);
 
 /* Totals since timing was enabled, three values per family in the order
- * far field, near pairs, stress evaluation, Gram-Schmidt:
+ * far field, near pairs, stress evaluation, Gram-Schmidt, preconditioner:
  * out[3i] milliseconds, out[3i+1] launches, out[3i+2] algorithmic work
  * (ordered source-target pairs; near entries; target-node pairs; bytes of
- * Krylov basis streamed).  Returns 0 or -1. */
+ * Krylov basis streamed; bytes of vector read and written).  `out` holds 15
+ * doubles.  Returns 0 or -1. */
 int erosion_kernel_timers(double* out);
+
+/* Right preconditioner of stokessolver_ (no counterpart in the reference,
+ * whose solveBIE runs GMRES unpreconditioned, stokesSolver.f:490).
+ * kind 0: none (default, the reference's iteration).  kind 1: per-curve
+ * two-level preconditioner -- on each curve (wall, every body) the exact
+ * inverse of the curve's own diagonal operator block on its Fourier modes
+ * 0..K (K = wall_modes for the wall, body_modes for a body, each in [1, 16];
+ * a body's block includes its stokeslet/rotlet columns and constraint rows),
+ * and -2 (the inverse of the jump -1/2) on the remaining modes.  GMRES then
+ * iterates on A P^{-1}; its residual is still that of A x = b, so the stopping
+ * test and the returned density keep their meaning and only the iteration
+ * count changes.  A solve fails if npts <= 2 body_modes + 1 or
+ * nouter <= 2 wall_modes + 1.  Returns 0, or -1 on invalid arguments. */
+int erosion_set_preconditioner(int kind, int body_modes, int wall_modes);
+/* out[0] kind, out[1] body modes, out[2] wall modes.  Returns 0 or -1. */
+int erosion_get_preconditioner(int* out);
+
+/* Test hook: z = P^{-1} r for the selected preconditioner (kind must be 1) on
+ * the given geometry; r and z have the den layout.  Returns 0 or -1. */
+int erosion_precond_apply(int64_t npts, int64_t nbods, int64_t nouter, const double* xx,
+                          const double* yy, const double* r, double* z);
 
 #ifdef __cplusplus
 }
