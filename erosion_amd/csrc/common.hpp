@@ -108,6 +108,12 @@ struct Geometry {
     DevBuf<double> dtab_n, dtab_no;
     // host copies needed to build near lists
     std::vector<double> h_x, h_y, h_speed, h_cx, h_cy;
+    // build inputs and scratch
+    DevBuf<double> in_x, in_y, t1x, t1y, t2x, t2y;
+    // the boundary this geometry was built from; version counts builds
+    std::vector<double> key_x, key_y;
+    int key_n = -1, key_m = -1, key_no = -1;
+    long long version = 0;
 };
 
 // Durations of one kernel family, measured with HIP events recorded around
@@ -165,6 +171,8 @@ struct NearList {
     DevBuf<int> tgt_ptr, tgt_ent;    // CSR: entries of each target point
     DevBuf<int> active;              // active source curves
     DevBuf<double> corr_u, corr_v;   // per-entry correction (bary - trapezoid)
+    long long built_version = -1;    // geometry version and ibary of this list
+    int built_ibary = -1;
 };
 
 // Per-solve GMRES state kept on the device.
@@ -186,6 +194,8 @@ struct Context {
     DevBuf<double> bd;              // boundary values [5 fields][2][npts]
     DevBuf<double> fld;             // field densities [5][2][npts]
     DevBuf<double> scal;            // sigdotn etc.
+    DevBuf<double> stk_log;           // log |z_i - c_b|^2 [npts][m]
+    long long stk_log_version = -1;   // geometry version stk_log was built for
     int ff_splits = 0, ff_tpt = 0;
     std::map<int, int> ff_choice;      // boundary points -> far-field shape index
     std::map<int, float> ff_tuned_ms;  // boundary points -> tuned launch time

@@ -481,6 +481,24 @@ __global__ __launch_bounds__(kBlk) void k_reduce_small(
 }
 
 // ----------------------------------------------------------- assembly
+// lg[i m + b] = log |z_i - c_b|^2 for every boundary point i and body b: the
+// geometry-only part of the stokeslet, computed once per geometry.
+__global__ __launch_bounds__(kBlk) void k_stokes_logs(const double* __restrict__ x,
+                                                      const double* __restrict__ y,
+                                                      const double* __restrict__ cx,
+                                                      const double* __restrict__ cy, int npts,
+                                                      int m, double* __restrict__ lg) {
+    const size_t e = static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x;
+    if (e >= static_cast<size_t>(npts) * m) return;
+    const int i = static_cast<int>(e / m), b = static_cast<int>(e - static_cast<size_t>(i) * m);
+    const double rx = x[i] - cx[b], ry = y[i] - cy[b];
+    lg[e] = log(rx * rx + ry * ry);
+}
+
+// kAsmSub lanes per target: the stokeslet/rotlet sum over the bodies is
+// split across neighbouring lanes and reduced with cross-lane moves, so a
+// launch has kAsmSub npts / 64 waves to hide the latency of the logarithms.
+constexpr int kAsmSub = 16;
 __global__ __launch_bounds__(kBlk) void k_assemble(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ px,
     const double* __restrict__ py, const double* __restrict__ cur,
@@ -489,38 +507,16 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
     const double* __restrict__ den, const double* __restrict__ part_u,
     const double* __restrict__ part_v, int nsplit, const int* __restrict__ tgt_ptr,
     const int* __restrict__ tgt_ent, const double* __restrict__ corr_u,
-    const double* __restrict__ corr_v, int has_near, const double* __restrict__ scal, int no,
-    int n, int m, int npts, double* __restrict__ out) {
+    const double* __restrict__ corr_v, int has_near, const double* __restrict__ scal,
+    const double* __restrict__ lgtab, int no, int n, int m, int npts, double* __restrict__ out) {
     __shared__ double sc[5][kBlk];
-    const int i = blockIdx.x * kBlk + threadIdx.x;
+    const int sub = threadIdx.x & (kAsmSub - 1);
+    const int i = blockIdx.x * (kBlk / kAsmSub) + threadIdx.x / kAsmSub;
     const bool valid = i < npts;
     const int ii = valid ? i : 0;
-    double u = 0.0, v = 0.0;
-    if (valid) {
-        for (int s = 0; s < nsplit; ++s) {
-            u += part_u[static_cast<size_t>(s) * npts + i];
-            v += part_v[static_cast<size_t>(s) * npts + i];
-        }
-    }
     const double xi = x[ii], yi = y[ii];
-    // near-pair replacement
-    if (valid && has_near) {
-        for (int e = tgt_ptr[i]; e < tgt_ptr[i + 1]; ++e) {
-            u += corr_u[tgt_ent[e]];
-            v += corr_v[tgt_ent[e]];
-        }
-    }
-    // curvature diagonal and jump
-    {
-        const double nsrc = ii < no ? static_cast<double>(no) : static_cast<double>(n);
-        const double dx = ex[ii], dy = ey[ii];
-        const double tdotden = py[ii] * dx - px[ii] * dy;
-        u = u - cur[ii] * tdotden * py[ii] * speed[ii] / nsrc;
-        v = v + cur[ii] * tdotden * px[ii] * speed[ii] / nsrc;
-        u = u - 0.5 * dx;
-        v = v - 0.5 * dy;
-    }
-    // stokeslets and rotlets of every body
+    // stokeslets and rotlets of every body, bodies q = sub mod kAsmSub on this lane
+    double us = 0.0, vs = 0.0;
     const int base = 2 * no + 2 * n * m;
     for (int b0 = 0; b0 < m; b0 += kBlk) {
         __syncthreads();
@@ -534,18 +530,47 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
         }
         __syncthreads();
         const int cnt = min(kBlk, m - b0);
-        for (int q = 0; q < cnt; ++q) {
+        for (int q = sub; q < cnt; q += kAsmSub) {
             const double rx = xi - sc[0][q], ry = yi - sc[1][q];
             const double sto1 = sc[2][q], sto2 = sc[3][q], rot = sc[4][q];
             const double rho2 = rx * rx + ry * ry;
-            const double irho2 = 1.0 / rho2;
-            const double lg = log(rho2);
+            const double irho2 = rcp64(rho2);
+            const double lg = lgtab[static_cast<size_t>(ii) * m + b0 + q];
             const double rdots = rx * sto1 + ry * sto2;
-            u += 0.5 / kTwoPi * (-0.5 * lg * sto1 + rdots * irho2 * rx) + rot * ry * irho2;
-            v += 0.5 / kTwoPi * (-0.5 * lg * sto2 + rdots * irho2 * ry) - rot * rx * irho2;
+            us += 0.5 / kTwoPi * (-0.5 * lg * sto1 + rdots * irho2 * rx) + rot * ry * irho2;
+            vs += 0.5 / kTwoPi * (-0.5 * lg * sto2 + rdots * irho2 * ry) - rot * rx * irho2;
         }
     }
-    if (!valid) return;
+#pragma unroll
+    for (int off = 1; off < kAsmSub; off <<= 1) {
+        us += __shfl_xor(us, off);
+        vs += __shfl_xor(vs, off);
+    }
+    if (!valid || sub != 0) return;
+    double u = 0.0, v = 0.0;
+    for (int s = 0; s < nsplit; ++s) {
+        u += part_u[static_cast<size_t>(s) * npts + i];
+        v += part_v[static_cast<size_t>(s) * npts + i];
+    }
+    // near-pair replacement
+    if (has_near) {
+        for (int e = tgt_ptr[i]; e < tgt_ptr[i + 1]; ++e) {
+            u += corr_u[tgt_ent[e]];
+            v += corr_v[tgt_ent[e]];
+        }
+    }
+    // curvature diagonal and jump
+    {
+        const double nsrc = i < no ? static_cast<double>(no) : static_cast<double>(n);
+        const double dx = ex[i], dy = ey[i];
+        const double tdotden = py[i] * dx - px[i] * dy;
+        u = u - cur[i] * tdotden * py[i] * speed[i] / nsrc;
+        v = v + cur[i] * tdotden * px[i] * speed[i] / nsrc;
+        u = u - 0.5 * dx;
+        v = v - 0.5 * dy;
+    }
+    u += us;
+    v += vs;
     if (i < no) {
         const double sig = scal[0];
         u += sig * px[i];
@@ -692,10 +717,18 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary) {
     k_reduce_small<<<g.m + 1, kBlk, 0, s>>>(g.x.p, g.y.p, g.px.p, g.py.p, g.speed.p, c.ex.p,
                                             c.ey.p, x_dev, g.no, g.n, g.m, c.scal.p, y_dev);
     STOKES_LAUNCH_CHECK();
-    k_assemble<<<cdiv_int(np, kBlk), kBlk, 0, s>>>(
+    if (g.m > 0 && c.stk_log_version != g.version) {
+        c.stk_log.ensure(N * static_cast<size_t>(g.m));
+        const size_t tot = N * static_cast<size_t>(g.m);
+        k_stokes_logs<<<static_cast<int>((tot + kBlk - 1) / kBlk), kBlk, 0, s>>>(
+            g.x.p, g.y.p, g.cx.p, g.cy.p, np, g.m, c.stk_log.p);
+        STOKES_LAUNCH_CHECK();
+        c.stk_log_version = g.version;
+    }
+    k_assemble<<<cdiv_int(np, kBlk / kAsmSub), kBlk, 0, s>>>(
         g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p, c.ex.p, c.ey.p, g.cx.p, g.cy.p, x_dev,
         c.part_u.p, c.part_v.p, nsplit, L.tgt_ptr.p, L.tgt_ent.p, L.corr_u.p, L.corr_v.p,
-        has_near ? 1 : 0, c.scal.p, g.no, g.n, g.m, np, y_dev);
+        has_near ? 1 : 0, c.scal.p, g.m > 0 ? c.stk_log.p : nullptr, g.no, g.n, g.m, np, y_dev);
     STOKES_LAUNCH_CHECK();
 }
 

@@ -32,13 +32,20 @@ namespace stokes {
 namespace {
 
 constexpr int kBlk = 256;
-constexpr int kApplyBlk = 1024;
+constexpr int kApplyBlk = 256;
 constexpr int kQC = 17;                        // basis functions per pair pass
 constexpr int kMaxModes = 16;                  // K <= 16 per curve
 constexpr int kMaxNF = 2 * kMaxModes + 1;      // functions per component
 constexpr int kMaxNQ = 2 * kMaxNF + 3;         // coarse unknowns per curve
 
+constexpr int kCoarseBlk = 1024;
+
 inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
 
 // phi[f][i], f = 0 (constant), 2k-1 (cos k theta), 2k (sin k theta), orthonormal
 // over the N equispaced parameter points theta_i = 2 pi i / N.
@@ -157,9 +164,9 @@ __global__ __launch_bounds__(kBlk) void k_pc_pairs(
 
 // Coarse matrix G = Q^T B Q of one curve, inverted in place in LDS by
 // Gauss-Jordan elimination with partial (row) pivoting; writes G^{-1}
-// row-major to out + blockIdx.x nq^2.  Coarse unknown order: (x, f) for
+// column-major (transposed) to out + blockIdx.x nq^2.  Coarse unknown order: (x, f) for
 // f < nf, then (y, f), then a body's stokeslet x, y and rotlet.
-__global__ __launch_bounds__(kBlk) void k_pc_coarse(
+__global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ px,
     const double* __restrict__ py, const double* __restrict__ cur,
     const double* __restrict__ speed, const double* __restrict__ cx,
@@ -180,9 +187,11 @@ __global__ __launch_bounds__(kBlk) void k_pc_coarse(
     const double dlen = static_cast<double>(len);
     const double cxb = wall ? 0.0 : cx[c - 1], cyb = wall ? 0.0 : cy[c - 1];
     const double k4pi = 0.5 / kTwoPi;
-    const int tid = threadIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    constexpr int kCoarseWaves = kCoarseBlk / 64;
 
-    for (int e = tid; e < nq * nq; e += kBlk) {
+    // one wave per entry, lanes over the curve's points
+    for (int e = wv; e < nq * nq; e += kCoarseWaves) {
         const int r = e / nq, col = e - r * nq;
         double s = 0.0;
         if (r < nc && col < nc) {
@@ -190,7 +199,7 @@ __global__ __launch_bounds__(kBlk) void k_pc_coarse(
             const double* Srow = Sc + static_cast<size_t>((a + b) * nf + q) * len;
             const double* pp = phi + static_cast<size_t>(p) * len;
             const double* pq = phi + static_cast<size_t>(q) * len;
-            for (int i = 0; i < len; ++i) {
+            for (int i = lane; i < len; i += 64) {
                 const int gi = cv.beg + i;
                 const double f = cur[gi] * speed[gi] / dlen;
                 const double nx = px[gi], ny = py[gi];
@@ -206,19 +215,21 @@ __global__ __launch_bounds__(kBlk) void k_pc_coarse(
             if (wall) {
                 // rank-one completion n_i sum_j (n_j . eta_j) speed_j 2 pi / no
                 double s1 = 0.0, s2 = 0.0;
-                for (int i = 0; i < len; ++i) {
+                for (int i = lane; i < len; i += 64) {
                     const double na = a == 0 ? px[i] : py[i];
                     const double nb = b == 0 ? px[i] : py[i];
                     s1 = __builtin_fma(pp[i], na, s1);
                     s2 = __builtin_fma(speed[i] * kTwoPi / dlen * nb, pq[i], s2);
                 }
-                s = __builtin_fma(s1, s2, s);
+                s = __builtin_fma(wave_sum(s1), wave_sum(s2), wave_sum(s));
+            } else {
+                s = wave_sum(s);
             }
         } else if (r < nc) {
             // stokeslet / rotlet column of the body's own centre
             const int a = r / nf, p = r - a * nf, cc = col - nc;
             const double* pp = phi + static_cast<size_t>(p) * len;
-            for (int i = 0; i < len; ++i) {
+            for (int i = lane; i < len; i += 64) {
                 const int gi = cv.beg + i;
                 const double qx = x[gi] - cxb, qy = y[gi] - cyb;
                 const double r2 = __builtin_fma(qx, qx, qy * qy);
@@ -232,11 +243,12 @@ __global__ __launch_bounds__(kBlk) void k_pc_coarse(
                     t = a == 0 ? qy * ir2 : -qx * ir2;
                 s = __builtin_fma(pp[i], t, s);
             }
+            s = wave_sum(s);
         } else if (col < nc) {
             // constraint row of the body
             const int cr = r - nc, b = col / nf, q = col - b * nf;
             const double* pq = phi + static_cast<size_t>(q) * len;
-            for (int i = 0; i < len; ++i) {
+            for (int i = lane; i < len; i += 64) {
                 const int gi = cv.beg + i;
                 const double w = speed[gi] / dlen;
                 double t;
@@ -248,10 +260,11 @@ __global__ __launch_bounds__(kBlk) void k_pc_coarse(
                     t = b == 0 ? -y[gi] * w : x[gi] * w;
                 s = __builtin_fma(t, pq[i], s);
             }
+            s = wave_sum(s);
         } else {
             s = r == col ? 1.0 : 0.0;
         }
-        G[e] = s;
+        if (lane == 0) G[e] = s;
     }
     __syncthreads();
 
@@ -279,7 +292,7 @@ __global__ __launch_bounds__(kBlk) void k_pc_coarse(
         __syncthreads();
         const int p = perm[k];
         if (p != k) {
-            for (int col = tid; col < nq; col += kBlk) {
+            for (int col = tid; col < nq; col += kCoarseBlk) {
                 const double t = G[k * nq + col];
                 G[k * nq + col] = G[p * nq + col];
                 G[p * nq + col] = t;
@@ -287,12 +300,12 @@ __global__ __launch_bounds__(kBlk) void k_pc_coarse(
         }
         __syncthreads();
         const double piv = G[k * nq + k];
-        for (int r = tid; r < nq; r += kBlk) fcol[r] = r == k ? 0.0 : G[r * nq + k];
+        for (int r = tid; r < nq; r += kCoarseBlk) fcol[r] = r == k ? 0.0 : G[r * nq + k];
         __syncthreads();
-        for (int col = tid; col < nq; col += kBlk)
+        for (int col = tid; col < nq; col += kCoarseBlk)
             G[k * nq + col] = (col == k ? 1.0 : G[k * nq + col]) / piv;
         __syncthreads();
-        for (int e = tid; e < nq * nq; e += kBlk) {
+        for (int e = tid; e < nq * nq; e += kCoarseBlk) {
             const int r = e / nq, col = e - r * nq;
             if (r == k) continue;
             G[e] = (col == k ? 0.0 : G[e]) - fcol[r] * G[k * nq + col];
@@ -302,7 +315,7 @@ __global__ __launch_bounds__(kBlk) void k_pc_coarse(
     for (int k = nq - 1; k >= 0; --k) {
         const int p = perm[k];
         if (p != k) {
-            for (int r = tid; r < nq; r += kBlk) {
+            for (int r = tid; r < nq; r += kCoarseBlk) {
                 const double t = G[r * nq + k];
                 G[r * nq + k] = G[r * nq + p];
                 G[r * nq + p] = t;
@@ -311,51 +324,98 @@ __global__ __launch_bounds__(kBlk) void k_pc_coarse(
         __syncthreads();
     }
     double* o = out + static_cast<size_t>(blockIdx.x) * nq * nq;
-    for (int e = tid; e < nq * nq; e += kBlk) o[e] = G[e];
+    // transposed: k_pc_apply reads column k of row-lanes coalesced
+    for (int e = tid; e < nq * nq; e += kCoarseBlk) {
+        const int r = e / nq, col = e - r * nq;
+        o[static_cast<size_t>(col) * nq + r] = G[e];
+    }
 }
 
 // z = P^{-1} r, one block per curve (wall = block 0, body b = block b + 1).
+// Each lane walks its points once, generating the basis functions by the
+// rotation e^{i(k+1)theta} = e^{ik theta} e^{i theta} (no table loads), and
+// keeps the 2 nf coefficient partials in registers; the partials meet in LDS
+// and are summed in two conflict-free stages, kVals at a time.  The inverse
+// coarse matrices are stored transposed, so d = G^{-1} c reads them
+// coalesced with one row per lane.
 __global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
-    const double* __restrict__ phi_w, const double* __restrict__ phi_b, int nfw, int nfb,
-    int no, int n, int m, const double* __restrict__ Gw, const double* __restrict__ Gb,
-    const double* __restrict__ r, double* __restrict__ z) {
-    __shared__ double part[kApplyBlk];
+    int nfw, int nfb, int no, int n, int m, const double* __restrict__ Gw,
+    const double* __restrict__ Gb, const double* __restrict__ r, double* __restrict__ z) {
+    constexpr int kVals = 24;
+    constexpr int kSeg = 16;
+    constexpr int kMaxNC = 2 * kMaxNF;
+    __shared__ double part[kVals][kApplyBlk];
+    __shared__ double red[kVals][kSeg];
     __shared__ double cc[kMaxNQ], dd[kMaxNQ];
     const int c = blockIdx.x;
     const bool wall = c == 0;
-    const CurveView cv = curve_view(c, no, n, nfw, nfb, phi_w, phi_b);
-    const int nf = cv.nf, len = cv.len, nc = 2 * nf;
+    const int len = wall ? no : n;
+    const int nf = wall ? nfw : nfb, nc = 2 * nf, K = (nf - 1) / 2;
     const int nq = wall ? nc : nc + 3;
     const int xoff = wall ? 0 : 2 * no + 2 * (c - 1) * n;
     const int yoff = wall ? no : xoff + n;
     const int coff = 2 * no + 2 * n * m + 3 * (c - 1);
-    const double* G = wall ? Gw : Gb + static_cast<size_t>(c - 1) * nq * nq;
+    const double* Gt = wall ? Gw : Gb + static_cast<size_t>(c - 1) * nq * nq;
     const int tid = threadIdx.x;
+    const double c0 = 1.0 / sqrt(static_cast<double>(len));
+    const double c1 = sqrt(2.0 / static_cast<double>(len));
 
-    // coarse coefficients Q^T r: lane -> (coefficient tid % nc, slice tid / nc)
-    const int nparts = kApplyBlk / nc;
-    const int idx = tid % nc, pt = tid / nc;
-    double acc = 0.0;
-    if (pt < nparts) {
-        const int a = idx / nf, p = idx - a * nf;
-        const double* rv = r + (a ? yoff : xoff);
-        const double* pp = cv.phi + static_cast<size_t>(p) * len;
-        for (int i = pt; i < len; i += nparts) acc = __builtin_fma(pp[i], rv[i], acc);
+    // partials in fixed slots: x coefficients 0..kMaxNF-1, y coefficients after
+    double ax[kMaxNF], ay[kMaxNF];
+#pragma unroll
+    for (int v = 0; v < kMaxNF; ++v) ax[v] = ay[v] = 0.0;
+    for (int i = tid; i < len; i += kApplyBlk) {
+        const double rx = r[xoff + i], ry = r[yoff + i];
+        double s1, c1t;
+        sincospi(2.0 * static_cast<double>(i) / static_cast<double>(len), &s1, &c1t);
+        ax[0] = __builtin_fma(c0, rx, ax[0]);
+        ay[0] = __builtin_fma(c0, ry, ay[0]);
+        double ck = 1.0, sk = 0.0;
+#pragma unroll
+        for (int k = 1; k <= kMaxModes; ++k) {
+            // modes above K stay zero (their slots are never read)
+            const double cn = __builtin_fma(ck, c1t, -sk * s1);
+            sk = __builtin_fma(sk, c1t, ck * s1);
+            ck = cn;
+            const double pc = c1 * ck, ps = c1 * sk;
+            ax[2 * k - 1] = __builtin_fma(pc, rx, ax[2 * k - 1]);
+            ax[2 * k] = __builtin_fma(ps, rx, ax[2 * k]);
+            ay[2 * k - 1] = __builtin_fma(pc, ry, ay[2 * k - 1]);
+            ay[2 * k] = __builtin_fma(ps, ry, ay[2 * k]);
+        }
     }
-    part[tid] = acc;
-    __syncthreads();
-    if (tid < nc) {
-        double s = 0.0;
-        for (int q = 0; q < nparts; ++q) s += part[q * nc + tid];
-        cc[tid] = s;
-    } else if (!wall && tid < nq) {
-        cc[tid] = r[coff + tid - nc];
+#pragma unroll
+    for (int j = 0; j * kVals < kMaxNC; ++j) {
+#pragma unroll
+        for (int v = 0; v < kVals; ++v) {
+            const int slot = j * kVals + v;
+            if (slot < kMaxNC) part[v][tid] = slot < kMaxNF ? ax[slot] : ay[slot - kMaxNF];
+        }
+        __syncthreads();
+        for (int job = tid; job < kVals * kSeg; job += kApplyBlk) {
+            const int v = job / kSeg, sg = job - v * kSeg;
+            double s = 0.0;
+            for (int k = sg; k < kApplyBlk; k += kSeg) s += part[v][k];
+            red[v][sg] = s;
+        }
+        __syncthreads();
+        if (tid < kVals) {
+            const int slot = j * kVals + tid;
+            const int comp = slot / kMaxNF, p = slot - comp * kMaxNF;
+            if (slot < kMaxNC && p < nf) {
+                double s = 0.0;
+#pragma unroll
+                for (int sg = 0; sg < kSeg; ++sg) s += red[tid][sg];
+                cc[comp * nf + p] = s;
+            }
+        }
+        __syncthreads();
     }
+    if (!wall && tid < 3) cc[nc + tid] = r[coff + tid];
     __syncthreads();
     if (tid < nq) {
         double d = 0.0;
-        const double* g = G + static_cast<size_t>(tid) * nq;
-        for (int k = 0; k < nq; ++k) d = __builtin_fma(g[k], cc[k], d);
+        for (int k = 0; k < nq; ++k) d = __builtin_fma(Gt[static_cast<size_t>(k) * nq + tid], cc[k], d);
         if (tid < nc)
             dd[tid] = d + 2.0 * cc[tid];
         else
@@ -363,11 +423,18 @@ __global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
     }
     __syncthreads();
     for (int i = tid; i < len; i += kApplyBlk) {
-        double zx = -2.0 * r[xoff + i], zy = -2.0 * r[yoff + i];
-        for (int p = 0; p < nf; ++p) {
-            const double ph = cv.phi[static_cast<size_t>(p) * len + i];
-            zx = __builtin_fma(ph, dd[p], zx);
-            zy = __builtin_fma(ph, dd[nf + p], zy);
+        double s1, c1t;
+        sincospi(2.0 * static_cast<double>(i) / static_cast<double>(len), &s1, &c1t);
+        double zx = __builtin_fma(c0, dd[0], -2.0 * r[xoff + i]);
+        double zy = __builtin_fma(c0, dd[nf], -2.0 * r[yoff + i]);
+        double ck = 1.0, sk = 0.0;
+        for (int k = 1; k <= K; ++k) {
+            const double cn = __builtin_fma(ck, c1t, -sk * s1);
+            sk = __builtin_fma(sk, c1t, ck * s1);
+            ck = cn;
+            const double pc = c1 * ck, ps = c1 * sk;
+            zx = __builtin_fma(pc, dd[2 * k - 1], __builtin_fma(ps, dd[2 * k], zx));
+            zy = __builtin_fma(pc, dd[nf + 2 * k - 1], __builtin_fma(ps, dd[nf + 2 * k], zy));
         }
         z[xoff + i] = zx;
         z[yoff + i] = zy;
@@ -407,7 +474,7 @@ void precond_setup(Context& c) {
             g.x.p, g.y.p, g.mx.p, g.my.p, c.pc_phi_w.p, c.pc_phi_b.p, nfw, nfb, no, n, 0,
             c.pc_S.p);
         STOKES_LAUNCH_CHECK();
-        k_pc_coarse<<<1, kBlk, 0, s>>>(g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p,
+        k_pc_coarse<<<1, kCoarseBlk, 0, s>>>(g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p,
                                        g.cx.p, g.cy.p, c.pc_phi_w.p, c.pc_phi_b.p, nfw, nfb,
                                        no, n, 0, c.pc_S.p, c.pc_Gw.p);
         STOKES_LAUNCH_CHECK();
@@ -420,7 +487,7 @@ void precond_setup(Context& c) {
             g.x.p, g.y.p, g.mx.p, g.my.p, c.pc_phi_w.p, c.pc_phi_b.p, nfw, nfb, no, n, 1,
             c.pc_S.p);
         STOKES_LAUNCH_CHECK();
-        k_pc_coarse<<<m, kBlk, 0, s>>>(g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p,
+        k_pc_coarse<<<m, kCoarseBlk, 0, s>>>(g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p,
                                        g.cx.p, g.cy.p, c.pc_phi_w.p, c.pc_phi_b.p, nfw, nfb,
                                        no, n, 1, c.pc_S.p, c.pc_Gb.p);
         STOKES_LAUNCH_CHECK();
@@ -432,9 +499,8 @@ void precond_setup(Context& c) {
 void precond_apply(Context& c, const double* r, double* z) {
     Geometry& g = c.g;
     if (c.time_kernels) c.t_precond.start(c.stream);
-    k_pc_apply<<<g.m + 1, kApplyBlk, 0, c.stream>>>(c.pc_phi_w.p, c.pc_phi_b.p, c.pc_nfw,
-                                                   c.pc_nfb, g.no, g.n, g.m, c.pc_Gw.p,
-                                                   c.pc_Gb.p, r, z);
+    k_pc_apply<<<g.m + 1, kApplyBlk, 0, c.stream>>>(c.pc_nfw, c.pc_nfb, g.no, g.n, g.m,
+                                                   c.pc_Gw.p, c.pc_Gb.p, r, z);
     STOKES_LAUNCH_CHECK();
     // bytes: the vector read and written once
     if (c.time_kernels) c.t_precond.stop(c.stream, 2.0 * 8.0 * static_cast<double>(g.nunk));

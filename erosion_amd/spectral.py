@@ -29,30 +29,39 @@ def _ifftnice(fh: np.ndarray) -> np.ndarray:
 
 
 def specdiff(fx: np.ndarray, intvlen: float = 2 * np.pi) -> np.ndarray:
-    """Spectral derivative, Nyquist mode dropped (SpectralMethods.jl:35-41)."""
-    fh = _fftnice(np.asarray(fx, dtype=np.float64))
-    kv = kvec(len(fx), 0)
-    df = _ifftnice(2j * np.pi * kv * fh)
+    """Spectral derivative, Nyquist mode dropped (SpectralMethods.jl:35-41),
+    along the last axis."""
+    fx = np.asarray(fx, dtype=np.float64)
+    fh = np.fft.fftshift(np.fft.fft(fx, axis=-1), axes=-1)
+    kv = kvec(fx.shape[-1], 0)
+    df = np.fft.ifft(np.fft.ifftshift(2j * np.pi * kv * fh, axes=-1), axis=-1)
     return np.real(df) / intvlen
 
 
 def specint(fx: np.ndarray, intvlen: float = 2 * np.pi) -> np.ndarray:
-    """Mean-free spectral antiderivative (SpectralMethods.jl:45-54)."""
-    fh = _fftnice(np.asarray(fx, dtype=np.float64))
-    kv = kvec(len(fx), 0)
+    """Mean-free spectral antiderivative (SpectralMethods.jl:45-54), along the
+    last axis (a 2-D array is a batch of periodic functions)."""
+    fx = np.asarray(fx, dtype=np.float64)
+    fh = np.fft.fftshift(np.fft.fft(fx, axis=-1), axes=-1)
+    kv = kvec(fx.shape[-1], 0)
     Fh = np.zeros_like(fh)
     nz = kv != 0
-    Fh[nz] = fh[nz] / (2j * np.pi * kv[nz])
-    Fx = _ifftnice(Fh)
+    Fh[..., nz] = fh[..., nz] / (2j * np.pi * kv[nz])
+    Fx = np.fft.ifft(np.fft.ifftshift(Fh, axes=-1), axis=-1)
     return np.real(Fx) * intvlen
 
 
-def expsmooth(fx: np.ndarray, factor: float) -> np.ndarray:
-    """Multiply Fourier mode k by exp(-factor k^2) (SpectralMethods.jl:57-64)."""
-    fh = _fftnice(np.asarray(fx, dtype=np.float64))
-    kv = kvec(len(fx), 1)
-    fh = fh * np.exp(-factor * np.abs(kv) ** 2)
-    return np.real(_ifftnice(fh))
+def expsmooth(fx: np.ndarray, factor) -> np.ndarray:
+    """Multiply Fourier mode k by exp(-factor k^2) (SpectralMethods.jl:57-64),
+    along the last axis; ``factor`` may hold one value per row."""
+    fx = np.asarray(fx, dtype=np.float64)
+    fh = np.fft.fftshift(np.fft.fft(fx, axis=-1), axes=-1)
+    kv = kvec(fx.shape[-1], 1)
+    fac = np.asarray(factor, dtype=np.float64)
+    if fac.ndim:
+        fac = fac[..., None]
+    fh = fh * np.exp(-fac * np.abs(kv) ** 2)
+    return np.real(np.fft.ifft(np.fft.ifftshift(fh, axes=-1), axis=-1))
 
 
 def gaussfilter(fx: np.ndarray, sigma: float) -> np.ndarray:

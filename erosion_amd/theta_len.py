@@ -95,8 +95,26 @@ def getnxy(thlenden: ThLenDenType):
     nb = len(thlenden.thlenvec)
     if nb == 0:
         return 0, np.zeros(0), np.zeros(0)
-    xs, ys = zip(*(getxy(t) for t in thlenden.thlenvec))
-    return nb, np.concatenate(xs), np.concatenate(ys)
+    tv = thlenden.thlenvec
+    if len({len(t.theta) for t in tv}) != 1:
+        xs, ys = zip(*(getxy(t) for t in tv))
+        return nb, np.concatenate(xs), np.concatenate(ys)
+    # every body at once: the same arithmetic as getxy, batched over bodies
+    theta = np.stack([t.theta for t in tv])
+    ln = np.array([t.len for t in tv])
+    if not np.all(ln > 0.0):
+        raise ValueError("getxy: body length must be positive")
+    c, s = np.cos(theta), np.sin(theta)
+    mc, ms = np.mean(c, axis=1, keepdims=True), np.mean(s, axis=1, keepdims=True)
+    maxmean = np.maximum(np.abs(mc), np.abs(ms))[:, 0]
+    thresh = 20.0 / theta.shape[1]
+    if np.any(maxmean > thresh):
+        warnings.warn(f"theta means: {maxmean.max():.3g} > {thresh:.3g}")
+    dx = ln[:, None] / (2 * np.pi) * (c - mc)
+    dy = ln[:, None] / (2 * np.pi) * (s - ms)
+    x = specint(dx) + np.array([t.xsm for t in tv])[:, None]
+    y = specint(dy) + np.array([t.ysm for t in tv])[:, None]
+    return nb, x.ravel(), y.ravel()
 
 
 def circ2thlen(npts: int, rad: float, xc: float, yc: float) -> ThetaLenType:

@@ -57,10 +57,36 @@ def getderivs_body(thlen: ThetaLenType, params: ParamSet, atau: np.ndarray) -> D
     return DerivsType(mterm, nterm, xsmdot, ysmdot)
 
 
+def getderivs_bodies(thlenvec, params: ParamSet, atau: np.ndarray) -> List[DerivsType]:
+    """getderivs_body for every body at once (bodies with equal point counts):
+    the same arithmetic, batched along the body axis."""
+    theta = np.stack([t.theta for t in thlenvec])
+    ln = np.array([t.len for t in thlenvec])
+    matau = np.array([t.matau for t in thlenvec])[:, None]
+    at = np.asarray(atau, dtype=np.float64).T  # (bodies, points)
+    ataueff = at - float(params.fixarea) * matau
+    alpha = getalpha(theta.shape[1])
+    dtheta = specdiff(theta - alpha) + 1.0
+    vnorm = ataueff + params.epsilon * matau * (dtheta - 1.0)
+    dthvn = dtheta * vnorm
+    mdthvn = np.mean(dthvn, axis=1, keepdims=True)
+    mterm = -2 * math.pi * mdthvn[:, 0]
+    vtang = specint(dthvn - mdthvn)
+    datau = specdiff(at)
+    nterm = 2 * math.pi / ln[:, None] * (datau + dtheta * vtang)
+    xsmdot = np.mean(-vnorm * np.sin(theta) + vtang * np.cos(theta), axis=1)
+    ysmdot = np.mean(vnorm * np.cos(theta) + vtang * np.sin(theta), axis=1)
+    return [DerivsType(float(mterm[b]), nterm[b], float(xsmdot[b]), float(ysmdot[b]))
+            for b in range(len(thlenvec))]
+
+
 def getderivs(thlenden: ThLenDenType, params: ParamSet, den0=None, timings=None) -> List[DerivsType]:
     """Solve, stress, then per-body derivatives (TimeStepping.jl:60-69)."""
     atau = getstress_(thlenden, params, den0=den0, timings=timings)
-    return [getderivs_body(t, params, atau[:, b]) for b, t in enumerate(thlenden.thlenvec)]
+    tv = thlenden.thlenvec
+    if tv and len({len(t.theta) for t in tv}) == 1:
+        return getderivs_bodies(tv, params, atau)
+    return [getderivs_body(t, params, atau[:, b]) for b, t in enumerate(tv)]
 
 
 def delete_indices(thld0: ThLenDenType, dvec: List[DerivsType], dt: float) -> List[int]:
@@ -97,22 +123,26 @@ def timestep_(thld0: ThLenDenType, thld_derivs: ThLenDenType, dt1: float, dt2: f
     def zetafun(ln, matau):
         return 2 * math.pi / ln * matau
 
-    thlv1 = []
-    for bod in range(nbods):
-        t0 = thld0.thlenvec[bod]
-        td = thld_derivs.thlenvec[bod]
-        d = dvec[bod]
-        len1 = t0.len + dt1 * d.mterm
-        if not len1 > 0.0:
-            raise FloatingPointError("body length became non-positive")
-        zeta0 = zetafun(t0.len, t0.matau)
-        zetad = zetafun(td.len, td.matau)
-        fac1 = eps * dt1 * zetad
-        fac2 = eps * dt2 * (2 * zetad - zeta0)
-        th1 = expsmooth(t0.theta - alpha, fac1) + alpha
-        th1 = th1 + dt1 * expsmooth(d.nterm, fac2)
-        thlv1.append(ThetaLenType(th1, len1, t0.xsm + dt1 * d.xsmdot, t0.ysm + dt1 * d.ysmdot,
-                                  float("nan")))
+    if nbods == 0:
+        return new_thlenden([])
+    # every body at once; per body this is exactly the loop of TimeStepping.jl:127-152
+    t0v, tdv = thld0.thlenvec, thld_derivs.thlenvec
+    len0 = np.array([t.len for t in t0v])
+    mterm = np.array([d.mterm for d in dvec])
+    len1 = len0 + dt1 * mterm
+    if not np.all(len1 > 0.0):
+        raise FloatingPointError("body length became non-positive")
+    zeta0 = np.array([zetafun(t.len, t.matau) for t in t0v])
+    zetad = np.array([zetafun(t.len, t.matau) for t in tdv])
+    fac1 = eps * dt1 * zetad
+    fac2 = eps * dt2 * (2 * zetad - zeta0)
+    theta0 = np.stack([t.theta for t in t0v])
+    nterm = np.stack([d.nterm for d in dvec])
+    th1 = expsmooth(theta0 - alpha, fac1) + alpha
+    th1 = th1 + dt1 * expsmooth(nterm, fac2)
+    thlv1 = [ThetaLenType(th1[b], float(len1[b]), t0v[b].xsm + dt1 * dvec[b].xsmdot,
+                          t0v[b].ysm + dt1 * dvec[b].ysmdot, float("nan"))
+             for b in range(nbods)]
     return new_thlenden(thlv1)
 
 

@@ -1,0 +1,133 @@
+"""Host side of the erosion step (theta-L state, spectral routines, RK2).
+
+Mirrors ErosionPackage/src/SpectralMethods.jl, ThetaLen.jl and
+TimeStepping.jl.  Pinned by closed forms (trigonometric polynomials, circles)
+and by the per-body formulas the batched code must reproduce exactly.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from erosion_amd import time_stepping as TS
+from erosion_amd.geometry import make_bodies
+from erosion_amd.spectral import expsmooth, gaussfilter, specdiff, specint
+from erosion_amd.theta_len import (ParamSet, circ2thlen, getalpha, getnxy, getxy,
+                                   new_thlenden)
+
+
+def test_specdiff_trig_polynomial():
+    n = 64
+    x = 2 * np.pi * np.arange(n) / n
+    f = np.sin(3 * x) + 0.5 * np.cos(7 * x)
+    assert np.abs(specdiff(f) - (3 * np.cos(3 * x) - 3.5 * np.sin(7 * x))).max() < 1e-12
+    # interval length scales the derivative
+    assert np.abs(specdiff(f, 1.0) - 2 * np.pi * specdiff(f)).max() < 1e-11
+
+
+def test_specint_inverts_specdiff_on_mean_free():
+    n = 128
+    x = 2 * np.pi * np.arange(n) / n
+    f = np.cos(2 * x) - 0.3 * np.sin(5 * x)
+    F = specint(f)
+    assert abs(F.mean()) < 1e-13
+    assert np.abs(specdiff(F) - f).max() < 1e-12
+    assert np.abs(F - (np.sin(2 * x) / 2 + 0.3 * np.cos(5 * x) / 5)).max() < 1e-13
+
+
+def test_expsmooth_mode_factors():
+    n = 32
+    x = 2 * np.pi * np.arange(n) / n
+    f = np.cos(4 * x)
+    assert np.abs(expsmooth(f, 0.01) - np.exp(-0.16) * f).max() < 1e-13
+    assert np.abs(gaussfilter(f, 0.2) - np.exp(-0.5 * 0.04 * 16) * f).max() < 1e-13
+
+
+def test_batched_spectral_equals_rows():
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal((5, 48))
+    for fn in (specdiff, specint):
+        B = fn(A)
+        for r in range(5):
+            assert np.array_equal(B[r], fn(A[r]))
+    fac = rng.random(5) * 0.01
+    B = expsmooth(A, fac)
+    for r in range(5):
+        assert np.array_equal(B[r], expsmooth(A[r], fac[r]))
+
+
+def test_circle_state():
+    t = circ2thlen(128, 0.2, 0.3, -0.1)
+    x, y = getxy(t)
+    assert np.abs(np.hypot(x - 0.3, y + 0.1) - 0.2).max() < 1e-13
+    area = TS.body_areas(new_thlenden([t]))[0]
+    assert abs(area - np.pi * 0.04) < 1e-13
+
+
+def test_getnxy_batched_equals_per_body():
+    thl = make_bodies(7, 64, 0.05, 0.1, 0.05, 9)
+    nb, x, y = getnxy(new_thlenden(thl))
+    xs, ys = zip(*(getxy(t) for t in thl))
+    assert nb == 7
+    assert np.array_equal(x, np.concatenate(xs)) and np.array_equal(y, np.concatenate(ys))
+
+
+def test_getderivs_batched_equals_per_body():
+    thl = make_bodies(6, 64, 0.05, 0.1, 0.05, 4)
+    rng = np.random.default_rng(5)
+    for t in thl:
+        t.matau = 1.0 + rng.random()
+    params = ParamSet(npts=64, nouter=256)
+    atau = rng.random((64, 6))
+    d1 = [TS.getderivs_body(t, params, atau[:, b]) for b, t in enumerate(thl)]
+    d2 = TS.getderivs_bodies(thl, params, atau)
+    for a, b in zip(d1, d2):
+        assert a.mterm == b.mterm and a.xsmdot == b.xsmdot and a.ysmdot == b.ysmdot
+        assert np.array_equal(a.nterm, b.nterm)
+
+
+def test_rk2_step_matches_per_body_formulas(monkeypatch):
+    """One RK2 step with a fixed synthetic stress: the batched update equals
+    the per-body loop of TimeStepping.jl:127-152 evaluated here directly."""
+    thl = make_bodies(4, 64, 0.05, 0.1, 0.05, 12)
+    params = ParamSet(npts=64, nouter=256, dt=1e-3)
+    stress = {}
+
+    def fake_getstress(thlenden, params, den0=None, timings=None):
+        tv = thlenden.thlenvec
+        at = np.stack([0.5 + 0.1 * np.cos(t.theta) for t in tv]).T
+        for b, t in enumerate(tv):
+            t.matau = float(np.mean(np.abs(at[:, b])))
+        stress[id(thlenden)] = at
+        return at
+
+    monkeypatch.setattr(TS, "getstress_", fake_getstress)
+    th0 = new_thlenden(thl)
+    th1 = TS.rungekutta2(th0, params)
+
+    # per-body restatement of the two half steps
+    def half(t0s, tds, dt1, dt2):
+        at = np.stack([0.5 + 0.1 * np.cos(t.theta) for t in tds]).T
+        for b, t in enumerate(tds):
+            t.matau = float(np.mean(np.abs(at[:, b])))
+        out = []
+        alpha = getalpha(64)
+        for b, (t0, td) in enumerate(zip(t0s, tds)):
+            d = TS.getderivs_body(td, params, at[:, b])
+            z0 = 2 * math.pi / t0.len * t0.matau
+            zd = 2 * math.pi / td.len * td.matau
+            th = expsmooth(t0.theta - alpha, params.epsilon * dt1 * zd) + alpha
+            th = th + dt1 * expsmooth(d.nterm, params.epsilon * dt2 * (2 * zd - z0))
+            out.append((th, t0.len + dt1 * d.mterm, t0.xsm + dt1 * d.xsmdot,
+                        t0.ysm + dt1 * d.ysmdot))
+        return out
+
+    import copy
+    ref0 = copy.deepcopy(make_bodies(4, 64, 0.05, 0.1, 0.05, 12))
+    h = half(ref0, ref0, 0.5e-3, 0.5e-3)
+    mid = [TS.ThetaLenType(a, b, c, d, float("nan")) for a, b, c, d in h]
+    full = half(ref0, mid, 1e-3, 0.5e-3)
+    for t, (th, ln, xs, ys) in zip(th1.thlenvec, full):
+        assert np.abs(t.theta - th).max() < 1e-14
+        assert abs(t.len - ln) < 1e-14 and abs(t.xsm - xs) < 1e-15 and abs(t.ysm - ys) < 1e-15
+    assert th1.tt == pytest.approx(1e-3)
