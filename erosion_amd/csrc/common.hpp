@@ -180,8 +180,8 @@ struct GmresState {
     double bnrm, r0nrm, prod, rho, snormw, vnrm, inv_snormw;
     int k;          // iterations done
     int done;       // 1 converged, 2 stopped (maxl / breakdown)
-    int reorth;     // reorthogonalisation requested for this iteration
-    int nreorth;    // number of reorthogonalised iterations
+    int reorth;     // unused (every iteration runs two Gram-Schmidt passes)
+    int nreorth;    // iterations with a second Gram-Schmidt pass
 };
 
 struct Context {
@@ -205,7 +205,7 @@ struct Context {
     DevBuf<double> V;       // (maxl+1) x npad
     DevBuf<double> H;       // (maxl+1) x maxl, column-major
     DevBuf<double> q;       // Givens (c, s) pairs
-    DevBuf<double> hbuf, hpart, wpart, npart, ybuf;
+    DevBuf<double> hbuf, hpart, wpart, npart, ybuf, hcur, bpart;
     DevBuf<double> rhs, xvec, rvec;
     DevBuf<GmresState> st;
     GmresState* h_st = nullptr;  // pinned mirror

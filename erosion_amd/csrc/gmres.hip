@@ -35,7 +35,6 @@ constexpr int kChunkIters = 16;
 // tolerance, and the most cycles one solve may run
 constexpr double kPcTolFrac = 0.5;
 constexpr int kMaxCycles = 4;
-constexpr double kReorthEta = 0.70710678118654752440;
 
 inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
 inline size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
@@ -78,6 +77,12 @@ __global__ __launch_bounds__(kBlk) void k_add(double* __restrict__ x, const doub
     if (e < n) x[e] += y[e];
 }
 
+// zero the padding [n, npad) of rows [0, nrows) of a row-major (npad) array
+__global__ void k_zero_tails(double* __restrict__ v, int n, int npad) {
+    const int i = n + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < npad) v[static_cast<size_t>(blockIdx.y) * npad + i] = 0.0;
+}
+
 __global__ void k_zero_tail(double* __restrict__ v, int n, int npad) {
     int i = n + blockIdx.x * blockDim.x + threadIdx.x;
     if (i < npad) v[i] = 0.0;
@@ -112,16 +117,15 @@ __global__ void k_scale_row(double* __restrict__ v, size_t n, const GmresState* 
         v[e] *= s;
 }
 
-// Partial dots of rows [r0, r0+kRows) of V with w over one chunk, plus the
-// chunk's ||w||^2.  pass 1 runs unless done; pass 2 only when reorth is set.
+// Partial dots of rows [r0, r0+kRows) of V with w over one chunk:
+// hpart[r][chunk]; k_update sums them over the chunks.
 __global__ __launch_bounds__(kBlk) void k_dots(const double* __restrict__ V, size_t npad,
                                                const double* __restrict__ w, int nrows,
                                                size_t chunk, int nchunk,
-                                               const GmresState* __restrict__ st, int pass,
-                                               double* __restrict__ hpart,
-                                               double* __restrict__ npart) {
+                                               const GmresState* __restrict__ st,
+                                               double* __restrict__ hpart) {
     __shared__ double sh[kBlk];
-    if (st->done || (pass == 2 && !st->reorth)) return;
+    if (st->done) return;
     const int r0 = blockIdx.x * kRows;
     const int c = blockIdx.y;
     const size_t e0 = static_cast<size_t>(c) * chunk;
@@ -129,7 +133,6 @@ __global__ __launch_bounds__(kBlk) void k_dots(const double* __restrict__ V, siz
     double acc[kRows];
 #pragma unroll
     for (int r = 0; r < kRows; ++r) acc[r] = 0.0;
-    double nacc = 0.0;
     for (size_t e = e0 + 2 * threadIdx.x; e < e1; e += 2 * kBlk) {
         const double2 wv = *reinterpret_cast<const double2*>(w + e);
 #pragma unroll
@@ -140,84 +143,91 @@ __global__ __launch_bounds__(kBlk) void k_dots(const double* __restrict__ V, siz
                 acc[r] = __builtin_fma(vv.x, wv.x, __builtin_fma(vv.y, wv.y, acc[r]));
             }
         }
-        nacc = __builtin_fma(wv.x, wv.x, __builtin_fma(wv.y, wv.y, nacc));
     }
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
         double s = block_sum(acc[r], sh);
         if (threadIdx.x == 0 && r0 + r < nrows) hpart[static_cast<size_t>(r0 + r) * nchunk + c] = s;
     }
-    if (blockIdx.x == 0) {
-        double s = block_sum(nacc, sh);
-        if (threadIdx.x == 0) npart[c] = s;
-    }
 }
 
-// h = sum over chunks; pass 1 stores it as the Hessenberg column and sets
-// ||w_in||; pass 2 adds the correction.
-__global__ void k_dots_reduce(const double* __restrict__ hpart, int nrows, int nchunk,
-                              const double* __restrict__ npart, GmresState* __restrict__ st,
-                              int pass, double* __restrict__ hcur, double* __restrict__ hsum) {
-    if (st->done || (pass == 2 && !st->reorth)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nrows) {
-        double s = 0.0;
-        for (int c = 0; c < nchunk; ++c) s += hpart[static_cast<size_t>(i) * nchunk + c];
-        hcur[i] = s;
-        hsum[i] = pass == 1 ? s : hsum[i] + s;
-    }
-    if (pass == 1 && i == 0) {
-        double s = 0.0;
-        for (int c = 0; c < nchunk; ++c) s += npart[c];
-        st->vnrm = sqrt(s);
-    }
-}
-
-// wpart[split][e] = sum_{rows of split} h_r V[r][e]
+// wpart[split][e] = sum_{rows of split} h_r V[r][e].
+// Gram-Schmidt passes (mode 1, 2): h_r is the sum of the chunk partials of
+// k_dots, formed by every block for its own rows, 256 rows at a time in LDS;
+// the blocks of the first element range also store h (hcur) and add it to
+// the Hessenberg column (hsum: pass 1 stores, pass 2 adds).  Final solution
+// (mode 0): h is given.
 __global__ __launch_bounds__(kBlk) void k_update(const double* __restrict__ V, size_t npad,
-                                                 const double* __restrict__ h, int nrows,
-                                                 int rows_per_split,
+                                                 const double* __restrict__ h,
+                                                 const double* __restrict__ hpart, int nchunk,
+                                                 int nrows, int rows_per_split,
                                                  const GmresState* __restrict__ st, int mode,
+                                                 double* __restrict__ hcur,
+                                                 double* __restrict__ hsum,
                                                  double* __restrict__ wpart) {
-    // mode 1: pass 1, mode 2: reorth pass, mode 0: final solution (always)
-    if (mode != 0 && (st->done || (mode == 2 && !st->reorth))) return;
+    __shared__ double hs[kBlk];
+    if (mode != 0 && st->done) return;
     const size_t e = (static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x) * 2;
-    if (e >= npad) return;
+    const bool valid = e < npad;
     const int s = blockIdx.y;
     const int ra = s * rows_per_split, rb = min(nrows, ra + rows_per_split);
     double ax = 0.0, ay = 0.0;
-    int r = ra;
-    for (; r + 3 < rb; r += 4) {
-        const double h0 = h[r], h1 = h[r + 1], h2 = h[r + 2], h3 = h[r + 3];
-        const double2 v0 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r) * npad + e);
-        const double2 v1 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r + 1) * npad + e);
-        const double2 v2 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r + 2) * npad + e);
-        const double2 v3 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r + 3) * npad + e);
-        ax = __builtin_fma(h0, v0.x, ax);
-        ay = __builtin_fma(h0, v0.y, ay);
-        ax = __builtin_fma(h1, v1.x, ax);
-        ay = __builtin_fma(h1, v1.y, ay);
-        ax = __builtin_fma(h2, v2.x, ax);
-        ay = __builtin_fma(h2, v2.y, ay);
-        ax = __builtin_fma(h3, v3.x, ax);
-        ay = __builtin_fma(h3, v3.y, ay);
+    for (int t0 = ra; t0 < rb; t0 += kBlk) {
+        const int tn = min(kBlk, rb - t0);
+        __syncthreads();
+        if (static_cast<int>(threadIdx.x) < tn) {
+            const int r = t0 + threadIdx.x;
+            double v;
+            if (mode == 0) {
+                v = h[r];
+            } else {
+                v = 0.0;
+                for (int cc = 0; cc < nchunk; ++cc) v += hpart[static_cast<size_t>(r) * nchunk + cc];
+                if (blockIdx.x == 0) {
+                    hcur[r] = v;
+                    hsum[r] = mode == 1 ? v : hsum[r] + v;
+                }
+            }
+            hs[threadIdx.x] = v;
+        }
+        __syncthreads();
+        if (!valid) continue;
+        int q = 0;
+        for (; q + 3 < tn; q += 4) {
+            const int r = t0 + q;
+            const double h0 = hs[q], h1 = hs[q + 1], h2 = hs[q + 2], h3 = hs[q + 3];
+            const double2 v0 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r) * npad + e);
+            const double2 v1 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r + 1) * npad + e);
+            const double2 v2 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r + 2) * npad + e);
+            const double2 v3 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r + 3) * npad + e);
+            ax = __builtin_fma(h0, v0.x, ax);
+            ay = __builtin_fma(h0, v0.y, ay);
+            ax = __builtin_fma(h1, v1.x, ax);
+            ay = __builtin_fma(h1, v1.y, ay);
+            ax = __builtin_fma(h2, v2.x, ax);
+            ay = __builtin_fma(h2, v2.y, ay);
+            ax = __builtin_fma(h3, v3.x, ax);
+            ay = __builtin_fma(h3, v3.y, ay);
+        }
+        for (; q < tn; ++q) {
+            const double hr = hs[q];
+            const double2 v0 =
+                *reinterpret_cast<const double2*>(V + static_cast<size_t>(t0 + q) * npad + e);
+            ax = __builtin_fma(hr, v0.x, ax);
+            ay = __builtin_fma(hr, v0.y, ay);
+        }
     }
-    for (; r < rb; ++r) {
-        const double hr = h[r];
-        const double2 v0 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r) * npad + e);
-        ax = __builtin_fma(hr, v0.x, ax);
-        ay = __builtin_fma(hr, v0.y, ay);
-    }
-    *reinterpret_cast<double2*>(wpart + static_cast<size_t>(s) * npad + e) = make_double2(ax, ay);
+    if (valid)
+        *reinterpret_cast<double2*>(wpart + static_cast<size_t>(s) * npad + e) = make_double2(ax, ay);
 }
 
-// w += sign * sum_s wpart[s]; per-block ||w||^2
+// w += sign * sum_s wpart[s]; except after the first pass also the per-block ||w||^2
 __global__ __launch_bounds__(kBlk) void k_combine(double* __restrict__ w, size_t npad,
                                                   const double* __restrict__ wpart, int nsplit,
                                                   double sign, const GmresState* __restrict__ st,
                                                   int mode, double* __restrict__ npart) {
     __shared__ double sh[kBlk];
-    if (mode != 0 && (st->done || (mode == 2 && !st->reorth))) return;
+    if (mode != 0 && st->done) return;
     const size_t e = (static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x) * 2;
     double acc = 0.0;
     if (e < npad) {
@@ -233,22 +243,11 @@ __global__ __launch_bounds__(kBlk) void k_combine(double* __restrict__ w, size_t
         *reinterpret_cast<double2*>(w + e) = wv;
         acc = wv.x * wv.x + wv.y * wv.y;
     }
+    if (mode == 1) return;  // block-uniform
     double s = block_sum(acc, sh);
     if (threadIdx.x == 0) npart[blockIdx.x] = s;
 }
 
-// ||w|| after a Gram-Schmidt pass; after pass 1 decide on a second pass.
-__global__ void k_norm_decide(const double* __restrict__ npart, int nb, GmresState* __restrict__ st,
-                              int pass, double eta) {
-    if (threadIdx.x != 0 || st->done || (pass == 2 && !st->reorth)) return;
-    double s = 0.0;
-    for (int i = 0; i < nb; ++i) s += npart[i];
-    st->snormw = sqrt(s);
-    if (pass == 1) {
-        st->reorth = (st->snormw < eta * st->vnrm) ? 1 : 0;
-        st->nreorth += st->reorth;
-    }
-}
 
 // Hessenberg column k, Givens rotations (DHEQR, IJOB>1), stop test.
 // The k stored rotations act on (col[i], col[i+1]) in order; the value
@@ -258,6 +257,7 @@ constexpr int kGivT = 1024;
 __global__ __launch_bounds__(kGivT) void k_givens(double* __restrict__ H, int ldh,
                                                   double* __restrict__ q,
                                                   const double* __restrict__ hsum,
+                                                  const double* __restrict__ npart, int nb,
                                                   GmresState* __restrict__ st, double tol,
                                                   int maxl) {
     __shared__ double sA[kGivT], sB[kGivT];
@@ -265,6 +265,21 @@ __global__ __launch_bounds__(kGivT) void k_givens(double* __restrict__ H, int ld
     const int k = st->k;
     double* __restrict__ col = H + static_cast<size_t>(k) * ldh;
     const int tid = threadIdx.x;
+    // ||w|| after the second Gram-Schmidt pass from the per-block partials
+    {
+        double p = 0.0;
+        for (int i = tid; i < nb; i += kGivT) p += npart[i];
+        sA[tid] = p;
+        __syncthreads();
+        for (int off = kGivT / 2; off > 0; off >>= 1) {
+            if (tid < off) sA[tid] += sA[tid + off];
+            __syncthreads();
+        }
+        if (tid == 0) sB[0] = sqrt(sA[0]);
+        __syncthreads();
+    }
+    const double snormw = sB[0];
+    __syncthreads();
     // segment of rotations for this thread
     const int L = (k + kGivT - 1) / kGivT;
     const int i0 = min(k, tid * L), i1 = min(k, i0 + L);
@@ -304,7 +319,9 @@ __global__ __launch_bounds__(kGivT) void k_givens(double* __restrict__ H, int ld
     // the thread owning the last rotation (or thread 0 when k == 0) finishes
     const bool last = (k == 0) ? (tid == 0) : (i0 < i1 && i1 == k);
     if (!last) return;
-    const double t1 = carry, t2 = st->snormw;
+    const double t1 = carry, t2 = snormw;
+    st->snormw = snormw;
+    st->nreorth += 1;
     double c, s;
     if (t2 == 0.0) {
         c = 1.0;
@@ -381,27 +398,24 @@ Plan dots_plan(int nrows, size_t npad) {
 
 }  // namespace
 
-// One Gram-Schmidt pass over rows [0, nrows) of V against w.
+// One classical Gram-Schmidt pass over rows [0, nrows) of V against w; every
+// iteration runs two (CGS2), the second removing what rounding left of the
+// first, and the second leaves the per-block partial norms of w in npart.
 static void gs_pass(Context& c, double* w, int nrows, int pass, double* hcur) {
     const size_t npad = static_cast<size_t>(c.npad);
     hipStream_t s = c.stream;
     Plan p = dots_plan(nrows, npad);
     k_dots<<<dim3(p.bx, p.nchunk), kBlk, 0, s>>>(c.V.p, npad, w, nrows, p.chunk, p.nchunk, c.st.p,
-                                                 pass, c.hpart.p, c.npart.p);
-    STOKES_LAUNCH_CHECK();
-    k_dots_reduce<<<cdiv_int(nrows, kBlk), kBlk, 0, s>>>(c.hpart.p, nrows, p.nchunk, c.npart.p,
-                                                         c.st.p, pass, hcur, c.hbuf.p);
+                                                 c.hpart.p);
     STOKES_LAUNCH_CHECK();
     const int ebx = static_cast<int>(npad / (2 * kBlk));
     int nsplit = std::min(nrows, std::max(1, cdiv_int(1024, ebx)));
     int rps = cdiv_int(nrows, nsplit);
     nsplit = cdiv_int(nrows, rps);
-    k_update<<<dim3(ebx, nsplit), kBlk, 0, s>>>(c.V.p, npad, hcur, nrows, rps, c.st.p, pass,
-                                                c.wpart.p);
+    k_update<<<dim3(ebx, nsplit), kBlk, 0, s>>>(c.V.p, npad, nullptr, c.hpart.p, p.nchunk, nrows,
+                                                rps, c.st.p, pass, hcur, c.hbuf.p, c.wpart.p);
     STOKES_LAUNCH_CHECK();
     k_combine<<<ebx, kBlk, 0, s>>>(w, npad, c.wpart.p, nsplit, -1.0, c.st.p, pass, c.npart.p);
-    STOKES_LAUNCH_CHECK();
-    k_norm_decide<<<1, 64, 0, s>>>(c.npart.p, ebx, c.st.p, pass, kReorthEta);
     STOKES_LAUNCH_CHECK();
 }
 
@@ -431,10 +445,16 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
     c.xvec.ensure(npad);
     c.rvec.ensure(npad);
     c.st.ensure(1);
+    // the operator writes rows [0, N): zero the padding of every Krylov row once
+    if (npad > static_cast<size_t>(N)) {
+        k_zero_tails<<<dim3(cdiv_int(static_cast<int>(npad) - N, kBlk), maxl + 1), kBlk, 0, s>>>(
+            c.V.p, N, static_cast<int>(npad));
+        STOKES_LAUNCH_CHECK();
+    }
     if (!c.h_st) STOKES_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.h_st), sizeof(GmresState)));
-    DevBuf<double> hcur;
+    DevBuf<double>& hcur = c.hcur;
     hcur.ensure(static_cast<size_t>(maxl + 1 + kRows));
-    DevBuf<double> bpart;
+    DevBuf<double>& bpart = c.bpart;
     bpart.ensure(1024);
 
     // rhs and initial guess (Julia passes zeros, DensityStress.jl:45)
@@ -496,18 +516,15 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
                 } else {
                     apply_operator(c, vk, w, ibary);
                 }
-                k_zero_tail<<<cdiv_int(static_cast<int>(npad) - N + 1, kBlk), kBlk, 0, s>>>(
-                    w, N, static_cast<int>(npad));
-                STOKES_LAUNCH_CHECK();
                 if (c.time_kernels) c.t_gs.start(s);
                 gs_pass(c, w, kk + 1, 1, hcur.p);
                 gs_pass(c, w, kk + 1, 2, hcur.p);
-                k_givens<<<1, kGivT, 0, s>>>(c.H.p, ldh, c.q.p, c.hbuf.p, c.st.p, tol_cycle,
-                                             maxl_cycle);
+                k_givens<<<1, kGivT, 0, s>>>(c.H.p, ldh, c.q.p, c.hbuf.p, c.npart.p, ebx,
+                                             c.st.p, tol_cycle, maxl_cycle);
                 STOKES_LAUNCH_CHECK();
                 if (c.time_kernels) {
-                    // Krylov rows streamed by the dot and update kernels of pass 1
-                    c.t_gs.stop(s, 2.0 * static_cast<double>(kk + 1) * static_cast<double>(npad) * 8.0);
+                    // Krylov rows streamed by the dot and update kernels of both passes
+                    c.t_gs.stop(s, 4.0 * static_cast<double>(kk + 1) * static_cast<double>(npad) * 8.0);
                 }
                 k_scale_row<<<std::min(1024, ebx * 2), kBlk, 0, s>>>(w, npad, c.st.p, 0);
                 STOKES_LAUNCH_CHECK();
@@ -529,8 +546,9 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
             int nsplit = std::min(iters, std::max(1, cdiv_int(1024, ebx)));
             int rps = cdiv_int(iters, nsplit);
             nsplit = cdiv_int(iters, rps);
-            k_update<<<dim3(ebx, nsplit), kBlk, 0, s>>>(c.V.p, npad, c.ybuf.p, iters, rps, c.st.p,
-                                                        0, c.wpart.p);
+            k_update<<<dim3(ebx, nsplit), kBlk, 0, s>>>(c.V.p, npad, c.ybuf.p, nullptr, 0, iters,
+                                                        rps, c.st.p, 0, nullptr, nullptr,
+                                                        c.wpart.p);
             STOKES_LAUNCH_CHECK();
             if (pc) {
                 // u = V y, then x += P^{-1} u
