@@ -144,9 +144,13 @@ def main() -> int:
     thld0 = workload_geometry(w)
     _m, xx0, yy0 = getnxy(thld0)
 
+    def configure(accelerated):
+        stokes.set_preconditioner(1 if accelerated else 0)
+        stokes.set_precision(stokes.PRECISION_MIXED if accelerated else stokes.PRECISION_FP64)
+
     def run_variant(label, accelerated):
         """Warm-up steps, then exactly args.steps timed steps from the initial state."""
-        stokes.set_preconditioner(1 if accelerated else 0)
+        configure(accelerated)
         guess = PreviousDensity() if accelerated else None
         th = copy.deepcopy(thld0)
         for i in range(args.warmup):
@@ -167,7 +171,7 @@ def main() -> int:
         elapsed = time.perf_counter() - t0
         kt = stokes.kernel_timers()
         stokes.kernel_timing(False)
-        stokes.set_preconditioner(0)
+        configure(False)
         log(f"[{label}] host-side timings: " +
             ", ".join(f"{k} {v:.3f}" if isinstance(v, float) else f"{k} {v}"
                       for k, v in timings.items()))
@@ -176,9 +180,9 @@ def main() -> int:
 
     def first_stress(accelerated):
         """Shear stress of the initial state through one variant's solve."""
-        stokes.set_preconditioner(1 if accelerated else 0)
+        configure(accelerated)
         atau = getstress_(copy.deepcopy(thld0), params)
-        stokes.set_preconditioner(0)
+        configure(False)
         return np.asarray(atau)
 
     stress_err = theta_err = None
@@ -199,6 +203,7 @@ def main() -> int:
                 "ms_per_step": 1e3 * r["elapsed"] / args.steps,
                 "gmres_iters_per_solve": r["timings"].get("gmres_iters", 0) / solves,
                 "true_relres_max": max(si["true_relres"] for si in r["solve_info"]),
+                "fp32_iters_last_solve": r["solve_info"][-1]["fp32_iters"],
                 "matvecs_per_s": r["kt"]["farfield"]["launches"] / r["elapsed"]}
 
     sp, sa = (summary(plain) if plain else None), summary(acc)
@@ -209,6 +214,10 @@ def main() -> int:
     ff_ms = ff["ms"] / max(1, ff["launches"])
     ff_pairs = ff["work"] / max(1, ff["launches"])
     ff_tflops = FF_FLOP_PER_PAIR * ff_pairs / (ff_ms * 1e-3) / 1e12
+    f32 = kt["farfield32"]
+    f32_ms = f32["ms"] / f32["launches"] if f32["launches"] else None
+    f32_tflops = (FF_FLOP_PER_PAIR * f32["work"] / f32["launches"] / (f32_ms * 1e-3) / 1e12
+                  if f32["launches"] else None)
     gs = kt["gram_schmidt"]
     gs_gbs = gs["work"] / (gs["ms"] * 1e-3) / 1e9 if gs["ms"] > 0 else 0.0
     se = kt["stress_eval"]
@@ -217,6 +226,9 @@ def main() -> int:
     log(f"accelerated: {sa}")
     log(f"kernel time shares of the accelerated timed region: " +
         ", ".join(f"{k} {100 * v:.1f}%" for k, v in share.items()))
+    if f32_ms:
+        log(f"FP32 far field: {f32_ms:.3f} ms/launch, {f32_tflops:.2f} TFLOP/s over "
+            f"{f32['launches']} launches")
     log(f"far field: {ff_ms:.3f} ms/launch, {ff_tflops:.2f} TFLOP/s; Gram-Schmidt {gs_gbs:.0f} GB/s; "
         f"stress eval {se['ms'] / max(1, se['launches']):.2f} ms/launch")
 
@@ -243,8 +255,9 @@ def main() -> int:
         "config": {
             "workload": w.name, "bodies": w.nbodies, "points_per_body": w.npts,
             "wall_points": w.nouter, "gmres_tol": args.tol,
-            "variant": "accelerated: per-curve two-level right preconditioner + previous-density "
-                       "initial guess, FP64",
+            "variant": "accelerated: per-curve two-level right preconditioner, FP32 far field "
+                       "once the residual is below 1e4 tol (FP64 true residual), previous-density "
+                       "initial guess",
             "dt": w.dt, "fixpdrop": True, "ibc": w.ibc,
             "accelerated": sa,
             "plain_fp64": sp,
@@ -253,6 +266,9 @@ def main() -> int:
             "theta_maxdiff_after_steps": theta_err,
             "matvecs_per_s": sa["matvecs_per_s"],
             "farfield_ms": ff_ms,
+            "farfield32_ms": f32_ms,
+            "farfield32_TFLOPs": f32_tflops,
+            "farfield32_launches": f32["launches"],
             "gram_schmidt_GBps": gs_gbs,
             "kernel_share": share,
         },

@@ -210,6 +210,7 @@ int erosion_last_solve_info(double* info) {
         info[5] = static_cast<double>(c.ff_tpt);
         info[6] = static_cast<double>(c.ff_splits);
         info[7] = static_cast<double>(c.pc_kind);
+        info[8] = static_cast<double>(c.last_fp32_iters);
         return 0;
     } catch (const std::exception& e) {
         return record_error("erosion_last_solve_info", e.what());
@@ -233,7 +234,9 @@ int erosion_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t ibary,
         c.rvec.ensure(std::max(nunk, c.rvec.cap));
         STOKES_HIP(hipMemcpyAsync(c.xvec.p, den, nunk * sizeof(double), hipMemcpyHostToDevice,
                                   c.stream));
+        c.ff_fp32_now = c.precision == 2;
         stokes::apply_operator(c, c.xvec.p, c.rvec.p, static_cast<int>(ibary));
+        c.ff_fp32_now = false;
         STOKES_HIP(hipMemcpyAsync(vel, c.rvec.p, nunk * sizeof(double), hipMemcpyDeviceToHost,
                                   c.stream));
         STOKES_HIP(hipStreamSynchronize(c.stream));
@@ -261,6 +264,7 @@ int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t iba
         c.rvec.ensure(std::max(nunk, c.rvec.cap));
         STOKES_HIP(hipMemcpyAsync(c.xvec.p, den, nunk * sizeof(double), hipMemcpyHostToDevice,
                                   c.stream));
+        c.ff_fp32_now = c.precision == 2;
         stokes::apply_operator(c, c.xvec.p, c.rvec.p, static_cast<int>(ibary));  // warm-up
         STOKES_HIP(hipStreamSynchronize(c.stream));
         // whole operator applications, back to back
@@ -278,16 +282,18 @@ int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t iba
         STOKES_HIP(hipEventDestroy(e1));
         // per-kernel durations
         const bool was = c.time_kernels;
-        c.t_farfield.reset();
+        stokes::KernelTimer& tff = c.ff_fp32_now ? c.t_farfield32 : c.t_farfield;
+        tff.reset();
         c.t_near.reset();
         c.time_kernels = true;
         for (int64_t r = 0; r < reps; ++r)
             stokes::apply_operator(c, c.xvec.p, c.rvec.p, static_cast<int>(ibary));
-        c.t_farfield.harvest();
+        tff.harvest();
         c.t_near.harvest();
         c.time_kernels = was;
+        c.ff_fp32_now = false;
         out[0] = static_cast<double>(tot) / static_cast<double>(reps);
-        out[1] = c.t_farfield.ms / static_cast<double>(std::max<long long>(1, c.t_farfield.launches));
+        out[1] = tff.ms / static_cast<double>(std::max<long long>(1, tff.launches));
         const double np = static_cast<double>(c.g.npts);
         out[2] = np * (np - 1.0);
         out[3] = static_cast<double>(c.near.nentries);
@@ -297,7 +303,26 @@ int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t iba
     } catch (const std::exception& e) {
         Context& c = stokes::ctx();
         c.time_kernels = false;
+        c.ff_fp32_now = false;
         return record_error("erosion_time_matvec", e.what());
+    }
+}
+
+int erosion_set_precision(int mode) {
+    try {
+        if (mode < 0 || mode > 2) return -1;
+        stokes::ctx().precision = mode;
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_set_precision", e.what());
+    }
+}
+
+int erosion_get_precision(void) {
+    try {
+        return stokes::ctx().precision;
+    } catch (...) {
+        return -1;
     }
 }
 
@@ -363,12 +388,12 @@ int erosion_kernel_timing(int enable) {
     try {
         Context& c = stokes::ctx();
         if (enable) {
-            for (stokes::KernelTimer* t :
-                 {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs, &c.t_precond})
+            for (stokes::KernelTimer* t : {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs,
+                                           &c.t_precond, &c.t_farfield32})
                 t->reset();
         } else {
-            for (stokes::KernelTimer* t :
-                 {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs, &c.t_precond})
+            for (stokes::KernelTimer* t : {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs,
+                                           &c.t_precond, &c.t_farfield32})
                 t->harvest();
         }
         c.time_kernels = enable != 0;
@@ -382,9 +407,9 @@ int erosion_kernel_timers(double* out) {
     if (!out) return -1;
     try {
         Context& c = stokes::ctx();
-        stokes::KernelTimer* ts[5] = {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs,
-                                      &c.t_precond};
-        for (int i = 0; i < 5; ++i) {
+        stokes::KernelTimer* ts[6] = {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs,
+                                      &c.t_precond, &c.t_farfield32};
+        for (int i = 0; i < 6; ++i) {
             ts[i]->harvest();
             out[3 * i + 0] = ts[i]->ms;
             out[3 * i + 1] = static_cast<double>(ts[i]->launches);

@@ -199,6 +199,17 @@ struct Context {
     int ff_splits = 0, ff_tpt = 0;
     std::map<int, int> ff_choice;      // boundary points -> far-field shape index
     std::map<int, float> ff_tuned_ms;  // boundary points -> tuned launch time
+    std::map<int, int> ff32_choice;    // the same for the FP32 far field
+    std::map<int, float> ff32_tuned_ms;
+    // FP32 far field: per 256-point tile the FP64 centre, per source the FP32
+    // offset from it and the FP32 weighted normal; built once per geometry
+    DevBuf<double> ff32_ctr;
+    DevBuf<float> ff32_xy, ff32_m;
+    long long ff32_version = -1;
+    // precision of the far field: 0 FP64; 1 mixed (GMRES switches the far
+    // field to FP32 once its residual is small, see gmres.hip); 2 FP32 always
+    int precision = 0;
+    bool ff_fp32_now = false;  // the far field of the next operator applications
     // GMRES buffers
     int npad = 0;
     int maxl_alloc = 0;
@@ -223,13 +234,14 @@ struct Context {
     DevBuf<double> pc_phi_b, pc_phi_w, pc_S, pc_Gb, pc_Gw, pc_z;
     // per-kernel timing with HIP events on `stream` (measurement hooks only)
     bool time_kernels = false;
-    KernelTimer t_farfield, t_near, t_stress_eval, t_gs, t_precond;
+    KernelTimer t_farfield, t_near, t_stress_eval, t_gs, t_precond, t_farfield32;
     int status = 0;
     std::string last_error;
     // last solve record
     long long last_iters = 0;
     double last_true_relres = -1.0, last_implicit_relres = -1.0, last_seconds = 0.0;
     int last_reorth = 0;
+    long long last_fp32_iters = 0;  // iterations whose far field ran in FP32
 };
 
 Context& ctx();

@@ -35,6 +35,14 @@ constexpr int kChunkIters = 16;
 // tolerance, and the most cycles one solve may run
 constexpr double kPcTolFrac = 0.5;
 constexpr int kMaxCycles = 4;
+// mixed precision: the far field runs in FP32 once the implicit residual is
+// below kMixedSwitch * tol.  An inexact operator whose error is a fraction
+// eps of ||A x|| perturbs the attainable residual by about eps * ||r_k|| /
+// ||b|| at iteration k (Simoncini and Szyld 2003), so with eps ~ 1e-7 for the
+// FP32 far field the switch costs nothing measurable at 1e4 * tol; the true
+// residual is always formed in FP64 and a cycle that misses the tolerance
+// restarts.
+constexpr double kMixedSwitch = 1.0e4;
 
 inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
 inline size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
@@ -475,17 +483,20 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
     k_axpby_norm<<<nb, kBlk, 0, s>>>(1.0, c.rhs.p, 0.0, nullptr, nullptr, npad, bpart.p);
     STOKES_LAUNCH_CHECK();
     // The reference's iteration: one GMRES cycle stopped by the implicit
-    // residual estimate.  With a preconditioner the cycle aims at half the
-    // tolerance and, should the true residual of the returned density still
-    // exceed the tolerance, GMRES restarts from it (at most kMaxCycles cycles
-    // within maxl iterations in total).
-    const double tol_cycle = pc ? kPcTolFrac * tol : tol;
+    // residual estimate.  With a preconditioner or the mixed-precision far
+    // field the cycle aims at half the tolerance and, should the true residual
+    // of the returned density still exceed the tolerance, GMRES restarts from
+    // it (at most kMaxCycles cycles within maxl iterations in total).
+    const bool accel = pc || c.precision != 0;
+    const double tol_cycle = accel ? kPcTolFrac * tol : tol;
     double* V0 = c.V.p;
-    long long total_iters = 0;
+    long long total_iters = 0, fp32_iters = 0;
     int status = 0, nreorth = 0;
     double implicit_relres = 0.0, true_relres = 0.0;
+    const bool mixed = c.precision != 0;
     for (int cycle = 0;; ++cycle) {
         const int maxl_cycle = maxl - static_cast<int>(total_iters);
+        c.ff_fp32_now = false;  // residuals are formed with the FP64 operator
         if (cycle == 0) {
             // r0 = b - A x0 into V[0]
             apply_operator(c, c.xvec.p, c.rvec.p, ibary);
@@ -504,8 +515,14 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
         STOKES_LAUNCH_CHECK();
 
         int k = 0;
+        // implicit residual / ||b|| after the last chunk; a restarted cycle
+        // starts from the true residual the previous cycle left
+        double relres_now = cycle > 0 ? true_relres : 1.0;
         while (k < maxl_cycle) {
-            const int chunk = std::min(kChunkIters, maxl_cycle - k);
+            // mixed precision decides per chunk: shorter chunks switch sooner
+            const int chunk = std::min(mixed ? kChunkIters / 2 : kChunkIters, maxl_cycle - k);
+            c.ff_fp32_now = c.precision == 2 || (mixed && relres_now <= kMixedSwitch * tol);
+            if (c.ff_fp32_now) fp32_iters += chunk;
             for (int j = 0; j < chunk; ++j) {
                 const int kk = k + j;
                 double* vk = c.V.p + static_cast<size_t>(kk) * npad;
@@ -532,8 +549,13 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
             k += chunk;
             STOKES_HIP(hipMemcpyAsync(c.h_st, c.st.p, sizeof(GmresState), hipMemcpyDeviceToHost, s));
             STOKES_HIP(hipStreamSynchronize(s));
-            if (c.h_st->done) break;
+            relres_now = c.h_st->rho / c.h_st->bnrm;
+            if (c.h_st->done) {
+                if (c.ff_fp32_now) fp32_iters -= chunk - (c.h_st->k - (k - chunk));
+                break;
+            }
         }
+        c.ff_fp32_now = false;
         GmresState fin = *c.h_st;
         const int iters = fin.k;
         status = 0;
@@ -585,17 +607,18 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
         nreorth += fin.nreorth;
         implicit_relres = fin.rho / fin.bnrm;
         true_relres = std::sqrt(rr) / fin.bnrm;
-        if (!pc || status != 0 || true_relres <= tol || total_iters >= maxl ||
+        if (!accel || status != 0 || true_relres <= tol || total_iters >= maxl ||
             cycle + 1 >= kMaxCycles)
             break;
     }
-    if (pc && status == 0 && true_relres > tol) status = 1;
+    if (accel && status == 0 && true_relres > tol) status = 1;
     STOKES_HIP(hipMemcpyAsync(den_host, c.xvec.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
     STOKES_HIP(hipStreamSynchronize(s));
     c.last_iters = total_iters;
     c.last_implicit_relres = implicit_relres;
     c.last_true_relres = true_relres;
     c.last_reorth = nreorth;
+    c.last_fp32_iters = fp32_iters;
     c.status = status;
     c.last_seconds =
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();

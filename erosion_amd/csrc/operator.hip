@@ -147,6 +147,140 @@ __global__ __launch_bounds__(kBlk) void k_farfield(
     }
 }
 
+// ------------------------------------------------------ FP32 far field
+// The same pair term in FP32.  Sources are stored relative to the centre of
+// their 256-point tile (FP32 offsets of a few body radii at most), each
+// target's offset from that centre is formed in FP64 once per tile and
+// rounded, so rx = x_t - x_s keeps its relative accuracy for near and far
+// pairs alike.  Pair terms are summed in FP32 over 64 sources and the
+// partial sums are accumulated in FP64.
+constexpr int kSub32 = 64;
+
+template <int TPT, bool CHECK>
+__device__ __forceinline__ void ff32_tile(const float2* __restrict__ sxy,
+                                          const float2* __restrict__ sm,
+                                          const float2* __restrict__ se, const float* dx,
+                                          const float* dy, const int* self, double* u, double* v) {
+    for (int sub = 0; sub < kTile; sub += kSub32) {
+        float uf[TPT], vf[TPT];
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) uf[q] = vf[q] = 0.0f;
+#pragma unroll 4
+        for (int jj = sub; jj < sub + kSub32; ++jj) {
+            const float2 p = sxy[jj];
+            const float2 mm = sm[jj];
+            const float2 e = se[jj];
+#pragma unroll
+            for (int q = 0; q < TPT; ++q) {
+                const float rx = dx[q] - p.x, ry = dy[q] - p.y;
+                const float rho2 = __builtin_fmaf(rx, rx, ry * ry);
+                const float a = __builtin_fmaf(rx, mm.x, ry * mm.y);
+                const float b = __builtin_fmaf(rx, e.x, ry * e.y);
+                const float inv = __builtin_amdgcn_rcpf(rho2);
+                float cf = (a * b) * (inv * inv);
+                if (CHECK) cf = (jj == self[q]) ? 0.0f : cf;
+                uf[q] = __builtin_fmaf(cf, rx, uf[q]);
+                vf[q] = __builtin_fmaf(cf, ry, vf[q]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            u[q] += static_cast<double>(uf[q]);
+            v[q] += static_cast<double>(vf[q]);
+        }
+    }
+}
+
+// Tile centres (point means) and FP32 source offsets and weights; padding
+// sources past npts sit far away with zero weight.  One block per tile.
+__global__ __launch_bounds__(kBlk) void k_ff32_prep(const double* __restrict__ x,
+                                                    const double* __restrict__ y,
+                                                    const double* __restrict__ mx,
+                                                    const double* __restrict__ my, int npts,
+                                                    double2* __restrict__ tctr,
+                                                    float2* __restrict__ sxy,
+                                                    float2* __restrict__ sm) {
+    __shared__ double shx[kBlk], shy[kBlk];
+    const int j = blockIdx.x * kTile + threadIdx.x;
+    const bool valid = j < npts;
+    shx[threadIdx.x] = valid ? x[j] : 0.0;
+    shy[threadIdx.x] = valid ? y[j] : 0.0;
+    __syncthreads();
+    for (int w = kBlk / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) {
+            shx[threadIdx.x] += shx[threadIdx.x + w];
+            shy[threadIdx.x] += shy[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    const int cnt = min(kTile, npts - static_cast<int>(blockIdx.x) * kTile);
+    const double cx = shx[0] / cnt, cy = shy[0] / cnt;
+    if (threadIdx.x == 0) tctr[blockIdx.x] = make_double2(cx, cy);
+    if (valid) {
+        sxy[j] = make_float2(static_cast<float>(x[j] - cx), static_cast<float>(y[j] - cy));
+        sm[j] = make_float2(static_cast<float>(mx[j]), static_cast<float>(my[j]));
+    } else {
+        sxy[j] = make_float2(1.0e3f, 1.0e3f);
+        sm[j] = make_float2(0.0f, 0.0f);
+    }
+}
+
+template <int TPT>
+__global__ __launch_bounds__(kBlk) void k_farfield32(
+    const double* __restrict__ x, const double* __restrict__ y, const float2* __restrict__ s32xy,
+    const float2* __restrict__ s32m, const double2* __restrict__ tctr,
+    const double* __restrict__ ex, const double* __restrict__ ey, int npts, int split_len,
+    double* __restrict__ part_u, double* __restrict__ part_v) {
+    __shared__ float2 sxy[kTile], sm[kTile], se[kTile];
+    const int tid = threadIdx.x;
+    const int tb0 = blockIdx.x * TPT * kBlk;
+    const int tb1 = min(tb0 + TPT * kBlk, npts);
+    double xt[TPT], yt[TPT], u[TPT], v[TPT];
+    int t[TPT];
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+        t[q] = tb0 + q * kBlk + tid;
+        const bool valid = t[q] < npts;
+        xt[q] = valid ? x[t[q]] : 0.0;
+        yt[q] = valid ? y[t[q]] : 0.0;
+        u[q] = v[q] = 0.0;
+    }
+    const int s0 = blockIdx.y * split_len;
+    const int s1 = min(s0 + split_len, npts);
+    for (int base = s0; base < s1; base += kTile) {
+        const int j = base + tid;
+        __syncthreads();
+        sxy[tid] = s32xy[j];
+        sm[tid] = s32m[j];
+        se[tid] = j < s1 ? make_float2(static_cast<float>(ex[j]), static_cast<float>(ey[j]))
+                         : make_float2(0.0f, 0.0f);
+        __syncthreads();
+        const double2 ctr = tctr[base / kTile];
+        float dx[TPT], dy[TPT];
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            dx[q] = static_cast<float>(xt[q] - ctr.x);
+            dy[q] = static_cast<float>(yt[q] - ctr.y);
+        }
+        const bool diag = (base < tb1) && (base + kTile > tb0);
+        if (diag) {
+            int self[TPT];
+#pragma unroll
+            for (int q = 0; q < TPT; ++q) self[q] = t[q] - base;
+            ff32_tile<TPT, true>(sxy, sm, se, dx, dy, self, u, v);
+        } else {
+            ff32_tile<TPT, false>(sxy, sm, se, dx, dy, nullptr, u, v);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+        if (t[q] < npts) {
+            part_u[static_cast<size_t>(blockIdx.y) * npts + t[q]] = u[q];
+            part_v[static_cast<size_t>(blockIdx.y) * npts + t[q]] = v[q];
+        }
+    }
+}
+
 // Far-field launch shape: targets per lane and the grid size it aims for.
 struct FFShape {
     int tpt;
@@ -167,6 +301,29 @@ FFPlan ff_plan(int np, FFShape sh) {
     p.split_len = cdiv_int(cdiv_int(np, want), kTile) * kTile;
     p.nsplit = cdiv_int(np, p.split_len);
     return p;
+}
+
+void ff32_launch(const Context& c, const FFPlan& p, double* pu, double* pv) {
+    const Geometry& g = c.g;
+    const dim3 grid(p.tblocks, p.nsplit);
+    const float2* sxy = reinterpret_cast<const float2*>(c.ff32_xy.p);
+    const float2* sm = reinterpret_cast<const float2*>(c.ff32_m.p);
+    const double2* tc = reinterpret_cast<const double2*>(c.ff32_ctr.p);
+    switch (p.tpt) {
+    case 1:
+        k_farfield32<1><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, sxy, sm, tc, c.ex.p, c.ey.p,
+                                                     g.npts, p.split_len, pu, pv);
+        break;
+    case 2:
+        k_farfield32<2><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, sxy, sm, tc, c.ex.p, c.ey.p,
+                                                     g.npts, p.split_len, pu, pv);
+        break;
+    default:
+        k_farfield32<4><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, sxy, sm, tc, c.ex.p, c.ey.p,
+                                                     g.npts, p.split_len, pu, pv);
+        break;
+    }
+    STOKES_LAUNCH_CHECK();
 }
 
 void ff_launch(const Context& c, const FFPlan& p, double* pu, double* pv) {
@@ -638,10 +795,11 @@ void boundary_values(Context& c, const int* curves_dev, int ncurves, int wall_fi
 // Autotuner: the first operator application at a given number of boundary
 // points times every far-field shape (one warm-up launch, then three timed
 // launches each, HIP events on the library stream) and keeps the fastest.
-static int tune_farfield(Context& c) {
+static int tune_farfield(Context& c, bool fp32) {
     const int np = c.g.npts;
-    auto it = c.ff_choice.find(np);
-    if (it != c.ff_choice.end()) return it->second;
+    std::map<int, int>& choice = fp32 ? c.ff32_choice : c.ff_choice;
+    auto it = choice.find(np);
+    if (it != choice.end()) return it->second;
     const int nshape = static_cast<int>(sizeof(kFFShapes) / sizeof(kFFShapes[0]));
     int best = 0;
     float best_ms = 1e30f;
@@ -652,9 +810,15 @@ static int tune_farfield(Context& c) {
         const FFPlan p = ff_plan(np, kFFShapes[i]);
         c.part_u.ensure(static_cast<size_t>(np) * static_cast<size_t>(p.nsplit));
         c.part_v.ensure(static_cast<size_t>(np) * static_cast<size_t>(p.nsplit));
-        ff_launch(c, p, c.part_u.p, c.part_v.p);
+        auto launch = [&]() {
+            if (fp32)
+                ff32_launch(c, p, c.part_u.p, c.part_v.p);
+            else
+                ff_launch(c, p, c.part_u.p, c.part_v.p);
+        };
+        launch();
         STOKES_HIP(hipEventRecord(e0, c.stream));
-        for (int r = 0; r < 3; ++r) ff_launch(c, p, c.part_u.p, c.part_v.p);
+        for (int r = 0; r < 3; ++r) launch();
         STOKES_HIP(hipEventRecord(e1, c.stream));
         STOKES_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -666,8 +830,8 @@ static int tune_farfield(Context& c) {
     }
     STOKES_HIP(hipEventDestroy(e0));
     STOKES_HIP(hipEventDestroy(e1));
-    c.ff_choice[np] = best;
-    c.ff_tuned_ms[np] = best_ms / 3.0f;
+    choice[np] = best;
+    (fp32 ? c.ff32_tuned_ms : c.ff_tuned_ms)[np] = best_ms / 3.0f;
     return best;
 }
 
@@ -680,17 +844,32 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary) {
     c.scal.ensure(8);
 
     gather_density(c, x_dev);
-    const FFPlan plan = ff_plan(np, kFFShapes[tune_farfield(c)]);
+    const bool fp32 = c.ff_fp32_now;
+    if (fp32 && c.ff32_version != g.version) {
+        const int ntiles = cdiv_int(np, kTile);
+        c.ff32_ctr.ensure(2 * static_cast<size_t>(ntiles));
+        c.ff32_xy.ensure(2 * static_cast<size_t>(ntiles) * kTile);
+        c.ff32_m.ensure(2 * static_cast<size_t>(ntiles) * kTile);
+        k_ff32_prep<<<ntiles, kBlk, 0, s>>>(g.x.p, g.y.p, g.mx.p, g.my.p, np,
+                                            reinterpret_cast<double2*>(c.ff32_ctr.p),
+                                            reinterpret_cast<float2*>(c.ff32_xy.p),
+                                            reinterpret_cast<float2*>(c.ff32_m.p));
+        STOKES_LAUNCH_CHECK();
+        c.ff32_version = g.version;
+    }
+    const FFPlan plan = ff_plan(np, kFFShapes[tune_farfield(c, fp32)]);
     const int nsplit = plan.nsplit;
     c.ff_splits = nsplit;
     c.ff_tpt = plan.tpt;
     c.part_u.ensure(N * static_cast<size_t>(nsplit));
     c.part_v.ensure(N * static_cast<size_t>(nsplit));
-    if (c.time_kernels) c.t_farfield.start(s);
-    ff_launch(c, plan, c.part_u.p, c.part_v.p);
-    if (c.time_kernels) {
-        c.t_farfield.stop(s, static_cast<double>(np) * static_cast<double>(np - 1));
-    }
+    KernelTimer& tff = fp32 ? c.t_farfield32 : c.t_farfield;
+    if (c.time_kernels) tff.start(s);
+    if (fp32)
+        ff32_launch(c, plan, c.part_u.p, c.part_v.p);
+    else
+        ff_launch(c, plan, c.part_u.p, c.part_v.p);
+    if (c.time_kernels) tff.stop(s, static_cast<double>(np) * static_cast<double>(np - 1));
     const bool has_near = ibary == 1 && L.nentries > 0;
     if (has_near && c.time_kernels) c.t_near.start(s);
     if (has_near) {

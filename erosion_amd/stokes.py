@@ -52,6 +52,9 @@ def lib():
         L.erosion_kernel_timing.restype = ctypes.c_int
         L.erosion_kernel_timers.argtypes = [_dp]
         L.erosion_kernel_timers.restype = ctypes.c_int
+        L.erosion_set_precision.argtypes = [ctypes.c_int]
+        L.erosion_set_precision.restype = ctypes.c_int
+        L.erosion_get_precision.restype = ctypes.c_int
         L.erosion_set_preconditioner.argtypes = [ctypes.c_int] * 3
         L.erosion_set_preconditioner.restype = ctypes.c_int
         L.erosion_get_preconditioner.argtypes = [ctypes.POINTER(ctypes.c_int)]
@@ -102,12 +105,12 @@ def get_gmres_tol() -> float:
 
 
 def last_solve_info() -> dict:
-    info = np.zeros(8)
+    info = np.zeros(16)
     lib().erosion_last_solve_info(_p(info))
     return {"iters": int(info[0]), "implicit_relres": float(info[1]),
             "true_relres": float(info[2]), "seconds": float(info[3]), "reorth": int(info[4]),
             "ff_targets_per_lane": int(info[5]), "ff_splits": int(info[6]),
-            "preconditioner": int(info[7])}
+            "preconditioner": int(info[7]), "fp32_iters": int(info[8])}
 
 
 def stokessolver(npts: int, nbods: int, nouter: int, ifmm: int, ibary: int, ibc: int, maxl: int,
@@ -146,7 +149,7 @@ def time_matvec(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: int = 1,
             "near_ms": out[4], "splits": int(out[5])}
 
 
-TIMER_FAMILIES = ("farfield", "near", "stress_eval", "gram_schmidt", "precond")
+TIMER_FAMILIES = ("farfield", "near", "stress_eval", "gram_schmidt", "precond", "farfield32")
 
 
 def kernel_timing(enable: bool) -> None:
@@ -209,3 +212,20 @@ def precond_apply(npts: int, nbods: int, nouter: int, xx, yy, r) -> np.ndarray:
     if lib().erosion_precond_apply(npts, nbods, nouter, _p(xx), _p(yy), _p(r), _p(z)) != 0:
         raise StokesError(f"erosion_precond_apply: {last_error()}")
     return z
+
+
+PRECISION_FP64 = 0
+PRECISION_MIXED = 1
+PRECISION_FP32_FARFIELD = 2
+
+
+def set_precision(mode: int) -> None:
+    """Far-field precision of stokessolver: 0 FP64 (as the reference), 1 mixed
+    (FP32 far field once the residual is small, FP64 true residual and
+    restart), 2 FP32 far field everywhere (tests and measurement)."""
+    if lib().erosion_set_precision(int(mode)) != 0:
+        raise StokesError(f"invalid precision mode {mode}")
+
+
+def get_precision() -> int:
+    return int(lib().erosion_get_precision())

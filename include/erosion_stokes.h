@@ -94,8 +94,9 @@ const char* erosion_last_error(void);
  * info[0] iterations, info[1] implicit relative residual, info[2] true relative
  * residual ||b - A x|| / ||b|| of the returned density, info[3] seconds,
  * info[4] re-orthogonalised iterations, info[5] far-field targets per lane,
- * info[6] far-field source splits chosen by the autotuner and info[7] the
- * preconditioner kind.  `info` holds at least 8 doubles.  Returns 0. */
+ * info[6] far-field source splits chosen by the autotuner, info[7] the
+ * preconditioner kind and info[8] the iterations whose far field ran in FP32.
+ * `info` holds at least 9 doubles.  Returns 0. */
 int erosion_last_solve_info(double* info);
 
 /* One application of the completed operator vel = A den, as matvec_DLP_fmmbary
@@ -121,12 +122,26 @@ int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t iba
 int erosion_kernel_timing(int enable);
 
 /* Totals since timing was enabled, three values per family in the order
- * far field, near pairs, stress evaluation, Gram-Schmidt, preconditioner:
- * out[3i] milliseconds, out[3i+1] launches, out[3i+2] algorithmic work
- * (ordered source-target pairs; near entries; target-node pairs; bytes of
- * Krylov basis streamed; bytes of vector read and written).  `out` holds 15
- * doubles.  Returns 0 or -1. */
+ * far field (FP64), near pairs, stress evaluation, Gram-Schmidt,
+ * preconditioner, far field (FP32): out[3i] milliseconds, out[3i+1] launches,
+ * out[3i+2] algorithmic work (ordered source-target pairs; near entries;
+ * target-node pairs; bytes of Krylov basis streamed; bytes of vector read and
+ * written; ordered source-target pairs).  `out` holds 18 doubles.  Returns 0
+ * or -1. */
 int erosion_kernel_timers(double* out);
+
+/* Precision of the far field in stokessolver_ (no counterpart in the
+ * reference, which is FP64 throughout).  mode 0: FP64 (default).  mode 1:
+ * mixed -- once GMRES's implicit residual is below 1e4 times the tolerance the
+ * far-field pair sums run in FP32 (sources re-centred per 256-point tile,
+ * FP32 partial sums of 64 terms accumulated in FP64); the near-pair
+ * corrections, local terms, Gram-Schmidt, Givens update and the true
+ * residual stay FP64, and a solve whose FP64 true residual misses the
+ * tolerance restarts from its result.  mode 2: FP32 far field in every
+ * operator application (measurement and tests; erosion_matvec follows it).
+ * Returns 0, or -1 for another mode. */
+int erosion_set_precision(int mode);
+int erosion_get_precision(void);
 
 /* Right preconditioner of stokessolver_ (no counterpart in the reference,
  * whose solveBIE runs GMRES unpreconditioned, stokesSolver.f:490).

@@ -1,0 +1,98 @@
+"""Mixed precision: FP32 far field inside FP64-refined GMRES (GPU).
+
+The FP32 far field is checked against the FP64 operator, and mixed solves
+against FP64 solves of the same geometry: the FP64 true residual meets the
+tolerance and the density and shear stress agree to what the tolerance allows
+(shear stress within 1e-8 relative).
+"""
+import numpy as np
+import pytest
+
+from conftest import load_golden, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture
+def st():
+    from erosion_amd import stokes
+    stokes.set_gmres_tol(1e-8)
+    yield stokes
+    stokes.set_precision(0)
+    stokes.set_preconditioner(0)
+    stokes.set_gmres_tol(1e-8)
+
+
+def _full():
+    from erosion_amd.geometry import WORKLOADS, workload_xy
+    return workload_xy(WORKLOADS["bodies100_n256"])
+
+
+@pytest.mark.parametrize("case", ["tight8_n64", "bodies10_n128", "bodies100_n256"])
+def test_fp32_farfield_operator_accuracy(st, case):
+    if case == "bodies100_n256":
+        n, m, no, xx, yy = _full()
+    else:
+        g = load_golden(case)
+        n, m, no, xx, yy = int(g["n"]), int(g["m"]), int(g["no"]), g["xx"], g["yy"]
+    N = 2 * no + 2 * n * m + 3 * m
+    eta = np.random.default_rng(21).standard_normal(N)
+    st.set_precision(0)
+    v64 = st.matvec(n, m, no, xx, yy, eta)
+    st.set_precision(2)
+    v32 = st.matvec(n, m, no, xx, yy, eta)
+    err = relerr(v32, v64)
+    rms = np.sqrt(np.mean((v32 - v64) ** 2) / np.mean(v64 ** 2))
+    print(f"{case}: FP32 far field max rel err {err:.2e}, rms {rms:.2e}")
+    assert err < 1e-5 and rms < 1e-6
+
+
+@pytest.mark.parametrize("pc", [0, 1])
+def test_mixed_solve_bodies10(st, pc):
+    g = load_golden("bodies10_n128")
+    n, m, no, xx, yy = int(g["n"]), int(g["m"]), int(g["no"]), g["xx"], g["yy"]
+    st.set_gmres_tol(1e-10)
+    st.set_preconditioner(pc)
+    st.set_precision(0)
+    d64, it64 = st.stokessolver(n, m, no, 1, 1, 0, 8000, xx, yy)
+    st.set_precision(1)
+    d32, it32 = st.stokessolver(n, m, no, 1, 1, 0, 8000, xx, yy)
+    info = st.last_solve_info()
+    print(f"pc={pc}: fp64 {it64} iterations, mixed {it32} ({info['fp32_iters']} FP32), "
+          f"true relres {info['true_relres']:.2e}")
+    assert st.last_status() == 0
+    assert info["true_relres"] <= 1e-10
+    # unpreconditioned GMRES here falls from a plateau to the tolerance within
+    # one 16-iteration chunk, before the residual reaches the FP32 threshold
+    assert info["fp32_iters"] > 0 or pc == 0
+    assert relerr(d32, g["den_tol12"]) < 1e-8
+    st.set_precision(0)
+    t64 = st.computeshearstress(n, m, no, xx, yy, d64)
+    t32 = st.computeshearstress(n, m, no, xx, yy, d32)
+    assert relerr(t32, t64) < 1e-8
+
+
+def test_mixed_solve_full_size(st):
+    n, m, no, xx, yy = _full()
+    st.set_gmres_tol(1e-10)
+    st.set_preconditioner(1)
+    st.set_precision(0)
+    d64, it64 = st.stokessolver(n, m, no, 1, 1, 0, 8000, xx, yy)
+    st.set_precision(1)
+    d32, it32 = st.stokessolver(n, m, no, 1, 1, 0, 8000, xx, yy)
+    info = st.last_solve_info()
+    print(f"full size: fp64 {it64} iterations, mixed {it32} ({info['fp32_iters']} FP32), "
+          f"true relres {info['true_relres']:.2e}")
+    assert info["true_relres"] <= 1e-10 and info["fp32_iters"] > 0
+    st.set_precision(0)
+    t64 = st.computeshearstress(n, m, no, xx, yy, d64)
+    t32 = st.computeshearstress(n, m, no, xx, yy, d32)
+    print(f"full size: stress rel diff {relerr(t32, t64):.2e}, density {relerr(d32, d64):.2e}")
+    assert relerr(t32, t64) < 1e-8
+
+
+def test_precision_mode_validation(st):
+    with pytest.raises(st.StokesError):
+        st.set_precision(3)
+    st.set_precision(1)
+    assert st.get_precision() == 1
