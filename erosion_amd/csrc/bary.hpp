@@ -21,8 +21,10 @@ struct NodeView {
 // The reference clears tmp1/tmp2 once per target, not per node; two nodes
 // of one curve within 1e-10 of the same target cannot occur for distinct
 // boundary nodes, so one accumulation per call is the same thing.
-__device__ inline cplx coincident_term(const NodeView& nv, int off, int len, int k, int f,
-                                       cplx zt, cplx a, cplx d) {
+// Rare (a target on a node of another curve); kept out of line so the hot
+// loops around it keep their registers.
+__device__ __noinline__ cplx coincident_term(const NodeView& nv, int off, int len, int k, int f,
+                                             cplx zt, cplx a, cplx d) {
     const size_t N = nv.N;
     const cplx bk = cmk(nv.bd[(2 * f) * N + off + k], nv.bd[(2 * f + 1) * N + off + k]);
     const cplx zk = cmk(nv.x[off + k], nv.y[off + k]);

@@ -318,6 +318,16 @@ int erosion_set_precision(int mode) {
     }
 }
 
+int erosion_set_mixed_switch(double factor) {
+    try {
+        if (!(factor >= 1.0)) return -1;
+        stokes::ctx().mixed_switch = factor;
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_set_mixed_switch", e.what());
+    }
+}
+
 int erosion_get_precision(void) {
     try {
         return stokes::ctx().precision;

@@ -209,6 +209,7 @@ struct Context {
     // precision of the far field: 0 FP64; 1 mixed (GMRES switches the far
     // field to FP32 once its residual is small, see gmres.hip); 2 FP32 always
     int precision = 0;
+    double mixed_switch = 1.0e4;  // FP32 far field once residual <= mixed_switch * tol
     bool ff_fp32_now = false;  // the far field of the next operator applications
     // GMRES buffers
     int npad = 0;

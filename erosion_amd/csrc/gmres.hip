@@ -36,13 +36,13 @@ constexpr int kChunkIters = 16;
 constexpr double kPcTolFrac = 0.5;
 constexpr int kMaxCycles = 4;
 // mixed precision: the far field runs in FP32 once the implicit residual is
-// below kMixedSwitch * tol.  An inexact operator whose error is a fraction
+// below mixed_switch * tol (default kMixedSwitchDefault).  An inexact operator whose error is a fraction
 // eps of ||A x|| perturbs the attainable residual by about eps * ||r_k|| /
 // ||b|| at iteration k (Simoncini and Szyld 2003), so with eps ~ 1e-7 for the
 // FP32 far field the switch costs nothing measurable at 1e4 * tol; the true
 // residual is always formed in FP64 and a cycle that misses the tolerance
 // restarts.
-constexpr double kMixedSwitch = 1.0e4;
+constexpr double kMixedSwitchDefault = 1.0e4;
 
 inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
 inline size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
@@ -521,7 +521,7 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
         while (k < maxl_cycle) {
             // mixed precision decides per chunk: shorter chunks switch sooner
             const int chunk = std::min(mixed ? kChunkIters / 2 : kChunkIters, maxl_cycle - k);
-            c.ff_fp32_now = c.precision == 2 || (mixed && relres_now <= kMixedSwitch * tol);
+            c.ff_fp32_now = c.precision == 2 || (mixed && relres_now <= c.mixed_switch * tol);
             if (c.ff_fp32_now) fp32_iters += chunk;
             for (int j = 0; j < chunk; ++j) {
                 const int kk = k + j;

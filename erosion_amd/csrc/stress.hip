@@ -23,42 +23,52 @@ namespace stokes {
 
 namespace {
 
-constexpr int kNS = 4;  // waves per block: node slices of each source curve
 constexpr int kBlk = 256;
 
 inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
 
-// Sum NV values per lane across the kNS slices; every thread gets totals.
-template <int NV>
-__device__ __forceinline__ void slice_sum(double (&v)[NV], double (*red)[kNS][64], int slice,
-                                          int lane) {
-#pragma unroll
-    for (int q = 0; q < NV; ++q) red[q][slice][lane] = v[q];
+// nder=2 barycentric evaluation of source curves [grp_beg[g], grp_end[g]) at
+// kBT targets per block, one per lane.  The node data of a source curve
+// (coordinates, dz, the five boundary values, 1/(z - a)) is staged in LDS
+// kNodeTile nodes at a time and read by every lane with broadcast loads; a
+// curve that fits one tile is staged once for all three passes.  MODE 0
+// (stress): targets are the body points (tgt_x == nullptr), a body's own
+// curve is skipped, and part[(g*3 + {0,1,2}) * ntar + q] = E11, E12, E22.
+// MODE 1 (targets in the fluid): every curve counts and part[...] = u, v,
+// vorticity.
+constexpr int kBT = 256;
+constexpr int kNodeTile = 256;
+
+struct NodeTileLds {
+    double x[kNodeTile], y[kNodeTile], dzr[kNodeTile], dzi[kNodeTile];
+    double iar[kNodeTile], iai[kNodeTile];
+    double bd[10][kNodeTile];
+};
+
+__device__ __forceinline__ void stage_nodes(NodeTileLds& T, const NodeView& nv, int p0, int cnt) {
     __syncthreads();
+    for (int k = threadIdx.x; k < cnt; k += kBT) {
+        const int p = p0 + k;
+        T.x[k] = nv.x[p];
+        T.y[k] = nv.y[p];
+        T.dzr[k] = nv.dzr[p];
+        T.dzi[k] = nv.dzi[p];
+        T.iar[k] = nv.iar[p];
+        T.iai[k] = nv.iai[p];
 #pragma unroll
-    for (int q = 0; q < NV; ++q) {
-        double s = 0.0;
-#pragma unroll
-        for (int t = 0; t < kNS; ++t) s += red[q][t][lane];
-        v[q] = s;
+        for (int f = 0; f < 10; ++f) T.bd[f][k] = nv.bd[static_cast<size_t>(f) * nv.N + p];
     }
     __syncthreads();
 }
 
-// nder=2 barycentric evaluation of source curves [grp_beg[g], grp_end[g]) at
-// 64 targets per block.  MODE 0 (stress): targets are the body points
-// (tgt_x == nullptr), a body's own curve is skipped, and
-// part[(g*3 + {0,1,2}) * ntar + q] = E11, E12, E22.  MODE 1 (targets in the
-// fluid): every curve counts and part[...] = u, v, vorticity.
 template <int MODE>
-__global__ __launch_bounds__(64 * kNS) void k_bary_eval(
+__global__ __launch_bounds__(kBT) void k_bary_eval(
     const int* __restrict__ grp_beg, const int* __restrict__ grp_end, NodeView nv,
     const double* __restrict__ ax, const double* __restrict__ ay, int no, int n,
     const double* __restrict__ tgt_x, const double* __restrict__ tgt_y, int ntar,
     double* __restrict__ spart) {
-    __shared__ double red[12][kNS][64];
-    const int lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
-    const int q = blockIdx.x * 64 + lane;
+    __shared__ NodeTileLds T;
+    const int q = blockIdx.x * kBT + threadIdx.x;
     const bool valid = q < ntar;
     const int qq = valid ? q : 0;
     const int bt = MODE == 0 ? qq / n : -2;
@@ -69,30 +79,32 @@ __global__ __launch_bounds__(64 * kNS) void k_bary_eval(
     const int g = blockIdx.y;
     for (int c = grp_beg[g]; c < grp_end[g]; ++c) {
         const bool skip = !valid || c == 1 + bt;
-        if (__all(skip)) continue;  // same target set in every wave: uniform
+        if (__syncthreads_count(skip ? 0 : 1) == 0) continue;  // block-uniform
         const int len = c == 0 ? no : n;
         const int off = c == 0 ? 0 : no + (c - 1) * n;
         const bool ext = c != 0;
+        const bool one_tile = len <= kNodeTile;
         const cplx a = cmk(ax[c], ay[c]);
-        const int k0 = (slice * len) / kNS, k1 = ((slice + 1) * len) / kNS;
         // pass 1: S0 = sum w, N_f = sum b_f w, w = dz/(z - x)
         double v1[12];
 #pragma unroll
         for (int i = 0; i < 12; ++i) v1[i] = 0.0;
-        for (int k = k0; k < k1; ++k) {
-            const int p = off + k;
-            const cplx d = cmk(nv.x[p] - xt, nv.y[p] - yt);
-            const cplx w = cmul(cmk(nv.dzr[p], nv.dzi[p]), cinv(d));
-            v1[0] += w.r;
-            v1[1] += w.i;
+        for (int t0 = 0; t0 < len; t0 += kNodeTile) {
+            const int tn = min(kNodeTile, len - t0);
+            stage_nodes(T, nv, off + t0, tn);
+            for (int k = 0; k < tn; ++k) {
+                const cplx d = cmk(T.x[k] - xt, T.y[k] - yt);
+                const cplx w = cmul(cmk(T.dzr[k], T.dzi[k]), cinv(d));
+                v1[0] += w.r;
+                v1[1] += w.i;
 #pragma unroll
-            for (int f = 0; f < 5; ++f) {
-                const cplx b = bdv(nv, f, p);
-                v1[2 + 2 * f] = __builtin_fma(b.r, w.r, __builtin_fma(-b.i, w.i, v1[2 + 2 * f]));
-                v1[3 + 2 * f] = __builtin_fma(b.r, w.i, __builtin_fma(b.i, w.r, v1[3 + 2 * f]));
+                for (int f = 0; f < 5; ++f) {
+                    const cplx b = cmk(T.bd[2 * f][k], T.bd[2 * f + 1][k]);
+                    v1[2 + 2 * f] = __builtin_fma(b.r, w.r, __builtin_fma(-b.i, w.i, v1[2 + 2 * f]));
+                    v1[3 + 2 * f] = __builtin_fma(b.r, w.i, __builtin_fma(b.i, w.r, v1[3 + 2 * f]));
+                }
             }
         }
-        slice_sum<12>(v1, red, slice, lane);
         const cplx S0 = cmk(v1[0], v1[1]);
         const cplx idv = cinv(ext ? cmk(S0.r, S0.i - kTwoPi) : S0);
         cplx u[5];
@@ -102,55 +114,61 @@ __global__ __launch_bounds__(64 * kNS) void k_bary_eval(
         double v2[12];
 #pragma unroll
         for (int i = 0; i < 12; ++i) v2[i] = 0.0;
-        for (int k = k0; k < k1; ++k) {
-            const int p = off + k;
-            const cplx d = cmk(nv.x[p] - xt, nv.y[p] - yt);
-            const double rho2 = __builtin_fma(d.r, d.r, d.i * d.i);
-            const double ir = rcp64(rho2);
-            const cplx id = cmk(d.r * ir, -d.i * ir);
-            const cplx w = cmul(cmk(nv.dzr[p], nv.dzi[p]), id);
-            if (ext && !(sqrt(rho2) > 1.0e-10)) {
+        for (int t0 = 0; t0 < len; t0 += kNodeTile) {
+            const int tn = min(kNodeTile, len - t0);
+            if (!one_tile) stage_nodes(T, nv, off + t0, tn);
+            for (int k = 0; k < tn; ++k) {
+                const cplx d = cmk(T.x[k] - xt, T.y[k] - yt);
+                const double rho2 = __builtin_fma(d.r, d.r, d.i * d.i);
+                const double ir = rcp64(rho2);
+                const cplx id = cmk(d.r * ir, -d.i * ir);
+                const cplx w = cmul(cmk(T.dzr[k], T.dzi[k]), id);
+                if (ext && !(sqrt(rho2) > 1.0e-10)) {
 #pragma unroll
-                for (int f = 0; f < 5; ++f) {
-                    cplx t = coincident_term(nv, off, len, k, f, zt, a, d);
-                    v2[2 * f] += t.r;
-                    v2[2 * f + 1] += t.i;
-                }
-            } else {
-                const cplx w2 = cmul(w, id);
+                    for (int f = 0; f < 5; ++f) {
+                        cplx t = coincident_term(nv, off, len, t0 + k, f, zt, a, d);
+                        v2[2 * f] += t.r;
+                        v2[2 * f + 1] += t.i;
+                    }
+                } else {
+                    const cplx w2 = cmul(w, id);
 #pragma unroll
-                for (int f = 0; f < 5; ++f) {
-                    const cplx t = csub(bdv(nv, f, p), u[f]);
-                    v2[2 * f] = __builtin_fma(t.r, w2.r, __builtin_fma(-t.i, w2.i, v2[2 * f]));
-                    v2[2 * f + 1] = __builtin_fma(t.r, w2.i, __builtin_fma(t.i, w2.r, v2[2 * f + 1]));
+                    for (int f = 0; f < 5; ++f) {
+                        const cplx t = csub(cmk(T.bd[2 * f][k], T.bd[2 * f + 1][k]), u[f]);
+                        v2[2 * f] = __builtin_fma(t.r, w2.r, __builtin_fma(-t.i, w2.i, v2[2 * f]));
+                        v2[2 * f + 1] =
+                            __builtin_fma(t.r, w2.i, __builtin_fma(t.i, w2.r, v2[2 * f + 1]));
+                    }
                 }
-            }
-            if (ext) {
-                const cplx wa = cmul(w, cmk(nv.iar[p], nv.iai[p]));
-                v2[10] += wa.r;
-                v2[11] += wa.i;
+                if (ext) {
+                    const cplx wa = cmul(w, cmk(T.iar[k], T.iai[k]));
+                    v2[10] += wa.r;
+                    v2[11] += wa.i;
+                }
             }
         }
-        slice_sum<12>(v2, red, slice, lane);
         const cplx iden = cinv(ext ? cmul(csub(zt, a), cmk(v2[10], v2[11])) : S0);
         cplx du[5];
 #pragma unroll
         for (int f = 0; f < 5; ++f) du[f] = cmul(cmk(v2[2 * f], v2[2 * f + 1]), iden);
         // pass 3 (fields 0..2): P_f = sum (b_f - u_f - du_f (z-x)) w/(z-x)^2
         double v3[6] = {0, 0, 0, 0, 0, 0};
-        for (int k = k0; k < k1; ++k) {
-            const int p = off + k;
-            const cplx d = cmk(nv.x[p] - xt, nv.y[p] - yt);
-            const cplx id = cinv(d);
-            const cplx w3 = cmul(cmul(cmul(cmk(nv.dzr[p], nv.dzi[p]), id), id), id);
+        for (int t0 = 0; t0 < len; t0 += kNodeTile) {
+            const int tn = min(kNodeTile, len - t0);
+            if (!one_tile) stage_nodes(T, nv, off + t0, tn);
+            for (int k = 0; k < tn; ++k) {
+                const cplx d = cmk(T.x[k] - xt, T.y[k] - yt);
+                const cplx id = cinv(d);
+                const cplx w3 = cmul(cmul(cmul(cmk(T.dzr[k], T.dzi[k]), id), id), id);
 #pragma unroll
-            for (int f = 0; f < 3; ++f) {
-                const cplx t = csub(csub(bdv(nv, f, p), u[f]), cmul(du[f], d));
-                v3[2 * f] = __builtin_fma(t.r, w3.r, __builtin_fma(-t.i, w3.i, v3[2 * f]));
-                v3[2 * f + 1] = __builtin_fma(t.r, w3.i, __builtin_fma(t.i, w3.r, v3[2 * f + 1]));
+                for (int f = 0; f < 3; ++f) {
+                    const cplx t =
+                        csub(csub(cmk(T.bd[2 * f][k], T.bd[2 * f + 1][k]), u[f]), cmul(du[f], d));
+                    v3[2 * f] = __builtin_fma(t.r, w3.r, __builtin_fma(-t.i, w3.i, v3[2 * f]));
+                    v3[2 * f + 1] = __builtin_fma(t.r, w3.i, __builtin_fma(t.i, w3.r, v3[2 * f + 1]));
+                }
             }
         }
-        slice_sum<6>(v3, red, slice, lane);
         cplx dd[3];
 #pragma unroll
         for (int f = 0; f < 3; ++f) dd[f] = cscale(cmul(cmk(v3[2 * f], v3[2 * f + 1]), iden), 2.0);
@@ -171,7 +189,7 @@ __global__ __launch_bounds__(64 * kNS) void k_bary_eval(
             E22 += u2x - u1y;
         }
     }
-    if (slice == 0 && valid) {
+    if (valid) {
         spart[(static_cast<size_t>(g) * 3 + 0) * ntar + q] = E11;
         spart[(static_cast<size_t>(g) * 3 + 1) * ntar + q] = E12;
         spart[(static_cast<size_t>(g) * 3 + 2) * ntar + q] = E22;
@@ -440,7 +458,7 @@ void shear_stress(Context& c, const double* den_dev, double* tau_dev) {
     if (nbp == 0) return;
     hipStream_t s = c.stream;
     all_boundary_values(c, den_dev);
-    const int tiles = cdiv_int(nbp, 64);
+    const int tiles = cdiv_int(nbp, kBT);
     std::vector<int> gb, ge;
     const int G = plan_curve_groups(c, tiles, gb, ge);
     c.grp_beg.ensure(gb.size());
@@ -451,7 +469,7 @@ void shear_stress(Context& c, const double* den_dev, double* tau_dev) {
     NodeView nv{g.x.p, g.y.p, g.dzr.p, g.dzi.p, c.bd.p, g.iar.p, g.iai.p, g.mx.p, g.my.p,
                 c.ex.p, c.ey.p, static_cast<size_t>(g.npts)};
     if (c.time_kernels) c.t_stress_eval.start(s);
-    k_bary_eval<0><<<dim3(tiles, G), 64 * kNS, 0, s>>>(c.grp_beg.p, c.grp_end.p, nv, g.ax.p,
+    k_bary_eval<0><<<dim3(tiles, G), kBT, 0, s>>>(c.grp_beg.p, c.grp_end.p, nv, g.ax.p,
                                                        g.ay.p, no, n, nullptr, nullptr, nbp,
                                                        c.spart.p);
     STOKES_LAUNCH_CHECK();
@@ -485,7 +503,7 @@ void qoi_targets(Context& c, int ibary, const double* den_dev, int ntar, const d
                                      out_dev, c.iside.p);
     STOKES_LAUNCH_CHECK();
     if (ibary == 1) {
-        const int tiles = cdiv_int(ntar, 64);
+        const int tiles = cdiv_int(ntar, kBT);
         std::vector<int> gb, ge;
         // one group per curve keeps the grid wide for a handful of targets
         for (int cc = 0; cc <= m; ++cc) {
@@ -501,7 +519,7 @@ void qoi_targets(Context& c, int ibary, const double* den_dev, int ntar, const d
         c.spart.ensure(static_cast<size_t>(G) * 3 * static_cast<size_t>(ntar));
         NodeView nv{g.x.p, g.y.p, g.dzr.p, g.dzi.p, c.bd.p, g.iar.p, g.iai.p, g.mx.p, g.my.p,
                     c.ex.p, c.ey.p, static_cast<size_t>(g.npts)};
-        k_bary_eval<1><<<dim3(cdiv_int(ntar, 64), G), 64 * kNS, 0, s>>>(
+        k_bary_eval<1><<<dim3(cdiv_int(ntar, kBT), G), kBT, 0, s>>>(
             c.grp_beg.p, c.grp_end.p, nv, g.ax.p, g.ay.p, no, n, tx_dev, ty_dev, ntar, c.spart.p);
         STOKES_LAUNCH_CHECK();
     }

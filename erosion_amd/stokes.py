@@ -55,6 +55,8 @@ def lib():
         L.erosion_set_precision.argtypes = [ctypes.c_int]
         L.erosion_set_precision.restype = ctypes.c_int
         L.erosion_get_precision.restype = ctypes.c_int
+        L.erosion_set_mixed_switch.argtypes = [ctypes.c_double]
+        L.erosion_set_mixed_switch.restype = ctypes.c_int
         L.erosion_set_preconditioner.argtypes = [ctypes.c_int] * 3
         L.erosion_set_preconditioner.restype = ctypes.c_int
         L.erosion_get_preconditioner.argtypes = [ctypes.POINTER(ctypes.c_int)]
@@ -229,3 +231,9 @@ def set_precision(mode: int) -> None:
 
 def get_precision() -> int:
     return int(lib().erosion_get_precision())
+
+
+def set_mixed_switch(factor: float) -> None:
+    """Mixed mode: FP32 far field once the implicit residual <= factor * tol."""
+    if lib().erosion_set_mixed_switch(float(factor)) != 0:
+        raise StokesError(f"invalid mixed switch factor {factor}")
