@@ -20,7 +20,6 @@ Prints ONE JSON line on stdout (everything else goes to stderr):
 from __future__ import annotations
 
 import argparse
-import contextlib
 import copy
 import json
 import math
@@ -36,51 +35,61 @@ METRIC = "erosion steps/s at 1e-10 GMRES residual: learned+mixed vs plain FP64 o
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak
 PEAK_HBM_GBS = 8000.0
 FF_FLOP_PER_PAIR = 20.0  # canonical flops per ordered source-target pair
+# HBM bytes per FP64 far-field launch on the 100-body workload, from two
+# rocprofv3 counter passes (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors
+# from a 256 MiB device copy); the per-split partial sums are most of it.
+FF_TRAFFIC_BYTES = {"bodies100_n256": 34.6e6}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-@contextlib.contextmanager
-def quiet_stdout(workdir):
-    """Silence the reference's Fortran prints and keep its side files out of the tree."""
-    sys.stdout.flush()
-    fd = os.dup(1)
-    old = os.getcwd()
-    with open(os.devnull, "w") as dn:
-        os.dup2(dn.fileno(), 1)
-        os.chdir(workdir)
-        try:
-            yield
-        finally:
-            os.chdir(old)
-            os.dup2(fd, 1)
-            os.close(fd)
+def _reference_timings(n, m, no, msub, xx, yy):
+    """Time the reference's Fortran (run in a child process, see cpu_baseline)."""
+    import numpy as np
+    from oracle import reference as R
+
+    N = 2 * no + 2 * n * m + 3 * m
+    eta = np.random.default_rng(3).standard_normal(N)
+    R.matvec(n, m, no, xx, yy, eta, ibary=1, ifmm=1)  # warm-up (FMM tables, threads)
+    t0 = time.perf_counter()
+    reps = 2
+    for _ in range(reps):
+        R.matvec(n, m, no, xx, yy, eta, ibary=1, ifmm=1)
+    t_mv = (time.perf_counter() - t0) / reps
+    den = np.random.default_rng(4).standard_normal(2 * no + 2 * n * msub + 3 * msub) * 0.1
+    t0 = time.perf_counter()
+    R.shear_stress(n, msub, no, xx[: n * msub], yy[: n * msub], den)
+    return t_mv, reps, time.perf_counter() - t0
 
 
 def cpu_baseline(w, xx, yy, iters_per_solve):
-    """Reference CPU rate on a bounded sample, scaled to steps/s."""
+    """Reference CPU rate on a bounded sample, scaled to steps/s.
+
+    The reference's Fortran prints through its own buffered unit and writes an
+    FMM side file, so it runs in a child process with its output discarded and
+    a scratch working directory; the child writes its timings to a file."""
+    import subprocess
+
     import numpy as np
     from oracle import reference as R
 
     if not R.available():
         return None
     n, m, no = w.npts, w.nbodies, w.nouter
-    N = 2 * no + 2 * n * m + 3 * m
-    eta = np.random.default_rng(3).standard_normal(N)
     msub = min(10, m)
-    with tempfile.TemporaryDirectory() as td, quiet_stdout(td):
-        R.matvec(n, m, no, xx, yy, eta, ibary=1, ifmm=1)  # warm-up (FMM tables, threads)
-        t0 = time.perf_counter()
-        reps = 2
-        for _ in range(reps):
-            R.matvec(n, m, no, xx, yy, eta, ibary=1, ifmm=1)
-        t_mv = (time.perf_counter() - t0) / reps
-        den = np.random.default_rng(4).standard_normal(2 * no + 2 * n * msub + 3 * msub) * 0.1
-        t0 = time.perf_counter()
-        R.shear_stress(n, msub, no, xx[: n * msub], yy[: n * msub], den)
-        t_st_sub = time.perf_counter() - t0
+    with tempfile.TemporaryDirectory() as td:
+        np.save(os.path.join(td, "xx.npy"), xx)
+        np.save(os.path.join(td, "yy.npy"), yy)
+        code = (f"import sys, json, numpy as np; sys.path.insert(0, {ROOT!r}); import bench; "
+                f"xx = np.load('xx.npy'); yy = np.load('yy.npy'); "
+                f"r = bench._reference_timings({n}, {m}, {no}, {msub}, xx, yy); "
+                f"open('t.json', 'w').write(json.dumps(r))")
+        subprocess.run([sys.executable, "-c", code], cwd=td, check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        with open(os.path.join(td, "t.json")) as f:
+            t_mv, reps, t_st_sub = json.load(f)
 
     # reference stress cost in complex multiply-adds: wall->bodies, wall
     # boundary values per body, body->body pairs (stokesSolver.f:2707-2925)
@@ -99,7 +108,8 @@ def cpu_baseline(w, xx, yy, iters_per_solve):
                    f"{reps} matvec_DLP_fmmbary calls at the full workload ({t_mv:.3f} s each) and one "
                    f"computeShearStress on {msub} of the {m} bodies ({t_st_sub:.2f} s, scaled by its "
                    f"pair count to {t_st:.1f} s); step = 2 solves x {iters_per_solve:.0f} GMRES "
-                   f"iterations (the GPU's count) + 2 stresses; DGMRES orthogonalisation not counted"),
+                   f"iterations (the plain GPU path's count) + 2 stresses; DGMRES "
+                   f"orthogonalisation not counted"),
     }
 
 
@@ -117,8 +127,8 @@ def main() -> int:
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        log("bench.py: this build runs the Stokes solve on one GPU only; "
-            "multi-GPU runs are not part of it (see DESIGN.md for how the solve would shard).")
+        log("bench.py: the Stokes solve runs on one GPU only; multi-GPU runs are not "
+            "part of this implementation (the solve would shard by target rows).")
         return 2
 
     import numpy as np
@@ -127,7 +137,8 @@ def main() -> int:
     from erosion_amd.geometry import WORKLOADS, workload_geometry
     from erosion_amd.theta_len import ParamSet, getnxy
     from erosion_amd.density_stress import getstress_
-    from erosion_amd.time_stepping import PreviousDensity, rungekutta2
+    from erosion_amd.learned import WarmStart
+    from erosion_amd.time_stepping import rungekutta2
 
     try:
         import torch
@@ -151,7 +162,7 @@ def main() -> int:
     def run_variant(label, accelerated):
         """Warm-up steps, then exactly args.steps timed steps from the initial state."""
         configure(accelerated)
-        guess = PreviousDensity() if accelerated else None
+        guess = WarmStart(w.nouter, ibc=w.ibc) if accelerated else None
         th = copy.deepcopy(thld0)
         for i in range(args.warmup):
             th = rungekutta2(th, params, guess=guess, timings={})
@@ -256,8 +267,8 @@ def main() -> int:
             "workload": w.name, "bodies": w.nbodies, "points_per_body": w.npts,
             "wall_points": w.nouter, "gmres_tol": args.tol,
             "variant": "accelerated: per-curve two-level right preconditioner, FP32 far field "
-                       "once the residual is below 1e4 tol (FP64 true residual), previous-density "
-                       "initial guess",
+                       "once the residual is below 1e4 tol (FP64 true residual), initial guess "
+                       "from the previous density or, where none exists, the learned network",
             "dt": w.dt, "fixpdrop": True, "ibc": w.ibc,
             "accelerated": sa,
             "plain_fp64": sp,
@@ -280,7 +291,8 @@ def main() -> int:
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": ff_tflops / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": FF_TRAFFIC_BYTES.get(w.name),
+            "traffic_unit": "bytes per launch",
         },
         "cpu_baseline": cpu,
     }

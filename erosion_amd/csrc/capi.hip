@@ -72,8 +72,10 @@ void stokessolver_(const int64_t* npts, const int64_t* nbods, const int64_t* nou
         *nits = it;
         if (c.status == 1)
             std::fprintf(stderr,
-                         "libstokes_hip: GMRES reached maxl=%lld with relative residual %.3e\n",
-                         static_cast<long long>(*maxl), c.last_implicit_relres);
+                         "libstokes_hip: GMRES stopped after %lld iterations (maxl=%lld) with "
+                         "relative residual %.3e (true %.3e)\n",
+                         it, static_cast<long long>(*maxl), c.last_implicit_relres,
+                         c.last_true_relres);
         else if (c.status == 2)
             std::fprintf(stderr, "libstokes_hip: GMRES breakdown after %lld iterations\n", it);
     } catch (const std::exception& e) {
@@ -391,6 +393,87 @@ int erosion_precond_apply(int64_t npts, int64_t nbods, int64_t nouter, const dou
         return 0;
     } catch (const std::exception& e) {
         return record_error("erosion_precond_apply", e.what());
+    }
+}
+
+int erosion_mlp_param_count(int nin, int nhid, int nlayers, int nout, int64_t* count) {
+    try {
+        if (!count) throw stokes::Error("null argument");
+        *count = static_cast<int64_t>(stokes::mlp_param_count(nin, nhid, nlayers, nout));
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_mlp_param_count", e.what());
+    }
+}
+
+int erosion_mlp_forward(int nin, int nhid, int nlayers, int nout, const float* params,
+                        int64_t nsamples, const float* x, float* y) {
+    try {
+        if (!params || (nsamples > 0 && (!x || !y))) throw stokes::Error("null argument");
+        if (nsamples < 0 || nsamples > (int64_t{1} << 30)) throw stokes::Error("invalid sample count");
+        Context& c = stokes::ctx();
+        stokes::mlp_forward(c, nin, nhid, nlayers, nout, params, static_cast<int>(nsamples), x, y);
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_mlp_forward", e.what());
+    }
+}
+
+int erosion_mlp_gradients(int nin, int nhid, int nlayers, int nout, const float* params,
+                          int64_t nsamples, const float* x, const float* t, float* grads,
+                          double* loss) {
+    try {
+        if (!params || !x || !t || !grads || !loss) throw stokes::Error("null argument");
+        if (nsamples < 1 || nsamples > (int64_t{1} << 24)) throw stokes::Error("invalid sample count");
+        Context& c = stokes::ctx();
+        *loss = stokes::mlp_gradients(c, nin, nhid, nlayers, nout, params,
+                                      static_cast<int>(nsamples), x, t, grads);
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_mlp_gradients", e.what());
+    }
+}
+
+int erosion_mlp_train(int nin, int nhid, int nlayers, int nout, float* params, int64_t nsamples,
+                      const float* x, const float* t, int epochs, int batch, float lr,
+                      uint64_t seed, double* epoch_loss) {
+    try {
+        if (!params || !x || !t) throw stokes::Error("null argument");
+        if (nsamples < 1 || nsamples > (int64_t{1} << 26)) throw stokes::Error("invalid sample count");
+        Context& c = stokes::ctx();
+        stokes::mlp_train(c, nin, nhid, nlayers, nout, params, static_cast<int>(nsamples), x, t,
+                          epochs, batch, lr, seed, epoch_loss);
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_mlp_train", e.what());
+    }
+}
+
+int erosion_measure_copy(int64_t bytes, int64_t reps, double* ms) {
+    try {
+        if (bytes < 1 || reps < 1 || !ms) throw stokes::Error("invalid argument");
+        Context& c = stokes::ctx();
+        stokes::DevBuf<char> a, b;
+        a.ensure(static_cast<size_t>(bytes));
+        b.ensure(static_cast<size_t>(bytes));
+        STOKES_HIP(hipMemsetAsync(a.p, 1, static_cast<size_t>(bytes), c.stream));
+        hipEvent_t e0, e1;
+        STOKES_HIP(hipEventCreate(&e0));
+        STOKES_HIP(hipEventCreate(&e1));
+        STOKES_HIP(hipEventRecord(e0, c.stream));
+        for (int64_t r = 0; r < reps; ++r)
+            STOKES_HIP(hipMemcpyAsync(b.p, a.p, static_cast<size_t>(bytes),
+                                      hipMemcpyDeviceToDevice, c.stream));
+        STOKES_HIP(hipEventRecord(e1, c.stream));
+        STOKES_HIP(hipEventSynchronize(e1));
+        float t = 0.f;
+        STOKES_HIP(hipEventElapsedTime(&t, e0, e1));
+        STOKES_HIP(hipEventDestroy(e0));
+        STOKES_HIP(hipEventDestroy(e1));
+        *ms = static_cast<double>(t) / static_cast<double>(reps);
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_measure_copy", e.what());
     }
 }
 

@@ -264,6 +264,16 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl, double tol, doub
 void precond_setup(Context& c);
 void precond_apply(Context& c, const double* r_dev, double* z_dev);
 
+// mlp.hip: learned warm start network (FP32 parameters, BF16 products)
+size_t mlp_param_count(int nin, int nhid, int nlayers, int nout);
+void mlp_forward(Context& c, int nin, int nhid, int nlayers, int nout, const float* params,
+                 int nsamples, const float* x, float* y);
+double mlp_gradients(Context& c, int nin, int nhid, int nlayers, int nout, const float* params,
+                     int nsamples, const float* x, const float* t, float* grads);
+void mlp_train(Context& c, int nin, int nhid, int nlayers, int nout, float* params, int nsamples,
+               const float* x, const float* t, int epochs, int batch, float lr, uint64_t seed,
+               double* epoch_loss);
+
 // stress.hip
 void shear_stress(Context& c, const double* den_dev, double* tau_dev);
 // velocity, pressure, vorticity at targets; out[4*t + {0,1,2,3}]

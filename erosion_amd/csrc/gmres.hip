@@ -491,6 +491,7 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
     const double tol_cycle = accel ? kPcTolFrac * tol : tol;
     double* V0 = c.V.p;
     long long total_iters = 0, fp32_iters = 0;
+    double prev_true = 0.0;
     int status = 0, nreorth = 0;
     double implicit_relres = 0.0, true_relres = 0.0;
     const bool mixed = c.precision != 0;
@@ -521,7 +522,9 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
         while (k < maxl_cycle) {
             // mixed precision decides per chunk: shorter chunks switch sooner
             const int chunk = std::min(mixed ? kChunkIters / 2 : kChunkIters, maxl_cycle - k);
-            c.ff_fp32_now = c.precision == 2 || (mixed && relres_now <= c.mixed_switch * tol);
+            // after a mixed cycle missed the tolerance, later cycles stay FP64
+            c.ff_fp32_now = c.precision == 2 ||
+                            (mixed && cycle == 0 && relres_now <= c.mixed_switch * tol);
             if (c.ff_fp32_now) fp32_iters += chunk;
             for (int j = 0; j < chunk; ++j) {
                 const int kk = k + j;
@@ -607,11 +610,16 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
         nreorth += fin.nreorth;
         implicit_relres = fin.rho / fin.bnrm;
         true_relres = std::sqrt(rr) / fin.bnrm;
-        if (!accel || status != 0 || true_relres <= tol || total_iters >= maxl ||
+        // restart while the true residual misses the tolerance and the last
+        // restart still reduced it (at large sizes the FP64 evaluation of
+        // b - A x has a rounding floor near the tolerance; the implicit test
+        // above is then the reference's convergence criterion)
+        const bool stalled = cycle > 0 && true_relres > 0.5 * prev_true;
+        prev_true = true_relres;
+        if (!accel || status != 0 || true_relres <= tol || stalled || total_iters >= maxl ||
             cycle + 1 >= kMaxCycles)
             break;
     }
-    if (accel && status == 0 && true_relres > tol) status = 1;
     STOKES_HIP(hipMemcpyAsync(den_host, c.xvec.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
     STOKES_HIP(hipStreamSynchronize(s));
     c.last_iters = total_iters;

@@ -151,9 +151,10 @@ __global__ __launch_bounds__(kBlk) void k_farfield(
 // The same pair term in FP32.  Sources are stored relative to the centre of
 // their 256-point tile (FP32 offsets of a few body radii at most), each
 // target's offset from that centre is formed in FP64 once per tile and
-// rounded, so rx = x_t - x_s keeps its relative accuracy for near and far
-// pairs alike.  Pair terms are summed in FP32 over 64 sources and the
-// partial sums are accumulated in FP64.
+// rounded, so rx = x_t - x_s keeps its relative accuracy for pairs from
+// different tiles; the tile that holds a block's own targets (a curve's
+// neighbouring points) is summed in FP64.  Pair terms are summed in FP32
+// over 64 sources and the partial sums are accumulated in FP64.
 constexpr int kSub32 = 64;
 
 template <int TPT, bool CHECK>
@@ -227,11 +228,13 @@ __global__ __launch_bounds__(kBlk) void k_ff32_prep(const double* __restrict__ x
 
 template <int TPT>
 __global__ __launch_bounds__(kBlk) void k_farfield32(
-    const double* __restrict__ x, const double* __restrict__ y, const float2* __restrict__ s32xy,
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ mx,
+    const double* __restrict__ my, const float2* __restrict__ s32xy,
     const float2* __restrict__ s32m, const double2* __restrict__ tctr,
     const double* __restrict__ ex, const double* __restrict__ ey, int npts, int split_len,
     double* __restrict__ part_u, double* __restrict__ part_v) {
     __shared__ float2 sxy[kTile], sm[kTile], se[kTile];
+    __shared__ double2 dxy[kTile], dm[kTile], de[kTile];
     const int tid = threadIdx.x;
     const int tb0 = blockIdx.x * TPT * kBlk;
     const int tb1 = min(tb0 + TPT * kBlk, npts);
@@ -249,7 +252,27 @@ __global__ __launch_bounds__(kBlk) void k_farfield32(
     const int s1 = min(s0 + split_len, npts);
     for (int base = s0; base < s1; base += kTile) {
         const int j = base + tid;
+        // a tile holding the block's own targets (neighbouring points of the
+        // same curve, where FP32 offsets would cancel) is summed in FP64
+        const bool diag = (base < tb1) && (base + kTile > tb0);
         __syncthreads();
+        if (diag) {
+            if (j < s1) {
+                dxy[tid] = make_double2(x[j], y[j]);
+                dm[tid] = make_double2(mx[j], my[j]);
+                de[tid] = make_double2(ex[j], ey[j]);
+            } else {
+                dxy[tid] = make_double2(1.0e100, 1.0e100);
+                dm[tid] = make_double2(0.0, 0.0);
+                de[tid] = make_double2(0.0, 0.0);
+            }
+            __syncthreads();
+            int self[TPT];
+#pragma unroll
+            for (int q = 0; q < TPT; ++q) self[q] = t[q] - base;
+            ff_tile<TPT, true>(dxy, dm, de, xt, yt, self, u, v);
+            continue;
+        }
         sxy[tid] = s32xy[j];
         sm[tid] = s32m[j];
         se[tid] = j < s1 ? make_float2(static_cast<float>(ex[j]), static_cast<float>(ey[j]))
@@ -262,15 +285,7 @@ __global__ __launch_bounds__(kBlk) void k_farfield32(
             dx[q] = static_cast<float>(xt[q] - ctr.x);
             dy[q] = static_cast<float>(yt[q] - ctr.y);
         }
-        const bool diag = (base < tb1) && (base + kTile > tb0);
-        if (diag) {
-            int self[TPT];
-#pragma unroll
-            for (int q = 0; q < TPT; ++q) self[q] = t[q] - base;
-            ff32_tile<TPT, true>(sxy, sm, se, dx, dy, self, u, v);
-        } else {
-            ff32_tile<TPT, false>(sxy, sm, se, dx, dy, nullptr, u, v);
-        }
+        ff32_tile<TPT, false>(sxy, sm, se, dx, dy, nullptr, u, v);
     }
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
@@ -311,15 +326,18 @@ void ff32_launch(const Context& c, const FFPlan& p, double* pu, double* pv) {
     const double2* tc = reinterpret_cast<const double2*>(c.ff32_ctr.p);
     switch (p.tpt) {
     case 1:
-        k_farfield32<1><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, sxy, sm, tc, c.ex.p, c.ey.p,
+        k_farfield32<1><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, sxy, sm, tc,
+                                                     c.ex.p, c.ey.p,
                                                      g.npts, p.split_len, pu, pv);
         break;
     case 2:
-        k_farfield32<2><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, sxy, sm, tc, c.ex.p, c.ey.p,
+        k_farfield32<2><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, sxy, sm, tc,
+                                                     c.ex.p, c.ey.p,
                                                      g.npts, p.split_len, pu, pv);
         break;
     default:
-        k_farfield32<4><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, sxy, sm, tc, c.ex.p, c.ey.p,
+        k_farfield32<4><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, sxy, sm, tc,
+                                                     c.ex.p, c.ey.p,
                                                      g.npts, p.split_len, pu, pv);
         break;
     }

@@ -142,6 +142,15 @@ int erosion_kernel_timers(double* out);
  * Returns 0, or -1 for another mode. */
 int erosion_set_precision(int mode);
 int erosion_get_precision(void);
+/* Mixed mode switches the far field to FP32 once the implicit relative
+ * residual is at most factor * tol (default 1e4; factor >= 1).  Returns 0 or
+ * -1. */
+int erosion_set_mixed_switch(double factor);
+
+/* Measurement hook: `reps` device-to-device copies of `bytes` bytes on the
+ * library's stream (the reference traffic for hardware byte counters);
+ * ms = mean milliseconds per copy.  Returns 0 or -1. */
+int erosion_measure_copy(int64_t bytes, int64_t reps, double* ms);
 
 /* Right preconditioner of stokessolver_ (no counterpart in the reference,
  * whose solveBIE runs GMRES unpreconditioned, stokesSolver.f:490).
@@ -163,6 +172,31 @@ int erosion_get_preconditioner(int* out);
  * the given geometry; r and z have the den layout.  Returns 0 or -1. */
 int erosion_precond_apply(int64_t npts, int64_t nbods, int64_t nouter, const double* xx,
                           const double* yy, const double* r, double* z);
+
+/* ---- Learned warm start network (no counterpart in the reference, whose
+ * GMRES starts from zeros, DensityStress.jl:45) --------------------------
+ * A multilayer perceptron: nlayers hidden layers of nhid tanh units and a
+ * linear output, h_{l+1} = tanh(W_l h_l + b_l), y = W_L h_L + b_L.  params is
+ * one FP32 array, layer by layer, W_l (outputs x inputs, row-major) then b_l.
+ * Every product rounds both operands to BF16 (nearest even) and accumulates
+ * in FP32 on the matrix cores; bias, tanh, loss and Adam are FP32.  x is
+ * nsamples x nin, targets t and outputs y are nsamples x nout, row-major.
+ * Loss: mean squared error over samples and outputs.  All return 0 or -1. */
+int erosion_mlp_param_count(int nin, int nhid, int nlayers, int nout, int64_t* count);
+int erosion_mlp_forward(int nin, int nhid, int nlayers, int nout, const float* params,
+                        int64_t nsamples, const float* x, float* y);
+/* Gradient of the loss over all nsamples with respect to params (same
+ * layout), and the loss. */
+int erosion_mlp_gradients(int nin, int nhid, int nlayers, int nout, const float* params,
+                          int64_t nsamples, const float* x, const float* t, float* grads,
+                          double* loss);
+/* Adam (beta1 0.9, beta2 0.999, eps 1e-8) on shuffled mini-batches of
+ * `batch` samples (a final partial batch is dropped) for `epochs` passes;
+ * params is updated in place; epoch_loss[e] (if not null) is the mean batch
+ * loss of epoch e. */
+int erosion_mlp_train(int nin, int nhid, int nlayers, int nout, float* params, int64_t nsamples,
+                      const float* x, const float* t, int epochs, int batch, float lr,
+                      uint64_t seed, double* epoch_loss);
 
 #ifdef __cplusplus
 }
