@@ -166,6 +166,35 @@ void computeqoitargets_(const int64_t* npts, const int64_t* nbods, const int64_t
     }
 }
 
+void computepressure_(const int64_t* npts, const int64_t* nbods, const int64_t* nouter,
+                      const double* xx, const double* yy, const double* den, double* pressure) {
+    try {
+        if (!npts || !nbods || !nouter || !den || !pressure) throw stokes::Error("null argument");
+        check_sizes(*npts, *nbods, *nouter);
+        const int n = static_cast<int>(*npts), m = static_cast<int>(*nbods),
+                  no = static_cast<int>(*nouter);
+        const size_t nbp = static_cast<size_t>(n) * static_cast<size_t>(m);
+        std::memset(pressure, 0, sizeof(double) * nbp);
+        if (m == 0) return;
+        if (!xx || !yy) throw stokes::Error("null body coordinates");
+        Context& c = stokes::ctx();
+        c.status = 0;
+        c.last_error.clear();
+        stokes::build_geometry(c, n, m, no, xx, yy);
+        const size_t nunk = static_cast<size_t>(c.g.nunk);
+        c.rvec.ensure(std::max(nunk, c.rvec.cap));
+        c.tau.ensure(nbp);
+        STOKES_HIP(hipMemcpyAsync(c.rvec.p, den, nunk * sizeof(double), hipMemcpyHostToDevice,
+                                  c.stream));
+        stokes::pressure(c, c.rvec.p, c.tau.p);
+        STOKES_HIP(hipMemcpyAsync(pressure, c.tau.p, nbp * sizeof(double), hipMemcpyDeviceToHost,
+                                  c.stream));
+        STOKES_HIP(hipStreamSynchronize(c.stream));
+    } catch (const std::exception& e) {
+        record_error("computepressure_", e.what());
+    }
+}
+
 int erosion_set_gmres_tol(double tol) {
     try {
         if (!(tol > 0.0)) return -1;

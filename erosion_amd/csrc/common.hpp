@@ -276,6 +276,8 @@ void mlp_train(Context& c, int nin, int nhid, int nlayers, int nout, float* para
 
 // stress.hip
 void shear_stress(Context& c, const double* den_dev, double* tau_dev);
+// pressure on every body point (computePressure)
+void pressure(Context& c, const double* den_dev, double* out_dev);
 // velocity, pressure, vorticity at targets; out[4*t + {0,1,2,3}]
 void qoi_targets(Context& c, int ibary, const double* den_dev, int ntar, const double* tx_dev,
                  const double* ty_dev, double* out_dev);

@@ -411,6 +411,87 @@ __global__ __launch_bounds__(kBlk) void k_stress_final(
     tau[q] = tx * ny - ty * nx;
 }
 
+// computePressure (stokesSolver.f:4644-4882), source part: pressure at body
+// point q from the sources of one split, kernel (2 (r.n)(r.s)/rho^4 -
+// (n.s)/rho^2) speed 2/N; the wall and other bodies by the trapezoid rule,
+// the target's own body by the odd-even rule on s = eta_j - eta_q (nodes of
+// the other parity, weight doubled; :4731-4787).  grid (ceil(M n / 256),
+// splits); part[split][q].
+__global__ __launch_bounds__(kBlk) void k_pressure_pairs(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ px,
+    const double* __restrict__ py, const double* __restrict__ speed,
+    const double* __restrict__ ex, const double* __restrict__ ey, int no, int n, int m,
+    int split_len, double* __restrict__ part) {
+    const int nbp = n * m, npts = no + nbp;
+    const int q = blockIdx.x * kBlk + threadIdx.x;
+    if (q >= nbp) return;
+    const int gi = no + q, tb = q / n, tk = q - tb * n;
+    const double xt = x[gi], yt = y[gi], exq = ex[gi], eyq = ey[gi];
+    const int s0 = blockIdx.y * split_len, s1 = min(s0 + split_len, npts);
+    const double wno = 2.0 / static_cast<double>(no), wn = 2.0 / static_cast<double>(n);
+    double p = 0.0;
+    for (int j = s0; j < s1; ++j) {
+        double sx = ex[j], sy = ey[j], w;
+        if (j < no) {
+            w = speed[j] * wno;
+        } else {
+            const int sb = (j - no) / n, sk = (j - no) - sb * n;
+            w = speed[j] * wn;
+            if (sb == tb) {
+                if (((sk ^ tk) & 1) == 0) continue;  // same parity: not an odd-even partner
+                sx -= exq;
+                sy -= eyq;
+                w *= 2.0;
+            }
+        }
+        const double rx = xt - x[j], ry = yt - y[j];
+        const double rho2 = __builtin_fma(rx, rx, ry * ry);
+        const double ir = rcp64(rho2);
+        const double rdotn = __builtin_fma(rx, px[j], ry * py[j]);
+        const double rdots = __builtin_fma(rx, sx, ry * sy);
+        const double sdotn = __builtin_fma(px[j], sx, py[j] * sy);
+        p = __builtin_fma((2.0 * rdotn * rdots * ir - sdotn) * ir, w, p);
+    }
+    part[static_cast<size_t>(blockIdx.y) * nbp + q] = p;
+}
+
+// Sum of the splits, jump -(d eta/ds . t) with t = (-p_y, p_x) and the
+// spectral derivative of the body's density (:4826-4854), stokeslet pressure
+// (r.lambda)/(2 pi rho^2) of every body (:4857-4879).
+__global__ __launch_bounds__(kBlk) void k_pressure_final(
+    const double* __restrict__ part, int nsplit, const double* __restrict__ x,
+    const double* __restrict__ y, const double* __restrict__ px, const double* __restrict__ py,
+    const double* __restrict__ speed, const double* __restrict__ ex,
+    const double* __restrict__ ey, const double* __restrict__ dtab,
+    const double* __restrict__ cx, const double* __restrict__ cy,
+    const double* __restrict__ den, int no, int n, int m, double* __restrict__ out) {
+    const int nbp = n * m;
+    const int q = blockIdx.x * kBlk + threadIdx.x;
+    if (q >= nbp) return;
+    double p = 0.0;
+    for (int s = 0; s < nsplit; ++s) p += part[static_cast<size_t>(s) * nbp + q];
+    const int gi = no + q, tb = q / n, tk = q - tb * n, off = no + tb * n;
+    const double exq = ex[gi], eyq = ey[gi];
+    double dx = 0.0, dy = 0.0;
+    for (int k = 0; k < n; ++k) {
+        int idx = tk - k;
+        idx += idx < 0 ? n : 0;
+        const double w = dtab[idx];
+        dx = __builtin_fma(w, ex[off + k] - exq, dx);
+        dy = __builtin_fma(w, ey[off + k] - eyq, dy);
+    }
+    const double tx = -py[gi], ty = px[gi];
+    p = p - 1.0 * ((dx / speed[gi]) * tx + (dy / speed[gi]) * ty);
+    const double* ext = den + 2 * no + 2 * nbp;
+    const double xt = x[gi], yt = y[gi];
+    for (int b = 0; b < m; ++b) {
+        const double rx = xt - cx[b], ry = yt - cy[b];
+        const double rho2 = rx * rx + ry * ry;
+        p = p + 1.0 / kTwoPi * (rx * ext[3 * b] + ry * ext[3 * b + 1]) / rho2;
+    }
+    out[q] = p;
+}
+
 }  // namespace
 
 // Source-curve groups balanced by node count; uploads every curve's boundary
@@ -524,6 +605,25 @@ void qoi_targets(Context& c, int ibary, const double* den_dev, int ntar, const d
         STOKES_LAUNCH_CHECK();
     }
     k_qoi_final<<<cdiv_int(ntar, kBlk), kBlk, 0, s>>>(c.spart.p, G, ntar, m, c.iside.p, out_dev);
+    STOKES_LAUNCH_CHECK();
+    STOKES_HIP(hipStreamSynchronize(s));
+}
+
+void pressure(Context& c, const double* den_dev, double* out_dev) {
+    Geometry& g = c.g;
+    const int n = g.n, m = g.m, no = g.no, nbp = n * m;
+    if (nbp == 0) return;
+    hipStream_t s = c.stream;
+    gather_density(c, den_dev);
+    const int split_len = 1024;
+    const int nsplit = cdiv_int(g.npts, split_len);
+    c.spart.ensure(static_cast<size_t>(nsplit) * nbp);
+    k_pressure_pairs<<<dim3(cdiv_int(nbp, kBlk), nsplit), kBlk, 0, s>>>(
+        g.x.p, g.y.p, g.px.p, g.py.p, g.speed.p, c.ex.p, c.ey.p, no, n, m, split_len, c.spart.p);
+    STOKES_LAUNCH_CHECK();
+    k_pressure_final<<<cdiv_int(nbp, kBlk), kBlk, 0, s>>>(
+        c.spart.p, nsplit, g.x.p, g.y.p, g.px.p, g.py.p, g.speed.p, c.ex.p, c.ey.p, g.dtab_n.p,
+        g.cx.p, g.cy.p, den_dev, no, n, m, out_dev);
     STOKES_LAUNCH_CHECK();
     STOKES_HIP(hipStreamSynchronize(s));
 }

@@ -37,6 +37,8 @@ def lib():
         L.computeshearstress_.restype = None
         L.computeqoitargets_.argtypes = [_i64p] * 4 + [_dp, _dp, _dp, _i64p] + [_dp] * 6
         L.computeqoitargets_.restype = None
+        L.computepressure_.argtypes = [_i64p, _i64p, _i64p, _dp, _dp, _dp, _dp]
+        L.computepressure_.restype = None
         L.erosion_set_gmres_tol.argtypes = [ctypes.c_double]
         L.erosion_set_gmres_tol.restype = ctypes.c_int
         L.erosion_get_gmres_tol.restype = ctypes.c_double
@@ -137,6 +139,16 @@ def computeshearstress(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: i
                               _ref(ibary), _p(tau))
     _check("computeshearstress_")
     return tau
+
+
+def computepressure(npts: int, nbods: int, nouter: int, xx, yy, den) -> np.ndarray:
+    """Call computepressure_ exactly as PostProcess.jl:107-110 does."""
+    xx, yy, den = _f64(xx), _f64(yy), _f64(den)
+    out = np.zeros(npts * nbods)
+    lib().computepressure_(_ref(npts), _ref(nbods), _ref(nouter), _p(xx), _p(yy), _p(den),
+                           _p(out))
+    _check("computepressure_")
+    return out
 
 
 def time_matvec(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: int = 1,

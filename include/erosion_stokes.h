@@ -76,6 +76,16 @@ void computeqoitargets_(const int64_t* npts, const int64_t* nbods, const int64_t
                         const double* ytar, double* utar, double* vtar, double* ptar,
                         double* vortar);
 
+/* Pressure on every body point.
+ * Replaces subroutine computePressure (ErosionPackage/fortran_code/stokesSolver.f:4644-4882),
+ * bound at ErosionPackage/src/PostProcess.jl:107-110 (post-processing: drag).
+ * The wall and other bodies by the trapezoid rule, the point's own body by
+ * the odd-even rule on the density minus its value at the point, the jump
+ * term and the stokeslet pressure.  pressure[npts*nbods], body-major; zeroed
+ * on error. */
+void computepressure_(const int64_t* npts, const int64_t* nbods, const int64_t* nouter,
+                      const double* xx, const double* yy, const double* den, double* pressure);
+
 /* ---- Additions with no counterpart in the reference ------------------------
  * The reference fixes the GMRES tolerance in source (stokesSolver.f:482) and
  * has no status channel; these let a caller run at another tolerance and read

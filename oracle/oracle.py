@@ -177,3 +177,11 @@ def shear_stress(n, m, no, xx, yy, den) -> np.ndarray:
     tau = np.zeros(n * m)
     lib().orc_shear_stress(int(n), int(m), int(no), _p(xx), _p(yy), _p(den), _p(tau))
     return tau
+
+
+def pressure(n, m, no, xx, yy, den) -> np.ndarray:
+    """computePressure (stokesSolver.f:4644-4882) restated in C."""
+    xx, yy, den = _f64(xx), _f64(yy), _f64(den)
+    out = np.zeros(n * m)
+    lib().orc_pressure(int(n), int(m), int(no), _p(xx), _p(yy), _p(den), _p(out))
+    return out

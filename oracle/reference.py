@@ -227,3 +227,11 @@ def shear_stress(n, m, no, xx, yy, den, ib=1) -> np.ndarray:
     tau = np.zeros(n * m)
     lib().computeshearstress_(_i64(n), _i64(m), _i64(no), _p(xx), _p(yy), _p(den), _i64(ib), _p(tau))
     return tau
+
+
+def pressure(n, m, no, xx, yy, den) -> np.ndarray:
+    """computepressure_ called with int64 arguments as PostProcess.jl:107-110 does."""
+    xx, yy, den = _f64(xx).copy(), _f64(yy).copy(), _f64(den)
+    out = np.zeros(n * m)
+    lib().computepressure_(_i64(n), _i64(m), _i64(no), _p(xx), _p(yy), _p(den), _p(out))
+    return out

@@ -1226,3 +1226,80 @@ void orc_shear_stress(int n, int m, int no, const double *xx, const double *yy,
     free(e22);
     orc_geom_free(g);
 }
+
+/* computePressure (stokesSolver.f:4644-4882): pressure on every body point.
+ * Kernel of a source node j at a target: (2 (r.n_j)(r.s_j)/rho^4 -
+ * (n_j.s_j)/rho^2) speed_j 2pi/N / pi, with s the source density; the wall
+ * and other bodies by the trapezoid rule (:4693-4720, :4789-4819), the
+ * target's own body by the odd-even rule on s_j = eta_j - eta_k with the
+ * factor 2 (:4731-4787), then the jump -(d eta/ds . t) with d/ds =
+ * fourierDiff / speed, t = (-p_y, p_x) (:4826-4854), then the stokeslet
+ * pressure (r.lambda)/(2pi rho^2) of every body (:4857-4879; rotlets carry
+ * none). */
+static double pressure_term(double rx, double ry, double nx, double ny, double sx, double sy,
+                            double w) {
+    double rho2 = rx * rx + ry * ry;
+    double rdotn = rx * nx + ry * ny;
+    double rdots = rx * sx + ry * sy;
+    double sdotn = nx * sx + ny * sy;
+    return (2.0 * rdotn * rdots / rho2 / rho2 - sdotn / rho2) * w;
+}
+
+void orc_pressure(int n, int m, int no, const double *xx, const double *yy, const double *den,
+                  double *pressure) {
+    orc_geom *g = orc_geom_create(n, m, no, xx, yy);
+    const int nbp = n * m;
+#pragma omp parallel for OMP_NT schedule(dynamic, 16)
+    for (int q = 0; q < nbp; q++) {
+        const int tb = q / n, tk = q % n;
+        const double xt = g->x[q], yt = g->y[q];
+        double p = 0.0;
+        for (int j = 0; j < no; j++)
+            p += pressure_term(xt - g->xo[j], yt - g->yo[j], g->px0[j], g->py0[j], den[j],
+                               den[j + no], g->speed0[j] * TWOPI / (double)no / PI);
+        for (int s = 0; s < m; s++) {
+            const double *sdx = den + 2 * no + 2 * s * n;
+            const double *sdy = sdx + n;
+            if (s == tb) {
+                /* odd-even: sources of the other parity, density minus the target's */
+                for (int j = (tk % 2 == 0) ? 1 : 0; j < n; j += 2) {
+                    const int qs = s * n + j;
+                    p += 2.0 * pressure_term(xt - g->x[qs], yt - g->y[qs], g->px[qs], g->py[qs],
+                                             sdx[j] - sdx[tk], sdy[j] - sdy[tk],
+                                             g->speed[qs] * TWOPI / (double)n / PI);
+                }
+            } else {
+                for (int j = 0; j < n; j++) {
+                    const int qs = s * n + j;
+                    p += pressure_term(xt - g->x[qs], yt - g->y[qs], g->px[qs], g->py[qs], sdx[j],
+                                       sdy[j], g->speed[qs] * TWOPI / (double)n / PI);
+                }
+            }
+        }
+        pressure[q] = p;
+    }
+    cplx *z = xmalloc(sizeof(cplx) * (size_t)n);
+    for (int b = 0; b < m; b++) {
+        const double *sdx = den + 2 * no + 2 * b * n;
+        const double *sdy = sdx + n;
+        for (int k = 0; k < n; k++) z[k] = sdx[k] + I * sdy[k];
+        fourier_diff(n, z);
+        for (int k = 0; k < n; k++) {
+            const int q = b * n + k;
+            const double tx = -g->py[q], ty = g->px[q];
+            const double dsx = creal(z[k]) / g->speed[q], dsy = cimag(z[k]) / g->speed[q];
+            pressure[q] = pressure[q] - 1.0 * (dsx * tx + dsy * ty);
+        }
+    }
+    free(z);
+    const double *ext = den + 2 * no + 2 * nbp;
+    for (int b = 0; b < m; b++) {
+        const double sto1 = ext[3 * b], sto2 = ext[3 * b + 1];
+        for (int q = 0; q < nbp; q++) {
+            const double rx = g->x[q] - g->cx[b], ry = g->y[q] - g->cy[b];
+            const double rho2 = rx * rx + ry * ry;
+            pressure[q] = pressure[q] + 1.0 / TWOPI * (rx * sto1 + ry * sto2) / rho2;
+        }
+    }
+    orc_geom_free(g);
+}

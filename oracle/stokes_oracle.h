@@ -85,6 +85,10 @@ void orc_qoi_targets(int n, int m, int no, int ibary, const double *xx, const do
                      const double *den, int ntar, const double *xt, const double *yt, double *u,
                      double *v, double *p, double *vort);
 
+/* computePressure (stokesSolver.f:4644-4882): pressure on every body point. */
+void orc_pressure(int n, int m, int no, const double *xx, const double *yy, const double *den,
+                  double *pressure);
+
 /* Number of OpenMP threads the oracle uses (0 = runtime default). */
 void orc_set_threads(int nthreads);
 int orc_get_threads(void);

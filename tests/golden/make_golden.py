@@ -5,7 +5,8 @@ builds oracle/_ref/libstokes_ref.so from the reference sources with flang,
 and this script calls its routines through oracle/reference.py on seeded
 inputs, saving inputs and outputs as small .npz fixtures next to this file.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py                  # every fixture
+    python tests/golden/make_golden.py --pressure-only  # add computePressure
 
 Fixtures (all FP64):
   bodies10_n128.npz  10 bodies x 128, wall 512 (bodies far apart: the
@@ -66,6 +67,8 @@ def solver_case(name, n, m, no, xx, yy, ibc, maxl, seed):
     out["qoi_xt"], out["qoi_yt"] = xt, yt
     out["qoi_bary"] = np.array(R.qoi_targets(n, m, no, 1, xx, yy, den12, xt, yt))
     out["qoi_trap"] = np.array(R.qoi_targets(n, m, no, 0, xx, yy, den12, xt, yt))
+    # computePressure on the body points (PostProcess.jl:107-110)
+    out["pressure_tol12"] = R.pressure(n, m, no, xx, yy, den12)
     np.savez_compressed(os.path.join(HERE, name + ".npz"), **out)
     print(f"{name}: N={N} iters tol8={it8} tol12={it12} trap={it0}")
 
@@ -97,9 +100,24 @@ def dlp_case(name, seed):
     print(f"{name}: {len(xe)} exterior, {len(xi)} interior targets")
 
 
+def add_pressure(name):
+    """Add computePressure of the stored tol-1e-12 density to an existing fixture."""
+    path = os.path.join(HERE, name + ".npz")
+    with np.load(path, allow_pickle=False) as z:
+        out = {k: z[k] for k in z.files}
+    n, m, no = int(out["n"]), int(out["m"]), int(out["no"])
+    out["pressure_tol12"] = R.pressure(n, m, no, out["xx"], out["yy"], out["den_tol12"])
+    np.savez_compressed(path, **out)
+    print(f"{name}: pressure added")
+
+
 def main():
     if not R.available():
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    if "--pressure-only" in sys.argv:
+        for name in ("bodies10_n128", "tight8_n64"):
+            add_pressure(name)
+        return
     w = WORKLOADS["bodies10_n128"]
     n, m, no, xx, yy = workload_xy(w)
     solver_case("bodies10_n128", n, m, no, xx, yy, ibc=0, maxl=2000, seed=11)

@@ -186,3 +186,36 @@ def test_pipe_flow_pressure_drop():
     assert abs(pdrop - 8.0) < 0.4
     ys = -1 + (np.arange(13) + 0.5) * 2 / 13
     assert abs(qavg - np.mean(1 - ys ** 2)) < 0.05  # Poiseuille profile sampled at the slice
+
+
+@pytest.mark.parametrize("name", FIXTURES)
+def test_pressure_matches_reference(golden, name):
+    """computepressure_ against the reference's computePressure (PostProcess.jl:107-110)."""
+    g = golden(name)
+    n, m, no, xx, yy = _case(g)
+    p = stokes.computepressure(n, m, no, xx, yy, g["den_tol12"])
+    assert relerr(p, g["pressure_tol12"]) < 1e-10
+    assert relerr(p, O.pressure(n, m, no, xx, yy, g["den_tol12"])) < 1e-10
+
+
+def test_pressure_full_size_and_drag():
+    """100 bodies x 256: GPU pressure against the oracle, and the drag
+    bookkeeping of PostProcess.jl:130-160 on top of the GPU stress and pressure."""
+    from erosion_amd.post_process import drag
+    from erosion_amd.theta_len import ParamSet
+    from erosion_amd.geometry import workload_geometry
+    w = WORKLOADS["bodies100_n256"]
+    n, m, no, xx, yy = workload_xy(w)
+    N = 2 * no + 2 * n * m + 3 * m
+    den = np.random.default_rng(5).standard_normal(N)
+    p = stokes.computepressure(n, m, no, xx, yy, den)
+    po = O.pressure(n, m, no, xx, yy, den)
+    assert relerr(p, po) < 1e-10
+    th = workload_geometry(w, nbodies=10)
+    params = ParamSet(npts=n, nouter=no, ibary=1, ifmm=1, ibc=0, dt=1e-3, maxl=4000,
+                      fixpdrop=False)
+    d = drag(th, params)
+    assert d.press_all.shape == (n, 10) and np.all(np.isfinite(d.press_all))
+    print(f"drag on 10 bodies: pressure ({d.pdragx:.4f}, {d.pdragy:.4f}), "
+          f"viscous ({d.vdragx:.4f}, {d.vdragy:.4f}), umax {d.umax:.4f}")
+    assert np.isfinite([d.pdragx, d.pdragy, d.vdragx, d.vdragy]).all() and d.pdragx != 0.0

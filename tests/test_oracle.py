@@ -284,3 +284,14 @@ def test_qoi_targets_match_reference(golden, name, ibary, key):
     m = int(g["m"])
     centres = slice(26, 26 + m)
     assert np.all(np.array(out)[:, centres] == 0.0)
+
+
+@pytest.mark.parametrize("name", ["tight8_n64", "bodies10_n128"])
+def test_pressure_matches_reference(golden, name):
+    """computePressure (stokesSolver.f:4644-4882) against the reference's own build."""
+    g = golden(name)
+    n, m, no = int(g["n"]), int(g["m"]), int(g["no"])
+    p = O.pressure(n, m, no, g["xx"], g["yy"], g["den_tol12"])
+    # the densities reach 1e3 while the pressure is O(10): cancellation bounds
+    # the agreement of two FP64 evaluations (FFT versus direct derivative)
+    assert relerr(p, g["pressure_tol12"]) < 1e-10
