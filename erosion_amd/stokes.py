@@ -85,6 +85,14 @@ def nunk(npts: int, nbods: int, nouter: int) -> int:
     return 2 * nouter + 2 * npts * nbods + 3 * nbods
 
 
+def _lengths(npts: int, nbods: int, nouter: int, xx, yy, den=None) -> None:
+    """The library reads exactly these lengths from the caller's buffers."""
+    if len(xx) < npts * nbods or len(yy) < npts * nbods:
+        raise StokesError(f"xx, yy hold {len(xx)}, {len(yy)} values; need {npts * nbods}")
+    if den is not None and len(den) < nunk(npts, nbods, nouter):
+        raise StokesError(f"density holds {len(den)} values; need {nunk(npts, nbods, nouter)}")
+
+
 def last_status() -> int:
     return int(lib().erosion_last_status())
 
@@ -123,6 +131,7 @@ def stokessolver(npts: int, nbods: int, nouter: int, ifmm: int, ibary: int, ibc:
 
     ``den`` is the initial guess (zeros when omitted); returns (den, nits)."""
     xx, yy = _f64(xx), _f64(yy)
+    _lengths(npts, nbods, nouter, xx, yy, den)
     out = np.zeros(nunk(npts, nbods, nouter)) if den is None else _f64(den).copy()
     nits = ctypes.c_int64(0)
     lib().stokessolver_(_ref(npts), _ref(nbods), _ref(nouter), _ref(ifmm), _ref(ibary), _ref(ibc),
@@ -134,6 +143,7 @@ def stokessolver(npts: int, nbods: int, nouter: int, ifmm: int, ibary: int, ibc:
 def computeshearstress(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: int = 1) -> np.ndarray:
     """Call computeshearstress_ exactly as DensityStress.jl:195-198 does."""
     xx, yy, den = _f64(xx), _f64(yy), _f64(den)
+    _lengths(npts, nbods, nouter, xx, yy, den)
     tau = np.zeros(npts * nbods)
     lib().computeshearstress_(_ref(npts), _ref(nbods), _ref(nouter), _p(xx), _p(yy), _p(den),
                               _ref(ibary), _p(tau))
@@ -144,6 +154,7 @@ def computeshearstress(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: i
 def computepressure(npts: int, nbods: int, nouter: int, xx, yy, den) -> np.ndarray:
     """Call computepressure_ exactly as PostProcess.jl:107-110 does."""
     xx, yy, den = _f64(xx), _f64(yy), _f64(den)
+    _lengths(npts, nbods, nouter, xx, yy, den)
     out = np.zeros(npts * nbods)
     lib().computepressure_(_ref(npts), _ref(nbods), _ref(nouter), _p(xx), _p(yy), _p(den),
                            _p(out))
@@ -155,6 +166,7 @@ def time_matvec(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: int = 1,
                 reps: int = 10) -> dict:
     """Operator and far-field kernel durations measured with HIP events."""
     xx, yy, den = _f64(xx), _f64(yy), _f64(den)
+    _lengths(npts, nbods, nouter, xx, yy, den)
     out = np.zeros(8)
     if lib().erosion_time_matvec(npts, nbods, nouter, ibary, _p(xx), _p(yy), _p(den), int(reps),
                                  _p(out)) != 0:
@@ -184,6 +196,9 @@ def kernel_timers() -> dict:
 def computeqoitargets(npts: int, nbods: int, nouter: int, ibary: int, xx, yy, den, xtar, ytar):
     """Call computeqoitargets_ exactly as DensityStress.jl:81-86 does; returns (u, v, p, vort)."""
     xx, yy, den, xtar, ytar = map(_f64, (xx, yy, den, xtar, ytar))
+    _lengths(npts, nbods, nouter, xx, yy, den)
+    if len(ytar) != len(xtar):
+        raise StokesError("xtar and ytar differ in length")
     nt = len(xtar)
     outs = [np.zeros(nt) for _ in range(4)]
     lib().computeqoitargets_(_ref(npts), _ref(nbods), _ref(nouter), _ref(ibary), _p(xx), _p(yy),
@@ -195,6 +210,7 @@ def computeqoitargets(npts: int, nbods: int, nouter: int, ibary: int, xx, yy, de
 def matvec(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: int = 1) -> np.ndarray:
     """One application of the completed double-layer operator."""
     xx, yy, den = _f64(xx), _f64(yy), _f64(den)
+    _lengths(npts, nbods, nouter, xx, yy, den)
     vel = np.zeros(nunk(npts, nbods, nouter))
     if lib().erosion_matvec(npts, nbods, nouter, ibary, _p(xx), _p(yy), _p(den), _p(vel)) != 0:
         raise StokesError(f"erosion_matvec: {last_error()}")
@@ -222,6 +238,7 @@ def get_preconditioner() -> Tuple[int, int, int]:
 def precond_apply(npts: int, nbods: int, nouter: int, xx, yy, r) -> np.ndarray:
     """z = P^{-1} r for the selected preconditioner on this geometry (test hook)."""
     xx, yy, r = _f64(xx), _f64(yy), _f64(r)
+    _lengths(npts, nbods, nouter, xx, yy, r)
     z = np.zeros(nunk(npts, nbods, nouter))
     if lib().erosion_precond_apply(npts, nbods, nouter, _p(xx), _p(yy), _p(r), _p(z)) != 0:
         raise StokesError(f"erosion_precond_apply: {last_error()}")

@@ -211,9 +211,14 @@ def test_pressure_full_size_and_drag():
     p = stokes.computepressure(n, m, no, xx, yy, den)
     po = O.pressure(n, m, no, xx, yy, den)
     assert relerr(p, po) < 1e-10
+    from erosion_amd.density_stress import compute_density_
     th = workload_geometry(w, nbodies=10)
     params = ParamSet(npts=n, nouter=no, ibary=1, ifmm=1, ibc=0, dt=1e-3, maxl=4000,
                       fixpdrop=False)
+    # as in PostProcess.jl, drag reads a density computed beforehand
+    with pytest.raises(stokes.StokesError):
+        drag(th, params)
+    compute_density_(th, params)
     d = drag(th, params)
     assert d.press_all.shape == (n, 10) and np.all(np.isfinite(d.press_all))
     print(f"drag on 10 bodies: pressure ({d.pdragx:.4f}, {d.pdragy:.4f}), "
