@@ -195,6 +195,43 @@ void computepressure_(const int64_t* npts, const int64_t* nbods, const int64_t* 
     }
 }
 
+int erosion_solve_batch(int64_t ngeom, const int64_t* npts, const int64_t* nbods,
+                        const int64_t* nouter, const double* const* xx, const double* const* yy,
+                        double* const* den, double* const* tau, int64_t ibary, int64_t ibc,
+                        int64_t maxl, int64_t nworkers, int64_t* nits, int* status) {
+    try {
+        if (ngeom < 0 || ngeom > (int64_t{1} << 20)) throw stokes::Error("invalid geometry count");
+        if (ngeom == 0) return 0;
+        if (!npts || !nbods || !nouter || !xx || !yy || !den || !nits || !status)
+            throw stokes::Error("null argument");
+        if (ibary != 0 && ibary != 1) throw stokes::Error("ibary must be 0 or 1");
+        if (tau && ibary != 1) throw stokes::Error("shear stress needs ibary=1");
+        if (nworkers < 1 || nworkers > 64) throw stokes::Error("nworkers must lie in [1, 64]");
+        std::vector<stokes::BatchGeometry> geo(static_cast<size_t>(ngeom));
+        for (int64_t i = 0; i < ngeom; ++i) {
+            check_sizes(npts[i], nbods[i], nouter[i]);
+            if (!den[i] || (nbods[i] > 0 && (!xx[i] || !yy[i])))
+                throw stokes::Error("null array of geometry " + std::to_string(i));
+            geo[static_cast<size_t>(i)] = stokes::BatchGeometry{
+                static_cast<int>(npts[i]), static_cast<int>(nbods[i]), static_cast<int>(nouter[i]),
+                xx[i], yy[i], den[i], tau ? tau[i] : nullptr, nits + i, status + i};
+            nits[i] = 0;
+            status[i] = -1;
+        }
+        Context& c = stokes::ctx();
+        c.status = 0;
+        c.last_error.clear();
+        std::string err;
+        stokes::solve_batch(c, static_cast<int>(ngeom), geo.data(), static_cast<int>(ibary),
+                            static_cast<int>(ibc), static_cast<int>(maxl),
+                            static_cast<int>(nworkers), &err);
+        if (!err.empty()) return record_error("erosion_solve_batch", err);
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_solve_batch", e.what());
+    }
+}
+
 int erosion_set_gmres_tol(double tol) {
     try {
         if (!(tol > 0.0)) return -1;

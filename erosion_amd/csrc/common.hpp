@@ -264,6 +264,18 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl, double tol, doub
 void precond_setup(Context& c);
 void precond_apply(Context& c, const double* r_dev, double* z_dev);
 
+// batch.hip: independent geometries solved by a pool of worker contexts
+struct BatchGeometry {
+    int n, m, no;
+    const double *xx, *yy;
+    double* den;     // in: initial guess; out: density
+    double* tau;     // shear stress out, or null
+    int64_t* nits;
+    int* status;     // 0 ok, 1 not converged, 2 breakdown, -1 error
+};
+void solve_batch(Context& settings, int ngeom, const BatchGeometry* geo, int ibary, int ibc,
+                 int maxl, int nworkers, std::string* first_error);
+
 // mlp.hip: learned warm start network (FP32 parameters, BF16 products)
 size_t mlp_param_count(int nin, int nhid, int nlayers, int nout);
 void mlp_forward(Context& c, int nin, int nhid, int nlayers, int nout, const float* params,

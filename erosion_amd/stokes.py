@@ -266,3 +266,41 @@ def set_mixed_switch(factor: float) -> None:
     """Mixed mode: FP32 far field once the implicit residual <= factor * tol."""
     if lib().erosion_set_mixed_switch(float(factor)) != 0:
         raise StokesError(f"invalid mixed switch factor {factor}")
+
+
+def solve_batch(geoms, ibary: int = 1, ibc: int = 0, maxl: int = 8000, workers: int = 4,
+                stress: bool = False, guesses=None):
+    """Solve many independent geometries in one call (erosion_solve_batch).
+
+    ``geoms`` is a sequence of (npts, nbods, nouter, xx, yy); returns a list of
+    dicts with den, nits, status and (with ``stress``) tau per geometry."""
+    k = len(geoms)
+    n = np.array([g[0] for g in geoms], dtype=np.int64)
+    m = np.array([g[1] for g in geoms], dtype=np.int64)
+    no = np.array([g[2] for g in geoms], dtype=np.int64)
+    xs = [_f64(g[3]) for g in geoms]
+    ys = [_f64(g[4]) for g in geoms]
+    for i in range(k):
+        _lengths(int(n[i]), int(m[i]), int(no[i]), xs[i], ys[i])
+    dens = [np.zeros(nunk(int(n[i]), int(m[i]), int(no[i]))) if guesses is None or guesses[i] is None
+            else _f64(guesses[i]).copy() for i in range(k)]
+    taus = [np.zeros(int(n[i] * m[i])) for i in range(k)] if stress else None
+    P = ctypes.POINTER(ctypes.c_double) * k
+    px = P(*[_p(a) for a in xs])
+    py = P(*[_p(a) for a in ys])
+    pd = P(*[_p(a) for a in dens])
+    pt = P(*[_p(a) for a in taus]) if stress else None
+    nits = np.zeros(k, dtype=np.int64)
+    status = np.zeros(k, dtype=np.int32)
+    L = lib()
+    L.erosion_solve_batch.restype = ctypes.c_int
+    rc = L.erosion_solve_batch(ctypes.c_int64(k), n.ctypes.data_as(_i64p), m.ctypes.data_as(_i64p),
+                               no.ctypes.data_as(_i64p), px, py, pd, pt, ctypes.c_int64(ibary),
+                               ctypes.c_int64(ibc), ctypes.c_int64(maxl), ctypes.c_int64(workers),
+                               nits.ctypes.data_as(_i64p),
+                               status.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    out = [dict(den=dens[i], nits=int(nits[i]), status=int(status[i]),
+                tau=taus[i] if stress else None) for i in range(k)]
+    if rc != 0 and all(o["status"] != 0 for o in out):
+        raise StokesError(f"erosion_solve_batch: {last_error()}")
+    return out

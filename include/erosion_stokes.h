@@ -157,6 +157,21 @@ int erosion_get_precision(void);
  * -1. */
 int erosion_set_mixed_switch(double factor);
 
+/* Many independent geometries in one call (no counterpart in the
+ * reference).  Geometry i has npts[i], nbods[i], nouter[i] and the arrays
+ * xx[i], yy[i] (npts*nbods) and den[i] (initial guess in, density out);
+ * tau[i] (npts*nbods) receives its shear stress when tau is not null
+ * (ibary must then be 1).  The solves use the current tolerance,
+ * preconditioner and precision, run on `nworkers` worker contexts (own
+ * stream and buffers each) from as many host threads, and overlap on the
+ * GPU.  nits[i] and status[i] (0 ok, 1 not converged, 2 breakdown, -1 error)
+ * are per geometry.  Returns 0, or -1 if any geometry failed (first failure
+ * in erosion_last_error). */
+int erosion_solve_batch(int64_t ngeom, const int64_t* npts, const int64_t* nbods,
+                        const int64_t* nouter, const double* const* xx, const double* const* yy,
+                        double* const* den, double* const* tau, int64_t ibary, int64_t ibc,
+                        int64_t maxl, int64_t nworkers, int64_t* nits, int* status);
+
 /* Measurement hook: `reps` device-to-device copies of `bytes` bytes on the
  * library's stream (the reference traffic for hardware byte counters);
  * ms = mean milliseconds per copy.  Returns 0 or -1. */
