@@ -267,13 +267,14 @@ class LearnedGuess:
 
 
 class WarmStart:
-    """The initial guess a time stepper uses: the most recent density while
-    the body count is unchanged, the trained network's guess otherwise (the
-    first solve of a run, after a deletion), zeros without weights."""
+    """The initial guess a time stepper uses: extrapolated from the most
+    recent densities while the body count is unchanged, the trained network's
+    guess otherwise (the first solve of a run, after a deletion), zeros
+    without weights."""
 
-    def __init__(self, nouter: int, path: str = WEIGHTS, ibc: int = 0):
+    def __init__(self, nouter: int, path: str = WEIGHTS, ibc: int = 0, order: int = 2):
         from .time_stepping import PreviousDensity
-        self.prev = PreviousDensity()
+        self.prev = PreviousDensity(order)
         self.nouter = nouter
         self.net = LearnedGuess(path, nouter=nouter, ibc=ibc) if os.path.exists(path) else None
         self.learned_used = 0

@@ -131,3 +131,32 @@ def test_rk2_step_matches_per_body_formulas(monkeypatch):
         assert np.abs(t.theta - th).max() < 1e-14
         assert abs(t.len - ln) < 1e-14 and abs(t.xsm - xs) < 1e-15 and abs(t.ysm - ys) < 1e-15
     assert th1.tt == pytest.approx(1e-3)
+
+
+def test_previous_density_extrapolation():
+    """order 2 returns 2 x_k - x_(k-1) of the last two densities of states
+    with the same body count, order 1 the last one; a body-count change
+    empties the history."""
+    from erosion_amd.time_stepping import PreviousDensity
+    thl = make_bodies(3, 16, 0.05, 0.1, 0.05, 1)
+    n, m, no = 16, 3, 8
+    N = 2 * no + 2 * n * m + 3 * m
+
+    def state(v, bodies=thl):
+        s = new_thlenden(list(bodies))
+        s.density = np.full(2 * no + 2 * n * len(bodies) + 3 * len(bodies), float(v))
+        return s
+
+    g1, g2 = PreviousDensity(1), PreviousDensity(2)
+    assert g2(state(0)) is None
+    for g in (g1, g2):
+        g.seen(state(1.0))
+    assert np.all(g2(state(0)) == 1.0)          # one density: no extrapolation yet
+    for g in (g1, g2):
+        g.seen(state(1.5))
+    assert np.all(g1(state(0)) == 1.5)
+    assert np.all(g2(state(0)) == 2.0) and len(g2(state(0))) == N
+    fewer = thl[:2]
+    assert g2(state(0, fewer)) is None          # body count changed
+    g2.seen(state(7.0, fewer))
+    assert np.all(g2(state(0, fewer)) == 7.0)   # history restarted
