@@ -35,6 +35,7 @@ METRIC = "erosion steps/s at 1e-10 GMRES residual: learned+mixed vs plain FP64 o
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak
 PEAK_HBM_GBS = 8000.0
 FF_FLOP_PER_PAIR = 20.0  # canonical flops per ordered source-target pair
+STRESS_FLOP_PER_PAIR = 130.0  # per (target body point, source node) pair: 3 passes x 5 fields
 # HBM bytes per FP64 far-field launch on the 100-body workload, from two
 # rocprofv3 counter passes (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors
 # from a 256 MiB device copy); the per-split partial sums are most of it.
@@ -123,6 +124,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-plain", action="store_true",
                     help="time only the accelerated variant (for profiling runs)")
+    ap.add_argument("--graphs", action="store_true",
+                    help="replay GMRES iteration chunks as HIP graphs in the accelerated "
+                         "variant; kernel timing then runs on one extra untimed step")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -158,6 +162,7 @@ def main() -> int:
     def configure(accelerated):
         stokes.set_preconditioner(1 if accelerated else 0)
         stokes.set_precision(stokes.PRECISION_MIXED if accelerated else stokes.PRECISION_FP64)
+        stokes.set_graphs(bool(accelerated and args.graphs))
 
     def run_variant(label, accelerated):
         """Warm-up steps, then exactly args.steps timed steps from the initial state."""
@@ -168,7 +173,9 @@ def main() -> int:
             th = rungekutta2(th, params, guess=guess, timings={})
             log(f"[{label}] warmup step {i + 1}/{args.warmup}: bodies={len(th.thlenvec)}")
         sync()
-        stokes.kernel_timing(True)
+        # kernel timing bypasses graph replay: with graphs it runs on one more step
+        graphs = accelerated and args.graphs
+        stokes.kernel_timing(not graphs)
         timings, solve_info = {}, []
         t0 = time.perf_counter()
         for i in range(args.steps):
@@ -180,6 +187,10 @@ def main() -> int:
                 f"last true relres {solve_info[-1]['true_relres']:.2e}")
         sync()
         elapsed = time.perf_counter() - t0
+        if graphs:
+            log(f"[{label}] graph replay: {stokes.graph_stats()}")
+            stokes.kernel_timing(True)
+            rungekutta2(copy.deepcopy(th), params, guess=copy.deepcopy(guess), timings={})
         kt = stokes.kernel_timers()
         stokes.kernel_timing(False)
         configure(False)
@@ -187,7 +198,7 @@ def main() -> int:
             ", ".join(f"{k} {v:.3f}" if isinstance(v, float) else f"{k} {v}"
                       for k, v in timings.items()))
         return {"elapsed": elapsed, "timings": timings, "kt": kt, "solve_info": solve_info,
-                "state": th}
+                "state": th, "graphs": graphs}
 
     def first_stress(accelerated):
         """Shear stress of the initial state through one variant's solve."""
@@ -215,7 +226,10 @@ def main() -> int:
                 "gmres_iters_per_solve": r["timings"].get("gmres_iters", 0) / solves,
                 "true_relres_max": max(si["true_relres"] for si in r["solve_info"]),
                 "fp32_iters_last_solve": r["solve_info"][-1]["fp32_iters"],
-                "matvecs_per_s": r["kt"]["farfield"]["launches"] / r["elapsed"]}
+                # operator applications (FP64 and FP32 far field); with graphs
+                # the kernel counts come from one extra step, scaled
+                "matvecs_per_s": (r["kt"]["farfield"]["launches"] + r["kt"]["farfield32"]["launches"])
+                * (args.steps if r["graphs"] else 1) / r["elapsed"]}
 
     sp, sa = (summary(plain) if plain else None), summary(acc)
     elapsed, kt = acc["elapsed"], acc["kt"]
@@ -232,6 +246,10 @@ def main() -> int:
     gs = kt["gram_schmidt"]
     gs_gbs = gs["work"] / (gs["ms"] * 1e-3) / 1e9 if gs["ms"] > 0 else 0.0
     se = kt["stress_eval"]
+    se_tflops = (STRESS_FLOP_PER_PAIR * se["work"] / (se["ms"] * 1e-3) / 1e12
+                 if se["ms"] > 0 else None)
+    nr = kt["near"]
+    near_gbs = nr["work"] / (nr["ms"] * 1e-3) / 1e9 if nr["ms"] > 0 else None
     share = {k: kt[k]["ms"] / (elapsed * 1e3) for k in kt}
     log(f"plain-fp64: {sp}")
     log(f"accelerated: {sa}")
@@ -241,7 +259,10 @@ def main() -> int:
         log(f"FP32 far field: {f32_ms:.3f} ms/launch, {f32_tflops:.2f} TFLOP/s over "
             f"{f32['launches']} launches")
     log(f"far field: {ff_ms:.3f} ms/launch, {ff_tflops:.2f} TFLOP/s; Gram-Schmidt {gs_gbs:.0f} GB/s; "
-        f"stress eval {se['ms'] / max(1, se['launches']):.2f} ms/launch")
+        f"stress eval {se['ms'] / max(1, se['launches']):.2f} ms/launch"
+        + (f", {se_tflops:.2f} TFLOP/s" if se_tflops else "")
+        + (f"; near-pair gather {near_gbs:.0f} GB/s over {nr['launches']} passes"
+           if near_gbs else "; no near pairs"))
 
     cpu = None
     if not args.no_cpu_baseline and iters_per_solve:
@@ -270,6 +291,7 @@ def main() -> int:
                        "once the residual is below 1e4 tol (FP64 true residual), initial guess "
                        "from the previous density or, where none exists, the learned network",
             "dt": w.dt, "fixpdrop": True, "ibc": w.ibc,
+            "hip_graphs": bool(args.graphs),
             "accelerated": sa,
             "plain_fp64": sp,
             "speedup_vs_plain_fp64": sa["steps_per_s"] / sp["steps_per_s"] if sp else None,
@@ -281,6 +303,9 @@ def main() -> int:
             "farfield32_TFLOPs": f32_tflops,
             "farfield32_launches": f32["launches"],
             "gram_schmidt_GBps": gs_gbs,
+            "stress_eval_TFLOPs": se_tflops,
+            "near_gather_GBps": near_gbs,
+            "near_gather_launches": nr["launches"],
             "kernel_share": share,
         },
         "roofline": {

@@ -44,6 +44,8 @@ void solve_batch(Context& settings, int ngeom, const BatchGeometry* geo, int iba
         c->pc_kind = settings.pc_kind;
         c->pc_body_modes = settings.pc_body_modes;
         c->pc_wall_modes = settings.pc_wall_modes;
+        c->use_graphs = settings.use_graphs;
+        if (!c->use_graphs) c->graphs.clear();
         ws.push_back(c);
     }
     std::atomic<int> next{0};

@@ -367,6 +367,7 @@ int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t iba
         out[3] = static_cast<double>(c.near.nentries);
         out[4] = c.t_near.launches ? c.t_near.ms / static_cast<double>(c.t_near.launches) : 0.0;
         out[5] = static_cast<double>(c.ff_splits);
+        out[6] = c.near.gather_bytes;
         return 0;
     } catch (const std::exception& e) {
         Context& c = stokes::ctx();
@@ -419,6 +420,32 @@ int erosion_set_preconditioner(int kind, int body_modes, int wall_modes) {
         return 0;
     } catch (const std::exception& e) {
         return record_error("erosion_set_preconditioner", e.what());
+    }
+}
+
+int erosion_set_graphs(int on) {
+    try {
+        if (on != 0 && on != 1) return -1;
+        Context& c = stokes::ctx();
+        c.use_graphs = on;
+        if (!on) c.graphs.clear();
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_set_graphs", e.what());
+    }
+}
+
+int erosion_graph_stats(long long* out) {
+    if (!out) return -1;
+    try {
+        Context& c = stokes::ctx();
+        out[0] = c.use_graphs;
+        out[1] = c.graphs.captures;
+        out[2] = c.graphs.replays;
+        out[3] = static_cast<long long>(c.graphs.exec.size());
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_graph_stats", e.what());
     }
 }
 

@@ -5,6 +5,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -65,6 +67,11 @@ __device__ __forceinline__ void cfma(cplx& acc, cplx a, cplx b) {
 }
 
 // ------------------------------------------------------------ buffers
+// Count of device-buffer allocations in the process.  A captured graph holds
+// raw device pointers, so a graph is replayed only while this count is the one
+// it was captured under.
+inline std::atomic<unsigned long long> g_alloc_epoch{0};
+
 // Owning device allocation; grows only.
 template <typename T>
 struct DevBuf {
@@ -75,6 +82,7 @@ struct DevBuf {
         release();
         size_t c = count ? count : 1;
         STOKES_HIP(hipMalloc(reinterpret_cast<void**>(&p), c * sizeof(T)));
+        g_alloc_epoch.fetch_add(1, std::memory_order_relaxed);
         cap = c;
     }
     void release() {
@@ -162,12 +170,14 @@ struct KernelTimer {
 // an entry is (source curve, target point); interior rule iff source is wall.
 struct NearList {
     int nentries = 0;
-    int ntasks = 0;
     int nactive = 0;                 // source curves with at least one entry
     int wall_active = 0;             // 1 if the wall is a source (active[0] == 0)
-    int wall_tasks = 0;              // tasks with the wall as source (listed first)
+    // algorithmic bytes of one near-evaluation pass: per (target, source curve)
+    // entry, the source's nodes x 14 doubles (2 coordinates, 2 dz, 10
+    // boundary-value components) plus 16 bytes written
+    double gather_bytes = 0.0;
     DevBuf<int> ent_tgt;             // target point per entry (sorted by source)
-    DevBuf<int> task_src, task_beg, task_cnt;  // <=64 entries of one source
+    DevBuf<int> ent_src;             // source curve per entry
     DevBuf<int> tgt_ptr, tgt_ent;    // CSR: entries of each target point
     DevBuf<int> active;              // active source curves
     DevBuf<double> corr_u, corr_v;   // per-entry correction (bary - trapezoid)
@@ -182,6 +192,23 @@ struct GmresState {
     int done;       // 1 converged, 2 stopped (maxl / breakdown)
     int reorth;     // unused (every iteration runs two Gram-Schmidt passes)
     int nreorth;    // iterations with a second Gram-Schmidt pass
+};
+
+// GMRES iteration chunks captured as HIP graphs.  `sig` lists every value the
+// captured launches bake in besides the chunk key (sizes, grid shapes, the
+// allocation epoch); a different signature drops every graph.
+struct GraphCache {
+    std::vector<long long> sig;
+    std::map<std::array<long long, 4>, hipGraphExec_t> exec;  // (k0, chunk, fp32, maxl)
+    long long captures = 0, replays = 0;
+    void clear() {
+        for (auto& kv : exec) (void)hipGraphExecDestroy(kv.second);
+        exec.clear();
+    }
+    GraphCache() = default;
+    GraphCache(const GraphCache&) = delete;
+    GraphCache& operator=(const GraphCache&) = delete;
+    ~GraphCache() { clear(); }
 };
 
 struct Context {
@@ -235,6 +262,9 @@ struct Context {
     DevBuf<double> pc_phi_b, pc_phi_w, pc_S, pc_Gb, pc_Gw, pc_z;
     // per-kernel timing with HIP events on `stream` (measurement hooks only)
     bool time_kernels = false;
+    // replay GMRES iteration chunks as HIP graphs (off while timing kernels)
+    int use_graphs = 0;
+    GraphCache graphs;
     KernelTimer t_farfield, t_near, t_stress_eval, t_gs, t_precond, t_farfield32;
     int status = 0;
     std::string last_error;
@@ -253,6 +283,13 @@ void build_near_list(Context& c, int ibary);
 
 // operator.hip : y = A x for the completed double-layer operator
 void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary);
+// run the geometry-dependent set-up apply_operator would otherwise do inside
+// its first call (far-field tuning, FP32 source tiles, stokeslet logarithms),
+// so a captured sequence of operator applications holds none of it; x_dev is
+// a density the tuning launches may read (the solve's initial guess)
+void prepare_operator(Context& c, const double* x_dev, bool fp32);
+// far-field shape index in use for the current size (after prepare_operator)
+int farfield_choice(const Context& c, bool fp32);
 void gather_density(Context& c, const double* den_dev);
 void boundary_values(Context& c, const int* curves_dev, int ncurves, int wall_first);
 void bd_condition(Context& c, int ibc, double* rhs_dev);

@@ -315,17 +315,13 @@ void build_near_list(Context& c, int ibary) {
             }
         }
     }
-    // flatten: entries sorted by source curve; tasks of <= 64 entries
-    std::vector<int> ent_tgt, task_src, task_beg, task_cnt, active;
+    // flatten: entries sorted by source curve
+    std::vector<int> ent_tgt, ent_src, active;
     for (int src = 0; src <= m; ++src) {
         auto& v = by_src[static_cast<size_t>(src)];
         if (v.empty()) continue;
         active.push_back(src);
-        for (size_t b = 0; b < v.size(); b += 64) {
-            task_src.push_back(src);
-            task_beg.push_back(static_cast<int>(ent_tgt.size() + b));
-            task_cnt.push_back(static_cast<int>(std::min<size_t>(64, v.size() - b)));
-        }
+        ent_src.insert(ent_src.end(), v.size(), src);
         ent_tgt.insert(ent_tgt.end(), v.begin(), v.end());
     }
     const int ne = static_cast<int>(ent_tgt.size());
@@ -338,10 +334,12 @@ void build_near_list(Context& c, int ibary) {
             tgt_ent[static_cast<size_t>(fill[static_cast<size_t>(ent_tgt[static_cast<size_t>(e)])]++)] = e;
     }
     L.nentries = ne;
-    L.ntasks = static_cast<int>(task_src.size());
     L.nactive = static_cast<int>(active.size());
     L.wall_active = by_src[0].empty() ? 0 : 1;
-    L.wall_tasks = static_cast<int>((by_src[0].size() + 63) / 64);
+    {
+        const double wall_ent = static_cast<double>(by_src[0].size());
+        L.gather_bytes = 14.0 * 8.0 * (wall_ent * no + (ne - wall_ent) * n) + 16.0 * ne;
+    }
     auto up = [&](DevBuf<int>& d, const std::vector<int>& h) {
         d.ensure(std::max<size_t>(h.size(), 1));
         if (!h.empty())
@@ -349,9 +347,7 @@ void build_near_list(Context& c, int ibary) {
                                       c.stream));
     };
     up(L.ent_tgt, ent_tgt);
-    up(L.task_src, task_src);
-    up(L.task_beg, task_beg);
-    up(L.task_cnt, task_cnt);
+    up(L.ent_src, ent_src);
     up(L.tgt_ptr, tgt_ptr);
     up(L.tgt_ent, tgt_ent);
     up(L.active, active);

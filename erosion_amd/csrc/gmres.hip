@@ -19,6 +19,7 @@
 // The host enqueues chunks of iterations and reads the device state once per
 // chunk; kernels of iterations after convergence return immediately.
 #include <algorithm>
+#include <cstring>
 #include <chrono>
 #include <cmath>
 
@@ -427,6 +428,56 @@ static void gs_pass(Context& c, double* w, int nrows, int pass, double* hcur) {
     STOKES_LAUNCH_CHECK();
 }
 
+// Everything a captured chunk of iterations bakes in besides its key: the
+// allocation epoch (device pointers), sizes, the preconditioner's shape, the
+// far-field plans and the near-pair launch shapes.
+static std::vector<long long> chunk_signature(const Context& c, int ibary, bool pc, int ldh,
+                                              double tol_cycle) {
+    const Geometry& g = c.g;
+    const NearList& L = c.near;
+    long long tol_bits = 0;
+    static_assert(sizeof(tol_bits) == sizeof(tol_cycle), "double is 64-bit");
+    std::memcpy(&tol_bits, &tol_cycle, sizeof(tol_bits));
+    return {static_cast<long long>(g_alloc_epoch.load(std::memory_order_relaxed)),
+            g.n, g.m, g.no, g.npts, g.nunk, c.npad, ibary, pc ? 1 : 0, c.pc_nfb, c.pc_nfw,
+            farfield_choice(c, false), farfield_choice(c, true), L.nentries, L.nactive,
+            L.wall_active, ldh, tol_bits};
+}
+
+// Launch one chunk of iterations: replay its graph if one was captured under
+// the same signature, else capture the launches `run` makes, then replay.
+template <typename Run>
+static void replay_chunk(Context& c, const std::array<long long, 4>& key,
+                         const std::vector<long long>& sig, Run&& run) {
+    GraphCache& gc = c.graphs;
+    if (sig != gc.sig) {
+        gc.clear();
+        gc.sig = sig;
+    }
+    auto it = gc.exec.find(key);
+    if (it == gc.exec.end()) {
+        hipGraph_t graph = nullptr;
+        STOKES_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+        try {
+            run();
+        } catch (...) {
+            // leave the stream out of capture mode before reporting the error
+            (void)hipStreamEndCapture(c.stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        STOKES_HIP(hipStreamEndCapture(c.stream, &graph));
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        STOKES_HIP(e);
+        it = gc.exec.emplace(key, exec).first;
+        ++gc.captures;
+    }
+    STOKES_HIP(hipGraphLaunch(it->second, c.stream));
+    ++gc.replays;
+}
+
 long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, double* den_host) {
     auto t_start = std::chrono::steady_clock::now();
     Geometry& g = c.g;
@@ -487,6 +538,12 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
     // field the cycle aims at half the tolerance and, should the true residual
     // of the returned density still exceed the tolerance, GMRES restarts from
     // it (at most kMaxCycles cycles within maxl iterations in total).
+    if (c.use_graphs) {
+        // set-up work an operator application would otherwise launch inside
+        // a captured chunk (and then not repeat on a later replay)
+        prepare_operator(c, c.xvec.p, false);
+        if (c.precision != 0) prepare_operator(c, c.xvec.p, true);
+    }
     const bool accel = pc || c.precision != 0;
     const double tol_cycle = accel ? kPcTolFrac * tol : tol;
     double* V0 = c.V.p;
@@ -526,29 +583,40 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
             c.ff_fp32_now = c.precision == 2 ||
                             (mixed && cycle == 0 && relres_now <= c.mixed_switch * tol);
             if (c.ff_fp32_now) fp32_iters += chunk;
-            for (int j = 0; j < chunk; ++j) {
-                const int kk = k + j;
-                double* vk = c.V.p + static_cast<size_t>(kk) * npad;
-                double* w = c.V.p + static_cast<size_t>(kk + 1) * npad;
-                if (pc) {
-                    precond_apply(c, vk, c.pc_z.p);
-                    apply_operator(c, c.pc_z.p, w, ibary);
-                } else {
-                    apply_operator(c, vk, w, ibary);
+            // iterations k .. k+chunk-1 (each exits early on the device once
+            // the state reports convergence)
+            auto run_chunk = [&]() {
+                for (int j = 0; j < chunk; ++j) {
+                    const int kk = k + j;
+                    double* vk = c.V.p + static_cast<size_t>(kk) * npad;
+                    double* w = c.V.p + static_cast<size_t>(kk + 1) * npad;
+                    if (pc) {
+                        precond_apply(c, vk, c.pc_z.p);
+                        apply_operator(c, c.pc_z.p, w, ibary);
+                    } else {
+                        apply_operator(c, vk, w, ibary);
+                    }
+                    if (c.time_kernels) c.t_gs.start(s);
+                    gs_pass(c, w, kk + 1, 1, hcur.p);
+                    gs_pass(c, w, kk + 1, 2, hcur.p);
+                    k_givens<<<1, kGivT, 0, s>>>(c.H.p, ldh, c.q.p, c.hbuf.p, c.npart.p, ebx,
+                                                 c.st.p, tol_cycle, maxl_cycle);
+                    STOKES_LAUNCH_CHECK();
+                    if (c.time_kernels) {
+                        // Krylov rows streamed by the dot and update kernels of both passes
+                        c.t_gs.stop(s, 4.0 * static_cast<double>(kk + 1) *
+                                           static_cast<double>(npad) * 8.0);
+                    }
+                    k_scale_row<<<std::min(1024, ebx * 2), kBlk, 0, s>>>(w, npad, c.st.p, 0);
+                    STOKES_LAUNCH_CHECK();
                 }
-                if (c.time_kernels) c.t_gs.start(s);
-                gs_pass(c, w, kk + 1, 1, hcur.p);
-                gs_pass(c, w, kk + 1, 2, hcur.p);
-                k_givens<<<1, kGivT, 0, s>>>(c.H.p, ldh, c.q.p, c.hbuf.p, c.npart.p, ebx,
-                                             c.st.p, tol_cycle, maxl_cycle);
-                STOKES_LAUNCH_CHECK();
-                if (c.time_kernels) {
-                    // Krylov rows streamed by the dot and update kernels of both passes
-                    c.t_gs.stop(s, 4.0 * static_cast<double>(kk + 1) * static_cast<double>(npad) * 8.0);
-                }
-                k_scale_row<<<std::min(1024, ebx * 2), kBlk, 0, s>>>(w, npad, c.st.p, 0);
-                STOKES_LAUNCH_CHECK();
-            }
+            };
+            if (c.use_graphs && !c.time_kernels)
+                replay_chunk(c, {static_cast<long long>(k), chunk, c.ff_fp32_now ? 1 : 0,
+                                 maxl_cycle},
+                             chunk_signature(c, ibary, pc, ldh, tol_cycle), run_chunk);
+            else
+                run_chunk();
             k += chunk;
             STOKES_HIP(hipMemcpyAsync(c.h_st, c.st.p, sizeof(GmresState), hipMemcpyDeviceToHost, s));
             STOKES_HIP(hipStreamSynchronize(s));

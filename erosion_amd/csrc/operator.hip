@@ -17,8 +17,8 @@
 //   k_fields       the five Cauchy densities per point
 //   k_bdval        boundary values of the Cauchy integrals of the active
 //                  source curves, node-sliced across waves      (VALU bound)
-//   k_near_eval    barycentric value/derivative at near targets minus the
-//                  trapezoid part, one block per <=64 targets of one source
+//   k_near_wave    barycentric value/derivative at near targets minus the
+//                  trapezoid part, one wave per (target, source curve) entry
 //   k_reduce_small rank-one integral and constraint rows
 //   k_assemble     partial sums + near corrections + local terms + stokeslets
 #include <algorithm>
@@ -478,89 +478,76 @@ __global__ __launch_bounds__(64 * NSL) void k_bdval(
 }
 
 // ------------------------------------------------------- near evaluation
-// Barycentric value (pass 1) accumulators.
-struct Pass1 {
-    cplx S0, Nf[5];
-    double tu, tv;
-};
+// One wave per near entry (target point, source curve): the lanes stride over
+// the source's nodes and combine their sums with cross-lane butterflies, so
+// the launch has one wave per entry however few entries share a source (a
+// handful of targets near a 4096-point wall no longer sit on a few blocks).
+// Pass 1: S0 = sum dz/(z-x), N_f = sum b_f dz/(z-x) (5 fields) and the
+// trapezoid pair sum the correction replaces; pass 2: the derivative sums
+// M_f = sum (b_f - u_f) dz/(z-x)^2 of fields 0..2 and, for an exterior
+// source, D_a = sum dz/((z-a)(z-x)).
+constexpr int kNearWaves = 4;
 
-template <int NS>
-__global__ __launch_bounds__(64 * NS) void k_near_eval(
-    const int* __restrict__ task_src, const int* __restrict__ task_beg,
-    const int* __restrict__ task_cnt, const int* __restrict__ ent_tgt, NodeView nv,
-    const double* __restrict__ ax, const double* __restrict__ ay, int no, int n, int task0,
+__device__ __forceinline__ double wave_sum64(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__global__ __launch_bounds__(64 * kNearWaves) void k_near_wave(
+    const int* __restrict__ ent_src, const int* __restrict__ ent_tgt, int nentries, NodeView nv,
+    const double* __restrict__ ax, const double* __restrict__ ay, int no, int n,
     double* __restrict__ corr_u, double* __restrict__ corr_v) {
-    __shared__ double red[NS][64];
-    const int task = task0 + blockIdx.x;
-    const int src = task_src[task], beg = task_beg[task], cnt = task_cnt[task];
+    const int lane = threadIdx.x & 63;
+    const int e = blockIdx.x * kNearWaves + (threadIdx.x >> 6);
+    if (e >= nentries) return;  // uniform per wave
+    const int src = ent_src[e], tp = ent_tgt[e];
     const int len = src == 0 ? no : n;
     const int off = src == 0 ? 0 : no + (src - 1) * n;
     const bool ext = src != 0;
-    const int lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
-    const bool active = lane < cnt;
-    const int tp = ent_tgt[beg + (active ? lane : 0)];
     const double xt = nv.x[tp], yt = nv.y[tp];
     const cplx zt = cmk(xt, yt);
     const cplx a = cmk(ax[src], ay[src]);
     const size_t N = nv.N;
-    const int k0 = (slice * len) / NS, k1 = ((slice + 1) * len) / NS;
 
-    // pass 1: S0 = sum dz/(z-x), N_f = sum b_f dz/(z-x); trapezoid part
-    Pass1 p1;
-    p1.S0 = cmk(0, 0);
+    double v1[14];
 #pragma unroll
-    for (int f = 0; f < 5; ++f) p1.Nf[f] = cmk(0, 0);
-    p1.tu = p1.tv = 0.0;
-    for (int k = k0; k < k1; ++k) {
+    for (int q = 0; q < 14; ++q) v1[q] = 0.0;
+    for (int k = lane; k < len; k += 64) {
         const int p = off + k;
         const cplx d = cmk(nv.x[p] - xt, nv.y[p] - yt);
         const double rho2 = __builtin_fma(d.r, d.r, d.i * d.i);
         const double irho2 = rcp64(rho2);
         const cplx id = cmk(d.r * irho2, -d.i * irho2);
         const cplx w = cmul(cmk(nv.dzr[p], nv.dzi[p]), id);
-        p1.S0 = cadd(p1.S0, w);
+        v1[0] += w.r;
+        v1[1] += w.i;
 #pragma unroll
-        for (int f = 0; f < 5; ++f) cfma(p1.Nf[f], cmk(nv.bd[(2 * f) * N + p], nv.bd[(2 * f + 1) * N + p]), w);
+        for (int f = 0; f < 5; ++f) {
+            const cplx b = cmk(nv.bd[(2 * f) * N + p], nv.bd[(2 * f + 1) * N + p]);
+            v1[2 + 2 * f] = __builtin_fma(b.r, w.r, __builtin_fma(-b.i, w.i, v1[2 + 2 * f]));
+            v1[3 + 2 * f] = __builtin_fma(b.r, w.i, __builtin_fma(b.i, w.r, v1[3 + 2 * f]));
+        }
         // trapezoid pair with r = x_t - z_k = -d
         const double rx = -d.r, ry = -d.i;
         const double aa = __builtin_fma(rx, nv.mx[p], ry * nv.my[p]);
         const double bb = __builtin_fma(rx, nv.ex[p], ry * nv.ey[p]);
         const double cc = (aa * bb) * (irho2 * irho2);
-        p1.tu = __builtin_fma(cc, rx, p1.tu);
-        p1.tv = __builtin_fma(cc, ry, p1.tv);
+        v1[12] = __builtin_fma(cc, rx, v1[12]);
+        v1[13] = __builtin_fma(cc, ry, v1[13]);
     }
-    double v1[14];
-    v1[0] = p1.S0.r;
-    v1[1] = p1.S0.i;
 #pragma unroll
-    for (int f = 0; f < 5; ++f) {
-        v1[2 + 2 * f] = p1.Nf[f].r;
-        v1[3 + 2 * f] = p1.Nf[f].i;
-    }
-    v1[12] = p1.tu;
-    v1[13] = p1.tv;
-#pragma unroll
-    for (int q = 0; q < 14; ++q) {
-        red[slice][lane] = v1[q];
-        __syncthreads();
-        double s = 0.0;
-        for (int t = 0; t < NS; ++t) s += red[t][lane];
-        v1[q] = s;
-        __syncthreads();
-    }
+    for (int q = 0; q < 14; ++q) v1[q] = wave_sum64(v1[q]);
     const cplx S0 = cmk(v1[0], v1[1]);
     // exterior: num/(den - 2 pi i) (stokesSolver.f:5481); interior: num/den
-    const cplx dv = ext ? cmk(S0.r, S0.i - kTwoPi) : S0;
-    const cplx idv = cinv(dv);
+    const cplx idv = cinv(ext ? cmk(S0.r, S0.i - kTwoPi) : S0);
     cplx u[5];
 #pragma unroll
     for (int f = 0; f < 5; ++f) u[f] = cmul(cmk(v1[2 + 2 * f], v1[3 + 2 * f]), idv);
 
-    // pass 2 (fields 0..2): M_f = sum (b_f - u_f) dz/(z-x)^2;
-    // exterior also D_a = sum dz/((z-a)(z-x))
-    cplx M[3] = {cmk(0, 0), cmk(0, 0), cmk(0, 0)};
-    cplx Da = cmk(0, 0);
-    for (int k = k0; k < k1; ++k) {
+    double v2[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) v2[q] = 0.0;
+    for (int k = lane; k < len; k += 64) {
         const int p = off + k;
         const cplx d = cmk(nv.x[p] - xt, nv.y[p] - yt);
         const double rho2 = __builtin_fma(d.r, d.r, d.i * d.i);
@@ -569,42 +556,38 @@ __global__ __launch_bounds__(64 * NS) void k_near_eval(
         const cplx w = cmul(cmk(nv.dzr[p], nv.dzi[p]), id);
         if (ext && !(sqrt(rho2) > 1.0e-10)) {
 #pragma unroll
-            for (int f = 0; f < 3; ++f) M[f] = cadd(M[f], coincident_term(nv, off, len, k, f, zt, a, d));
+            for (int f = 0; f < 3; ++f) {
+                const cplx t = coincident_term(nv, off, len, k, f, zt, a, d);
+                v2[2 * f] += t.r;
+                v2[2 * f + 1] += t.i;
+            }
         } else {
             const cplx w2 = cmul(w, id);
 #pragma unroll
-            for (int f = 0; f < 3; ++f)
-                cfma(M[f], csub(cmk(nv.bd[(2 * f) * N + p], nv.bd[(2 * f + 1) * N + p]), u[f]), w2);
+            for (int f = 0; f < 3; ++f) {
+                const cplx t = csub(cmk(nv.bd[(2 * f) * N + p], nv.bd[(2 * f + 1) * N + p]), u[f]);
+                v2[2 * f] = __builtin_fma(t.r, w2.r, __builtin_fma(-t.i, w2.i, v2[2 * f]));
+                v2[2 * f + 1] = __builtin_fma(t.r, w2.i, __builtin_fma(t.i, w2.r, v2[2 * f + 1]));
+            }
         }
-        if (ext) cfma(Da, w, cmk(nv.iar[p], nv.iai[p]));
+        if (ext) {
+            const cplx wa = cmul(w, cmk(nv.iar[p], nv.iai[p]));
+            v2[6] += wa.r;
+            v2[7] += wa.i;
+        }
     }
-    double v2[8] = {M[0].r, M[0].i, M[1].r, M[1].i, M[2].r, M[2].i, Da.r, Da.i};
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        red[slice][lane] = v2[q];
-        __syncthreads();
-        double s = 0.0;
-        for (int t = 0; t < NS; ++t) s += red[t][lane];
-        v2[q] = s;
-        __syncthreads();
-    }
-    if (slice != 0 || !active) return;
+    for (int q = 0; q < 8; ++q) v2[q] = wave_sum64(v2[q]);
+    if (lane != 0) return;
+    const cplx iden = cinv(ext ? cmul(csub(zt, a), cmk(v2[6], v2[7])) : S0);  // (x-a) D_a or S0
     cplx du[3];
-    if (ext) {
-        const cplx den = cmul(csub(zt, a), cmk(v2[6], v2[7]));  // (x-a) D_a
-        const cplx iden = cinv(den);
 #pragma unroll
-        for (int f = 0; f < 3; ++f) du[f] = cmul(cmk(v2[2 * f], v2[2 * f + 1]), iden);
-    } else {
-        const cplx iS0 = cinv(S0);
-#pragma unroll
-        for (int f = 0; f < 3; ++f) du[f] = cmul(cmk(v2[2 * f], v2[2 * f + 1]), iS0);
-    }
+    for (int f = 0; f < 3; ++f) du[f] = cmul(cmk(v2[2 * f], v2[2 * f + 1]), iden);
     // u = Re v1' - x Re v2' - y Re v3' + Re v4 ; v = -Im v1' + x Im v2' + y Im v3' + Re v5
     const double ux = du[0].r - xt * du[1].r - yt * du[2].r + u[3].r;
     const double uy = -du[0].i + xt * du[1].i + yt * du[2].i + u[4].r;
-    corr_u[beg + lane] = ux - v1[12];
-    corr_v[beg + lane] = uy - v1[13];
+    corr_u[e] = ux - v1[12];
+    corr_v[e] = uy - v1[13];
 }
 
 // ------------------------------------------------- small reductions
@@ -853,6 +836,53 @@ static int tune_farfield(Context& c, bool fp32) {
     return best;
 }
 
+// FP32 far-field source tiles of the current geometry (once per geometry).
+static void ff32_sources(Context& c) {
+    Geometry& g = c.g;
+    if (c.ff32_version == g.version) return;
+    const int np = g.npts;
+    const int ntiles = cdiv_int(np, kTile);
+    c.ff32_ctr.ensure(2 * static_cast<size_t>(ntiles));
+    c.ff32_xy.ensure(2 * static_cast<size_t>(ntiles) * kTile);
+    c.ff32_m.ensure(2 * static_cast<size_t>(ntiles) * kTile);
+    k_ff32_prep<<<ntiles, kBlk, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, np,
+                                               reinterpret_cast<double2*>(c.ff32_ctr.p),
+                                               reinterpret_cast<float2*>(c.ff32_xy.p),
+                                               reinterpret_cast<float2*>(c.ff32_m.p));
+    STOKES_LAUNCH_CHECK();
+    c.ff32_version = g.version;
+}
+
+// Stokeslet logarithms of the current geometry (once per geometry).
+static void stokes_logs(Context& c) {
+    Geometry& g = c.g;
+    if (g.m == 0 || c.stk_log_version == g.version) return;
+    const size_t tot = static_cast<size_t>(g.npts) * static_cast<size_t>(g.m);
+    c.stk_log.ensure(tot);
+    k_stokes_logs<<<static_cast<int>((tot + kBlk - 1) / kBlk), kBlk, 0, c.stream>>>(
+        g.x.p, g.y.p, g.cx.p, g.cy.p, g.npts, g.m, c.stk_log.p);
+    STOKES_LAUNCH_CHECK();
+    c.stk_log_version = g.version;
+}
+
+void prepare_operator(Context& c, const double* x_dev, bool fp32) {
+    c.scal.ensure(8);
+    // the tuning launches read the density in point order: gather (and on
+    // the first call allocate) it before timing any far-field shape
+    gather_density(c, x_dev);
+    if (fp32) ff32_sources(c);
+    const FFPlan plan = ff_plan(c.g.npts, kFFShapes[tune_farfield(c, fp32)]);
+    c.part_u.ensure(static_cast<size_t>(c.g.npts) * static_cast<size_t>(plan.nsplit));
+    c.part_v.ensure(static_cast<size_t>(c.g.npts) * static_cast<size_t>(plan.nsplit));
+    stokes_logs(c);
+}
+
+int farfield_choice(const Context& c, bool fp32) {
+    const std::map<int, int>& choice = fp32 ? c.ff32_choice : c.ff_choice;
+    auto it = choice.find(c.g.npts);
+    return it == choice.end() ? -1 : it->second;
+}
+
 void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary) {
     Geometry& g = c.g;
     NearList& L = c.near;
@@ -863,18 +893,7 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary) {
 
     gather_density(c, x_dev);
     const bool fp32 = c.ff_fp32_now;
-    if (fp32 && c.ff32_version != g.version) {
-        const int ntiles = cdiv_int(np, kTile);
-        c.ff32_ctr.ensure(2 * static_cast<size_t>(ntiles));
-        c.ff32_xy.ensure(2 * static_cast<size_t>(ntiles) * kTile);
-        c.ff32_m.ensure(2 * static_cast<size_t>(ntiles) * kTile);
-        k_ff32_prep<<<ntiles, kBlk, 0, s>>>(g.x.p, g.y.p, g.mx.p, g.my.p, np,
-                                            reinterpret_cast<double2*>(c.ff32_ctr.p),
-                                            reinterpret_cast<float2*>(c.ff32_xy.p),
-                                            reinterpret_cast<float2*>(c.ff32_m.p));
-        STOKES_LAUNCH_CHECK();
-        c.ff32_version = g.version;
-    }
+    if (fp32) ff32_sources(c);
     const FFPlan plan = ff_plan(np, kFFShapes[tune_farfield(c, fp32)]);
     const int nsplit = plan.nsplit;
     c.ff_splits = nsplit;
@@ -889,39 +908,22 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary) {
         ff_launch(c, plan, c.part_u.p, c.part_v.p);
     if (c.time_kernels) tff.stop(s, static_cast<double>(np) * static_cast<double>(np - 1));
     const bool has_near = ibary == 1 && L.nentries > 0;
-    if (has_near && c.time_kernels) c.t_near.start(s);
     if (has_near) {
         boundary_values(c, L.active.p, L.nactive, L.wall_active);
         NodeView nv{g.x.p, g.y.p, g.dzr.p, g.dzi.p, c.bd.p, g.iar.p, g.iai.p, g.mx.p, g.my.p,
                     c.ex.p, c.ey.p, N};
-        if (L.wall_tasks > 0) {
-            k_near_eval<16><<<L.wall_tasks, 1024, 0, s>>>(L.task_src.p, L.task_beg.p,
-                                                          L.task_cnt.p, L.ent_tgt.p, nv, g.ax.p,
-                                                          g.ay.p, g.no, g.n, 0, L.corr_u.p,
-                                                          L.corr_v.p);
-            STOKES_LAUNCH_CHECK();
-        }
-        if (L.ntasks > L.wall_tasks) {
-            k_near_eval<4><<<L.ntasks - L.wall_tasks, 256, 0, s>>>(
-                L.task_src.p, L.task_beg.p, L.task_cnt.p, L.ent_tgt.p, nv, g.ax.p, g.ay.p, g.no,
-                g.n, L.wall_tasks, L.corr_u.p, L.corr_v.p);
-            STOKES_LAUNCH_CHECK();
-        }
+        // the timer covers the near-evaluation (gather) pass only
+        if (c.time_kernels) c.t_near.start(s);
+        k_near_wave<<<cdiv_int(L.nentries, kNearWaves), 64 * kNearWaves, 0, s>>>(
+            L.ent_src.p, L.ent_tgt.p, L.nentries, nv, g.ax.p, g.ay.p, g.no, g.n, L.corr_u.p,
+            L.corr_v.p);
+        STOKES_LAUNCH_CHECK();
     }
-    if (has_near && c.time_kernels) {
-        c.t_near.stop(s, static_cast<double>(L.nentries));
-    }
+    if (has_near && c.time_kernels) c.t_near.stop(s, L.gather_bytes);
     k_reduce_small<<<g.m + 1, kBlk, 0, s>>>(g.x.p, g.y.p, g.px.p, g.py.p, g.speed.p, c.ex.p,
                                             c.ey.p, x_dev, g.no, g.n, g.m, c.scal.p, y_dev);
     STOKES_LAUNCH_CHECK();
-    if (g.m > 0 && c.stk_log_version != g.version) {
-        c.stk_log.ensure(N * static_cast<size_t>(g.m));
-        const size_t tot = N * static_cast<size_t>(g.m);
-        k_stokes_logs<<<static_cast<int>((tot + kBlk - 1) / kBlk), kBlk, 0, s>>>(
-            g.x.p, g.y.p, g.cx.p, g.cy.p, np, g.m, c.stk_log.p);
-        STOKES_LAUNCH_CHECK();
-        c.stk_log_version = g.version;
-    }
+    stokes_logs(c);
     k_assemble<<<cdiv_int(np, kBlk / kAsmSub), kBlk, 0, s>>>(
         g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p, c.ex.p, c.ey.p, g.cx.p, g.cy.p, x_dev,
         c.part_u.p, c.part_v.p, nsplit, L.tgt_ptr.p, L.tgt_ent.p, L.corr_u.p, L.corr_v.p,

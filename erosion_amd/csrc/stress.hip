@@ -123,7 +123,10 @@ __global__ __launch_bounds__(kBT) void k_bary_eval(
                 const double ir = rcp64(rho2);
                 const cplx id = cmk(d.r * ir, -d.i * ir);
                 const cplx w = cmul(cmk(T.dzr[k], T.dzi[k]), id);
-                if (ext && !(sqrt(rho2) > 1.0e-10)) {
+                // lanes that skip this curve (their own body, when a block
+                // holds targets of two bodies) sit exactly on its nodes: they
+                // discard the sums, so they must not take the serial branch
+                if (ext && !skip && !(sqrt(rho2) > 1.0e-10)) {
 #pragma unroll
                     for (int f = 0; f < 5; ++f) {
                         cplx t = coincident_term(nv, off, len, t0 + k, f, zt, a, d);

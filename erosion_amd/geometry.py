@@ -56,6 +56,9 @@ WORKLOADS: Dict[str, Workload] = {
     "bodies200_n512": Workload("bodies200_n512", 200, 512, 4096, 0.01, 0.03, 0.01, 1003, steps=100),
     # 400 bodies x 256 points, wall 4096 (learned-warm-start workload).
     "bodies400_n256": Workload("bodies400_n256", 400, 256, 4096, 0.015, 0.035, 0.015, 1005, steps=1),
+    # 60 bodies x 128 points packed 0.004 apart, wall 1024: most bodies have
+    # near neighbours, so the barycentric near-pair pass does real work.
+    "bodies60_n128_tight": Workload("bodies60_n128_tight", 60, 128, 1024, 0.05, 0.10, 0.004, 1006),
 }
 
 

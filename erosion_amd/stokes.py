@@ -63,6 +63,10 @@ def lib():
         L.erosion_set_preconditioner.restype = ctypes.c_int
         L.erosion_get_preconditioner.argtypes = [ctypes.POINTER(ctypes.c_int)]
         L.erosion_get_preconditioner.restype = ctypes.c_int
+        L.erosion_set_graphs.argtypes = [ctypes.c_int]
+        L.erosion_set_graphs.restype = ctypes.c_int
+        L.erosion_graph_stats.argtypes = [ctypes.POINTER(ctypes.c_longlong)]
+        L.erosion_graph_stats.restype = ctypes.c_int
         L.erosion_precond_apply.argtypes = [ctypes.c_int64] * 3 + [_dp] * 4
         L.erosion_precond_apply.restype = ctypes.c_int
         _lib = L
@@ -172,7 +176,7 @@ def time_matvec(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: int = 1,
                                  _p(out)) != 0:
         raise StokesError(f"erosion_time_matvec: {last_error()}")
     return {"matvec_ms": out[0], "farfield_ms": out[1], "pairs": out[2], "near_entries": int(out[3]),
-            "near_ms": out[4], "splits": int(out[5])}
+            "near_ms": out[4], "splits": int(out[5]), "gather_bytes": out[6]}
 
 
 TIMER_FAMILIES = ("farfield", "near", "stress_eval", "gram_schmidt", "precond", "farfield32")
@@ -227,6 +231,21 @@ def set_preconditioner(kind: int, body_modes: int = 8, wall_modes: int = 16) -> 
     if lib().erosion_set_preconditioner(int(kind), int(body_modes), int(wall_modes)) != 0:
         raise StokesError(f"invalid preconditioner kind={kind} body_modes={body_modes} "
                           f"wall_modes={wall_modes}")
+
+
+def set_graphs(on: bool) -> None:
+    """Capture GMRES iteration chunks as HIP graphs and replay them on later
+    solves (results identical to direct launches)."""
+    if lib().erosion_set_graphs(1 if on else 0) != 0:
+        raise StokesError(f"erosion_set_graphs: {last_error()}")
+
+
+def graph_stats() -> dict:
+    out = (ctypes.c_longlong * 4)()
+    if lib().erosion_graph_stats(out) != 0:
+        raise StokesError(f"erosion_graph_stats: {last_error()}")
+    return {"on": bool(out[0]), "captures": int(out[1]), "replays": int(out[2]),
+            "kept": int(out[3])}
 
 
 def get_preconditioner() -> Tuple[int, int, int]:

@@ -119,8 +119,10 @@ int erosion_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t ibary,
  * applications with HIP events on the library's stream.
  * out[0] ms per application, out[1] ms per far-field kernel launch, out[2]
  * ordered source-target pairs per far-field launch, out[3] near-pair entries,
- * out[4] ms per near-pair correction (boundary values + evaluation), out[5]
- * far-field source splits.  Returns 0 or -1. */
+ * out[4] ms per near-pair evaluation (gather) pass, boundary values
+ * excluded, out[5] far-field source splits, out[6] algorithmic bytes of one
+ * gather pass (per near entry, the source curve's nodes x 14 doubles plus 16
+ * bytes written).  `out` holds 7 doubles.  Returns 0 or -1. */
 int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t ibary,
                         const double* xx, const double* yy, const double* den, int64_t reps,
                         double* out);
@@ -134,8 +136,8 @@ int erosion_kernel_timing(int enable);
 /* Totals since timing was enabled, three values per family in the order
  * far field (FP64), near pairs, stress evaluation, Gram-Schmidt,
  * preconditioner, far field (FP32): out[3i] milliseconds, out[3i+1] launches,
- * out[3i+2] algorithmic work (ordered source-target pairs; near entries;
- * target-node pairs; bytes of Krylov basis streamed; bytes of vector read and
+ * out[3i+2] algorithmic work (ordered source-target pairs; bytes of the
+ * near-pair gather pass, as erosion_time_matvec's out[6]; target-node pairs; bytes of Krylov basis streamed; bytes of vector read and
  * written; ordered source-target pairs).  `out` holds 18 doubles.  Returns 0
  * or -1. */
 int erosion_kernel_timers(double* out);
@@ -192,6 +194,20 @@ int erosion_measure_copy(int64_t bytes, int64_t reps, double* ms);
 int erosion_set_preconditioner(int kind, int body_modes, int wall_modes);
 /* out[0] kind, out[1] body modes, out[2] wall modes.  Returns 0 or -1. */
 int erosion_get_preconditioner(int* out);
+
+/* HIP graphs for GMRES (no counterpart in the reference).  on = 1: stokessolver_
+ * captures each chunk of iterations (8 or 16 iterations of preconditioner,
+ * operator, two Gram-Schmidt passes, Givens update) into a HIP graph the first
+ * time that chunk runs, and replays it on later solves while every size,
+ * launch shape and device buffer it holds is unchanged; any change drops the
+ * kept graphs.  The results are those of on = 0, bit for bit.  Kernel timing
+ * (erosion_kernel_timing) bypasses the graphs.  on = 0 (default) launches
+ * every kernel directly and frees the kept graphs.  Returns 0, or -1 for
+ * another value. */
+int erosion_set_graphs(int on);
+/* out[0] on/off, out[1] chunks captured, out[2] chunk replays, out[3] graphs
+ * kept, counted since the library was loaded.  Returns 0 or -1. */
+int erosion_graph_stats(long long* out);
 
 /* Test hook: z = P^{-1} r for the selected preconditioner (kind must be 1) on
  * the given geometry; r and z have the den layout.  Returns 0 or -1. */

@@ -101,6 +101,17 @@ def test_shear_stress_vs_oracle_20_bodies():
     assert relerr(tau, O.shear_stress(n, m, no, xx, yy, den)) < 1e-10
 
 
+def test_shear_stress_tight_geometry_vs_oracle():
+    """Bodies 0.002 apart, 128 points (two bodies' targets per 256-lane block)."""
+    thl = make_bodies(30, 128, 0.05, 0.12, 0.002, 31)
+    m, xx, yy = getnxy(new_thlenden(thl))
+    n, no = 128, 1024
+    N = 2 * no + 2 * n * m + 3 * m
+    den = np.random.default_rng(10).standard_normal(N) * 0.1
+    tau = stokes.computeshearstress(n, m, no, xx, yy, den, 1)
+    assert relerr(tau, O.shear_stress(n, m, no, xx, yy, den)) < 1e-10
+
+
 def test_trapezoid_solve_vs_reference(golden):
     g = golden("tight8_n64")
     n, m, no, xx, yy = _case(g)
