@@ -124,6 +124,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--skip-plain", action="store_true",
                     help="time only the accelerated variant (for profiling runs)")
+    ap.add_argument("--device-stepper", action="store_true",
+                    help="accelerated variant: run the whole step in the library "
+                         "(erosion_rungekutta2, theta-L update on the GPU, extrapolated guesses)")
     ap.add_argument("--graphs", action="store_true",
                     help="replay GMRES iteration chunks as HIP graphs in the accelerated "
                          "variant; kernel timing then runs on one extra untimed step")
@@ -143,6 +146,7 @@ def main() -> int:
     from erosion_amd.density_stress import getstress_
     from erosion_amd.learned import WarmStart
     from erosion_amd.time_stepping import rungekutta2
+    from erosion_amd.device_stepper import rungekutta2_device
 
     try:
         import torch
@@ -168,6 +172,13 @@ def main() -> int:
         """Warm-up steps, then exactly args.steps timed steps from the initial state."""
         configure(accelerated)
         guess = WarmStart(w.nouter, ibc=w.ibc) if accelerated else None
+        if accelerated and args.device_stepper:
+            stokes.stepper_reset()
+
+            def rungekutta2(th, params, guess=None, timings=None):  # noqa: F811
+                return rungekutta2_device(th, params, warm=2, timings=timings)
+        else:
+            from erosion_amd.time_stepping import rungekutta2
         th = copy.deepcopy(thld0)
         for i in range(args.warmup):
             th = rungekutta2(th, params, guess=guess, timings={})
@@ -292,6 +303,7 @@ def main() -> int:
                        "from the previous density or, where none exists, the learned network",
             "dt": w.dt, "fixpdrop": True, "ibc": w.ibc,
             "hip_graphs": bool(args.graphs),
+            "device_stepper": bool(args.device_stepper),
             "accelerated": sa,
             "plain_fp64": sp,
             "speedup_vs_plain_fp64": sa["steps_per_s"] / sp["steps_per_s"] if sp else None,

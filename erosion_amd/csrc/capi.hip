@@ -423,6 +423,72 @@ int erosion_set_preconditioner(int kind, int body_modes, int wall_modes) {
     }
 }
 
+int erosion_rungekutta2(int64_t npts, int64_t nbods, int64_t nouter, const double* params,
+                        double* theta, double* len, double* xsm, double* ysm,
+                        int64_t* nbods_out, double* info) {
+    try {
+        if (!params || !nbods_out || !info || (nbods > 0 && (!theta || !len || !xsm || !ysm)))
+            throw stokes::Error("null argument");
+        check_sizes(npts, nbods, nouter);
+        stokes::StepParams P;
+        P.dt = params[0];
+        P.eps = params[1];
+        P.sigma = params[2];
+        P.fixarea = params[3];
+        P.fixpdrop = params[4] != 0.0 ? 1 : 0;
+        P.ibc = static_cast<int>(params[5]);
+        P.ibary = static_cast<int>(params[6]);
+        P.maxl = static_cast<int>(params[7]);
+        P.warm = static_cast<int>(params[8]);
+        if (!(P.dt > 0.0) || P.warm < 0 || P.warm > 2 || P.maxl < 1)
+            throw stokes::Error("invalid step parameters");
+        Context& c = stokes::ctx();
+        c.status = 0;
+        c.last_error.clear();
+        stokes::StepReport rep;
+        int mo = static_cast<int>(nbods);
+        stokes::rk2_step(c, P, static_cast<int>(npts), static_cast<int>(nouter),
+                         static_cast<int>(nbods), theta, len, xsm, ysm, &mo, rep);
+        *nbods_out = mo;
+        info[0] = static_cast<double>(rep.iters[0]);
+        info[1] = static_cast<double>(rep.iters[1]);
+        info[2] = rep.deleted;
+        info[3] = rep.umax[0];
+        info[4] = rep.umax[1];
+        info[5] = rep.true_relres_max;
+        info[6] = rep.positive_mterm;
+        info[7] = rep.flux_warnings;
+        info[8] = rep.warm_starts;
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_rungekutta2", e.what());
+    }
+}
+
+int erosion_stepper_reset(void) {
+    try {
+        Context& c = stokes::ctx();
+        c.sp.hist_count = 0;
+        c.sp.hist_m = c.sp.hist_n = -1;
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_stepper_reset", e.what());
+    }
+}
+
+int erosion_spectral(int op, int64_t npts, int64_t rows, const double* in, const double* fac,
+                     double* out) {
+    try {
+        if (!in || !out || ((op == 2 || op == 3) && !fac)) throw stokes::Error("null argument");
+        if (rows < 1 || npts < 2) throw stokes::Error("invalid sizes");
+        stokes::spectral_rows(stokes::ctx(), op, static_cast<int>(npts), static_cast<int>(rows),
+                              in, fac, out);
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_spectral", e.what());
+    }
+}
+
 int erosion_set_graphs(int on) {
     try {
         if (on != 0 && on != 1) return -1;

@@ -211,6 +211,28 @@ struct GraphCache {
     ~GraphCache() { clear(); }
 };
 
+// Device-resident erosion step (stepper.hip): run parameters, per-step
+// report, and the device buffers kept between steps.
+struct StepParams {
+    double dt = 1e-3, eps = 0.0, sigma = 0.0, fixarea = 0.0;
+    int fixpdrop = 1, ibc = 0, ibary = 1, maxl = 8000;
+    int warm = 2;  // GMRES guess: 0 zeros, 1 previous density, 2 linear extrapolation
+};
+struct StepReport {
+    long long iters[2] = {0, 0};  // GMRES iterations of the two stages
+    int deleted = 0;              // bodies removed by delete_indices
+    double umax[2] = {1.0, 1.0};  // fixed-pressure-drop rescaling per stage
+    double true_relres_max = 0.0;
+    int positive_mterm = 0, flux_warnings = 0, warm_starts = 0;
+};
+struct StepperBufs {
+    DevBuf<double> th0, th05, th1, xx, yy, den, tau, atau, matau, nterm, dscal, scal, fac, tgt,
+        qout, tmp, hist0, hist1;
+    DevBuf<int> idx;
+    // densities kept for guesses, their body count and length
+    int hist_count = 0, hist_m = -1, hist_n = -1;
+};
+
 struct Context {
     Geometry g;
     NearList near;
@@ -265,6 +287,7 @@ struct Context {
     // replay GMRES iteration chunks as HIP graphs (off while timing kernels)
     int use_graphs = 0;
     GraphCache graphs;
+    StepperBufs sp;
     KernelTimer t_farfield, t_near, t_stress_eval, t_gs, t_precond, t_farfield32;
     int status = 0;
     std::string last_error;
@@ -278,7 +301,10 @@ struct Context {
 Context& ctx();
 
 // geometry.hip
-void build_geometry(Context& c, int n, int m, int no, const double* xx, const double* yy);
+// xx, yy: body points (n per body, body-major), in host memory or, with
+// on_device, in device memory
+void build_geometry(Context& c, int n, int m, int no, const double* xx, const double* yy,
+                    bool on_device = false);
 void build_near_list(Context& c, int ibary);
 
 // operator.hip : y = A x for the completed double-layer operator
@@ -295,7 +321,10 @@ void boundary_values(Context& c, const int* curves_dev, int ncurves, int wall_fi
 void bd_condition(Context& c, int ibc, double* rhs_dev);
 
 // gmres.hip
-long long gmres_solve(Context& c, int ibary, int ibc, int maxl, double tol, double* den_host);
+// den: initial guess in, density out (N doubles), in host memory or, with
+// on_device, in device memory
+long long gmres_solve(Context& c, int ibary, int ibc, int maxl, double tol, double* den,
+                      bool on_device = false);
 
 // precond.hip: set up for the current geometry, then z = P^{-1} r
 void precond_setup(Context& c);
@@ -322,6 +351,15 @@ double mlp_gradients(Context& c, int nin, int nhid, int nlayers, int nout, const
 void mlp_train(Context& c, int nin, int nhid, int nlayers, int nout, float* params, int nsamples,
                const float* x, const float* t, int epochs, int batch, float lr, uint64_t seed,
                double* epoch_loss);
+
+// stepper.hip: one theta-L RK2 step with the state (theta n x m, len, xsm,
+// ysm) in host memory, advanced in place; *m_out bodies remain
+void rk2_step(Context& c, const StepParams& P, int n, int no, int m, double* theta, double* len,
+              double* xsm, double* ysm, int* m_out, StepReport& rep);
+// op 0 specdiff, 1 specint, 2 expsmooth (fac per row), 3 getxy (fac = len,
+// xsm, ysm per row; out = x rows then y rows) on rows of n points
+void spectral_rows(Context& c, int op, int n, int rows, const double* in, const double* fac,
+                   double* out);
 
 // stress.hip
 void shear_stress(Context& c, const double* den_dev, double* tau_dev);

@@ -160,12 +160,14 @@ inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
 
 }  // namespace
 
-void build_geometry(Context& c, int n, int m, int no, const double* xx, const double* yy) {
+void build_geometry(Context& c, int n, int m, int no, const double* xx, const double* yy,
+                    bool on_device) {
     Geometry& g = c.g;
     const size_t nb = static_cast<size_t>(n) * static_cast<size_t>(m);
     // the time stepper hands the same boundary to the solve, the stress and
     // the target evaluations of one state: keep the built geometry for it
-    if (g.version > 0 && n == g.key_n && m == g.key_m && no == g.key_no &&
+    // (device-resident input is always built)
+    if (!on_device && g.version > 0 && n == g.key_n && m == g.key_m && no == g.key_no &&
         (nb == 0 || (std::memcmp(xx, g.key_x.data(), nb * sizeof(double)) == 0 &&
                      std::memcmp(yy, g.key_y.data(), nb * sizeof(double)) == 0)))
         return;
@@ -193,8 +195,9 @@ void build_geometry(Context& c, int n, int m, int no, const double* xx, const do
     dxx.ensure(nb);
     dyy.ensure(nb);
     if (nb) {
-        STOKES_HIP(hipMemcpyAsync(dxx.p, xx, nb * sizeof(double), hipMemcpyHostToDevice, s));
-        STOKES_HIP(hipMemcpyAsync(dyy.p, yy, nb * sizeof(double), hipMemcpyHostToDevice, s));
+        const hipMemcpyKind kind = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        STOKES_HIP(hipMemcpyAsync(dxx.p, xx, nb * sizeof(double), kind, s));
+        STOKES_HIP(hipMemcpyAsync(dyy.p, yy, nb * sizeof(double), kind, s));
     }
     std::vector<double> tn = deriv_table(std::max(n, 1)), tno = deriv_table(no);
     STOKES_HIP(hipMemcpyAsync(g.dtab_n.p, tn.data(), tn.size() * sizeof(double),
@@ -240,8 +243,9 @@ void build_geometry(Context& c, int n, int m, int no, const double* xx, const do
                                   hipMemcpyDeviceToHost, s));
     }
     STOKES_HIP(hipStreamSynchronize(s));
-    g.key_x.assign(xx, xx + nb);
-    g.key_y.assign(yy, yy + nb);
+    // the body points are copied unchanged into x, y after the wall points
+    g.key_x.assign(g.h_x.begin() + no, g.h_x.end());
+    g.key_y.assign(g.h_y.begin() + no, g.h_y.end());
     g.key_n = n;
     g.key_m = m;
     g.key_no = no;

@@ -478,7 +478,8 @@ static void replay_chunk(Context& c, const std::array<long long, 4>& key,
     ++gc.replays;
 }
 
-long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, double* den_host) {
+long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, double* den_io,
+                      bool on_device) {
     auto t_start = std::chrono::steady_clock::now();
     Geometry& g = c.g;
     hipStream_t s = c.stream;
@@ -520,7 +521,8 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
     STOKES_HIP(hipMemsetAsync(c.rhs.p, 0, npad * sizeof(double), s));
     STOKES_HIP(hipMemsetAsync(c.xvec.p, 0, npad * sizeof(double), s));
     bd_condition(c, ibc, c.rhs.p);
-    STOKES_HIP(hipMemcpyAsync(c.xvec.p, den_host, sizeof(double) * N, hipMemcpyHostToDevice, s));
+    STOKES_HIP(hipMemcpyAsync(c.xvec.p, den_io, sizeof(double) * N,
+                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
 
     // right preconditioning: GMRES on A P^{-1}, x = x0 + P^{-1} V y
     const bool pc = c.pc_kind != 0;
@@ -688,7 +690,8 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
             cycle + 1 >= kMaxCycles)
             break;
     }
-    STOKES_HIP(hipMemcpyAsync(den_host, c.xvec.p, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+    STOKES_HIP(hipMemcpyAsync(den_io, c.xvec.p, sizeof(double) * N,
+                              on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     STOKES_HIP(hipStreamSynchronize(s));
     c.last_iters = total_iters;
     c.last_implicit_relres = implicit_relres;

@@ -1,0 +1,106 @@
+"""Warm starts on an eroded 400-body geometry (python -m erosion_amd.eval_warmstart).
+
+Erodes the 400-body x 256-point workload (wall 4096, seed 1005) with the
+accelerated solve (preconditioner, mixed precision, tol 1e-10, fixpdrop) and
+the time stepper's warm start, until half of the initial body area is lost
+(or ``--max-steps``).  Then, on the eroded state, solves the same boundary
+from several initial guesses and reports GMRES iterations and seconds:
+
+* plain:                 the reference iteration (FP64, no preconditioner) from zeros;
+* accelerated, zeros:    preconditioner + mixed precision from zeros;
+* accelerated, learned:  from the trained network's guess (scaled by the factor
+                         that minimises its residual);
+* accelerated, previous: from 2 x_k - x_(k-1) of the run's last two densities.
+
+Prints one JSON line; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import learned as LR
+from . import stokes
+from .geometry import WORKLOADS, workload_geometry
+from .theta_len import ParamSet, getnxy
+from .time_stepping import body_areas, rungekutta2
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def _accelerated(on: bool) -> None:
+    stokes.set_preconditioner(stokes.PRECOND_CURVE_FOURIER if on else stokes.PRECOND_NONE)
+    stokes.set_precision(stokes.PRECISION_MIXED if on else stokes.PRECISION_FP64)
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--workload", default="bodies400_n256", choices=sorted(WORKLOADS))
+    ap.add_argument("--area-lost", type=float, default=0.5)
+    ap.add_argument("--max-steps", type=int, default=40)
+    ap.add_argument("--no-plain", action="store_true", help="skip the unpreconditioned solve")
+    a = ap.parse_args(argv)
+    w = WORKLOADS[a.workload]
+    params = ParamSet(npts=w.npts, nouter=w.nouter, ibary=1, ifmm=1, ibc=w.ibc, dt=w.dt,
+                      maxl=8000, fixpdrop=True)
+    stokes.set_gmres_tol(1e-10)
+    _accelerated(True)
+    th = workload_geometry(w)
+    area0 = float(body_areas(th).sum())
+    guess = LR.WarmStart(w.nouter, ibc=w.ibc)
+    steps, t_run, lost = 0, 0.0, 0.0
+    timings = {}
+    while lost < a.area_lost and steps < a.max_steps:
+        t0 = time.perf_counter()
+        th = rungekutta2(th, params, guess=guess, timings=timings)
+        t_run += time.perf_counter() - t0
+        steps += 1
+        lost = 1.0 - float(body_areas(th).sum()) / area0
+        log(f"step {steps}: {len(th.thlenvec)} bodies, area lost {100 * lost:.1f}%, "
+            f"{timings.get('gmres_iters', 0) / max(1, timings.get('solves', 1)):.1f} "
+            f"iterations per solve so far")
+    m, xx, yy = getnxy(th)
+    n, no = w.npts, w.nouter
+    prev = guess.prev(th)
+
+    def solve(label, accel, den0=None):
+        _accelerated(accel)
+        t0 = time.perf_counter()
+        den, it = stokes.stokessolver(n, m, no, 1, 1, w.ibc, params.maxl, xx, yy, den0)
+        dt = time.perf_counter() - t0
+        info = stokes.last_solve_info()
+        log(f"{label}: {it} iterations, {dt:.2f} s, true relres {info['true_relres']:.2e}")
+        return den, {"iterations": it, "seconds": dt, "true_relres": info["true_relres"]}
+
+    res = {}
+    d0, res["accelerated_zeros"] = solve("accelerated, zeros", True)
+    if guess.net is not None:
+        g = guess.net.predict(th, no, scale=True)
+        _, res["accelerated_learned"] = solve("accelerated, learned", True, g)
+        res["accelerated_learned"]["guess_relerr"] = float(np.linalg.norm(g - d0) / np.linalg.norm(d0))
+    if prev is not None:
+        _, res["accelerated_previous"] = solve("accelerated, previous densities", True, prev)
+        res["accelerated_previous"]["guess_relerr"] = float(
+            np.linalg.norm(prev - d0) / np.linalg.norm(d0))
+    if not a.no_plain:
+        _, res["plain_zeros"] = solve("plain FP64, zeros", False)
+    _accelerated(False)
+    print(json.dumps({
+        "workload": w.name, "bodies_initial": w.nbodies, "bodies_eroded": m,
+        "points_per_body": n, "wall_points": no, "gmres_tol": 1e-10,
+        "area_lost": lost, "steps": steps, "steps_per_s_accelerated_warm": steps / t_run,
+        "gmres_iters_per_solve_run": timings.get("gmres_iters", 0) / max(1, timings.get("solves", 1)),
+        "learned_guesses_used_in_run": guess.learned_used,
+        "solves": res,
+    }), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

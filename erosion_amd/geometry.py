@@ -82,7 +82,9 @@ def perturbed_circle_thetalen(npts: int, xc: float, yc: float, r0: float, a: np.
     mean_speed = ghat[0].real
     length = 2 * np.pi * mean_speed
     kk = np.fft.fftfreq(nfine, 1.0 / nfine)
-    keep = (kk != 0) & (np.abs(kk) < nfine // 2)
+    # modes of the (smooth) speed below 1e-17 of its mean change the
+    # arclength by less than its rounding: leave them out of the evaluations
+    keep = (kk != 0) & (np.abs(kk) < nfine // 2) & (np.abs(ghat) > 1e-17 * mean_speed)
     kk, gk = kk[keep], ghat[keep]
 
     def arclen(ph: np.ndarray) -> np.ndarray:

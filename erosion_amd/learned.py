@@ -274,7 +274,7 @@ class WarmStart:
 
     def __init__(self, nouter: int, path: str = WEIGHTS, ibc: int = 0, order: int = 2):
         from .time_stepping import PreviousDensity
-        self.prev = PreviousDensity(order)
+        self.prev = PreviousDensity(order, nouter)
         self.nouter = nouter
         self.net = LearnedGuess(path, nouter=nouter, ibc=ibc) if os.path.exists(path) else None
         self.learned_used = 0

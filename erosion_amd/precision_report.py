@@ -84,7 +84,7 @@ def run(workload: str, steps: int, tol: float, switch: float, save: str = ""):
     stokes.set_mixed_switch(switch)
     th64 = workload_geometry(w)
     th32 = copy.deepcopy(th64)
-    g64, g32 = PreviousDensity(2), PreviousDensity(2)
+    g64, g32 = PreviousDensity(2, w.nouter), PreviousDensity(2, w.nouter)
     area0 = body_areas(th64)
     records = []
     try:

@@ -69,6 +69,12 @@ def lib():
         L.erosion_graph_stats.restype = ctypes.c_int
         L.erosion_precond_apply.argtypes = [ctypes.c_int64] * 3 + [_dp] * 4
         L.erosion_precond_apply.restype = ctypes.c_int
+        L.erosion_rungekutta2.argtypes = [ctypes.c_int64] * 3 + [_dp] * 5 + [_i64p, _dp]
+        L.erosion_rungekutta2.restype = ctypes.c_int
+        L.erosion_stepper_reset.argtypes = []
+        L.erosion_stepper_reset.restype = ctypes.c_int
+        L.erosion_spectral.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _dp, _dp, _dp]
+        L.erosion_spectral.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -323,3 +329,53 @@ def solve_batch(geoms, ibary: int = 1, ibc: int = 0, maxl: int = 8000, workers: 
     if rc != 0 and all(o["status"] != 0 for o in out):
         raise StokesError(f"erosion_solve_batch: {last_error()}")
     return out
+
+
+RK2_INFO = ("iters_stage1", "iters_stage2", "deleted", "umax_stage1", "umax_stage2",
+            "true_relres_max", "positive_mterm", "flux_warnings", "warm_starts")
+
+
+def rungekutta2(npts: int, nouter: int, params, theta, length, xsm, ysm):
+    """One erosion step on the device (erosion_rungekutta2).
+
+    ``params`` = (dt, epsilon, sigma, fixarea, fixpdrop, ibc, ibary, maxl, warm);
+    theta is (nbods, npts).  Returns (theta, len, xsm, ysm, info) of the bodies
+    that remain; the inputs are not modified."""
+    theta = np.array(theta, dtype=np.float64, order="C", copy=True).reshape(-1)
+    m = len(theta) // npts if npts else 0
+    if len(theta) != m * npts:
+        raise StokesError("theta must hold npts values per body")
+    ln, xs, ys = (np.array(a, dtype=np.float64, copy=True) for a in (length, xsm, ysm))
+    if not (len(ln) == len(xs) == len(ys) == m):
+        raise StokesError("len, xsm, ysm need one value per body")
+    prm = _f64(params)
+    if len(prm) != 9:
+        raise StokesError("params holds 9 values")
+    info = np.zeros(len(RK2_INFO))
+    mo = ctypes.c_int64(0)
+    if lib().erosion_rungekutta2(npts, m, nouter, _p(prm), _p(theta), _p(ln), _p(xs), _p(ys),
+                                 ctypes.byref(mo), _p(info)) != 0:
+        raise StokesError(f"erosion_rungekutta2: {last_error()}")
+    k = int(mo.value)
+    return (theta[:k * npts].reshape(k, npts), ln[:k], xs[:k], ys[:k],
+            dict(zip(RK2_INFO, info.tolist())))
+
+
+def stepper_reset() -> None:
+    """Forget the densities the device stepper keeps for its GMRES guesses."""
+    if lib().erosion_stepper_reset() != 0:
+        raise StokesError(f"erosion_stepper_reset: {last_error()}")
+
+
+SPECDIFF, SPECINT, EXPSMOOTH, GETXY = 0, 1, 2, 3
+
+
+def spectral(op: int, rows, fac=None) -> np.ndarray:
+    """The device stepper's spectral operations on each row (test hook)."""
+    a = np.ascontiguousarray(rows, dtype=np.float64)
+    r, n = a.shape
+    f = _f64(fac) if fac is not None else np.zeros(1)
+    out = np.zeros(2 * r * n if op == GETXY else r * n)
+    if lib().erosion_spectral(int(op), n, r, _p(a), _p(f), _p(out)) != 0:
+        raise StokesError(f"erosion_spectral: {last_error()}")
+    return out.reshape(2, r, n) if op == GETXY else out.reshape(r, n)

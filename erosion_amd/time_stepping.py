@@ -156,23 +156,33 @@ class PreviousDensity:
     linear extrapolation 2 x_k - x_(k-1) of the last two (error O(dt^2)
     instead of O(dt) -- which matters because the solution is about 10^3
     times larger than the right-hand side, so a guess only lowers the
-    starting residual when its relative error is well below 1e-3)."""
+    starting residual when its relative error is well below 1e-3).
 
-    def __init__(self, order: int = 1):
+    A state whose bodies were deleted after its solve (timestep! deletes from
+    the state it just solved, TimeStepping.jl:120-126) holds a density of the
+    old body list: with ``nouter`` known such a density is recognised by its
+    length and empties the history instead of entering it."""
+
+    def __init__(self, order: int = 1, nouter: Optional[int] = None):
         if order not in (1, 2):
             raise ValueError("order must be 1 or 2")
         self.order = order
+        self.nouter = nouter
         self.hist: List[np.ndarray] = []
         self.hist_m = -1
 
+    def _fits(self, den: np.ndarray, thld: ThLenDenType) -> bool:
+        m = len(thld.thlenvec)
+        n = len(thld.thlenvec[0].theta) if m else 0
+        if self.nouter is not None:
+            return len(den) == 2 * self.nouter + (2 * n + 3) * m
+        # without nouter: the wall part 2*nouter must come out positive and even
+        no2 = len(den) - 2 * n * m - 3 * m
+        return no2 > 0 and no2 % 2 == 0
+
     def __call__(self, thld: ThLenDenType):
         m = len(thld.thlenvec)
-        if not self.hist or self.hist_m != m:
-            return None
-        n = len(thld.thlenvec[0].theta) if thld.thlenvec else 0
-        # the wall part is 2*nouter long: infer it from the stored vector
-        no2 = len(self.hist[-1]) - 2 * n * m - 3 * m
-        if not (no2 > 0 and no2 % 2 == 0):
+        if not self.hist or self.hist_m != m or not self._fits(self.hist[-1], thld):
             return None
         if self.order == 2 and len(self.hist) == 2:
             return 2.0 * self.hist[-1] - self.hist[-2]
@@ -183,12 +193,17 @@ class PreviousDensity:
         return self.hist[-1] if self.hist else None
 
     def seen(self, thld: ThLenDenType) -> None:
-        if len(thld.density) > 0:
-            m = len(thld.thlenvec)
-            if m != self.hist_m:
-                self.hist = []
-            self.hist = (self.hist + [np.asarray(thld.density)])[-2:]
-            self.hist_m = m
+        if len(thld.density) == 0:
+            return
+        m = len(thld.thlenvec)
+        den = np.asarray(thld.density)
+        if self.nouter is not None and not self._fits(den, thld):
+            self.hist, self.hist_m = [], -1  # bodies deleted since this solve
+            return
+        if m != self.hist_m:
+            self.hist = []
+        self.hist = (self.hist + [den])[-2:]
+        self.hist_m = m
 
 
 def rungekutta2(thld0: ThLenDenType, params: ParamSet,

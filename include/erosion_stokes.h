@@ -195,6 +195,39 @@ int erosion_set_preconditioner(int kind, int body_modes, int wall_modes);
 /* out[0] kind, out[1] body modes, out[2] wall modes.  Returns 0 or -1. */
 int erosion_get_preconditioner(int* out);
 
+/* One erosion step on the device: the theta-L midpoint RK2 of rungekutta2 /
+ * timestep! / getderivs / delete_indices (ErosionPackage/src/TimeStepping.jl:
+ * 16-176) with getstress! and the fixed-pressure-drop rescaling
+ * (DensityStress.jl:23-36, 129-167), replacing the Julia step that calls
+ * stokessolver_, computeshearstress_ and computeqoitargets_ per stage.  The
+ * boundary, densities and stresses of both stages stay on the GPU.
+ *   params[0] dt, [1] epsilon (epsfac/npts), [2] sigma (sigfac/npts),
+ *   [3] fixarea (0/1), [4] fixpdrop (0/1), [5] ibc, [6] ibary (must be 1),
+ *   [7] maxl, [8] GMRES guess: 0 zeros (the reference), 1 the previous
+ *   density, 2 the linear extrapolation of the last two (kept on the device
+ *   across calls while the body count is unchanged).
+ *   theta (npts x nbods, body-major), len, xsm, ysm: the state at t, replaced
+ *   by the state at t + dt of the *nbods_out bodies that remain (bodies are
+ *   deleted as delete_indices does).
+ *   info[0..8]: GMRES iterations of stage 1 and 2, bodies deleted, umax of
+ *   stage 1 and 2, largest true relative residual, positive-mterm count,
+ *   flux-mismatch count, warm-started solves.
+ * GMRES settings (tolerance, preconditioner, precision, graphs) are those of
+ * stokessolver_.  At most 512 points per body.  Returns 0 or -1. */
+int erosion_rungekutta2(int64_t npts, int64_t nbods, int64_t nouter, const double* params,
+                        double* theta, double* len, double* xsm, double* ysm,
+                        int64_t* nbods_out, double* info);
+/* Forget the densities erosion_rungekutta2 keeps for its GMRES guesses (a new
+ * run).  Returns 0 or -1. */
+int erosion_stepper_reset(void);
+/* Test hook: the stepper's spectral operations (SpectralMethods.jl:35-67,
+ * ThetaLen.jl:92-101) on `rows` rows of npts points: op 0 specdiff, 1
+ * specint, 2 expsmooth with fac[r] per row, 3 getxy with fac = (len[rows],
+ * xsm[rows], ysm[rows]) writing x rows then y rows (2 x rows x npts values).
+ * Returns 0 or -1. */
+int erosion_spectral(int op, int64_t npts, int64_t rows, const double* in, const double* fac,
+                     double* out);
+
 /* HIP graphs for GMRES (no counterpart in the reference).  on = 1: stokessolver_
  * captures each chunk of iterations (8 or 16 iterations of preconditioner,
  * operator, two Gram-Schmidt passes, Givens update) into a HIP graph the first

@@ -1,0 +1,115 @@
+"""The erosion step on the device (erosion_rungekutta2) against the host
+stepper, and its spectral kernels against numpy's FFT restatement.
+
+Bars: spectral operations 1e-12 relative to the row's largest value; one RK2
+step 1e-9 in theta and 1e-10 relative in length and surface means (both
+steppers call the same solver; the boundaries they hand it differ by
+rounding, and GMRES runs to 1e-12).
+"""
+import copy
+
+import numpy as np
+import pytest
+
+from erosion_amd import stokes
+from erosion_amd.device_stepper import rungekutta2_device
+from erosion_amd.geometry import make_bodies
+from erosion_amd.spectral import expsmooth, specdiff, specint
+from erosion_amd.theta_len import ParamSet, circ2thlen, getxy, new_thlenden
+from erosion_amd.time_stepping import rungekutta2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _settings():
+    stokes.set_gmres_tol(1e-12)
+    stokes.set_preconditioner(stokes.PRECOND_CURVE_FOURIER)
+    stokes.stepper_reset()
+    yield
+    stokes.set_preconditioner(stokes.PRECOND_NONE)
+    stokes.set_gmres_tol(1e-8)
+    stokes.stepper_reset()
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / np.max(np.abs(b)))
+
+
+@pytest.mark.parametrize("n", [64, 128, 255, 512])
+def test_spectral_rows_match_numpy(n):
+    rng = np.random.default_rng(n)
+    x = 2 * np.pi * np.arange(n) / n
+    rows = np.stack([np.cos(k * x) + 0.3 * np.sin((k + 3) * x) + 0.01 * rng.standard_normal(n)
+                     for k in range(1, 6)])
+    assert _rel(stokes.spectral(stokes.SPECDIFF, rows), specdiff(rows)) < 1e-12
+    assert _rel(stokes.spectral(stokes.SPECINT, rows), specint(rows)) < 1e-12
+    fac = np.array([0.0, 1e-4, 1e-3, 0.01, 0.1])
+    assert _rel(stokes.spectral(stokes.EXPSMOOTH, rows, fac), expsmooth(rows, fac)) < 1e-12
+
+
+def test_getxy_matches_host():
+    thl = make_bodies(7, 128, 0.05, 0.1, 0.05, 3)
+    theta = np.stack([t.theta for t in thl])
+    fac = np.concatenate([[t.len for t in thl], [t.xsm for t in thl], [t.ysm for t in thl]])
+    xy = stokes.spectral(stokes.GETXY, theta, fac)
+    for b, t in enumerate(thl):
+        x, y = getxy(t)
+        assert np.max(np.abs(xy[0, b] - x)) < 1e-14 and np.max(np.abs(xy[1, b] - y)) < 1e-14
+
+
+def _params(**kw):
+    p = dict(npts=128, nouter=512, ibc=0, dt=1e-3, fixpdrop=True, maxl=3000)
+    p.update(kw)
+    return ParamSet(**p)
+
+
+def _compare(dev, host):
+    assert len(dev.thlenvec) == len(host.thlenvec)
+    for a, b in zip(dev.thlenvec, host.thlenvec):
+        assert np.max(np.abs(a.theta - b.theta)) < 1e-9
+        assert abs(a.len - b.len) < 1e-10 * abs(b.len)
+        assert abs(a.xsm - b.xsm) < 1e-10 and abs(a.ysm - b.ysm) < 1e-10
+    assert dev.tt == pytest.approx(host.tt)
+
+
+@pytest.mark.parametrize("fixpdrop", [True, False])
+def test_rk2_step_matches_host(fixpdrop):
+    params = _params(fixpdrop=fixpdrop)
+    th0 = new_thlenden(make_bodies(10, 128, 0.05, 0.1, 0.05, 1001))
+    dev = rungekutta2_device(copy.deepcopy(th0), params, warm=0)
+    host = rungekutta2(copy.deepcopy(th0), params)
+    _compare(dev, host)
+
+
+def test_rk2_step_deletes_a_vanishing_body_like_host():
+    """A body of radius 1e-3 in the flow: its dL/dt deletes it in the first
+    stage of a 0.05 step, on both steppers."""
+    n = 64
+    thl = make_bodies(5, n, 0.05, 0.1, 0.1, 12)
+    thl.insert(2, circ2thlen(n, 1e-3, -0.9, 0.0))
+    params = _params(npts=n, dt=0.05)
+    th0 = new_thlenden(thl)
+    timings = {}
+    dev = rungekutta2_device(copy.deepcopy(th0), params, warm=0, timings=timings)
+    host = rungekutta2(copy.deepcopy(th0), params)
+    assert len(host.thlenvec) < len(thl)
+    assert timings["deleted"] == len(thl) - len(host.thlenvec)
+    _compare(dev, host)
+
+
+def test_device_warm_start_lowers_iterations_same_states():
+    params = _params()
+    th0 = new_thlenden(make_bodies(10, 128, 0.05, 0.1, 0.05, 1001))
+    t_warm, t_cold = {}, {}
+    a = copy.deepcopy(th0)
+    for _ in range(3):
+        a = rungekutta2_device(a, params, warm=2, timings=t_warm)
+    stokes.stepper_reset()
+    b = copy.deepcopy(th0)
+    for _ in range(3):
+        b = rungekutta2_device(b, params, warm=0, timings=t_cold)
+    assert t_warm["last_info"]["warm_starts"] == 2
+    assert t_warm["gmres_iters"] < t_cold["gmres_iters"]
+    for x, y in zip(a.thlenvec, b.thlenvec):
+        assert np.max(np.abs(x.theta - y.theta)) < 1e-9

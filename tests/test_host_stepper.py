@@ -160,3 +160,26 @@ def test_previous_density_extrapolation():
     assert g2(state(0, fewer)) is None          # body count changed
     g2.seen(state(7.0, fewer))
     assert np.all(g2(state(0, fewer)) == 7.0)   # history restarted
+
+
+def test_previous_density_drops_a_density_of_deleted_bodies():
+    """timestep! deletes bodies from the state it just solved: with nouter
+    known, that state's density (old body count) empties the history."""
+    from erosion_amd.time_stepping import PreviousDensity
+    thl = make_bodies(3, 16, 0.05, 0.1, 0.05, 1)
+    n, no = 16, 8
+
+    def state(v, bodies, m_solved):
+        s = new_thlenden(list(bodies))
+        s.density = np.full(2 * no + (2 * n + 3) * m_solved, float(v))
+        return s
+
+    g = PreviousDensity(2, nouter=no)
+    g.seen(state(1.0, thl, 3))
+    g.seen(state(2.0, thl, 3))
+    assert np.all(g(state(0, thl, 3)) == 3.0)
+    g.seen(state(4.0, thl[:2], 3))         # solved with 3 bodies, one deleted since
+    assert g(state(0, thl[:2], 2)) is None
+    g.seen(state(5.0, thl[:2], 2))
+    assert np.all(g(state(0, thl[:2], 2)) == 5.0)
+    assert len(g(state(0, thl[:2], 2))) == 2 * no + (2 * n + 3) * 2
