@@ -425,7 +425,7 @@ int erosion_set_preconditioner(int kind, int body_modes, int wall_modes) {
 
 int erosion_rungekutta2(int64_t npts, int64_t nbods, int64_t nouter, const double* params,
                         double* theta, double* len, double* xsm, double* ysm,
-                        int64_t* nbods_out, double* info) {
+                        const double* guess, int64_t* nbods_out, double* info) {
     try {
         if (!params || !nbods_out || !info || (nbods > 0 && (!theta || !len || !xsm || !ysm)))
             throw stokes::Error("null argument");
@@ -448,7 +448,7 @@ int erosion_rungekutta2(int64_t npts, int64_t nbods, int64_t nouter, const doubl
         stokes::StepReport rep;
         int mo = static_cast<int>(nbods);
         stokes::rk2_step(c, P, static_cast<int>(npts), static_cast<int>(nouter),
-                         static_cast<int>(nbods), theta, len, xsm, ysm, &mo, rep);
+                         static_cast<int>(nbods), theta, len, xsm, ysm, guess, &mo, rep);
         *nbods_out = mo;
         info[0] = static_cast<double>(rep.iters[0]);
         info[1] = static_cast<double>(rep.iters[1]);
@@ -459,9 +459,23 @@ int erosion_rungekutta2(int64_t npts, int64_t nbods, int64_t nouter, const doubl
         info[6] = rep.positive_mterm;
         info[7] = rep.flux_warnings;
         info[8] = rep.warm_starts;
+        info[9] = rep.given_guesses;
         return 0;
     } catch (const std::exception& e) {
         return record_error("erosion_rungekutta2", e.what());
+    }
+}
+
+int erosion_stepper_history(int64_t* out) {
+    if (!out) return -1;
+    try {
+        Context& c = stokes::ctx();
+        out[0] = c.sp.hist_count;
+        out[1] = c.sp.hist_m;
+        out[2] = c.sp.hist_n;
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_stepper_history", e.what());
     }
 }
 

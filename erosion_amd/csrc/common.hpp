@@ -223,7 +223,7 @@ struct StepReport {
     int deleted = 0;              // bodies removed by delete_indices
     double umax[2] = {1.0, 1.0};  // fixed-pressure-drop rescaling per stage
     double true_relres_max = 0.0;
-    int positive_mterm = 0, flux_warnings = 0, warm_starts = 0;
+    int positive_mterm = 0, flux_warnings = 0, warm_starts = 0, given_guesses = 0;
 };
 struct StepperBufs {
     DevBuf<double> th0, th05, th1, xx, yy, den, tau, atau, matau, nterm, dscal, scal, fac, tgt,
@@ -354,8 +354,10 @@ void mlp_train(Context& c, int nin, int nhid, int nlayers, int nout, float* para
 
 // stepper.hip: one theta-L RK2 step with the state (theta n x m, len, xsm,
 // ysm) in host memory, advanced in place; *m_out bodies remain
+// guess0 (host, may be null): stage-1 GMRES guess used when no kept density
+// fits the state
 void rk2_step(Context& c, const StepParams& P, int n, int no, int m, double* theta, double* len,
-              double* xsm, double* ysm, int* m_out, StepReport& rep);
+              double* xsm, double* ysm, const double* guess0, int* m_out, StepReport& rep);
 // op 0 specdiff, 1 specint, 2 expsmooth (fac per row), 3 getxy (fac = len,
 // xsm, ysm per row; out = x rows then y rows) on rows of n points
 void spectral_rows(Context& c, int op, int n, int rows, const double* in, const double* fac,

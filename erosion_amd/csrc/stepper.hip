@@ -357,7 +357,8 @@ void keep_rows(Context& c, DevBuf<double>& a, const std::vector<int>& idx, int n
 // and the density in c.sp.den; fills sd.matau and mterm / xdot / ydot.
 void stage_derivs(Context& c, const StepParams& P, int n, int no, int m, const double* theta_d,
                   Scalars& sd, std::vector<double>& mterm, std::vector<double>& xdot,
-                  std::vector<double>& ydot, StepReport& rep, int stage) {
+                  std::vector<double>& ydot, StepReport& rep, int stage,
+                  const double* guess_host) {
     StepperBufs& S = c.sp;
     hipStream_t s = c.stream;
     const size_t nb = static_cast<size_t>(n) * static_cast<size_t>(m);
@@ -384,6 +385,10 @@ void stage_derivs(Context& c, const StepParams& P, int n, int no, int m, const d
                                                  P.warm == 2 && S.hist_count == 2 ? 2 : 1, N,
                                                  S.den.p);
         ++rep.warm_starts;
+    } else if (guess_host) {
+        STOKES_HIP(hipMemcpyAsync(S.den.p, guess_host, N * sizeof(double), hipMemcpyHostToDevice,
+                                  s));
+        ++rep.given_guesses;
     } else {
         k_fill0<<<cdiv_int(N, 256), 256, 0, s>>>(S.den.p, N);
     }
@@ -534,7 +539,7 @@ void advance(Context& c, const StepParams& P, int n, int m, const DevBuf<double>
 }  // namespace
 
 void rk2_step(Context& c, const StepParams& P, int n, int no, int m, double* theta, double* len,
-              double* xsm, double* ysm, int* m_out, StepReport& rep) {
+              double* xsm, double* ysm, const double* guess0, int* m_out, StepReport& rep) {
     if (n > kMaxSpecN) throw Error("the device stepper takes at most 512 points per body");
     if (P.ibary != 1) throw Error("the device stepper needs ibary=1 (barycentric shear stress)");
     StepperBufs& S = c.sp;
@@ -552,7 +557,7 @@ void rk2_step(Context& c, const StepParams& P, int n, int no, int m, double* the
 
     // stage 1: timestep!(thld0, thld0, dt/2, dt/2)
     const double h = 0.5 * P.dt;
-    stage_derivs(c, P, n, no, m, S.th0.p, s0, mterm, xdot, ydot, rep, 0);
+    stage_derivs(c, P, n, no, m, S.th0.p, s0, mterm, xdot, ydot, rep, 0, guess0);
     std::vector<int> keep = kept_bodies(s0.len, mterm, h, rep);
     const int N0 = 2 * no + 2 * n * m + 3 * m;
     remember_density(c, N0, m, static_cast<int>(keep.size()) != m);
@@ -574,7 +579,7 @@ void rk2_step(Context& c, const StepParams& P, int n, int no, int m, double* the
     advance(c, P, n, m, S.th0, s0, s0, mterm, xdot, ydot, h, h, S.th05, s05);
 
     // stage 2: timestep!(thld0, thld05, dt, dt/2)
-    stage_derivs(c, P, n, no, m, S.th05.p, s05, mterm, xdot, ydot, rep, 1);
+    stage_derivs(c, P, n, no, m, S.th05.p, s05, mterm, xdot, ydot, rep, 1, nullptr);
     keep = kept_bodies(s0.len, mterm, P.dt, rep);
     const int N1 = 2 * no + 2 * n * m + 3 * m;
     remember_density(c, N1, m, static_cast<int>(keep.size()) != m);

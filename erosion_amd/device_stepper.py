@@ -22,7 +22,10 @@ from .theta_len import ParamSet, ThetaLenType, ThLenDenType, new_thlenden
 
 
 def rungekutta2_device(thld0: ThLenDenType, params: ParamSet, warm: int = 2,
-                       timings: dict = None) -> ThLenDenType:
+                       timings: dict = None, guess=None) -> ThLenDenType:
+    """``guess(thld0)`` (for example the trained network, ``LearnedGuess``) is
+    asked for a first-stage GMRES guess only when the library keeps no density
+    that fits the state (the first step of a run, after a deletion)."""
     tv = thld0.thlenvec
     n = params.npts
     if not tv:
@@ -33,9 +36,15 @@ def rungekutta2_device(thld0: ThLenDenType, params: ParamSet, warm: int = 2,
         raise ValueError("every body needs params.npts points")
     prm = [params.dt, params.epsilon, params.sigma, float(params.fixarea), float(params.fixpdrop),
            params.ibc, params.ibary, params.maxl, warm]
+    g0 = None
+    if guess is not None and warm > 0:
+        h = stokes.stepper_history()
+        if h["count"] == 0 or h["bodies"] != len(tv) or \
+                h["length"] != stokes.nunk(n, len(tv), params.nouter):
+            g0 = guess(thld0)
     theta, ln, xs, ys, info = stokes.rungekutta2(
         n, params.nouter, prm, np.stack([t.theta for t in tv]), [t.len for t in tv],
-        [t.xsm for t in tv], [t.ysm for t in tv])
+        [t.xsm for t in tv], [t.ysm for t in tv], guess=g0)
     out = new_thlenden([ThetaLenType(theta[b].copy(), float(ln[b]), float(xs[b]), float(ys[b]),
                                      math.nan) for b in range(len(ln))])
     out.tt = thld0.tt + params.dt

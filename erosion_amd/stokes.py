@@ -69,7 +69,9 @@ def lib():
         L.erosion_graph_stats.restype = ctypes.c_int
         L.erosion_precond_apply.argtypes = [ctypes.c_int64] * 3 + [_dp] * 4
         L.erosion_precond_apply.restype = ctypes.c_int
-        L.erosion_rungekutta2.argtypes = [ctypes.c_int64] * 3 + [_dp] * 5 + [_i64p, _dp]
+        L.erosion_rungekutta2.argtypes = [ctypes.c_int64] * 3 + [_dp] * 6 + [_i64p, _dp]
+        L.erosion_stepper_history.argtypes = [_i64p]
+        L.erosion_stepper_history.restype = ctypes.c_int
         L.erosion_rungekutta2.restype = ctypes.c_int
         L.erosion_stepper_reset.argtypes = []
         L.erosion_stepper_reset.restype = ctypes.c_int
@@ -332,15 +334,17 @@ def solve_batch(geoms, ibary: int = 1, ibc: int = 0, maxl: int = 8000, workers: 
 
 
 RK2_INFO = ("iters_stage1", "iters_stage2", "deleted", "umax_stage1", "umax_stage2",
-            "true_relres_max", "positive_mterm", "flux_warnings", "warm_starts")
+            "true_relres_max", "positive_mterm", "flux_warnings", "warm_starts",
+            "given_guesses")
 
 
-def rungekutta2(npts: int, nouter: int, params, theta, length, xsm, ysm):
+def rungekutta2(npts: int, nouter: int, params, theta, length, xsm, ysm, guess=None):
     """One erosion step on the device (erosion_rungekutta2).
 
     ``params`` = (dt, epsilon, sigma, fixarea, fixpdrop, ibc, ibary, maxl, warm);
-    theta is (nbods, npts).  Returns (theta, len, xsm, ysm, info) of the bodies
-    that remain; the inputs are not modified."""
+    theta is (nbods, npts); ``guess`` an optional first-stage GMRES guess used
+    when no kept density fits.  Returns (theta, len, xsm, ysm, info) of the
+    bodies that remain; the inputs are not modified."""
     theta = np.array(theta, dtype=np.float64, order="C", copy=True).reshape(-1)
     m = len(theta) // npts if npts else 0
     if len(theta) != m * npts:
@@ -351,14 +355,28 @@ def rungekutta2(npts: int, nouter: int, params, theta, length, xsm, ysm):
     prm = _f64(params)
     if len(prm) != 9:
         raise StokesError("params holds 9 values")
+    g = None
+    if guess is not None:
+        g = _f64(guess)
+        if len(g) != nunk(npts, m, nouter):
+            raise StokesError(f"guess holds {len(g)} values; need {nunk(npts, m, nouter)}")
     info = np.zeros(len(RK2_INFO))
     mo = ctypes.c_int64(0)
     if lib().erosion_rungekutta2(npts, m, nouter, _p(prm), _p(theta), _p(ln), _p(xs), _p(ys),
-                                 ctypes.byref(mo), _p(info)) != 0:
+                                 _p(g) if g is not None else None, ctypes.byref(mo),
+                                 _p(info)) != 0:
         raise StokesError(f"erosion_rungekutta2: {last_error()}")
     k = int(mo.value)
     return (theta[:k * npts].reshape(k, npts), ln[:k], xs[:k], ys[:k],
             dict(zip(RK2_INFO, info.tolist())))
+
+
+def stepper_history() -> dict:
+    """Densities the device stepper keeps for guesses: count, body count, length."""
+    out = np.zeros(3, dtype=np.int64)
+    if lib().erosion_stepper_history(out.ctypes.data_as(_i64p)) != 0:
+        raise StokesError(f"erosion_stepper_history: {last_error()}")
+    return {"count": int(out[0]), "bodies": int(out[1]), "length": int(out[2])}
 
 
 def stepper_reset() -> None:

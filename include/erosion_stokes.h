@@ -209,14 +209,20 @@ int erosion_get_preconditioner(int* out);
  *   theta (npts x nbods, body-major), len, xsm, ysm: the state at t, replaced
  *   by the state at t + dt of the *nbods_out bodies that remain (bodies are
  *   deleted as delete_indices does).
- *   info[0..8]: GMRES iterations of stage 1 and 2, bodies deleted, umax of
+ *   guess: null, or a density (2 nouter + (2 npts + 3) nbods values) used as
+ *   the first stage's GMRES guess when no kept density fits the state.
+ *   info[0..9]: GMRES iterations of stage 1 and 2, bodies deleted, umax of
  *   stage 1 and 2, largest true relative residual, positive-mterm count,
- *   flux-mismatch count, warm-started solves.
+ *   flux-mismatch count, solves started from kept densities, solves started
+ *   from `guess`.
  * GMRES settings (tolerance, preconditioner, precision, graphs) are those of
  * stokessolver_.  At most 512 points per body.  Returns 0 or -1. */
 int erosion_rungekutta2(int64_t npts, int64_t nbods, int64_t nouter, const double* params,
                         double* theta, double* len, double* xsm, double* ysm,
-                        int64_t* nbods_out, double* info);
+                        const double* guess, int64_t* nbods_out, double* info);
+/* out[0] densities kept for guesses (0-2), out[1] their body count, out[2]
+ * their length.  Returns 0 or -1. */
+int erosion_stepper_history(int64_t* out);
 /* Forget the densities erosion_rungekutta2 keeps for its GMRES guesses (a new
  * run).  Returns 0 or -1. */
 int erosion_stepper_reset(void);

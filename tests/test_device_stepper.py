@@ -113,3 +113,27 @@ def test_device_warm_start_lowers_iterations_same_states():
     assert t_warm["gmres_iters"] < t_cold["gmres_iters"]
     for x, y in zip(a.thlenvec, b.thlenvec):
         assert np.max(np.abs(x.theta - y.theta)) < 1e-9
+
+
+def test_given_guess_used_only_without_history():
+    """The first-stage guess is taken when no kept density fits, and a wrong
+    length is refused before any GPU work."""
+    params = _params()
+    th0 = new_thlenden(make_bodies(10, 128, 0.05, 0.1, 0.05, 1001))
+    calls = []
+
+    def guess(th):
+        calls.append(1)
+        m = len(th.thlenvec)
+        return np.zeros(2 * params.nouter + (2 * params.npts + 3) * m)
+
+    t1, t2 = {}, {}
+    a = rungekutta2_device(copy.deepcopy(th0), params, warm=2, timings=t1, guess=guess)
+    assert len(calls) == 1 and t1["last_info"]["given_guesses"] == 1
+    rungekutta2_device(a, params, warm=2, timings=t2, guess=guess)
+    assert len(calls) == 1 and t2["last_info"]["given_guesses"] == 0
+    with pytest.raises(stokes.StokesError):
+        stokes.rungekutta2(128, params.nouter, [1e-3, 0.1, 0.1, 0, 1, 0, 1, 100, 2],
+                           np.stack([t.theta for t in th0.thlenvec]),
+                           [t.len for t in th0.thlenvec], [t.xsm for t in th0.thlenvec],
+                           [t.ysm for t in th0.thlenvec], guess=np.zeros(5))
