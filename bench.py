@@ -130,6 +130,9 @@ def main() -> int:
                     help="accelerated variant: theta-L updates on the host between solver "
                          "calls, as the plain variant (default: the whole step in the library, "
                          "erosion_rungekutta2)")
+    ap.add_argument("--body-modes", type=int, default=None,
+                    help="accelerated variant: Fourier modes per body in the preconditioner's "
+                         "coarse space (default: the library's)")
     ap.add_argument("--graphs", action="store_true",
                     help="replay GMRES iteration chunks as HIP graphs in the accelerated "
                          "variant; kernel timing then runs on one extra untimed step")
@@ -167,7 +170,10 @@ def main() -> int:
     _m, xx0, yy0 = getnxy(thld0)
 
     def configure(accelerated):
-        stokes.set_preconditioner(1 if accelerated else 0)
+        if accelerated and args.body_modes is not None:
+            stokes.set_preconditioner(1, args.body_modes)
+        else:
+            stokes.set_preconditioner(1 if accelerated else 0)
         stokes.set_precision(stokes.PRECISION_MIXED if accelerated else stokes.PRECISION_FP64)
         stokes.set_graphs(bool(accelerated and args.graphs))
 

@@ -241,6 +241,7 @@ struct Context {
     DevBuf<double> ex, ey;          // density in point order
     DevBuf<double> part_u, part_v;  // far-field partial sums [split][npts]
     DevBuf<double> bd;              // boundary values [5 fields][2][npts]
+    DevBuf<double> bdpart;          // wall boundary-value partial sums [split][20][nouter]
     DevBuf<double> fld;             // field densities [5][2][npts]
     DevBuf<double> scal;            // sigdotn etc.
     DevBuf<double> stk_log;           // log |z_i - c_b|^2 [npts][m]
@@ -278,7 +279,7 @@ struct Context {
     double tol = 1e-8;
     // right preconditioner (precond.hip): kind 0 none (the reference's
     // unpreconditioned GMRES), 1 per-curve two-level Fourier
-    int pc_kind = 0, pc_body_modes = 8, pc_wall_modes = 16;
+    int pc_kind = 0, pc_body_modes = 16, pc_wall_modes = 16;
     int pc_nfb = 0, pc_nfw = 0;
     int pc_wall_no = -1, pc_wall_kw = -1;  // key of the kept wall block
     DevBuf<double> pc_phi_b, pc_phi_w, pc_S, pc_Gb, pc_Gw, pc_z;

@@ -396,14 +396,18 @@ __global__ void k_fields(const double* __restrict__ x, const double* __restrict_
 // for the five fields of one curve at once:
 //   B_j = sum_{k!=j} -(f_k - f_j) dz_k/(z_k - z_j) - dtheta f'_j
 //   exterior: bd = i B / 2pi       interior: bd = -i B / 2pi - f_j
-// grid: (ceil(len/64), curves); block 64*NSL threads: lane -> node j,
-// wave -> slice of the k range; slices combined through LDS.
+// grid: (ceil(len/64), curves, ksplit); block 64*NSL threads: lane -> node j,
+// wave -> slice of the block's part of the k range; slices combined through
+// LDS.  With part == nullptr (ksplit 1) the block finishes the boundary
+// values; otherwise it writes its 20 partial sums per node to
+// part[(z*20 + q)*len + j] and k_bdval_finish adds the ksplit parts.
 template <int NSL>
 __global__ __launch_bounds__(64 * NSL) void k_bdval(
     const int* __restrict__ curves, int no, int n, int npts, const double* __restrict__ x,
     const double* __restrict__ y, const double* __restrict__ dzr, const double* __restrict__ dzi,
     const double* __restrict__ dno, const double* __restrict__ dn,
-    const double* __restrict__ fld, double* __restrict__ bd) {
+    const double* __restrict__ fld, double* __restrict__ bd, int ksplit,
+    double* __restrict__ part) {
     __shared__ double red[NSL][64];
     const int c = curves[blockIdx.y];
     const int len = c == 0 ? no : n;
@@ -422,7 +426,8 @@ __global__ __launch_bounds__(64 * NSL) void k_bdval(
     cplx B[5], D[5];
 #pragma unroll
     for (int f = 0; f < 5; ++f) B[f] = D[f] = cmk(0.0, 0.0);
-    const int k0 = (slice * len) / NSL, k1 = ((slice + 1) * len) / NSL;
+    const int c0 = (blockIdx.z * len) / ksplit, c1 = ((blockIdx.z + 1) * len) / ksplit;
+    const int k0 = c0 + (slice * (c1 - c0)) / NSL, k1 = c0 + ((slice + 1) * (c1 - c0)) / NSL;
     for (int k = k0; k < k1; ++k) {
         const int p = off + k;
         cplx fk[5];
@@ -464,6 +469,12 @@ __global__ __launch_bounds__(64 * NSL) void k_bdval(
         __syncthreads();
     }
     if (slice != 0 || !valid) return;
+    if (part) {
+#pragma unroll
+        for (int q = 0; q < 20; ++q)
+            part[(static_cast<size_t>(blockIdx.z) * 20 + q) * len + j] = vals[q];
+        return;
+    }
     const double dth = kTwoPi / static_cast<double>(len);
     const bool ext = c != 0;
 #pragma unroll
@@ -474,6 +485,34 @@ __global__ __launch_bounds__(64 * NSL) void k_bdval(
         cplx out = ext ? iB : cmk(-iB.r - fj[f].r, -iB.i - fj[f].i);
         bd[(2 * f) * N + off + j] = out.r;
         bd[(2 * f + 1) * N + off + j] = out.i;
+    }
+}
+
+// Boundary values of the wall (curve 0, interior rule) from the ksplit
+// partial sums of k_bdval: bd = -i B / 2pi - f_j, B = sum_parts(B) - dtheta
+// sum_parts(D).
+__global__ void k_bdval_finish(int no, int npts, const double* __restrict__ fld,
+                               const double* __restrict__ part, int ksplit,
+                               double* __restrict__ bd) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= no) return;
+    const size_t N = static_cast<size_t>(npts);
+    double vals[20];
+#pragma unroll
+    for (int q = 0; q < 20; ++q) vals[q] = 0.0;
+    for (int z = 0; z < ksplit; ++z) {
+#pragma unroll
+        for (int q = 0; q < 20; ++q) vals[q] += part[(static_cast<size_t>(z) * 20 + q) * no + j];
+    }
+    const double dth = kTwoPi / static_cast<double>(no);
+#pragma unroll
+    for (int f = 0; f < 5; ++f) {
+        const cplx fj = cmk(fld[(2 * f) * N + j], fld[(2 * f + 1) * N + j]);
+        const cplx Bf =
+            cmk(vals[4 * f + 0] - dth * vals[4 * f + 2], vals[4 * f + 1] - dth * vals[4 * f + 3]);
+        const cplx iB = cmk(-Bf.i / kTwoPi, Bf.r / kTwoPi);
+        bd[(2 * f) * N + j] = -iB.r - fj.r;
+        bd[(2 * f + 1) * N + j] = -iB.i - fj.i;
     }
 }
 
@@ -779,16 +818,23 @@ void boundary_values(Context& c, const int* curves_dev, int ncurves, int wall_fi
                                                  c.fld.p);
     STOKES_LAUNCH_CHECK();
     if (wall_first) {
-        k_bdval<16><<<dim3(cdiv_int(g.no, 64), 1), 1024, 0, s>>>(curves_dev, g.no, g.n, np, g.x.p,
-                                                                 g.y.p, g.dzr.p, g.dzi.p,
-                                                                 g.dtab_no.p, g.dtab_n.p, c.fld.p,
-                                                                 c.bd.p);
+        // the wall alone is one curve of up to a few thousand nodes: split its
+        // k range over blocks so the launch has ~256 blocks, then add the parts
+        const int jb = cdiv_int(g.no, 64);
+        const int ks = std::max(1, std::min(32, cdiv_int(256, jb)));
+        c.bdpart.ensure(static_cast<size_t>(ks) * 20 * static_cast<size_t>(g.no));
+        k_bdval<4><<<dim3(jb, 1, ks), 256, 0, s>>>(curves_dev, g.no, g.n, np, g.x.p, g.y.p,
+                                                   g.dzr.p, g.dzi.p, g.dtab_no.p, g.dtab_n.p,
+                                                   c.fld.p, c.bd.p, ks, c.bdpart.p);
+        STOKES_LAUNCH_CHECK();
+        k_bdval_finish<<<cdiv_int(g.no, kBlk), kBlk, 0, s>>>(g.no, np, c.fld.p, c.bdpart.p, ks,
+                                                             c.bd.p);
         STOKES_LAUNCH_CHECK();
     }
     if (ncurves > wall_first) {
-        k_bdval<4><<<dim3(cdiv_int(g.n, 64), ncurves - wall_first), 256, 0, s>>>(
+        k_bdval<4><<<dim3(cdiv_int(g.n, 64), ncurves - wall_first, 1), 256, 0, s>>>(
             curves_dev + wall_first, g.no, g.n, np, g.x.p, g.y.p, g.dzr.p, g.dzi.p, g.dtab_no.p,
-            g.dtab_n.p, c.fld.p, c.bd.p);
+            g.dtab_n.p, c.fld.p, c.bd.p, 1, nullptr);
         STOKES_LAUNCH_CHECK();
     }
 }

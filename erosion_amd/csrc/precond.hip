@@ -447,15 +447,15 @@ void precond_setup(Context& c) {
     Geometry& g = c.g;
     hipStream_t s = c.stream;
     const int n = g.n, m = g.m, no = g.no;
-    const int kb = c.pc_body_modes, kw = c.pc_wall_modes;
-    if (kb < 1 || kb > kMaxModes || kw < 1 || kw > kMaxModes)
+    if (c.pc_body_modes < 1 || c.pc_body_modes > kMaxModes || c.pc_wall_modes < 1 ||
+        c.pc_wall_modes > kMaxModes)
         throw Error("preconditioner modes must lie in [1, 16]");
-    if (m > 0 && 2 * kb + 1 >= n)
-        throw Error("preconditioner body modes " + std::to_string(kb) + " need npts > " +
-                    std::to_string(2 * kb + 1));
-    if (2 * kw + 1 >= no)
-        throw Error("preconditioner wall modes " + std::to_string(kw) + " need nouter > " +
-                    std::to_string(2 * kw + 1));
+    // a curve of len points holds modes 0..(len-2)/2 without the Nyquist mode
+    const int kb = std::min(c.pc_body_modes, (n - 2) / 2);
+    const int kw = std::min(c.pc_wall_modes, (no - 2) / 2);
+    if (m > 0 && kb < 1)
+        throw Error("the preconditioner needs at least 4 points per body");
+    if (kw < 1) throw Error("the preconditioner needs at least 4 wall points");
     const int nfb = 2 * kb + 1, nfw = 2 * kw + 1;
     const int nqb = 2 * nfb + 3, nqw = 2 * nfw;
     c.pc_phi_b.ensure(static_cast<size_t>(nfb) * std::max(n, 1));

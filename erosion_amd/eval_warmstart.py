@@ -10,9 +10,12 @@ from several initial guesses and reports GMRES iterations and seconds:
 * accelerated, zeros:    preconditioner + mixed precision from zeros;
 * accelerated, learned:  from the trained network's guess (scaled by the factor
                          that minimises its residual);
-* accelerated, previous: from 2 x_k - x_(k-1) of the run's last two densities.
+* accelerated, previous: from 2 x_k - x_(k-1) of the run's last two densities
+                         (host stepper only);
+* with ``--body-modes``: accelerated from zeros with other body preconditioner sizes.
 
-Prints one JSON line; progress goes to stderr.
+Also records the GMRES iterations of every step of the run.  Prints one JSON
+line; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -34,8 +37,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def _accelerated(on: bool) -> None:
-    stokes.set_preconditioner(stokes.PRECOND_CURVE_FOURIER if on else stokes.PRECOND_NONE)
+def _accelerated(on: bool, body_modes=None) -> None:
+    if on:
+        if body_modes is None:
+            stokes.set_preconditioner(stokes.PRECOND_CURVE_FOURIER)
+        else:
+            stokes.set_preconditioner(stokes.PRECOND_CURVE_FOURIER, body_modes)
+    else:
+        stokes.set_preconditioner(stokes.PRECOND_NONE)
     stokes.set_precision(stokes.PRECISION_MIXED if on else stokes.PRECISION_FP64)
 
 
@@ -45,32 +54,49 @@ def main(argv=None) -> int:
     ap.add_argument("--area-lost", type=float, default=0.5)
     ap.add_argument("--max-steps", type=int, default=40)
     ap.add_argument("--no-plain", action="store_true", help="skip the unpreconditioned solve")
+    ap.add_argument("--device-stepper", action="store_true",
+                    help="erode with the whole step in the library (erosion_rungekutta2)")
+    ap.add_argument("--body-modes", type=int, nargs="*", default=[],
+                    help="also solve the eroded state with these body preconditioner sizes")
+    ap.add_argument("--save", default="", help="write the eroded boundary (.npz)")
+    ap.add_argument("--run-body-modes", type=int, default=None,
+                    help="body preconditioner size during the run (default: the library's)")
     a = ap.parse_args(argv)
     w = WORKLOADS[a.workload]
     params = ParamSet(npts=w.npts, nouter=w.nouter, ibary=1, ifmm=1, ibc=w.ibc, dt=w.dt,
                       maxl=8000, fixpdrop=True)
     stokes.set_gmres_tol(1e-10)
-    _accelerated(True)
+    _accelerated(True, a.run_body_modes)
     th = workload_geometry(w)
     area0 = float(body_areas(th).sum())
     guess = LR.WarmStart(w.nouter, ibc=w.ibc)
     steps, t_run, lost = 0, 0.0, 0.0
     timings = {}
+    per_step = []
+    if a.device_stepper:
+        from .device_stepper import rungekutta2_device
+        stokes.stepper_reset()
     while lost < a.area_lost and steps < a.max_steps:
         t0 = time.perf_counter()
-        th = rungekutta2(th, params, guess=guess, timings=timings)
+        before = timings.get("gmres_iters", 0)
+        if a.device_stepper:
+            th = rungekutta2_device(th, params, warm=2, timings=timings, guess=guess.net)
+        else:
+            th = rungekutta2(th, params, guess=guess, timings=timings)
         t_run += time.perf_counter() - t0
         steps += 1
+        per_step.append(int(timings.get("gmres_iters", 0) - before))
         lost = 1.0 - float(body_areas(th).sum()) / area0
         log(f"step {steps}: {len(th.thlenvec)} bodies, area lost {100 * lost:.1f}%, "
-            f"{timings.get('gmres_iters', 0) / max(1, timings.get('solves', 1)):.1f} "
-            f"iterations per solve so far")
+            f"{per_step[-1]} GMRES iterations this step")
     m, xx, yy = getnxy(th)
     n, no = w.npts, w.nouter
-    prev = guess.prev(th)
+    if a.save:
+        np.savez(a.save, xx=xx, yy=yy, n=n, m=m, no=no)
+    prev = guess.prev(th) if not a.device_stepper else None
 
-    def solve(label, accel, den0=None):
-        _accelerated(accel)
+    def solve(label, accel, den0=None, body_modes=None):
+        _accelerated(accel, body_modes if body_modes is not None else a.run_body_modes)
         t0 = time.perf_counter()
         den, it = stokes.stokessolver(n, m, no, 1, 1, w.ibc, params.maxl, xx, yy, den0)
         dt = time.perf_counter() - t0
@@ -88,6 +114,9 @@ def main(argv=None) -> int:
         _, res["accelerated_previous"] = solve("accelerated, previous densities", True, prev)
         res["accelerated_previous"]["guess_relerr"] = float(
             np.linalg.norm(prev - d0) / np.linalg.norm(d0))
+    for k in a.body_modes:
+        _, res[f"accelerated_zeros_body_modes_{k}"] = solve(f"accelerated, zeros, body modes {k}",
+                                                              True, body_modes=k)
     if not a.no_plain:
         _, res["plain_zeros"] = solve("plain FP64, zeros", False)
     _accelerated(False)
@@ -96,6 +125,9 @@ def main(argv=None) -> int:
         "points_per_body": n, "wall_points": no, "gmres_tol": 1e-10,
         "area_lost": lost, "steps": steps, "steps_per_s_accelerated_warm": steps / t_run,
         "gmres_iters_per_solve_run": timings.get("gmres_iters", 0) / max(1, timings.get("solves", 1)),
+        "gmres_iters_per_step": per_step,
+        "body_modes_run": list(stokes.get_preconditioner())[1] if a.run_body_modes is None
+        else a.run_body_modes,
         "learned_guesses_used_in_run": guess.learned_used,
         "solves": res,
     }), flush=True)

@@ -184,13 +184,15 @@ int erosion_measure_copy(int64_t bytes, int64_t reps, double* ms);
  * kind 0: none (default, the reference's iteration).  kind 1: per-curve
  * two-level preconditioner -- on each curve (wall, every body) the exact
  * inverse of the curve's own diagonal operator block on its Fourier modes
- * 0..K (K = wall_modes for the wall, body_modes for a body, each in [1, 16];
- * a body's block includes its stokeslet/rotlet columns and constraint rows),
- * and -2 (the inverse of the jump -1/2) on the remaining modes.  GMRES then
- * iterates on A P^{-1}; its residual is still that of A x = b, so the stopping
- * test and the returned density keep their meaning and only the iteration
- * count changes.  A solve fails if npts <= 2 body_modes + 1 or
- * nouter <= 2 wall_modes + 1.  Returns 0, or -1 on invalid arguments. */
+ * 0..K (K = wall_modes for the wall, body_modes for a body, each in [1, 16],
+ * lowered to (points - 2)/2 on a curve with fewer points; a body's block
+ * includes its stokeslet/rotlet columns and constraint rows), and -2 (the
+ * inverse of the jump -1/2) on the remaining modes.  GMRES then iterates on
+ * A P^{-1}; its residual is still that of A x = b, so the stopping test and
+ * the returned density keep their meaning and only the iteration count
+ * changes.  16 and 16 are the defaults of erosion_get_preconditioner before
+ * any call (eroded, non-circular bodies need the higher body modes).
+ * Returns 0, or -1 on invalid arguments. */
 int erosion_set_preconditioner(int kind, int body_modes, int wall_modes);
 /* out[0] kind, out[1] body modes, out[2] wall modes.  Returns 0 or -1. */
 int erosion_get_preconditioner(int* out);

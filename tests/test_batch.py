@@ -47,14 +47,14 @@ def test_batch_equals_one_by_one(st, accel):
 
 
 def test_batch_isolates_a_failing_geometry(st):
-    """A geometry whose point count is too small for the preconditioner's
-    modes fails alone; the others are solved."""
+    """A geometry with too few points per body for the preconditioner (3: no
+    Fourier mode besides the mean) fails alone; the others are solved."""
     from erosion_amd.geometry import make_bodies
     from erosion_amd.theta_len import getnxy, new_thlenden
     st.set_preconditioner(1, 8, 16)
     geoms = _geoms()[:3]
-    _, xx, yy = getnxy(new_thlenden(make_bodies(2, 16, 0.1, 0.12, 0.1, 9)))
-    geoms.insert(1, (16, 2, 256, xx, yy))
+    _, xx, yy = getnxy(new_thlenden(make_bodies(2, 3, 0.1, 0.12, 0.1, 9)))
+    geoms.insert(1, (3, 2, 256, xx, yy))
     res = st.solve_batch(geoms, workers=2)
     assert [r["status"] for r in res] == [0, -1, 0, 0]
-    assert "geometry 1" in st.last_error()
+    assert "geometry 1" in st.last_error() and "points per body" in st.last_error()

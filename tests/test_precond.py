@@ -178,8 +178,14 @@ def test_gpu_preconditioner_mode_limits(pc_reset):
         stokes.set_preconditioner(1, 8, 17)
     with pytest.raises(stokes.StokesError):
         stokes.set_preconditioner(2)
+    # 32 points per body hold modes 0..15: 16 body modes are lowered to 15
     thl = make_bodies(2, 32, 0.08, 0.12, 0.05, 5)
     _, xx, yy = getnxy(new_thlenden(thl))
     stokes.set_preconditioner(1, 16, 16)
-    with pytest.raises(stokes.StokesError, match="body modes"):
-        stokes.stokessolver(32, 2, 128, 1, 1, 0, 100, xx, yy)
+    stokes.set_gmres_tol(1e-10)
+    den16, it16 = stokes.stokessolver(32, 2, 128, 1, 1, 0, 500, xx, yy)
+    assert stokes.last_status() == 0 and stokes.get_preconditioner() == (1, 16, 16)
+    stokes.set_preconditioner(1, 15, 16)
+    den15, it15 = stokes.stokessolver(32, 2, 128, 1, 1, 0, 500, xx, yy)
+    assert it15 == it16 and np.array_equal(den15, den16)
+    stokes.set_gmres_tol(1e-8)
