@@ -503,6 +503,18 @@ int erosion_spectral(int op, int64_t npts, int64_t rows, const double* in, const
     }
 }
 
+int erosion_set_farfield_shape(int index) {
+    try {
+        if (index < -1 || index >= stokes::farfield_shape_count()) return -1;
+        stokes::ctx().ff_force = index;
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_set_farfield_shape", e.what());
+    }
+}
+
+int erosion_farfield_shape_count(void) { return stokes::farfield_shape_count(); }
+
 int erosion_set_graphs(int on) {
     try {
         if (on != 0 && on != 1) return -1;

@@ -250,6 +250,7 @@ struct Context {
     std::map<int, int> ff_choice;      // boundary points -> far-field shape index
     std::map<int, float> ff_tuned_ms;  // boundary points -> tuned launch time
     std::map<int, int> ff32_choice;    // the same for the FP32 far field
+    int ff_force = -1;                 // a fixed far-field shape index (tests), -1 tunes
     std::map<int, float> ff32_tuned_ms;
     // FP32 far field: per 256-point tile the FP64 centre, per source the FP32
     // offset from it and the FP32 weighted normal; built once per geometry
@@ -317,6 +318,7 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary);
 void prepare_operator(Context& c, const double* x_dev, bool fp32);
 // far-field shape index in use for the current size (after prepare_operator)
 int farfield_choice(const Context& c, bool fp32);
+int farfield_shape_count();
 void gather_density(Context& c, const double* den_dev);
 void boundary_values(Context& c, const int* curves_dev, int ncurves, int wall_first);
 void bd_condition(Context& c, int ibc, double* rhs_dev);

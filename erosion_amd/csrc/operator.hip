@@ -91,10 +91,59 @@ __device__ __forceinline__ void ff_tile(const double2* __restrict__ sxy,
     }
 }
 
+// The same sum for TPT = 2 or 4 targets with one reciprocal per source: the
+// product P of the targets' rho^2, q = 1/P (estimate + one Newton step), and
+// 1/rho_k^2 = q times the other factors of P.  Three of four hardware
+// reciprocals become three multiplies.  Off-diagonal tiles only (a self pair
+// has rho^2 = 0); padding sources sit at 1e30, so P stays finite.
+template <int TPT>
+__device__ __forceinline__ void ff_tile_shared(const double2* __restrict__ sxy,
+                                               const double2* __restrict__ sm,
+                                               const double2* __restrict__ se, const double* xt,
+                                               const double* yt, double* u, double* v) {
+    static_assert(TPT == 2 || TPT == 4, "shared reciprocal over 2 or 4 targets");
+#pragma unroll 4
+    for (int jj = 0; jj < kTile; ++jj) {
+        const double2 p = sxy[jj];
+        const double2 mm = sm[jj];
+        const double2 e = se[jj];
+        double rx[TPT], ry[TPT], r2[TPT], ab[TPT], inv[TPT];
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            rx[q] = xt[q] - p.x;
+            ry[q] = yt[q] - p.y;
+            r2[q] = __builtin_fma(rx[q], rx[q], ry[q] * ry[q]);
+            const double a = __builtin_fma(rx[q], mm.x, ry[q] * mm.y);
+            const double b = __builtin_fma(rx[q], e.x, ry[q] * e.y);
+            ab[q] = a * b;
+        }
+        if (TPT == 4) {
+            const double p01 = r2[0] * r2[1], p23 = r2[2 % TPT] * r2[3 % TPT];
+            const double qq = rcp64_1(p01 * p23);
+            const double q01 = qq * p23, q23 = qq * p01;  // 1/p01, 1/p23
+            inv[0] = q01 * r2[1];
+            inv[1] = q01 * r2[0];
+            inv[2 % TPT] = q23 * r2[3 % TPT];
+            inv[3 % TPT] = q23 * r2[2 % TPT];
+        } else {
+            const double qq = rcp64_1(r2[0] * r2[1]);
+            inv[0] = qq * r2[1];
+            inv[1] = qq * r2[0];
+        }
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            const double c = ab[q] * (inv[q] * inv[q]);
+            u[q] = __builtin_fma(c, rx[q], u[q]);
+            v[q] = __builtin_fma(c, ry[q], v[q]);
+        }
+    }
+}
+
 // grid: (ceil(npts / (256 TPT)), source splits); block 256 threads.  Lane l
 // of block b owns targets (b TPT + q) 256 + l, q < TPT; the block stages
 // 256-source tiles of its split in LDS (double2 SoA, wave-uniform reads).
-template <int TPT>
+// SHR: off-diagonal tiles share one reciprocal over the lane's targets.
+template <int TPT, bool SHR>
 __global__ __launch_bounds__(kBlk) void k_farfield(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ mx,
     const double* __restrict__ my, const double* __restrict__ ex, const double* __restrict__ ey,
@@ -123,7 +172,7 @@ __global__ __launch_bounds__(kBlk) void k_farfield(
             sm[tid] = make_double2(mx[j], my[j]);
             se[tid] = make_double2(ex[j], ey[j]);
         } else {  // padding source: zero weight far away
-            sxy[tid] = make_double2(1.0e100, 1.0e100);
+            sxy[tid] = make_double2(1.0e30, 1.0e30);
             sm[tid] = make_double2(0.0, 0.0);
             se[tid] = make_double2(0.0, 0.0);
         }
@@ -134,6 +183,8 @@ __global__ __launch_bounds__(kBlk) void k_farfield(
 #pragma unroll
             for (int q = 0; q < TPT; ++q) self[q] = t[q] - base;
             ff_tile<TPT, true>(sxy, sm, se, xt, yt, self, u, v);
+        } else if (SHR) {
+            if constexpr (TPT > 1) ff_tile_shared<TPT>(sxy, sm, se, xt, yt, u, v);
         } else {
             ff_tile<TPT, false>(sxy, sm, se, xt, yt, nullptr, u, v);
         }
@@ -296,21 +347,28 @@ __global__ __launch_bounds__(kBlk) void k_farfield32(
     }
 }
 
-// Far-field launch shape: targets per lane and the grid size it aims for.
+// Far-field launch shape: targets per lane, the grid size it aims for, and
+// (FP64 kernel) whether a lane's targets share one reciprocal per source;
+// the FP32 kernel ignores `shared`.
 struct FFShape {
     int tpt;
     int blocks;
+    bool shared;
 };
-constexpr FFShape kFFShapes[] = {{1, 2048}, {1, 4096}, {2, 1024}, {2, 2048},
-                                 {2, 4096}, {4, 1024}, {4, 2048}};
+constexpr FFShape kFFShapes[] = {{1, 2048, false}, {1, 4096, false}, {2, 1024, false},
+                                 {2, 2048, false}, {2, 4096, false}, {4, 1024, false},
+                                 {4, 2048, false}, {2, 2048, true},  {4, 1024, true},
+                                 {4, 2048, true}};
 
 struct FFPlan {
     int tpt, tblocks, split_len, nsplit;
+    bool shared;
 };
 
 FFPlan ff_plan(int np, FFShape sh) {
     FFPlan p{};
     p.tpt = sh.tpt;
+    p.shared = sh.shared && sh.tpt > 1;
     p.tblocks = cdiv_int(np, kBlk * sh.tpt);
     const int want = std::max(1, cdiv_int(sh.blocks, p.tblocks));
     p.split_len = cdiv_int(cdiv_int(np, want), kTile) * kTile;
@@ -347,20 +405,20 @@ void ff32_launch(const Context& c, const FFPlan& p, double* pu, double* pv) {
 void ff_launch(const Context& c, const FFPlan& p, double* pu, double* pv) {
     const Geometry& g = c.g;
     const dim3 grid(p.tblocks, p.nsplit);
-    switch (p.tpt) {
-    case 1:
-        k_farfield<1><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
-                                                   g.npts, p.split_len, pu, pv);
-        break;
-    case 2:
-        k_farfield<2><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
-                                                   g.npts, p.split_len, pu, pv);
-        break;
-    default:
-        k_farfield<4><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
-                                                   g.npts, p.split_len, pu, pv);
-        break;
-    }
+#define STOKES_FF(T, S)                                                                      \
+    k_farfield<T, S><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p, \
+                                                  g.npts, p.split_len, pu, pv)
+    if (p.tpt == 1)
+        STOKES_FF(1, false);
+    else if (p.tpt == 2 && p.shared)
+        STOKES_FF(2, true);
+    else if (p.tpt == 2)
+        STOKES_FF(2, false);
+    else if (p.shared)
+        STOKES_FF(4, true);
+    else
+        STOKES_FF(4, false);
+#undef STOKES_FF
     STOKES_LAUNCH_CHECK();
 }
 
@@ -842,7 +900,10 @@ void boundary_values(Context& c, const int* curves_dev, int ncurves, int wall_fi
 // Autotuner: the first operator application at a given number of boundary
 // points times every far-field shape (one warm-up launch, then three timed
 // launches each, HIP events on the library stream) and keeps the fastest.
+// Shapes include the shared-reciprocal FP64 tiles, so they are used only
+// where they measure faster.
 static int tune_farfield(Context& c, bool fp32) {
+    if (c.ff_force >= 0) return c.ff_force;
     const int np = c.g.npts;
     std::map<int, int>& choice = fp32 ? c.ff32_choice : c.ff_choice;
     auto it = choice.find(np);
@@ -854,6 +915,7 @@ static int tune_farfield(Context& c, bool fp32) {
     STOKES_HIP(hipEventCreate(&e0));
     STOKES_HIP(hipEventCreate(&e1));
     for (int i = 0; i < nshape; ++i) {
+        if (fp32 && kFFShapes[i].shared) continue;  // the FP32 kernel has no shared form
         const FFPlan p = ff_plan(np, kFFShapes[i]);
         c.part_u.ensure(static_cast<size_t>(np) * static_cast<size_t>(p.nsplit));
         c.part_v.ensure(static_cast<size_t>(np) * static_cast<size_t>(p.nsplit));
@@ -923,7 +985,10 @@ void prepare_operator(Context& c, const double* x_dev, bool fp32) {
     stokes_logs(c);
 }
 
+int farfield_shape_count() { return static_cast<int>(sizeof(kFFShapes) / sizeof(kFFShapes[0])); }
+
 int farfield_choice(const Context& c, bool fp32) {
+    if (c.ff_force >= 0) return c.ff_force;
     const std::map<int, int>& choice = fp32 ? c.ff32_choice : c.ff_choice;
     auto it = choice.find(c.g.npts);
     return it == choice.end() ? -1 : it->second;

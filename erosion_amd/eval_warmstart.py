@@ -67,6 +67,7 @@ def main(argv=None) -> int:
                       maxl=8000, fixpdrop=True)
     stokes.set_gmres_tol(1e-10)
     _accelerated(True, a.run_body_modes)
+    modes_run = stokes.get_preconditioner()[1]
     th = workload_geometry(w)
     area0 = float(body_areas(th).sum())
     guess = LR.WarmStart(w.nouter, ibc=w.ibc)
@@ -126,8 +127,7 @@ def main(argv=None) -> int:
         "area_lost": lost, "steps": steps, "steps_per_s_accelerated_warm": steps / t_run,
         "gmres_iters_per_solve_run": timings.get("gmres_iters", 0) / max(1, timings.get("solves", 1)),
         "gmres_iters_per_step": per_step,
-        "body_modes_run": list(stokes.get_preconditioner())[1] if a.run_body_modes is None
-        else a.run_body_modes,
+        "body_modes_run": modes_run,
         "learned_guesses_used_in_run": guess.learned_used,
         "solves": res,
     }), flush=True)

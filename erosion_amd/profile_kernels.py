@@ -38,6 +38,8 @@ def main(argv=None) -> int:
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--copy-mib", type=int, default=256)
     ap.add_argument("--no-stress", action="store_true")
+    ap.add_argument("--shapes", action="store_true",
+                    help="time every far-field launch shape (FP64 and FP32)")
     a = ap.parse_args(argv)
     L = stokes.lib()
     if a.copy_mib > 0:
@@ -65,6 +67,16 @@ def main(argv=None) -> int:
             print(f"near-pair gather pass: {r['near_entries']} entries, "
                   f"{r['gather_bytes'] / 1e6:.2f} MB, {r['near_ms']:.4f} ms, {gbs:.0f} GB/s "
                   f"({gbs / PEAK_HBM_GBS:.3f} of HBM)", flush=True)
+    if a.shapes:
+        for mode, name in ((stokes.PRECISION_FP64, "FP64"), (stokes.PRECISION_FP32_FARFIELD, "FP32")):
+            stokes.set_precision(mode)
+            for k in range(stokes.farfield_shape_count()):
+                stokes.set_farfield_shape(k)
+                r = stokes.time_matvec(n, m, no, xx, yy, eta, reps=a.reps)
+                tf = 20.0 * r["pairs"] / (r["farfield_ms"] * 1e-3) / 1e12
+                print(f"{name} shape {k}: {r['farfield_ms']:.4f} ms per launch, {tf:.2f} TFLOP/s, "
+                      f"{r['splits']} splits", flush=True)
+        stokes.set_farfield_shape(-1)
     stokes.set_precision(stokes.PRECISION_FP64)
     if not a.no_stress:
         stokes.set_gmres_tol(1e-8)

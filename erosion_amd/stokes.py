@@ -63,6 +63,10 @@ def lib():
         L.erosion_set_preconditioner.restype = ctypes.c_int
         L.erosion_get_preconditioner.argtypes = [ctypes.POINTER(ctypes.c_int)]
         L.erosion_get_preconditioner.restype = ctypes.c_int
+        L.erosion_set_farfield_shape.argtypes = [ctypes.c_int]
+        L.erosion_set_farfield_shape.restype = ctypes.c_int
+        L.erosion_farfield_shape_count.argtypes = []
+        L.erosion_farfield_shape_count.restype = ctypes.c_int
         L.erosion_set_graphs.argtypes = [ctypes.c_int]
         L.erosion_set_graphs.restype = ctypes.c_int
         L.erosion_graph_stats.argtypes = [ctypes.POINTER(ctypes.c_longlong)]
@@ -239,6 +243,16 @@ def set_preconditioner(kind: int, body_modes: int = 16, wall_modes: int = 16) ->
     if lib().erosion_set_preconditioner(int(kind), int(body_modes), int(wall_modes)) != 0:
         raise StokesError(f"invalid preconditioner kind={kind} body_modes={body_modes} "
                           f"wall_modes={wall_modes}")
+
+
+def set_farfield_shape(index: int) -> None:
+    """Fix the far-field launch shape (-1: the autotuner chooses)."""
+    if lib().erosion_set_farfield_shape(int(index)) != 0:
+        raise StokesError(f"far-field shape {index} out of range")
+
+
+def farfield_shape_count() -> int:
+    return int(lib().erosion_farfield_shape_count())
 
 
 def set_graphs(on: bool) -> None:

@@ -236,6 +236,13 @@ int erosion_stepper_reset(void);
 int erosion_spectral(int op, int64_t npts, int64_t rows, const double* in, const double* fac,
                      double* out);
 
+/* Test and measurement hook: use far-field launch shape `index` (0 ..
+ * erosion_farfield_shape_count() - 1: targets per lane, grid size, shared
+ * reciprocal) for every size instead of the autotuner's choice; -1 restores
+ * the autotuner.  Returns 0, or -1 for an index out of range. */
+int erosion_set_farfield_shape(int index);
+int erosion_farfield_shape_count(void);
+
 /* HIP graphs for GMRES (no counterpart in the reference).  on = 1: stokessolver_
  * captures each chunk of iterations (8 or 16 iterations of preconditioner,
  * operator, two Gram-Schmidt passes, Givens update) into a HIP graph the first

@@ -56,6 +56,23 @@ def test_matvec_full_size_vs_oracle():
     assert np.sqrt(np.mean((v - vo) ** 2)) / np.sqrt(np.mean(vo ** 2)) < 1e-12
 
 
+def test_matvec_every_farfield_shape_vs_oracle():
+    """Each far-field launch shape the autotuner may pick (targets per lane,
+    grid, shared reciprocal), in FP64, on 20 bodies x 256 points."""
+    n, m, no, xx, yy = workload_xy(WORKLOADS["bodies100_n256"], nbodies=20)
+    N = 2 * no + 2 * n * m + 3 * m
+    eta = np.random.default_rng(11).standard_normal(N)
+    vo = O.Geometry(n, m, no, xx, yy).matvec(eta, 1)
+    try:
+        for k in range(stokes.farfield_shape_count()):
+            stokes.set_farfield_shape(k)
+            assert relerr(stokes.matvec(n, m, no, xx, yy, eta, ibary=1), vo) < 1e-12, k
+    finally:
+        stokes.set_farfield_shape(-1)
+    with pytest.raises(stokes.StokesError):
+        stokes.set_farfield_shape(stokes.farfield_shape_count())
+
+
 def test_matvec_tight_geometry_many_near_pairs():
     """Bodies 0.002 apart and against the wall: every near branch with many entries."""
     thl = make_bodies(30, 128, 0.05, 0.12, 0.002, 31)
