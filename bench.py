@@ -119,7 +119,7 @@ def cpu_baseline(w, xx, yy, iters_per_solve):
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="bodies100_n256")
     ap.add_argument("--tol", type=float, default=1e-10)
