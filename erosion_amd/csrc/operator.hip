@@ -101,7 +101,7 @@ __device__ __forceinline__ void ff_tile_shared(const double2* __restrict__ sxy,
                                                const double2* __restrict__ sm,
                                                const double2* __restrict__ se, const double* xt,
                                                const double* yt, double* u, double* v) {
-    static_assert(TPT == 2 || TPT == 4, "shared reciprocal over 2 or 4 targets");
+    static_assert(TPT == 2 || TPT == 4 || TPT == 8, "shared reciprocal over 2, 4 or 8 targets");
 #pragma unroll 4
     for (int jj = 0; jj < kTile; ++jj) {
         const double2 p = sxy[jj];
@@ -117,14 +117,20 @@ __device__ __forceinline__ void ff_tile_shared(const double2* __restrict__ sxy,
             const double b = __builtin_fma(rx[q], e.x, ry[q] * e.y);
             ab[q] = a * b;
         }
-        if (TPT == 4) {
-            const double p01 = r2[0] * r2[1], p23 = r2[2 % TPT] * r2[3 % TPT];
-            const double qq = rcp64_1(p01 * p23);
-            const double q01 = qq * p23, q23 = qq * p01;  // 1/p01, 1/p23
-            inv[0] = q01 * r2[1];
-            inv[1] = q01 * r2[0];
-            inv[2 % TPT] = q23 * r2[3 % TPT];
-            inv[3 % TPT] = q23 * r2[2 % TPT];
+        if (TPT >= 4) {
+            // groups of four targets (a product of eight could overflow)
+#pragma unroll
+            for (int g0 = 0; g0 < TPT; g0 += 4) {
+                const int i0 = g0 % TPT, i1 = (g0 + 1) % TPT, i2 = (g0 + 2) % TPT,
+                          i3 = (g0 + 3) % TPT;
+                const double p01 = r2[i0] * r2[i1], p23 = r2[i2] * r2[i3];
+                const double qq = rcp64_1(p01 * p23);
+                const double q01 = qq * p23, q23 = qq * p01;  // 1/p01, 1/p23
+                inv[i0] = q01 * r2[i1];
+                inv[i1] = q01 * r2[i0];
+                inv[i2] = q23 * r2[i3];
+                inv[i3] = q23 * r2[i2];
+            }
         } else {
             const double qq = rcp64_1(r2[0] * r2[1]);
             inv[0] = qq * r2[1];
@@ -358,17 +364,18 @@ struct FFShape {
 constexpr FFShape kFFShapes[] = {{1, 2048, false}, {1, 4096, false}, {2, 1024, false},
                                  {2, 2048, false}, {2, 4096, false}, {4, 1024, false},
                                  {4, 2048, false}, {2, 2048, true},  {4, 1024, true},
-                                 {4, 2048, true}};
+                                 {4, 2048, true},  {8, 1024, true},  {8, 2048, true}};
 
 struct FFPlan {
     int tpt, tblocks, split_len, nsplit;
     bool shared;
 };
 
-FFPlan ff_plan(int np, FFShape sh) {
+// The FP32 kernel takes at most 4 targets per lane and no shared reciprocal.
+FFPlan ff_plan(int np, FFShape sh, bool fp32) {
     FFPlan p{};
-    p.tpt = sh.tpt;
-    p.shared = sh.shared && sh.tpt > 1;
+    p.tpt = fp32 ? std::min(sh.tpt, 4) : sh.tpt;
+    p.shared = !fp32 && sh.shared && sh.tpt > 1;
     p.tblocks = cdiv_int(np, kBlk * sh.tpt);
     const int want = std::max(1, cdiv_int(sh.blocks, p.tblocks));
     p.split_len = cdiv_int(cdiv_int(np, want), kTile) * kTile;
@@ -414,6 +421,8 @@ void ff_launch(const Context& c, const FFPlan& p, double* pu, double* pv) {
         STOKES_FF(2, true);
     else if (p.tpt == 2)
         STOKES_FF(2, false);
+    else if (p.tpt == 8)
+        STOKES_FF(8, true);
     else if (p.shared)
         STOKES_FF(4, true);
     else
@@ -915,8 +924,9 @@ static int tune_farfield(Context& c, bool fp32) {
     STOKES_HIP(hipEventCreate(&e0));
     STOKES_HIP(hipEventCreate(&e1));
     for (int i = 0; i < nshape; ++i) {
-        if (fp32 && kFFShapes[i].shared) continue;  // the FP32 kernel has no shared form
-        const FFPlan p = ff_plan(np, kFFShapes[i]);
+        // the FP32 kernel has no shared form (and no 8-target form)
+        if (fp32 && kFFShapes[i].shared) continue;
+        const FFPlan p = ff_plan(np, kFFShapes[i], fp32);
         c.part_u.ensure(static_cast<size_t>(np) * static_cast<size_t>(p.nsplit));
         c.part_v.ensure(static_cast<size_t>(np) * static_cast<size_t>(p.nsplit));
         auto launch = [&]() {
@@ -979,7 +989,7 @@ void prepare_operator(Context& c, const double* x_dev, bool fp32) {
     // the first call allocate) it before timing any far-field shape
     gather_density(c, x_dev);
     if (fp32) ff32_sources(c);
-    const FFPlan plan = ff_plan(c.g.npts, kFFShapes[tune_farfield(c, fp32)]);
+    const FFPlan plan = ff_plan(c.g.npts, kFFShapes[tune_farfield(c, fp32)], fp32);
     c.part_u.ensure(static_cast<size_t>(c.g.npts) * static_cast<size_t>(plan.nsplit));
     c.part_v.ensure(static_cast<size_t>(c.g.npts) * static_cast<size_t>(plan.nsplit));
     stokes_logs(c);
@@ -1005,7 +1015,7 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary) {
     gather_density(c, x_dev);
     const bool fp32 = c.ff_fp32_now;
     if (fp32) ff32_sources(c);
-    const FFPlan plan = ff_plan(np, kFFShapes[tune_farfield(c, fp32)]);
+    const FFPlan plan = ff_plan(np, kFFShapes[tune_farfield(c, fp32)], fp32);
     const int nsplit = plan.nsplit;
     c.ff_splits = nsplit;
     c.ff_tpt = plan.tpt;
