@@ -199,6 +199,132 @@ __global__ __launch_bounds__(kBT) void k_bary_eval(
     }
 }
 
+// The MODE 1 evaluation of k_bary_eval for a few targets: one wave per
+// (target, curve) pair, the lanes striding over the curve's nodes and meeting
+// in cross-lane butterflies after each of the three passes.  A handful of
+// targets would leave k_bary_eval with one wave walking a 4096-point wall
+// alone; here every (target, curve) pair is a wave of its own.  Writes
+// part[(c*3 + {0,1,2}) * ntar + q] = u, v, vorticity, as k_bary_eval with one
+// curve per group.
+constexpr int kQoiWaves = 4;
+
+__device__ __forceinline__ double qoi_wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__global__ __launch_bounds__(64 * kQoiWaves) void k_qoi_wave(
+    NodeView nv, const double* __restrict__ ax, const double* __restrict__ ay, int no, int n,
+    int m, const double* __restrict__ tgt_x, const double* __restrict__ tgt_y, int ntar,
+    double* __restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const long long e = static_cast<long long>(blockIdx.x) * kQoiWaves + (threadIdx.x >> 6);
+    if (e >= static_cast<long long>(ntar) * (m + 1)) return;  // uniform per wave
+    const int c = static_cast<int>(e / ntar), q = static_cast<int>(e - static_cast<long long>(c) * ntar);
+    const int len = c == 0 ? no : n;
+    const int off = c == 0 ? 0 : no + (c - 1) * n;
+    const bool ext = c != 0;
+    const double xt = tgt_x[q], yt = tgt_y[q];
+    const cplx zt = cmk(xt, yt);
+    const cplx a = cmk(ax[c], ay[c]);
+    const size_t N = nv.N;
+
+    // pass 1: S0 = sum w, N_f = sum b_f w, w = dz/(z - x)
+    double v1[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) v1[i] = 0.0;
+    for (int k = lane; k < len; k += 64) {
+        const int p = off + k;
+        const cplx d = cmk(nv.x[p] - xt, nv.y[p] - yt);
+        const cplx w = cmul(cmk(nv.dzr[p], nv.dzi[p]), cinv(d));
+        v1[0] += w.r;
+        v1[1] += w.i;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            const cplx b = cmk(nv.bd[(2 * f) * N + p], nv.bd[(2 * f + 1) * N + p]);
+            v1[2 + 2 * f] = __builtin_fma(b.r, w.r, __builtin_fma(-b.i, w.i, v1[2 + 2 * f]));
+            v1[3 + 2 * f] = __builtin_fma(b.r, w.i, __builtin_fma(b.i, w.r, v1[3 + 2 * f]));
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 12; ++i) v1[i] = qoi_wave_sum(v1[i]);
+    const cplx S0 = cmk(v1[0], v1[1]);
+    const cplx idv = cinv(ext ? cmk(S0.r, S0.i - kTwoPi) : S0);
+    cplx u[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) u[f] = cmul(cmk(v1[2 + 2 * f], v1[3 + 2 * f]), idv);
+
+    // pass 2: M_f = sum (b_f - u_f) w/(z-x); exterior D_a = sum w/(z-a)
+    double v2[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) v2[i] = 0.0;
+    for (int k = lane; k < len; k += 64) {
+        const int p = off + k;
+        const cplx d = cmk(nv.x[p] - xt, nv.y[p] - yt);
+        const double rho2 = __builtin_fma(d.r, d.r, d.i * d.i);
+        const double ir = rcp64(rho2);
+        const cplx id = cmk(d.r * ir, -d.i * ir);
+        const cplx w = cmul(cmk(nv.dzr[p], nv.dzi[p]), id);
+        if (ext && !(sqrt(rho2) > 1.0e-10)) {
+#pragma unroll
+            for (int f = 0; f < 5; ++f) {
+                const cplx t = coincident_term(nv, off, len, k, f, zt, a, d);
+                v2[2 * f] += t.r;
+                v2[2 * f + 1] += t.i;
+            }
+        } else {
+            const cplx w2 = cmul(w, id);
+#pragma unroll
+            for (int f = 0; f < 5; ++f) {
+                const cplx t = csub(cmk(nv.bd[(2 * f) * N + p], nv.bd[(2 * f + 1) * N + p]), u[f]);
+                v2[2 * f] = __builtin_fma(t.r, w2.r, __builtin_fma(-t.i, w2.i, v2[2 * f]));
+                v2[2 * f + 1] = __builtin_fma(t.r, w2.i, __builtin_fma(t.i, w2.r, v2[2 * f + 1]));
+            }
+        }
+        if (ext) {
+            const cplx wa = cmul(w, cmk(nv.iar[p], nv.iai[p]));
+            v2[10] += wa.r;
+            v2[11] += wa.i;
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 12; ++i) v2[i] = qoi_wave_sum(v2[i]);
+    const cplx iden = cinv(ext ? cmul(csub(zt, a), cmk(v2[10], v2[11])) : S0);
+    cplx du[5];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) du[f] = cmul(cmk(v2[2 * f], v2[2 * f + 1]), iden);
+
+    // pass 3 (fields 0..2): P_f = sum (b_f - u_f - du_f (z-x)) w/(z-x)^2
+    double v3[6] = {0, 0, 0, 0, 0, 0};
+    for (int k = lane; k < len; k += 64) {
+        const int p = off + k;
+        const cplx d = cmk(nv.x[p] - xt, nv.y[p] - yt);
+        const cplx id = cinv(d);
+        const cplx w3 = cmul(cmul(cmul(cmk(nv.dzr[p], nv.dzi[p]), id), id), id);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            const cplx t = csub(csub(cmk(nv.bd[(2 * f) * N + p], nv.bd[(2 * f + 1) * N + p]), u[f]),
+                                cmul(du[f], d));
+            v3[2 * f] = __builtin_fma(t.r, w3.r, __builtin_fma(-t.i, w3.i, v3[2 * f]));
+            v3[2 * f + 1] = __builtin_fma(t.r, w3.i, __builtin_fma(t.i, w3.r, v3[2 * f + 1]));
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 6; ++i) v3[i] = qoi_wave_sum(v3[i]);
+    if (lane != 0) return;
+    cplx dd[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) dd[f] = cscale(cmul(cmk(v3[2 * f], v3[2 * f + 1]), iden), 2.0);
+    // velocity (nder=1 outputs) and vorticity v_x - u_y (stokesSolver.f:5297-5376)
+    const double u1y = -dd[0].i + xt * dd[1].i - du[2].r + yt * dd[2].i - du[3].i;
+    const double u2x = -dd[0].i + du[1].i + xt * dd[1].i + yt * dd[2].i + du[4].r;
+    part[(static_cast<size_t>(c) * 3 + 0) * ntar + q] =
+        du[0].r - xt * du[1].r - yt * du[2].r + u[3].r;
+    part[(static_cast<size_t>(c) * 3 + 1) * ntar + q] =
+        -du[0].i + xt * du[1].i + yt * du[2].i + u[4].r;
+    part[(static_cast<size_t>(c) * 3 + 2) * ntar + q] = u2x - u1y;
+}
+
 // computeQoiTargets (stokesSolver.f:3099-3323), per target: trapezoid
 // pressure from every source node (and, for ibary=0, trapezoid velocity and
 // vorticity), closest body node for the inside/outside test of
@@ -603,8 +729,16 @@ void qoi_targets(Context& c, int ibary, const double* den_dev, int ntar, const d
         c.spart.ensure(static_cast<size_t>(G) * 3 * static_cast<size_t>(ntar));
         NodeView nv{g.x.p, g.y.p, g.dzr.p, g.dzi.p, c.bd.p, g.iar.p, g.iai.p, g.mx.p, g.my.p,
                     c.ex.p, c.ey.p, static_cast<size_t>(g.npts)};
-        k_bary_eval<1><<<dim3(cdiv_int(ntar, kBT), G), kBT, 0, s>>>(
-            c.grp_beg.p, c.grp_end.p, nv, g.ax.p, g.ay.p, no, n, tx_dev, ty_dev, ntar, c.spart.p);
+        if (ntar <= 64) {
+            // few targets: one wave per (target, curve)
+            const long long pairs = static_cast<long long>(ntar) * G;
+            k_qoi_wave<<<static_cast<int>((pairs + kQoiWaves - 1) / kQoiWaves), 64 * kQoiWaves, 0,
+                         s>>>(nv, g.ax.p, g.ay.p, no, n, m, tx_dev, ty_dev, ntar, c.spart.p);
+        } else {
+            k_bary_eval<1><<<dim3(cdiv_int(ntar, kBT), G), kBT, 0, s>>>(
+                c.grp_beg.p, c.grp_end.p, nv, g.ax.p, g.ay.p, no, n, tx_dev, ty_dev, ntar,
+                c.spart.p);
+        }
         STOKES_LAUNCH_CHECK();
     }
     k_qoi_final<<<cdiv_int(ntar, kBlk), kBlk, 0, s>>>(c.spart.p, G, ntar, m, c.iside.p, out_dev);

@@ -200,6 +200,22 @@ def test_qoi_targets_match_reference(golden, name, ibary, key):
         assert relerr(out[i], g[key][i]) < 1e-10, i
 
 
+def test_qoi_few_targets_path_matches_many_targets_path(golden):
+    """Up to 64 targets take one wave per (target, curve); more take the
+    tiled kernel.  The same targets through both agree to 1e-11 of the
+    largest value (the sums run in different orders; near a boundary their
+    terms grow like 1/rho^3)."""
+    g = golden("bodies10_n128")
+    n, m, no, xx, yy = _case(g)
+    rng = np.random.default_rng(12)
+    xt = rng.uniform(-1.5, 1.5, 150)
+    yt = rng.uniform(-0.9, 0.9, 150)
+    many = stokes.computeqoitargets(n, m, no, 1, xx, yy, g["den_tol12"], xt, yt)
+    few = stokes.computeqoitargets(n, m, no, 1, xx, yy, g["den_tol12"], xt[:40], yt[:40])
+    for i in range(4):
+        assert np.max(np.abs(few[i] - many[i][:40])) <= 1e-11 * max(1.0, np.max(np.abs(many[i]))), i
+
+
 def test_pipe_flow_pressure_drop():
     """No bodies, parabolic inflow u = 1 - y^2: the pressure drop between
     x = -2 and x = 2 is close to the Poiseuille value 8 (DensityStress.jl:158)."""
