@@ -38,10 +38,12 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak
 PEAK_HBM_GBS = 8000.0
 FF_FLOP_PER_PAIR = 20.0  # canonical flops per ordered source-target pair
 STRESS_FLOP_PER_PAIR = 130.0  # per (target body point, source node) pair: 3 passes x 5 fields
-# HBM bytes per FP64 far-field launch on the 100-body workload, from two
-# rocprofv3 counter passes (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00, factors
-# from a 256 MiB device copy); the per-split partial sums are most of it.
-FF_TRAFFIC_BYTES = {"bodies100_n256": 34.6e6}
+# HBM bytes read per FP64 far-field launch on the 100-body workload, from a
+# rocprofv3 FETCH_SIZE pass (x 2.00, the factor a 256 MiB device copy gives);
+# the same for every launch shape.  The bytes written equal the per-split
+# partial sums (a WRITE_SIZE pass reads exactly splits x points x 16 B), so
+# the traffic of the shape in use is this plus that.
+FF_FETCH_BYTES = {"bodies100_n256": 10.5e6}
 
 
 def log(*a):
@@ -257,6 +259,7 @@ def main() -> int:
     steps_per_s = sa["steps_per_s"]
     iters_per_solve = sp["gmres_iters_per_solve"] if sp else None
     ff = kt["farfield"]
+    ff_splits = acc["solve_info"][-1]["ff_splits"] if acc["solve_info"] else 0
     ff_ms = ff["ms"] / max(1, ff["launches"])
     ff_pairs = ff["work"] / max(1, ff["launches"])
     ff_tflops = FF_FLOP_PER_PAIR * ff_pairs / (ff_ms * 1e-3) / 1e12
@@ -341,7 +344,8 @@ def main() -> int:
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": ff_tflops / PEAK_FP64_TFLOPS,
-            "traffic": FF_TRAFFIC_BYTES.get(w.name),
+            "traffic": (FF_FETCH_BYTES[w.name] + 16.0 * ff_splits * (w.nbodies * w.npts + w.nouter)
+                        if w.name in FF_FETCH_BYTES and ff_splits else None),
             "traffic_unit": "bytes per launch",
         },
         "cpu_baseline": cpu,
