@@ -120,7 +120,7 @@ __global__ void k_curve_means(const double* __restrict__ x, const double* __rest
     s[3][threadIdx.x] = qy;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w)
+        if (static_cast<int>(threadIdx.x) < w)
             for (int q = 0; q < 4; ++q) s[q][threadIdx.x] += s[q][threadIdx.x + w];
         __syncthreads();
     }

@@ -37,13 +37,12 @@ constexpr int kChunkIters = 16;
 constexpr double kPcTolFrac = 0.5;
 constexpr int kMaxCycles = 4;
 // mixed precision: the far field runs in FP32 once the implicit residual is
-// below mixed_switch * tol (default kMixedSwitchDefault).  An inexact operator whose error is a fraction
-// eps of ||A x|| perturbs the attainable residual by about eps * ||r_k|| /
-// ||b|| at iteration k (Simoncini and Szyld 2003), so with eps ~ 1e-7 for the
-// FP32 far field the switch costs nothing measurable at 1e4 * tol; the true
-// residual is always formed in FP64 and a cycle that misses the tolerance
-// restarts.
-constexpr double kMixedSwitchDefault = 1.0e4;
+// below mixed_switch * tol (default 1e4, Context::mixed_switch).  An inexact
+// operator whose error is a fraction eps of ||A x|| perturbs the attainable
+// residual by about eps * ||r_k|| / ||b|| at iteration k (Simoncini and Szyld
+// 2003), so with eps ~ 1e-7 for the FP32 far field the switch costs nothing
+// measurable at 1e4 * tol; the true residual is always formed in FP64 and a
+// cycle that misses the tolerance restarts.
 
 inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
 inline size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
@@ -52,7 +51,7 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
     sh[threadIdx.x] = v;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        if (static_cast<int>(threadIdx.x) < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
     double r = sh[0];

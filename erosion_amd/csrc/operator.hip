@@ -482,7 +482,7 @@ __global__ __launch_bounds__(64 * NSL) void k_bdval(
     const double* __restrict__ dt = c == 0 ? dno : dn;
     const int lane = threadIdx.x & 63, slice = threadIdx.x >> 6;
     const int jl = blockIdx.x * 64 + lane;
-    if (blockIdx.x * 64 >= len) return;  // uniform per block
+    if (static_cast<int>(blockIdx.x) * 64 >= len) return;  // uniform per block
     const bool valid = jl < len;
     const int j = valid ? jl : 0;
     const size_t N = static_cast<size_t>(npts);
@@ -724,7 +724,7 @@ __global__ __launch_bounds__(kBlk) void k_reduce_small(
     s[2][threadIdx.x] = a2;
     __syncthreads();
     for (int w = kBlk / 2; w > 0; w >>= 1) {
-        if (threadIdx.x < w) {
+        if (static_cast<int>(threadIdx.x) < w) {
             s[0][threadIdx.x] += s[0][threadIdx.x + w];
             s[1][threadIdx.x] += s[1][threadIdx.x + w];
             s[2][threadIdx.x] += s[2][threadIdx.x + w];

@@ -379,7 +379,7 @@ __global__ __launch_bounds__(kBlk) void k_qoi_trap(
     si[threadIdx.x] = imin;
     __syncthreads();
     for (int s = kBlk / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
+        if (static_cast<int>(threadIdx.x) < s) {
             for (int k = 0; k < 4; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + s];
             const double d2 = sh[4][threadIdx.x + s];
             const int i2 = si[threadIdx.x + s];
