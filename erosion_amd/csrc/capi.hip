@@ -408,7 +408,7 @@ int erosion_get_precision(void) {
 int erosion_set_preconditioner(int kind, int body_modes, int wall_modes) {
     try {
         if (kind != 0 && kind != 1) return -1;
-        if (kind == 1 && (body_modes < 1 || body_modes > 16 || wall_modes < 1 || wall_modes > 16))
+        if (kind == 1 && (body_modes < 1 || body_modes > 32 || wall_modes < 1 || wall_modes > 32))
             return -1;
         Context& c = stokes::ctx();
         c.pc_kind = kind;

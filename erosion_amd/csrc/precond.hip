@@ -34,7 +34,9 @@ namespace {
 constexpr int kBlk = 256;
 constexpr int kApplyBlk = 256;
 constexpr int kQC = 17;                        // basis functions per pair pass
-constexpr int kMaxModes = 16;                  // K <= 16 per curve
+constexpr int kMaxModes = 32;                  // K <= 32 per curve
+constexpr int kChunkModes = 16;                // modes per register chunk in k_pc_apply
+constexpr int kChunkNF = 2 * kChunkModes + 1;  // slots per component and chunk
 constexpr int kMaxNF = 2 * kMaxModes + 1;      // functions per component
 constexpr int kMaxNQ = 2 * kMaxNF + 3;         // coarse unknowns per curve
 
@@ -166,6 +168,9 @@ __global__ __launch_bounds__(kBlk) void k_pc_pairs(
 // Gauss-Jordan elimination with partial (row) pivoting; writes G^{-1}
 // column-major (transposed) to out + blockIdx.x nq^2.  Coarse unknown order: (x, f) for
 // f < nf, then (y, f), then a body's stokeslet x, y and rotlet.
+// MAXNQ bounds nq: the LDS matrix of the 16-mode instance (69^2) leaves room
+// for several blocks per CU; the 32-mode one (133^2, 140 KB) for one.
+template <int MAXNQ>
 __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ px,
     const double* __restrict__ py, const double* __restrict__ cur,
@@ -173,9 +178,9 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
     const double* __restrict__ cy, const double* __restrict__ phi_w,
     const double* __restrict__ phi_b, int nfw, int nfb, int no, int n, int first_curve,
     const double* __restrict__ S, double* __restrict__ out) {
-    __shared__ double G[kMaxNQ * kMaxNQ];
-    __shared__ double fcol[kMaxNQ];
-    __shared__ int perm[kMaxNQ];
+    __shared__ double G[MAXNQ * MAXNQ];
+    __shared__ double fcol[MAXNQ];
+    __shared__ int perm[MAXNQ];
     const int c = first_curve + static_cast<int>(blockIdx.x);
     const bool wall = c == 0;
     const CurveView cv = curve_view(c, no, n, nfw, nfb, phi_w, phi_b);
@@ -332,18 +337,18 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
 }
 
 // z = P^{-1} r, one block per curve (wall = block 0, body b = block b + 1).
-// Each lane walks its points once, generating the basis functions by the
-// rotation e^{i(k+1)theta} = e^{ik theta} e^{i theta} (no table loads), and
-// keeps the 2 nf coefficient partials in registers; the partials meet in LDS
-// and are summed in two conflict-free stages, kVals at a time.  The inverse
-// coarse matrices are stored transposed, so d = G^{-1} c reads them
-// coalesced with one row per lane.
+// Each lane walks its points once per chunk of 16 modes, generating the
+// basis functions by the rotation e^{i(k+1)theta} = e^{ik theta} e^{i theta}
+// (no table loads), and keeps the chunk's coefficient partials in registers;
+// the partials meet in LDS and are summed in two conflict-free stages, kVals
+// at a time.  The inverse coarse matrices are stored transposed, so
+// d = G^{-1} c reads them coalesced with one row per lane.
 __global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
     int nfw, int nfb, int no, int n, int m, const double* __restrict__ Gw,
     const double* __restrict__ Gb, const double* __restrict__ r, double* __restrict__ z) {
     constexpr int kVals = 24;
     constexpr int kSeg = 16;
-    constexpr int kMaxNC = 2 * kMaxNF;
+    constexpr int kChunkNC = 2 * kChunkNF;
     __shared__ double part[kVals][kApplyBlk];
     __shared__ double red[kVals][kSeg];
     __shared__ double cc[kMaxNQ], dd[kMaxNQ];
@@ -359,57 +364,70 @@ __global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
     const int tid = threadIdx.x;
     const double c0 = 1.0 / sqrt(static_cast<double>(len));
     const double c1 = sqrt(2.0 / static_cast<double>(len));
+    const int nchunks = K > 0 ? (K + kChunkModes - 1) / kChunkModes : 1;
 
-    // partials in fixed slots: x coefficients 0..kMaxNF-1, y coefficients after
-    double ax[kMaxNF], ay[kMaxNF];
+    for (int ch = 0; ch < nchunks; ++ch) {
+        // chunk ch holds modes kb+1 .. kb+16 in slots 2l-1 (cos), 2l (sin),
+        // l = 1..16, and (chunk 0 only) the mean in slot 0; x slots first
+        const int kb = ch * kChunkModes;
+        double ax[kChunkNF], ay[kChunkNF];
 #pragma unroll
-    for (int v = 0; v < kMaxNF; ++v) ax[v] = ay[v] = 0.0;
-    for (int i = tid; i < len; i += kApplyBlk) {
-        const double rx = r[xoff + i], ry = r[yoff + i];
-        double s1, c1t;
-        sincospi(2.0 * static_cast<double>(i) / static_cast<double>(len), &s1, &c1t);
-        ax[0] = __builtin_fma(c0, rx, ax[0]);
-        ay[0] = __builtin_fma(c0, ry, ay[0]);
-        double ck = 1.0, sk = 0.0;
+        for (int v = 0; v < kChunkNF; ++v) ax[v] = ay[v] = 0.0;
+        for (int i = tid; i < len; i += kApplyBlk) {
+            const double rx = r[xoff + i], ry = r[yoff + i];
+            double s1, c1t;
+            sincospi(2.0 * static_cast<double>(i) / static_cast<double>(len), &s1, &c1t);
+            double ck = 1.0, sk = 0.0;
+            if (ch == 0) {
+                ax[0] = __builtin_fma(c0, rx, ax[0]);
+                ay[0] = __builtin_fma(c0, ry, ay[0]);
+            } else {
+                // start from e^{i kb theta}, the angle reduced mod 2 pi exactly
+                const long long ki = (static_cast<long long>(kb) * i) % len;
+                sincospi(2.0 * static_cast<double>(ki) / static_cast<double>(len), &sk, &ck);
+            }
 #pragma unroll
-        for (int k = 1; k <= kMaxModes; ++k) {
-            // modes above K stay zero (their slots are never read)
-            const double cn = __builtin_fma(ck, c1t, -sk * s1);
-            sk = __builtin_fma(sk, c1t, ck * s1);
-            ck = cn;
-            const double pc = c1 * ck, ps = c1 * sk;
-            ax[2 * k - 1] = __builtin_fma(pc, rx, ax[2 * k - 1]);
-            ax[2 * k] = __builtin_fma(ps, rx, ax[2 * k]);
-            ay[2 * k - 1] = __builtin_fma(pc, ry, ay[2 * k - 1]);
-            ay[2 * k] = __builtin_fma(ps, ry, ay[2 * k]);
-        }
-    }
-#pragma unroll
-    for (int j = 0; j * kVals < kMaxNC; ++j) {
-#pragma unroll
-        for (int v = 0; v < kVals; ++v) {
-            const int slot = j * kVals + v;
-            if (slot < kMaxNC) part[v][tid] = slot < kMaxNF ? ax[slot] : ay[slot - kMaxNF];
-        }
-        __syncthreads();
-        for (int job = tid; job < kVals * kSeg; job += kApplyBlk) {
-            const int v = job / kSeg, sg = job - v * kSeg;
-            double s = 0.0;
-            for (int k = sg; k < kApplyBlk; k += kSeg) s += part[v][k];
-            red[v][sg] = s;
-        }
-        __syncthreads();
-        if (tid < kVals) {
-            const int slot = j * kVals + tid;
-            const int comp = slot / kMaxNF, p = slot - comp * kMaxNF;
-            if (slot < kMaxNC && p < nf) {
-                double s = 0.0;
-#pragma unroll
-                for (int sg = 0; sg < kSeg; ++sg) s += red[tid][sg];
-                cc[comp * nf + p] = s;
+            for (int l = 1; l <= kChunkModes; ++l) {
+                // modes above K stay zero (their slots are never read)
+                const double cn = __builtin_fma(ck, c1t, -sk * s1);
+                sk = __builtin_fma(sk, c1t, ck * s1);
+                ck = cn;
+                const double pc = c1 * ck, ps = c1 * sk;
+                ax[2 * l - 1] = __builtin_fma(pc, rx, ax[2 * l - 1]);
+                ax[2 * l] = __builtin_fma(ps, rx, ax[2 * l]);
+                ay[2 * l - 1] = __builtin_fma(pc, ry, ay[2 * l - 1]);
+                ay[2 * l] = __builtin_fma(ps, ry, ay[2 * l]);
             }
         }
-        __syncthreads();
+#pragma unroll
+        for (int j = 0; j * kVals < kChunkNC; ++j) {
+#pragma unroll
+            for (int v = 0; v < kVals; ++v) {
+                const int slot = j * kVals + v;
+                if (slot < kChunkNC) part[v][tid] = slot < kChunkNF ? ax[slot] : ay[slot - kChunkNF];
+            }
+            __syncthreads();
+            for (int job = tid; job < kVals * kSeg; job += kApplyBlk) {
+                const int v = job / kSeg, sg = job - v * kSeg;
+                double s = 0.0;
+                for (int k = sg; k < kApplyBlk; k += kSeg) s += part[v][k];
+                red[v][sg] = s;
+            }
+            __syncthreads();
+            if (tid < kVals) {
+                const int slot = j * kVals + tid;
+                const int comp = slot / kChunkNF, pl = slot - comp * kChunkNF;
+                // local slot pl of chunk ch is coefficient 2 kb + pl of the component
+                const int pg = 2 * kb + pl;
+                if (slot < kChunkNC && (pl > 0 || ch == 0) && pg < nf) {
+                    double s = 0.0;
+#pragma unroll
+                    for (int sg = 0; sg < kSeg; ++sg) s += red[tid][sg];
+                    cc[comp * nf + pg] = s;
+                }
+            }
+            __syncthreads();
+        }
     }
     if (!wall && tid < 3) cc[nc + tid] = r[coff + tid];
     __syncthreads();
@@ -443,13 +461,29 @@ __global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
 
 }  // namespace
 
+// Coarse inverses of `blocks` curves starting at first_curve, each nq x nq.
+static void coarse_launch(Context& c, int blocks, int nq, int nfw, int nfb, int first_curve,
+                          double* out) {
+    const Geometry& g = c.g;
+    constexpr int kSmallNQ = 2 * (2 * 16 + 1) + 3;  // 16 modes
+    if (nq <= kSmallNQ)
+        k_pc_coarse<kSmallNQ><<<blocks, kCoarseBlk, 0, c.stream>>>(
+            g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p, g.cx.p, g.cy.p, c.pc_phi_w.p,
+            c.pc_phi_b.p, nfw, nfb, g.no, g.n, first_curve, c.pc_S.p, out);
+    else
+        k_pc_coarse<kMaxNQ><<<blocks, kCoarseBlk, 0, c.stream>>>(
+            g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p, g.cx.p, g.cy.p, c.pc_phi_w.p,
+            c.pc_phi_b.p, nfw, nfb, g.no, g.n, first_curve, c.pc_S.p, out);
+    STOKES_LAUNCH_CHECK();
+}
+
 void precond_setup(Context& c) {
     Geometry& g = c.g;
     hipStream_t s = c.stream;
     const int n = g.n, m = g.m, no = g.no;
     if (c.pc_body_modes < 1 || c.pc_body_modes > kMaxModes || c.pc_wall_modes < 1 ||
         c.pc_wall_modes > kMaxModes)
-        throw Error("preconditioner modes must lie in [1, 16]");
+        throw Error("preconditioner modes must lie in [1, 32]");
     // a curve of len points holds modes 0..(len-2)/2 without the Nyquist mode
     const int kb = std::min(c.pc_body_modes, (n - 2) / 2);
     const int kw = std::min(c.pc_wall_modes, (no - 2) / 2);
@@ -474,10 +508,7 @@ void precond_setup(Context& c) {
             g.x.p, g.y.p, g.mx.p, g.my.p, c.pc_phi_w.p, c.pc_phi_b.p, nfw, nfb, no, n, 0,
             c.pc_S.p);
         STOKES_LAUNCH_CHECK();
-        k_pc_coarse<<<1, kCoarseBlk, 0, s>>>(g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p,
-                                       g.cx.p, g.cy.p, c.pc_phi_w.p, c.pc_phi_b.p, nfw, nfb,
-                                       no, n, 0, c.pc_S.p, c.pc_Gw.p);
-        STOKES_LAUNCH_CHECK();
+        coarse_launch(c, 1, nqw, nfw, nfb, 0, c.pc_Gw.p);
         c.pc_wall_no = no;
         c.pc_wall_kw = kw;
     }
@@ -487,10 +518,7 @@ void precond_setup(Context& c) {
             g.x.p, g.y.p, g.mx.p, g.my.p, c.pc_phi_w.p, c.pc_phi_b.p, nfw, nfb, no, n, 1,
             c.pc_S.p);
         STOKES_LAUNCH_CHECK();
-        k_pc_coarse<<<m, kCoarseBlk, 0, s>>>(g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p,
-                                       g.cx.p, g.cy.p, c.pc_phi_w.p, c.pc_phi_b.p, nfw, nfb,
-                                       no, n, 1, c.pc_S.p, c.pc_Gb.p);
-        STOKES_LAUNCH_CHECK();
+        coarse_launch(c, m, nqb, nfw, nfb, 1, c.pc_Gb.p);
     }
     c.pc_nfb = nfb;
     c.pc_nfw = nfw;

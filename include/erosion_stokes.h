@@ -184,7 +184,7 @@ int erosion_measure_copy(int64_t bytes, int64_t reps, double* ms);
  * kind 0: none (default, the reference's iteration).  kind 1: per-curve
  * two-level preconditioner -- on each curve (wall, every body) the exact
  * inverse of the curve's own diagonal operator block on its Fourier modes
- * 0..K (K = wall_modes for the wall, body_modes for a body, each in [1, 16],
+ * 0..K (K = wall_modes for the wall, body_modes for a body, each in [1, 32],
  * lowered to (points - 2)/2 on a curve with fewer points; a body's block
  * includes its stokeslet/rotlet columns and constraint rows), and -2 (the
  * inverse of the jump -1/2) on the remaining modes.  GMRES then iterates on

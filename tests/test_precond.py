@@ -119,7 +119,7 @@ def pc_reset():
 
 @gpu
 @pytest.mark.parametrize("name", ["tight8_n64", "bodies10_n128"])
-@pytest.mark.parametrize("modes", [(8, 16), (3, 5)])
+@pytest.mark.parametrize("modes", [(8, 16), (3, 5), (16, 16), (20, 32)])
 def test_gpu_apply_matches_numpy(pc_reset, name, modes):
     stokes = pc_reset
     g, n, m, no, geo = _geo(name)
@@ -175,7 +175,7 @@ def test_gpu_preconditioner_mode_limits(pc_reset):
     with pytest.raises(stokes.StokesError):
         stokes.set_preconditioner(1, 0, 16)
     with pytest.raises(stokes.StokesError):
-        stokes.set_preconditioner(1, 8, 17)
+        stokes.set_preconditioner(1, 8, 33)
     with pytest.raises(stokes.StokesError):
         stokes.set_preconditioner(2)
     # 32 points per body hold modes 0..15: 16 body modes are lowered to 15
