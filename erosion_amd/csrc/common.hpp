@@ -255,7 +255,7 @@ struct Context {
     // FP32 far field: per 256-point tile the FP64 centre, per source the FP32
     // offset from it and the FP32 weighted normal; built once per geometry
     DevBuf<double> ff32_ctr;
-    DevBuf<float> ff32_xy, ff32_m;
+    DevBuf<float> ff32_xy;
     long long ff32_version = -1;
     // precision of the far field: 0 FP64; 1 mixed (GMRES switches the far
     // field to FP32 once its residual is small, see gmres.hip); 2 FP32 always

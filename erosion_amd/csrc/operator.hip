@@ -63,27 +63,32 @@ __device__ __forceinline__ double rcp64_1(double a) {
 }
 
 // Pair term (1/pi)(r.n)(r.eta) r w / rho^4 with w = speed 2 pi / N folded
-// into (mx, my), for TPT targets per lane against one LDS tile of sources.
-// CHECK zeroes the self pair (source index == target index).
+// into m = (mx, my).  The product (r.m)(r.eta) is expanded over the source's
+// three moments A = mx ex, B = mx ey + my ex, C = my ey:
+//   (r.m)(r.eta) = rx (A rx + B ry) + C ry^2,
+// which reuses ry^2 from rho^2 and saves one multiply per pair.  The source
+// tile holds (x, y), (A, B) and C.  TPT targets per lane; CHECK zeroes the
+// self pair (source index == target index).
 template <int TPT, bool CHECK>
 __device__ __forceinline__ void ff_tile(const double2* __restrict__ sxy,
-                                        const double2* __restrict__ sm,
-                                        const double2* __restrict__ se, const double* xt,
+                                        const double2* __restrict__ sab,
+                                        const double* __restrict__ sc, const double* xt,
                                         const double* yt, const int* self, double* u,
                                         double* v) {
 #pragma unroll 4
     for (int jj = 0; jj < kTile; ++jj) {
         const double2 p = sxy[jj];
-        const double2 mm = sm[jj];
-        const double2 e = se[jj];
+        const double2 ab = sab[jj];
+        const double cc = sc[jj];
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
             const double rx = xt[q] - p.x, ry = yt[q] - p.y;
-            const double rho2 = __builtin_fma(rx, rx, ry * ry);
-            const double a = __builtin_fma(rx, mm.x, ry * mm.y);
-            const double b = __builtin_fma(rx, e.x, ry * e.y);
+            const double ry2 = ry * ry;
+            const double rho2 = __builtin_fma(rx, rx, ry2);
+            const double t = __builtin_fma(ab.x, rx, ab.y * ry);
+            const double w = __builtin_fma(t, rx, cc * ry2);
             const double inv = rcp64_1(rho2);
-            double c = (a * b) * (inv * inv);
+            double c = w * (inv * inv);
             if (CHECK) c = (jj == self[q]) ? 0.0 : c;
             u[q] = __builtin_fma(c, rx, u[q]);
             v[q] = __builtin_fma(c, ry, v[q]);
@@ -91,31 +96,36 @@ __device__ __forceinline__ void ff_tile(const double2* __restrict__ sxy,
     }
 }
 
-// The same sum for TPT = 2 or 4 targets with one reciprocal per source: the
-// product P of the targets' rho^2, q = 1/P (estimate + one Newton step), and
-// 1/rho_k^2 = q times the other factors of P.  Three of four hardware
-// reciprocals become three multiplies.  Off-diagonal tiles only (a self pair
-// has rho^2 = 0); padding sources sit at 1e30, so P stays finite.
-template <int TPT>
+// The same sum for TPT = 2, 4 or 8 targets with one reciprocal per source and
+// group of targets: the product P of the group's rho^2, q = 1/P (estimate +
+// one Newton step), and 1/rho_k^2 = q times the other factors of P.  Three of
+// four hardware reciprocals become three multiplies.  Off-diagonal tiles only
+// unless DIAG; padding sources sit at 1e30, so P stays finite.  DIAG: the
+// tile holds some of the lane's own targets.  A pair at distance zero (the
+// self pair) has rx = ry = 0, so its moment product w is exactly 0; its rho^2
+// is raised to 1e-100 (every other pair has rho^2 >= 1e-20 and is unchanged),
+// which keeps P, 1/P and 1/rho^4 <= 1e200 finite, and the pair adds 0.
+template <int TPT, bool DIAG>
 __device__ __forceinline__ void ff_tile_shared(const double2* __restrict__ sxy,
-                                               const double2* __restrict__ sm,
-                                               const double2* __restrict__ se, const double* xt,
+                                               const double2* __restrict__ sab,
+                                               const double* __restrict__ sc, const double* xt,
                                                const double* yt, double* u, double* v) {
     static_assert(TPT == 2 || TPT == 4 || TPT == 8, "shared reciprocal over 2, 4 or 8 targets");
 #pragma unroll 4
     for (int jj = 0; jj < kTile; ++jj) {
         const double2 p = sxy[jj];
-        const double2 mm = sm[jj];
-        const double2 e = se[jj];
-        double rx[TPT], ry[TPT], r2[TPT], ab[TPT], inv[TPT];
+        const double2 ab = sab[jj];
+        const double cc = sc[jj];
+        double rx[TPT], ry[TPT], r2[TPT], w[TPT], inv[TPT];
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
             rx[q] = xt[q] - p.x;
             ry[q] = yt[q] - p.y;
-            r2[q] = __builtin_fma(rx[q], rx[q], ry[q] * ry[q]);
-            const double a = __builtin_fma(rx[q], mm.x, ry[q] * mm.y);
-            const double b = __builtin_fma(rx[q], e.x, ry[q] * e.y);
-            ab[q] = a * b;
+            const double ry2 = ry[q] * ry[q];
+            r2[q] = __builtin_fma(rx[q], rx[q], ry2);
+            if (DIAG) r2[q] = __builtin_fmax(r2[q], 1.0e-100);
+            const double t = __builtin_fma(ab.x, rx[q], ab.y * ry[q]);
+            w[q] = __builtin_fma(t, rx[q], cc * ry2);
         }
         if (TPT >= 4) {
             // groups of four targets (a product of eight could overflow)
@@ -138,11 +148,18 @@ __device__ __forceinline__ void ff_tile_shared(const double2* __restrict__ sxy,
         }
 #pragma unroll
         for (int q = 0; q < TPT; ++q) {
-            const double c = ab[q] * (inv[q] * inv[q]);
+            const double c = w[q] * (inv[q] * inv[q]);
             u[q] = __builtin_fma(c, rx[q], u[q]);
             v[q] = __builtin_fma(c, ry[q], v[q]);
         }
     }
+}
+
+// Source moments of one point for the expanded pair term.
+__device__ __forceinline__ void src_moments(double mx, double my, double ex, double ey,
+                                            double2* ab, double* c) {
+    *ab = make_double2(mx * ex, __builtin_fma(mx, ey, my * ex));
+    *c = my * ey;
 }
 
 // grid: (ceil(npts / (256 TPT)), source splits); block 256 threads.  Lane l
@@ -154,7 +171,8 @@ __global__ __launch_bounds__(kBlk) void k_farfield(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ mx,
     const double* __restrict__ my, const double* __restrict__ ex, const double* __restrict__ ey,
     int npts, int split_len, double* __restrict__ part_u, double* __restrict__ part_v) {
-    __shared__ double2 sxy[kTile], sm[kTile], se[kTile];
+    __shared__ double2 sxy[kTile], sab[kTile];
+    __shared__ double sc[kTile];
     const int tid = threadIdx.x;
     const int tb0 = blockIdx.x * TPT * kBlk;
     const int tb1 = min(tb0 + TPT * kBlk, npts);
@@ -175,24 +193,25 @@ __global__ __launch_bounds__(kBlk) void k_farfield(
         __syncthreads();
         if (j < s1) {
             sxy[tid] = make_double2(x[j], y[j]);
-            sm[tid] = make_double2(mx[j], my[j]);
-            se[tid] = make_double2(ex[j], ey[j]);
+            src_moments(mx[j], my[j], ex[j], ey[j], &sab[tid], &sc[tid]);
         } else {  // padding source: zero weight far away
             sxy[tid] = make_double2(1.0e30, 1.0e30);
-            sm[tid] = make_double2(0.0, 0.0);
-            se[tid] = make_double2(0.0, 0.0);
+            sab[tid] = make_double2(0.0, 0.0);
+            sc[tid] = 0.0;
         }
         __syncthreads();
         const bool diag = (base < tb1) && (base + kTile > tb0);
-        if (diag) {
+        if (diag && SHR) {
+            if constexpr (TPT > 1) ff_tile_shared<TPT, true>(sxy, sab, sc, xt, yt, u, v);
+        } else if (diag) {
             int self[TPT];
 #pragma unroll
             for (int q = 0; q < TPT; ++q) self[q] = t[q] - base;
-            ff_tile<TPT, true>(sxy, sm, se, xt, yt, self, u, v);
+            ff_tile<TPT, true>(sxy, sab, sc, xt, yt, self, u, v);
         } else if (SHR) {
-            if constexpr (TPT > 1) ff_tile_shared<TPT>(sxy, sm, se, xt, yt, u, v);
+            if constexpr (TPT > 1) ff_tile_shared<TPT, false>(sxy, sab, sc, xt, yt, u, v);
         } else {
-            ff_tile<TPT, false>(sxy, sm, se, xt, yt, nullptr, u, v);
+            ff_tile<TPT, false>(sxy, sab, sc, xt, yt, nullptr, u, v);
         }
     }
 #pragma unroll
@@ -216,8 +235,8 @@ constexpr int kSub32 = 64;
 
 template <int TPT, bool CHECK>
 __device__ __forceinline__ void ff32_tile(const float2* __restrict__ sxy,
-                                          const float2* __restrict__ sm,
-                                          const float2* __restrict__ se, const float* dx,
+                                          const float2* __restrict__ sab,
+                                          const float* __restrict__ sc, const float* dx,
                                           const float* dy, const int* self, double* u, double* v) {
     for (int sub = 0; sub < kTile; sub += kSub32) {
         float uf[TPT], vf[TPT];
@@ -226,16 +245,17 @@ __device__ __forceinline__ void ff32_tile(const float2* __restrict__ sxy,
 #pragma unroll 4
         for (int jj = sub; jj < sub + kSub32; ++jj) {
             const float2 p = sxy[jj];
-            const float2 mm = sm[jj];
-            const float2 e = se[jj];
+            const float2 ab = sab[jj];
+            const float cc = sc[jj];
 #pragma unroll
             for (int q = 0; q < TPT; ++q) {
                 const float rx = dx[q] - p.x, ry = dy[q] - p.y;
-                const float rho2 = __builtin_fmaf(rx, rx, ry * ry);
-                const float a = __builtin_fmaf(rx, mm.x, ry * mm.y);
-                const float b = __builtin_fmaf(rx, e.x, ry * e.y);
+                const float ry2 = ry * ry;
+                const float rho2 = __builtin_fmaf(rx, rx, ry2);
+                const float t = __builtin_fmaf(ab.x, rx, ab.y * ry);
+                const float w = __builtin_fmaf(t, rx, cc * ry2);
                 const float inv = __builtin_amdgcn_rcpf(rho2);
-                float cf = (a * b) * (inv * inv);
+                float cf = w * (inv * inv);
                 if (CHECK) cf = (jj == self[q]) ? 0.0f : cf;
                 uf[q] = __builtin_fmaf(cf, rx, uf[q]);
                 vf[q] = __builtin_fmaf(cf, ry, vf[q]);
@@ -249,15 +269,12 @@ __device__ __forceinline__ void ff32_tile(const float2* __restrict__ sxy,
     }
 }
 
-// Tile centres (point means) and FP32 source offsets and weights; padding
-// sources past npts sit far away with zero weight.  One block per tile.
+// Tile centres (point means) and FP32 source offsets; padding sources past
+// npts sit far away (their moments are zero).  One block per tile.
 __global__ __launch_bounds__(kBlk) void k_ff32_prep(const double* __restrict__ x,
-                                                    const double* __restrict__ y,
-                                                    const double* __restrict__ mx,
-                                                    const double* __restrict__ my, int npts,
+                                                    const double* __restrict__ y, int npts,
                                                     double2* __restrict__ tctr,
-                                                    float2* __restrict__ sxy,
-                                                    float2* __restrict__ sm) {
+                                                    float2* __restrict__ sxy) {
     __shared__ double shx[kBlk], shy[kBlk];
     const int j = blockIdx.x * kTile + threadIdx.x;
     const bool valid = j < npts;
@@ -274,24 +291,21 @@ __global__ __launch_bounds__(kBlk) void k_ff32_prep(const double* __restrict__ x
     const int cnt = min(kTile, npts - static_cast<int>(blockIdx.x) * kTile);
     const double cx = shx[0] / cnt, cy = shy[0] / cnt;
     if (threadIdx.x == 0) tctr[blockIdx.x] = make_double2(cx, cy);
-    if (valid) {
-        sxy[j] = make_float2(static_cast<float>(x[j] - cx), static_cast<float>(y[j] - cy));
-        sm[j] = make_float2(static_cast<float>(mx[j]), static_cast<float>(my[j]));
-    } else {
-        sxy[j] = make_float2(1.0e3f, 1.0e3f);
-        sm[j] = make_float2(0.0f, 0.0f);
-    }
+    sxy[j] = valid ? make_float2(static_cast<float>(x[j] - cx), static_cast<float>(y[j] - cy))
+                   : make_float2(1.0e3f, 1.0e3f);
 }
 
 template <int TPT>
 __global__ __launch_bounds__(kBlk) void k_farfield32(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ mx,
     const double* __restrict__ my, const float2* __restrict__ s32xy,
-    const float2* __restrict__ s32m, const double2* __restrict__ tctr,
-    const double* __restrict__ ex, const double* __restrict__ ey, int npts, int split_len,
-    double* __restrict__ part_u, double* __restrict__ part_v) {
-    __shared__ float2 sxy[kTile], sm[kTile], se[kTile];
-    __shared__ double2 dxy[kTile], dm[kTile], de[kTile];
+    const double2* __restrict__ tctr, const double* __restrict__ ex,
+    const double* __restrict__ ey, int npts, int split_len, double* __restrict__ part_u,
+    double* __restrict__ part_v) {
+    __shared__ float2 sxy[kTile], sab[kTile];
+    __shared__ float sc[kTile];
+    __shared__ double2 dxy[kTile], dab[kTile];
+    __shared__ double dc[kTile];
     const int tid = threadIdx.x;
     const int tb0 = blockIdx.x * TPT * kBlk;
     const int tb1 = min(tb0 + TPT * kBlk, npts);
@@ -316,24 +330,30 @@ __global__ __launch_bounds__(kBlk) void k_farfield32(
         if (diag) {
             if (j < s1) {
                 dxy[tid] = make_double2(x[j], y[j]);
-                dm[tid] = make_double2(mx[j], my[j]);
-                de[tid] = make_double2(ex[j], ey[j]);
+                src_moments(mx[j], my[j], ex[j], ey[j], &dab[tid], &dc[tid]);
             } else {
-                dxy[tid] = make_double2(1.0e100, 1.0e100);
-                dm[tid] = make_double2(0.0, 0.0);
-                de[tid] = make_double2(0.0, 0.0);
+                dxy[tid] = make_double2(1.0e30, 1.0e30);
+                dab[tid] = make_double2(0.0, 0.0);
+                dc[tid] = 0.0;
             }
             __syncthreads();
             int self[TPT];
 #pragma unroll
             for (int q = 0; q < TPT; ++q) self[q] = t[q] - base;
-            ff_tile<TPT, true>(dxy, dm, de, xt, yt, self, u, v);
+            ff_tile<TPT, true>(dxy, dab, dc, xt, yt, self, u, v);
             continue;
         }
         sxy[tid] = s32xy[j];
-        sm[tid] = s32m[j];
-        se[tid] = j < s1 ? make_float2(static_cast<float>(ex[j]), static_cast<float>(ey[j]))
-                         : make_float2(0.0f, 0.0f);
+        if (j < s1) {  // moments formed in FP64, then rounded
+            double2 m2;
+            double m1;
+            src_moments(mx[j], my[j], ex[j], ey[j], &m2, &m1);
+            sab[tid] = make_float2(static_cast<float>(m2.x), static_cast<float>(m2.y));
+            sc[tid] = static_cast<float>(m1);
+        } else {
+            sab[tid] = make_float2(0.0f, 0.0f);
+            sc[tid] = 0.0f;
+        }
         __syncthreads();
         const double2 ctr = tctr[base / kTile];
         float dx[TPT], dy[TPT];
@@ -342,7 +362,7 @@ __global__ __launch_bounds__(kBlk) void k_farfield32(
             dx[q] = static_cast<float>(xt[q] - ctr.x);
             dy[q] = static_cast<float>(yt[q] - ctr.y);
         }
-        ff32_tile<TPT, false>(sxy, sm, se, dx, dy, nullptr, u, v);
+        ff32_tile<TPT, false>(sxy, sab, sc, dx, dy, nullptr, u, v);
     }
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
@@ -376,7 +396,7 @@ FFPlan ff_plan(int np, FFShape sh, bool fp32) {
     FFPlan p{};
     p.tpt = fp32 ? std::min(sh.tpt, 4) : sh.tpt;
     p.shared = !fp32 && sh.shared && sh.tpt > 1;
-    p.tblocks = cdiv_int(np, kBlk * sh.tpt);
+    p.tblocks = cdiv_int(np, kBlk * p.tpt);
     const int want = std::max(1, cdiv_int(sh.blocks, p.tblocks));
     p.split_len = cdiv_int(cdiv_int(np, want), kTile) * kTile;
     p.nsplit = cdiv_int(np, p.split_len);
@@ -387,21 +407,20 @@ void ff32_launch(const Context& c, const FFPlan& p, double* pu, double* pv) {
     const Geometry& g = c.g;
     const dim3 grid(p.tblocks, p.nsplit);
     const float2* sxy = reinterpret_cast<const float2*>(c.ff32_xy.p);
-    const float2* sm = reinterpret_cast<const float2*>(c.ff32_m.p);
     const double2* tc = reinterpret_cast<const double2*>(c.ff32_ctr.p);
     switch (p.tpt) {
     case 1:
-        k_farfield32<1><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, sxy, sm, tc,
+        k_farfield32<1><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, sxy, tc,
                                                      c.ex.p, c.ey.p,
                                                      g.npts, p.split_len, pu, pv);
         break;
     case 2:
-        k_farfield32<2><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, sxy, sm, tc,
+        k_farfield32<2><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, sxy, tc,
                                                      c.ex.p, c.ey.p,
                                                      g.npts, p.split_len, pu, pv);
         break;
     default:
-        k_farfield32<4><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, sxy, sm, tc,
+        k_farfield32<4><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, sxy, tc,
                                                      c.ex.p, c.ey.p,
                                                      g.npts, p.split_len, pu, pv);
         break;
@@ -962,11 +981,9 @@ static void ff32_sources(Context& c) {
     const int ntiles = cdiv_int(np, kTile);
     c.ff32_ctr.ensure(2 * static_cast<size_t>(ntiles));
     c.ff32_xy.ensure(2 * static_cast<size_t>(ntiles) * kTile);
-    c.ff32_m.ensure(2 * static_cast<size_t>(ntiles) * kTile);
-    k_ff32_prep<<<ntiles, kBlk, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, np,
+    k_ff32_prep<<<ntiles, kBlk, 0, c.stream>>>(g.x.p, g.y.p, np,
                                                reinterpret_cast<double2*>(c.ff32_ctr.p),
-                                               reinterpret_cast<float2*>(c.ff32_xy.p),
-                                               reinterpret_cast<float2*>(c.ff32_m.p));
+                                               reinterpret_cast<float2*>(c.ff32_xy.p));
     STOKES_LAUNCH_CHECK();
     c.ff32_version = g.version;
 }

@@ -47,6 +47,28 @@ def test_fp32_farfield_operator_accuracy(st, case):
     assert err < 1e-5 and rms < 1e-6
 
 
+def test_fp32_farfield_every_shape(st):
+    """Each far-field launch shape in FP32 (shapes asking for 8 targets per
+    lane run the FP32 kernel's 4 and must still cover every target), on 20
+    bodies x 256 points."""
+    from erosion_amd.geometry import WORKLOADS, workload_xy
+    n, m, no, xx, yy = workload_xy(WORKLOADS["bodies100_n256"], nbodies=20)
+    N = 2 * no + 2 * n * m + 3 * m
+    eta = np.random.default_rng(12).standard_normal(N)
+    st.set_precision(0)
+    v64 = st.matvec(n, m, no, xx, yy, eta)
+    st.set_precision(2)
+    try:
+        for k in range(st.farfield_shape_count()):
+            st.set_farfield_shape(k)
+            v32 = st.matvec(n, m, no, xx, yy, eta)
+            err = relerr(v32, v64)
+            rms = np.sqrt(np.mean((v32 - v64) ** 2) / np.mean(v64 ** 2))
+            assert err < 1e-6 and rms < 1e-7, (k, err, rms)
+    finally:
+        st.set_farfield_shape(-1)
+
+
 @pytest.mark.parametrize("pc", [0, 1])
 def test_mixed_solve_bodies10(st, pc):
     g = load_golden("bodies10_n128")
