@@ -778,9 +778,10 @@ __global__ __launch_bounds__(kBlk) void k_stokes_logs(const double* __restrict__
     lg[e] = log(rx * rx + ry * ry);
 }
 
-// kAsmSub lanes per target: the stokeslet/rotlet sum over the bodies is
-// split across neighbouring lanes and reduced with cross-lane moves, so a
-// launch has kAsmSub npts / 64 waves to hide the latency of the logarithms.
+// kAsmSub lanes per target: the stokeslet/rotlet sum over the bodies and the
+// sum of the far field's per-split partials are split across neighbouring
+// lanes and reduced with cross-lane moves, so a launch has kAsmSub npts / 64
+// waves to hide the latency of the logarithms and of the partial-sum loads.
 constexpr int kAsmSub = 16;
 __global__ __launch_bounds__(kBlk) void k_assemble(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ px,
@@ -824,6 +825,13 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
             vs += 0.5 / kTwoPi * (-0.5 * lg * sto2 + rdots * irho2 * ry) - rot * rx * irho2;
         }
     }
+    // far-field partial sums of the source splits s = sub mod kAsmSub
+    if (valid) {
+        for (int s = sub; s < nsplit; s += kAsmSub) {
+            us += part_u[static_cast<size_t>(s) * npts + i];
+            vs += part_v[static_cast<size_t>(s) * npts + i];
+        }
+    }
 #pragma unroll
     for (int off = 1; off < kAsmSub; off <<= 1) {
         us += __shfl_xor(us, off);
@@ -831,10 +839,6 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
     }
     if (!valid || sub != 0) return;
     double u = 0.0, v = 0.0;
-    for (int s = 0; s < nsplit; ++s) {
-        u += part_u[static_cast<size_t>(s) * npts + i];
-        v += part_v[static_cast<size_t>(s) * npts + i];
-    }
     // near-pair replacement
     if (has_near) {
         for (int e = tgt_ptr[i]; e < tgt_ptr[i + 1]; ++e) {
