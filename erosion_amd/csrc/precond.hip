@@ -41,6 +41,7 @@ constexpr int kMaxNF = 2 * kMaxModes + 1;      // functions per component
 constexpr int kMaxNQ = 2 * kMaxNF + 3;         // coarse unknowns per curve
 
 constexpr int kCoarseBlk = 1024;
+static_assert(kMaxNQ < kApplyBlk, "the coarse apply needs a lane per coarse unknown");
 
 inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
 
@@ -178,9 +179,14 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
     const double* __restrict__ cy, const double* __restrict__ phi_w,
     const double* __restrict__ phi_b, int nfw, int nfb, int no, int n, int first_curve,
     const double* __restrict__ S, double* __restrict__ out) {
+    constexpr int MAXNF = (MAXNQ - 3) / 2;
+    constexpr int kCh = 32;  // points per LDS chunk
+    constexpr int kEPT = (3 * MAXNF * MAXNF + kCoarseBlk - 1) / kCoarseBlk;
+    static_assert(4 * kCh * MAXNF <= MAXNQ * MAXNQ, "chunk buffers fit in G's memory");
     __shared__ double G[MAXNQ * MAXNQ];
     __shared__ double fcol[MAXNQ];
     __shared__ int perm[MAXNQ];
+    __shared__ double uv[4 * MAXNF];
     const int c = first_curve + static_cast<int>(blockIdx.x);
     const bool wall = c == 0;
     const CurveView cv = curve_view(c, no, n, nfw, nfb, phi_w, phi_b);
@@ -195,42 +201,100 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     constexpr int kCoarseWaves = kCoarseBlk / 64;
 
-    // one wave per entry, lanes over the curve's points
-    for (int e = wv; e < nq * nq; e += kCoarseWaves) {
-        const int r = e / nq, col = e - r * nq;
-        double s = 0.0;
-        if (r < nc && col < nc) {
-            const int a = r / nf, p = r - a * nf, b = col / nf, q = col - b * nf;
-            const double* Srow = Sc + static_cast<size_t>((a + b) * nf + q) * len;
+    // Wall rank-one completion n_i sum_j (n_j . eta_j) speed_j 2 pi / no in
+    // the coarse space: U_a[p] = sum_i phi_p(i) n_a(i) and
+    // V_b[q] = sum_j speed_j (2 pi / no) n_b(j) phi_q(j), uv = [U_x U_y V_x V_y].
+    if (wall) {
+        for (int d = wv; d < 4 * nf; d += kCoarseWaves) {
+            const bool second = d >= 2 * nf;
+            const int rem = second ? d - 2 * nf : d;
+            const int a = rem / nf, p = rem - a * nf;
             const double* pp = phi + static_cast<size_t>(p) * len;
-            const double* pq = phi + static_cast<size_t>(q) * len;
+            double sum = 0.0;
             for (int i = lane; i < len; i += 64) {
-                const int gi = cv.beg + i;
+                const double na = a == 0 ? px[i] : py[i];
+                sum = second ? __builtin_fma(speed[i] * kTwoPi / dlen * na, pp[i], sum)
+                             : __builtin_fma(pp[i], na, sum);
+            }
+            sum = wave_sum(sum);
+            if (lane == 0) uv[d] = sum;
+        }
+    }
+
+    // Coupling of the curve's own densities, for a, b in {x, y}:
+    //   G_ab[p][q] = sum_i phi_p(i) (S_{a+b}[q][i] + L_ab(i) phi_q(i)),
+    // L_xx = -f ny^2 - 1/2, L_xy = L_yx = f nx ny, L_yy = -f nx^2 - 1/2 with
+    // f = curvature x speed / len.  Both depend on a + b only: three nf x nf
+    // products over the curve's points, staged kCh points at a time in LDS
+    // (in G's memory, unused until they are done) with each thread's entries
+    // in registers.
+    double* Pc = G;            // [kCh][nf]: phi_q at the chunk's points
+    double* Mc = G + kCh * nf;  // [3][kCh][nf]: S_k + L_k phi_q
+    const int nn = nf * nf;
+    double acc[kEPT];
+    int poff[kEPT], moff[kEPT];  // entry u: Pc column p, Mc offset k kCh nf + q
+#pragma unroll
+    for (int u = 0; u < kEPT; ++u) {
+        acc[u] = 0.0;
+        const int e = min(tid + u * kCoarseBlk, 3 * nn - 1);
+        const int k = e / nn, rem = e - k * nn, p = rem / nf, q = rem - p * nf;
+        poff[u] = p;
+        moff[u] = k * kCh * nf + q;
+    }
+    for (int i0 = 0; i0 < len; i0 += kCh) {
+        __syncthreads();
+        for (int t = tid; t < kCh * nf; t += kCoarseBlk) {
+            const int q = t / kCh, i = t - q * kCh;
+            const int pt = i0 + i;
+            double ph = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
+            if (pt < len) {
+                const int gi = cv.beg + pt;
                 const double f = cur[gi] * speed[gi] / dlen;
                 const double nx = px[gi], ny = py[gi];
-                double L;
-                if (a == 0 && b == 0)
-                    L = -f * ny * ny - 0.5;
-                else if (a == 1 && b == 1)
-                    L = -f * nx * nx - 0.5;
-                else
-                    L = f * nx * ny;
-                s = __builtin_fma(pp[i], __builtin_fma(L, pq[i], Srow[i]), s);
+                ph = phi[static_cast<size_t>(q) * len + pt];
+                m0 = __builtin_fma(-f * ny * ny - 0.5, ph, Sc[static_cast<size_t>(q) * len + pt]);
+                m1 = __builtin_fma(f * nx * ny, ph, Sc[static_cast<size_t>(nf + q) * len + pt]);
+                m2 = __builtin_fma(-f * nx * nx - 0.5, ph,
+                                   Sc[static_cast<size_t>(2 * nf + q) * len + pt]);
             }
-            if (wall) {
-                // rank-one completion n_i sum_j (n_j . eta_j) speed_j 2 pi / no
-                double s1 = 0.0, s2 = 0.0;
-                for (int i = lane; i < len; i += 64) {
-                    const double na = a == 0 ? px[i] : py[i];
-                    const double nb = b == 0 ? px[i] : py[i];
-                    s1 = __builtin_fma(pp[i], na, s1);
-                    s2 = __builtin_fma(speed[i] * kTwoPi / dlen * nb, pq[i], s2);
-                }
-                s = __builtin_fma(wave_sum(s1), wave_sum(s2), wave_sum(s));
-            } else {
-                s = wave_sum(s);
+            Pc[i * nf + q] = ph;
+            Mc[(0 * kCh + i) * nf + q] = m0;
+            Mc[(1 * kCh + i) * nf + q] = m1;
+            Mc[(2 * kCh + i) * nf + q] = m2;
+        }
+        __syncthreads();
+        // entries past 3 nn repeat the last one; they are never stored
+        const int cnt = min(kCh, len - i0);
+        for (int i = 0; i < cnt; ++i) {
+            const double* pi = Pc + i * nf;
+            const double* mi = Mc + i * nf;
+#pragma unroll
+            for (int u = 0; u < kEPT; ++u) acc[u] = __builtin_fma(pi[poff[u]], mi[moff[u]], acc[u]);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kEPT; ++u) {
+        const int e = tid + u * kCoarseBlk;
+        if (e < 3 * nn) {
+            const int k = e / nn, rem = e - k * nn, p = rem / nf, q = rem - p * nf;
+            for (int a = 0; a < 2; ++a) {
+                const int b = k - a;
+                if (b < 0 || b > 1) continue;
+                double g = acc[u];
+                if (wall) g = __builtin_fma(uv[a * nf + p], uv[2 * nf + b * nf + q], g);
+                G[(a * nf + p) * nq + b * nf + q] = g;
             }
-        } else if (r < nc) {
+        }
+    }
+
+    // the body's stokeslet/rotlet columns, constraint rows and identity
+    // block: one wave per entry, lanes over the curve's points
+    for (int e = wv; e < nq * nq; e += kCoarseWaves) {
+        const int r = e / nq, col = e - r * nq;
+        if (r < nc && col < nc) continue;
+        double s = 0.0;
+        if (r < nc) {
             // stokeslet / rotlet column of the body's own centre
             const int a = r / nf, p = r - a * nf, cc = col - nc;
             const double* pp = phi + static_cast<size_t>(p) * len;
@@ -431,9 +495,26 @@ __global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
     }
     if (!wall && tid < 3) cc[nc + tid] = r[coff + tid];
     __syncthreads();
+    // d = G^{-1} c: the k range is split over kApplyBlk / nq groups of nq
+    // lanes so that each lane waits on a few L2 loads, not nq in sequence
+    {
+        const int groups = kApplyBlk / nq;  // nq <= kMaxNQ < kApplyBlk
+        const int span = (nq + groups - 1) / groups;
+        const int j = tid % nq, g = tid / nq;
+        if (g < groups) {
+            const int k0 = g * span, k1 = min(nq, k0 + span);
+            double d = 0.0;
+#pragma unroll 8
+            for (int k = k0; k < k1; ++k)
+                d = __builtin_fma(Gt[static_cast<size_t>(k) * nq + j], cc[k], d);
+            part[0][g * nq + j] = d;
+        }
+    }
+    __syncthreads();
     if (tid < nq) {
+        const int groups = kApplyBlk / nq;
         double d = 0.0;
-        for (int k = 0; k < nq; ++k) d = __builtin_fma(Gt[static_cast<size_t>(k) * nq + tid], cc[k], d);
+        for (int g = 0; g < groups; ++g) d += part[0][g * nq + tid];
         if (tid < nc)
             dd[tid] = d + 2.0 * cc[tid];
         else

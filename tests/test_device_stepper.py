@@ -99,7 +99,10 @@ def test_rk2_step_deletes_a_vanishing_body_like_host():
 
 
 def test_device_warm_start_lowers_iterations_same_states():
-    params = _params()
+    """Steps of 1e-5: consecutive densities differ little, so the guess
+    extrapolated from the last two is close and must save iterations (at
+    dt = 1e-3 the saving on 10 bodies is within a few iterations of zero)."""
+    params = _params(dt=1e-5)
     th0 = new_thlenden(make_bodies(10, 128, 0.05, 0.1, 0.05, 1001))
     t_warm, t_cold = {}, {}
     a = copy.deepcopy(th0)
@@ -109,6 +112,7 @@ def test_device_warm_start_lowers_iterations_same_states():
     b = copy.deepcopy(th0)
     for _ in range(3):
         b = rungekutta2_device(b, params, warm=0, timings=t_cold)
+    print(f"iters warm {t_warm['gmres_iters']} cold {t_cold['gmres_iters']}")
     assert t_warm["last_info"]["warm_starts"] == 2
     assert t_warm["gmres_iters"] < t_cold["gmres_iters"]
     for x, y in zip(a.thlenvec, b.thlenvec):
