@@ -283,7 +283,7 @@ struct Context {
     int pc_kind = 0, pc_body_modes = 16, pc_wall_modes = 16;
     int pc_nfb = 0, pc_nfw = 0;
     int pc_wall_no = -1, pc_wall_kw = -1;  // key of the kept wall block
-    DevBuf<double> pc_phi_b, pc_phi_w, pc_S, pc_Gb, pc_Gw, pc_z;
+    DevBuf<double> pc_phi_b, pc_phi_w, pc_S, pc_Gb, pc_Gw, pc_z, pc_wpart;
     // per-kernel timing with HIP events on `stream` (measurement hooks only)
     bool time_kernels = false;
     // replay GMRES iteration chunks as HIP graphs (off while timing kernels)

@@ -400,36 +400,39 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
     }
 }
 
-// z = P^{-1} r, one block per curve (wall = block 0, body b = block b + 1).
-// Each lane walks its points once per chunk of 16 modes, generating the
-// basis functions by the rotation e^{i(k+1)theta} = e^{ik theta} e^{i theta}
-// (no table loads), and keeps the chunk's coefficient partials in registers;
-// the partials meet in LDS and are summed in two conflict-free stages, kVals
-// at a time.  The inverse coarse matrices are stored transposed, so
-// d = G^{-1} c reads them coalesced with one row per lane.
-__global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
-    int nfw, int nfb, int no, int n, int m, const double* __restrict__ Gw,
-    const double* __restrict__ Gb, const double* __restrict__ r, double* __restrict__ z) {
-    constexpr int kVals = 24;
-    constexpr int kSeg = 16;
+// z = P^{-1} r.  A body is one block; the wall, whose points are many, is
+// spread over nseg = ceil(no / kApplyBlk) blocks of kApplyBlk points each:
+// k_pc_apply projects the segments (blocks 0..nseg-1, partial coefficients
+// to wpart) and the bodies (blocks nseg..nseg+m-1, the whole apply), and
+// k_pc_wall sums the wall's partial coefficients, solves its coarse system
+// and expands each segment.
+//
+// Projection: each lane walks its points once per chunk of 16 modes,
+// generating the basis functions by the rotation
+// e^{i(k+1)theta} = e^{ik theta} e^{i theta} (no table loads), and keeps the
+// chunk's coefficient partials in registers; the partials meet in LDS and are
+// summed in two conflict-free stages, kVals at a time.  The inverse coarse
+// matrices are stored transposed, so d = G^{-1} c reads them coalesced with
+// one row per lane.
+constexpr int kVals = 24;
+constexpr int kSeg = 16;
+
+struct ApplyLds {
+    double part[kVals][kApplyBlk];
+    double red[kVals][kSeg];
+    double cc[kMaxNQ], dd[kMaxNQ];
+};
+
+// cc[comp nf + f] = sum over points [i0, i1) of the curve of phi_f(i) r_comp(i)
+// (block-wide; every thread calls it).
+__device__ __forceinline__ void pc_project(ApplyLds& L, int len, int nf, int i0, int i1, const double* r,
+                           int xoff, int yoff, double* cc) {
     constexpr int kChunkNC = 2 * kChunkNF;
-    __shared__ double part[kVals][kApplyBlk];
-    __shared__ double red[kVals][kSeg];
-    __shared__ double cc[kMaxNQ], dd[kMaxNQ];
-    const int c = blockIdx.x;
-    const bool wall = c == 0;
-    const int len = wall ? no : n;
-    const int nf = wall ? nfw : nfb, nc = 2 * nf, K = (nf - 1) / 2;
-    const int nq = wall ? nc : nc + 3;
-    const int xoff = wall ? 0 : 2 * no + 2 * (c - 1) * n;
-    const int yoff = wall ? no : xoff + n;
-    const int coff = 2 * no + 2 * n * m + 3 * (c - 1);
-    const double* Gt = wall ? Gw : Gb + static_cast<size_t>(c - 1) * nq * nq;
     const int tid = threadIdx.x;
+    const int K = (nf - 1) / 2;
     const double c0 = 1.0 / sqrt(static_cast<double>(len));
     const double c1 = sqrt(2.0 / static_cast<double>(len));
     const int nchunks = K > 0 ? (K + kChunkModes - 1) / kChunkModes : 1;
-
     for (int ch = 0; ch < nchunks; ++ch) {
         // chunk ch holds modes kb+1 .. kb+16 in slots 2l-1 (cos), 2l (sin),
         // l = 1..16, and (chunk 0 only) the mean in slot 0; x slots first
@@ -437,7 +440,7 @@ __global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
         double ax[kChunkNF], ay[kChunkNF];
 #pragma unroll
         for (int v = 0; v < kChunkNF; ++v) ax[v] = ay[v] = 0.0;
-        for (int i = tid; i < len; i += kApplyBlk) {
+        for (int i = i0 + tid; i < i1; i += kApplyBlk) {
             const double rx = r[xoff + i], ry = r[yoff + i];
             double s1, c1t;
             sincospi(2.0 * static_cast<double>(i) / static_cast<double>(len), &s1, &c1t);
@@ -468,14 +471,14 @@ __global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
 #pragma unroll
             for (int v = 0; v < kVals; ++v) {
                 const int slot = j * kVals + v;
-                if (slot < kChunkNC) part[v][tid] = slot < kChunkNF ? ax[slot] : ay[slot - kChunkNF];
+                if (slot < kChunkNC) L.part[v][tid] = slot < kChunkNF ? ax[slot] : ay[slot - kChunkNF];
             }
             __syncthreads();
             for (int job = tid; job < kVals * kSeg; job += kApplyBlk) {
                 const int v = job / kSeg, sg = job - v * kSeg;
                 double s = 0.0;
-                for (int k = sg; k < kApplyBlk; k += kSeg) s += part[v][k];
-                red[v][sg] = s;
+                for (int k = sg; k < kApplyBlk; k += kSeg) s += L.part[v][k];
+                L.red[v][sg] = s;
             }
             __syncthreads();
             if (tid < kVals) {
@@ -486,19 +489,24 @@ __global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
                 if (slot < kChunkNC && (pl > 0 || ch == 0) && pg < nf) {
                     double s = 0.0;
 #pragma unroll
-                    for (int sg = 0; sg < kSeg; ++sg) s += red[tid][sg];
+                    for (int sg = 0; sg < kSeg; ++sg) s += L.red[tid][sg];
                     cc[comp * nf + pg] = s;
                 }
             }
             __syncthreads();
         }
     }
-    if (!wall && tid < 3) cc[nc + tid] = r[coff + tid];
-    __syncthreads();
-    // d = G^{-1} c: the k range is split over kApplyBlk / nq groups of nq
-    // lanes so that each lane waits on a few L2 loads, not nq in sequence
+}
+
+// d = G^{-1} c with G^{-1} stored transposed (Gt[k nq + j]); dd = d + 2 c on
+// the density coefficients, and the body's constraint unknowns d[nc..nc+2]
+// go straight to zc.  The k range is split over kApplyBlk / nq groups of nq
+// lanes, so each lane waits on a few L2 loads, not nq in sequence.
+__device__ __forceinline__ void pc_coarse_solve(ApplyLds& L, const double* __restrict__ Gt, int nq, int nc,
+                                double* zc) {
+    const int tid = threadIdx.x;
+    const int groups = kApplyBlk / nq;  // nq <= kMaxNQ < kApplyBlk
     {
-        const int groups = kApplyBlk / nq;  // nq <= kMaxNQ < kApplyBlk
         const int span = (nq + groups - 1) / groups;
         const int j = tid % nq, g = tid / nq;
         if (g < groups) {
@@ -506,22 +514,31 @@ __global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
             double d = 0.0;
 #pragma unroll 8
             for (int k = k0; k < k1; ++k)
-                d = __builtin_fma(Gt[static_cast<size_t>(k) * nq + j], cc[k], d);
-            part[0][g * nq + j] = d;
+                d = __builtin_fma(Gt[static_cast<size_t>(k) * nq + j], L.cc[k], d);
+            L.part[0][g * nq + j] = d;
         }
     }
     __syncthreads();
     if (tid < nq) {
-        const int groups = kApplyBlk / nq;
         double d = 0.0;
-        for (int g = 0; g < groups; ++g) d += part[0][g * nq + tid];
+        for (int g = 0; g < groups; ++g) d += L.part[0][g * nq + tid];
         if (tid < nc)
-            dd[tid] = d + 2.0 * cc[tid];
+            L.dd[tid] = d + 2.0 * L.cc[tid];
         else
-            z[coff + tid - nc] = d;
+            zc[tid - nc] = d;
     }
     __syncthreads();
-    for (int i = tid; i < len; i += kApplyBlk) {
+}
+
+// z = Q dd - 2 r at the curve's points [i0, i1)
+__device__ __forceinline__ void pc_expand(const ApplyLds& L, int len, int nf, int i0, int i1,
+                          const double* __restrict__ r, int xoff, int yoff,
+                          double* __restrict__ z) {
+    const int K = (nf - 1) / 2;
+    const double c0 = 1.0 / sqrt(static_cast<double>(len));
+    const double c1 = sqrt(2.0 / static_cast<double>(len));
+    const double* dd = L.dd;
+    for (int i = i0 + static_cast<int>(threadIdx.x); i < i1; i += kApplyBlk) {
         double s1, c1t;
         sincospi(2.0 * static_cast<double>(i) / static_cast<double>(len), &s1, &c1t);
         double zx = __builtin_fma(c0, dd[0], -2.0 * r[xoff + i]);
@@ -538,6 +555,50 @@ __global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
         z[xoff + i] = zx;
         z[yoff + i] = zy;
     }
+}
+
+// grid nseg + m: wall segments, then bodies.  wpart[s kMaxNQ + j] receives
+// segment s's partial wall coefficients.
+__global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
+    int nfw, int nfb, int no, int n, int m, int nseg, const double* __restrict__ Gb,
+    const double* __restrict__ r, double* __restrict__ wpart, double* __restrict__ z) {
+    __shared__ ApplyLds L;
+    const int tid = threadIdx.x;
+    if (static_cast<int>(blockIdx.x) < nseg) {
+        const int i0 = blockIdx.x * kApplyBlk, i1 = min(no, i0 + kApplyBlk);
+        pc_project(L, no, nfw, i0, i1, r, 0, no, wpart + static_cast<size_t>(blockIdx.x) * kMaxNQ);
+        return;
+    }
+    const int c = static_cast<int>(blockIdx.x) - nseg + 1;  // body c, curve index 1..m
+    const int nf = nfb, nc = 2 * nf, nq = nc + 3;
+    const int xoff = 2 * no + 2 * (c - 1) * n;
+    const int yoff = xoff + n;
+    const int coff = 2 * no + 2 * n * m + 3 * (c - 1);
+    pc_project(L, n, nf, 0, n, r, xoff, yoff, L.cc);
+    if (tid < 3) L.cc[nc + tid] = r[coff + tid];
+    __syncthreads();
+    pc_coarse_solve(L, Gb + static_cast<size_t>(c - 1) * nq * nq, nq, nc, z + coff);
+    pc_expand(L, n, nf, 0, n, r, xoff, yoff, z);
+}
+
+// The wall after k_pc_apply: grid nseg, block s expands segment s.
+__global__ __launch_bounds__(kApplyBlk) void k_pc_wall(int nfw, int no, int nseg,
+                                                       const double* __restrict__ Gw,
+                                                       const double* __restrict__ wpart,
+                                                       const double* __restrict__ r,
+                                                       double* __restrict__ z) {
+    __shared__ ApplyLds L;
+    const int tid = threadIdx.x;
+    const int nc = 2 * nfw;
+    if (tid < nc) {
+        double s = 0.0;
+        for (int g = 0; g < nseg; ++g) s += wpart[static_cast<size_t>(g) * kMaxNQ + tid];
+        L.cc[tid] = s;
+    }
+    __syncthreads();
+    pc_coarse_solve(L, Gw, nc, nc, nullptr);
+    const int i0 = blockIdx.x * kApplyBlk, i1 = min(no, i0 + kApplyBlk);
+    pc_expand(L, no, nfw, i0, i1, r, 0, no, z);
 }
 
 }  // namespace
@@ -576,6 +637,9 @@ void precond_setup(Context& c) {
     c.pc_phi_b.ensure(static_cast<size_t>(nfb) * std::max(n, 1));
     c.pc_phi_w.ensure(static_cast<size_t>(nfw) * no);
     c.pc_S.ensure(static_cast<size_t>(no) * 3 * nfw + static_cast<size_t>(m) * n * 3 * nfb);
+    // the wall's partial coefficients of the apply (allocated here, before
+    // any GMRES chunk may be captured into a graph)
+    c.pc_wpart.ensure(static_cast<size_t>(cdiv_int(no, kApplyBlk)) * kMaxNQ);
     k_pc_basis<<<cdiv_int(no, kBlk), kBlk, 0, s>>>(no, kw, c.pc_phi_w.p);
     STOKES_LAUNCH_CHECK();
     if (m > 0) {
@@ -608,8 +672,13 @@ void precond_setup(Context& c) {
 void precond_apply(Context& c, const double* r, double* z) {
     Geometry& g = c.g;
     if (c.time_kernels) c.t_precond.start(c.stream);
-    k_pc_apply<<<g.m + 1, kApplyBlk, 0, c.stream>>>(c.pc_nfw, c.pc_nfb, g.no, g.n, g.m,
-                                                   c.pc_Gw.p, c.pc_Gb.p, r, z);
+    const int nseg = cdiv_int(g.no, kApplyBlk);
+    c.pc_wpart.ensure(static_cast<size_t>(nseg) * kMaxNQ);
+    k_pc_apply<<<nseg + g.m, kApplyBlk, 0, c.stream>>>(c.pc_nfw, c.pc_nfb, g.no, g.n, g.m, nseg,
+                                                      c.pc_Gb.p, r, c.pc_wpart.p, z);
+    STOKES_LAUNCH_CHECK();
+    k_pc_wall<<<nseg, kApplyBlk, 0, c.stream>>>(c.pc_nfw, g.no, nseg, c.pc_Gw.p, c.pc_wpart.p, r,
+                                                z);
     STOKES_LAUNCH_CHECK();
     // bytes: the vector read and written once
     if (c.time_kernels) c.t_precond.stop(c.stream, 2.0 * 8.0 * static_cast<double>(g.nunk));
