@@ -425,8 +425,8 @@ struct ApplyLds {
 
 // cc[comp nf + f] = sum over points [i0, i1) of the curve of phi_f(i) r_comp(i)
 // (block-wide; every thread calls it).
-__device__ __forceinline__ void pc_project(ApplyLds& L, int len, int nf, int i0, int i1, const double* r,
-                           int xoff, int yoff, double* cc) {
+__device__ __forceinline__ void pc_project(ApplyLds& L, int len, int nf, int i0, int i1,
+                                           const double* r, int xoff, int yoff, double* cc) {
     constexpr int kChunkNC = 2 * kChunkNF;
     const int tid = threadIdx.x;
     const int K = (nf - 1) / 2;
@@ -502,8 +502,8 @@ __device__ __forceinline__ void pc_project(ApplyLds& L, int len, int nf, int i0,
 // the density coefficients, and the body's constraint unknowns d[nc..nc+2]
 // go straight to zc.  The k range is split over kApplyBlk / nq groups of nq
 // lanes, so each lane waits on a few L2 loads, not nq in sequence.
-__device__ __forceinline__ void pc_coarse_solve(ApplyLds& L, const double* __restrict__ Gt, int nq, int nc,
-                                double* zc) {
+__device__ __forceinline__ void pc_coarse_solve(ApplyLds& L, const double* __restrict__ Gt,
+                                                int nq, int nc, double* zc) {
     const int tid = threadIdx.x;
     const int groups = kApplyBlk / nq;  // nq <= kMaxNQ < kApplyBlk
     {
@@ -532,8 +532,8 @@ __device__ __forceinline__ void pc_coarse_solve(ApplyLds& L, const double* __res
 
 // z = Q dd - 2 r at the curve's points [i0, i1)
 __device__ __forceinline__ void pc_expand(const ApplyLds& L, int len, int nf, int i0, int i1,
-                          const double* __restrict__ r, int xoff, int yoff,
-                          double* __restrict__ z) {
+                                          const double* __restrict__ r, int xoff, int yoff,
+                                          double* __restrict__ z) {
     const int K = (nf - 1) / 2;
     const double c0 = 1.0 / sqrt(static_cast<double>(len));
     const double c1 = sqrt(2.0 / static_cast<double>(len));
