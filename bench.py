@@ -35,6 +35,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "erosion steps/s at 1e-10 GMRES residual: learned+mixed vs plain FP64 on GPU"
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector (= FP64 matrix) peak
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (packed)
 PEAK_HBM_GBS = 8000.0
 FF_FLOP_PER_PAIR = 20.0  # canonical flops per ordered source-target pair
 STRESS_FLOP_PER_PAIR = 130.0  # per (target body point, source node) pair: 3 passes x 5 fields
@@ -242,17 +243,37 @@ def main() -> int:
         theta_err = max(float(np.max(np.abs(a.theta - b.theta)))
                         for a, b in zip(acc["state"].thlenvec, plain["state"].thlenvec))
 
+    def roofline_of_run(r):
+        """The run's canonical work at the chip's peaks: far-field pairs at 20
+        FLOP (FP64 launches at the FP64 peak, FP32 ones at the FP32 peak) and
+        stress pairs at 130 FLOP (FP64).  Returns (ideal seconds for the timed
+        steps, ideal seconds per operator application)."""
+        k = r["kt"]
+        scale = args.steps if r["graphs"] else 1
+        t64 = FF_FLOP_PER_PAIR * k["farfield"]["work"] / (PEAK_FP64_TFLOPS * 1e12)
+        t32 = FF_FLOP_PER_PAIR * k["farfield32"]["work"] / (PEAK_FP32_TFLOPS * 1e12)
+        tst = STRESS_FLOP_PER_PAIR * k["stress_eval"]["work"] / (PEAK_FP64_TFLOPS * 1e12)
+        nmv = k["farfield"]["launches"] + k["farfield32"]["launches"]
+        return scale * (t64 + t32 + tst), (t64 + t32) / max(1, nmv)
+
     def summary(r):
         solves = max(1, r["timings"].get("solves", 1))
+        ideal_s, ideal_mv = roofline_of_run(r)
+        mvps = ((r["kt"]["farfield"]["launches"] + r["kt"]["farfield32"]["launches"])
+                * (args.steps if r["graphs"] else 1) / r["elapsed"])
         return {"steps_per_s": args.steps / r["elapsed"],
+                # the same steps with every far-field and stress FLOP at peak
+                "steps_per_s_at_peak": args.steps / ideal_s if ideal_s > 0 else None,
+                "steps_roofline_frac": ideal_s / r["elapsed"],
+                "matvecs_per_s_at_peak": 1.0 / ideal_mv if ideal_mv > 0 else None,
+                "matvecs_roofline_frac": mvps * ideal_mv,
                 "ms_per_step": 1e3 * r["elapsed"] / args.steps,
                 "gmres_iters_per_solve": r["timings"].get("gmres_iters", 0) / solves,
                 "true_relres_max": max(si["true_relres"] for si in r["solve_info"]),
                 "fp32_iters_last_solve": r["solve_info"][-1]["fp32_iters"],
                 # operator applications (FP64 and FP32 far field); with graphs
                 # the kernel counts come from one extra step, scaled
-                "matvecs_per_s": (r["kt"]["farfield"]["launches"] + r["kt"]["farfield32"]["launches"])
-                * (args.steps if r["graphs"] else 1) / r["elapsed"]}
+                "matvecs_per_s": mvps}
 
     sp, sa = (summary(plain) if plain else None), summary(acc)
     elapsed, kt = acc["elapsed"], acc["kt"]
