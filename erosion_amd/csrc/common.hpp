@@ -332,6 +332,9 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl, double tol, doub
 // precond.hip: set up for the current geometry, then z = P^{-1} r
 void precond_setup(Context& c);
 void precond_apply(Context& c, const double* r_dev, double* z_dev);
+// The same with r_dev first scaled in place by st->inv_snormw unless st is
+// null or has stopped (GMRES's normalisation of its new Krylov vector).
+void precond_apply_scaled(Context& c, double* r_dev, double* z_dev, const GmresState* st);
 
 // batch.hip: independent geometries solved by a pool of worker contexts
 struct BatchGeometry {

@@ -592,7 +592,9 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
                     double* vk = c.V.p + static_cast<size_t>(kk) * npad;
                     double* w = c.V.p + static_cast<size_t>(kk + 1) * npad;
                     if (pc) {
-                        precond_apply(c, vk, c.pc_z.p);
+                        // v_k = w_k / ||w_k|| happens in the preconditioner's
+                        // first read of it (v_0 is scaled before the loop)
+                        precond_apply_scaled(c, vk, c.pc_z.p, kk > 0 ? c.st.p : nullptr);
                         apply_operator(c, c.pc_z.p, w, ibary);
                     } else {
                         apply_operator(c, vk, w, ibary);
@@ -608,8 +610,10 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
                         c.t_gs.stop(s, 4.0 * static_cast<double>(kk + 1) *
                                            static_cast<double>(npad) * 8.0);
                     }
-                    k_scale_row<<<std::min(1024, ebx * 2), kBlk, 0, s>>>(w, npad, c.st.p, 0);
-                    STOKES_LAUNCH_CHECK();
+                    if (!pc) {
+                        k_scale_row<<<std::min(1024, ebx * 2), kBlk, 0, s>>>(w, npad, c.st.p, 0);
+                        STOKES_LAUNCH_CHECK();
+                    }
                 }
             };
             if (c.use_graphs && !c.time_kernels)

@@ -425,8 +425,11 @@ struct ApplyLds {
 
 // cc[comp nf + f] = sum over points [i0, i1) of the curve of phi_f(i) r_comp(i)
 // (block-wide; every thread calls it).
+// With scale, r is first multiplied by sc in place (the Krylov vector's
+// normalisation, left to this kernel by GMRES).
 __device__ __forceinline__ void pc_project(ApplyLds& L, int len, int nf, int i0, int i1,
-                                           const double* r, int xoff, int yoff, double* cc) {
+                                           double* r, int xoff, int yoff, bool scale, double sc,
+                                           double* cc) {
     constexpr int kChunkNC = 2 * kChunkNF;
     const int tid = threadIdx.x;
     const int K = (nf - 1) / 2;
@@ -441,7 +444,13 @@ __device__ __forceinline__ void pc_project(ApplyLds& L, int len, int nf, int i0,
 #pragma unroll
         for (int v = 0; v < kChunkNF; ++v) ax[v] = ay[v] = 0.0;
         for (int i = i0 + tid; i < i1; i += kApplyBlk) {
-            const double rx = r[xoff + i], ry = r[yoff + i];
+            double rx = r[xoff + i], ry = r[yoff + i];
+            if (scale && ch == 0) {
+                rx *= sc;
+                ry *= sc;
+                r[xoff + i] = rx;
+                r[yoff + i] = ry;
+            }
             double s1, c1t;
             sincospi(2.0 * static_cast<double>(i) / static_cast<double>(len), &s1, &c1t);
             double ck = 1.0, sk = 0.0;
@@ -558,15 +567,21 @@ __device__ __forceinline__ void pc_expand(const ApplyLds& L, int len, int nf, in
 }
 
 // grid nseg + m: wall segments, then bodies.  wpart[s kMaxNQ + j] receives
-// segment s's partial wall coefficients.
+// segment s's partial wall coefficients.  With sst (a GMRES state that has not
+// stopped), r is first scaled in place by sst->inv_snormw: the new Krylov
+// vector's normalisation, folded into its first reader.
 __global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
     int nfw, int nfb, int no, int n, int m, int nseg, const double* __restrict__ Gb,
-    const double* __restrict__ r, double* __restrict__ wpart, double* __restrict__ z) {
+    const GmresState* __restrict__ sst, double* __restrict__ r, double* __restrict__ wpart,
+    double* __restrict__ z) {
     __shared__ ApplyLds L;
     const int tid = threadIdx.x;
+    const bool scale = sst != nullptr && !sst->done;
+    const double sc = scale ? sst->inv_snormw : 1.0;
     if (static_cast<int>(blockIdx.x) < nseg) {
         const int i0 = blockIdx.x * kApplyBlk, i1 = min(no, i0 + kApplyBlk);
-        pc_project(L, no, nfw, i0, i1, r, 0, no, wpart + static_cast<size_t>(blockIdx.x) * kMaxNQ);
+        pc_project(L, no, nfw, i0, i1, r, 0, no, scale, sc,
+                   wpart + static_cast<size_t>(blockIdx.x) * kMaxNQ);
         return;
     }
     const int c = static_cast<int>(blockIdx.x) - nseg + 1;  // body c, curve index 1..m
@@ -574,8 +589,15 @@ __global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
     const int xoff = 2 * no + 2 * (c - 1) * n;
     const int yoff = xoff + n;
     const int coff = 2 * no + 2 * n * m + 3 * (c - 1);
-    pc_project(L, n, nf, 0, n, r, xoff, yoff, L.cc);
-    if (tid < 3) L.cc[nc + tid] = r[coff + tid];
+    pc_project(L, n, nf, 0, n, r, xoff, yoff, scale, sc, L.cc);
+    if (tid < 3) {
+        double v = r[coff + tid];
+        if (scale) {
+            v *= sc;
+            r[coff + tid] = v;
+        }
+        L.cc[nc + tid] = v;
+    }
     __syncthreads();
     pc_coarse_solve(L, Gb + static_cast<size_t>(c - 1) * nq * nq, nq, nc, z + coff);
     pc_expand(L, n, nf, 0, n, r, xoff, yoff, z);
@@ -669,19 +691,24 @@ void precond_setup(Context& c) {
     c.pc_nfw = nfw;
 }
 
-void precond_apply(Context& c, const double* r, double* z) {
+void precond_apply_scaled(Context& c, double* r, double* z, const GmresState* st) {
     Geometry& g = c.g;
     if (c.time_kernels) c.t_precond.start(c.stream);
     const int nseg = cdiv_int(g.no, kApplyBlk);
     c.pc_wpart.ensure(static_cast<size_t>(nseg) * kMaxNQ);
     k_pc_apply<<<nseg + g.m, kApplyBlk, 0, c.stream>>>(c.pc_nfw, c.pc_nfb, g.no, g.n, g.m, nseg,
-                                                      c.pc_Gb.p, r, c.pc_wpart.p, z);
+                                                      c.pc_Gb.p, st, r, c.pc_wpart.p, z);
     STOKES_LAUNCH_CHECK();
     k_pc_wall<<<nseg, kApplyBlk, 0, c.stream>>>(c.pc_nfw, g.no, nseg, c.pc_Gw.p, c.pc_wpart.p, r,
                                                 z);
     STOKES_LAUNCH_CHECK();
     // bytes: the vector read and written once
     if (c.time_kernels) c.t_precond.stop(c.stream, 2.0 * 8.0 * static_cast<double>(g.nunk));
+}
+
+void precond_apply(Context& c, const double* r, double* z) {
+    // unscaled: the kernel writes r back only when it scales
+    precond_apply_scaled(c, const_cast<double*>(r), z, nullptr);
 }
 
 }  // namespace stokes
