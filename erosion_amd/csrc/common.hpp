@@ -310,7 +310,10 @@ void build_geometry(Context& c, int n, int m, int no, const double* xx, const do
 void build_near_list(Context& c, int ibary);
 
 // operator.hip : y = A x for the completed double-layer operator
-void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary);
+// gathered: c.ex/c.ey already hold x_dev's density in point order (the
+// preconditioner's apply writes them beside its output).
+void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary,
+                    bool gathered = false);
 // run the geometry-dependent set-up apply_operator would otherwise do inside
 // its first call (far-field tuning, FP32 source tiles, stokeslet logarithms),
 // so a captured sequence of operator applications holds none of it; x_dev is
@@ -333,7 +336,8 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl, double tol, doub
 void precond_setup(Context& c);
 void precond_apply(Context& c, const double* r_dev, double* z_dev);
 // The same with r_dev first scaled in place by st->inv_snormw unless st is
-// null or has stopped (GMRES's normalisation of its new Krylov vector).
+// null or has stopped (GMRES's normalisation of its new Krylov vector).  Both
+// also leave z's density in point order in c.ex/c.ey (the operator's gather).
 void precond_apply_scaled(Context& c, double* r_dev, double* z_dev, const GmresState* st);
 
 // batch.hip: independent geometries solved by a pool of worker contexts

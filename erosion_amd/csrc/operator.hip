@@ -11,7 +11,8 @@
 //                 (:1055-1096), rank-one wall term (:1102-1118)
 //
 // Kernel map, one matvec:
-//   k_gather       den -> (etax, etay) in point order
+//   k_gather       den -> (etax, etay) in point order (skipped when the
+//                  preconditioner's apply has written them)
 //   k_farfield     LDS-tiled all-pairs sum, target per lane, partial sums per
 //                  source split                                (VALU bound)
 //   k_fields       the five Cauchy densities per point
@@ -1025,7 +1026,7 @@ int farfield_choice(const Context& c, bool fp32) {
     return it == choice.end() ? -1 : it->second;
 }
 
-void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary) {
+void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, bool gathered) {
     Geometry& g = c.g;
     NearList& L = c.near;
     hipStream_t s = c.stream;
@@ -1033,7 +1034,7 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary) {
     const size_t N = static_cast<size_t>(np);
     c.scal.ensure(8);
 
-    gather_density(c, x_dev);
+    if (!gathered) gather_density(c, x_dev);
     const bool fp32 = c.ff_fp32_now;
     if (fp32) ff32_sources(c);
     const FFPlan plan = ff_plan(np, kFFShapes[tune_farfield(c, fp32)], fp32);

@@ -539,10 +539,12 @@ __device__ __forceinline__ void pc_coarse_solve(ApplyLds& L, const double* __res
     __syncthreads();
 }
 
-// z = Q dd - 2 r at the curve's points [i0, i1)
+// z = Q dd - 2 r at the curve's points [i0, i1); the same values also go to
+// ex/ey at point index poff + i (the density in point order).
 __device__ __forceinline__ void pc_expand(const ApplyLds& L, int len, int nf, int i0, int i1,
                                           const double* __restrict__ r, int xoff, int yoff,
-                                          double* __restrict__ z) {
+                                          double* __restrict__ z, int poff,
+                                          double* __restrict__ ex, double* __restrict__ ey) {
     const int K = (nf - 1) / 2;
     const double c0 = 1.0 / sqrt(static_cast<double>(len));
     const double c1 = sqrt(2.0 / static_cast<double>(len));
@@ -563,6 +565,8 @@ __device__ __forceinline__ void pc_expand(const ApplyLds& L, int len, int nf, in
         }
         z[xoff + i] = zx;
         z[yoff + i] = zy;
+        ex[poff + i] = zx;
+        ey[poff + i] = zy;
     }
 }
 
@@ -573,7 +577,7 @@ __device__ __forceinline__ void pc_expand(const ApplyLds& L, int len, int nf, in
 __global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
     int nfw, int nfb, int no, int n, int m, int nseg, const double* __restrict__ Gb,
     const GmresState* __restrict__ sst, double* __restrict__ r, double* __restrict__ wpart,
-    double* __restrict__ z) {
+    double* __restrict__ z, double* __restrict__ ex, double* __restrict__ ey) {
     __shared__ ApplyLds L;
     const int tid = threadIdx.x;
     const bool scale = sst != nullptr && !sst->done;
@@ -600,7 +604,7 @@ __global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
     }
     __syncthreads();
     pc_coarse_solve(L, Gb + static_cast<size_t>(c - 1) * nq * nq, nq, nc, z + coff);
-    pc_expand(L, n, nf, 0, n, r, xoff, yoff, z);
+    pc_expand(L, n, nf, 0, n, r, xoff, yoff, z, no + (c - 1) * n, ex, ey);
 }
 
 // The wall after k_pc_apply: grid nseg, block s expands segment s.
@@ -608,7 +612,9 @@ __global__ __launch_bounds__(kApplyBlk) void k_pc_wall(int nfw, int no, int nseg
                                                        const double* __restrict__ Gw,
                                                        const double* __restrict__ wpart,
                                                        const double* __restrict__ r,
-                                                       double* __restrict__ z) {
+                                                       double* __restrict__ z,
+                                                       double* __restrict__ ex,
+                                                       double* __restrict__ ey) {
     __shared__ ApplyLds L;
     const int tid = threadIdx.x;
     const int nc = 2 * nfw;
@@ -620,7 +626,7 @@ __global__ __launch_bounds__(kApplyBlk) void k_pc_wall(int nfw, int no, int nseg
     __syncthreads();
     pc_coarse_solve(L, Gw, nc, nc, nullptr);
     const int i0 = blockIdx.x * kApplyBlk, i1 = min(no, i0 + kApplyBlk);
-    pc_expand(L, no, nfw, i0, i1, r, 0, no, z);
+    pc_expand(L, no, nfw, i0, i1, r, 0, no, z, 0, ex, ey);
 }
 
 }  // namespace
@@ -696,11 +702,14 @@ void precond_apply_scaled(Context& c, double* r, double* z, const GmresState* st
     if (c.time_kernels) c.t_precond.start(c.stream);
     const int nseg = cdiv_int(g.no, kApplyBlk);
     c.pc_wpart.ensure(static_cast<size_t>(nseg) * kMaxNQ);
+    c.ex.ensure(static_cast<size_t>(g.npts));
+    c.ey.ensure(static_cast<size_t>(g.npts));
     k_pc_apply<<<nseg + g.m, kApplyBlk, 0, c.stream>>>(c.pc_nfw, c.pc_nfb, g.no, g.n, g.m, nseg,
-                                                      c.pc_Gb.p, st, r, c.pc_wpart.p, z);
+                                                      c.pc_Gb.p, st, r, c.pc_wpart.p, z, c.ex.p,
+                                                      c.ey.p);
     STOKES_LAUNCH_CHECK();
     k_pc_wall<<<nseg, kApplyBlk, 0, c.stream>>>(c.pc_nfw, g.no, nseg, c.pc_Gw.p, c.pc_wpart.p, r,
-                                                z);
+                                                z, c.ex.p, c.ey.p);
     STOKES_LAUNCH_CHECK();
     // bytes: the vector read and written once
     if (c.time_kernels) c.t_precond.stop(c.stream, 2.0 * 8.0 * static_cast<double>(g.nunk));
