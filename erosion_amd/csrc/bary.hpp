@@ -22,18 +22,23 @@ struct NodeView {
 // of one curve within 1e-10 of the same target cannot occur for distinct
 // boundary nodes, so one accumulation per call is the same thing.
 // Rare (a target on a node of another curve); kept out of line so the hot
-// loops around it keep their registers.
-__device__ __noinline__ cplx coincident_term(const NodeView& nv, int off, int len, int k, int f,
-                                             cplx zt, cplx a, cplx d) {
-    const size_t N = nv.N;
-    const cplx bk = cmk(nv.bd[(2 * f) * N + off + k], nv.bd[(2 * f + 1) * N + off + k]);
-    const cplx zk = cmk(nv.x[off + k], nv.y[off + k]);
+// loops around it keep their registers.  The node arrays come by value: a
+// reference to the caller's NodeView would make the caller keep the whole
+// struct in scratch memory.
+__device__ __noinline__ cplx coincident_term_p(const double* __restrict__ x,
+                                               const double* __restrict__ y,
+                                               const double* __restrict__ dzr,
+                                               const double* __restrict__ dzi,
+                                               const double* __restrict__ bd, size_t N, int off,
+                                               int len, int k, int f, cplx zt, cplx a, cplx d) {
+    const cplx bk = cmk(bd[(2 * f) * N + off + k], bd[(2 * f + 1) * N + off + k]);
+    const cplx zk = cmk(x[off + k], y[off + k]);
     cplx tmp1 = cmk(0, 0), tmp2 = cmk(0, 0);
     for (int l = 0; l < len; ++l) {
         const int p = off + l;
-        const cplx zl = cmk(nv.x[p], nv.y[p]);
-        const cplx dzl = cmk(nv.dzr[p], nv.dzi[p]);
-        const cplx bl = cmk(nv.bd[(2 * f) * N + p], nv.bd[(2 * f + 1) * N + p]);
+        const cplx zl = cmk(x[p], y[p]);
+        const cplx dzl = cmk(dzr[p], dzi[p]);
+        const cplx bl = cmk(bd[(2 * f) * N + p], bd[(2 * f + 1) * N + p]);
         const cplx izl = cinv(csub(zl, zt));
         const cplx izla = cinv(csub(zl, a));
         if (l != k) {
@@ -44,7 +49,12 @@ __device__ __noinline__ cplx coincident_term(const NodeView& nv, int off, int le
     }
     cplx num = csub(cdiv(tmp1, tmp2), cmul(d, bk));
     cplx d2 = cmul(d, d);
-    return cmul(cdiv(cdiv(num, d2), csub(zt, a)), cmk(nv.dzr[off + k], nv.dzi[off + k]));
+    return cmul(cdiv(cdiv(num, d2), csub(zt, a)), cmk(dzr[off + k], dzi[off + k]));
+}
+
+__device__ __forceinline__ cplx coincident_term(const NodeView& nv, int off, int len, int k,
+                                                int f, cplx zt, cplx a, cplx d) {
+    return coincident_term_p(nv.x, nv.y, nv.dzr, nv.dzi, nv.bd, nv.N, off, len, k, f, zt, a, d);
 }
 
 __device__ __forceinline__ cplx bdv(const NodeView& nv, int f, int p) {

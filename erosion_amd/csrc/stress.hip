@@ -114,6 +114,12 @@ __global__ __launch_bounds__(kBT) void k_bary_eval(
         double v2[12];
 #pragma unroll
         for (int i = 0; i < 12; ++i) v2[i] = 0.0;
+        // node of an exterior curve within 1e-10 of the target: its term is
+        // the coincident-node form, added after the loop so that the loop
+        // holds no call.  Lanes that skip this curve (their own body, when a
+        // block holds targets of two bodies) sit exactly on its nodes: they
+        // discard the sums, so they never take that form.
+        int kc = -1;
         for (int t0 = 0; t0 < len; t0 += kNodeTile) {
             const int tn = min(kNodeTile, len - t0);
             if (!one_tile) stage_nodes(T, nv, off + t0, tn);
@@ -123,16 +129,8 @@ __global__ __launch_bounds__(kBT) void k_bary_eval(
                 const double ir = rcp64(rho2);
                 const cplx id = cmk(d.r * ir, -d.i * ir);
                 const cplx w = cmul(cmk(T.dzr[k], T.dzi[k]), id);
-                // lanes that skip this curve (their own body, when a block
-                // holds targets of two bodies) sit exactly on its nodes: they
-                // discard the sums, so they must not take the serial branch
                 if (ext && !skip && !(sqrt(rho2) > 1.0e-10)) {
-#pragma unroll
-                    for (int f = 0; f < 5; ++f) {
-                        cplx t = coincident_term(nv, off, len, t0 + k, f, zt, a, d);
-                        v2[2 * f] += t.r;
-                        v2[2 * f + 1] += t.i;
-                    }
+                    kc = t0 + k;
                 } else {
                     const cplx w2 = cmul(w, id);
 #pragma unroll
@@ -148,6 +146,15 @@ __global__ __launch_bounds__(kBT) void k_bary_eval(
                     v2[10] += wa.r;
                     v2[11] += wa.i;
                 }
+            }
+        }
+        if (kc >= 0) {
+            const cplx d = cmk(nv.x[off + kc] - xt, nv.y[off + kc] - yt);
+#pragma unroll
+            for (int f = 0; f < 5; ++f) {
+                const cplx t = coincident_term(nv, off, len, kc, f, zt, a, d);
+                v2[2 * f] += t.r;
+                v2[2 * f + 1] += t.i;
             }
         }
         const cplx iden = cinv(ext ? cmul(csub(zt, a), cmk(v2[10], v2[11])) : S0);
