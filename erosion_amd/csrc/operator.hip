@@ -577,26 +577,24 @@ __global__ __launch_bounds__(64 * NSL) void k_bdval(
 
 // Boundary values of the wall (curve 0, interior rule) from the ksplit
 // partial sums of k_bdval: bd = -i B / 2pi - f_j, B = sum_parts(B) - dtheta
-// sum_parts(D).
+// sum_parts(D).  grid (ceil(no / 256), 5): one thread per (node, field).
 __global__ void k_bdval_finish(int no, int npts, const double* __restrict__ fld,
                                const double* __restrict__ part, int ksplit,
                                double* __restrict__ bd) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
     if (j >= no) return;
     const size_t N = static_cast<size_t>(npts);
-    double vals[20];
-#pragma unroll
-    for (int q = 0; q < 20; ++q) vals[q] = 0.0;
+    double vals[4] = {0.0, 0.0, 0.0, 0.0};
     for (int z = 0; z < ksplit; ++z) {
 #pragma unroll
-        for (int q = 0; q < 20; ++q) vals[q] += part[(static_cast<size_t>(z) * 20 + q) * no + j];
+        for (int q = 0; q < 4; ++q)
+            vals[q] += part[(static_cast<size_t>(z) * 20 + 4 * f + q) * no + j];
     }
     const double dth = kTwoPi / static_cast<double>(no);
-#pragma unroll
-    for (int f = 0; f < 5; ++f) {
+    {
         const cplx fj = cmk(fld[(2 * f) * N + j], fld[(2 * f + 1) * N + j]);
-        const cplx Bf =
-            cmk(vals[4 * f + 0] - dth * vals[4 * f + 2], vals[4 * f + 1] - dth * vals[4 * f + 3]);
+        const cplx Bf = cmk(vals[0] - dth * vals[2], vals[1] - dth * vals[3]);
         const cplx iB = cmk(-Bf.i / kTwoPi, Bf.r / kTwoPi);
         bd[(2 * f) * N + j] = -iB.r - fj.r;
         bd[(2 * f + 1) * N + j] = -iB.i - fj.i;
@@ -918,8 +916,8 @@ void boundary_values(Context& c, const int* curves_dev, int ncurves, int wall_fi
                                                    g.dzr.p, g.dzi.p, g.dtab_no.p, g.dtab_n.p,
                                                    c.fld.p, c.bd.p, ks, c.bdpart.p);
         STOKES_LAUNCH_CHECK();
-        k_bdval_finish<<<cdiv_int(g.no, kBlk), kBlk, 0, s>>>(g.no, np, c.fld.p, c.bdpart.p, ks,
-                                                             c.bd.p);
+        k_bdval_finish<<<dim3(cdiv_int(g.no, kBlk), 5), kBlk, 0, s>>>(g.no, np, c.fld.p,
+                                                                      c.bdpart.p, ks, c.bd.p);
         STOKES_LAUNCH_CHECK();
     }
     if (ncurves > wall_first) {
