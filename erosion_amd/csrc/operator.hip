@@ -156,6 +156,10 @@ __device__ __forceinline__ void ff_tile_shared(const double2* __restrict__ sxy,
     }
 }
 
+// Coordinate of a lane's unused target slots in the FP64 far field: far from
+// every source and from the padding sources at +1e30.
+constexpr double kNoTarget = -1.0e30;
+
 // Source moments of one point for the expanded pair term.
 __device__ __forceinline__ void src_moments(double mx, double my, double ex, double ey,
                                             double2* ab, double* c) {
@@ -181,10 +185,12 @@ __global__ __launch_bounds__(kBlk) void k_farfield(
     int t[TPT];
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
+        // targets past npts sit where no source is (a source at the same
+        // point would zero the product of rho^2 the lane's targets share)
         t[q] = tb0 + q * kBlk + tid;
         const bool valid = t[q] < npts;
-        xt[q] = valid ? x[t[q]] : 0.0;
-        yt[q] = valid ? y[t[q]] : 0.0;
+        xt[q] = valid ? x[t[q]] : kNoTarget;
+        yt[q] = valid ? y[t[q]] : kNoTarget;
         u[q] = v[q] = 0.0;
     }
     const int s0 = blockIdx.y * split_len;
