@@ -280,7 +280,7 @@ struct Context {
     double tol = 1e-8;
     // right preconditioner (precond.hip): kind 0 none (the reference's
     // unpreconditioned GMRES), 1 per-curve two-level Fourier
-    int pc_kind = 0, pc_body_modes = 16, pc_wall_modes = 16;
+    int pc_kind = 0, pc_body_modes = 32, pc_wall_modes = 16;
     int pc_nfb = 0, pc_nfw = 0;
     int pc_wall_no = -1, pc_wall_kw = -1;  // key of the kept wall block
     DevBuf<double> pc_phi_b, pc_phi_w, pc_S, pc_Gb, pc_Gw, pc_z, pc_wpart;

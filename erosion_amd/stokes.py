@@ -237,7 +237,7 @@ PRECOND_NONE = 0
 PRECOND_CURVE_FOURIER = 1
 
 
-def set_preconditioner(kind: int, body_modes: int = 16, wall_modes: int = 16) -> None:
+def set_preconditioner(kind: int, body_modes: int = 32, wall_modes: int = 16) -> None:
     """Select the right preconditioner of stokessolver (0 none, as the reference;
     1 per-curve two-level Fourier).  Only the iteration count changes."""
     if lib().erosion_set_preconditioner(int(kind), int(body_modes), int(wall_modes)) != 0:

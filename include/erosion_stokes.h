@@ -190,8 +190,10 @@ int erosion_measure_copy(int64_t bytes, int64_t reps, double* ms);
  * inverse of the jump -1/2) on the remaining modes.  GMRES then iterates on
  * A P^{-1}; its residual is still that of A x = b, so the stopping test and
  * the returned density keep their meaning and only the iteration count
- * changes.  16 and 16 are the defaults of erosion_get_preconditioner before
- * any call (eroded, non-circular bodies need the higher body modes).
+ * changes.  32 body modes and 16 wall modes are the defaults of
+ * erosion_get_preconditioner before any call (eroding bodies move their
+ * density into higher modes; 16 body modes cost fewer operations per
+ * iteration and suit round bodies).
  * Returns 0, or -1 on invalid arguments. */
 int erosion_set_preconditioner(int kind, int body_modes, int wall_modes);
 /* out[0] kind, out[1] body modes, out[2] wall modes.  Returns 0 or -1. */
