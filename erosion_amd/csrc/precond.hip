@@ -521,9 +521,21 @@ __device__ __forceinline__ void pc_coarse_solve(ApplyLds& L, const double* __res
         if (g < groups) {
             const int k0 = g * span, k1 = min(nq, k0 + span);
             double d = 0.0;
-#pragma unroll 8
-            for (int k = k0; k < k1; ++k)
-                d = __builtin_fma(Gt[static_cast<size_t>(k) * nq + j], L.cc[k], d);
+            // batches of kBatch loads issued together (indices clamped into
+            // the slice, coefficients past it read as 0), so the chain waits
+            // on L2 once per batch; the sum keeps its order
+            constexpr int kBatch = 24;
+            for (int kb = k0; kb < k1; kb += kBatch) {
+                double gv[kBatch], cv[kBatch];
+#pragma unroll
+                for (int kk = 0; kk < kBatch; ++kk) {
+                    const int k = min(kb + kk, k1 - 1);
+                    gv[kk] = Gt[static_cast<size_t>(k) * nq + j];
+                    cv[kk] = kb + kk < k1 ? L.cc[k] : 0.0;
+                }
+#pragma unroll
+                for (int kk = 0; kk < kBatch; ++kk) d = __builtin_fma(gv[kk], cv[kk], d);
+            }
             L.part[0][g * nq + j] = d;
         }
     }
