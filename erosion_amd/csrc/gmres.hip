@@ -189,8 +189,21 @@ __global__ __launch_bounds__(kBlk) void k_update(const double* __restrict__ V, s
             if (mode == 0) {
                 v = h[r];
             } else {
+                // the chunk partials in order, loaded 16 at a time (indices
+                // clamped, values past nchunk replaced by 0) so the sum
+                // waits on L2 once per 16 rather than once per partial
                 v = 0.0;
-                for (int cc = 0; cc < nchunk; ++cc) v += hpart[static_cast<size_t>(r) * nchunk + cc];
+                const double* hp = hpart + static_cast<size_t>(r) * nchunk;
+                for (int c0 = 0; c0 < nchunk; c0 += 16) {
+                    double t[16];
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+                        const double x = hp[min(c0 + q, nchunk - 1)];
+                        t[q] = c0 + q < nchunk ? x : 0.0;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) v += t[q];
+                }
                 if (blockIdx.x == 0) {
                     hcur[r] = v;
                     hsum[r] = mode == 1 ? v : hsum[r] + v;
@@ -240,11 +253,21 @@ __global__ __launch_bounds__(kBlk) void k_combine(double* __restrict__ w, size_t
     double acc = 0.0;
     if (e < npad) {
         double2 wv = *reinterpret_cast<const double2*>(w + e);
+        // the split partials in order, loaded 8 at a time (as in k_update)
         double sx = 0.0, sy = 0.0;
-        for (int s = 0; s < nsplit; ++s) {
-            const double2 p = *reinterpret_cast<const double2*>(wpart + static_cast<size_t>(s) * npad + e);
-            sx += p.x;
-            sy += p.y;
+        for (int s0 = 0; s0 < nsplit; s0 += 8) {
+            double2 t[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const double2 p = *reinterpret_cast<const double2*>(
+                    wpart + static_cast<size_t>(min(s0 + q, nsplit - 1)) * npad + e);
+                t[q] = s0 + q < nsplit ? p : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                sx += t[q].x;
+                sy += t[q].y;
+            }
         }
         wv.x = __builtin_fma(sign, sx, wv.x);
         wv.y = __builtin_fma(sign, sy, wv.y);
