@@ -47,14 +47,17 @@ constexpr int kMaxCycles = 4;
 inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
 inline size_t round_up(size_t a, size_t b) { return (a + b - 1) / b * b; }
 
+// Sum over the block (blockDim.x a multiple of 64), returned to every
+// thread: a butterfly within each wave, then the waves' sums through LDS in
+// wave order.  Two barriers; sh holds one double per wave and may be reused
+// after the call.
 __device__ __forceinline__ double block_sum(double v, double* sh) {
-    sh[threadIdx.x] = v;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    const int nw = static_cast<int>(blockDim.x) >> 6;
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-        if (static_cast<int>(threadIdx.x) < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    double r = sh[0];
+    double r = 0.0;
+    for (int w = 0; w < nw; ++w) r += sh[w];
     __syncthreads();
     return r;
 }
