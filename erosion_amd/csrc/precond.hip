@@ -21,7 +21,8 @@
 // self-interaction, O(n^2 K) per body), k_pc_coarse (coarse matrix and its
 // Gauss-Jordan inverse in LDS, one block per curve).  The wall block depends
 // only on the number of wall points and is kept between solves.  Apply:
-// k_pc_apply, one block per curve.
+// k_pc_apply (one block per body, the wall's projection in 256-point
+// segments) and k_pc_wall (the wall's coarse solve and expansion).
 #include <algorithm>
 #include <cmath>
 
