@@ -168,6 +168,14 @@ def test_gpu_preconditioned_solve_full_size(pc_reset):
 
 
 @gpu
+def test_gpu_preconditioner_default_modes(pc_reset):
+    """32 body modes and 16 wall modes unless the caller asks for others."""
+    stokes = pc_reset
+    stokes.set_preconditioner(stokes.PRECOND_CURVE_FOURIER)
+    assert stokes.get_preconditioner() == (1, 32, 16)
+
+
+@gpu
 def test_gpu_preconditioner_mode_limits(pc_reset):
     from erosion_amd.geometry import make_bodies
     from erosion_amd.theta_len import getnxy, new_thlenden
