@@ -10,6 +10,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -200,10 +201,14 @@ struct GmresState {
 struct GraphCache {
     std::vector<long long> sig;
     std::map<std::array<long long, 4>, hipGraphExec_t> exec;  // (k0, chunk, fp32, maxl)
+    // chunks run once directly under the current signature: a chunk is
+    // captured the second time it comes, so a one-off solve pays no capture
+    std::set<std::array<long long, 4>> seen;
     long long captures = 0, replays = 0;
     void clear() {
         for (auto& kv : exec) (void)hipGraphExecDestroy(kv.second);
         exec.clear();
+        seen.clear();
     }
     GraphCache() = default;
     GraphCache(const GraphCache&) = delete;

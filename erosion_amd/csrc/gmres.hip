@@ -470,7 +470,8 @@ static std::vector<long long> chunk_signature(const Context& c, int ibary, bool 
 }
 
 // Launch one chunk of iterations: replay its graph if one was captured under
-// the same signature, else capture the launches `run` makes, then replay.
+// the same signature; else, the first time the chunk comes, launch `run`
+// directly, and the second time capture its launches, then replay.
 template <typename Run>
 static void replay_chunk(Context& c, const std::array<long long, 4>& key,
                          const std::vector<long long>& sig, Run&& run) {
@@ -480,6 +481,10 @@ static void replay_chunk(Context& c, const std::array<long long, 4>& key,
         gc.sig = sig;
     }
     auto it = gc.exec.find(key);
+    if (it == gc.exec.end() && gc.seen.insert(key).second) {
+        run();
+        return;
+    }
     if (it == gc.exec.end()) {
         hipGraph_t graph = nullptr;
         STOKES_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));

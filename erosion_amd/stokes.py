@@ -256,8 +256,9 @@ def farfield_shape_count() -> int:
 
 
 def set_graphs(on: bool) -> None:
-    """Capture GMRES iteration chunks as HIP graphs and replay them on later
-    solves (results identical to direct launches)."""
+    """Capture GMRES iteration chunks as HIP graphs (the second time a chunk
+    comes under the same sizes and buffers) and replay them on later solves
+    (results identical to direct launches)."""
     if lib().erosion_set_graphs(1 if on else 0) != 0:
         raise StokesError(f"erosion_set_graphs: {last_error()}")
 

@@ -246,11 +246,12 @@ int erosion_set_farfield_shape(int index);
 int erosion_farfield_shape_count(void);
 
 /* HIP graphs for GMRES (no counterpart in the reference).  on = 1: stokessolver_
- * captures each chunk of iterations (8 or 16 iterations of preconditioner,
- * operator, two Gram-Schmidt passes, Givens update) into a HIP graph the first
- * time that chunk runs, and replays it on later solves while every size,
- * launch shape and device buffer it holds is unchanged; any change drops the
- * kept graphs.  The results are those of on = 0, bit for bit.  Kernel timing
+ * runs each chunk of iterations (8 or 16 iterations of preconditioner,
+ * operator, two Gram-Schmidt passes, Givens update) directly the first time
+ * it comes, captures it into a HIP graph the second time, and replays that on
+ * later solves while every size, launch shape and device buffer it holds is
+ * unchanged; any change drops the kept graphs (so a one-off solve captures
+ * nothing).  The results are those of on = 0, bit for bit.  Kernel timing
  * (erosion_kernel_timing) bypasses the graphs.  on = 0 (default) launches
  * every kernel directly and frees the kept graphs.  Returns 0, or -1 for
  * another value. */

@@ -1,8 +1,10 @@
 """HIP-graph replay of GMRES iteration chunks (erosion_set_graphs).
 
-A replayed chunk must give the densities of direct launches bit for bit: on
-the solve that captures the graphs, on later solves that only replay them, and
-after other work (another geometry, with its own graphs) has run in between.
+A chunk runs directly the first time it comes under a signature, is captured
+the second time and replayed after that.  A replayed chunk must give the
+densities of direct launches bit for bit: on the solve that captures the
+graphs, on later solves that only replay them, and after other work (another
+geometry, with its own graphs) has run in between.
 """
 import os
 import subprocess
@@ -50,21 +52,26 @@ def test_graph_replay_matches_direct(accel):
 
     stokes.set_graphs(True)
     s0 = stokes.graph_stats()
-    d1, it1 = stokes.stokessolver(n, m, no, 1, 1, 0, 3000, xx, yy)  # captures
+    d1, it1 = stokes.stokessolver(n, m, no, 1, 1, 0, 3000, xx, yy)  # first sight: direct
     s1 = stokes.graph_stats()
-    assert s1["captures"] > s0["captures"] and s1["replays"] > s0["replays"]
-    d2, it2 = stokes.stokessolver(n, m, no, 1, 1, 0, 3000, xx, yy)  # replays only
+    assert s1["captures"] == s0["captures"] and s1["replays"] == s0["replays"]
+    d2, it2 = stokes.stokessolver(n, m, no, 1, 1, 0, 3000, xx, yy)  # captures
     s2 = stokes.graph_stats()
-    assert s2["captures"] == s1["captures"] and s2["replays"] > s1["replays"]
+    assert s2["captures"] > s1["captures"] and s2["replays"] > s1["replays"]
+    d3, it3 = stokes.stokessolver(n, m, no, 1, 1, 0, 3000, xx, yy)  # replays only
+    s3 = stokes.graph_stats()
+    assert s3["captures"] == s2["captures"] and s3["replays"] > s2["replays"]
     # other work in between: another geometry (its own graphs), a stress call
     e1, jt1 = stokes.stokessolver(tn, tm, tno, 1, 1, 0, 3000, txx, tyy)
-    stokes.computeshearstress(tn, tm, tno, txx, tyy, e1, 1)
-    d3, it3 = stokes.stokessolver(n, m, no, 1, 1, 0, 3000, xx, yy)
+    e2, jt2 = stokes.stokessolver(tn, tm, tno, 1, 1, 0, 3000, txx, tyy)
+    stokes.computeshearstress(tn, tm, tno, txx, tyy, e2, 1)
     d4, it4 = stokes.stokessolver(n, m, no, 1, 1, 0, 3000, xx, yy)
-    for d, it in ((d1, it1), (d2, it2), (d3, it3), (d4, it4)):
+    d5, it5 = stokes.stokessolver(n, m, no, 1, 1, 0, 3000, xx, yy)
+    for d, it in ((d1, it1), (d2, it2), (d3, it3), (d4, it4), (d5, it5)):
         assert it == it0
         assert np.array_equal(d, d0)
     assert jt1 == jt0 and np.array_equal(e1, e0)
+    assert jt2 == jt0 and np.array_equal(e2, e0)
     assert stokes.last_solve_info()["true_relres"] < (1e-10 if accel else 1e-9)
 
 
@@ -72,7 +79,8 @@ def test_graphs_off_frees_and_direct_path_unchanged():
     _configure(True)
     n, m, no, xx, yy = workload_xy(WORKLOADS["bodies100_n256"], nbodies=10)
     stokes.set_graphs(True)
-    d1, _ = stokes.stokessolver(n, m, no, 1, 1, 0, 3000, xx, yy)
+    stokes.stokessolver(n, m, no, 1, 1, 0, 3000, xx, yy)  # first sight: direct
+    d1, _ = stokes.stokessolver(n, m, no, 1, 1, 0, 3000, xx, yy)  # captures
     assert stokes.graph_stats()["kept"] > 0
     stokes.set_graphs(False)
     assert stokes.graph_stats()["kept"] == 0
@@ -113,10 +121,12 @@ def test_graphs_on_from_the_first_call_of_a_process(tmp_path):
         stokes.set_preconditioner(stokes.PRECOND_CURVE_FOURIER)
         stokes.set_precision(stokes.PRECISION_MIXED)
         n, m, no, xx, yy = workload_xy(WORKLOADS["bodies100_n256"], nbodies=10)
-        print("first solve with graphs", flush=True)
-        d, it = stokes.stokessolver(n, m, no, 1, 1, 0, 3000, xx, yy)
-        print("batch with graphs on fresh workers", flush=True)
-        b = stokes.solve_batch([(n, m, no, xx, yy)] * 2, workers=2)
+        print("first solves with graphs (direct, then captured)", flush=True)
+        for _ in range(2):
+            d, it = stokes.stokessolver(n, m, no, 1, 1, 0, 3000, xx, yy)
+        print("batches with graphs on fresh workers", flush=True)
+        for _ in range(2):
+            b = stokes.solve_batch([(n, m, no, xx, yy)] * 2, workers=2)
         np.savez({str(out)!r}, d=d, it=it, b0=b[0]["den"], b1=b[1]["den"],
                  captures=stokes.graph_stats()["captures"])
     """)
