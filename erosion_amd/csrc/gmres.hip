@@ -303,13 +303,14 @@ __global__ __launch_bounds__(kGivT) void k_givens(double* __restrict__ H, int ld
     {
         double p = 0.0;
         for (int i = tid; i < nb; i += kGivT) p += npart[i];
-        sA[tid] = p;
+        for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off);
+        if ((tid & 63) == 0) sA[tid >> 6] = p;
         __syncthreads();
-        for (int off = kGivT / 2; off > 0; off >>= 1) {
-            if (tid < off) sA[tid] += sA[tid + off];
-            __syncthreads();
+        if (tid == 0) {
+            double t = 0.0;
+            for (int w = 0; w < kGivT / 64; ++w) t += sA[w];
+            sB[0] = sqrt(t);
         }
-        if (tid == 0) sB[0] = sqrt(sA[0]);
         __syncthreads();
     }
     const double snormw = sB[0];

@@ -728,7 +728,7 @@ __global__ __launch_bounds__(kBlk) void k_reduce_small(
     const double* __restrict__ py, const double* __restrict__ speed,
     const double* __restrict__ ex, const double* __restrict__ ey, const double* __restrict__ den,
     int no, int n, int m, double* __restrict__ scal, double* __restrict__ out) {
-    __shared__ double s[3][kBlk];
+    __shared__ double s[3][kBlk / 64];
     const int b = blockIdx.x;
     double a0 = 0, a1 = 0, a2 = 0;
     if (b == 0) {
@@ -743,19 +743,25 @@ __global__ __launch_bounds__(kBlk) void k_reduce_small(
             a2 -= (ex[p] * y[p] - ey[p] * x[p]) * speed[p];
         }
     }
-    s[0][threadIdx.x] = a0;
-    s[1][threadIdx.x] = a1;
-    s[2][threadIdx.x] = a2;
-    __syncthreads();
-    for (int w = kBlk / 2; w > 0; w >>= 1) {
-        if (static_cast<int>(threadIdx.x) < w) {
-            s[0][threadIdx.x] += s[0][threadIdx.x + w];
-            s[1][threadIdx.x] += s[1][threadIdx.x + w];
-            s[2][threadIdx.x] += s[2][threadIdx.x + w];
-        }
-        __syncthreads();
+    // butterflies within each wave, then the waves' sums in order (one barrier)
+    for (int off = 32; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off);
+        a1 += __shfl_xor(a1, off);
+        a2 += __shfl_xor(a2, off);
     }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        s[0][wave] = a0;
+        s[1][wave] = a1;
+        s[2][wave] = a2;
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
+    for (int w = 1; w < kBlk / 64; ++w) {
+        s[0][0] += s[0][w];
+        s[1][0] += s[1][w];
+        s[2][0] += s[2][w];
+    }
     if (b == 0) {
         scal[0] = s[0][0] * kTwoPi / static_cast<double>(no);
     } else {
