@@ -375,10 +375,14 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
         for (int col = tid; col < nq; col += kCoarseBlk)
             G[k * nq + col] = (col == k ? 1.0 : G[k * nq + col]) / piv;
         __syncthreads();
-        for (int e = tid; e < nq * nq; e += kCoarseBlk) {
-            const int r = e / nq, col = e - r * nq;
+        // thread (tid / 32, tid % 32) walks rows and columns in steps of
+        // 32: no division per entry (a runtime divide costs more than the
+        // update it indexes)
+        for (int r = tid >> 5; r < nq; r += kCoarseBlk >> 5) {
             if (r == k) continue;
-            G[e] = (col == k ? 0.0 : G[e]) - fcol[r] * G[k * nq + col];
+            const double f = fcol[r];
+            for (int col = tid & 31; col < nq; col += 32)
+                G[r * nq + col] = (col == k ? 0.0 : G[r * nq + col]) - f * G[k * nq + col];
         }
         __syncthreads();
     }
