@@ -97,7 +97,8 @@ __device__ __forceinline__ CurveView curve_view(int c, int no, int n, int nfw, i
 // S[(t nf + f) len + i] = sum_{j != i} w_ij r_a r_b phi_f(j) over the curve's
 // own points, t = 0 (xx), 1 (xy), 2 (yy), w_ij = (r.n_j) speed_j (2/len) / rho^4:
 // the trapezoid self-interaction of the curve applied to each basis function.
-// grid (curves, ceil(len / 256)); block 256, one target per lane.
+// grid (curves, ceil(len / 256), ceil(nf / kQC)); block 256, one target per
+// lane, kQC basis functions per block.
 __global__ __launch_bounds__(kBlk) void k_pc_pairs(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ mx,
     const double* __restrict__ my, const double* __restrict__ phi_w,
@@ -115,7 +116,10 @@ __global__ __launch_bounds__(kBlk) void k_pc_pairs(
     const double xi = valid ? x[cv.beg + i] : 1.0e3;
     const double yi = valid ? y[cv.beg + i] : 1.0e3;
     double* Sc = S + cv.soff;
-    for (int q0 = 0; q0 < cv.nf; q0 += kQC) {
+    // blockIdx.z picks the block of kQC basis functions
+    const int q0 = blockIdx.z * kQC;
+    if (q0 >= cv.nf) return;  // uniform per block
+    {
         double axx[kQC], axy[kQC], ayy[kQC];
 #pragma unroll
         for (int q = 0; q < kQC; ++q) axx[q] = axy[q] = ayy[q] = 0.0;
@@ -694,7 +698,7 @@ void precond_setup(Context& c) {
     // the wall is fixed by its point count, so its block is kept between solves
     if (c.pc_wall_no != no || c.pc_wall_kw != kw) {
         c.pc_Gw.ensure(static_cast<size_t>(nqw) * nqw);
-        k_pc_pairs<<<dim3(1, cdiv_int(no, kBlk)), kBlk, 0, s>>>(
+        k_pc_pairs<<<dim3(1, cdiv_int(no, kBlk), cdiv_int(nfw, kQC)), kBlk, 0, s>>>(
             g.x.p, g.y.p, g.mx.p, g.my.p, c.pc_phi_w.p, c.pc_phi_b.p, nfw, nfb, no, n, 0,
             c.pc_S.p);
         STOKES_LAUNCH_CHECK();
@@ -704,7 +708,7 @@ void precond_setup(Context& c) {
     }
     if (m > 0) {
         c.pc_Gb.ensure(static_cast<size_t>(m) * nqb * nqb);
-        k_pc_pairs<<<dim3(m, cdiv_int(n, kBlk)), kBlk, 0, s>>>(
+        k_pc_pairs<<<dim3(m, cdiv_int(n, kBlk), cdiv_int(nfb, kQC)), kBlk, 0, s>>>(
             g.x.p, g.y.p, g.mx.p, g.my.p, c.pc_phi_w.p, c.pc_phi_b.p, nfw, nfb, no, n, 1,
             c.pc_S.p);
         STOKES_LAUNCH_CHECK();
