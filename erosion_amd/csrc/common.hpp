@@ -119,6 +119,10 @@ struct Geometry {
     std::vector<double> h_x, h_y, h_speed, h_cx, h_cy;
     // build inputs and scratch
     DevBuf<double> in_x, in_y, t1x, t1y, t2x, t2y;
+    // first and second derivatives of the wall (it depends on no only), kept
+    // between builds: [t1x | t1y | t2x | t2y], each of wall_no doubles
+    DevBuf<double> wall_d;
+    int wall_no = -1;
     // the boundary this geometry was built from; version counts builds
     std::vector<double> key_x, key_y;
     int key_n = -1, key_m = -1, key_no = -1;

@@ -42,8 +42,8 @@ __global__ void k_points(const double* __restrict__ xx, const double* __restrict
 // table dno), curve c>0 = body c-1 (length n, table dn).
 __global__ void k_deriv(const double* __restrict__ fx, const double* __restrict__ fy, int no,
                         int n, const double* __restrict__ dno, const double* __restrict__ dn,
-                        double* __restrict__ ox, double* __restrict__ oy) {
-    const int c = blockIdx.y;
+                        int first_curve, double* __restrict__ ox, double* __restrict__ oy) {
+    const int c = first_curve + static_cast<int>(blockIdx.y);
     const int len = c == 0 ? no : n;
     const int off = c == 0 ? 0 : no + (c - 1) * n;
     const double* __restrict__ d = c == 0 ? dno : dn;
@@ -211,11 +211,33 @@ void build_geometry(Context& c, int n, int m, int no, const double* xx, const do
 
     k_points<<<cdiv_int(g.npts, kBlk), kBlk, 0, s>>>(dxx.p, dyy.p, no, n * m, g.x.p, g.y.p);
     STOKES_LAUNCH_CHECK();
-    dim3 dgrid(cdiv_int(std::max(n, no), kBlk), m + 1);
-    k_deriv<<<dgrid, kBlk, 0, s>>>(g.x.p, g.y.p, no, n, g.dtab_no.p, g.dtab_n.p, t1x.p, t1y.p);
-    STOKES_LAUNCH_CHECK();
-    k_deriv<<<dgrid, kBlk, 0, s>>>(t1x.p, t1y.p, no, n, g.dtab_no.p, g.dtab_n.p, t2x.p, t2y.p);
-    STOKES_LAUNCH_CHECK();
+    // the wall's derivatives are kept from the last build with the same no;
+    // only the bodies' are recomputed (the wall's O(no^2) sums were the
+    // longest blocks of the launch)
+    const bool wall_kept = g.wall_no == no;
+    const int c0 = wall_kept ? 1 : 0;
+    if (m + 1 - c0 > 0) {
+        dim3 dgrid(cdiv_int(wall_kept ? n : std::max(n, no), kBlk), m + 1 - c0);
+        k_deriv<<<dgrid, kBlk, 0, s>>>(g.x.p, g.y.p, no, n, g.dtab_no.p, g.dtab_n.p, c0, t1x.p,
+                                       t1y.p);
+        STOKES_LAUNCH_CHECK();
+        k_deriv<<<dgrid, kBlk, 0, s>>>(t1x.p, t1y.p, no, n, g.dtab_no.p, g.dtab_n.p, c0, t2x.p,
+                                       t2y.p);
+        STOKES_LAUNCH_CHECK();
+    }
+    double* const wd[4] = {t1x.p, t1y.p, t2x.p, t2y.p};
+    const size_t wb = static_cast<size_t>(no) * sizeof(double);
+    if (wall_kept) {
+        for (int q = 0; q < 4; ++q)
+            STOKES_HIP(hipMemcpyAsync(wd[q], g.wall_d.p + static_cast<size_t>(q) * no, wb,
+                                      hipMemcpyDeviceToDevice, s));
+    } else {
+        g.wall_d.ensure(4 * static_cast<size_t>(no));
+        for (int q = 0; q < 4; ++q)
+            STOKES_HIP(hipMemcpyAsync(g.wall_d.p + static_cast<size_t>(q) * no, wd[q], wb,
+                                      hipMemcpyDeviceToDevice, s));
+        g.wall_no = no;
+    }
     k_frame<<<cdiv_int(g.npts, kBlk), kBlk, 0, s>>>(t1x.p, t1y.p, t2x.p, t2y.p, no, g.npts, n,
                                                     g.px.p, g.py.p, g.cur.p, g.speed.p, g.dzr.p,
                                                     g.dzi.p, g.mx.p, g.my.p);
