@@ -19,8 +19,8 @@ per-step solve error is measured on identical inputs:
 * per-body area after the step: max_b |A_mixed - A_fp64| / A_fp64 between the
   two trajectories (the erosion model's own output);
 * the FP64 answer's own spread: a second FP64 solve of the same state along
-  another Krylov path (body preconditioner with 8 modes instead of 16),
-  compared with the first in density and shear stress.
+  another Krylov path (a body preconditioner with half the modes), compared
+  with the first in density and shear stress.
 
 The shear stress is that of the solver's density (computeshearstress_), before
 the fixed-pressure-drop scaling the time stepper applies.
@@ -50,12 +50,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def _configure(mixed: bool, body_modes: int = 16) -> None:
+def _configure(mixed: bool, body_modes: int = 32) -> None:
     stokes.set_preconditioner(stokes.PRECOND_CURVE_FOURIER, body_modes)
     stokes.set_precision(stokes.PRECISION_MIXED if mixed else stokes.PRECISION_FP64)
 
 
-def _solve(params: ParamSet, thld, mixed: bool, den0=None, body_modes: int = 16):
+def _solve(params: ParamSet, thld, mixed: bool, den0=None, body_modes: int = 32):
     """Density, shear stress, iterations, solver info and seconds for one state."""
     _configure(mixed, body_modes)
     m, xx, yy = getnxy(thld)
@@ -96,7 +96,7 @@ def run(workload: str, steps: int, tol: float, switch: float, save: str = ""):
             # a second FP64 solve along another Krylov path (coarser body
             # preconditioner): how far two FP64 solutions that both meet the
             # stopping test lie apart -- the FP64 answer's own spread
-            d2, t2, n2, *_ = _solve(params, probe, mixed=False, body_modes=8)
+            d2, t2, n2, *_ = _solve(params, probe, mixed=False, body_modes=16)
             if save and k == 0:
                 m0, xx0, yy0 = getnxy(probe)
                 np.savez(save, xx=xx0, yy=yy0, den_fp64=d64, den_mixed=d32, den_fp64_b=d2,
