@@ -420,10 +420,10 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
 // generating the basis functions by the rotation
 // e^{i(k+1)theta} = e^{ik theta} e^{i theta} (no table loads), and keeps the
 // chunk's coefficient partials in registers; the partials meet in LDS and are
-// summed in two conflict-free stages, kVals at a time.  The inverse coarse
+// summed in two conflict-free stages, all of a chunk's at once.  The inverse coarse
 // matrices are stored transposed, so d = G^{-1} c reads them coalesced with
 // one row per lane.
-constexpr int kVals = 24;
+constexpr int kVals = 2 * kChunkNF;  // every coefficient of a chunk in one LDS round (~146 KB)
 constexpr int kSeg = 16;
 
 struct ApplyLds {
