@@ -319,10 +319,12 @@ void build_geometry(Context& c, int n, int m, int no, const double* xx, const do
 void build_near_list(Context& c, int ibary);
 
 // operator.hip : y = A x for the completed double-layer operator
-// gathered: c.ex/c.ey already hold x_dev's density in point order (the
-// preconditioner's apply writes them beside its output).
+// from_precond: x_dev is the output of precond_apply_scaled(..., y_dev), which
+// has left x_dev's density in point order in c.ex/c.ey, the body constraint
+// rows in y_dev and the wall segments' parts of the rank-one integral in
+// c.scal[0 .. precond_wall_segments).
 void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary,
-                    bool gathered = false);
+                    bool from_precond = false);
 // run the geometry-dependent set-up apply_operator would otherwise do inside
 // its first call (far-field tuning, FP32 source tiles, stokeslet logarithms),
 // so a captured sequence of operator applications holds none of it; x_dev is
@@ -347,7 +349,13 @@ void precond_apply(Context& c, const double* r_dev, double* z_dev);
 // The same with r_dev first scaled in place by st->inv_snormw unless st is
 // null or has stopped (GMRES's normalisation of its new Krylov vector).  Both
 // also leave z's density in point order in c.ex/c.ey (the operator's gather).
-void precond_apply_scaled(Context& c, double* r_dev, double* z_dev, const GmresState* st);
+// With y_dev, the apply also does the operator's small reductions of z: the
+// body constraint rows of y_dev = A z and the wall segments' parts of the
+// rank-one integral (apply_operator(c, z_dev, y_dev, ibary, true) follows).
+void precond_apply_scaled(Context& c, double* r_dev, double* z_dev, const GmresState* st,
+                          double* y_dev = nullptr);
+// blocks over the wall in the apply: ceil(no / 256)
+int precond_wall_segments(const Geometry& g);
 
 // batch.hip: independent geometries solved by a pool of worker contexts
 struct BatchGeometry {

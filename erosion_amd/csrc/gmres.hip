@@ -626,7 +626,7 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
                     if (pc) {
                         // v_k = w_k / ||w_k|| happens in the preconditioner's
                         // first read of it (v_0 is scaled before the loop)
-                        precond_apply_scaled(c, vk, c.pc_z.p, kk > 0 ? c.st.p : nullptr);
+                        precond_apply_scaled(c, vk, c.pc_z.p, kk > 0 ? c.st.p : nullptr, w);
                         apply_operator(c, c.pc_z.p, w, ibary, true);
                     } else {
                         apply_operator(c, vk, w, ibary);

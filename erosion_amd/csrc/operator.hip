@@ -20,7 +20,8 @@
 //                  source curves, node-sliced across waves      (VALU bound)
 //   k_near_wave    barycentric value/derivative at near targets minus the
 //                  trapezoid part, one wave per (target, source curve) entry
-//   k_reduce_small rank-one integral and constraint rows
+//   k_reduce_small rank-one integral and constraint rows (skipped when the
+//                  preconditioner's apply has formed them)
 //   k_assemble     partial sums + near corrections + local terms + stokeslets
 #include <algorithm>
 
@@ -721,7 +722,8 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_wave(
 }
 
 // ------------------------------------------------- small reductions
-// block 0: rank-one integral sigdotn (stokesSolver.f:1107-1111);
+// block 0: rank-one integral sigdotn (stokesSolver.f:1107-1111), unscaled,
+// in scal[0] (k_assemble applies 2 pi / no);
 // block 1+b: constraint rows of body b (stokesSolver.f:1055-1096).
 __global__ __launch_bounds__(kBlk) void k_reduce_small(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ px,
@@ -763,7 +765,7 @@ __global__ __launch_bounds__(kBlk) void k_reduce_small(
         s[2][0] += s[2][w];
     }
     if (b == 0) {
-        scal[0] = s[0][0] * kTwoPi / static_cast<double>(no);
+        scal[0] = s[0][0];
     } else {
         const int base = 2 * no + 2 * n * m;
         const int r = base + 3 * (b - 1);
@@ -802,7 +804,7 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
     const double* __restrict__ den, const double* __restrict__ part_u,
     const double* __restrict__ part_v, int nsplit, const int* __restrict__ tgt_ptr,
     const int* __restrict__ tgt_ent, const double* __restrict__ corr_u,
-    const double* __restrict__ corr_v, int has_near, const double* __restrict__ scal,
+    const double* __restrict__ corr_v, int has_near, const double* __restrict__ scal, int nsig,
     const double* __restrict__ lgtab, int no, int n, int m, int npts, double* __restrict__ out) {
     __shared__ double sc[5][kBlk];
     const int sub = threadIdx.x & (kAsmSub - 1);
@@ -870,7 +872,10 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
     u += us;
     v += vs;
     if (i < no) {
-        const double sig = scal[0];
+        // the rank-one integral from its nsig unscaled parts, in order
+        double sig = 0.0;
+        for (int j = 0; j < nsig; ++j) sig += scal[j];
+        sig = sig * kTwoPi / static_cast<double>(no);
         u += sig * px[i];
         v += sig * py[i];
     }
@@ -1016,7 +1021,7 @@ static void stokes_logs(Context& c) {
 }
 
 void prepare_operator(Context& c, const double* x_dev, bool fp32) {
-    c.scal.ensure(8);
+    c.scal.ensure(std::max<size_t>(8, static_cast<size_t>(precond_wall_segments(c.g))));
     // the tuning launches read the density in point order: gather (and on
     // the first call allocate) it before timing any far-field shape
     gather_density(c, x_dev);
@@ -1036,15 +1041,15 @@ int farfield_choice(const Context& c, bool fp32) {
     return it == choice.end() ? -1 : it->second;
 }
 
-void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, bool gathered) {
+void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, bool from_precond) {
     Geometry& g = c.g;
     NearList& L = c.near;
     hipStream_t s = c.stream;
     const int np = g.npts;
     const size_t N = static_cast<size_t>(np);
-    c.scal.ensure(8);
+    c.scal.ensure(std::max<size_t>(8, static_cast<size_t>(precond_wall_segments(g))));
 
-    if (!gathered) gather_density(c, x_dev);
+    if (!from_precond) gather_density(c, x_dev);
     const bool fp32 = c.ff_fp32_now;
     if (fp32) ff32_sources(c);
     const FFPlan plan = ff_plan(np, kFFShapes[tune_farfield(c, fp32)], fp32);
@@ -1073,14 +1078,20 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, b
         STOKES_LAUNCH_CHECK();
     }
     if (has_near && c.time_kernels) c.t_near.stop(s, L.gather_bytes);
-    k_reduce_small<<<g.m + 1, kBlk, 0, s>>>(g.x.p, g.y.p, g.px.p, g.py.p, g.speed.p, c.ex.p,
-                                            c.ey.p, x_dev, g.no, g.n, g.m, c.scal.p, y_dev);
-    STOKES_LAUNCH_CHECK();
+    // after the preconditioner's apply, the body constraint rows of y and the
+    // wall segments' parts of the rank-one integral are already formed
+    if (!from_precond) {
+        k_reduce_small<<<g.m + 1, kBlk, 0, s>>>(g.x.p, g.y.p, g.px.p, g.py.p, g.speed.p, c.ex.p,
+                                                c.ey.p, x_dev, g.no, g.n, g.m, c.scal.p, y_dev);
+        STOKES_LAUNCH_CHECK();
+    }
+    const int nsig = from_precond ? precond_wall_segments(g) : 1;
     stokes_logs(c);
     k_assemble<<<cdiv_int(np, kBlk / kAsmSub), kBlk, 0, s>>>(
         g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p, c.ex.p, c.ey.p, g.cx.p, g.cy.p, x_dev,
         c.part_u.p, c.part_v.p, nsplit, L.tgt_ptr.p, L.tgt_ent.p, L.corr_u.p, L.corr_v.p,
-        has_near ? 1 : 0, c.scal.p, g.m > 0 ? c.stk_log.p : nullptr, g.no, g.n, g.m, np, y_dev);
+        has_near ? 1 : 0, c.scal.p, nsig, g.m > 0 ? c.stk_log.p : nullptr, g.no, g.n, g.m, np,
+        y_dev);
     STOKES_LAUNCH_CHECK();
 }
 

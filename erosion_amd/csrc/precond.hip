@@ -426,6 +426,11 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
 constexpr int kVals = 2 * kChunkNF;  // every coefficient of a chunk in one LDS round (~146 KB)
 constexpr int kSeg = 16;
 
+// geometry arrays the apply's reductions read (point order)
+struct SumGeom {
+    const double *x, *y, *px, *py, *speed;
+};
+
 struct ApplyLds {
     double part[kVals][kApplyBlk];
     double red[kVals][kSeg];
@@ -562,10 +567,16 @@ __device__ __forceinline__ void pc_coarse_solve(ApplyLds& L, const double* __res
 
 // z = Q dd - 2 r at the curve's points [i0, i1); the same values also go to
 // ex/ey at point index poff + i (the density in point order).
+// SUMS 1 (wall): a0 += (px ex + py ey) speed, the rank-one integral's
+// integrand; SUMS 2 (body): a0, a1, a2 -= (ex, ey, ex y - ey x) speed, the
+// constraint rows' integrands (the terms k_reduce_small sums, same order per
+// lane).  Point p = poff + i of the geometry arrays.
+template <int SUMS>
 __device__ __forceinline__ void pc_expand(const ApplyLds& L, int len, int nf, int i0, int i1,
                                           const double* __restrict__ r, int xoff, int yoff,
                                           double* __restrict__ z, int poff,
-                                          double* __restrict__ ex, double* __restrict__ ey) {
+                                          double* __restrict__ ex, double* __restrict__ ey,
+                                          const SumGeom& sg, double& a0, double& a1, double& a2) {
     const int K = (nf - 1) / 2;
     const double c0 = 1.0 / sqrt(static_cast<double>(len));
     const double c1 = sqrt(2.0 / static_cast<double>(len));
@@ -588,6 +599,38 @@ __device__ __forceinline__ void pc_expand(const ApplyLds& L, int len, int nf, in
         z[yoff + i] = zy;
         ex[poff + i] = zx;
         ey[poff + i] = zy;
+        const int p = poff + i;
+        if (SUMS == 1) {
+            a0 += (sg.px[p] * zx + sg.py[p] * zy) * sg.speed[p];
+        } else if (SUMS == 2) {
+            a0 -= zx * sg.speed[p];
+            a1 -= zy * sg.speed[p];
+            a2 -= (zx * sg.y[p] - zy * sg.x[p]) * sg.speed[p];
+        }
+    }
+}
+
+// Block sums of a0, a1, a2 as k_reduce_small forms them (butterflies within
+// each wave, then the waves in order); the result is valid in thread 0.
+__device__ __forceinline__ void pc_block_sum3(ApplyLds& L, double& a0, double& a1, double& a2) {
+    for (int off = 32; off > 0; off >>= 1) {
+        a0 += __shfl_xor(a0, off);
+        a1 += __shfl_xor(a1, off);
+        a2 += __shfl_xor(a2, off);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        L.red[0][wave] = a0;
+        L.red[1][wave] = a1;
+        L.red[2][wave] = a2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kApplyBlk / 64; ++w) {
+            a0 += L.red[0][w];
+            a1 += L.red[1][w];
+            a2 += L.red[2][w];
+        }
     }
 }
 
@@ -595,10 +638,13 @@ __device__ __forceinline__ void pc_expand(const ApplyLds& L, int len, int nf, in
 // segment s's partial wall coefficients.  With sst (a GMRES state that has not
 // stopped), r is first scaled in place by sst->inv_snormw: the new Krylov
 // vector's normalisation, folded into its first reader.
+// With y (the operator's output A z), a body block also writes the body's
+// three constraint rows of y (as k_reduce_small does, stokesSolver.f:1055-1096).
 __global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
     int nfw, int nfb, int no, int n, int m, int nseg, const double* __restrict__ Gb,
     const GmresState* __restrict__ sst, double* __restrict__ r, double* __restrict__ wpart,
-    double* __restrict__ z, double* __restrict__ ex, double* __restrict__ ey) {
+    double* __restrict__ z, double* __restrict__ ex, double* __restrict__ ey, SumGeom sg,
+    double* __restrict__ y) {
     __shared__ ApplyLds L;
     const int tid = threadIdx.x;
     const bool scale = sst != nullptr && !sst->done;
@@ -625,17 +671,32 @@ __global__ __launch_bounds__(kApplyBlk) void k_pc_apply(
     }
     __syncthreads();
     pc_coarse_solve(L, Gb + static_cast<size_t>(c - 1) * nq * nq, nq, nc, z + coff);
-    pc_expand(L, n, nf, 0, n, r, xoff, yoff, z, no + (c - 1) * n, ex, ey);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    if (y == nullptr) {
+        pc_expand<0>(L, n, nf, 0, n, r, xoff, yoff, z, no + (c - 1) * n, ex, ey, sg, a0, a1, a2);
+        return;
+    }
+    pc_expand<2>(L, n, nf, 0, n, r, xoff, yoff, z, no + (c - 1) * n, ex, ey, sg, a0, a1, a2);
+    pc_block_sum3(L, a0, a1, a2);  // its barrier also orders z[coff..] before the reads below
+    if (tid == 0) {
+        const double rs = kTwoPi / static_cast<double>(n);
+        y[coff + 0] = a0 / kTwoPi * rs + z[coff + 0];
+        y[coff + 1] = a1 / kTwoPi * rs + z[coff + 1];
+        y[coff + 2] = a2 / kTwoPi * rs + z[coff + 2];
+    }
 }
 
-// The wall after k_pc_apply: grid nseg, block s expands segment s.
+// The wall after k_pc_apply: grid nseg, block s expands segment s.  With
+// scal, block s also writes scal[s], its segment's part of the unscaled
+// rank-one integral (stokesSolver.f:1107-1111; k_assemble adds the parts).
 __global__ __launch_bounds__(kApplyBlk) void k_pc_wall(int nfw, int no, int nseg,
                                                        const double* __restrict__ Gw,
                                                        const double* __restrict__ wpart,
                                                        const double* __restrict__ r,
                                                        double* __restrict__ z,
                                                        double* __restrict__ ex,
-                                                       double* __restrict__ ey) {
+                                                       double* __restrict__ ey, SumGeom sg,
+                                                       double* __restrict__ scal) {
     __shared__ ApplyLds L;
     const int tid = threadIdx.x;
     const int nc = 2 * nfw;
@@ -647,7 +708,14 @@ __global__ __launch_bounds__(kApplyBlk) void k_pc_wall(int nfw, int no, int nseg
     __syncthreads();
     pc_coarse_solve(L, Gw, nc, nc, nullptr);
     const int i0 = blockIdx.x * kApplyBlk, i1 = min(no, i0 + kApplyBlk);
-    pc_expand(L, no, nfw, i0, i1, r, 0, no, z, 0, ex, ey);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    if (scal == nullptr) {
+        pc_expand<0>(L, no, nfw, i0, i1, r, 0, no, z, 0, ex, ey, sg, a0, a1, a2);
+        return;
+    }
+    pc_expand<1>(L, no, nfw, i0, i1, r, 0, no, z, 0, ex, ey, sg, a0, a1, a2);
+    pc_block_sum3(L, a0, a1, a2);
+    if (tid == 0) scal[blockIdx.x] = a0;
 }
 
 }  // namespace
@@ -689,6 +757,7 @@ void precond_setup(Context& c) {
     // the wall's partial coefficients of the apply (allocated here, before
     // any GMRES chunk may be captured into a graph)
     c.pc_wpart.ensure(static_cast<size_t>(cdiv_int(no, kApplyBlk)) * kMaxNQ);
+    c.scal.ensure(std::max<size_t>(8, static_cast<size_t>(cdiv_int(no, kApplyBlk))));
     k_pc_basis<<<cdiv_int(no, kBlk), kBlk, 0, s>>>(no, kw, c.pc_phi_w.p);
     STOKES_LAUNCH_CHECK();
     if (m > 0) {
@@ -718,19 +787,24 @@ void precond_setup(Context& c) {
     c.pc_nfw = nfw;
 }
 
-void precond_apply_scaled(Context& c, double* r, double* z, const GmresState* st) {
+int precond_wall_segments(const Geometry& g) { return cdiv_int(g.no, kApplyBlk); }
+
+void precond_apply_scaled(Context& c, double* r, double* z, const GmresState* st, double* y) {
     Geometry& g = c.g;
     if (c.time_kernels) c.t_precond.start(c.stream);
     const int nseg = cdiv_int(g.no, kApplyBlk);
     c.pc_wpart.ensure(static_cast<size_t>(nseg) * kMaxNQ);
     c.ex.ensure(static_cast<size_t>(g.npts));
     c.ey.ensure(static_cast<size_t>(g.npts));
+    if (y != nullptr) c.scal.ensure(std::max<size_t>(8, static_cast<size_t>(nseg)));
+    const SumGeom sg{g.x.p, g.y.p, g.px.p, g.py.p, g.speed.p};
     k_pc_apply<<<nseg + g.m, kApplyBlk, 0, c.stream>>>(c.pc_nfw, c.pc_nfb, g.no, g.n, g.m, nseg,
                                                       c.pc_Gb.p, st, r, c.pc_wpart.p, z, c.ex.p,
-                                                      c.ey.p);
+                                                      c.ey.p, sg, y);
     STOKES_LAUNCH_CHECK();
     k_pc_wall<<<nseg, kApplyBlk, 0, c.stream>>>(c.pc_nfw, g.no, nseg, c.pc_Gw.p, c.pc_wpart.p, r,
-                                                z, c.ex.p, c.ey.p);
+                                                z, c.ex.p, c.ey.p, sg,
+                                                y != nullptr ? c.scal.p : nullptr);
     STOKES_LAUNCH_CHECK();
     // bytes: the vector read and written once
     if (c.time_kernels) c.t_precond.stop(c.stream, 2.0 * 8.0 * static_cast<double>(g.nunk));
