@@ -279,6 +279,9 @@ int erosion_last_solve_info(double* info) {
         info[6] = static_cast<double>(c.ff_splits);
         info[7] = static_cast<double>(c.pc_kind);
         info[8] = static_cast<double>(c.last_fp32_iters);
+        info[9] = static_cast<double>(c.last_tol_miss);
+        info[10] = static_cast<double>(c.tol_misses);
+        info[11] = static_cast<double>(c.last_cycles);
         return 0;
     } catch (const std::exception& e) {
         return record_error("erosion_last_solve_info", e.what());
@@ -394,6 +397,29 @@ int erosion_set_mixed_switch(double factor) {
         return 0;
     } catch (const std::exception& e) {
         return record_error("erosion_set_mixed_switch", e.what());
+    }
+}
+
+int erosion_set_guess_space(int space) {
+    try {
+        if (space != 0 && space != 1) return -1;
+        stokes::ctx().guess_space = space;
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_set_guess_space", e.what());
+    }
+}
+
+int erosion_last_preconditioned(double* u, int64_t n) {
+    try {
+        Context& c = stokes::ctx();
+        if (!u || !c.u_valid || n != c.g.nunk) return -1;
+        STOKES_HIP(hipMemcpyAsync(u, c.uvec.p, sizeof(double) * static_cast<size_t>(n),
+                                  hipMemcpyDeviceToHost, c.stream));
+        STOKES_HIP(hipStreamSynchronize(c.stream));
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_last_preconditioned", e.what());
     }
 }
 

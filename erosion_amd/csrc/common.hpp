@@ -249,6 +249,7 @@ struct Context {
     // operator work buffers
     DevBuf<double> ex, ey;          // density in point order
     DevBuf<double> part_u, part_v;  // far-field partial sums [split][npts]
+    DevBuf<double> part_cu, part_cv;  // their corrections (compensated evaluation)
     DevBuf<double> bd;              // boundary values [5 fields][2][npts]
     DevBuf<double> bdpart;          // wall boundary-value partial sums [split][20][nouter]
     DevBuf<double> fld;             // field densities [5][2][npts]
@@ -271,6 +272,13 @@ struct Context {
     int precision = 0;
     double mixed_switch = 1.0e4;  // FP32 far field once residual <= mixed_switch * tol
     bool ff_fp32_now = false;  // the far field of the next operator applications
+    // space of the initial guess a solve is handed: 0 the density x; 1 the
+    // preconditioned variable u = P x (x0 = P^{-1} u0; u = x without a
+    // preconditioner).  With guess_space 1 a solve also leaves its final u
+    // in uvec (u_valid), the starting point a later, nearby solve can take.
+    int guess_space = 0;
+    DevBuf<double> uvec;
+    bool u_valid = false;
     // GMRES buffers
     int npad = 0;
     int maxl_alloc = 0;
@@ -307,6 +315,9 @@ struct Context {
     double last_true_relres = -1.0, last_implicit_relres = -1.0, last_seconds = 0.0;
     int last_reorth = 0;
     long long last_fp32_iters = 0;  // iterations whose far field ran in FP32
+    int last_cycles = 0;            // GMRES cycles (restarts + 1) of the last solve
+    int last_tol_miss = 0;          // 1: the last solve ended above the tolerance
+    long long tol_misses = 0;       // such solves since the process started
 };
 
 Context& ctx();
@@ -323,8 +334,10 @@ void build_near_list(Context& c, int ibary);
 // has left x_dev's density in point order in c.ex/c.ey, the body constraint
 // rows in y_dev and the wall segments' parts of the rank-one integral in
 // c.scal[0 .. precond_wall_segments).
+// compensated: a true-residual evaluation, FP64 far field whose sums carry
+// error-free corrections (k_farfield_comp) and an assembly in double-double
 void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary,
-                    bool from_precond = false);
+                    bool from_precond = false, bool compensated = false);
 // run the geometry-dependent set-up apply_operator would otherwise do inside
 // its first call (far-field tuning, FP32 source tiles, stokeslet logarithms),
 // so a captured sequence of operator applications holds none of it; x_dev is

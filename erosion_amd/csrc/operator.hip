@@ -231,6 +231,128 @@ __global__ __launch_bounds__(kBlk) void k_farfield(
     }
 }
 
+// ---------------------------------------------- compensated FP64 far field
+// The far field of a true-residual evaluation.  Where bodies carry large,
+// mostly cancelling densities, the running sum of a target's pair terms is
+// many orders above the final value, and rounding in the sum (not in the
+// terms) bounds how small a residual b - A x can be shown to be.  Each lane
+// therefore keeps, per target, the sum and a correction: every pair term
+// t = c rx enters through an error-free product (t + fma(c, rx, -t) = c rx)
+// and an error-free sum (TwoSum, Knuth), the lost low parts gathering in the
+// correction (Ogita, Rump and Oishi, "Accurate sum and dot product", 2005).
+// The pair terms themselves are the plain kernel's (their rounding is a few
+// units in the last place of each term).  Contraction is off, so the
+// error-free transformations are evaluated as written.
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+#pragma clang fp contract(off)
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// (hi, lo) += x
+__device__ __forceinline__ void dd_add(double& hi, double& lo, double x) {
+    double s, e;
+    two_sum(hi, x, s, e);
+    hi = s;
+    lo += e;
+}
+
+// (hi, lo) += c * r
+__device__ __forceinline__ void dd_add_prod(double& hi, double& lo, double c, double r) {
+#pragma clang fp contract(off)
+    const double p = c * r;
+    const double ep = __builtin_fma(c, r, -p);
+    double s, e;
+    two_sum(hi, p, s, e);
+    hi = s;
+    lo += e + ep;
+}
+
+template <int TPT, bool CHECK>
+__device__ __forceinline__ void ff_tile_comp(const double2* __restrict__ sxy,
+                                             const double2* __restrict__ sab,
+                                             const double* __restrict__ sc, const double* xt,
+                                             const double* yt, const int* self, double* u,
+                                             double* cu, double* v, double* cv) {
+    for (int jj = 0; jj < kTile; ++jj) {
+        const double2 p = sxy[jj];
+        const double2 ab = sab[jj];
+        const double cc = sc[jj];
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            const double rx = xt[q] - p.x, ry = yt[q] - p.y;
+            const double ry2 = ry * ry;
+            const double rho2 = __builtin_fma(rx, rx, ry2);
+            const double t = __builtin_fma(ab.x, rx, ab.y * ry);
+            const double w = __builtin_fma(t, rx, cc * ry2);
+            const double inv = rcp64(rho2);
+            double c = w * (inv * inv);
+            if (CHECK) c = (jj == self[q]) ? 0.0 : c;
+            dd_add_prod(u[q], cu[q], c, rx);
+            dd_add_prod(v[q], cv[q], c, ry);
+        }
+    }
+}
+
+// grid and target layout as k_farfield (no shared reciprocal); writes each
+// split's sums to part_u/part_v and their corrections to part_cu/part_cv.
+template <int TPT>
+__global__ __launch_bounds__(kBlk) void k_farfield_comp(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ mx,
+    const double* __restrict__ my, const double* __restrict__ ex, const double* __restrict__ ey,
+    int npts, int split_len, double* __restrict__ part_u, double* __restrict__ part_v,
+    double* __restrict__ part_cu, double* __restrict__ part_cv) {
+    __shared__ double2 sxy[kTile], sab[kTile];
+    __shared__ double sc[kTile];
+    const int tid = threadIdx.x;
+    const int tb0 = blockIdx.x * TPT * kBlk;
+    const int tb1 = min(tb0 + TPT * kBlk, npts);
+    double xt[TPT], yt[TPT], u[TPT], v[TPT], cu[TPT], cv[TPT];
+    int t[TPT];
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+        t[q] = tb0 + q * kBlk + tid;
+        const bool valid = t[q] < npts;
+        xt[q] = valid ? x[t[q]] : kNoTarget;
+        yt[q] = valid ? y[t[q]] : kNoTarget;
+        u[q] = v[q] = cu[q] = cv[q] = 0.0;
+    }
+    const int s0 = blockIdx.y * split_len;
+    const int s1 = min(s0 + split_len, npts);
+    for (int base = s0; base < s1; base += kTile) {
+        const int j = base + tid;
+        __syncthreads();
+        if (j < s1) {
+            sxy[tid] = make_double2(x[j], y[j]);
+            src_moments(mx[j], my[j], ex[j], ey[j], &sab[tid], &sc[tid]);
+        } else {
+            sxy[tid] = make_double2(1.0e30, 1.0e30);
+            sab[tid] = make_double2(0.0, 0.0);
+            sc[tid] = 0.0;
+        }
+        __syncthreads();
+        if ((base < tb1) && (base + kTile > tb0)) {
+            int self[TPT];
+#pragma unroll
+            for (int q = 0; q < TPT; ++q) self[q] = t[q] - base;
+            ff_tile_comp<TPT, true>(sxy, sab, sc, xt, yt, self, u, cu, v, cv);
+        } else {
+            ff_tile_comp<TPT, false>(sxy, sab, sc, xt, yt, nullptr, u, cu, v, cv);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < TPT; ++q) {
+        if (t[q] < npts) {
+            const size_t o = static_cast<size_t>(blockIdx.y) * npts + t[q];
+            part_u[o] = u[q];
+            part_v[o] = v[q];
+            part_cu[o] = cu[q];
+            part_cv[o] = cv[q];
+        }
+    }
+}
+
 // ------------------------------------------------------ FP32 far field
 // The same pair term in FP32.  Sources are stored relative to the centre of
 // their 256-point tile (FP32 offsets of a few body radii at most), each
@@ -796,13 +918,18 @@ __global__ __launch_bounds__(kBlk) void k_stokes_logs(const double* __restrict__
 // lanes and reduced with cross-lane moves, so a launch has kAsmSub npts / 64
 // waves to hide the latency of the logarithms and of the partial-sum loads.
 constexpr int kAsmSub = 16;
+// COMP (a true-residual evaluation): the far field's split sums come with
+// their corrections (part_cu, part_cv), and every sum of the assembly is kept
+// as a value and a correction (TwoSum), rounded once at the end.
+template <bool COMP>
 __global__ __launch_bounds__(kBlk) void k_assemble(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ px,
     const double* __restrict__ py, const double* __restrict__ cur,
     const double* __restrict__ speed, const double* __restrict__ ex,
     const double* __restrict__ ey, const double* __restrict__ cx, const double* __restrict__ cy,
     const double* __restrict__ den, const double* __restrict__ part_u,
-    const double* __restrict__ part_v, int nsplit, const int* __restrict__ tgt_ptr,
+    const double* __restrict__ part_v, const double* __restrict__ part_cu,
+    const double* __restrict__ part_cv, int nsplit, const int* __restrict__ tgt_ptr,
     const int* __restrict__ tgt_ent, const double* __restrict__ corr_u,
     const double* __restrict__ corr_v, int has_near, const double* __restrict__ scal, int nsig,
     const double* __restrict__ lgtab, int no, int n, int m, int npts, double* __restrict__ out) {
@@ -813,7 +940,7 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
     const int ii = valid ? i : 0;
     const double xi = x[ii], yi = y[ii];
     // stokeslets and rotlets of every body, bodies q = sub mod kAsmSub on this lane
-    double us = 0.0, vs = 0.0;
+    double us = 0.0, vs = 0.0, lus = 0.0, lvs = 0.0;
     const int base = 2 * no + 2 * n * m;
     for (int b0 = 0; b0 < m; b0 += kBlk) {
         __syncthreads();
@@ -834,29 +961,60 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
             const double irho2 = rcp64(rho2);
             const double lg = lgtab[static_cast<size_t>(ii) * m + b0 + q];
             const double rdots = rx * sto1 + ry * sto2;
-            us += 0.5 / kTwoPi * (-0.5 * lg * sto1 + rdots * irho2 * rx) + rot * ry * irho2;
-            vs += 0.5 / kTwoPi * (-0.5 * lg * sto2 + rdots * irho2 * ry) - rot * rx * irho2;
+            const double tu = 0.5 / kTwoPi * (-0.5 * lg * sto1 + rdots * irho2 * rx) + rot * ry * irho2;
+            const double tv = 0.5 / kTwoPi * (-0.5 * lg * sto2 + rdots * irho2 * ry) - rot * rx * irho2;
+            if (COMP) {
+                dd_add(us, lus, tu);
+                dd_add(vs, lvs, tv);
+            } else {
+                us += tu;
+                vs += tv;
+            }
         }
     }
     // far-field partial sums of the source splits s = sub mod kAsmSub
     if (valid) {
         for (int s = sub; s < nsplit; s += kAsmSub) {
-            us += part_u[static_cast<size_t>(s) * npts + i];
-            vs += part_v[static_cast<size_t>(s) * npts + i];
+            const size_t o = static_cast<size_t>(s) * npts + i;
+            if (COMP) {
+                dd_add(us, lus, part_u[o]);
+                dd_add(vs, lvs, part_v[o]);
+                lus += part_cu[o];
+                lvs += part_cv[o];
+            } else {
+                us += part_u[o];
+                vs += part_v[o];
+            }
         }
     }
 #pragma unroll
     for (int off = 1; off < kAsmSub; off <<= 1) {
-        us += __shfl_xor(us, off);
-        vs += __shfl_xor(vs, off);
+        if (COMP) {
+            // (hi, lo) pairs: TwoSum of the values, corrections added
+            double s, e;
+            two_sum(us, __shfl_xor(us, off), s, e);
+            lus = lus + __shfl_xor(lus, off) + e;
+            us = s;
+            two_sum(vs, __shfl_xor(vs, off), s, e);
+            lvs = lvs + __shfl_xor(lvs, off) + e;
+            vs = s;
+        } else {
+            us += __shfl_xor(us, off);
+            vs += __shfl_xor(vs, off);
+        }
     }
     if (!valid || sub != 0) return;
-    double u = 0.0, v = 0.0;
+    double u = 0.0, v = 0.0, lu = 0.0, lv = 0.0;
     // near-pair replacement
     if (has_near) {
         for (int e = tgt_ptr[i]; e < tgt_ptr[i + 1]; ++e) {
-            u += corr_u[tgt_ent[e]];
-            v += corr_v[tgt_ent[e]];
+            if (COMP) {
+                dd_add(u, lu, corr_u[tgt_ent[e]]);
+                dd_add(v, lv, corr_v[tgt_ent[e]]);
+            } else {
+                u += corr_u[tgt_ent[e]];
+                v += corr_v[tgt_ent[e]];
+            }
         }
     }
     // curvature diagonal and jump
@@ -864,20 +1022,43 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
         const double nsrc = i < no ? static_cast<double>(no) : static_cast<double>(n);
         const double dx = ex[i], dy = ey[i];
         const double tdotden = py[i] * dx - px[i] * dy;
-        u = u - cur[i] * tdotden * py[i] * speed[i] / nsrc;
-        v = v + cur[i] * tdotden * px[i] * speed[i] / nsrc;
-        u = u - 0.5 * dx;
-        v = v - 0.5 * dy;
+        if (COMP) {
+            dd_add(u, lu, -(cur[i] * tdotden * py[i] * speed[i] / nsrc));
+            dd_add(v, lv, cur[i] * tdotden * px[i] * speed[i] / nsrc);
+            dd_add(u, lu, -0.5 * dx);
+            dd_add(v, lv, -0.5 * dy);
+        } else {
+            u = u - cur[i] * tdotden * py[i] * speed[i] / nsrc;
+            v = v + cur[i] * tdotden * px[i] * speed[i] / nsrc;
+            u = u - 0.5 * dx;
+            v = v - 0.5 * dy;
+        }
     }
-    u += us;
-    v += vs;
+    if (COMP) {
+        dd_add(u, lu, us);
+        dd_add(v, lv, vs);
+        lu += lus;
+        lv += lvs;
+    } else {
+        u += us;
+        v += vs;
+    }
     if (i < no) {
         // the rank-one integral from its nsig unscaled parts, in order
         double sig = 0.0;
         for (int j = 0; j < nsig; ++j) sig += scal[j];
         sig = sig * kTwoPi / static_cast<double>(no);
-        u += sig * px[i];
-        v += sig * py[i];
+        if (COMP) {
+            dd_add(u, lu, sig * px[i]);
+            dd_add(v, lv, sig * py[i]);
+        } else {
+            u += sig * px[i];
+            v += sig * py[i];
+        }
+    }
+    if (COMP) {
+        u += lu;
+        v += lv;
     }
     out[row_x(i, no, n)] = u;
     out[row_y(i, no, n)] = v;
@@ -1041,7 +1222,8 @@ int farfield_choice(const Context& c, bool fp32) {
     return it == choice.end() ? -1 : it->second;
 }
 
-void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, bool from_precond) {
+void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, bool from_precond,
+                    bool compensated) {
     Geometry& g = c.g;
     NearList& L = c.near;
     hipStream_t s = c.stream;
@@ -1050,21 +1232,33 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, b
     c.scal.ensure(std::max<size_t>(8, static_cast<size_t>(precond_wall_segments(g))));
 
     if (!from_precond) gather_density(c, x_dev);
-    const bool fp32 = c.ff_fp32_now;
+    const bool fp32 = c.ff_fp32_now && !compensated;
     if (fp32) ff32_sources(c);
-    const FFPlan plan = ff_plan(np, kFFShapes[tune_farfield(c, fp32)], fp32);
+    // a compensated evaluation runs its own far field: 4 targets per lane,
+    // per-pair reciprocals, about 2048 blocks
+    const FFPlan plan = compensated ? ff_plan(np, FFShape{4, 2048, false}, false)
+                                    : ff_plan(np, kFFShapes[tune_farfield(c, fp32)], fp32);
     const int nsplit = plan.nsplit;
-    c.ff_splits = nsplit;
-    c.ff_tpt = plan.tpt;
     c.part_u.ensure(N * static_cast<size_t>(nsplit));
     c.part_v.ensure(N * static_cast<size_t>(nsplit));
-    KernelTimer& tff = fp32 ? c.t_farfield32 : c.t_farfield;
-    if (c.time_kernels) tff.start(s);
-    if (fp32)
-        ff32_launch(c, plan, c.part_u.p, c.part_v.p);
-    else
-        ff_launch(c, plan, c.part_u.p, c.part_v.p);
-    if (c.time_kernels) tff.stop(s, static_cast<double>(np) * static_cast<double>(np - 1));
+    if (compensated) {
+        c.part_cu.ensure(N * static_cast<size_t>(nsplit));
+        c.part_cv.ensure(N * static_cast<size_t>(nsplit));
+        k_farfield_comp<4><<<dim3(plan.tblocks, plan.nsplit), kBlk, 0, s>>>(
+            g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p, np, plan.split_len, c.part_u.p,
+            c.part_v.p, c.part_cu.p, c.part_cv.p);
+        STOKES_LAUNCH_CHECK();
+    } else {
+        c.ff_splits = nsplit;
+        c.ff_tpt = plan.tpt;
+        KernelTimer& tff = fp32 ? c.t_farfield32 : c.t_farfield;
+        if (c.time_kernels) tff.start(s);
+        if (fp32)
+            ff32_launch(c, plan, c.part_u.p, c.part_v.p);
+        else
+            ff_launch(c, plan, c.part_u.p, c.part_v.p);
+        if (c.time_kernels) tff.stop(s, static_cast<double>(np) * static_cast<double>(np - 1));
+    }
     const bool has_near = ibary == 1 && L.nentries > 0;
     if (has_near) {
         boundary_values(c, L.active.p, L.nactive, L.wall_active);
@@ -1087,11 +1281,17 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, b
     }
     const int nsig = from_precond ? precond_wall_segments(g) : 1;
     stokes_logs(c);
-    k_assemble<<<cdiv_int(np, kBlk / kAsmSub), kBlk, 0, s>>>(
-        g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p, c.ex.p, c.ey.p, g.cx.p, g.cy.p, x_dev,
-        c.part_u.p, c.part_v.p, nsplit, L.tgt_ptr.p, L.tgt_ent.p, L.corr_u.p, L.corr_v.p,
-        has_near ? 1 : 0, c.scal.p, nsig, g.m > 0 ? c.stk_log.p : nullptr, g.no, g.n, g.m, np,
-        y_dev);
+#define STOKES_ASM(COMP)                                                                       \
+    k_assemble<COMP><<<cdiv_int(np, kBlk / kAsmSub), kBlk, 0, s>>>(                             \
+        g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p, c.ex.p, c.ey.p, g.cx.p, g.cy.p, x_dev, \
+        c.part_u.p, c.part_v.p, c.part_cu.p, c.part_cv.p, nsplit, L.tgt_ptr.p, L.tgt_ent.p,      \
+        L.corr_u.p, L.corr_v.p, has_near ? 1 : 0, c.scal.p, nsig,                                 \
+        g.m > 0 ? c.stk_log.p : nullptr, g.no, g.n, g.m, np, y_dev)
+    if (compensated)
+        STOKES_ASM(true);
+    else
+        STOKES_ASM(false);
+#undef STOKES_ASM
     STOKES_LAUNCH_CHECK();
 }
 

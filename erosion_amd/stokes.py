@@ -81,6 +81,10 @@ def lib():
         L.erosion_stepper_reset.restype = ctypes.c_int
         L.erosion_spectral.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, _dp, _dp, _dp]
         L.erosion_spectral.restype = ctypes.c_int
+        L.erosion_set_guess_space.argtypes = [ctypes.c_int]
+        L.erosion_set_guess_space.restype = ctypes.c_int
+        L.erosion_last_preconditioned.argtypes = [_dp, ctypes.c_int64]
+        L.erosion_last_preconditioned.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -138,7 +142,26 @@ def last_solve_info() -> dict:
     return {"iters": int(info[0]), "implicit_relres": float(info[1]),
             "true_relres": float(info[2]), "seconds": float(info[3]), "reorth": int(info[4]),
             "ff_targets_per_lane": int(info[5]), "ff_splits": int(info[6]),
-            "preconditioner": int(info[7]), "fp32_iters": int(info[8])}
+            "preconditioner": int(info[7]), "fp32_iters": int(info[8]),
+            "tol_miss": int(info[9]), "tol_misses_total": int(info[10]), "cycles": int(info[11])}
+
+
+GUESS_DENSITY, GUESS_PRECONDITIONED = 0, 1
+
+
+def set_guess_space(space: int) -> None:
+    """Space of the initial guess stokessolver_ takes: GUESS_DENSITY (x0) or
+    GUESS_PRECONDITIONED (u0 = P x0 of the right preconditioner)."""
+    if lib().erosion_set_guess_space(int(space)) != 0:
+        raise StokesError(f"invalid guess space {space}")
+
+
+def last_preconditioned(npts: int, nbods: int, nouter: int) -> np.ndarray:
+    """u = P x of the last solve (which must have run with GUESS_PRECONDITIONED)."""
+    u = np.zeros(nunk(npts, nbods, nouter))
+    if lib().erosion_last_preconditioned(_p(u), len(u)) != 0:
+        raise StokesError("the last solve did not track the preconditioned variable")
+    return u
 
 
 def stokessolver(npts: int, nbods: int, nouter: int, ifmm: int, ibary: int, ibc: int, maxl: int,

@@ -105,9 +105,26 @@ const char* erosion_last_error(void);
  * residual ||b - A x|| / ||b|| of the returned density, info[3] seconds,
  * info[4] re-orthogonalised iterations, info[5] far-field targets per lane,
  * info[6] far-field source splits chosen by the autotuner, info[7] the
- * preconditioner kind and info[8] the iterations whose far field ran in FP32.
- * `info` holds at least 9 doubles.  Returns 0. */
+ * preconditioner kind, info[8] the iterations whose far field ran in FP32,
+ * info[9] 1 if the returned density's true residual is above the tolerance
+ * (0 otherwise), info[10] the number of such solves since the library was
+ * loaded and info[11] the GMRES cycles run (1 + restarts).  The true residual
+ * is formed with compensated sums (error-free transformations in the far
+ * field and the assembly), and a solve whose true residual misses the
+ * tolerance restarts from its density, aiming at half the tolerance.
+ * `info` holds at least 12 doubles.  Returns 0. */
 int erosion_last_solve_info(double* info);
+
+/* Initial guess of stokessolver_ (no counterpart in the reference, whose
+ * solves start from the den the caller passes, DensityStress.jl:45).
+ * space 0 (default): den holds the density x0.  space 1: den holds u0 in the
+ * preconditioned variable u = P x of the right preconditioner (x0 = P^{-1}
+ * u0; u = x without a preconditioner).  The density out is x in both cases.
+ * Returns 0, or -1 for another space. */
+int erosion_set_guess_space(int space);
+/* After a solve with guess space 1: its final u (n = N_unk values) into u.
+ * Returns 0, or -1 when the last solve did not track u or n differs. */
+int erosion_last_preconditioned(double* u, int64_t n);
 
 /* One application of the completed operator vel = A den, as matvec_DLP_fmmbary
  * (ibary=1, stokesSolver.f:570-1124, far field summed directly) or matvec_DLP
