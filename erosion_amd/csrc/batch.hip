@@ -62,6 +62,7 @@ void solve_batch(Context& settings, int ngeom, const BatchGeometry* geo, int iba
                 build_near_list(c, ibary);
                 *b.nits = gmres_solve(c, ibary, ibc, maxl, c.tol, b.den);
                 *b.status = c.status;
+                if (b.relres) *b.relres = c.last_true_relres;
                 if (b.tau && b.m > 0) {
                     const size_t nunk = static_cast<size_t>(c.g.nunk);
                     const size_t nbp = static_cast<size_t>(b.n) * static_cast<size_t>(b.m);

@@ -198,7 +198,8 @@ void computepressure_(const int64_t* npts, const int64_t* nbods, const int64_t* 
 int erosion_solve_batch(int64_t ngeom, const int64_t* npts, const int64_t* nbods,
                         const int64_t* nouter, const double* const* xx, const double* const* yy,
                         double* const* den, double* const* tau, int64_t ibary, int64_t ibc,
-                        int64_t maxl, int64_t nworkers, int64_t* nits, int* status) {
+                        int64_t maxl, int64_t nworkers, int64_t* nits, int* status,
+                        double* relres) {
     try {
         if (ngeom < 0 || ngeom > (int64_t{1} << 20)) throw stokes::Error("invalid geometry count");
         if (ngeom == 0) return 0;
@@ -214,7 +215,8 @@ int erosion_solve_batch(int64_t ngeom, const int64_t* npts, const int64_t* nbods
                 throw stokes::Error("null array of geometry " + std::to_string(i));
             geo[static_cast<size_t>(i)] = stokes::BatchGeometry{
                 static_cast<int>(npts[i]), static_cast<int>(nbods[i]), static_cast<int>(nouter[i]),
-                xx[i], yy[i], den[i], tau ? tau[i] : nullptr, nits + i, status + i};
+                xx[i], yy[i], den[i], tau ? tau[i] : nullptr, nits + i, status + i,
+                relres ? relres + i : nullptr};
             nits[i] = 0;
             status[i] = -1;
         }
@@ -282,6 +284,7 @@ int erosion_last_solve_info(double* info) {
         info[9] = static_cast<double>(c.last_tol_miss);
         info[10] = static_cast<double>(c.tol_misses);
         info[11] = static_cast<double>(c.last_cycles);
+        info[12] = c.last_r0_relres;
         return 0;
     } catch (const std::exception& e) {
         return record_error("erosion_last_solve_info", e.what());
@@ -314,6 +317,34 @@ int erosion_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t ibary,
         return 0;
     } catch (const std::exception& e) {
         return record_error("erosion_matvec", e.what());
+    }
+}
+
+int erosion_geometry(int64_t npts, int64_t nbods, int64_t nouter, const double* xx,
+                     const double* yy, double* out) {
+    try {
+        check_sizes(npts, nbods, nouter);
+        if (!out || (nbods > 0 && (!xx || !yy))) throw stokes::Error("null argument");
+        Context& c = stokes::ctx();
+        c.status = 0;
+        stokes::build_geometry(c, static_cast<int>(npts), static_cast<int>(nbods),
+                               static_cast<int>(nouter), xx, yy);
+        const stokes::Geometry& g = c.g;
+        const size_t np = static_cast<size_t>(g.npts), m = static_cast<size_t>(g.m);
+        const double* src[4] = {g.px.p, g.py.p, g.cur.p, g.speed.p};
+        for (int a = 0; a < 4; ++a)
+            STOKES_HIP(hipMemcpyAsync(out + a * np, src[a], np * sizeof(double),
+                                      hipMemcpyDeviceToHost, c.stream));
+        if (m > 0) {
+            STOKES_HIP(hipMemcpyAsync(out + 4 * np, g.cx.p, m * sizeof(double),
+                                      hipMemcpyDeviceToHost, c.stream));
+            STOKES_HIP(hipMemcpyAsync(out + 4 * np + m, g.cy.p, m * sizeof(double),
+                                      hipMemcpyDeviceToHost, c.stream));
+        }
+        STOKES_HIP(hipStreamSynchronize(c.stream));
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_geometry", e.what());
     }
 }
 
@@ -407,6 +438,14 @@ int erosion_set_guess_space(int space) {
         return 0;
     } catch (const std::exception& e) {
         return record_error("erosion_set_guess_space", e.what());
+    }
+}
+
+int erosion_get_guess_space(void) {
+    try {
+        return stokes::ctx().guess_space;
+    } catch (...) {
+        return -1;
     }
 }
 

@@ -225,7 +225,9 @@ struct GraphCache {
 struct StepParams {
     double dt = 1e-3, eps = 0.0, sigma = 0.0, fixarea = 0.0;
     int fixpdrop = 1, ibc = 0, ibary = 1, maxl = 8000;
-    int warm = 2;  // GMRES guess: 0 zeros, 1 previous density, 2 linear extrapolation
+    // GMRES guess, in the preconditioned variable u = P x of the solves: 0
+    // zeros, 1 the previous solution, 2 linear extrapolation of the last two
+    int warm = 2;
 };
 struct StepReport {
     long long iters[2] = {0, 0};  // GMRES iterations of the two stages
@@ -238,7 +240,8 @@ struct StepperBufs {
     DevBuf<double> th0, th05, th1, xx, yy, den, tau, atau, matau, nterm, dscal, scal, fac, tgt,
         qout, tmp, hist0, hist1;
     DevBuf<int> idx;
-    // densities kept for guesses, their body count and length
+    // solutions (preconditioned variable) kept for guesses, their body count
+    // and length
     int hist_count = 0, hist_m = -1, hist_n = -1;
 };
 
@@ -316,6 +319,7 @@ struct Context {
     int last_reorth = 0;
     long long last_fp32_iters = 0;  // iterations whose far field ran in FP32
     int last_cycles = 0;            // GMRES cycles (restarts + 1) of the last solve
+    double last_r0_relres = -1.0;   // ||b - A x0|| / ||b|| of its initial guess
     int last_tol_miss = 0;          // 1: the last solve ended above the tolerance
     long long tol_misses = 0;       // such solves since the process started
 };
@@ -378,6 +382,7 @@ struct BatchGeometry {
     double* tau;     // shear stress out, or null
     int64_t* nits;
     int* status;     // 0 ok, 1 not converged, 2 breakdown, -1 error
+    double* relres;  // true relative residual of the returned density, or null
 };
 void solve_batch(Context& settings, int ngeom, const BatchGeometry* geo, int ibary, int ibc,
                  int maxl, int nworkers, std::string* first_error);

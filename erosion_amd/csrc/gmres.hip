@@ -679,6 +679,7 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
         }
         c.ff_fp32_now = false;
         GmresState fin = *c.h_st;
+        if (cycle == 0) c.last_r0_relres = fin.r0nrm / fin.bnrm;
         const int iters = fin.k;
         status = 0;
         if (fin.done == 3) {

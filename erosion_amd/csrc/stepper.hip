@@ -1,6 +1,6 @@
 // One erosion step on the device: the theta-L midpoint RK2 of
-// ErosionPackage/src/TimeStepping.jl -- rungekutta2 (:163-176), timestep!
-// (:104-159), getderivs (:37-69), tangvel (:16-26), delete_indices (:77-100)
+// ErosionPackage/src/TimeStepping.jl -- rungekutta2 (:189-202), timestep!
+// (:130-185), getderivs (:63-95), tangvel (:42-52), delete_indices (:103-126)
 // -- with the spectral operations of SpectralMethods.jl (specdiff :35-41,
 // specint :45-54, expsmooth :57-64, gaussfilter :67), getxy of
 // ThetaLen.jl:92-101 and getstress! / getumax / getpdrop of
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(kSB) void k_stress_filter(const double* __restrict_
     if (threadIdx.x == 0) matau[b] = s / n;
 }
 
-// getderivs (TimeStepping.jl:37-58) with tangvel (:16-26) for one body:
+// getderivs (TimeStepping.jl:63-84) with tangvel (:42-52) for one body:
 //   dtheta = specdiff(theta - alpha) + 1
 //   vnorm  = atau - fixarea matau + eps matau (dtheta - 1)
 //   mterm  = -2 pi mean(dtheta vnorm), vtang = specint(dtheta vnorm - mean)
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(kSB) void k_derivs(const double* __restrict__ theta
     }
 }
 
-// The theta update of timestep! (TimeStepping.jl:138-141):
+// The theta update of timestep! (TimeStepping.jl:174-175):
 //   theta1 = expsmooth(theta0 - alpha, fac1) + alpha + dt1 expsmooth(nterm, fac2)
 __global__ __launch_bounds__(kSB) void k_theta_update(const double* __restrict__ theta0,
                                                       const double* __restrict__ nterm,
@@ -299,12 +299,19 @@ __global__ void k_keep_rows(const double* __restrict__ src, const int* __restric
     dst[e] = src[static_cast<size_t>(idx[r]) * n + j];
 }
 
-// x0 = 2 h1 - h0 (order 2) or h1 (order 1).
+// u0 = 2 h1 - h0 (order 2) or h1 (order 1): the guess from the last two solutions.
 __global__ void k_guess(const double* __restrict__ h0, const double* __restrict__ h1, int order,
                         int N, double* __restrict__ x) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     x[i] = order == 2 ? 2.0 * h1[i] - h0[i] : h1[i];
+}
+
+// dst[i] = src[idx[i]] (the unknowns of the bodies kept after a deletion).
+__global__ void k_gather_idx(const double* __restrict__ src, const int* __restrict__ idx, int N,
+                             double* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) dst[i] = src[idx[i]];
 }
 
 __global__ void k_fill0(double* __restrict__ x, int N) {
@@ -378,7 +385,9 @@ void stage_derivs(Context& c, const StepParams& P, int n, int no, int m, const d
     build_geometry(c, n, m, no, S.xx.p, S.yy.p, true);
     build_near_list(c, P.ibary);
 
-    // GMRES initial guess from the densities of earlier states with this body count
+    // GMRES initial guess in the preconditioned variable u = P x (the density
+    // itself without a preconditioner), from the solves of earlier states
+    // with these bodies
     S.den.ensure(static_cast<size_t>(N));
     if (P.warm > 0 && S.hist_count > 0 && S.hist_m == m && S.hist_n == N) {
         k_guess<<<cdiv_int(N, 256), 256, 0, s>>>(S.hist0.p, S.hist1.p,
@@ -393,7 +402,16 @@ void stage_derivs(Context& c, const StepParams& P, int n, int no, int m, const d
         k_fill0<<<cdiv_int(N, 256), 256, 0, s>>>(S.den.p, N);
     }
     STOKES_LAUNCH_CHECK();
-    const long long it = gmres_solve(c, P.ibary, P.ibc, P.maxl, c.tol, S.den.p, true);
+    const int space = c.guess_space;
+    c.guess_space = 1;
+    long long it = 0;
+    try {
+        it = gmres_solve(c, P.ibary, P.ibc, P.maxl, c.tol, S.den.p, true);
+    } catch (...) {
+        c.guess_space = space;
+        throw;
+    }
+    c.guess_space = space;
     if (c.status == 1)
         std::fprintf(stderr,
                      "libstokes_hip: GMRES stopped after %lld iterations (maxl=%d) with relative "
@@ -471,7 +489,7 @@ void stage_derivs(Context& c, const StepParams& P, int n, int no, int m, const d
     }
 }
 
-// delete_indices (TimeStepping.jl:77-100): keep the bodies with
+// delete_indices (TimeStepping.jl:103-126): keep the bodies with
 // len > -sqrt(2) mterm dt and mterm <= 10.
 std::vector<int> kept_bodies(const std::vector<double>& len0, const std::vector<double>& mterm,
                              double dt, StepReport& rep) {
@@ -484,11 +502,11 @@ std::vector<int> kept_bodies(const std::vector<double>& len0, const std::vector<
     return keep;
 }
 
-// After a solve at m bodies: keep the density for later guesses unless the
-// state lost bodies since (its density then no longer matches the state).
-void remember_density(Context& c, int N, int m, bool lost_bodies) {
+// After a solve at N unknowns and m bodies: keep its preconditioned variable
+// (c.uvec) for the guesses of later solves.
+void remember_solution(Context& c, int N, int m) {
     StepperBufs& S = c.sp;
-    if (lost_bodies) {
+    if (!c.u_valid) {
         S.hist_count = 0;
         S.hist_m = S.hist_n = -1;
         return;
@@ -499,14 +517,48 @@ void remember_density(Context& c, int N, int m, bool lost_bodies) {
     if (S.hist_count > 0)
         STOKES_HIP(hipMemcpyAsync(S.hist0.p, S.hist1.p, N * sizeof(double),
                                   hipMemcpyDeviceToDevice, c.stream));
-    STOKES_HIP(hipMemcpyAsync(S.hist1.p, S.den.p, N * sizeof(double), hipMemcpyDeviceToDevice,
+    STOKES_HIP(hipMemcpyAsync(S.hist1.p, c.uvec.p, N * sizeof(double), hipMemcpyDeviceToDevice,
                               c.stream));
     S.hist_count = std::min(S.hist_count + 1, 2);
     S.hist_m = m;
     S.hist_n = N;
 }
 
-// theta1, len1, xsm1, ysm1 of timestep! (TimeStepping.jl:127-152) from the
+// Bodies `keep` (of m) remain: the kept solutions drop the unknowns of the
+// others (their density rows and constraint triples), so the guesses of later
+// solves still cover every remaining body.
+void keep_solutions(Context& c, int n, int no, int m, const std::vector<int>& keep) {
+    StepperBufs& S = c.sp;
+    if (S.hist_count == 0) return;
+    const int mk = static_cast<int>(keep.size());
+    const int Nk = 2 * no + 2 * n * mk + 3 * mk;
+    std::vector<int> idx(static_cast<size_t>(Nk));
+    for (int i = 0; i < 2 * no; ++i) idx[static_cast<size_t>(i)] = i;
+    for (int r = 0; r < mk; ++r) {
+        const int b = keep[static_cast<size_t>(r)];
+        for (int j = 0; j < 2 * n; ++j)
+            idx[static_cast<size_t>(2 * no + 2 * n * r + j)] = 2 * no + 2 * n * b + j;
+        for (int t = 0; t < 3; ++t)
+            idx[static_cast<size_t>(2 * no + 2 * n * mk + 3 * r + t)] = 2 * no + 2 * n * m + 3 * b + t;
+    }
+    S.idx.ensure(idx.size());
+    STOKES_HIP(hipMemcpyAsync(S.idx.p, idx.data(), idx.size() * sizeof(int),
+                              hipMemcpyHostToDevice, c.stream));
+    S.tmp.ensure(static_cast<size_t>(Nk));
+    for (DevBuf<double>* h : {&S.hist1, &S.hist0}) {
+        if (h == &S.hist0 && S.hist_count < 2) break;
+        k_gather_idx<<<cdiv_int(Nk, 256), 256, 0, c.stream>>>(h->p, S.idx.p, Nk, S.tmp.p);
+        STOKES_LAUNCH_CHECK();
+        STOKES_HIP(hipMemcpyAsync(h->p, S.tmp.p, Nk * sizeof(double), hipMemcpyDeviceToDevice,
+                                  c.stream));
+    }
+    // the host vector and index buffer must outlive the copies above
+    STOKES_HIP(hipStreamSynchronize(c.stream));
+    S.hist_m = mk;
+    S.hist_n = Nk;
+}
+
+// theta1, len1, xsm1, ysm1 of timestep! (TimeStepping.jl:150-182) from the
 // state at t0 (theta in th0, scalars s0) and the derivatives of the state sd.
 void advance(Context& c, const StepParams& P, int n, int m, const DevBuf<double>& th0,
              const Scalars& s0, const Scalars& sd, const std::vector<double>& mterm,
@@ -560,8 +612,9 @@ void rk2_step(Context& c, const StepParams& P, int n, int no, int m, double* the
     stage_derivs(c, P, n, no, m, S.th0.p, s0, mterm, xdot, ydot, rep, 0, guess0);
     std::vector<int> keep = kept_bodies(s0.len, mterm, h, rep);
     const int N0 = 2 * no + 2 * n * m + 3 * m;
-    remember_density(c, N0, m, static_cast<int>(keep.size()) != m);
+    remember_solution(c, N0, m);
     if (static_cast<int>(keep.size()) != m) {
+        keep_solutions(c, n, no, m, keep);
         rep.deleted += m - static_cast<int>(keep.size());
         keep_rows(c, S.th0, keep, n);
         keep_rows(c, S.nterm, keep, n);
@@ -582,8 +635,9 @@ void rk2_step(Context& c, const StepParams& P, int n, int no, int m, double* the
     stage_derivs(c, P, n, no, m, S.th05.p, s05, mterm, xdot, ydot, rep, 1, nullptr);
     keep = kept_bodies(s0.len, mterm, P.dt, rep);
     const int N1 = 2 * no + 2 * n * m + 3 * m;
-    remember_density(c, N1, m, static_cast<int>(keep.size()) != m);
+    remember_solution(c, N1, m);
     if (static_cast<int>(keep.size()) != m) {
+        keep_solutions(c, n, no, m, keep);
         rep.deleted += m - static_cast<int>(keep.size());
         keep_rows(c, S.th0, keep, n);
         keep_rows(c, S.nterm, keep, n);

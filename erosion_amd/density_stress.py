@@ -156,6 +156,8 @@ def getstress_(thlenden: ThLenDenType, params: ParamSet, den0=None, timings: dic
         if nits is not None:
             timings["gmres_iters"] = timings.get("gmres_iters", 0) + nits
             timings["solves"] = timings.get("solves", 0) + 1
+            timings["true_relres_max"] = max(timings.get("true_relres_max", 0.0),
+                                             stokes.last_solve_info()["true_relres"])
         timings["stresses"] = timings.get("stresses", 0) + 1
     for bod in range(len(thlenden.thlenvec)):
         stress[:, bod] = gaussfilter(np.abs(stress[:, bod]), params.sigma)

@@ -1,15 +1,17 @@
 """The erosion step with the theta-L update on the GPU.
 
 ``rungekutta2_device`` has the meaning of ``time_stepping.rungekutta2``
-(ErosionPackage/src/TimeStepping.jl:163-176): one midpoint RK2 step of every
+(ErosionPackage/src/TimeStepping.jl:189-202): one midpoint RK2 step of every
 body, two density solves, two shear-stress evaluations, the fixed-pressure-drop
 rescaling and the deletion of vanishing bodies.  The whole step runs in the
 library (erosion_rungekutta2): the boundary points, densities and stresses of
 both stages stay on the GPU, and only the theta-L state crosses to the host,
-once in and once out.  GMRES guesses come from the densities the library keeps
-between steps (``warm`` 2: linear extrapolation of the last two, 1: the last,
-0: zeros as the reference).  Unlike the host stepper, ``thld0`` is left
-unchanged (the host version deletes bodies from it in place).
+once in and once out.  GMRES guesses come from the solutions the library keeps
+between solves, in the preconditioned variable u = P x (``warm`` 2: linear
+extrapolation of the last two, 1: the last, 0: zeros as the reference); when
+bodies are deleted the kept solutions drop their unknowns.  Unlike the host
+stepper, ``thld0`` is left unchanged (the host version deletes bodies from it
+in place).
 """
 from __future__ import annotations
 
@@ -24,8 +26,9 @@ from .theta_len import ParamSet, ThetaLenType, ThLenDenType, new_thlenden
 def rungekutta2_device(thld0: ThLenDenType, params: ParamSet, warm: int = 2,
                        timings: dict = None, guess=None) -> ThLenDenType:
     """``guess(thld0)`` (for example the trained network, ``LearnedGuess``) is
-    asked for a first-stage GMRES guess only when the library keeps no density
-    that fits the state (the first step of a run, after a deletion)."""
+    asked for a first-stage GMRES guess, in the preconditioned variable u, only
+    when the library keeps no solution that fits the state (the first step of
+    a run)."""
     tv = thld0.thlenvec
     n = params.npts
     if not tv:
@@ -55,5 +58,9 @@ def rungekutta2_device(thld0: ThLenDenType, params: ParamSet, warm: int = 2,
         timings["stresses"] = timings.get("stresses", 0) + 2
         if info["deleted"]:
             timings["deleted"] = timings.get("deleted", 0) + int(info["deleted"])
+        for k in ("warm_starts", "given_guesses"):
+            timings[k] = timings.get(k, 0) + int(info[k])
+        timings["true_relres_max"] = max(timings.get("true_relres_max", 0.0),
+                                         info["true_relres_max"])
         timings["last_info"] = info
     return out

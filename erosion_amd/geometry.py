@@ -20,6 +20,7 @@ centre proposals), so a seed fixes the geometry bit for bit.
 from __future__ import annotations
 
 import dataclasses
+import os
 from typing import Dict, List, Optional, Tuple
 
 import numpy as np
@@ -45,6 +46,9 @@ class Workload:
     ibc: int = 0
     steps: int = 1
     dt: float = 1e-3
+    # a committed theta-L state (erosion_amd/data) the workload starts from
+    # instead of the seeded initial geometry
+    state: Optional[str] = None
 
 
 WORKLOADS: Dict[str, Workload] = {
@@ -56,6 +60,12 @@ WORKLOADS: Dict[str, Workload] = {
     "bodies200_n512": Workload("bodies200_n512", 200, 512, 4096, 0.01, 0.03, 0.01, 1003, steps=100),
     # 400 bodies x 256 points, wall 4096 (learned-warm-start workload).
     "bodies400_n256": Workload("bodies400_n256", 400, 256, 4096, 0.015, 0.035, 0.015, 1005, steps=1),
+    # The 400-body workload eroded until 50% of the total body area is lost
+    # (46 RK2 steps, 386 bodies left; python -m erosion_amd.eroded_state):
+    # the learned-warm-start workload, 102,912 boundary points.
+    "bodies400_n256_eroded50": Workload("bodies400_n256_eroded50", 400, 256, 4096, 0.015, 0.035,
+                                        0.015, 1005, steps=20,
+                                        state="bodies400_n256_eroded50.npz"),
     # 60 bodies x 128 points packed 0.004 apart, wall 1024: most bodies have
     # near neighbours, so the barycentric near-pair pass does real work.
     "bodies60_n128_tight": Workload("bodies60_n128_tight", 60, 128, 1024, 0.05, 0.10, 0.004, 1006),
@@ -135,6 +145,12 @@ def make_bodies(nbodies: int, npts: int, rmin: float, rmax: float, gap: float, s
 
 def workload_geometry(w: Workload, nbodies: Optional[int] = None):
     """theta-L state of a workload (optionally truncated to fewer bodies)."""
+    if w.state is not None:
+        from .eroded_state import DATA, load_state
+        th = load_state(os.path.join(DATA, w.state))
+        if nbodies is not None:
+            th.thlenvec = th.thlenvec[:nbodies]
+        return th
     thl = make_bodies(w.nbodies, w.npts, w.rmin, w.rmax, w.gap, w.seed)
     if nbodies is not None:
         thl = thl[:nbodies]

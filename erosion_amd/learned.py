@@ -1,24 +1,32 @@
 """Learned GMRES initial guess for the density solve (no counterpart in the
 reference, whose solves start from zeros, ErosionPackage/src/DensityStress.jl:45).
 
+The guess is made in the preconditioned variable u = P x of the right
+preconditioner (precond.hip), not in the density x.  The density of a small
+body is huge and almost all rigid rotation (about 10^3 to 10^5 times the
+right-hand side), so a density guess with a relative error of 1e-3 still
+leaves a starting residual near ||b||.  u is O(||b||) instead: on a body's
+rows A x = b gives P x = -(the velocity the other curves induce there) up to
+the high-mode part of the body's own block, a smooth flow along the body.  A
+guess u0 enters as x0 = P^{-1} u0, where P^{-1} supplies the large rigid part
+exactly.
+
 A multilayer perceptron (3 hidden layers x 256 tanh units, BF16 products with
 FP32 accumulation on the matrix cores, libstokes_hip.so's erosion_mlp_*) maps
-each body's features to that body's density in Fourier coefficients:
+each body's features to that body's part of u in Fourier coefficients:
 
   features (128): the body's theta-L descriptor -- Fourier modes 1..16 of
     theta - alpha (32 values), cos and sin of its mean (the orientation), the
     length L and the centre (xsm, ysm) -- and, for each of its 8 nearest
     neighbours by centre distance, the relative centre (2), L (1) and modes
     1..4 of theta - alpha (8); zero padded to 128.
-  output (69): Fourier coefficients 0..16 (cosine, sine) of eta_x and eta_y
-    along the body's points, times the body's speed L / 2 pi (66; the density
-    of a small body is large, its rigid-motion part scaling like the
-    stokeslet over the speed), and its stokeslet x, y and rotlet (3).
+  output (69): Fourier coefficients 0..16 (cosine, sine) of u_x and u_y
+    along the body's points (66) and u's three constraint entries (3).
 
-The wall part of the guess is the density of the same wall with no bodies
-(one solve, kept per wall size).  The guess only changes the GMRES iteration
-count: every solve still meets the same tolerance, and zeros remain the
-fallback (for bodies whose point count the network was not trained on).
+The wall part of the guess is u of the same wall with no bodies (one solve,
+kept per wall size).  The guess only changes the GMRES iteration count: every
+solve still meets the same tolerance, and zeros remain the fallback (for
+bodies whose point count the network was not trained on).
 """
 from __future__ import annotations
 
@@ -172,34 +180,33 @@ def _basis(n: int) -> np.ndarray:
     return np.array(cols).T
 
 
-def density_targets(den: np.ndarray, n: int, m: int, no: int, speed: np.ndarray) -> np.ndarray:
-    """[bodies, NOUT]: least-squares Fourier coefficients of each body's
-    density (eta_x, eta_y) times its speed, and its stokeslet/rotlet."""
+def solution_targets(u: np.ndarray, n: int, m: int, no: int) -> np.ndarray:
+    """[bodies, NOUT]: least-squares Fourier coefficients of each body's part
+    of u (u_x, u_y along its points) and its three constraint entries."""
     B = _basis(n)
     P = np.linalg.pinv(B)
     T = np.zeros((m, NOUT))
     nf = B.shape[1]
     for b in range(m):
         o = 2 * no + 2 * b * n
-        T[b, :nf] = P @ den[o:o + n] * speed[b]
-        T[b, nf:2 * nf] = P @ den[o + n:o + 2 * n] * speed[b]
-        T[b, 2 * nf:] = den[2 * no + 2 * n * m + 3 * b:2 * no + 2 * n * m + 3 * b + 3]
+        T[b, :nf] = P @ u[o:o + n]
+        T[b, nf:2 * nf] = P @ u[o + n:o + 2 * n]
+        T[b, 2 * nf:] = u[2 * no + 2 * n * m + 3 * b:2 * no + 2 * n * m + 3 * b + 3]
     return T
 
 
-def assemble_density(T: np.ndarray, n: int, m: int, no: int, wall: np.ndarray,
-                     speed: np.ndarray) -> np.ndarray:
-    """Density vector from per-body coefficients and a wall density."""
+def assemble_solution(T: np.ndarray, n: int, m: int, no: int, wall: np.ndarray) -> np.ndarray:
+    """u vector from per-body coefficients and a wall part."""
     B = _basis(n)
     nf = B.shape[1]
-    den = np.zeros(2 * no + 2 * n * m + 3 * m)
-    den[:2 * no] = wall
+    u = np.zeros(2 * no + 2 * n * m + 3 * m)
+    u[:2 * no] = wall
     for b in range(m):
         o = 2 * no + 2 * b * n
-        den[o:o + n] = B @ T[b, :nf] / speed[b]
-        den[o + n:o + 2 * n] = B @ T[b, nf:2 * nf] / speed[b]
-        den[2 * no + 2 * n * m + 3 * b:2 * no + 2 * n * m + 3 * b + 3] = T[b, 2 * nf:]
-    return den
+        u[o:o + n] = B @ T[b, :nf]
+        u[o + n:o + 2 * n] = B @ T[b, nf:2 * nf]
+        u[2 * no + 2 * n * m + 3 * b:2 * no + 2 * n * m + 3 * b + 3] = T[b, 2 * nf:]
+    return u
 
 
 def body_speeds(thlenden: ThLenDenType) -> np.ndarray:
@@ -219,8 +226,11 @@ def wall_rhs(no: int, m: int, n: int, ibc: int = 0) -> np.ndarray:
 
 # ------------------------------------------------------------ the guess
 class LearnedGuess:
-    """GMRES initial guess from the trained network, in the ``guess(thlenden)``
-    form that ``time_stepping.rungekutta2`` accepts."""
+    """GMRES initial guess u0 from the trained network, in the ``guess(thlenden)``
+    form that ``time_stepping.rungekutta2`` and ``rungekutta2_device`` accept.
+    The solve must take it in the preconditioned variable
+    (``stokes.set_guess_space(stokes.GUESS_PRECONDITIONED)``; the device
+    stepper always does) with the preconditioner on."""
 
     def __init__(self, path: str = WEIGHTS, nouter: Optional[int] = None, ibc: int = 0):
         z = np.load(path, allow_pickle=False)
@@ -232,18 +242,29 @@ class LearnedGuess:
         self._wall: Dict[int, np.ndarray] = {}
         self.nouter = nouter
 
-    def wall_density(self, no: int) -> np.ndarray:
-        """Density of the wall alone (no bodies), one solve per wall size."""
+    def wall_solution(self, no: int) -> np.ndarray:
+        """u of the wall alone (no bodies), one solve per wall size."""
         if no not in self._wall:
-            den, _ = stokes.stokessolver(self.npts, 0, no, 1, 1, self.ibc, 4000,
-                                         np.zeros(0), np.zeros(0))
-            self._wall[no] = den[:2 * no].copy()
+            old = stokes.get_guess_space()
+            stokes.set_guess_space(stokes.GUESS_PRECONDITIONED)
+            try:
+                stokes.stokessolver(self.npts, 0, no, 1, 1, self.ibc, 4000, np.zeros(0), np.zeros(0))
+                self._wall[no] = stokes.last_preconditioned(self.npts, 0, no)[:2 * no].copy()
+            finally:
+                stokes.set_guess_space(old)
         return self._wall[no]
 
     def predict(self, thlenden: ThLenDenType, no: int, scale: bool = True) -> Optional[np.ndarray]:
-        """The network's density; with ``scale`` multiplied by the factor
-        alpha minimising ||b - alpha A guess|| (one operator application), so
-        the starting residual is never above that of zeros."""
+        """The network's u0 for every body of the state, with the wall part;
+        with ``scale`` multiplied by the factor alpha that minimises
+        ||b - alpha A P^{-1} u0|| (one preconditioner and one operator
+        application), so the starting residual is never above that of zeros.
+
+        Each body's Fourier modes 0 and 1 of u (its mean and linear ambient
+        flow) set its large rigid-motion density through P^{-1}; they have to
+        be right to about 1e-3 of their spread over the bodies before the
+        starting residual falls well below ||b||, an accuracy this network does
+        not reach."""
         tv = thlenden.thlenvec
         if not tv or len(tv[0].theta) != self.npts:
             return None
@@ -251,14 +272,14 @@ class LearnedGuess:
         Y = mlp_forward(self.params, (F - self.xmean) / self.xstd)
         T = Y.astype(np.float64) * self.ystd + self.ymean
         n, m = self.npts, len(tv)
-        den = assemble_density(T, n, m, no, self.wall_density(no), body_speeds(thlenden))
+        u0 = assemble_solution(T, n, m, no, self.wall_solution(no))
         if scale:
             _, xx, yy = getnxy(thlenden)
+            ag = stokes.matvec(n, m, no, xx, yy, stokes.precond_apply(n, m, no, xx, yy, u0))
             b = wall_rhs(no, m, n, self.ibc)
-            ag = stokes.matvec(n, m, no, xx, yy, den)
             aa = float(ag @ ag)
-            den = den * (float(ag @ b) / aa if aa > 0.0 else 0.0)
-        return den
+            u0 = u0 * (float(ag @ b) / aa if aa > 0.0 else 0.0)
+        return u0
 
     def __call__(self, thlenden: ThLenDenType) -> Optional[np.ndarray]:
         if self.nouter is None:
@@ -266,15 +287,50 @@ class LearnedGuess:
         return self.predict(thlenden, self.nouter)
 
 
-class WarmStart:
-    """The initial guess a time stepper uses: extrapolated from the most
-    recent densities while the body count is unchanged, the trained network's
-    guess otherwise (the first solve of a run, after a deletion), zeros
-    without weights."""
+class PreviousSolution:
+    """Guess from the solutions of the most recent solves, in the
+    preconditioned variable: u_k (``order`` 1) or 2 u_k - u_(k-1) (``order``
+    2), while the body count is unchanged.  ``seen`` reads u of the solve
+    just made (stokes.last_preconditioned), so the solves must run with the
+    preconditioned guess space."""
 
-    def __init__(self, nouter: int, path: str = WEIGHTS, ibc: int = 0, order: int = 2):
-        from .time_stepping import PreviousDensity
-        self.prev = PreviousDensity(order, nouter)
+    def __init__(self, npts: int, nouter: int, order: int = 2):
+        if order not in (1, 2):
+            raise ValueError("order must be 1 or 2")
+        self.npts, self.nouter, self.order = npts, nouter, order
+        self.hist, self.hist_m = [], -1
+
+    def __call__(self, thld: ThLenDenType) -> Optional[np.ndarray]:
+        if not self.hist or self.hist_m != len(thld.thlenvec):
+            return None
+        if self.order == 2 and len(self.hist) == 2:
+            return 2.0 * self.hist[-1] - self.hist[-2]
+        return self.hist[-1]
+
+    def seen(self, thld: ThLenDenType) -> None:
+        den = np.asarray(thld.density)
+        if len(den) == 0:
+            return
+        m = (len(den) - 2 * self.nouter) // (2 * self.npts + 3)
+        u = stokes.last_preconditioned(self.npts, m, self.nouter)
+        if m != len(thld.thlenvec):  # bodies deleted since this solve
+            self.hist, self.hist_m = [], -1
+            return
+        if m != self.hist_m:
+            self.hist = []
+        self.hist = (self.hist + [u])[-2:]
+        self.hist_m = m
+
+
+class WarmStart:
+    """The initial guess a host time stepper uses, in the preconditioned
+    variable: extrapolated from the most recent solutions while the body count
+    is unchanged, the trained network's guess otherwise (the first solve of a
+    run, after a deletion), zeros without weights."""
+
+    def __init__(self, nouter: int, npts: int = 256, path: str = WEIGHTS, ibc: int = 0,
+                 order: int = 2):
+        self.prev = PreviousSolution(npts, nouter, order)
         self.nouter = nouter
         self.net = LearnedGuess(path, nouter=nouter, ibc=ibc) if os.path.exists(path) else None
         self.learned_used = 0

@@ -1,7 +1,7 @@
 """Per-step error of the mixed-precision solve against the FP64 solve.
 
 Runs ``--steps`` erosion steps (``rungekutta2``, ErosionPackage/src/
-TimeStepping.jl:163-176) of a workload twice from the same seeded state:
+TimeStepping.jl:189-202) of a workload twice from the same seeded state:
 
 * the FP64 trajectory: every far field in FP64;
 * the mixed trajectory: the FP32 far field once GMRES's implicit residual is

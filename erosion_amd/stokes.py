@@ -85,6 +85,8 @@ def lib():
         L.erosion_set_guess_space.restype = ctypes.c_int
         L.erosion_last_preconditioned.argtypes = [_dp, ctypes.c_int64]
         L.erosion_last_preconditioned.restype = ctypes.c_int
+        L.erosion_get_guess_space.argtypes = []
+        L.erosion_get_guess_space.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -143,7 +145,8 @@ def last_solve_info() -> dict:
             "true_relres": float(info[2]), "seconds": float(info[3]), "reorth": int(info[4]),
             "ff_targets_per_lane": int(info[5]), "ff_splits": int(info[6]),
             "preconditioner": int(info[7]), "fp32_iters": int(info[8]),
-            "tol_miss": int(info[9]), "tol_misses_total": int(info[10]), "cycles": int(info[11])}
+            "tol_miss": int(info[9]), "tol_misses_total": int(info[10]), "cycles": int(info[11]),
+            "r0_relres": float(info[12])}
 
 
 GUESS_DENSITY, GUESS_PRECONDITIONED = 0, 1
@@ -154,6 +157,10 @@ def set_guess_space(space: int) -> None:
     GUESS_PRECONDITIONED (u0 = P x0 of the right preconditioner)."""
     if lib().erosion_set_guess_space(int(space)) != 0:
         raise StokesError(f"invalid guess space {space}")
+
+
+def get_guess_space() -> int:
+    return int(lib().erosion_get_guess_space())
 
 
 def last_preconditioned(npts: int, nbods: int, nouter: int) -> np.ndarray:
@@ -199,6 +206,23 @@ def computepressure(npts: int, nbods: int, nouter: int, xx, yy, den) -> np.ndarr
                            _p(out))
     _check("computepressure_")
     return out
+
+
+def geometry(npts: int, nbods: int, nouter: int, xx, yy) -> dict:
+    """Normals, curvature, speed (wall first, then bodies) and body centres
+    as the library builds them (test hook)."""
+    xx, yy = _f64(xx), _f64(yy)
+    _lengths(npts, nbods, nouter, xx, yy)
+    np_ = nouter + npts * nbods
+    out = np.zeros(4 * np_ + 2 * nbods)
+    L = lib()
+    L.erosion_geometry.argtypes = [ctypes.c_int64] * 3 + [_dp, _dp, _dp]
+    L.erosion_geometry.restype = ctypes.c_int
+    if L.erosion_geometry(npts, nbods, nouter, _p(xx), _p(yy), _p(out)) != 0:
+        raise StokesError(f"erosion_geometry: {last_error()}")
+    px, py, cur, sp = (out[a * np_:(a + 1) * np_] for a in range(4))
+    return {"px": px, "py": py, "cur": cur, "speed": sp,
+            "cx": out[4 * np_:4 * np_ + nbods], "cy": out[4 * np_ + nbods:]}
 
 
 def time_matvec(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: int = 1,
@@ -357,15 +381,17 @@ def solve_batch(geoms, ibary: int = 1, ibc: int = 0, maxl: int = 8000, workers: 
     pt = P(*[_p(a) for a in taus]) if stress else None
     nits = np.zeros(k, dtype=np.int64)
     status = np.zeros(k, dtype=np.int32)
+    relres = np.zeros(k)
     L = lib()
     L.erosion_solve_batch.restype = ctypes.c_int
     rc = L.erosion_solve_batch(ctypes.c_int64(k), n.ctypes.data_as(_i64p), m.ctypes.data_as(_i64p),
                                no.ctypes.data_as(_i64p), px, py, pd, pt, ctypes.c_int64(ibary),
                                ctypes.c_int64(ibc), ctypes.c_int64(maxl), ctypes.c_int64(workers),
                                nits.ctypes.data_as(_i64p),
-                               status.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+                               status.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(relres))
     out = [dict(den=dens[i], nits=int(nits[i]), status=int(status[i]),
-                tau=taus[i] if stress else None) for i in range(k)]
+                tau=taus[i] if stress else None, true_relres=float(relres[i]))
+           for i in range(k)]
     if rc != 0 and all(o["status"] != 0 for o in out):
         raise StokesError(f"erosion_solve_batch: {last_error()}")
     return out

@@ -1,8 +1,8 @@
 """Erosion time stepping in the theta-L variables (host side).
 
-Restates ErosionPackage/src/TimeStepping.jl: ``tangvel`` (:16-26),
-``getderivs`` (:37-58, :60-69), ``delete_indices`` (:77-100), ``timestep!``
-(:104-159, here ``timestep_``) and ``rungekutta2`` (:163-176).  One erosion
+Restates ErosionPackage/src/TimeStepping.jl: ``tangvel`` (:42-52),
+``getderivs`` (:63-84, :86-95), ``delete_indices`` (:103-126), ``timestep!``
+(:130-185, here ``timestep_``) and ``rungekutta2`` (:189-202).  One erosion
 step is one ``rungekutta2`` call: two density solves and two shear-stress
 evaluations on the GPU, with O(n log n) spectral updates per body on the host
 in between.
@@ -34,7 +34,7 @@ class DerivsType:
 
 
 def tangvel(dtheta: np.ndarray, vnorm: np.ndarray):
-    """Tangential velocity and mterm = dL/dt (TimeStepping.jl:16-26)."""
+    """Tangential velocity and mterm = dL/dt (TimeStepping.jl:42-52)."""
     dthvn = dtheta * vnorm
     mdthvn = float(np.mean(dthvn))
     mterm = -2 * math.pi * mdthvn
@@ -43,7 +43,7 @@ def tangvel(dtheta: np.ndarray, vnorm: np.ndarray):
 
 
 def getderivs_body(thlen: ThetaLenType, params: ParamSet, atau: np.ndarray) -> DerivsType:
-    """Time derivatives of one body (TimeStepping.jl:37-58)."""
+    """Time derivatives of one body (TimeStepping.jl:63-84)."""
     theta, ln, matau = thlen.theta, thlen.len, thlen.matau
     ataueff = atau - float(params.fixarea) * matau
     alpha = getalpha(len(theta))
@@ -81,7 +81,7 @@ def getderivs_bodies(thlenvec, params: ParamSet, atau: np.ndarray) -> List[Deriv
 
 
 def getderivs(thlenden: ThLenDenType, params: ParamSet, den0=None, timings=None) -> List[DerivsType]:
-    """Solve, stress, then per-body derivatives (TimeStepping.jl:60-69)."""
+    """Solve, stress, then per-body derivatives (TimeStepping.jl:86-95)."""
     atau = getstress_(thlenden, params, den0=den0, timings=timings)
     tv = thlenden.thlenvec
     if tv and len({len(t.theta) for t in tv}) == 1:
@@ -90,7 +90,7 @@ def getderivs(thlenden: ThLenDenType, params: ParamSet, den0=None, timings=None)
 
 
 def delete_indices(thld0: ThLenDenType, dvec: List[DerivsType], dt: float) -> List[int]:
-    """Bodies to remove: len <= -sqrt(2) mterm dt or mterm > 10 (TimeStepping.jl:77-100)."""
+    """Bodies to remove: len <= -sqrt(2) mterm dt or mterm > 10 (TimeStepping.jl:103-126)."""
     ndts, mthresh = 1.0, 10.0
     assert len(dvec) == len(thld0.thlenvec)
     out = []
@@ -105,7 +105,7 @@ def delete_indices(thld0: ThLenDenType, dvec: List[DerivsType], dt: float) -> Li
 
 def timestep_(thld0: ThLenDenType, thld_derivs: ThLenDenType, dt1: float, dt2: float,
               params: ParamSet, den0=None, timings=None) -> ThLenDenType:
-    """Advance every body by dt1 with derivatives taken at thld_derivs (TimeStepping.jl:104-159)."""
+    """Advance every body by dt1 with derivatives taken at thld_derivs (TimeStepping.jl:130-185)."""
     dvec = getderivs(thld_derivs, params, den0=den0, timings=timings)
     deletevec = delete_indices(thld0, dvec, dt1)
     for bod in sorted(deletevec, reverse=True):
@@ -125,7 +125,7 @@ def timestep_(thld0: ThLenDenType, thld_derivs: ThLenDenType, dt1: float, dt2: f
 
     if nbods == 0:
         return new_thlenden([])
-    # every body at once; per body this is exactly the loop of TimeStepping.jl:127-152
+    # every body at once; per body this is exactly the loop of TimeStepping.jl:150-182
     t0v, tdv = thld0.thlenvec, thld_derivs.thlenvec
     len0 = np.array([t.len for t in t0v])
     mterm = np.array([d.mterm for d in dvec])
@@ -159,7 +159,7 @@ class PreviousDensity:
     starting residual when its relative error is well below 1e-3).
 
     A state whose bodies were deleted after its solve (timestep! deletes from
-    the state it just solved, TimeStepping.jl:120-126) holds a density of the
+    the state it just solved, TimeStepping.jl:135-141) holds a density of the
     old body list: with ``nouter`` known such a density is recognised by its
     length and empties the history instead of entering it."""
 
@@ -209,7 +209,7 @@ class PreviousDensity:
 def rungekutta2(thld0: ThLenDenType, params: ParamSet,
                 guess: Optional[Callable[[ThLenDenType], Optional[np.ndarray]]] = None,
                 timings=None) -> ThLenDenType:
-    """One erosion step: midpoint RK2 (TimeStepping.jl:163-176).
+    """One erosion step: midpoint RK2 (TimeStepping.jl:189-202).
 
     ``guess(thlenden)`` may return a GMRES initial guess for that state; if it
     has a ``seen`` method it is told about every state whose density was just

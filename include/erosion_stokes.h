@@ -108,11 +108,12 @@ const char* erosion_last_error(void);
  * preconditioner kind, info[8] the iterations whose far field ran in FP32,
  * info[9] 1 if the returned density's true residual is above the tolerance
  * (0 otherwise), info[10] the number of such solves since the library was
- * loaded and info[11] the GMRES cycles run (1 + restarts).  The true residual
+ * loaded, info[11] the GMRES cycles run (1 + restarts) and info[12] the
+ * relative residual ||b - A x0|| / ||b|| of the initial guess.  The true residual
  * is formed with compensated sums (error-free transformations in the far
  * field and the assembly), and a solve whose true residual misses the
  * tolerance restarts from its density, aiming at half the tolerance.
- * `info` holds at least 12 doubles.  Returns 0. */
+ * `info` holds at least 13 doubles.  Returns 0. */
 int erosion_last_solve_info(double* info);
 
 /* Initial guess of stokessolver_ (no counterpart in the reference, whose
@@ -122,6 +123,7 @@ int erosion_last_solve_info(double* info);
  * u0; u = x without a preconditioner).  The density out is x in both cases.
  * Returns 0, or -1 for another space. */
 int erosion_set_guess_space(int space);
+int erosion_get_guess_space(void);
 /* After a solve with guess space 1: its final u (n = N_unk values) into u.
  * Returns 0, or -1 when the last solve did not track u or n differs. */
 int erosion_last_preconditioned(double* u, int64_t n);
@@ -131,6 +133,15 @@ int erosion_last_preconditioned(double* u, int64_t n);
  * (ibary=0, stokesSolver.f:1519-1912).  Returns 0 or -1. */
 int erosion_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t ibary,
                    const double* xx, const double* yy, const double* den, double* vel);
+
+/* Test hook (no counterpart in the reference): the geometry the library
+ * builds from the boundary (inner_geometry / outer_geometry,
+ * stokesSolver.f:136-204) -- out[0..npts) unit normal x, [npts..2npts) unit
+ * normal y, then curvature and speed (npts each; wall points first, then the
+ * bodies body-major), then the body centres x (nbods) and y (nbods);
+ * npts = nouter + npts*nbods here.  Returns 0 or -1. */
+int erosion_geometry(int64_t npts, int64_t nbods, int64_t nouter, const double* xx,
+                     const double* yy, double* out);
 
 /* Measurement hook: set up the geometry once, then time `reps` operator
  * applications with HIP events on the library's stream.
@@ -184,12 +195,14 @@ int erosion_set_mixed_switch(double factor);
  * preconditioner and precision, run on `nworkers` worker contexts (own
  * stream and buffers each) from as many host threads, and overlap on the
  * GPU.  nits[i] and status[i] (0 ok, 1 not converged, 2 breakdown, -1 error)
- * are per geometry.  Returns 0, or -1 if any geometry failed (first failure
- * in erosion_last_error). */
+ * are per geometry; relres[i] (when relres is not null) receives the true
+ * relative residual of geometry i's density.  Returns 0, or -1 if any
+ * geometry failed (first failure in erosion_last_error). */
 int erosion_solve_batch(int64_t ngeom, const int64_t* npts, const int64_t* nbods,
                         const int64_t* nouter, const double* const* xx, const double* const* yy,
                         double* const* den, double* const* tau, int64_t ibary, int64_t ibc,
-                        int64_t maxl, int64_t nworkers, int64_t* nits, int* status);
+                        int64_t maxl, int64_t nworkers, int64_t* nits, int* status,
+                        double* relres);
 
 /* Measurement hook: `reps` device-to-device copies of `bytes` bytes on the
  * library's stream (the reference traffic for hardware byte counters);
@@ -218,20 +231,23 @@ int erosion_get_preconditioner(int* out);
 
 /* One erosion step on the device: the theta-L midpoint RK2 of rungekutta2 /
  * timestep! / getderivs / delete_indices (ErosionPackage/src/TimeStepping.jl:
- * 16-176) with getstress! and the fixed-pressure-drop rescaling
+ * 42-202) with getstress! and the fixed-pressure-drop rescaling
  * (DensityStress.jl:23-36, 129-167), replacing the Julia step that calls
  * stokessolver_, computeshearstress_ and computeqoitargets_ per stage.  The
  * boundary, densities and stresses of both stages stay on the GPU.
  *   params[0] dt, [1] epsilon (epsfac/npts), [2] sigma (sigfac/npts),
  *   [3] fixarea (0/1), [4] fixpdrop (0/1), [5] ibc, [6] ibary (must be 1),
  *   [7] maxl, [8] GMRES guess: 0 zeros (the reference), 1 the previous
- *   density, 2 the linear extrapolation of the last two (kept on the device
- *   across calls while the body count is unchanged).
+ *   solution, 2 the linear extrapolation of the last two.  Solutions are
+ *   kept on the device across calls in the preconditioned variable u = P x
+ *   (the density without a preconditioner); a deletion drops the deleted
+ *   bodies' unknowns from them.
  *   theta (npts x nbods, body-major), len, xsm, ysm: the state at t, replaced
  *   by the state at t + dt of the *nbods_out bodies that remain (bodies are
  *   deleted as delete_indices does).
- *   guess: null, or a density (2 nouter + (2 npts + 3) nbods values) used as
- *   the first stage's GMRES guess when no kept density fits the state.
+ *   guess: null, or u0 (2 nouter + (2 npts + 3) nbods values, the
+ *   preconditioned variable) used as the first stage's GMRES guess when no
+ *   kept solution fits the state.
  *   info[0..9]: GMRES iterations of stage 1 and 2, bodies deleted, umax of
  *   stage 1 and 2, largest true relative residual, positive-mterm count,
  *   flux-mismatch count, solves started from kept densities, solves started
@@ -241,7 +257,7 @@ int erosion_get_preconditioner(int* out);
 int erosion_rungekutta2(int64_t npts, int64_t nbods, int64_t nouter, const double* params,
                         double* theta, double* len, double* xsm, double* ysm,
                         const double* guess, int64_t* nbods_out, double* info);
-/* out[0] densities kept for guesses (0-2), out[1] their body count, out[2]
+/* out[0] solutions kept for guesses (0-2), out[1] their body count, out[2]
  * their length.  Returns 0 or -1. */
 int erosion_stepper_history(int64_t* out);
 /* Forget the densities erosion_rungekutta2 keeps for its GMRES guesses (a new

@@ -46,6 +46,8 @@ def lib():
         L.orc_geom_free.argtypes = [ctypes.c_void_p]
         L.orc_geom_array.restype = _dp
         L.orc_geom_array.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_geom_set_array.restype = ctypes.c_int
+        L.orc_geom_set_array.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         L.orc_fourier_diff.argtypes = [ctypes.c_int, _dp, _dp]
         L.orc_rhs.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp]
         L.orc_matvec.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp]
@@ -110,6 +112,16 @@ class Geometry:
         ptr = lib().orc_geom_array(self._h, which)
         return np.ctypeslib.as_array(ptr, shape=(size,)).copy() if size else np.zeros(0)
 
+    def set_array(self, name: str, values) -> None:
+        """Overwrite one geometry array (same name and length as ``array``)."""
+        which = self.NAMES.index(name)
+        v = _f64(values)
+        size = {"cx": self.m, "cy": self.m}.get(name, self.n * self.m if which < 4 else self.no)
+        if len(v) != size:
+            raise ValueError(f"{name} holds {size} values, got {len(v)}")
+        if lib().orc_geom_set_array(self._h, which, _p(v)) != 0:
+            raise RuntimeError("orc_geom_set_array failed")
+
     def rhs(self, ibc: int) -> np.ndarray:
         out = np.zeros(self.nunk)
         lib().orc_rhs(self._h, int(ibc), _p(out))
@@ -129,6 +141,15 @@ class Geometry:
         it = lib().orc_gmres(self._h, int(ibary), _p(rhs), _p(x), int(maxl), float(tol),
                              ctypes.byref(err), ctypes.byref(ierr))
         return x, int(it), float(err.value), int(ierr.value)
+
+    def shear_stress(self, den) -> np.ndarray:
+        """computeShearStress (ib=1) on this geometry: tau = t . (p_y, -p_x),
+        t = -2 E n (stokesSolver.f:3075-3091)."""
+        e11, e12, e22 = self.deformation(den)
+        px, py = self.array("px"), self.array("py")
+        tx = -2.0 * e11 * px - 2.0 * e12 * py
+        ty = -2.0 * e12 * px - 2.0 * e22 * py
+        return tx * py - ty * px
 
     def deformation(self, den):
         den = _f64(den)

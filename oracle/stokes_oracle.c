@@ -232,6 +232,21 @@ const double *orc_geom_array(const orc_geom *g, int which) {
     }
 }
 
+/* Overwrite one geometry array with caller values (the same `which` and
+ * length as orc_geom_array).  Lets a test evaluate this restatement's
+ * operator on exactly another implementation's normals, curvature and
+ * speed, so that their difference measures the operator code alone. */
+int orc_geom_set_array(orc_geom *g, int which, const double *v) {
+    size_t len;
+    double *dst = (double *)orc_geom_array(g, which);
+    if (!dst || !v) return -1;
+    if (which < 4) len = (size_t)g->n * (size_t)g->m;
+    else if (which < 6) len = (size_t)g->m;
+    else len = (size_t)g->no;
+    memcpy(dst, v, len * sizeof(double));
+    return 0;
+}
+
 /* bd_condition + bgFlow (stokesSolver.f:208-314, 392-413): the live branch
  * sets wall rows to the background flow and every other row to zero. */
 void orc_rhs(const orc_geom *g, int ibc, double *rhs) {
@@ -253,8 +268,16 @@ void orc_rhs(const orc_geom *g, int ibc, double *rhs) {
 /* ---------------------------------------------------- trapezoid kernels */
 /* One source node of the trapezoid double layer as written in matvec_DLP
  * (stokesSolver.f:1570-1579). */
+/* The operator's sums are accumulated in x87 extended precision (64-bit
+ * significand): with bodies whose densities are 10^4..10^6 times the
+ * right-hand side, the running sums of a target's pair terms are that much
+ * larger than the result, and FP64 accumulation alone would leave the operator
+ * uncertain at ~1e-8 of ||b||; each pair term is still formed in FP64 as the
+ * reference forms it. */
+typedef long double acc_t;
+
 static inline void trap_pair(double rx, double ry, double nx, double ny, double dx, double dy,
-                             double sp, double *ux, double *uy) {
+                             double sp, acc_t *ux, acc_t *uy) {
     double rdotn = rx * nx + ry * ny;
     double rdotden = rx * dx + ry * dy;
     double rho2 = rx * rx + ry * ry;
@@ -528,7 +551,7 @@ void orc_matvec(const orc_geom *g, int ibary, const double *den, double *vel) {
     const size_t nunk = (size_t)(2 * no + 2 * n * m + 3 * m);
     const int nbp = n * m;
     const int ntgt = no + nbp;
-    memset(vel, 0, sizeof(double) * nunk);
+    acc_t *acc = xcalloc(nunk, sizeof(acc_t));
     const double *wdx = den, *wdy = den + no;
 
     /* Far field: trapezoid rule over every source node except the target
@@ -547,8 +570,8 @@ void orc_matvec(const orc_geom *g, int ibary, const double *den, double *vel) {
             xt = g->x[tb * n + tk];
             yt = g->y[tb * n + tk];
         }
-        double ut = 0.0, vt = 0.0;
-        double ux = 0.0, uy = 0.0;
+        acc_t ut = 0.0, vt = 0.0;
+        acc_t ux = 0.0, uy = 0.0;
         for (int j = 0; j < no; j++) {
             if (t < no && j == t) continue;
             trap_pair(xt - g->xo[j], yt - g->yo[j], g->px0[j], g->py0[j], wdx[j], wdy[j],
@@ -570,11 +593,11 @@ void orc_matvec(const orc_geom *g, int ibary, const double *den, double *vel) {
             vt += uy * TWOPI / (double)n / PI;
         }
         if (t < no) {
-            vel[t] = ut;
-            vel[t + no] = vt;
+            acc[t] = ut;
+            acc[t + no] = vt;
         } else {
-            vel[2 * no + 2 * tb * n + tk] = ut;
-            vel[2 * no + 2 * tb * n + n + tk] = vt;
+            acc[2 * no + 2 * tb * n + tk] = ut;
+            acc[2 * no + 2 * tb * n + n + tk] = vt;
         }
     }
 
@@ -615,14 +638,14 @@ void orc_matvec(const orc_geom *g, int ibary, const double *den, double *vel) {
             for (int k = 0; k < n; k++) {
                 int q = it * n + k;
                 if (!(fabs(g->y[q]) >= dsum2)) continue;
-                double sx = 0.0, sy = 0.0;
+                acc_t sx = 0.0, sy = 0.0;
                 for (int j = 0; j < no; j++)
                     trap_pair(g->x[q] - g->xo[j], g->y[q] - g->yo[j], g->px0[j], g->py0[j],
                               wdx[j], wdy[j], g->speed0[j], &sx, &sy);
                 sx = sx * TWOPI / (double)no / PI;
                 sy = sy * TWOPI / (double)no / PI;
-                vel[2 * no + 2 * it * n + k] -= sx;
-                vel[2 * no + 2 * it * n + n + k] -= sy;
+                acc[2 * no + 2 * it * n + k] -= sx;
+                acc[2 * no + 2 * it * n + n + k] -= sy;
                 wxt[mw] = g->x[q];
                 wyt[mw] = g->y[q];
                 wtgt[mw] = q;
@@ -643,7 +666,7 @@ void orc_matvec(const orc_geom *g, int ibary, const double *den, double *vel) {
                 }
             }
             for (int k = 0; k < nw; k++) {
-                double sx = 0.0, sy = 0.0;
+                acc_t sx = 0.0, sy = 0.0;
                 for (int j = 0; j < n; j++) {
                     int q = it * n + j;
                     trap_pair(xtar[k] - g->x[q], ytar[k] - g->y[q], g->px[q], g->py[q], bdx[j],
@@ -651,16 +674,16 @@ void orc_matvec(const orc_geom *g, int ibary, const double *den, double *vel) {
                 }
                 sx = sx * TWOPI / (double)n / PI;
                 sy = sy * TWOPI / (double)n / PI;
-                vel[itgt[k]] -= sx;
-                vel[itgt[k] + no] -= sy;
+                acc[itgt[k]] -= sx;
+                acc[itgt[k] + no] -= sy;
             }
             if (nw > 0) {
                 orc_stokes_exterior_dlp(n, g->x + it * n, g->y + it * n, bdx, bdy,
                                         g->px + it * n, g->py + it * n, nw, xtar, ytar, 1, ux,
                                         uy, NULL, NULL, NULL, NULL);
                 for (int k = 0; k < nw; k++) {
-                    vel[itgt[k]] += ux[k];
-                    vel[itgt[k] + no] += uy[k];
+                    acc[itgt[k]] += ux[k];
+                    acc[itgt[k] + no] += uy[k];
                 }
             }
         }
@@ -669,8 +692,8 @@ void orc_matvec(const orc_geom *g, int ibary, const double *den, double *vel) {
                                     ux, uy, NULL, NULL, NULL, NULL);
             for (int k = 0; k < mw; k++) {
                 int b = wtgt[k] / n, kk = wtgt[k] % n;
-                vel[2 * no + 2 * b * n + kk] += ux[k];
-                vel[2 * no + 2 * b * n + n + kk] += uy[k];
+                acc[2 * no + 2 * b * n + kk] += ux[k];
+                acc[2 * no + 2 * b * n + n + kk] += uy[k];
             }
         }
         /* body -> body (stokesSolver.f:838-926) */
@@ -695,7 +718,7 @@ void orc_matvec(const orc_geom *g, int ibary, const double *den, double *vel) {
                     ytar[l] = g->y[q];
                     itgt[l] = q;
                     l++;
-                    double sx = 0.0, sy = 0.0;
+                    acc_t sx = 0.0, sy = 0.0;
                     for (int j = 0; j < n; j++) {
                         int p = is * n + j;
                         trap_pair(g->x[q] - g->x[p], g->y[q] - g->y[p], g->px[p], g->py[p],
@@ -703,8 +726,8 @@ void orc_matvec(const orc_geom *g, int ibary, const double *den, double *vel) {
                     }
                     sx = sx * TWOPI / (double)n / PI;
                     sy = sy * TWOPI / (double)n / PI;
-                    vel[2 * no + 2 * it * n + k] -= sx;
-                    vel[2 * no + 2 * it * n + n + k] -= sy;
+                    acc[2 * no + 2 * it * n + k] -= sx;
+                    acc[2 * no + 2 * it * n + n + k] -= sy;
                 }
             }
             if (l > 0) {
@@ -713,8 +736,8 @@ void orc_matvec(const orc_geom *g, int ibary, const double *den, double *vel) {
                                         NULL, NULL, NULL, NULL);
                 for (int k = 0; k < l; k++) {
                     int b = itgt[k] / n, kk = itgt[k] % n;
-                    vel[2 * no + 2 * b * n + kk] += ux[k];
-                    vel[2 * no + 2 * b * n + n + kk] += uy[k];
+                    acc[2 * no + 2 * b * n + kk] += ux[k];
+                    acc[2 * no + 2 * b * n + n + kk] += uy[k];
                 }
             }
         }
@@ -733,18 +756,18 @@ void orc_matvec(const orc_geom *g, int ibary, const double *den, double *vel) {
      * (stokesSolver.f:932-951) and on every body (:958-987). */
     for (int k = 0; k < no; k++) {
         double tdotden = g->py0[k] * wdx[k] - g->px0[k] * wdy[k];
-        vel[k] = vel[k] - g->cur0[k] * tdotden * g->py0[k] * g->speed0[k] / (double)no;
-        vel[k + no] = vel[k + no] + g->cur0[k] * tdotden * g->px0[k] * g->speed0[k] / (double)no;
+        acc[k] = acc[k] - g->cur0[k] * tdotden * g->py0[k] * g->speed0[k] / (double)no;
+        acc[k + no] = acc[k + no] + g->cur0[k] * tdotden * g->px0[k] * g->speed0[k] / (double)no;
     }
     for (int k = 0; k < no; k++) {
-        vel[k] = vel[k] - 0.5 * wdx[k];
-        vel[k + no] = vel[k + no] - 0.5 * wdy[k];
+        acc[k] = acc[k] - 0.5 * wdx[k];
+        acc[k + no] = acc[k + no] - 0.5 * wdy[k];
     }
     for (int s = 0; s < m; s++) {
         const double *sdx = den + 2 * no + 2 * s * n;
         const double *sdy = sdx + n;
-        double *vx = vel + 2 * no + 2 * s * n;
-        double *vy = vx + n;
+        acc_t *vx = acc + 2 * no + 2 * s * n;
+        acc_t *vy = vx + n;
         for (int k = 0; k < n; k++) {
             int q = s * n + k;
             double tdotden = g->py[q] * sdx[k] - g->px[q] * sdy[k];
@@ -763,17 +786,17 @@ void orc_matvec(const orc_geom *g, int ibary, const double *den, double *vel) {
         double sto1 = ext[3 * b], sto2 = ext[3 * b + 1], rot = ext[3 * b + 2];
         for (int t = 0; t < ntgt; t++) {
             double xt, yt;
-            double *vx, *vy;
+            acc_t *vx, *vy;
             if (t < no) {
                 xt = g->xo[t];
                 yt = g->yo[t];
-                vx = vel + t;
-                vy = vel + t + no;
+                vx = acc + t;
+                vy = acc + t + no;
             } else {
                 int tb = (t - no) / n, tk = (t - no) % n;
                 xt = g->x[tb * n + tk];
                 yt = g->y[tb * n + tk];
-                vx = vel + 2 * no + 2 * tb * n + tk;
+                vx = acc + 2 * no + 2 * tb * n + tk;
                 vy = vx + n;
             }
             double rx = xt - g->cx[b], ry = yt - g->cy[b];
@@ -792,7 +815,7 @@ void orc_matvec(const orc_geom *g, int ibary, const double *den, double *vel) {
     for (int b = 0; b < m; b++) {
         const double *sdx = den + 2 * no + 2 * b * n;
         const double *sdy = sdx + n;
-        double *c = vel + 2 * no + 2 * nbp + 3 * b;
+        acc_t *c = acc + 2 * no + 2 * nbp + 3 * b;
         for (int k = 0; k < n; k++) {
             int q = b * n + k;
             c[0] = c[0] - sdx[k] * g->speed[q];
@@ -806,14 +829,17 @@ void orc_matvec(const orc_geom *g, int ibary, const double *den, double *vel) {
     }
 
     /* Rank-one completion on the wall rows (stokesSolver.f:1102-1118). */
-    double sigdotn = 0.0;
+    acc_t sigdotn = 0.0;
     for (int k = 0; k < no; k++) sigdotn = sigdotn + (g->px0[k] * wdx[k] + g->py0[k] * wdy[k]) * g->speed0[k];
     sigdotn = sigdotn * TWOPI / (double)no;
     for (int k = 0; k < no; k++) {
-        vel[k] = vel[k] + sigdotn * g->px0[k];
-        vel[k + no] = vel[k + no] + sigdotn * g->py0[k];
+        acc[k] = acc[k] + sigdotn * g->px0[k];
+        acc[k + no] = acc[k + no] + sigdotn * g->py0[k];
     }
+    for (size_t i = 0; i < nunk; i++) vel[i] = (double)acc[i];
+    free(acc);
 }
+
 
 /* ------------------------------------------------------------------ GMRES */
 /* DNRM2 (gmres_routines.f, reference BLAS with scaling). */

@@ -33,6 +33,10 @@ void orc_geom_free(orc_geom *g);
 /* which: 0 px 1 py 2 cur 3 speed 4 cx 5 cy 6 xo 7 yo 8 px0 9 py0 10 cur0 11 speed0 */
 const double *orc_geom_array(const orc_geom *g, int which);
 
+/* Overwrite array `which` (as orc_geom_array, same length) with v; returns
+ * 0 or -1.  Test hook: evaluate on another implementation's geometry. */
+int orc_geom_set_array(orc_geom *g, int which, const double *v);
+
 /* fourierDiff (stokesSolver.f:348-388) on a complex vector stored re/im. */
 void orc_fourier_diff(int n, double *re, double *im);
 

@@ -88,7 +88,7 @@ def test_getderivs_batched_equals_per_body():
 
 def test_rk2_step_matches_per_body_formulas(monkeypatch):
     """One RK2 step with a fixed synthetic stress: the batched update equals
-    the per-body loop of TimeStepping.jl:127-152 evaluated here directly."""
+    the per-body loop of TimeStepping.jl:150-182 evaluated here directly."""
     thl = make_bodies(4, 64, 0.05, 0.1, 0.05, 12)
     params = ParamSet(npts=64, nouter=256, dt=1e-3)
     stress = {}

@@ -69,16 +69,18 @@ def test_planted_oracle_mistake_exceeds_gpu_tolerance():
     assert np.linalg.norm(gbad - g) / np.linalg.norm(g) > 0.05
 
 
-def test_targets_roundtrip_band_limited_density():
+def test_targets_roundtrip_band_limited_solution():
     from erosion_amd import learned as LR
     n, m, no = 64, 3, 32
     rng = np.random.default_rng(0)
     T = rng.standard_normal((m, LR.NOUT))
     wall = rng.standard_normal(2 * no)
-    sp = rng.random(m) + 0.01
-    den = LR.assemble_density(T, n, m, no, wall, sp)
-    assert np.abs(LR.density_targets(den, n, m, no, sp) - T).max() < 1e-10
-    assert np.array_equal(den[:2 * no], wall)
+    u = LR.assemble_solution(T, n, m, no, wall)
+    assert np.abs(LR.solution_targets(u, n, m, no) - T).max() < 1e-10
+    assert np.array_equal(u[:2 * no], wall)
+    # body b's x part is its Fourier series: mode-0 coefficient / sqrt-free mean
+    o = 2 * no
+    assert abs(np.mean(u[o:o + n]) - T[0, 0]) < 1e-12
 
 
 def test_features_shape_and_neighbours():
@@ -154,10 +156,14 @@ def test_gpu_training_fits_a_smooth_map():
 @pytest.mark.skipif(not os.path.exists(
     os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
                  "erosion_amd", "data", "warmstart_mlp.npz")), reason="weights not trained")
-def test_learned_guess_gain_held_out():
-    """Held-out 100-body geometry (seed outside the training runs): GMRES at
-    1e-10 from the learned guess versus from zeros, with and without the
-    preconditioner; the guess only moves the iteration count."""
+def test_learned_guess_never_above_zeros_held_out():
+    """Held-out 100-body geometry (seed outside the training runs),
+    preconditioned GMRES at 1e-10 in the preconditioned variable: the
+    network's guess, scaled by the factor that minimises its residual, starts
+    no higher than zeros and costs at most two iterations more.  (The
+    unscaled guess starts far above ||b||: each body's rigid-motion part of u
+    would have to be right to about 1e-3, which the network is not.)  The
+    test records what the guess does; it does not claim a gain."""
     from conftest import relerr
     from erosion_amd import learned as LR
     from erosion_amd import stokes
@@ -167,33 +173,22 @@ def test_learned_guess_gain_held_out():
     g = LR.LearnedGuess()
     th = new_thlenden(make_bodies(100, n, 0.02, 0.05, 0.02, 2300))
     m, xx, yy = getnxy(th)
-    guess = g.predict(th, no)
     stokes.set_gmres_tol(1e-10)
+    stokes.set_preconditioner(stokes.PRECOND_CURVE_FOURIER)
+    stokes.set_guess_space(stokes.GUESS_PRECONDITIONED)
     try:
-        b = LR.wall_rhs(no, m, n)
-        stokes.set_preconditioner(0)
-        rhs_res = stokes.matvec(n, m, no, xx, yy, guess)
-        res = {}
-        for pc in (0, 1):
-            stokes.set_preconditioner(pc)
-            d0, it0 = stokes.stokessolver(n, m, no, 1, 1, 0, 8000, xx, yy)
-            d1, it1 = stokes.stokessolver(n, m, no, 1, 1, 0, 8000, xx, yy, den=guess)
-            info = stokes.last_solve_info()
-            assert stokes.last_status() == 0 and info["implicit_relres"] <= 1e-10
-            # the reference's GMRES stops on the implicit estimate; the
-            # preconditioned path also checks the true residual
-            assert info["true_relres"] < (1e-10 if pc else 1e-9)
-            assert relerr(d1, d0) < 1e-6
-            res[pc] = (it0, it1)
-        # the guess's own residual against the right-hand side of the solve
-        r0 = np.linalg.norm(b - rhs_res) / np.linalg.norm(b)
-        print(f"learned guess: initial relative residual {r0:.3e}; iterations zeros/learned "
-              f"plain {res[0]}, preconditioned {res[1]}")
-        # what the network gains: a starting residual below that of zeros (1)
-        # after the residual-minimising scale; the iteration counts it leads
-        # to are printed above and recorded with the trained weights
-        assert r0 < 1.0
-        assert res[0][1] <= res[0][0] + 5 and res[1][1] <= res[1][0] + 5
+        d0, it0 = stokes.stokessolver(n, m, no, 1, 1, 0, 8000, xx, yy)
+        raw = g.predict(th, no, scale=False)
+        _, itr = stokes.stokessolver(n, m, no, 1, 1, 0, 8000, xx, yy, den=raw)
+        r_raw = stokes.last_solve_info()["r0_relres"]
+        d1, it1 = stokes.stokessolver(n, m, no, 1, 1, 0, 8000, xx, yy, den=g.predict(th, no))
+        info = stokes.last_solve_info()
+        print(f"learned guess: raw starting residual {r_raw:.3e} ({itr} iterations), scaled "
+              f"{info['r0_relres']:.3e} ({it1}); zeros {it0}")
+        assert stokes.last_status() == 0 and info["true_relres"] <= 1e-10
+        assert relerr(d1, d0) < 1e-7
+        assert info["r0_relres"] <= 1.0 and it1 <= it0 + 2
     finally:
-        stokes.set_preconditioner(0)
+        stokes.set_guess_space(stokes.GUESS_DENSITY)
+        stokes.set_preconditioner(stokes.PRECOND_NONE)
         stokes.set_gmres_tol(1e-8)
