@@ -34,6 +34,11 @@ int record_error(const char* where, const std::string& msg) {
     return -1;
 }
 
+std::string& noalloc_error() {
+    static std::string e;
+    return e;
+}
+
 void check_sizes(int64_t n, int64_t m, int64_t no) {
     if (n < 2 || m < 0 || no < 4)
         throw stokes::Error("invalid sizes npts=" + std::to_string(n) + " nbods=" +
@@ -668,6 +673,25 @@ int erosion_mlp_forward(int nin, int nhid, int nlayers, int nout, const float* p
         return record_error("erosion_mlp_forward", e.what());
     }
 }
+
+int erosion_mlp_forward_noalloc(int nin, int nhid, int nlayers, int nout, const float* params,
+                                int64_t nsamples, const float* x, float* y, void* stream) {
+    // no library context here (creating one would create a stream): errors
+    // go to stderr and to the message erosion_noalloc_last_error returns
+    try {
+        if (!params || (nsamples > 0 && (!x || !y))) throw stokes::Error("null argument");
+        if (nsamples < 0 || nsamples > (int64_t{1} << 30)) throw stokes::Error("invalid sample count");
+        stokes::mlp_forward_noalloc(nin, nhid, nlayers, nout, params, static_cast<int>(nsamples), x,
+                                    y, static_cast<hipStream_t>(stream));
+        return 0;
+    } catch (const std::exception& e) {
+        noalloc_error() = std::string("erosion_mlp_forward_noalloc: ") + e.what();
+        std::fprintf(stderr, "libstokes_hip: %s\n", noalloc_error().c_str());
+        return -1;
+    }
+}
+
+const char* erosion_noalloc_last_error(void) { return noalloc_error().c_str(); }
 
 int erosion_mlp_gradients(int nin, int nhid, int nlayers, int nout, const float* params,
                           int64_t nsamples, const float* x, const float* t, float* grads,

@@ -391,6 +391,11 @@ void solve_batch(Context& settings, int ngeom, const BatchGeometry* geo, int iba
 size_t mlp_param_count(int nin, int nhid, int nlayers, int nout);
 void mlp_forward(Context& c, int nin, int nhid, int nlayers, int nout, const float* params,
                  int nsamples, const float* x, float* y);
+// The same forward pass with every pointer in device memory, launched on
+// stream s, with no device allocation: hidden activations in a file-scope
+// device array of mlp.hip (shared by every call on a GPU).
+void mlp_forward_noalloc(int nin, int nhid, int nlayers, int nout, const float* params_dev,
+                         int nsamples, const float* x_dev, float* y_dev, hipStream_t s);
 double mlp_gradients(Context& c, int nin, int nhid, int nlayers, int nout, const float* params,
                      int nsamples, const float* x, const float* t, float* grads);
 void mlp_train(Context& c, int nin, int nhid, int nlayers, int nout, float* params, int nsamples,

@@ -17,7 +17,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <map>
+#include <mutex>
 #include <random>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -245,6 +248,36 @@ void check_shape(int nin, int nhid, int nlayers, int nout) {
 
 }  // namespace
 
+namespace {
+
+// ----------------------------------------------- allocation-free forward
+// Hidden activations of mlp_forward_noalloc live in this file-scope array
+// (8 MiB on every GPU the library runs on, held until the process ends):
+// two halves of kScratchFloats / 2 floats, layer l writing one half and
+// reading the other, rows of samples processed in chunks that fit a half.
+constexpr size_t kScratchFloats = size_t{1} << 21;
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+std::mutex g_scratch_mu;
+std::map<int, float*> g_scratch_ptr;  // device -> address of `scratch` there
+
+// The device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy of the array).
+float* scratch_on_current_device() {
+    int dev = 0;
+    STOKES_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) return it->second;
+    void* p = nullptr;
+    STOKES_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error("no device address for the forward pass's scratch array");
+    g_scratch_ptr.emplace(dev, static_cast<float*>(p));
+    return static_cast<float*>(p);
+}
+
+}  // namespace
+
 size_t mlp_param_count(int nin, int nhid, int nlayers, int nout) {
     check_shape(nin, nhid, nlayers, nout);
     size_t total = 0;
@@ -272,6 +305,38 @@ void mlp_forward(Context& c, int nin, int nhid, int nlayers, int nout, const flo
                                   hipMemcpyDeviceToHost, s));
     }
     STOKES_HIP(hipStreamSynchronize(s));
+}
+
+void mlp_forward_noalloc(int nin, int nhid, int nlayers, int nout, const float* params_dev,
+                         int nsamples, const float* x_dev, float* y_dev, hipStream_t s) {
+    check_shape(nin, nhid, nlayers, nout);
+    if (nsamples < 0) throw Error("negative sample count");
+    const size_t half = kScratchFloats / 2;
+    // a chunk holds at least one 32-row tile pair of hidden activations
+    if (static_cast<size_t>(nhid) * 2 * kT > half)
+        throw Error("hidden width " + std::to_string(nhid) + " exceeds the forward scratch (" +
+                    std::to_string(half / (2 * kT)) + " units at most)");
+    float* base = scratch_on_current_device();
+    if (nsamples == 0) return;
+    size_t total = 0;
+    const std::vector<Layer> L = layout(nin, nhid, nlayers, nout, &total);
+    // rows per chunk: as many as a half holds, a multiple of the 32-row tile
+    const int rows = static_cast<int>(std::min<size_t>(half / nhid / (2 * kT) * (2 * kT),
+                                                       static_cast<size_t>(1) << 20));
+    if (rows < 2 * kT) throw Error("forward scratch holds no row chunk");
+    float* buf[2] = {base, base + half};
+    for (int b0 = 0; b0 < nsamples; b0 += rows) {
+        const int nb = std::min(rows, nsamples - b0);
+        const float* in = x_dev + static_cast<size_t>(b0) * nin;
+        for (int l = 0; l <= nlayers; ++l) {
+            const Layer& y = L[static_cast<size_t>(l)];
+            const bool last = l == nlayers;
+            float* out = last ? y_dev + static_cast<size_t>(b0) * nout : buf[l & 1];
+            gemm(s, nb, y.nout, y.nin, in, y.nin, 1, params_dev + y.w, 1, y.nin, out, y.nout,
+                 params_dev + y.b, last ? 1 : 2);
+            in = out;
+        }
+    }
 }
 
 double mlp_gradients(Context& c, int nin, int nhid, int nlayers, int nout, const float* params,

@@ -310,6 +310,21 @@ int erosion_precond_apply(int64_t npts, int64_t nbods, int64_t nouter, const dou
 int erosion_mlp_param_count(int nin, int nhid, int nlayers, int nout, int64_t* count);
 int erosion_mlp_forward(int nin, int nhid, int nlayers, int nout, const float* params,
                         int64_t nsamples, const float* x, float* y);
+/* The forward pass of erosion_mlp_forward with no allocation, for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving
+ * loops that forbid hipMalloc).  No counterpart in the reference.  params,
+ * x and y are DEVICE pointers (same layouts); the kernels run on `stream` (a
+ * hipStream_t; null = the default stream) and the call returns once they are
+ * enqueued.  Hidden activations live in a file-scope device array of 8 MiB
+ * per GPU (held until the process ends): samples run in chunks of up to
+ * 2^20 / nhid rows (rounded down to a multiple of 32), and a hidden width
+ * above 32768 units is refused.  Every call on one GPU shares that array, so
+ * calls that overlap on one GPU (other streams, other threads) race.  Errors
+ * go to stderr and erosion_noalloc_last_error (this entry point does not
+ * create the library context).  Returns 0 or -1. */
+int erosion_mlp_forward_noalloc(int nin, int nhid, int nlayers, int nout, const float* params,
+                                int64_t nsamples, const float* x, float* y, void* stream);
+const char* erosion_noalloc_last_error(void);
 /* Gradient of the loss over all nsamples with respect to params (same
  * layout), and the loss. */
 int erosion_mlp_gradients(int nin, int nhid, int nlayers, int nout, const float* params,
