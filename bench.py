@@ -58,7 +58,7 @@ STRESS_FLOP_PER_PAIR = 130.0  # per (target body point, source node) pair: 3 pas
 # shape.  The bytes written equal the per-split partial sums (a WRITE_SIZE
 # pass reads exactly splits x points x 16 B), so the traffic of the shape in
 # use is this plus that.
-FF_FETCH_BYTES = {"bodies100_n256": 10.5e6}
+FF_FETCH_BYTES = {"bodies100_n256": 10.5e6, "bodies400_n256_eroded50": 45.9e6}
 
 
 def log(*a):
