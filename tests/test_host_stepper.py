@@ -72,6 +72,29 @@ def test_getnxy_batched_equals_per_body():
     assert np.array_equal(x, np.concatenate(xs)) and np.array_equal(y, np.concatenate(ys))
 
 
+def test_getderivs_circle_closed_form():
+    """A circle under the stress tau0 + tau1 cos(alpha) (alpha the position
+    angle): erosion at that inward normal speed keeps it a circle to first
+    order, shrinks its length at dL/dt = -2 pi tau0 and moves its centre by
+    -tau1 along x (the mode-1 part of the normal speed shifts the circle), so
+    getderivs / tangvel (TimeStepping.jl:42-84) must give mterm = -2 pi tau0,
+    nterm = 0, xsmdot = -tau1, ysmdot = 0; with fixarea the mean stress is
+    removed and the length stays."""
+    n, rad = 128, 0.3
+    t = circ2thlen(n, rad, 0.1, -0.2)
+    t.matau = 0.7
+    alpha = getalpha(n)
+    for tau0, tau1 in ((0.7, 0.0), (0.7, 0.25), (1.3, -0.4)):
+        atau = tau0 + tau1 * np.cos(alpha)
+        for fixarea in (0, 1):
+            params = ParamSet(npts=n, nouter=256, fixarea=fixarea, epsfac=15)
+            d = TS.getderivs_body(t, params, atau)
+            mean_tau = tau0 - fixarea * t.matau
+            assert abs(d.mterm - (-2 * math.pi * mean_tau)) < 1e-13
+            assert np.abs(d.nterm).max() < 1e-12
+            assert abs(d.xsmdot - (-tau1)) < 1e-13 and abs(d.ysmdot) < 1e-13
+
+
 def test_getderivs_batched_equals_per_body():
     thl = make_bodies(6, 64, 0.05, 0.1, 0.05, 4)
     rng = np.random.default_rng(5)
