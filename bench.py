@@ -59,6 +59,9 @@ STRESS_FLOP_PER_PAIR = 130.0  # per (target body point, source node) pair: 3 pas
 # pass reads exactly splits x points x 16 B), so the traffic of the shape in
 # use is this plus that.
 FF_FETCH_BYTES = {"bodies100_n256": 10.5e6, "bodies400_n256_eroded50": 45.9e6}
+# HBM bytes per k_fmm_eval launch (FETCH_SIZE x 2.00 + WRITE_SIZE), from
+# rocprofv3 counter passes on the same workload
+FMM_EVAL_BYTES = {}
 
 
 def log(*a):
@@ -315,20 +318,27 @@ def main() -> int:
         steps, ideal seconds per operator application)."""
         k = r["kt"]
         scale = r["steps"] if r["graphs"] else 1
-        t64 = FF_FLOP_PER_PAIR * k["farfield"]["work"] / (PEAK_FP64_TFLOPS * 1e12)
+        # pairs summed directly: the direct far field, and the direct parts
+        # of the body-cluster far field (its expansions are not counted)
+        pairs64 = k["farfield"]["work"] + k["fmm_direct"]["work"] + k["fmm_wall"]["work"]
+        t64 = FF_FLOP_PER_PAIR * pairs64 / (PEAK_FP64_TFLOPS * 1e12)
         t32 = FF_FLOP_PER_PAIR * k["farfield32"]["work"] / (PEAK_FP32_TFLOPS * 1e12)
         tst = STRESS_FLOP_PER_PAIR * k["stress_eval"]["work"] / (PEAK_FP64_TFLOPS * 1e12)
-        nmv = k["farfield"]["launches"] + k["farfield32"]["launches"]
-        return scale * (t64 + t32 + tst), (t64 + t32) / max(1, nmv)
+        return scale * (t64 + t32 + tst), (t64 + t32) / max(1, applications(k))
+
+    def applications(k):
+        """Operator applications: direct far-field launches (FP64, FP32) and
+        body-cluster far-field applications."""
+        return k["farfield"]["launches"] + k["farfield32"]["launches"] + k["fmm"]["launches"]
 
     def summary(r):
         solves = max(1, r["timings"].get("solves", 1))
         ideal_s, ideal_mv = roofline_of_run(r)
-        mvps = ((r["kt"]["farfield"]["launches"] + r["kt"]["farfield32"]["launches"])
-                * (r["steps"] if r["graphs"] else 1) / r["elapsed"])
+        mvps = applications(r["kt"]) * (r["steps"] if r["graphs"] else 1) / r["elapsed"]
         return {"steps_per_s": r["steps"] / r["elapsed"],
                 "steps_timed": r["steps"],
-                # the same steps with every far-field and stress FLOP at peak
+                # the same steps with every directly summed far-field pair
+                # and every stress pair at peak
                 "steps_per_s_at_peak": r["steps"] / ideal_s if ideal_s > 0 else None,
                 "steps_roofline_frac": ideal_s / r["elapsed"],
                 "matvecs_per_s_at_peak": 1.0 / ideal_mv if ideal_mv > 0 else None,
@@ -418,6 +428,21 @@ def main() -> int:
                  if se["ms"] > 0 else None)
     nr = kt["near"]
     near_gbs = nr["work"] / (nr["ms"] * 1e-3) / 1e9 if nr["ms"] > 0 else None
+    # body-cluster far field: whole application, its direct sums at body and
+    # wall targets, and (the rest) the multipole/local expansions
+    fm, fd, fw = kt["fmm"], kt["fmm_direct"], kt["fmm_wall"]
+    fmm = None
+    if fm["launches"]:
+        napp = fm["launches"]
+        fmm = {"ms_per_application": fm["ms"] / napp,
+               "equivalent_TFLOPs": FF_FLOP_PER_PAIR * fm["work"] / (fm["ms"] * 1e-3) / 1e12,
+               "body_direct_ms": fd["ms"] / napp,
+               "body_direct_pairs": fd["work"] / napp,
+               "body_direct_TFLOPs": FF_FLOP_PER_PAIR * fd["work"] / (fd["ms"] * 1e-3) / 1e12,
+               "wall_direct_ms": fw["ms"] / napp,
+               "wall_direct_TFLOPs": FF_FLOP_PER_PAIR * fw["work"] / (fw["ms"] * 1e-3) / 1e12,
+               "expansions_ms": (fm["ms"] - fd["ms"] - fw["ms"]) / napp,
+               "direct_fraction_of_pairs": (fd["work"] + fw["work"]) / fm["work"]}
     scale = args.steps if acc["graphs"] else 1
     share = {k: scale * kt[k]["ms"] / (elapsed * 1e3) for k in kt}
     log(f"plain-fp64: {sp}")
@@ -428,6 +453,11 @@ def main() -> int:
     if f32_ms:
         log(f"FP32 far field: {f32_ms:.3f} ms/launch, {f32_tflops:.2f} TFLOP/s over "
             f"{f32['launches']} launches")
+    if fmm:
+        log(f"body-cluster far field: {fmm['ms_per_application']:.3f} ms/application "
+            f"(body direct {fmm['body_direct_ms']:.3f} ms at {fmm['body_direct_TFLOPs']:.1f} "
+            f"TFLOP/s, wall direct {fmm['wall_direct_ms']:.3f} ms, expansions "
+            f"{fmm['expansions_ms']:.3f} ms), {fmm['equivalent_TFLOPs']:.0f} TFLOP/s equivalent")
     log(f"far field: {ff_ms:.3f} ms/launch, {ff_tflops:.2f} TFLOP/s; Gram-Schmidt {gs_gbs:.0f} GB/s; "
         f"stress eval {se['ms'] / max(1, se['launches']):.2f} ms/launch"
         + (f", {se_tflops:.2f} TFLOP/s" if se_tflops else "")
@@ -451,6 +481,32 @@ def main() -> int:
                 log(f"near-pair gather on {pw.name}: {r['near_entries']} entries, {g:.0f} GB/s")
         except Exception as e:  # a reported extra, never required
             log(f"packed near-pair measurement unavailable: {e}")
+
+    # the dominant kernel: the direct sums of the body-cluster far field where
+    # the autotuner chose it, else the direct far field
+    if fmm and fd["ms"] >= ff["ms"]:
+        ach = fmm["body_direct_TFLOPs"]
+        roofline = {
+            "bound": "valu",
+            "pipe": "VALU FP64 (vector ALU; no matrix instructions in this kernel)",
+            "kernel": "k_fmm_eval (direct sums of the body-cluster far field at body targets, "
+                      "FP64, 20 FLOP per pair)",
+            "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": ach / PEAK_FP64_TFLOPS,
+            "traffic": FMM_EVAL_BYTES.get(w.name),
+            "traffic_unit": "bytes per launch",
+        }
+    else:
+        roofline = {
+            "bound": "valu",
+            "pipe": "VALU FP64 (vector ALU; no matrix instructions in this kernel)",
+            "kernel": "k_farfield (all-pairs double layer, FP64)",
+            "achieved": ff_tflops, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": ff_tflops / PEAK_FP64_TFLOPS,
+            "traffic": (FF_FETCH_BYTES[w.name] + 16.0 * ff_splits * (m0 * n + no)
+                        if w.name in FF_FETCH_BYTES and ff_splits else None),
+            "traffic_unit": "bytes per launch",
+        }
 
     cpu = None
     if not args.no_cpu_baseline and sp:
@@ -494,6 +550,7 @@ def main() -> int:
             "first_solve_of_state": first,
             "stress_relerr_vs_plain_fp64": stress_err,
             "matvecs_per_s": sa["matvecs_per_s"],
+            "body_cluster_farfield": fmm,
             "farfield_ms": ff_ms,
             "farfield32_ms": f32_ms,
             "farfield32_TFLOPs": f32_tflops,
@@ -505,18 +562,7 @@ def main() -> int:
             "near_gather_packed_60x128": near_packed,
             "kernel_share": share,
         },
-        "roofline": {
-            "bound": "valu",
-            "pipe": "VALU FP64 (vector ALU; no matrix instructions in this kernel)",
-            "kernel": "k_farfield (all-pairs double layer, FP64)",
-            "achieved": ff_tflops,
-            "peak": PEAK_FP64_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": ff_tflops / PEAK_FP64_TFLOPS,
-            "traffic": (FF_FETCH_BYTES[w.name] + 16.0 * ff_splits * (m0 * n + no)
-                        if w.name in FF_FETCH_BYTES and ff_splits else None),
-            "traffic_unit": "bytes per launch",
-        },
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -45,6 +45,9 @@ void solve_batch(Context& settings, int ngeom, const BatchGeometry* geo, int iba
         c->pc_body_modes = settings.pc_body_modes;
         c->pc_wall_modes = settings.pc_wall_modes;
         c->use_graphs = settings.use_graphs;
+        c->ff_fmm = settings.ff_fmm;
+        c->ff_method = settings.ff_method;
+        c->ff_force = settings.ff_force;
         if (!c->use_graphs) c->graphs.clear();
         ws.push_back(c);
     }

@@ -68,6 +68,9 @@ void stokessolver_(const int64_t* npts, const int64_t* nbods, const int64_t* nou
         Context& c = stokes::ctx();
         c.status = 0;
         c.last_error.clear();
+        // ifmm = 1: the far field may be the body-cluster expansion scheme
+        // (the reference's FMM switch, stokesSolver.f:530-536)
+        c.ff_fmm = *ifmm != 0 && c.ff_method == 1 ? 1 : 0;
         const int n = static_cast<int>(*npts), m = static_cast<int>(*nbods),
                   no = static_cast<int>(*nouter);
         stokes::build_geometry(c, n, m, no, xx, yy);
@@ -372,6 +375,12 @@ int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t iba
         STOKES_HIP(hipMemcpyAsync(c.xvec.p, den, nunk * sizeof(double), hipMemcpyHostToDevice,
                                   c.stream));
         c.ff_fp32_now = c.precision == 2;
+        // the direct far field unless the body-cluster far field is forced
+        // (then its whole application is the far-field time)
+        const bool fmm = c.ff_force == stokes::farfield_shape_count() - 1;
+        const int fmm_setting = c.ff_fmm;
+        c.ff_fmm = fmm ? 1 : 0;
+        if (fmm) c.ff_fp32_now = false;
         stokes::apply_operator(c, c.xvec.p, c.rvec.p, static_cast<int>(ibary));  // warm-up
         STOKES_HIP(hipStreamSynchronize(c.stream));
         // whole operator applications, back to back
@@ -389,7 +398,7 @@ int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t iba
         STOKES_HIP(hipEventDestroy(e1));
         // per-kernel durations
         const bool was = c.time_kernels;
-        stokes::KernelTimer& tff = c.ff_fp32_now ? c.t_farfield32 : c.t_farfield;
+        stokes::KernelTimer& tff = fmm ? c.t_fmm : (c.ff_fp32_now ? c.t_farfield32 : c.t_farfield);
         tff.reset();
         c.t_near.reset();
         c.time_kernels = true;
@@ -399,6 +408,7 @@ int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t iba
         c.t_near.harvest();
         c.time_kernels = was;
         c.ff_fp32_now = false;
+        c.ff_fmm = fmm_setting;
         out[0] = static_cast<double>(tot) / static_cast<double>(reps);
         out[1] = tff.ms / static_cast<double>(std::max<long long>(1, tff.launches));
         const double np = static_cast<double>(c.g.npts);
@@ -412,6 +422,7 @@ int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t iba
         Context& c = stokes::ctx();
         c.time_kernels = false;
         c.ff_fp32_now = false;
+        c.ff_fmm = 1;
         return record_error("erosion_time_matvec", e.what());
     }
 }
@@ -585,6 +596,18 @@ int erosion_set_farfield_shape(int index) {
 
 int erosion_farfield_shape_count(void) { return stokes::farfield_shape_count(); }
 
+int erosion_set_farfield_method(int method) {
+    try {
+        if (method != 0 && method != 1) return -1;
+        Context& c = stokes::ctx();
+        c.ff_method = method;
+        c.ff_fmm = method;
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_set_farfield_method", e.what());
+    }
+}
+
 int erosion_set_graphs(int on) {
     try {
         if (on != 0 && on != 1) return -1;
@@ -756,11 +779,13 @@ int erosion_kernel_timing(int enable) {
         Context& c = stokes::ctx();
         if (enable) {
             for (stokes::KernelTimer* t : {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs,
-                                           &c.t_precond, &c.t_farfield32})
+                                           &c.t_precond, &c.t_farfield32, &c.t_fmm,
+                                           &c.t_fmm_direct, &c.t_fmm_wall})
                 t->reset();
         } else {
             for (stokes::KernelTimer* t : {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs,
-                                           &c.t_precond, &c.t_farfield32})
+                                           &c.t_precond, &c.t_farfield32, &c.t_fmm,
+                                           &c.t_fmm_direct, &c.t_fmm_wall})
                 t->harvest();
         }
         c.time_kernels = enable != 0;
@@ -774,9 +799,10 @@ int erosion_kernel_timers(double* out) {
     if (!out) return -1;
     try {
         Context& c = stokes::ctx();
-        stokes::KernelTimer* ts[6] = {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs,
-                                      &c.t_precond, &c.t_farfield32};
-        for (int i = 0; i < 6; ++i) {
+        stokes::KernelTimer* ts[9] = {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs,
+                                      &c.t_precond, &c.t_farfield32, &c.t_fmm, &c.t_fmm_direct,
+                                      &c.t_fmm_wall};
+        for (int i = 0; i < 9; ++i) {
             ts[i]->harvest();
             out[3 * i + 0] = ts[i]->ms;
             out[3 * i + 1] = static_cast<double>(ts[i]->launches);

@@ -190,6 +190,21 @@ struct NearList {
     int built_ibary = -1;
 };
 
+// Body-cluster far field (tilefmm.hip): per-body centre and radius, the far
+// bodies of each target body (multipole/local expansions), the source tiles
+// each target body sums directly, and the expansions of one application.
+struct TileFmm {
+    long long version = -1;             // geometry version the lists were built for
+    DevBuf<double> cr;                  // (cx, cy, R) per body
+    DevBuf<int> far_ptr, far_idx;       // CSR: far bodies of each target body
+    DevBuf<int> tile_ptr, tiles;        // CSR: (first point, count) source tiles per body
+    DevBuf<double> mp, lc;              // multipoles [m][3][32], locals [m][4][32] (complex)
+    int ysplit = 1;                     // source-tile splits of the body targets
+    int wall_nsplit = 0;
+    long long far_pairs = 0;            // (target body, far body) pairs
+    long long direct_pairs = 0;         // point pairs summed directly per application
+};
+
 // Per-solve GMRES state kept on the device.
 struct GmresState {
     double bnrm, r0nrm, prod, rho, snormw, vnrm, inv_snormw;
@@ -260,10 +275,16 @@ struct Context {
     DevBuf<double> stk_log;           // log |z_i - c_b|^2 [npts][m]
     long long stk_log_version = -1;   // geometry version stk_log was built for
     int ff_splits = 0, ff_tpt = 0;
-    std::map<int, int> ff_choice;      // boundary points -> far-field shape index
-    std::map<int, float> ff_tuned_ms;  // boundary points -> tuned launch time
+    std::map<int, int> ff_choice;      // size class -> far-field shape index
+    std::map<int, float> ff_tuned_ms;  // size class -> tuned launch time
     std::map<int, int> ff32_choice;    // the same for the FP32 far field
     int ff_force = -1;                 // a fixed far-field shape index (tests), -1 tunes
+    // the body-cluster far field may be chosen: ff_method is the caller's
+    // setting (1 allowed, 0 direct sums only), ff_fmm the current call's
+    // (stokessolver_'s ifmm = 1 and the setting)
+    int ff_method = 1;
+    int ff_fmm = 1;
+    TileFmm fmm;
     std::map<int, float> ff32_tuned_ms;
     // FP32 far field: per 256-point tile the FP64 centre, per source the FP32
     // offset from it and the FP32 weighted normal; built once per geometry
@@ -311,6 +332,10 @@ struct Context {
     GraphCache graphs;
     StepperBufs sp;
     KernelTimer t_farfield, t_near, t_stress_eval, t_gs, t_precond, t_farfield32;
+    // body-cluster far field: the whole application (work: the canonical
+    // Npts (Npts - 1) pairs it stands for), its direct sums at the body
+    // targets (k_fmm_eval; work: pairs) and at the wall targets (work: pairs)
+    KernelTimer t_fmm, t_fmm_direct, t_fmm_wall;
     int status = 0;
     std::string last_error;
     // last solve record
@@ -349,7 +374,17 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary,
 void prepare_operator(Context& c, const double* x_dev, bool fp32);
 // far-field shape index in use for the current size (after prepare_operator)
 int farfield_choice(const Context& c, bool fp32);
+// direct launch shapes plus the body-cluster far field (the last index)
 int farfield_shape_count();
+// true when the tuned FP32 far field is faster than the FP64 far field in use
+bool farfield_fp32_pays(const Context& c);
+// the direct far field at the wall's targets only, into part_u/part_v
+void farfield_wall_targets(Context& c, int split_len, int nsplit);
+
+// tilefmm.hip: the body-cluster far field (ifmm = 1)
+bool tilefmm_available(const Context& c);
+void tilefmm_prepare(Context& c);
+void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit);
 void gather_density(Context& c, const double* den_dev);
 void boundary_values(Context& c, const int* curves_dev, int ncurves, int wall_first);
 void bd_condition(Context& c, int ibc, double* rhs_dev);

@@ -590,6 +590,15 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
         if (c.precision != 0) prepare_operator(c, c.xvec.p, true);
     }
     const bool accel = pc || c.precision != 0;
+    // mixed precision switches to the FP32 far field only where that is
+    // faster than the FP64 far field in use (the body-cluster far field can
+    // be faster than the direct FP32 sum at 1e5 points)
+    bool fp32_pays = true;
+    if (c.precision == 1) {
+        prepare_operator(c, c.xvec.p, false);
+        prepare_operator(c, c.xvec.p, true);
+        fp32_pays = farfield_fp32_pays(c);
+    }
     const double tol_cycle = accel ? kPcTolFrac * tol : tol;
     double* V0 = c.V.p;
     long long total_iters = 0, fp32_iters = 0;
@@ -628,7 +637,8 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
             const int chunk = std::min(mixed ? kChunkIters / 2 : kChunkIters, maxl_cycle - k);
             // after a mixed cycle missed the tolerance, later cycles stay FP64
             c.ff_fp32_now = c.precision == 2 ||
-                            (mixed && cycle == 0 && relres_now <= c.mixed_switch * tol);
+                            (mixed && fp32_pays && cycle == 0 &&
+                             relres_now <= c.mixed_switch * tol);
             if (c.ff_fp32_now) fp32_iters += chunk;
             // iterations k .. k+chunk-1 (each exits early on the device once
             // the state reports convergence)

@@ -24,16 +24,18 @@
 //                  preconditioner's apply has formed them)
 //   k_assemble     partial sums + near corrections + local terms + stokeslets
 #include <algorithm>
+#include <cmath>
+#include <functional>
 
 #include "bary.hpp"
 #include "common.hpp"
+#include "farfield.hpp"
 
 namespace stokes {
 
 namespace {
 
 constexpr int kBlk = 256;
-constexpr int kTile = 256;
 
 inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
 
@@ -57,139 +59,32 @@ __global__ void k_gather(const double* __restrict__ den, int no, int n, int npts
 }
 
 // ---------------------------------------------------------------- far field
-// FP64 reciprocal for the far field: hardware estimate (4.6e-8 relative on
-// MI355X) plus one Newton step, measured at <= 2.2e-15 relative error.
-__device__ __forceinline__ double rcp64_1(double a) {
-    double r = __builtin_amdgcn_rcp(a);
-    return __builtin_fma(r, __builtin_fma(-a, r, 1.0), r);
-}
+using namespace ff;
 
-// Pair term (1/pi)(r.n)(r.eta) r w / rho^4 with w = speed 2 pi / N folded
-// into m = (mx, my).  The product (r.m)(r.eta) is expanded over the source's
-// three moments A = mx ex, B = mx ey + my ex, C = my ey:
-//   (r.m)(r.eta) = rx (A rx + B ry) + C ry^2,
-// which reuses ry^2 from rho^2 and saves one multiply per pair.  The source
-// tile holds (x, y), (A, B) and C.  TPT targets per lane; CHECK zeroes the
-// self pair (source index == target index).
-template <int TPT, bool CHECK>
-__device__ __forceinline__ void ff_tile(const double2* __restrict__ sxy,
-                                        const double2* __restrict__ sab,
-                                        const double* __restrict__ sc, const double* xt,
-                                        const double* yt, const int* self, double* u,
-                                        double* v) {
-#pragma unroll 4
-    for (int jj = 0; jj < kTile; ++jj) {
-        const double2 p = sxy[jj];
-        const double2 ab = sab[jj];
-        const double cc = sc[jj];
-#pragma unroll
-        for (int q = 0; q < TPT; ++q) {
-            const double rx = xt[q] - p.x, ry = yt[q] - p.y;
-            const double ry2 = ry * ry;
-            const double rho2 = __builtin_fma(rx, rx, ry2);
-            const double t = __builtin_fma(ab.x, rx, ab.y * ry);
-            const double w = __builtin_fma(t, rx, cc * ry2);
-            const double inv = rcp64_1(rho2);
-            double c = w * (inv * inv);
-            if (CHECK) c = (jj == self[q]) ? 0.0 : c;
-            u[q] = __builtin_fma(c, rx, u[q]);
-            v[q] = __builtin_fma(c, ry, v[q]);
-        }
-    }
-}
-
-// The same sum for TPT = 2, 4 or 8 targets with one reciprocal per source and
-// group of targets: the product P of the group's rho^2, q = 1/P (estimate +
-// one Newton step), and 1/rho_k^2 = q times the other factors of P.  Three of
-// four hardware reciprocals become three multiplies.  Off-diagonal tiles only
-// unless DIAG; padding sources sit at 1e30, so P stays finite.  DIAG: the
-// tile holds some of the lane's own targets.  A pair at distance zero (the
-// self pair) has rx = ry = 0, so its moment product w is exactly 0; its rho^2
-// is raised to 1e-100 (every other pair has rho^2 >= 1e-20 and is unchanged),
-// which keeps P, 1/P and 1/rho^4 <= 1e200 finite, and the pair adds 0.
-template <int TPT, bool DIAG>
-__device__ __forceinline__ void ff_tile_shared(const double2* __restrict__ sxy,
-                                               const double2* __restrict__ sab,
-                                               const double* __restrict__ sc, const double* xt,
-                                               const double* yt, double* u, double* v) {
-    static_assert(TPT == 2 || TPT == 4 || TPT == 8, "shared reciprocal over 2, 4 or 8 targets");
-#pragma unroll 4
-    for (int jj = 0; jj < kTile; ++jj) {
-        const double2 p = sxy[jj];
-        const double2 ab = sab[jj];
-        const double cc = sc[jj];
-        double rx[TPT], ry[TPT], r2[TPT], w[TPT], inv[TPT];
-#pragma unroll
-        for (int q = 0; q < TPT; ++q) {
-            rx[q] = xt[q] - p.x;
-            ry[q] = yt[q] - p.y;
-            const double ry2 = ry[q] * ry[q];
-            r2[q] = __builtin_fma(rx[q], rx[q], ry2);
-            if (DIAG) r2[q] = __builtin_fmax(r2[q], 1.0e-100);
-            const double t = __builtin_fma(ab.x, rx[q], ab.y * ry[q]);
-            w[q] = __builtin_fma(t, rx[q], cc * ry2);
-        }
-        if (TPT >= 4) {
-            // groups of four targets (a product of eight could overflow)
-#pragma unroll
-            for (int g0 = 0; g0 < TPT; g0 += 4) {
-                const int i0 = g0 % TPT, i1 = (g0 + 1) % TPT, i2 = (g0 + 2) % TPT,
-                          i3 = (g0 + 3) % TPT;
-                const double p01 = r2[i0] * r2[i1], p23 = r2[i2] * r2[i3];
-                const double qq = rcp64_1(p01 * p23);
-                const double q01 = qq * p23, q23 = qq * p01;  // 1/p01, 1/p23
-                inv[i0] = q01 * r2[i1];
-                inv[i1] = q01 * r2[i0];
-                inv[i2] = q23 * r2[i3];
-                inv[i3] = q23 * r2[i2];
-            }
-        } else {
-            const double qq = rcp64_1(r2[0] * r2[1]);
-            inv[0] = qq * r2[1];
-            inv[1] = qq * r2[0];
-        }
-#pragma unroll
-        for (int q = 0; q < TPT; ++q) {
-            const double c = w[q] * (inv[q] * inv[q]);
-            u[q] = __builtin_fma(c, rx[q], u[q]);
-            v[q] = __builtin_fma(c, ry[q], v[q]);
-        }
-    }
-}
-
-// Coordinate of a lane's unused target slots in the FP64 far field: far from
-// every source and from the padding sources at +1e30.
-constexpr double kNoTarget = -1.0e30;
-
-// Source moments of one point for the expanded pair term.
-__device__ __forceinline__ void src_moments(double mx, double my, double ex, double ey,
-                                            double2* ab, double* c) {
-    *ab = make_double2(mx * ex, __builtin_fma(mx, ey, my * ex));
-    *c = my * ey;
-}
-
-// grid: (ceil(npts / (256 TPT)), source splits); block 256 threads.  Lane l
+// grid: (ceil(ntgt / (256 TPT)), source splits); block 256 threads.  Lane l
 // of block b owns targets (b TPT + q) 256 + l, q < TPT; the block stages
 // 256-source tiles of its split in LDS (double2 SoA, wave-uniform reads).
-// SHR: off-diagonal tiles share one reciprocal over the lane's targets.
+// Targets are the points [0, ntgt) (every point, or the wall alone), sources
+// all npts points.  SHR: off-diagonal tiles share one reciprocal over the
+// lane's targets.
 template <int TPT, bool SHR>
 __global__ __launch_bounds__(kBlk) void k_farfield(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ mx,
     const double* __restrict__ my, const double* __restrict__ ex, const double* __restrict__ ey,
-    int npts, int split_len, double* __restrict__ part_u, double* __restrict__ part_v) {
+    int npts, int ntgt, int split_len, double* __restrict__ part_u, double* __restrict__ part_v) {
     __shared__ double2 sxy[kTile], sab[kTile];
     __shared__ double sc[kTile];
     const int tid = threadIdx.x;
     const int tb0 = blockIdx.x * TPT * kBlk;
-    const int tb1 = min(tb0 + TPT * kBlk, npts);
+    const int tb1 = min(tb0 + TPT * kBlk, ntgt);
     double xt[TPT], yt[TPT], u[TPT], v[TPT];
     int t[TPT];
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
-        // targets past npts sit where no source is (a source at the same
+        // targets past ntgt sit where no source is (a source at the same
         // point would zero the product of rho^2 the lane's targets share)
         t[q] = tb0 + q * kBlk + tid;
-        const bool valid = t[q] < npts;
+        const bool valid = t[q] < ntgt;
         xt[q] = valid ? x[t[q]] : kNoTarget;
         yt[q] = valid ? y[t[q]] : kNoTarget;
         u[q] = v[q] = 0.0;
@@ -224,7 +119,7 @@ __global__ __launch_bounds__(kBlk) void k_farfield(
     }
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
-        if (t[q] < npts) {
+        if (t[q] < ntgt) {
             part_u[static_cast<size_t>(blockIdx.y) * npts + t[q]] = u[q];
             part_v[static_cast<size_t>(blockIdx.y) * npts + t[q]] = v[q];
         }
@@ -558,12 +453,15 @@ void ff32_launch(const Context& c, const FFPlan& p, double* pu, double* pv) {
     STOKES_LAUNCH_CHECK();
 }
 
-void ff_launch(const Context& c, const FFPlan& p, double* pu, double* pv) {
+// ntgt: targets [0, ntgt) (default: every point); the plan's target blocks
+// are then ceil(ntgt / (256 tpt))
+void ff_launch(const Context& c, const FFPlan& p, double* pu, double* pv, int ntgt = -1) {
     const Geometry& g = c.g;
-    const dim3 grid(p.tblocks, p.nsplit);
+    if (ntgt < 0) ntgt = g.npts;
+    const dim3 grid(cdiv_int(ntgt, kBlk * p.tpt), p.nsplit);
 #define STOKES_FF(T, S)                                                                      \
     k_farfield<T, S><<<grid, kBlk, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p, \
-                                                  g.npts, p.split_len, pu, pv)
+                                                  g.npts, ntgt, p.split_len, pu, pv)
     if (p.tpt == 1)
         STOKES_FF(1, false);
     else if (p.tpt == 2 && p.shared)
@@ -929,7 +827,8 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
     const double* __restrict__ ey, const double* __restrict__ cx, const double* __restrict__ cy,
     const double* __restrict__ den, const double* __restrict__ part_u,
     const double* __restrict__ part_v, const double* __restrict__ part_cu,
-    const double* __restrict__ part_cv, int nsplit, const int* __restrict__ tgt_ptr,
+    const double* __restrict__ part_cv, int nsplit, int nsplit_body,
+    const int* __restrict__ tgt_ptr,
     const int* __restrict__ tgt_ent, const double* __restrict__ corr_u,
     const double* __restrict__ corr_v, int has_near, const double* __restrict__ scal, int nsig,
     const double* __restrict__ lgtab, int no, int n, int m, int npts, double* __restrict__ out) {
@@ -972,9 +871,11 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
             }
         }
     }
-    // far-field partial sums of the source splits s = sub mod kAsmSub
+    // far-field partial sums of the source splits s = sub mod kAsmSub (the
+    // body-cluster far field leaves nsplit_body partials for body points)
     if (valid) {
-        for (int s = sub; s < nsplit; s += kAsmSub) {
+        const int ns = i < no ? nsplit : nsplit_body;
+        for (int s = sub; s < ns; s += kAsmSub) {
             const size_t o = static_cast<size_t>(s) * npts + i;
             if (COMP) {
                 dd_add(us, lus, part_u[o]);
@@ -1131,30 +1032,57 @@ void boundary_values(Context& c, const int* curves_dev, int ncurves, int wall_fi
 // launches each, HIP events on the library stream) and keeps the fastest.
 // Shapes include the shared-reciprocal FP64 tiles, so they are used only
 // where they measure faster.
+constexpr int kFFDirectShapes = static_cast<int>(sizeof(kFFShapes) / sizeof(kFFShapes[0]));
+constexpr int kFmmIndex = kFFDirectShapes;  // the body-cluster far field
+
+// Tuning results are kept per size class -- boundary points in buckets of
+// 2^(1/16), about 4% -- and, for FP64, whether the body-cluster far field may
+// be chosen: an erosion run loses a body every few steps, and every exact
+// size would otherwise time every shape again.
+static int tune_key(const Context& c, bool fp32) {
+    const bool fmm = !fp32 && c.ff_fmm && tilefmm_available(c);
+    const int bucket = static_cast<int>(std::floor(16.0 * std::log2(static_cast<double>(c.g.npts))));
+    return 2 * bucket + (fmm ? 1 : 0);
+}
+
 static int tune_farfield(Context& c, bool fp32) {
-    if (c.ff_force >= 0) return c.ff_force;
+    if (c.ff_force >= 0) {
+        if (c.ff_force == kFmmIndex && !tilefmm_available(c))
+            throw Error("the body-cluster far field needs 2+ bodies of at most 512 points");
+        return c.ff_force;
+    }
     const int np = c.g.npts;
+    const int key = tune_key(c, fp32);
     std::map<int, int>& choice = fp32 ? c.ff32_choice : c.ff_choice;
-    auto it = choice.find(np);
+    auto it = choice.find(key);
     if (it != choice.end()) return it->second;
-    const int nshape = static_cast<int>(sizeof(kFFShapes) / sizeof(kFFShapes[0]));
+    const bool fmm = key & 1;
+    if (fmm) tilefmm_prepare(c);
     int best = 0;
     float best_ms = 1e30f;
     hipEvent_t e0, e1;
     STOKES_HIP(hipEventCreate(&e0));
     STOKES_HIP(hipEventCreate(&e1));
-    for (int i = 0; i < nshape; ++i) {
-        // the FP32 kernel has no shared form (and no 8-target form)
-        if (fp32 && kFFShapes[i].shared) continue;
-        const FFPlan p = ff_plan(np, kFFShapes[i], fp32);
-        c.part_u.ensure(static_cast<size_t>(np) * static_cast<size_t>(p.nsplit));
-        c.part_v.ensure(static_cast<size_t>(np) * static_cast<size_t>(p.nsplit));
-        auto launch = [&]() {
-            if (fp32)
-                ff32_launch(c, p, c.part_u.p, c.part_v.p);
-            else
-                ff_launch(c, p, c.part_u.p, c.part_v.p);
-        };
+    for (int i = 0; i < kFFDirectShapes + (fmm ? 1 : 0); ++i) {
+        std::function<void()> launch;
+        if (i == kFmmIndex) {
+            launch = [&]() {
+                int a = 0, b = 0;
+                tilefmm_apply(c, &a, &b);
+            };
+        } else {
+            // the FP32 kernel has no shared form (and no 8-target form)
+            if (fp32 && kFFShapes[i].shared) continue;
+            const FFPlan p = ff_plan(np, kFFShapes[i], fp32);
+            c.part_u.ensure(static_cast<size_t>(np) * static_cast<size_t>(p.nsplit));
+            c.part_v.ensure(static_cast<size_t>(np) * static_cast<size_t>(p.nsplit));
+            launch = [&c, p, fp32]() {
+                if (fp32)
+                    ff32_launch(c, p, c.part_u.p, c.part_v.p);
+                else
+                    ff_launch(c, p, c.part_u.p, c.part_v.p);
+            };
+        }
         launch();
         STOKES_HIP(hipEventRecord(e0, c.stream));
         for (int r = 0; r < 3; ++r) launch();
@@ -1169,9 +1097,25 @@ static int tune_farfield(Context& c, bool fp32) {
     }
     STOKES_HIP(hipEventDestroy(e0));
     STOKES_HIP(hipEventDestroy(e1));
-    choice[np] = best;
-    (fp32 ? c.ff32_tuned_ms : c.ff_tuned_ms)[np] = best_ms / 3.0f;
+    choice[key] = best;
+    (fp32 ? c.ff32_tuned_ms : c.ff_tuned_ms)[key] = best_ms / 3.0f;
     return best;
+}
+
+bool farfield_fp32_pays(const Context& c) {
+    auto a = c.ff32_tuned_ms.find(tune_key(c, true));
+    auto b = c.ff_tuned_ms.find(tune_key(c, false));
+    if (a == c.ff32_tuned_ms.end() || b == c.ff_tuned_ms.end()) return true;
+    return a->second < b->second;
+}
+
+void farfield_wall_targets(Context& c, int split_len, int nsplit) {
+    const Geometry& g = c.g;
+    const FFPlan p{2, cdiv_int(g.no, 2 * kBlk), split_len, nsplit, true};
+    if (c.time_kernels) c.t_fmm_wall.start(c.stream);
+    ff_launch(c, p, c.part_u.p, c.part_v.p, g.no);
+    if (c.time_kernels)
+        c.t_fmm_wall.stop(c.stream, static_cast<double>(g.no) * static_cast<double>(g.npts - 1));
 }
 
 // FP32 far-field source tiles of the current geometry (once per geometry).
@@ -1207,18 +1151,25 @@ void prepare_operator(Context& c, const double* x_dev, bool fp32) {
     // the first call allocate) it before timing any far-field shape
     gather_density(c, x_dev);
     if (fp32) ff32_sources(c);
-    const FFPlan plan = ff_plan(c.g.npts, kFFShapes[tune_farfield(c, fp32)], fp32);
-    c.part_u.ensure(static_cast<size_t>(c.g.npts) * static_cast<size_t>(plan.nsplit));
-    c.part_v.ensure(static_cast<size_t>(c.g.npts) * static_cast<size_t>(plan.nsplit));
+    const int choice = tune_farfield(c, fp32);
+    if (choice == kFmmIndex) {
+        // lists and part buffers of the body-cluster far field
+        int a = 0, b = 0;
+        tilefmm_apply(c, &a, &b);
+    } else {
+        const FFPlan plan = ff_plan(c.g.npts, kFFShapes[choice], fp32);
+        c.part_u.ensure(static_cast<size_t>(c.g.npts) * static_cast<size_t>(plan.nsplit));
+        c.part_v.ensure(static_cast<size_t>(c.g.npts) * static_cast<size_t>(plan.nsplit));
+    }
     stokes_logs(c);
 }
 
-int farfield_shape_count() { return static_cast<int>(sizeof(kFFShapes) / sizeof(kFFShapes[0])); }
+int farfield_shape_count() { return kFFDirectShapes + 1; }
 
 int farfield_choice(const Context& c, bool fp32) {
     if (c.ff_force >= 0) return c.ff_force;
     const std::map<int, int>& choice = fp32 ? c.ff32_choice : c.ff_choice;
-    auto it = choice.find(c.g.npts);
+    auto it = choice.find(tune_key(c, fp32));
     return it == choice.end() ? -1 : it->second;
 }
 
@@ -1232,16 +1183,31 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, b
     c.scal.ensure(std::max<size_t>(8, static_cast<size_t>(precond_wall_segments(g))));
 
     if (!from_precond) gather_density(c, x_dev);
-    const bool fp32 = c.ff_fp32_now && !compensated;
-    if (fp32) ff32_sources(c);
+    bool fp32 = c.ff_fp32_now && !compensated;
     // a compensated evaluation runs its own far field: 4 targets per lane,
     // per-pair reciprocals, about 2048 blocks
-    const FFPlan plan = compensated ? ff_plan(np, FFShape{4, 2048, false}, false)
-                                    : ff_plan(np, kFFShapes[tune_farfield(c, fp32)], fp32);
-    const int nsplit = plan.nsplit;
-    c.part_u.ensure(N * static_cast<size_t>(nsplit));
-    c.part_v.ensure(N * static_cast<size_t>(nsplit));
-    if (compensated) {
+    const int choice = compensated ? -1 : tune_farfield(c, fp32);
+    if (choice == kFmmIndex) fp32 = false;  // the body-cluster far field is FP64
+    if (fp32) ff32_sources(c);
+    int nsplit = 0, nsplit_body = 0;
+    if (choice == kFmmIndex) {
+        if (c.time_kernels) c.t_fmm.start(s);
+        tilefmm_apply(c, &nsplit, &nsplit_body);
+        if (c.time_kernels) c.t_fmm.stop(s, static_cast<double>(np) * static_cast<double>(np - 1));
+        c.ff_splits = nsplit;
+        c.ff_tpt = -1;
+    }
+    const FFPlan plan = choice == kFmmIndex ? FFPlan{}
+                        : compensated ? ff_plan(np, FFShape{4, 2048, false}, false)
+                                      : ff_plan(np, kFFShapes[choice], fp32);
+    if (choice != kFmmIndex) {
+        nsplit = nsplit_body = plan.nsplit;
+        c.part_u.ensure(N * static_cast<size_t>(nsplit));
+        c.part_v.ensure(N * static_cast<size_t>(nsplit));
+    }
+    if (choice == kFmmIndex) {
+        // done above
+    } else if (compensated) {
         c.part_cu.ensure(N * static_cast<size_t>(nsplit));
         c.part_cv.ensure(N * static_cast<size_t>(nsplit));
         k_farfield_comp<4><<<dim3(plan.tblocks, plan.nsplit), kBlk, 0, s>>>(
@@ -1284,7 +1250,8 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, b
 #define STOKES_ASM(COMP)                                                                       \
     k_assemble<COMP><<<cdiv_int(np, kBlk / kAsmSub), kBlk, 0, s>>>(                             \
         g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p, c.ex.p, c.ey.p, g.cx.p, g.cy.p, x_dev, \
-        c.part_u.p, c.part_v.p, c.part_cu.p, c.part_cv.p, nsplit, L.tgt_ptr.p, L.tgt_ent.p,      \
+        c.part_u.p, c.part_v.p, c.part_cu.p, c.part_cv.p, nsplit, nsplit_body, L.tgt_ptr.p,      \
+        L.tgt_ent.p,                                                                             \
         L.corr_u.p, L.corr_v.p, has_near ? 1 : 0, c.scal.p, nsig,                                 \
         g.m > 0 ? c.stk_log.p : nullptr, g.no, g.n, g.m, np, y_dev)
     if (compensated)

@@ -159,6 +159,20 @@ def set_guess_space(space: int) -> None:
         raise StokesError(f"invalid guess space {space}")
 
 
+FARFIELD_DIRECT, FARFIELD_AUTO = 0, 1
+
+
+def set_farfield_method(method: int) -> None:
+    """FARFIELD_AUTO (default): the direct sums or the body-cluster
+    expansions, whichever the autotuner measures faster; FARFIELD_DIRECT:
+    the direct all-pairs sums only."""
+    L = lib()
+    L.erosion_set_farfield_method.argtypes = [ctypes.c_int]
+    L.erosion_set_farfield_method.restype = ctypes.c_int
+    if L.erosion_set_farfield_method(int(method)) != 0:
+        raise StokesError(f"invalid far-field method {method}")
+
+
 def get_guess_space() -> int:
     return int(lib().erosion_get_guess_space())
 
@@ -238,7 +252,8 @@ def time_matvec(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: int = 1,
             "near_ms": out[4], "splits": int(out[5]), "gather_bytes": out[6]}
 
 
-TIMER_FAMILIES = ("farfield", "near", "stress_eval", "gram_schmidt", "precond", "farfield32")
+TIMER_FAMILIES = ("farfield", "near", "stress_eval", "gram_schmidt", "precond", "farfield32",
+                  "fmm", "fmm_direct", "fmm_wall")
 
 
 def kernel_timing(enable: bool) -> None:

@@ -162,12 +162,15 @@ int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t iba
 int erosion_kernel_timing(int enable);
 
 /* Totals since timing was enabled, three values per family in the order
- * far field (FP64), near pairs, stress evaluation, Gram-Schmidt,
- * preconditioner, far field (FP32): out[3i] milliseconds, out[3i+1] launches,
- * out[3i+2] algorithmic work (ordered source-target pairs; bytes of the
- * near-pair gather pass, as erosion_time_matvec's out[6]; target-node pairs; bytes of Krylov basis streamed; bytes of vector read and
- * written; ordered source-target pairs).  `out` holds 18 doubles.  Returns 0
- * or -1. */
+ * far field (FP64, direct), near pairs, stress evaluation, Gram-Schmidt,
+ * preconditioner, far field (FP32, direct), body-cluster far field (whole
+ * application), its direct sums at body targets (k_fmm_eval), its direct sums
+ * at wall targets: out[3i] milliseconds, out[3i+1] launches, out[3i+2]
+ * algorithmic work (ordered source-target pairs; bytes of the near-pair
+ * gather pass, as erosion_time_matvec's out[6]; target-node pairs; bytes of
+ * Krylov basis streamed; bytes of vector read and written; ordered
+ * source-target pairs; the Npts (Npts - 1) pairs the application stands for;
+ * pairs summed; pairs summed).  `out` holds 27 doubles.  Returns 0 or -1. */
 int erosion_kernel_timers(double* out);
 
 /* Precision of the far field in stokessolver_ (no counterpart in the
@@ -277,6 +280,16 @@ int erosion_spectral(int op, int64_t npts, int64_t rows, const double* in, const
  * the autotuner.  Returns 0, or -1 for an index out of range. */
 int erosion_set_farfield_shape(int index);
 int erosion_farfield_shape_count(void);
+/* Far field of the completed operator (the reference's FMM switch ifmm,
+ * stokesSolver.f:530-536, which stokessolver_ still takes): method 1
+ * (default) lets the autotuner pick, per boundary size, between the direct
+ * all-pairs sums and the body-cluster expansions of tilefmm.hip (multipole
+ * and local expansions of 32 terms between bodies at least three times
+ * their radii apart, truncation below 5e-16 of the terms; direct sums for
+ * the rest and the wall), whichever is faster, on calls with ifmm = 1;
+ * method 0 keeps the direct sums.  The last shape index of
+ * erosion_set_farfield_shape is the body-cluster far field.  Returns 0 or -1. */
+int erosion_set_farfield_method(int method);
 
 /* HIP graphs for GMRES (no counterpart in the reference).  on = 1: stokessolver_
  * runs each chunk of iterations (8 or 16 iterations of preconditioner,
