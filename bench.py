@@ -318,9 +318,9 @@ def main() -> int:
         steps, ideal seconds per operator application)."""
         k = r["kt"]
         scale = r["steps"] if r["graphs"] else 1
-        # pairs summed directly: the direct far field, and the direct parts
-        # of the body-cluster far field (its expansions are not counted)
-        pairs64 = k["farfield"]["work"] + k["fmm_direct"]["work"] + k["fmm_wall"]["work"]
+        # pairs summed directly: the direct far field, and the direct part
+        # of the cluster far field (its expansions are not counted)
+        pairs64 = k["farfield"]["work"] + k["fmm_direct"]["work"]
         t64 = FF_FLOP_PER_PAIR * pairs64 / (PEAK_FP64_TFLOPS * 1e12)
         t32 = FF_FLOP_PER_PAIR * k["farfield32"]["work"] / (PEAK_FP32_TFLOPS * 1e12)
         tst = STRESS_FLOP_PER_PAIR * k["stress_eval"]["work"] / (PEAK_FP64_TFLOPS * 1e12)
@@ -428,21 +428,20 @@ def main() -> int:
                  if se["ms"] > 0 else None)
     nr = kt["near"]
     near_gbs = nr["work"] / (nr["ms"] * 1e-3) / 1e9 if nr["ms"] > 0 else None
-    # body-cluster far field: whole application, its direct sums at body and
-    # wall targets, and (the rest) the multipole/local expansions
-    fm, fd, fw = kt["fmm"], kt["fmm_direct"], kt["fmm_wall"]
+    # cluster far field: whole application, its direct sums (with the local
+    # expansions' evaluation), and the multipole and local expansions
+    fm, fd, fx = kt["fmm"], kt["fmm_direct"], kt["fmm_expand"]
     fmm = None
     if fm["launches"]:
         napp = fm["launches"]
         fmm = {"ms_per_application": fm["ms"] / napp,
                "equivalent_TFLOPs": FF_FLOP_PER_PAIR * fm["work"] / (fm["ms"] * 1e-3) / 1e12,
-               "body_direct_ms": fd["ms"] / napp,
-               "body_direct_pairs": fd["work"] / napp,
-               "body_direct_TFLOPs": FF_FLOP_PER_PAIR * fd["work"] / (fd["ms"] * 1e-3) / 1e12,
-               "wall_direct_ms": fw["ms"] / napp,
-               "wall_direct_TFLOPs": FF_FLOP_PER_PAIR * fw["work"] / (fw["ms"] * 1e-3) / 1e12,
-               "expansions_ms": (fm["ms"] - fd["ms"] - fw["ms"]) / napp,
-               "direct_fraction_of_pairs": (fd["work"] + fw["work"]) / fm["work"]}
+               "direct_ms": fd["ms"] / napp,
+               "direct_pairs": fd["work"] / napp,
+               "direct_TFLOPs": FF_FLOP_PER_PAIR * fd["work"] / (fd["ms"] * 1e-3) / 1e12,
+               "expansions_ms": fx["ms"] / napp,
+               "translations": fx["work"] / napp,
+               "direct_fraction_of_pairs": fd["work"] / fm["work"]}
     scale = args.steps if acc["graphs"] else 1
     share = {k: scale * kt[k]["ms"] / (elapsed * 1e3) for k in kt}
     log(f"plain-fp64: {sp}")
@@ -454,10 +453,10 @@ def main() -> int:
         log(f"FP32 far field: {f32_ms:.3f} ms/launch, {f32_tflops:.2f} TFLOP/s over "
             f"{f32['launches']} launches")
     if fmm:
-        log(f"body-cluster far field: {fmm['ms_per_application']:.3f} ms/application "
-            f"(body direct {fmm['body_direct_ms']:.3f} ms at {fmm['body_direct_TFLOPs']:.1f} "
-            f"TFLOP/s, wall direct {fmm['wall_direct_ms']:.3f} ms, expansions "
-            f"{fmm['expansions_ms']:.3f} ms), {fmm['equivalent_TFLOPs']:.0f} TFLOP/s equivalent")
+        log(f"cluster far field: {fmm['ms_per_application']:.3f} ms/application "
+            f"(direct {fmm['direct_ms']:.3f} ms at {fmm['direct_TFLOPs']:.1f} TFLOP/s, "
+            f"expansions {fmm['expansions_ms']:.3f} ms for {fmm['translations']:.0f} "
+            f"translations), {fmm['equivalent_TFLOPs']:.0f} TFLOP/s equivalent")
     log(f"far field: {ff_ms:.3f} ms/launch, {ff_tflops:.2f} TFLOP/s; Gram-Schmidt {gs_gbs:.0f} GB/s; "
         f"stress eval {se['ms'] / max(1, se['launches']):.2f} ms/launch"
         + (f", {se_tflops:.2f} TFLOP/s" if se_tflops else "")
@@ -482,15 +481,14 @@ def main() -> int:
         except Exception as e:  # a reported extra, never required
             log(f"packed near-pair measurement unavailable: {e}")
 
-    # the dominant kernel: the direct sums of the body-cluster far field where
-    # the autotuner chose it, else the direct far field
+    # the dominant kernel: the direct sums of the cluster far field where the
+    # autotuner chose it, else the direct far field
     if fmm and fd["ms"] >= ff["ms"]:
-        ach = fmm["body_direct_TFLOPs"]
+        ach = fmm["direct_TFLOPs"]
         roofline = {
             "bound": "valu",
             "pipe": "VALU FP64 (vector ALU; no matrix instructions in this kernel)",
-            "kernel": "k_fmm_eval (direct sums of the body-cluster far field at body targets, "
-                      "FP64, 20 FLOP per pair)",
+            "kernel": "k_fmm_eval (direct sums of the cluster far field, FP64, 20 FLOP per pair)",
             "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": ach / PEAK_FP64_TFLOPS,
             "traffic": FMM_EVAL_BYTES.get(w.name),
@@ -550,7 +548,7 @@ def main() -> int:
             "first_solve_of_state": first,
             "stress_relerr_vs_plain_fp64": stress_err,
             "matvecs_per_s": sa["matvecs_per_s"],
-            "body_cluster_farfield": fmm,
+            "cluster_farfield": fmm,
             "farfield_ms": ff_ms,
             "farfield32_ms": f32_ms,
             "farfield32_TFLOPs": f32_tflops,

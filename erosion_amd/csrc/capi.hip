@@ -780,12 +780,12 @@ int erosion_kernel_timing(int enable) {
         if (enable) {
             for (stokes::KernelTimer* t : {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs,
                                            &c.t_precond, &c.t_farfield32, &c.t_fmm,
-                                           &c.t_fmm_direct, &c.t_fmm_wall})
+                                           &c.t_fmm_direct, &c.t_fmm_expand})
                 t->reset();
         } else {
             for (stokes::KernelTimer* t : {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs,
                                            &c.t_precond, &c.t_farfield32, &c.t_fmm,
-                                           &c.t_fmm_direct, &c.t_fmm_wall})
+                                           &c.t_fmm_direct, &c.t_fmm_expand})
                 t->harvest();
         }
         c.time_kernels = enable != 0;
@@ -801,7 +801,7 @@ int erosion_kernel_timers(double* out) {
         Context& c = stokes::ctx();
         stokes::KernelTimer* ts[9] = {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs,
                                       &c.t_precond, &c.t_farfield32, &c.t_fmm, &c.t_fmm_direct,
-                                      &c.t_fmm_wall};
+                                      &c.t_fmm_expand};
         for (int i = 0; i < 9; ++i) {
             ts[i]->harvest();
             out[3 * i + 0] = ts[i]->ms;

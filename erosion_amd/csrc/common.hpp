@@ -190,18 +190,24 @@ struct NearList {
     int built_ibary = -1;
 };
 
-// Body-cluster far field (tilefmm.hip): per-body centre and radius, the far
-// bodies of each target body (multipole/local expansions), the source tiles
-// each target body sums directly, and the expansions of one application.
+// Cluster far field (tilefmm.hip).  Clusters are the wall's segments of 64
+// points and the bodies; per cluster its centre and radius, its first point
+// and count, the far clusters of each target cluster (multipole/local
+// expansions), the source tiles each target cluster sums directly, and the
+// expansions of one application.
 struct TileFmm {
     long long version = -1;             // geometry version the lists were built for
-    DevBuf<double> cr;                  // (cx, cy, R) per body
-    DevBuf<int> far_ptr, far_idx;       // CSR: far bodies of each target body
-    DevBuf<int> tile_ptr, tiles;        // CSR: (first point, count) source tiles per body
-    DevBuf<double> mp, lc;              // multipoles [m][3][32], locals [m][4][32] (complex)
-    int ysplit = 1;                     // source-tile splits of the body targets
-    int wall_nsplit = 0;
-    long long far_pairs = 0;            // (target body, far body) pairs
+    int nc = 0, nwall = 0;              // clusters; the first nwall are wall segments
+    DevBuf<double> cr;                  // (cx, cy, R) per cluster
+    DevBuf<int> cl;                     // (first point, count) per cluster
+    DevBuf<int> far_ptr, far_idx;       // CSR: far clusters of each target cluster
+    DevBuf<int> tile_ptr, tiles;        // CSR: (first point, count) source tiles per cluster
+    DevBuf<double> mp, lc;              // multipoles [nc][3][32], locals [nc][3][32] (complex)
+    DevBuf<double> bin;                 // binomials of the translation, matrix-core layout
+    DevBuf<int> work;                   // (cluster, tile) per direct-sum block: bodies, wall
+    int work_body = 0, work_wall = 0;   // work items of body and wall targets
+    int max_tiles = 1;                  // most source tiles of one cluster
+    long long far_pairs = 0;            // (target cluster, far cluster) pairs
     long long direct_pairs = 0;         // point pairs summed directly per application
 };
 
@@ -332,10 +338,12 @@ struct Context {
     GraphCache graphs;
     StepperBufs sp;
     KernelTimer t_farfield, t_near, t_stress_eval, t_gs, t_precond, t_farfield32;
-    // body-cluster far field: the whole application (work: the canonical
-    // Npts (Npts - 1) pairs it stands for), its direct sums at the body
-    // targets (k_fmm_eval; work: pairs) and at the wall targets (work: pairs)
-    KernelTimer t_fmm, t_fmm_direct, t_fmm_wall;
+    // cluster far field: the whole application (work: the canonical
+    // Npts (Npts - 1) pairs it stands for), its direct sums and local
+    // expansions at every target (k_fmm_eval; work: pairs), and the
+    // multipole and local expansions (k_fmm_p2m + k_fmm_m2l; work: cluster
+    // translations)
+    KernelTimer t_fmm, t_fmm_direct, t_fmm_expand;
     int status = 0;
     std::string last_error;
     // last solve record
@@ -378,10 +386,9 @@ int farfield_choice(const Context& c, bool fp32);
 int farfield_shape_count();
 // true when the tuned FP32 far field is faster than the FP64 far field in use
 bool farfield_fp32_pays(const Context& c);
-// the direct far field at the wall's targets only, into part_u/part_v
-void farfield_wall_targets(Context& c, int split_len, int nsplit);
 
-// tilefmm.hip: the body-cluster far field (ifmm = 1)
+// tilefmm.hip: the cluster far field (ifmm = 1); both split counts are the
+// number of source splits whose partial sums it leaves in part_u/part_v
 bool tilefmm_available(const Context& c);
 void tilefmm_prepare(Context& c);
 void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit);

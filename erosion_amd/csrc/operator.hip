@@ -1048,7 +1048,7 @@ static int tune_key(const Context& c, bool fp32) {
 static int tune_farfield(Context& c, bool fp32) {
     if (c.ff_force >= 0) {
         if (c.ff_force == kFmmIndex && !tilefmm_available(c))
-            throw Error("the body-cluster far field needs 2+ bodies of at most 512 points");
+            throw Error("the cluster far field needs bodies of 4 to 512 points");
         return c.ff_force;
     }
     const int np = c.g.npts;
@@ -1107,15 +1107,6 @@ bool farfield_fp32_pays(const Context& c) {
     auto b = c.ff_tuned_ms.find(tune_key(c, false));
     if (a == c.ff32_tuned_ms.end() || b == c.ff_tuned_ms.end()) return true;
     return a->second < b->second;
-}
-
-void farfield_wall_targets(Context& c, int split_len, int nsplit) {
-    const Geometry& g = c.g;
-    const FFPlan p{2, cdiv_int(g.no, 2 * kBlk), split_len, nsplit, true};
-    if (c.time_kernels) c.t_fmm_wall.start(c.stream);
-    ff_launch(c, p, c.part_u.p, c.part_v.p, g.no);
-    if (c.time_kernels)
-        c.t_fmm_wall.stop(c.stream, static_cast<double>(g.no) * static_cast<double>(g.npts - 1));
 }
 
 // FP32 far-field source tiles of the current geometry (once per geometry).

@@ -1,12 +1,12 @@
-// Far field by body clusters: the trapezoid double layer of well-separated
-// bodies through multipole and local expansions, the rest summed directly.
+// Far field by clusters: the trapezoid double layer of well-separated
+// clusters through multipole and local expansions, the rest summed directly.
 //
 // This replaces the reference's FMM (stokesDLPnew -> bhfmm2dparttarg,
 // ErosionPackage/fortran_code/fmm/bhfmm2dpart.f:159-203, tolerance 0.5e-12)
 // in the role the reference gives it: the far field of matvec_DLP_fmmbary
 // when ifmm = 1 (stokesSolver.f:636-681).  It is a one-level scheme over the
-// bodies of a porous geometry, not a tree: every body is a cluster, the wall
-// is summed directly.
+// clusters of a porous geometry, not a tree: every body is a cluster, and so
+// is every segment of kWallSeg consecutive wall points.
 //
 // Complex form of the pair term.  With r = z - w_j (target minus source),
 // m_j the weighted normal and eta_j the density as complex numbers,
@@ -15,7 +15,7 @@
 // So a cluster s contributes, through the Cauchy sums F(z) = sum a_j/(z-w_j),
 // S(z) = sum s_j/(z-w_j) and E(z) = sum a_j conj(w_j - c_s)/(z-w_j),
 //   u(z) = (1/4) [ F + conj(S) + conj(E' - conj(z - c_s) F') ].
-// Summed over far clusters into local expansions about the target body's
+// Summed over far clusters into local expansions about the target cluster's
 // centre c_t, with G = sum_s conj(c_t - c_s) F_s,
 //   u(z) = (1/4) [ F_L + conj(S_L) + conj(E_L' - conj(z - c_t) F_L' - G_L') ].
 //
@@ -23,17 +23,15 @@
 // f(z) = sum_n M_n R_s^n / (z - c_s)^(n+1), M_n = sum_j coef_j ((w_j-c_s)/R_s)^n;
 // local about c_t scaled by R_t, f(z) = sum_k L_k ((z - c_t)/R_t)^k, with
 // L_k = -(1/d) (R_t/d)^k sum_n binom(n+k, k) (-R_s/d)^n M_n, d = c_s - c_t.
-// kP = 32 terms; a pair of bodies is far when R_s + R_t <= |d| / 3, so the
+// kP = 32 terms; a pair of clusters is far when R_s + R_t <= |d| / 3, so the
 // truncation is below (1/3)^32 = 5e-16 of the cluster's term sizes -- the
-// size of the rounding of the direct sum.  Other pairs (and the body itself)
-// are summed directly, as are the wall as a source of every target and every
-// source at the wall's targets.
+// size of the rounding of the direct sum.  Other pairs (and the cluster
+// itself) are summed directly.
 //
-// Kernels per operator application: k_farfield restricted to the wall's
-// targets (source splits), k_fmm_p2m (one block per body), k_fmm_m2l (one
-// block per target body), k_fmm_eval (target body x source-tile split: the
-// direct part, and the local expansion on split 0).  Lists are built on the
-// host once per geometry.
+// Kernels per operator application: k_fmm_p2m (one block per cluster),
+// k_fmm_m2l (one block per target cluster), k_fmm_eval (target cluster x
+// source-tile split: the direct part, and the local expansion on split 0).
+// Lists are built on the host once per geometry.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -50,8 +48,8 @@ using namespace ff;
 constexpr int kBlk = 256;
 constexpr int kP = 32;                 // expansion terms
 constexpr double kSep = 1.0 / 3.0;     // far: R_s + R_t <= kSep |c_s - c_t|
-constexpr int kSlots = kBlk / kP;      // sources in flight per M2L block
 constexpr int kMaxBodyPts = 512;
+constexpr int kWallSeg = 64;           // wall points per cluster
 
 inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
 
@@ -66,290 +64,409 @@ __device__ __forceinline__ cplx cpow_int(cplx z, int k) {
     return r;
 }
 
-// Multipole coefficients of every body: grid m, block 128; thread q*32 + k
-// (q < 3: F, S, E) forms M^q_k.  cr holds (cx, cy, R) per body.
-__global__ __launch_bounds__(128) void k_fmm_p2m(const double* __restrict__ x,
+// Multipole coefficients of every cluster: grid nc, block 256 = 4 waves.
+// Wave w forms the orders k = 8w .. 8w + 7 of F, S and E: lane l sums the
+// cluster's points l, l + 64, ... (its first power by squaring, the next
+// seven by products), then the wave sums its lanes.  cr holds (cx, cy, R)
+// and cl (first point, count) per cluster; mp[c][q][k], q < 3: F, S, E.
+__global__ __launch_bounds__(256) void k_fmm_p2m(const double* __restrict__ x,
                                                  const double* __restrict__ y,
                                                  const double* __restrict__ mx,
                                                  const double* __restrict__ my,
                                                  const double* __restrict__ ex,
                                                  const double* __restrict__ ey,
-                                                 const double* __restrict__ cr, int no, int n,
+                                                 const double* __restrict__ cr,
+                                                 const int* __restrict__ cl,
                                                  double* __restrict__ mp) {
-    __shared__ double zr[kMaxBodyPts], zi[kMaxBodyPts], ar[kMaxBodyPts], ai[kMaxBodyPts],
-        sr[kMaxBodyPts], er[kMaxBodyPts], ei[kMaxBodyPts];
-    const int b = blockIdx.x;
-    const double cx = cr[3 * b], cy = cr[3 * b + 1], iR = 1.0 / cr[3 * b + 2];
-    const int off = no + b * n;
-    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    constexpr int kPer = kP / 4;
+    const int c = blockIdx.x, w = threadIdx.x / 64, l = threadIdx.x % 64;
+    const double cx = cr[3 * c], cy = cr[3 * c + 1], iR = 1.0 / cr[3 * c + 2];
+    const int off = cl[2 * c], len = cl[2 * c + 1];
+    cplx acc[3][kPer];
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) acc[q][i] = cmk(0.0, 0.0);
+    for (int j = l; j < len; j += 64) {
         const int p = off + j;
         const double dx = x[p] - cx, dy = y[p] - cy;
-        zr[j] = dx * iR;
-        zi[j] = dy * iR;
+        const cplx z = cmk(dx * iR, dy * iR);
         const double mxx = mx[p], myy = my[p], exx = ex[p], eyy = ey[p];
-        const double are = mxx * exx - myy * eyy, aim = mxx * eyy + myy * exx;
-        ar[j] = are;
-        ai[j] = aim;
-        sr[j] = 2.0 * (mxx * exx + myy * eyy);
-        er[j] = are * dx + aim * dy;  // a conj(w - c)
-        ei[j] = aim * dx - are * dy;
+        const cplx a = cmk(mxx * exx - myy * eyy, mxx * eyy + myy * exx);
+        const double sj = 2.0 * (mxx * exx + myy * eyy);
+        const cplx e = cmk(a.r * dx + a.i * dy, a.i * dx - a.r * dy);  // a conj(w - c)
+        cplx zk = cpow_int(z, kPer * w);  // the same count on the whole wave
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            cfma(acc[0][i], a, zk);
+            acc[1][i].r = __builtin_fma(sj, zk.r, acc[1][i].r);
+            acc[1][i].i = __builtin_fma(sj, zk.i, acc[1][i].i);
+            cfma(acc[2][i], e, zk);
+            zk = cmul(zk, z);
+        }
     }
-    __syncthreads();
-    const int q = threadIdx.x / kP, k = threadIdx.x % kP;
-    if (q >= 3) return;
-    double accr = 0.0, acci = 0.0;
-    for (int j = 0; j < n; ++j) {
-        const cplx zk = cpow_int(cmk(zr[j], zi[j]), k);
-        const double c_r = q == 0 ? ar[j] : (q == 1 ? sr[j] : er[j]);
-        const double c_i = q == 0 ? ai[j] : (q == 1 ? 0.0 : ei[j]);
-        accr = __builtin_fma(c_r, zk.r, __builtin_fma(-c_i, zk.i, accr));
-        acci = __builtin_fma(c_r, zk.i, __builtin_fma(c_i, zk.r, acci));
-    }
-    const size_t o = (static_cast<size_t>(b) * 3 + q) * kP + k;
-    mp[2 * o] = accr;
-    mp[2 * o + 1] = acci;
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int i = 0; i < kPer; ++i)
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                acc[q][i].r += __shfl_xor(acc[q][i].r, o);
+                acc[q][i].i += __shfl_xor(acc[q][i].i, o);
+            }
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+#pragma unroll
+        for (int i = 0; i < kPer; ++i)
+            if (l == q * kPer + i) {
+                const size_t o = (static_cast<size_t>(c) * 3 + q) * kP + kPer * w + i;
+                mp[2 * o] = acc[q][i].r;
+                mp[2 * o + 1] = acc[q][i].i;
+            }
 }
 
-// Local expansions of every target body from its far bodies: grid m, block
-// 256 = kSlots groups of kP lanes; group g translates far sources g, g +
-// kSlots, ...; lane k forms coefficient k.  lc[t][q][k], q < 4: F, S, E, G.
-__global__ __launch_bounds__(kBlk) void k_fmm_m2l(const double* __restrict__ cr,
-                                                  const double* __restrict__ mp,
-                                                  const int* __restrict__ far_ptr,
-                                                  const int* __restrict__ far_idx,
-                                                  double* __restrict__ lc) {
-    __shared__ double binom[2 * kP - 1][kP];
-    __shared__ double U[kSlots][kP][2];
-    __shared__ double MM[kSlots][3][kP][2];
-    __shared__ double RED[kSlots][4][kP][2];
-    const int t = blockIdx.x, tid = threadIdx.x;
-    for (int e = tid; e < (2 * kP - 1) * kP; e += kBlk) {
-        const int i = e / kP, k = e % kP;
-        double v = 0.0;
-        if (k <= i) {
-            v = 1.0;
-            for (int r = 1; r <= k; ++r) v = v * static_cast<double>(i - k + r) / r;
-        }
-        binom[i][k] = v;
-    }
-    const int slot = tid / kP, k = tid % kP;
+// Local expansions of every target cluster from its far clusters, on the
+// matrix cores.  For a batch of 16 far sources j and one series q, the
+// translation is a real matrix product followed by a complex scaling:
+//   Y[k][j] = sum_n B[k][n] W[n][j],   B[k][n] = binom(n + k, k),
+//   W[n][j] = (-R_j/d_j)^n M_j[n]      (real and imaginary parts apart),
+//   L[k]   += f_j(k) Y[k][j],          f_j(k) = -(1/d_j) (R_t/d_j)^k,
+// so that v_mfma_f64_16x16x4f64 forms Y (k in two tiles of 16, n in eight
+// steps of 4) and the vector ALU the powers and the scaling.  Lane l of a
+// wave holds source j = l % 16 of the batch: W rows n = 4 s + l / 16 (step s)
+// and Y rows k = 16 kt + l / 16 + 4 r (register r).  Grid nc, block 256 =
+// 4 waves taking batches of 16 sources in turn; the waves' sums meet in LDS.
+// bin holds B in the A-operand layout: bin[(kt * 8 + s) * 64 + l] =
+// binom(4 s + l / 16 + 16 kt + l % 16, 16 kt + l % 16).  lc[t][q][k], q < 3:
+// F, S and H = E - G.
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(256) void k_fmm_m2l(const double* __restrict__ cr,
+                                                 const double* __restrict__ mp,
+                                                 const int* __restrict__ far_ptr,
+                                                 const int* __restrict__ far_idx,
+                                                 const double* __restrict__ bin,
+                                                 double* __restrict__ lc) {
+    __shared__ double RED[4][3][kP][2];
+    const int t = blockIdx.x, w = threadIdx.x / 64, l = threadIdx.x % 64;
+    const int j = l % 16, h = l / 16;
+    double A[2][8];
+#pragma unroll
+    for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+        for (int st = 0; st < 8; ++st) A[kt][st] = bin[(kt * 8 + st) * 64 + l];
     const double ctx = cr[3 * t], cty = cr[3 * t + 1], Rt = cr[3 * t + 2];
-    double L[4][2] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+    cplx LF[8], LS[8], LH[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) LF[e] = LS[e] = LH[e] = cmk(0.0, 0.0);
     const int f0 = far_ptr[t], nf = far_ptr[t + 1] - f0;
-    __syncthreads();
-    for (int i0 = 0; i0 < nf; i0 += kSlots) {  // uniform over the block
-        const int i = i0 + slot;
-        const bool act = i < nf;
-        cplx v = cmk(0.0, 0.0), id = cmk(0.0, 0.0), md = cmk(0.0, 0.0);
-        if (act) {
-            const int s = far_idx[f0 + i];
+    for (int b0 = 16 * w; b0 < nf; b0 += 64) {  // uniform over the wave
+        const int i = b0 + j;
+        const int s = i < nf ? far_idx[f0 + i] : -1;
+        cplx id = cmk(0.0, 0.0), u = cmk(0.0, 0.0), v = cmk(0.0, 0.0), cmd = cmk(0.0, 0.0);
+        if (s >= 0) {
             const cplx d = cmk(cr[3 * s] - ctx, cr[3 * s + 1] - cty);
             id = cinv(d);
-            const cplx u = cscale(id, -cr[3 * s + 2]);  // -R_s / d
-            v = cscale(id, Rt);                        // R_t / d
-            md = cmk(-d.r, -d.i);                      // c_t - c_s
-            const cplx un = cpow_int(u, k);
-            U[slot][k][0] = un.r;
-            U[slot][k][1] = un.i;
-#pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const size_t o = (static_cast<size_t>(s) * 3 + q) * kP + k;
-                MM[slot][q][k][0] = mp[2 * o];
-                MM[slot][q][k][1] = mp[2 * o + 1];
-            }
+            u = cscale(id, -cr[3 * s + 2]);  // -R_s / d
+            v = cscale(id, Rt);              // R_t / d
+            cmd = cmk(-d.r, d.i);            // conj(c_t - c_s)
         }
-        __syncthreads();
-        if (act) {
-            cplx acc[3] = {cmk(0.0, 0.0), cmk(0.0, 0.0), cmk(0.0, 0.0)};
-            // a few terms at a time: the whole unrolled translation spills
-#pragma unroll 4
-            for (int nn = 0; nn < kP; ++nn) {
-                const double bb = binom[nn + k][k];
-                const cplx tn = cmk(bb * U[slot][nn][0], bb * U[slot][nn][1]);
+        // f(k) for this lane's rows k = 16 kt + h + 4 r
+        const cplx v2 = cmul(v, v), v4 = cmul(v2, v2), v8 = cmul(v4, v4), v16 = cmul(v8, v8);
+        const cplx vh = h == 0 ? cmk(1.0, 0.0) : h == 1 ? v : h == 2 ? v2 : cmul(v2, v);
+        cplx f[8];
+        f[0] = cmul(cmk(-id.r, -id.i), vh);
 #pragma unroll
-                for (int q = 0; q < 3; ++q)
-                    cfma(acc[q], tn, cmk(MM[slot][q][nn][0], MM[slot][q][nn][1]));
-            }
-            const cplx f = cmul(cpow_int(v, k), cmk(-id.r, -id.i));  // -(R_t/d)^k / d
+        for (int r = 1; r < 4; ++r) f[r] = cmul(f[r - 1], v4);
 #pragma unroll
-            for (int q = 0; q < 3; ++q) {
-                const cplx w = cmul(f, acc[q]);
-                L[q][0] += w.r;
-                L[q][1] += w.i;
+        for (int r = 0; r < 4; ++r) f[4 + r] = cmul(f[r], v16);
+        // u^h and u^4 for this lane's rows n = 4 st + h
+        const cplx u2 = cmul(u, u), u4 = cmul(u2, u2);
+        const cplx uh = h == 0 ? cmk(1.0, 0.0) : h == 1 ? u : h == 2 ? u2 : cmul(u2, u);
+        const double* M = mp + static_cast<size_t>(s >= 0 ? s : 0) * 3 * kP * 2;
+        // E first, then F (whose product also enters H), then S: one Y live
+#pragma unroll
+        for (int qi = 0; qi < 3; ++qi) {
+            const int q = qi == 0 ? 2 : qi == 1 ? 0 : 1;
+            double wr[8], wi[8];
+            cplx up = uh;
+#pragma unroll
+            for (int st = 0; st < 8; ++st) {
+                const int nn = 4 * st + h;
+                const cplx mm = s >= 0 ? cmk(M[2 * (q * kP + nn)], M[2 * (q * kP + nn) + 1])
+                                       : cmk(0.0, 0.0);
+                const cplx wv = cmul(up, mm);
+                wr[st] = wv.r;
+                wi[st] = wv.i;
+                up = cmul(up, u4);
             }
-            const cplx g = cmul(cmk(md.r, -md.i), cmul(f, acc[0]));
-            L[3][0] += g.r;
-            L[3][1] += g.i;
+            v4d yr[2], yi[2];
+#pragma unroll
+            for (int kt = 0; kt < 2; ++kt) {
+                yr[kt] = v4d{0.0, 0.0, 0.0, 0.0};
+                yi[kt] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int st = 0; st < 8; ++st) {
+                    yr[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[kt][st], wr[st], yr[kt], 0, 0, 0);
+                    yi[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[kt][st], wi[st], yi[kt], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int e = 4 * kt + r;
+                    const cplx fy = cmul(f[e], cmk(yr[kt][r], yi[kt][r]));
+                    if (q == 2) {
+                        LH[e] = cadd(LH[e], fy);
+                    } else if (q == 0) {
+                        LF[e] = cadd(LF[e], fy);
+                        LH[e] = csub(LH[e], cmul(cmd, fy));
+                    } else {
+                        LS[e] = cadd(LS[e], fy);
+                    }
+                }
         }
-        __syncthreads();
     }
+    // sum over the 16 sources of a batch (lanes with the same l / 16)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        RED[slot][q][k][0] = L[q][0];
-        RED[slot][q][k][1] = L[q][1];
+    for (int e = 0; e < 8; ++e)
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            LF[e].r += __shfl_xor(LF[e].r, o);
+            LF[e].i += __shfl_xor(LF[e].i, o);
+            LS[e].r += __shfl_xor(LS[e].r, o);
+            LS[e].i += __shfl_xor(LS[e].i, o);
+            LH[e].r += __shfl_xor(LH[e].r, o);
+            LH[e].i += __shfl_xor(LH[e].i, o);
+        }
+    if (j == 0) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int k = 16 * (e / 4) + h + 4 * (e % 4);
+            RED[w][0][k][0] = LF[e].r;
+            RED[w][0][k][1] = LF[e].i;
+            RED[w][1][k][0] = LS[e].r;
+            RED[w][1][k][1] = LS[e].i;
+            RED[w][2][k][0] = LH[e].r;
+            RED[w][2][k][1] = LH[e].i;
+        }
     }
     __syncthreads();
-    if (tid < 4 * kP) {
-        const int q = tid / kP, kk = tid % kP;
+    const int tid = threadIdx.x;
+    if (tid < 3 * kP) {
+        const int q = tid / kP, k = tid % kP;
         double sr = 0.0, si = 0.0;
-        for (int sl = 0; sl < kSlots; ++sl) {
-            sr += RED[sl][q][kk][0];
-            si += RED[sl][q][kk][1];
+#pragma unroll
+        for (int ww = 0; ww < 4; ++ww) {
+            sr += RED[ww][q][k][0];
+            si += RED[ww][q][k][1];
         }
-        const size_t o = (static_cast<size_t>(t) * 4 + q) * kP + kk;
+        const size_t o = (static_cast<size_t>(t) * 3 + q) * kP + k;
         lc[2 * o] = sr;
         lc[2 * o + 1] = si;
     }
 }
 
-// Body targets: grid (m, ysplit), block BT threads; lane l owns the body's
-// points l and l + BT (two targets per lane, so that they share one
-// reciprocal per source; n <= 2 BT).  Block (b, y) sums the source tiles y,
-// y + ysplit, ... of body b's list (wall tiles and the tiles of its near
-// bodies, itself included) directly; blocks y = 0 also add the local
-// expansion.  Writes part_u/part_v[y][point].  tiles holds (first point,
-// count) per tile.
-template <int BT>
-__global__ __launch_bounds__(BT) void k_fmm_eval(
+// Direct sums: one block per work item (cluster c, tile j of c's source
+// list: the tiles of its near clusters, its own included), block BT <= 128
+// threads with TPT targets per lane (lane l owns the cluster's points l, l +
+// BT, ..., sharing one reciprocal per source).  Writes part_u/part_v[j][point]
+// for the cluster's points.  work holds (cluster, j) per item and tiles
+// (first point, count) per tile.
+template <int TPT>
+__global__ __launch_bounds__(128) void k_fmm_direct(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ mx,
     const double* __restrict__ my, const double* __restrict__ ex, const double* __restrict__ ey,
-    const double* __restrict__ cr, const double* __restrict__ lc,
-    const int* __restrict__ tile_ptr, const int* __restrict__ tiles, int no, int n, int npts,
-    int ysplit, double* __restrict__ part_u, double* __restrict__ part_v) {
-    constexpr int TPT = 2;
+    const int* __restrict__ cl, const int* __restrict__ work, const int* __restrict__ tile_ptr,
+    const int* __restrict__ tiles, int npts, double* __restrict__ part_u,
+    double* __restrict__ part_v) {
     __shared__ double2 sxy[kTile], sab[kTile];
     __shared__ double sc[kTile];
-    __shared__ double loc[4][kP][2];
-    const int b = blockIdx.x, ys = blockIdx.y, tid = threadIdx.x;
-    const int off = no + b * n;
+    const int c = work[2 * blockIdx.x], j = work[2 * blockIdx.x + 1];
+    const int tid = threadIdx.x, bt = blockDim.x;
+    const int off = cl[2 * c], len = cl[2 * c + 1];
+    const int ti = tile_ptr[c] + j;
+    const int ts = tiles[2 * ti], tl = tiles[2 * ti + 1];
+    for (int e = tid; e < kTile; e += bt) {
+        if (e < tl) {
+            const int k = ts + e;
+            sxy[e] = make_double2(x[k], y[k]);
+            src_moments(mx[k], my[k], ex[k], ey[k], &sab[e], &sc[e]);
+        } else {
+            sxy[e] = make_double2(1.0e30, 1.0e30);
+            sab[e] = make_double2(0.0, 0.0);
+            sc[e] = 0.0;
+        }
+    }
     double xt[TPT], yt[TPT], u[TPT], v[TPT];
     int tp[TPT];
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
         // unused slots sit where no source is (they share a product of rho^2)
-        const int k = tid + q * BT;
-        tp[q] = k < n ? off + k : -1;
+        const int k = tid + q * bt;
+        tp[q] = k < len ? off + k : -1;
         xt[q] = tp[q] >= 0 ? x[tp[q]] : kNoTarget;
         yt[q] = tp[q] >= 0 ? y[tp[q]] : kNoTarget;
         u[q] = v[q] = 0.0;
     }
-    const int t0 = tile_ptr[b], nt = tile_ptr[b + 1] - t0;
-    for (int i = ys; i < nt; i += ysplit) {  // uniform over the block
-        const int ts = tiles[2 * (t0 + i)], tl = tiles[2 * (t0 + i) + 1];
-        __syncthreads();
-        for (int e = tid; e < kTile; e += BT) {
-            if (e < tl) {
-                const int j = ts + e;
-                sxy[e] = make_double2(x[j], y[j]);
-                src_moments(mx[j], my[j], ex[j], ey[j], &sab[e], &sc[e]);
-            } else {
-                sxy[e] = make_double2(1.0e30, 1.0e30);
-                sab[e] = make_double2(0.0, 0.0);
-                sc[e] = 0.0;
-            }
-        }
-        __syncthreads();
-        // a tile of the body itself holds the self pairs (zero distance, zero
-        // moment product): the DIAG form keeps their shared product finite
-        if (ts >= off && ts < off + n)
-            ff_tile_shared<TPT, true>(sxy, sab, sc, xt, yt, u, v);
-        else
-            ff_tile_shared<TPT, false>(sxy, sab, sc, xt, yt, u, v);
-    }
-    if (ys == 0) {  // uniform over the block
-        for (int e = tid; e < 4 * kP; e += BT) {
-            const size_t o = static_cast<size_t>(b) * 4 * kP + e;
-            loc[e / kP][e % kP][0] = lc[2 * o];
-            loc[e / kP][e % kP][1] = lc[2 * o + 1];
-        }
-        __syncthreads();
-        const double ctx = cr[3 * b], cty = cr[3 * b + 1], Rt = cr[3 * b + 2];
-        const double iR = 1.0 / Rt;
-#pragma unroll
-        for (int q = 0; q < TPT; ++q) {
-            if (tp[q] < 0) continue;
-            const double dx = xt[q] - ctx, dy = yt[q] - cty;
-            const cplx z = cmk(dx * iR, dy * iR);
-            // Horner for the value and the derivative of each local series
-            cplx val[4], der[4];
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                cplx pv = cmk(loc[e][kP - 1][0], loc[e][kP - 1][1]), pd = cmk(0.0, 0.0);
-                for (int kk = kP - 2; kk >= 0; --kk) {
-                    pd = cadd(cmul(pd, z), pv);
-                    pv = cadd(cmul(pv, z), cmk(loc[e][kk][0], loc[e][kk][1]));
-                }
-                val[e] = pv;
-                der[e] = cscale(pd, iR);
-            }
-            // w = E' - conj(z - c_t) F' - G'
-            const cplx w = csub(csub(der[2], cmul(cmk(dx, -dy), der[0])), der[3]);
-            u[q] += 0.25 * (val[0].r + val[1].r + w.r);
-            v[q] += 0.25 * (val[0].i - val[1].i - w.i);
-        }
-    }
+    __syncthreads();
+    // a tile that holds points of the cluster itself holds self pairs (zero
+    // distance, zero moment product): the DIAG form keeps their shared
+    // product finite
+    if (ts < off + len && ts + tl > off)
+        ff_tile_shared<TPT, true>(sxy, sab, sc, xt, yt, u, v);
+    else
+        ff_tile_shared<TPT, false>(sxy, sab, sc, xt, yt, u, v);
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
         if (tp[q] >= 0) {
-            part_u[static_cast<size_t>(ys) * npts + tp[q]] = u[q];
-            part_v[static_cast<size_t>(ys) * npts + tp[q]] = v[q];
+            part_u[static_cast<size_t>(j) * npts + tp[q]] = u[q];
+            part_v[static_cast<size_t>(j) * npts + tp[q]] = v[q];
         }
     }
+}
+
+// Local expansions at every target point, plus the point's direct partial
+// sums (one thread per point, blocks of 256): wall point p belongs to cluster
+// p / kWallSeg, body point p >= no to cluster nwall + (p - no) / n.  Sums
+// part_u/part_v[j][p] over the cluster's tiles j and writes the total to
+// part_u/part_v[0][p].  lc[c][q][k], q < 3: F, S and H = E - G.
+__global__ __launch_bounds__(kBlk) void k_fmm_l2p(const double* __restrict__ x,
+                                                  const double* __restrict__ y,
+                                                  const double* __restrict__ cr,
+                                                  const double* __restrict__ lc,
+                                                  const int* __restrict__ tile_ptr, int no, int n,
+                                                  int nwall, int npts, double* __restrict__ part_u,
+                                                  double* __restrict__ part_v) {
+    const int p = blockIdx.x * kBlk + threadIdx.x;
+    if (p >= npts) return;
+    const int c = p < no ? p / kWallSeg : nwall + (p - no) / n;
+    const double dx = x[p] - cr[3 * c], dy = y[p] - cr[3 * c + 1];
+    const double iR = 1.0 / cr[3 * c + 2];
+    const cplx z = cmk(dx * iR, dy * iR);
+    const double* L = lc + static_cast<size_t>(c) * 3 * kP * 2;
+    // Horner: F and its derivative, S, the derivative of H
+    cplx fv = cmk(L[2 * (kP - 1)], L[2 * (kP - 1) + 1]), fd = cmk(0.0, 0.0);
+    cplx sv = cmk(L[2 * (2 * kP - 1)], L[2 * (2 * kP - 1) + 1]);
+    cplx hd = cmk(0.0, 0.0), hv = cmk(L[2 * (3 * kP - 1)], L[2 * (3 * kP - 1) + 1]);
+#pragma unroll 4
+    for (int k = kP - 2; k >= 0; --k) {
+        fd = cadd(cmul(fd, z), fv);
+        fv = cadd(cmul(fv, z), cmk(L[2 * k], L[2 * k + 1]));
+        sv = cadd(cmul(sv, z), cmk(L[2 * (kP + k)], L[2 * (kP + k) + 1]));
+        hd = cadd(cmul(hd, z), hv);
+        hv = cadd(cmul(hv, z), cmk(L[2 * (2 * kP + k)], L[2 * (2 * kP + k) + 1]));
+    }
+    // w = H' - conj(z - c_t) F'
+    const cplx w = csub(cscale(hd, iR), cmul(cmk(dx, -dy), cscale(fd, iR)));
+    double u = 0.25 * (fv.r + sv.r + w.r), v = 0.25 * (fv.i - sv.i - w.i);
+    const int nt = tile_ptr[c + 1] - tile_ptr[c];
+    for (int j = 0; j < nt; ++j) {
+        u += part_u[static_cast<size_t>(j) * npts + p];
+        v += part_v[static_cast<size_t>(j) * npts + p];
+    }
+    part_u[p] = u;
+    part_v[p] = v;
 }
 
 }  // namespace
 
 bool tilefmm_available(const Context& c) {
     const Geometry& g = c.g;
-    return g.m >= 2 && g.n >= 4 && g.n <= kMaxBodyPts;
+    return g.m >= 1 && g.n >= 4 && g.n <= kMaxBodyPts;
 }
 
-// Cluster data and lists of the current geometry (once per geometry).
+namespace {
+
+struct Cluster {
+    double cx, cy, r;
+    int start, len;
+};
+
+// Centre (given, or the bounding box's) and radius of points [start, start + len).
+Cluster make_cluster(const Geometry& g, int start, int len, bool bbox, double cx, double cy) {
+    if (bbox) {
+        double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+        for (int k = start; k < start + len; ++k) {
+            x0 = std::min(x0, g.h_x[static_cast<size_t>(k)]);
+            x1 = std::max(x1, g.h_x[static_cast<size_t>(k)]);
+            y0 = std::min(y0, g.h_y[static_cast<size_t>(k)]);
+            y1 = std::max(y1, g.h_y[static_cast<size_t>(k)]);
+        }
+        cx = 0.5 * (x0 + x1);
+        cy = 0.5 * (y0 + y1);
+    }
+    double r2 = 0.0;
+    for (int k = start; k < start + len; ++k) {
+        const double dx = g.h_x[static_cast<size_t>(k)] - cx, dy = g.h_y[static_cast<size_t>(k)] - cy;
+        r2 = std::max(r2, dx * dx + dy * dy);
+    }
+    return Cluster{cx, cy, std::max(std::sqrt(r2), 1e-300), start, len};
+}
+
+}  // namespace
+
+// Clusters, their lists and the expansion buffers of the current geometry
+// (once per geometry).
 void tilefmm_prepare(Context& c) {
     Geometry& g = c.g;
     TileFmm& F = c.fmm;
-    if (!tilefmm_available(c)) throw Error("the body-cluster far field needs 2+ bodies of <= 512 points");
+    if (!tilefmm_available(c)) throw Error("the cluster far field needs bodies of 4 to 512 points");
     if (F.version == g.version) return;
     const int n = g.n, m = g.m, no = g.no;
-    std::vector<double> cr(3 * static_cast<size_t>(m));
-    for (int b = 0; b < m; ++b) {
-        const double cx = g.h_cx[static_cast<size_t>(b)], cy = g.h_cy[static_cast<size_t>(b)];
-        double r2 = 0.0;
-        for (int k = 0; k < n; ++k) {
-            const size_t p = static_cast<size_t>(no + b * n + k);
-            const double dx = g.h_x[p] - cx, dy = g.h_y[p] - cy;
-            r2 = std::max(r2, dx * dx + dy * dy);
-        }
-        cr[3 * static_cast<size_t>(b)] = cx;
-        cr[3 * static_cast<size_t>(b) + 1] = cy;
-        cr[3 * static_cast<size_t>(b) + 2] = std::max(std::sqrt(r2), 1e-300);
+    std::vector<Cluster> C;
+    for (int w0 = 0; w0 < no; w0 += kWallSeg)
+        C.push_back(make_cluster(g, w0, std::min(kWallSeg, no - w0), true, 0.0, 0.0));
+    const int nwall = static_cast<int>(C.size());
+    for (int b = 0; b < m; ++b)
+        C.push_back(make_cluster(g, no + b * n, n, false, g.h_cx[static_cast<size_t>(b)],
+                                 g.h_cy[static_cast<size_t>(b)]));
+    const int nc = static_cast<int>(C.size());
+    std::vector<double> cr(3 * static_cast<size_t>(nc));
+    std::vector<int> cl(2 * static_cast<size_t>(nc));
+    for (int t = 0; t < nc; ++t) {
+        cr[3 * static_cast<size_t>(t)] = C[static_cast<size_t>(t)].cx;
+        cr[3 * static_cast<size_t>(t) + 1] = C[static_cast<size_t>(t)].cy;
+        cr[3 * static_cast<size_t>(t) + 2] = C[static_cast<size_t>(t)].r;
+        cl[2 * static_cast<size_t>(t)] = C[static_cast<size_t>(t)].start;
+        cl[2 * static_cast<size_t>(t) + 1] = C[static_cast<size_t>(t)].len;
     }
-    std::vector<int> far_ptr(static_cast<size_t>(m) + 1, 0), far_idx, tile_ptr(static_cast<size_t>(m) + 1, 0),
-        tiles;
+    std::vector<int> far_ptr(static_cast<size_t>(nc) + 1, 0), far_idx,
+        tile_ptr(static_cast<size_t>(nc) + 1, 0), tiles;
+    std::vector<std::pair<int, int>> near;  // (first point, count)
     long long direct = 0;
-    for (int t = 0; t < m; ++t) {
-        for (int w0 = 0; w0 < no; w0 += kTile) {  // the wall, directly
-            tiles.push_back(w0);
-            tiles.push_back(std::min(kTile, no - w0));
-        }
-        int nsrc = no;
-        for (int s = 0; s < m; ++s) {
-            const double dx = cr[3 * static_cast<size_t>(s)] - cr[3 * static_cast<size_t>(t)];
-            const double dy = cr[3 * static_cast<size_t>(s) + 1] - cr[3 * static_cast<size_t>(t) + 1];
-            const double rs = cr[3 * static_cast<size_t>(s) + 2] + cr[3 * static_cast<size_t>(t) + 2];
-            if (s != t && rs <= kSep * std::sqrt(dx * dx + dy * dy)) {
+    for (int t = 0; t < nc; ++t) {
+        const Cluster& T = C[static_cast<size_t>(t)];
+        near.clear();
+        long long nsrc = 0;
+        for (int s = 0; s < nc; ++s) {
+            const Cluster& S = C[static_cast<size_t>(s)];
+            const double dx = S.cx - T.cx, dy = S.cy - T.cy;
+            if (s != t && S.r + T.r <= kSep * std::sqrt(dx * dx + dy * dy)) {
                 far_idx.push_back(s);
             } else {
-                for (int k0 = 0; k0 < n; k0 += kTile) {
-                    tiles.push_back(no + s * n + k0);
-                    tiles.push_back(std::min(kTile, n - k0));
-                }
-                nsrc += n;
+                near.emplace_back(S.start, S.len);
+                nsrc += S.len;
             }
+        }
+        // near clusters adjacent in memory (runs of wall segments) merge
+        // into one range before it is cut into tiles
+        std::sort(near.begin(), near.end());
+        for (size_t i = 0; i < near.size();) {
+            const int st = near[i].first;
+            int en = st + near[i].second;
+            size_t j = i + 1;
+            while (j < near.size() && near[j].first == en) en += near[j++].second;
+            for (int k0 = st; k0 < en; k0 += kTile) {
+                tiles.push_back(k0);
+                tiles.push_back(std::min(kTile, en - k0));
+            }
+            i = j;
         }
         far_ptr[static_cast<size_t>(t) + 1] = static_cast<int>(far_idx.size());
         tile_ptr[static_cast<size_t>(t) + 1] = static_cast<int>(tiles.size() / 2);
-        direct += static_cast<long long>(n) * nsrc;
+        direct += static_cast<long long>(T.len) * nsrc;
     }
     hipStream_t s = c.stream;
     auto up = [&](auto& buf, const auto& vec) {
@@ -359,66 +476,114 @@ void tilefmm_prepare(Context& c) {
                                       hipMemcpyHostToDevice, s));
     };
     up(F.cr, cr);
+    up(F.cl, cl);
     up(F.far_ptr, far_ptr);
     up(F.far_idx, far_idx);
     up(F.tile_ptr, tile_ptr);
     up(F.tiles, tiles);
     STOKES_HIP(hipStreamSynchronize(s));
-    F.mp.ensure(static_cast<size_t>(m) * 3 * kP * 2);
-    F.lc.ensure(static_cast<size_t>(m) * 4 * kP * 2);
-    int maxtiles = 0;
-    for (int t = 0; t < m; ++t)
-        maxtiles = std::max(maxtiles, tile_ptr[static_cast<size_t>(t) + 1] - tile_ptr[static_cast<size_t>(t)]);
-    // about 2048 blocks over the bodies and their tile lists
-    F.ysplit = std::max(1, std::min(maxtiles, cdiv_int(2048, m)));
+    if (!F.bin.p) {
+        // binom(n + k, k) for n, k < 32 (exact in long double: < 2^63), in
+        // the matrix cores' A-operand layout
+        long double pas[2 * kP - 1][2 * kP - 1] = {};
+        for (int a = 0; a < 2 * kP - 1; ++a) {
+            pas[a][0] = 1.0L;
+            for (int b = 1; b <= a; ++b) pas[a][b] = pas[a - 1][b - 1] + (b < a ? pas[a - 1][b] : 0.0L);
+        }
+        std::vector<double> bin(2 * 8 * 64);
+        for (int kt = 0; kt < 2; ++kt)
+            for (int st = 0; st < 8; ++st)
+                for (int l = 0; l < 64; ++l) {
+                    const int k = 16 * kt + l % 16, nn = 4 * st + l / 16;
+                    bin[static_cast<size_t>((kt * 8 + st) * 64 + l)] = static_cast<double>(pas[nn + k][k]);
+                }
+        up(F.bin, bin);
+        STOKES_HIP(hipStreamSynchronize(s));
+    }
+    F.mp.ensure(static_cast<size_t>(nc) * 3 * kP * 2);
+    F.lc.ensure(static_cast<size_t>(nc) * 3 * kP * 2);
+    F.nc = nc;
+    F.nwall = nwall;
+    // one work item per (target cluster, source tile): bodies, then the wall
+    std::vector<int> work;
+    int max_tiles = 1;
+    for (int part = 0; part < 2; ++part) {
+        const int c0 = part == 0 ? nwall : 0, c1 = part == 0 ? nc : nwall;
+        for (int t = c0; t < c1; ++t) {
+            const int nt = tile_ptr[static_cast<size_t>(t) + 1] - tile_ptr[static_cast<size_t>(t)];
+            max_tiles = std::max(max_tiles, nt);
+            for (int j = 0; j < nt; ++j) {
+                work.push_back(t);
+                work.push_back(j);
+            }
+        }
+        (part == 0 ? F.work_body : F.work_wall) = static_cast<int>(work.size() / 2) -
+                                                  (part == 0 ? 0 : F.work_body);
+    }
+    up(F.work, work);
+    STOKES_HIP(hipStreamSynchronize(s));
+    F.max_tiles = max_tiles;
     F.far_pairs = static_cast<long long>(far_idx.size());
-    // direct pairs: body targets (their lists) and wall targets (every point)
-    F.direct_pairs = direct + static_cast<long long>(no) * (g.npts - 1);
-    F.wall_nsplit = 0;
+    F.direct_pairs = direct - static_cast<long long>(g.npts);  // no self pairs
     F.version = g.version;
 }
 
-// The far field of the current density (c.ex, c.ey) into part_u/part_v:
-// wall targets in *wall_nsplit source splits, body targets in *body_nsplit.
+namespace {
+
+inline int round32(int v) { return (v + 31) / 32 * 32; }
+
+// Direct sums of work items [w0, w0 + count) (clusters of len <= TPT bt).
+void direct_launch(Context& c, int tpt, int bt, int w0, int count) {
+    Geometry& g = c.g;
+    TileFmm& F = c.fmm;
+    if (count <= 0) return;
+    const int* work = F.work.p + 2 * static_cast<size_t>(w0);
+    if (tpt == 4)
+        k_fmm_direct<4><<<count, bt, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
+                                                    F.cl.p, work, F.tile_ptr.p, F.tiles.p, g.npts,
+                                                    c.part_u.p, c.part_v.p);
+    else
+        k_fmm_direct<2><<<count, bt, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
+                                                    F.cl.p, work, F.tile_ptr.p, F.tiles.p, g.npts,
+                                                    c.part_u.p, c.part_v.p);
+    STOKES_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+// The far field of the current density (c.ex, c.ey) into part_u/part_v[0]
+// (*wall_nsplit = *body_nsplit = 1): the direct sums one source tile per
+// block, then the local expansions, which also sum each point's tiles.
 void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit) {
     Geometry& g = c.g;
     TileFmm& F = c.fmm;
     tilefmm_prepare(c);
     hipStream_t s = c.stream;
-    const int np = g.npts, m = g.m, n = g.n, no = g.no;
-    // wall targets against every source: k_farfield, 2 targets per lane with
-    // the shared reciprocal, about 512 blocks
-    const int tb = cdiv_int(no, 2 * kBlk);
-    const int want = std::max(1, std::min(64, cdiv_int(512, tb)));
-    const int split_len = cdiv_int(cdiv_int(np, want), kTile) * kTile;
-    const int wsplit = cdiv_int(np, split_len);
-    const int need = std::max(wsplit, F.ysplit);
-    c.part_u.ensure(static_cast<size_t>(np) * need);
-    c.part_v.ensure(static_cast<size_t>(np) * need);
-    farfield_wall_targets(c, split_len, wsplit);
-    k_fmm_p2m<<<m, 128, 0, s>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p, F.cr.p, no, n,
-                                F.mp.p);
+    const int np = g.npts, nc = F.nc, n = g.n;
+    c.part_u.ensure(static_cast<size_t>(np) * F.max_tiles);
+    c.part_v.ensure(static_cast<size_t>(np) * F.max_tiles);
+    if (c.time_kernels) c.t_fmm_expand.start(s);
+    k_fmm_p2m<<<nc, 256, 0, s>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p, F.cr.p, F.cl.p,
+                                 F.mp.p);
     STOKES_LAUNCH_CHECK();
-    k_fmm_m2l<<<m, kBlk, 0, s>>>(F.cr.p, F.mp.p, F.far_ptr.p, F.far_idx.p, F.lc.p);
+    k_fmm_m2l<<<nc, 256, 0, s>>>(F.cr.p, F.mp.p, F.far_ptr.p, F.far_idx.p, F.bin.p, F.lc.p);
     STOKES_LAUNCH_CHECK();
-    const dim3 grid(m, F.ysplit);
-    if (c.time_kernels) c.t_fmm_direct.start(s);
-    // two targets per lane: 128-thread blocks up to 256 points per body
-    if (n <= 256)
-        k_fmm_eval<128><<<grid, 128, 0, s>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p, F.cr.p,
-                                             F.lc.p, F.tile_ptr.p, F.tiles.p, no, n, np, F.ysplit,
-                                             c.part_u.p, c.part_v.p);
+    if (c.time_kernels) {
+        c.t_fmm_expand.stop(s, static_cast<double>(F.far_pairs));
+        c.t_fmm_direct.start(s);
+    }
+    // bodies: 4 targets per lane above 128 points, else 2; wall segments: 2
+    if (n > 128)
+        direct_launch(c, 4, round32(cdiv_int(n, 4)), 0, F.work_body);
     else
-        k_fmm_eval<kBlk><<<grid, kBlk, 0, s>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
-                                               F.cr.p, F.lc.p, F.tile_ptr.p, F.tiles.p, no, n, np,
-                                               F.ysplit, c.part_u.p, c.part_v.p);
+        direct_launch(c, 2, round32(cdiv_int(n, 2)), 0, F.work_body);
+    direct_launch(c, 2, round32(cdiv_int(kWallSeg, 2)), F.work_body, F.work_wall);
+    k_fmm_l2p<<<cdiv_int(np, kBlk), kBlk, 0, s>>>(g.x.p, g.y.p, F.cr.p, F.lc.p, F.tile_ptr.p,
+                                                 g.no, n, F.nwall, np, c.part_u.p, c.part_v.p);
     STOKES_LAUNCH_CHECK();
-    if (c.time_kernels)
-        c.t_fmm_direct.stop(s, static_cast<double>(F.direct_pairs) -
-                                   static_cast<double>(no) * static_cast<double>(np - 1));
-    F.wall_nsplit = wsplit;
-    *wall_nsplit = wsplit;
-    *body_nsplit = F.ysplit;
+    if (c.time_kernels) c.t_fmm_direct.stop(s, static_cast<double>(F.direct_pairs));
+    *wall_nsplit = 1;
+    *body_nsplit = 1;
 }
 
 }  // namespace stokes

@@ -253,7 +253,7 @@ def time_matvec(npts: int, nbods: int, nouter: int, xx, yy, den, ibary: int = 1,
 
 
 TIMER_FAMILIES = ("farfield", "near", "stress_eval", "gram_schmidt", "precond", "farfield32",
-                  "fmm", "fmm_direct", "fmm_wall")
+                  "fmm", "fmm_direct", "fmm_expand")
 
 
 def kernel_timing(enable: bool) -> None:
