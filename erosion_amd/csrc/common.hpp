@@ -209,6 +209,13 @@ struct TileFmm {
     int max_tiles = 1;                  // most source tiles of one cluster
     long long far_pairs = 0;            // (target cluster, far cluster) pairs
     long long direct_pairs = 0;         // point pairs summed directly per application
+    // shear stress: far bodies of each body (CSR, body indices), the multipoles
+    // and locals of its five boundary-value series, and (host) the near
+    // bodies of each body, itself included, for the barycentric rule
+    DevBuf<int> sfar_ptr, sfar_idx;
+    DevBuf<double> smp, slc;            // [m][5][32] complex
+    std::vector<int> h_snear_ptr, h_snear_idx;
+    long long stress_direct_pairs = 0;  // (target point, source node) pairs of the near curves
 };
 
 // Per-solve GMRES state kept on the device.
@@ -322,7 +329,15 @@ struct Context {
     // stress / target-evaluation buffers
     DevBuf<double> spart;   // per curve-group partials
     DevBuf<double> tau;
-    DevBuf<int> curves_all, grp_beg, grp_end, iside;
+    DevBuf<int> curves_all, iside;
+    // barycentric curve lists: CSR over (target block, group), curve 0 the
+    // wall and 1 + b body b; a per-(body, curve) mask where the lists of one
+    // block serve several bodies.  stress_lists_version: geometry version of
+    // the near-curve lists of the shear stress (-1: none)
+    DevBuf<int> lst_ptr, lst;
+    DevBuf<unsigned char> lst_mask;
+    long long stress_lists_version = -1;
+    int stress_groups = 0;
     DevBuf<double> tgt;     // target coordinates and QoI outputs
     double tol = 1e-8;
     // right preconditioner (precond.hip): kind 0 none (the reference's
@@ -392,6 +407,9 @@ bool farfield_fp32_pays(const Context& c);
 bool tilefmm_available(const Context& c);
 void tilefmm_prepare(Context& c);
 void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit);
+// strain rate (E11, E12, E22 at out[{0,1,2} nbp + q]) of the far bodies'
+// boundary values in c.bd
+void tilefmm_stress(Context& c, double* out);
 void gather_density(Context& c, const double* den_dev);
 void boundary_values(Context& c, const int* curves_dev, int ncurves, int wall_first);
 void bd_condition(Context& c, int ibc, double* rhs_dev);

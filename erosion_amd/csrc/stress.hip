@@ -27,13 +27,18 @@ constexpr int kBlk = 256;
 
 inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
 
-// nder=2 barycentric evaluation of source curves [grp_beg[g], grp_end[g]) at
-// kBT targets per block, one per lane.  The node data of a source curve
+// nder=2 barycentric evaluation of source curves at kBT targets per block,
+// one per lane.  Block (tb, g) of a grid (target blocks, G) takes the curves
+// lst[lst_ptr[tb G + g] .. lst_ptr[tb G + g + 1]) (curve 0 the wall, 1 + b
+// body b); with a mask (MODE 0), a lane takes curve c only where
+// mask[b (m + 1) + c] is set for its body b.  The node data of a source curve
 // (coordinates, dz, the five boundary values, 1/(z - a)) is staged in LDS
 // kNodeTile nodes at a time and read by every lane with broadcast loads; a
 // curve that fits one tile is staged once for all three passes.  MODE 0
 // (stress): targets are the body points (tgt_x == nullptr), a body's own
 // curve is skipped, and part[(g*3 + {0,1,2}) * ntar + q] = E11, E12, E22.
+// G partial slots per target; the lists of one target block may hold
+// different numbers of curves per g.
 // MODE 1 (targets in the fluid): every curve counts and part[...] = u, v,
 // vorticity.
 constexpr int kBT = 256;
@@ -63,10 +68,10 @@ __device__ __forceinline__ void stage_nodes(NodeTileLds& T, const NodeView& nv, 
 
 template <int MODE>
 __global__ __launch_bounds__(kBT) void k_bary_eval(
-    const int* __restrict__ grp_beg, const int* __restrict__ grp_end, NodeView nv,
-    const double* __restrict__ ax, const double* __restrict__ ay, int no, int n,
-    const double* __restrict__ tgt_x, const double* __restrict__ tgt_y, int ntar,
-    double* __restrict__ spart) {
+    const int* __restrict__ lst_ptr, const int* __restrict__ lst,
+    const unsigned char* __restrict__ mask, int m, NodeView nv, const double* __restrict__ ax,
+    const double* __restrict__ ay, int no, int n, const double* __restrict__ tgt_x,
+    const double* __restrict__ tgt_y, int ntar, double* __restrict__ spart) {
     __shared__ NodeTileLds T;
     const int q = blockIdx.x * kBT + threadIdx.x;
     const bool valid = q < ntar;
@@ -76,9 +81,12 @@ __global__ __launch_bounds__(kBT) void k_bary_eval(
     const double yt = MODE == 0 ? nv.y[no + qq] : tgt_y[qq];
     const cplx zt = cmk(xt, yt);
     double E11 = 0.0, E12 = 0.0, E22 = 0.0;
-    const int g = blockIdx.y;
-    for (int c = grp_beg[g]; c < grp_end[g]; ++c) {
-        const bool skip = !valid || c == 1 + bt;
+    const int g = blockIdx.y, G = gridDim.y;
+    const unsigned char* __restrict__ mrow =
+        MODE == 0 && mask ? mask + static_cast<size_t>(bt) * (m + 1) : nullptr;
+    for (int li = lst_ptr[blockIdx.x * G + g]; li < lst_ptr[blockIdx.x * G + g + 1]; ++li) {
+        const int c = lst[li];
+        const bool skip = !valid || c == 1 + bt || (mrow && !mrow[c]);
         if (__syncthreads_count(skip ? 0 : 1) == 0) continue;  // block-uniform
         const int len = c == 0 ? no : n;
         const int off = c == 0 ? 0 : no + (c - 1) * n;
@@ -630,8 +638,8 @@ __global__ __launch_bounds__(kBlk) void k_pressure_final(
 
 }  // namespace
 
-// Source-curve groups balanced by node count; uploads every curve's boundary
-// values first.  Returns the number of groups; fills c.grp_beg / c.grp_end.
+// Source-curve groups balanced by node count (every curve, the same groups
+// for every target block).  Returns the number of groups.
 static int plan_curve_groups(Context& c, int tiles, std::vector<int>& gb, std::vector<int>& ge) {
     const Geometry& g = c.g;
     const int n = g.n, m = g.m, no = g.no;
@@ -656,6 +664,100 @@ static int plan_curve_groups(Context& c, int tiles, std::vector<int>& gb, std::v
     return static_cast<int>(gb.size());
 }
 
+// CSR lists (target block, group) -> curves, uploaded to c.lst_ptr / c.lst
+// (which then no longer hold the cached near-curve lists).
+static void upload_lists(Context& c, const std::vector<int>& ptr, const std::vector<int>& idx) {
+    c.stress_lists_version = -1;
+    c.lst_ptr.ensure(ptr.size());
+    c.lst.ensure(std::max<size_t>(idx.size(), 1));
+    STOKES_HIP(hipMemcpyAsync(c.lst_ptr.p, ptr.data(), ptr.size() * sizeof(int),
+                              hipMemcpyHostToDevice, c.stream));
+    if (!idx.empty())
+        STOKES_HIP(hipMemcpyAsync(c.lst.p, idx.data(), idx.size() * sizeof(int),
+                                  hipMemcpyHostToDevice, c.stream));
+    STOKES_HIP(hipStreamSynchronize(c.stream));
+}
+
+// Every curve for every target block, in G balanced groups.
+static int all_curve_lists(Context& c, int tiles) {
+    std::vector<int> gb, ge;
+    const int G = plan_curve_groups(c, tiles, gb, ge);
+    std::vector<int> ptr(1, 0), idx;
+    for (int tb = 0; tb < tiles; ++tb)
+        for (int gi = 0; gi < G; ++gi) {
+            for (int cc = gb[static_cast<size_t>(gi)]; cc < ge[static_cast<size_t>(gi)]; ++cc)
+                idx.push_back(cc);
+            ptr.push_back(static_cast<int>(idx.size()));
+        }
+    upload_lists(c, ptr, idx);
+    return G;
+}
+
+// The shear stress's near curves of every target block: the wall and the
+// near bodies (by the cluster rule) of each body in the block, in groups
+// balanced by node count (largest first onto the lightest group), with the
+// per-(body, curve) mask.  Cached per geometry.  Returns the number of groups.
+static int near_curve_lists(Context& c, int tiles) {
+    const Geometry& g = c.g;
+    if (c.stress_lists_version == g.version) return c.stress_groups;
+    const TileFmm& F = c.fmm;
+    const int n = g.n, m = g.m, no = g.no, nbp = n * m;
+    std::vector<unsigned char> mask(static_cast<size_t>(m) * (m + 1), 0);
+    std::vector<std::vector<int>> blk(static_cast<size_t>(tiles));
+    long long nodes = 0;
+    int heaviest = std::max(no, n);
+    for (int tb = 0; tb < tiles; ++tb) {
+        const int b0 = tb * kBT / n, b1 = (std::min(nbp, (tb + 1) * kBT) - 1) / n;
+        std::vector<int>& L = blk[static_cast<size_t>(tb)];
+        L.push_back(0);
+        for (int b = b0; b <= b1; ++b)
+            for (int i = F.h_snear_ptr[static_cast<size_t>(b)]; i < F.h_snear_ptr[static_cast<size_t>(b) + 1]; ++i)
+                if (F.h_snear_idx[static_cast<size_t>(i)] != b) L.push_back(1 + F.h_snear_idx[static_cast<size_t>(i)]);
+        std::sort(L.begin() + 1, L.end());
+        L.erase(std::unique(L.begin() + 1, L.end()), L.end());
+        nodes += no + static_cast<long long>(n) * (static_cast<long long>(L.size()) - 1);
+    }
+    for (int b = 0; b < m; ++b) {
+        mask[static_cast<size_t>(b) * (m + 1)] = 1;
+        for (int i = F.h_snear_ptr[static_cast<size_t>(b)]; i < F.h_snear_ptr[static_cast<size_t>(b) + 1]; ++i)
+            mask[static_cast<size_t>(b) * (m + 1) + 1 + F.h_snear_idx[static_cast<size_t>(i)]] = 1;
+    }
+    // groups: about (nodes of a block) / (heaviest curve), at most 8
+    const double per_block = static_cast<double>(nodes) / tiles;
+    const int G = std::max(1, std::min(8, static_cast<int>(std::lround(per_block / heaviest))));
+    std::vector<int> ptr(1, 0), idx;
+    std::vector<long long> load(static_cast<size_t>(G));
+    std::vector<std::vector<int>> grp(static_cast<size_t>(G));
+    for (int tb = 0; tb < tiles; ++tb) {
+        std::fill(load.begin(), load.end(), 0);
+        for (auto& v : grp) v.clear();
+        // the wall first (the heaviest), then bodies, each onto the lightest group
+        for (int cc : blk[static_cast<size_t>(tb)]) {
+            const size_t gi = static_cast<size_t>(std::min_element(load.begin(), load.end()) - load.begin());
+            grp[gi].push_back(cc);
+            load[gi] += cc == 0 ? no : n;
+        }
+        for (int gi = 0; gi < G; ++gi) {
+            idx.insert(idx.end(), grp[static_cast<size_t>(gi)].begin(), grp[static_cast<size_t>(gi)].end());
+            ptr.push_back(static_cast<int>(idx.size()));
+        }
+    }
+    upload_lists(c, ptr, idx);
+    c.lst_mask.ensure(mask.size());
+    STOKES_HIP(hipMemcpyAsync(c.lst_mask.p, mask.data(), mask.size(), hipMemcpyHostToDevice, c.stream));
+    STOKES_HIP(hipStreamSynchronize(c.stream));
+    c.stress_lists_version = g.version;
+    c.stress_groups = G;
+    return G;
+}
+
+// The far bodies of the shear stress go through the cluster expansions where
+// the cluster far field is allowed, available and the bodies many.
+constexpr int kStressFmmBodies = 32;
+static bool stress_by_clusters(const Context& c) {
+    return c.ff_method == 1 && c.g.m >= kStressFmmBodies && tilefmm_available(c);
+}
+
 // Density gathered, boundary values of every curve (wall first).
 static void all_boundary_values(Context& c, const double* den_dev) {
     const int m = c.g.m;
@@ -676,25 +778,26 @@ void shear_stress(Context& c, const double* den_dev, double* tau_dev) {
     hipStream_t s = c.stream;
     all_boundary_values(c, den_dev);
     const int tiles = cdiv_int(nbp, kBT);
-    std::vector<int> gb, ge;
-    const int G = plan_curve_groups(c, tiles, gb, ge);
-    c.grp_beg.ensure(gb.size());
-    c.grp_end.ensure(ge.size());
-    STOKES_HIP(hipMemcpyAsync(c.grp_beg.p, gb.data(), gb.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    STOKES_HIP(hipMemcpyAsync(c.grp_end.p, ge.data(), ge.size() * sizeof(int), hipMemcpyHostToDevice, s));
-    c.spart.ensure(static_cast<size_t>(G) * 3 * static_cast<size_t>(nbp));
+    const bool clusters = stress_by_clusters(c);
+    if (clusters) tilefmm_prepare(c);
+    const int G = clusters ? near_curve_lists(c, tiles) : all_curve_lists(c, tiles);
+    const int slots = G + (clusters ? 1 : 0);
+    c.spart.ensure(static_cast<size_t>(slots) * 3 * static_cast<size_t>(nbp));
     NodeView nv{g.x.p, g.y.p, g.dzr.p, g.dzi.p, c.bd.p, g.iar.p, g.iai.p, g.mx.p, g.my.p,
                 c.ex.p, c.ey.p, static_cast<size_t>(g.npts)};
     if (c.time_kernels) c.t_stress_eval.start(s);
-    k_bary_eval<0><<<dim3(tiles, G), kBT, 0, s>>>(c.grp_beg.p, c.grp_end.p, nv, g.ax.p,
-                                                       g.ay.p, no, n, nullptr, nullptr, nbp,
-                                                       c.spart.p);
+    k_bary_eval<0><<<dim3(tiles, G), kBT, 0, s>>>(c.lst_ptr.p, c.lst.p,
+                                                       clusters ? c.lst_mask.p : nullptr, m, nv,
+                                                       g.ax.p, g.ay.p, no, n, nullptr, nullptr,
+                                                       nbp, c.spart.p);
     STOKES_LAUNCH_CHECK();
     if (c.time_kernels) {
-        // (target point, source node) pairs: every other curve's nodes
-        c.t_stress_eval.stop(s, static_cast<double>(nbp) * static_cast<double>(g.npts - n));
+        // (target point, source node) pairs of the barycentric rule
+        c.t_stress_eval.stop(s, clusters ? static_cast<double>(c.fmm.stress_direct_pairs)
+                                         : static_cast<double>(nbp) * static_cast<double>(g.npts - n));
     }
-    k_stress_final<<<cdiv_int(nbp, kBlk), kBlk, 0, s>>>(c.spart.p, G, g.x.p, g.y.p, g.px.p,
+    if (clusters) tilefmm_stress(c, c.spart.p + static_cast<size_t>(G) * 3 * nbp);
+    k_stress_final<<<cdiv_int(nbp, kBlk), kBlk, 0, s>>>(c.spart.p, slots, g.x.p, g.y.p, g.px.p,
                                                         g.py.p, g.speed.p, g.cx.p, g.cy.p, c.ex.p,
                                                         c.ey.p, den_dev, g.dtab_n.p, no, n, m,
                                                         tau_dev);
@@ -721,18 +824,17 @@ void qoi_targets(Context& c, int ibary, const double* den_dev, int ntar, const d
     STOKES_LAUNCH_CHECK();
     if (ibary == 1) {
         const int tiles = cdiv_int(ntar, kBT);
-        std::vector<int> gb, ge;
         // one group per curve keeps the grid wide for a handful of targets
-        for (int cc = 0; cc <= m; ++cc) {
-            gb.push_back(cc);
-            ge.push_back(cc + 1);
-        }
         G = m + 1;
-        (void)tiles;
-        c.grp_beg.ensure(gb.size());
-        c.grp_end.ensure(ge.size());
-        STOKES_HIP(hipMemcpyAsync(c.grp_beg.p, gb.data(), gb.size() * sizeof(int), hipMemcpyHostToDevice, s));
-        STOKES_HIP(hipMemcpyAsync(c.grp_end.p, ge.data(), ge.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        if (ntar > 64) {
+            std::vector<int> ptr(1, 0), idx;
+            for (int tb = 0; tb < tiles; ++tb)
+                for (int cc = 0; cc <= m; ++cc) {
+                    idx.push_back(cc);
+                    ptr.push_back(static_cast<int>(idx.size()));
+                }
+            upload_lists(c, ptr, idx);
+        }
         c.spart.ensure(static_cast<size_t>(G) * 3 * static_cast<size_t>(ntar));
         NodeView nv{g.x.p, g.y.p, g.dzr.p, g.dzi.p, c.bd.p, g.iar.p, g.iai.p, g.mx.p, g.my.p,
                     c.ex.p, c.ey.p, static_cast<size_t>(g.npts)};
@@ -742,9 +844,9 @@ void qoi_targets(Context& c, int ibary, const double* den_dev, int ntar, const d
             k_qoi_wave<<<static_cast<int>((pairs + kQoiWaves - 1) / kQoiWaves), 64 * kQoiWaves, 0,
                          s>>>(nv, g.ax.p, g.ay.p, no, n, m, tx_dev, ty_dev, ntar, c.spart.p);
         } else {
-            k_bary_eval<1><<<dim3(cdiv_int(ntar, kBT), G), kBT, 0, s>>>(
-                c.grp_beg.p, c.grp_end.p, nv, g.ax.p, g.ay.p, no, n, tx_dev, ty_dev, ntar,
-                c.spart.p);
+            k_bary_eval<1><<<dim3(tiles, G), kBT, 0, s>>>(c.lst_ptr.p, c.lst.p, nullptr, m, nv,
+                                                           g.ax.p, g.ay.p, no, n, tx_dev, ty_dev,
+                                                           ntar, c.spart.p);
         }
         STOKES_LAUNCH_CHECK();
     }

@@ -50,6 +50,7 @@ constexpr int kP = 32;                 // expansion terms
 constexpr double kSep = 1.0 / 3.0;     // far: R_s + R_t <= kSep |c_s - c_t|
 constexpr int kMaxBodyPts = 512;
 constexpr int kWallSeg = 64;           // wall points per cluster
+constexpr int kStressSeries = 5;       // boundary-value series of the shear stress
 
 inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
 
@@ -141,13 +142,19 @@ __global__ __launch_bounds__(256) void k_fmm_p2m(const double* __restrict__ x,
 // F, S and H = E - G.
 typedef double v4d __attribute__((ext_vector_type(4)));
 
+//
+// MODE 0: the operator's three series per cluster (mp[c][3][k]), targets t =
+// blockIdx.x.  MODE 1: series q = blockIdx.y of NQ plain Cauchy series per
+// cluster (mp[c][NQ][k], F-type translation alone), lc[t][NQ][k].
+template <int MODE, int NQ>
 __global__ __launch_bounds__(256) void k_fmm_m2l(const double* __restrict__ cr,
                                                  const double* __restrict__ mp,
                                                  const int* __restrict__ far_ptr,
                                                  const int* __restrict__ far_idx,
                                                  const double* __restrict__ bin,
                                                  double* __restrict__ lc) {
-    __shared__ double RED[4][3][kP][2];
+    constexpr int NOUT = MODE == 0 ? 3 : 1;
+    __shared__ double RED[4][NOUT][kP][2];
     const int t = blockIdx.x, w = threadIdx.x / 64, l = threadIdx.x % 64;
     const int j = l % 16, h = l / 16;
     double A[2][8];
@@ -183,11 +190,11 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(const double* __restrict__ cr,
         // u^h and u^4 for this lane's rows n = 4 st + h
         const cplx u2 = cmul(u, u), u4 = cmul(u2, u2);
         const cplx uh = h == 0 ? cmk(1.0, 0.0) : h == 1 ? u : h == 2 ? u2 : cmul(u2, u);
-        const double* M = mp + static_cast<size_t>(s >= 0 ? s : 0) * 3 * kP * 2;
+        const double* M = mp + static_cast<size_t>(s >= 0 ? s : 0) * NQ * kP * 2;
         // E first, then F (whose product also enters H), then S: one Y live
 #pragma unroll
-        for (int qi = 0; qi < 3; ++qi) {
-            const int q = qi == 0 ? 2 : qi == 1 ? 0 : 1;
+        for (int qi = 0; qi < NOUT; ++qi) {
+            const int q = MODE == 1 ? static_cast<int>(blockIdx.y) : qi == 0 ? 2 : qi == 1 ? 0 : 1;
             double wr[8], wi[8];
             cplx up = uh;
 #pragma unroll
@@ -217,7 +224,9 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(const double* __restrict__ cr,
                 for (int r = 0; r < 4; ++r) {
                     const int e = 4 * kt + r;
                     const cplx fy = cmul(f[e], cmk(yr[kt][r], yi[kt][r]));
-                    if (q == 2) {
+                    if (MODE == 1) {
+                        LF[e] = cadd(LF[e], fy);
+                    } else if (q == 2) {
                         LH[e] = cadd(LH[e], fy);
                     } else if (q == 0) {
                         LF[e] = cadd(LF[e], fy);
@@ -235,10 +244,12 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(const double* __restrict__ cr,
         for (int o = 1; o < 16; o <<= 1) {
             LF[e].r += __shfl_xor(LF[e].r, o);
             LF[e].i += __shfl_xor(LF[e].i, o);
-            LS[e].r += __shfl_xor(LS[e].r, o);
-            LS[e].i += __shfl_xor(LS[e].i, o);
-            LH[e].r += __shfl_xor(LH[e].r, o);
-            LH[e].i += __shfl_xor(LH[e].i, o);
+            if constexpr (MODE == 0) {
+                LS[e].r += __shfl_xor(LS[e].r, o);
+                LS[e].i += __shfl_xor(LS[e].i, o);
+                LH[e].r += __shfl_xor(LH[e].r, o);
+                LH[e].i += __shfl_xor(LH[e].i, o);
+            }
         }
     if (j == 0) {
 #pragma unroll
@@ -246,15 +257,17 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(const double* __restrict__ cr,
             const int k = 16 * (e / 4) + h + 4 * (e % 4);
             RED[w][0][k][0] = LF[e].r;
             RED[w][0][k][1] = LF[e].i;
-            RED[w][1][k][0] = LS[e].r;
-            RED[w][1][k][1] = LS[e].i;
-            RED[w][2][k][0] = LH[e].r;
-            RED[w][2][k][1] = LH[e].i;
+            if constexpr (MODE == 0) {
+                RED[w][NOUT - 2][k][0] = LS[e].r;
+                RED[w][NOUT - 2][k][1] = LS[e].i;
+                RED[w][NOUT - 1][k][0] = LH[e].r;
+                RED[w][NOUT - 1][k][1] = LH[e].i;
+            }
         }
     }
     __syncthreads();
     const int tid = threadIdx.x;
-    if (tid < 3 * kP) {
+    if (tid < NOUT * kP) {
         const int q = tid / kP, k = tid % kP;
         double sr = 0.0, si = 0.0;
 #pragma unroll
@@ -262,10 +275,104 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(const double* __restrict__ cr,
             sr += RED[ww][q][k][0];
             si += RED[ww][q][k][1];
         }
-        const size_t o = (static_cast<size_t>(t) * 3 + q) * kP + k;
+        const int qo = MODE == 1 ? static_cast<int>(blockIdx.y) : q;
+        const size_t o = (static_cast<size_t>(t) * NQ + qo) * kP + k;
         lc[2 * o] = sr;
         lc[2 * o + 1] = si;
     }
+}
+
+// Multipoles of the five boundary-value series of the shear stress at every
+// body: grid (m, 5), block 256 as k_fmm_p2m (wave w: orders 8w .. 8w + 7).
+// Body b is cluster nwall + b; the series f of point p has the coefficient
+// a = b_f(p) dz(p) / (2 pi i), so that sum_p a / (z - w_p) is the exterior
+// Cauchy integral of b_f by the trapezoid rule -- the value the
+// barycentric exterior rule converges to away from the curve.  bd holds
+// b_f as bd[(2f + {0, 1}) N + p]; mp[b][f][k].
+__global__ __launch_bounds__(256) void k_fmm_p2m_bd(const double* __restrict__ x,
+                                                    const double* __restrict__ y,
+                                                    const double* __restrict__ dzr,
+                                                    const double* __restrict__ dzi,
+                                                    const double* __restrict__ bd, size_t N,
+                                                    const double* __restrict__ cr,
+                                                    const int* __restrict__ cl, int nwall,
+                                                    double* __restrict__ mp) {
+    constexpr int kPer = kP / 4;
+    const int b = blockIdx.x, f = blockIdx.y, c = nwall + b;
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    const double cx = cr[3 * c], cy = cr[3 * c + 1], iR = 1.0 / cr[3 * c + 2];
+    const int off = cl[2 * c], len = cl[2 * c + 1];
+    cplx acc[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) acc[i] = cmk(0.0, 0.0);
+    for (int j = l; j < len; j += 64) {
+        const int p = off + j;
+        const cplx z = cmk((x[p] - cx) * iR, (y[p] - cy) * iR);
+        const cplx bdz = cmul(cmk(bd[(2 * f) * N + p], bd[(2 * f + 1) * N + p]),
+                              cmk(dzr[p], dzi[p]));
+        const cplx a = cmk(bdz.i / kTwoPi, -bdz.r / kTwoPi);  // bdz / (2 pi i)
+        cplx zk = cpow_int(z, kPer * w);
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            cfma(acc[i], a, zk);
+            zk = cmul(zk, z);
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kPer; ++i)
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            acc[i].r += __shfl_xor(acc[i].r, o);
+            acc[i].i += __shfl_xor(acc[i].i, o);
+        }
+#pragma unroll
+    for (int i = 0; i < kPer; ++i)
+        if (l == i) {
+            const size_t o = (static_cast<size_t>(b) * kStressSeries + f) * kP + kPer * w + i;
+            mp[2 * o] = acc[i].r;
+            mp[2 * o + 1] = acc[i].i;
+        }
+}
+
+// Strain rate at every body point from the local expansions of its far
+// bodies' boundary-value series (one thread per body point q, blocks of
+// 256): with C_f the local series of field f about the body's centre, du_f =
+// C_f' and dd_f = C_f'', combined as the exterior rule's nder=2 output
+// (stokesSolver.f:5297-5376) into E11, E12, E22 at out[{0,1,2} nbp + q].
+__global__ __launch_bounds__(kBlk) void k_fmm_l2p_strain(const double* __restrict__ x,
+                                                         const double* __restrict__ y,
+                                                         const double* __restrict__ cr,
+                                                         const double* __restrict__ lc, int no,
+                                                         int n, int nwall, int nbp,
+                                                         double* __restrict__ out) {
+    const int q = blockIdx.x * kBlk + threadIdx.x;
+    if (q >= nbp) return;
+    const int b = q / n, c = nwall + b, p = no + q;
+    const double xt = x[p], yt = y[p];
+    const double iR = 1.0 / cr[3 * c + 2];
+    const cplx z = cmk((xt - cr[3 * c]) * iR, (yt - cr[3 * c + 1]) * iR);
+    cplx du[kStressSeries], dd[kStressSeries];
+#pragma unroll
+    for (int f = 0; f < kStressSeries; ++f) {
+        const double* L = lc + (static_cast<size_t>(b) * kStressSeries + f) * kP * 2;
+        // Horner: value, first derivative, half the second derivative
+        cplx pv = cmk(L[2 * (kP - 1)], L[2 * (kP - 1) + 1]), d1 = cmk(0.0, 0.0), d2 = cmk(0.0, 0.0);
+#pragma unroll 4
+        for (int k = kP - 2; k >= 0; --k) {
+            d2 = cadd(cmul(d2, z), d1);
+            d1 = cadd(cmul(d1, z), pv);
+            pv = cadd(cmul(pv, z), cmk(L[2 * k], L[2 * k + 1]));
+        }
+        du[f] = cscale(d1, iR);
+        dd[f] = cscale(d2, 2.0 * iR * iR);
+    }
+    const double u1x = dd[0].r - du[1].r - xt * dd[1].r - yt * dd[2].r + du[3].r;
+    const double u1y = -dd[0].i + xt * dd[1].i - du[2].r + yt * dd[2].i - du[3].i;
+    const double u2x = -dd[0].i + du[1].i + xt * dd[1].i + yt * dd[2].i + du[4].r;
+    const double u2y = -dd[0].r + xt * dd[1].r + du[2].i + yt * dd[2].r - du[4].i;
+    out[q] = u1x;
+    out[static_cast<size_t>(nbp) + q] = 0.5 * (u1y + u2x);
+    out[2 * static_cast<size_t>(nbp) + q] = u2y;
 }
 
 // Direct sums: one block per work item (cluster c, tile j of c's source
@@ -468,6 +575,28 @@ void tilefmm_prepare(Context& c) {
         tile_ptr[static_cast<size_t>(t) + 1] = static_cast<int>(tiles.size() / 2);
         direct += static_cast<long long>(T.len) * nsrc;
     }
+    // shear stress: far bodies of each body by the same rule (expansions),
+    // near bodies (itself included) for the barycentric rule
+    std::vector<int> sfar_ptr(static_cast<size_t>(m) + 1, 0), sfar_idx;
+    F.h_snear_ptr.assign(static_cast<size_t>(m) + 1, 0);
+    F.h_snear_idx.clear();
+    long long sdirect = 0;
+    for (int b = 0; b < m; ++b) {
+        const Cluster& T = C[static_cast<size_t>(nwall + b)];
+        for (int e = 0; e < m; ++e) {
+            const Cluster& S = C[static_cast<size_t>(nwall + e)];
+            const double dx = S.cx - T.cx, dy = S.cy - T.cy;
+            if (e != b && S.r + T.r <= kSep * std::sqrt(dx * dx + dy * dy))
+                sfar_idx.push_back(e);
+            else
+                F.h_snear_idx.push_back(e);
+        }
+        sfar_ptr[static_cast<size_t>(b) + 1] = static_cast<int>(sfar_idx.size());
+        F.h_snear_ptr[static_cast<size_t>(b) + 1] = static_cast<int>(F.h_snear_idx.size());
+        const long long nnear = F.h_snear_ptr[static_cast<size_t>(b) + 1] - F.h_snear_ptr[static_cast<size_t>(b)];
+        sdirect += static_cast<long long>(n) * (no + static_cast<long long>(n) * (nnear - 1));
+    }
+    F.stress_direct_pairs = sdirect;
     hipStream_t s = c.stream;
     auto up = [&](auto& buf, const auto& vec) {
         buf.ensure(std::max<size_t>(vec.size(), 1));
@@ -481,6 +610,8 @@ void tilefmm_prepare(Context& c) {
     up(F.far_idx, far_idx);
     up(F.tile_ptr, tile_ptr);
     up(F.tiles, tiles);
+    up(F.sfar_ptr, sfar_ptr);
+    up(F.sfar_idx, sfar_idx);
     STOKES_HIP(hipStreamSynchronize(s));
     if (!F.bin.p) {
         // binom(n + k, k) for n, k < 32 (exact in long double: < 2^63), in
@@ -502,6 +633,8 @@ void tilefmm_prepare(Context& c) {
     }
     F.mp.ensure(static_cast<size_t>(nc) * 3 * kP * 2);
     F.lc.ensure(static_cast<size_t>(nc) * 3 * kP * 2);
+    F.smp.ensure(std::max(1, m) * static_cast<size_t>(kStressSeries) * kP * 2);
+    F.slc.ensure(std::max(1, m) * static_cast<size_t>(kStressSeries) * kP * 2);
     F.nc = nc;
     F.nwall = nwall;
     // one work item per (target cluster, source tile): bodies, then the wall
@@ -566,7 +699,7 @@ void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit) {
     k_fmm_p2m<<<nc, 256, 0, s>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p, F.cr.p, F.cl.p,
                                  F.mp.p);
     STOKES_LAUNCH_CHECK();
-    k_fmm_m2l<<<nc, 256, 0, s>>>(F.cr.p, F.mp.p, F.far_ptr.p, F.far_idx.p, F.bin.p, F.lc.p);
+    k_fmm_m2l<0, 3><<<nc, 256, 0, s>>>(F.cr.p, F.mp.p, F.far_ptr.p, F.far_idx.p, F.bin.p, F.lc.p);
     STOKES_LAUNCH_CHECK();
     if (c.time_kernels) {
         c.t_fmm_expand.stop(s, static_cast<double>(F.far_pairs));
@@ -584,6 +717,29 @@ void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit) {
     if (c.time_kernels) c.t_fmm_direct.stop(s, static_cast<double>(F.direct_pairs));
     *wall_nsplit = 1;
     *body_nsplit = 1;
+}
+
+// Strain rate at every body point from the far bodies of the shear stress
+// (the bodies that are far by the clusters' rule), from the boundary values
+// in c.bd: out[{0,1,2} nbp + q] = E11, E12, E22.
+void tilefmm_stress(Context& c, double* out) {
+    Geometry& g = c.g;
+    TileFmm& F = c.fmm;
+    tilefmm_prepare(c);
+    hipStream_t s = c.stream;
+    const int m = g.m, n = g.n, nbp = n * m;
+    if (m == 0) return;
+    k_fmm_p2m_bd<<<dim3(m, kStressSeries), 256, 0, s>>>(g.x.p, g.y.p, g.dzr.p, g.dzi.p, c.bd.p,
+                                                       static_cast<size_t>(g.npts), F.cr.p, F.cl.p,
+                                                       F.nwall, F.smp.p);
+    STOKES_LAUNCH_CHECK();
+    k_fmm_m2l<1, kStressSeries><<<dim3(m, kStressSeries), 256, 0, s>>>(
+        F.cr.p + 3 * static_cast<size_t>(F.nwall), F.smp.p, F.sfar_ptr.p, F.sfar_idx.p, F.bin.p,
+        F.slc.p);
+    STOKES_LAUNCH_CHECK();
+    k_fmm_l2p_strain<<<cdiv_int(nbp, kBlk), kBlk, 0, s>>>(g.x.p, g.y.p, F.cr.p, F.slc.p, g.no, n,
+                                                         F.nwall, nbp, out);
+    STOKES_LAUNCH_CHECK();
 }
 
 }  // namespace stokes

@@ -222,3 +222,31 @@ def test_preconditioned_extrapolation_saves_iterations():
             iters[warm] = per
     print(f"100-body steps, GMRES iterations per step: zeros {iters[0]}, extrapolated {iters[2]}")
     assert sum(iters[2][1:]) < 0.95 * sum(iters[0][1:])
+
+
+@pytest.mark.parametrize("name", ["bodies100_n256", "bodies200_n512", "bodies60_n128_tight",
+                                  "bodies400_n256_eroded50"])
+def test_stress_far_bodies_by_expansions(name):
+    """The shear stress with the far bodies' boundary-value integrals through
+    the cluster expansions (the barycentric rule kept for the wall and the
+    near bodies) against the barycentric rule over every curve, on one
+    density: away from a curve the barycentric rule equals the trapezoid
+    Cauchy integral the expansions sum, so the two agree to rounding -- of
+    sums whose terms (the large rigid-motion densities of small bodies) are
+    10^5 times the stress, the scale at which the oracle's FP64 stress and
+    the library's already differ (4e-11 at 100 x 256).  Bar: the stress
+    parity bar, 1e-10 of max |tau|.  Covers one body per target block (256
+    points), half a body (512) and two bodies per block with the per-body
+    mask (128)."""
+    th, n, m, no, xx, yy = _state(name)
+    with solver():
+        x, _, _ = solve(n, m, no, xx, yy)
+    tau_c = stokes.computeshearstress(n, m, no, xx, yy, x)
+    stokes.set_farfield_method(stokes.FARFIELD_DIRECT)
+    try:
+        tau_d = stokes.computeshearstress(n, m, no, xx, yy, x)
+    finally:
+        stokes.set_farfield_method(stokes.FARFIELD_AUTO)
+    e = relerr(tau_c, tau_d)
+    print(f"{name}: shear stress, far bodies by expansions vs barycentric everywhere {e:.2e}")
+    assert e <= 1e-10
