@@ -1039,6 +1039,8 @@ constexpr int kFmmIndex = kFFDirectShapes;  // the body-cluster far field
 // 2^(1/16), about 4% -- and, for FP64, whether the body-cluster far field may
 // be chosen: an erosion run loses a body every few steps, and every exact
 // size would otherwise time every shape again.
+constexpr int kTuneReach = 3;
+
 static int tune_key(const Context& c, bool fp32) {
     const bool fmm = !fp32 && c.ff_fmm && tilefmm_available(c);
     const int bucket = static_cast<int>(std::floor(16.0 * std::log2(static_cast<double>(c.g.npts))));
@@ -1056,6 +1058,20 @@ static int tune_farfield(Context& c, bool fp32) {
     std::map<int, int>& choice = fp32 ? c.ff32_choice : c.ff_choice;
     auto it = choice.find(key);
     if (it != choice.end()) return it->second;
+    // a size class within kTuneReach classes (about 13%) of a tuned one
+    // keeps that one's choice: the ranking of the shapes does not turn over
+    // at that distance, and timing every shape again costs a third of a
+    // second at 10^5 points
+    std::map<int, float>& tuned = fp32 ? c.ff32_tuned_ms : c.ff_tuned_ms;
+    for (int d = 1; d <= kTuneReach; ++d)
+        for (int sgn : {-1, 1}) {
+            auto nb = choice.find(key + 2 * sgn * d);
+            if (nb != choice.end()) {
+                choice[key] = nb->second;
+                tuned[key] = tuned[key + 2 * sgn * d];
+                return nb->second;
+            }
+        }
     const bool fmm = key & 1;
     if (fmm) tilefmm_prepare(c);
     int best = 0;
