@@ -216,6 +216,10 @@ struct TileFmm {
     DevBuf<double> smp, slc;            // [m][5][32] complex
     std::vector<int> h_snear_ptr, h_snear_idx;
     long long stress_direct_pairs = 0;  // (target point, source node) pairs of the near curves
+    // the expansions run on a second stream (default priority) beside the
+    // direct sums: fork and join events on the context's stream
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 // Per-solve GMRES state kept on the device.
@@ -288,9 +292,9 @@ struct Context {
     DevBuf<double> stk_log;           // log |z_i - c_b|^2 [npts][m]
     long long stk_log_version = -1;   // geometry version stk_log was built for
     int ff_splits = 0, ff_tpt = 0;
-    std::map<int, int> ff_choice;      // size class -> far-field shape index
-    std::map<int, float> ff_tuned_ms;  // size class -> tuned launch time
-    std::map<int, int> ff32_choice;    // the same for the FP32 far field
+    std::map<long long, int> ff_choice;      // size class -> far-field shape index
+    std::map<long long, float> ff_tuned_ms;  // size class -> tuned launch time
+    std::map<long long, int> ff32_choice;    // the same for the FP32 far field
     int ff_force = -1;                 // a fixed far-field shape index (tests), -1 tunes
     // the body-cluster far field may be chosen: ff_method is the caller's
     // setting (1 allowed, 0 direct sums only), ff_fmm the current call's
@@ -298,7 +302,7 @@ struct Context {
     int ff_method = 1;
     int ff_fmm = 1;
     TileFmm fmm;
-    std::map<int, float> ff32_tuned_ms;
+    std::map<long long, float> ff32_tuned_ms;
     // FP32 far field: per 256-point tile the FP64 centre, per source the FP32
     // offset from it and the FP32 weighted normal; built once per geometry
     DevBuf<double> ff32_ctr;
@@ -354,9 +358,9 @@ struct Context {
     StepperBufs sp;
     KernelTimer t_farfield, t_near, t_stress_eval, t_gs, t_precond, t_farfield32;
     // cluster far field: the whole application (work: the canonical
-    // Npts (Npts - 1) pairs it stands for), its direct sums and local
-    // expansions at every target (k_fmm_eval; work: pairs), and the
-    // multipole and local expansions (k_fmm_p2m + k_fmm_m2l; work: cluster
+    // Npts (Npts - 1) pairs it stands for), its direct sums (k_fmm_direct;
+    // work: pairs), and the multipole and local expansions (k_fmm_p2m +
+    // k_fmm_m2l on the side stream, beside the direct sums; work: cluster
     // translations)
     KernelTimer t_fmm, t_fmm_direct, t_fmm_expand;
     int status = 0;

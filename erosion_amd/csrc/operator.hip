@@ -1035,16 +1035,19 @@ void boundary_values(Context& c, const int* curves_dev, int ncurves, int wall_fi
 constexpr int kFFDirectShapes = static_cast<int>(sizeof(kFFShapes) / sizeof(kFFShapes[0]));
 constexpr int kFmmIndex = kFFDirectShapes;  // the body-cluster far field
 
-// Tuning results are kept per size class -- boundary points in buckets of
-// 2^(1/16), about 4% -- and, for FP64, whether the body-cluster far field may
-// be chosen: an erosion run loses a body every few steps, and every exact
-// size would otherwise time every shape again.
+// Tuning results are kept per size class -- points per body, wall points,
+// boundary points in buckets of 2^(1/16) (about 4%) -- and, for FP64,
+// whether the cluster far field may be chosen: an erosion run loses a body
+// every few steps, and every exact size would otherwise time every shape
+// again.  Key: ((n 2^20 + no) 2^12 + bucket) 2 + cluster flag, so that the
+// neighbouring size classes of one body and wall size are key +- 2 d.
 constexpr int kTuneReach = 3;
 
-static int tune_key(const Context& c, bool fp32) {
+static long long tune_key(const Context& c, bool fp32) {
     const bool fmm = !fp32 && c.ff_fmm && tilefmm_available(c);
-    const int bucket = static_cast<int>(std::floor(16.0 * std::log2(static_cast<double>(c.g.npts))));
-    return 2 * bucket + (fmm ? 1 : 0);
+    const long long bucket = static_cast<long long>(std::floor(16.0 * std::log2(static_cast<double>(c.g.npts))));
+    const long long geo = (static_cast<long long>(c.g.n) << 20) | static_cast<long long>(c.g.no);
+    return (((geo << 12) | bucket) << 1) | (fmm ? 1 : 0);
 }
 
 static int tune_farfield(Context& c, bool fp32) {
@@ -1054,15 +1057,15 @@ static int tune_farfield(Context& c, bool fp32) {
         return c.ff_force;
     }
     const int np = c.g.npts;
-    const int key = tune_key(c, fp32);
-    std::map<int, int>& choice = fp32 ? c.ff32_choice : c.ff_choice;
+    const long long key = tune_key(c, fp32);
+    std::map<long long, int>& choice = fp32 ? c.ff32_choice : c.ff_choice;
     auto it = choice.find(key);
     if (it != choice.end()) return it->second;
     // a size class within kTuneReach classes (about 13%) of a tuned one
     // keeps that one's choice: the ranking of the shapes does not turn over
     // at that distance, and timing every shape again costs a third of a
     // second at 10^5 points
-    std::map<int, float>& tuned = fp32 ? c.ff32_tuned_ms : c.ff_tuned_ms;
+    std::map<long long, float>& tuned = fp32 ? c.ff32_tuned_ms : c.ff_tuned_ms;
     for (int d = 1; d <= kTuneReach; ++d)
         for (int sgn : {-1, 1}) {
             auto nb = choice.find(key + 2 * sgn * d);
@@ -1175,7 +1178,7 @@ int farfield_shape_count() { return kFFDirectShapes + 1; }
 
 int farfield_choice(const Context& c, bool fp32) {
     if (c.ff_force >= 0) return c.ff_force;
-    const std::map<int, int>& choice = fp32 ? c.ff32_choice : c.ff_choice;
+    const std::map<long long, int>& choice = fp32 ? c.ff32_choice : c.ff_choice;
     auto it = choice.find(tune_key(c, fp32));
     return it == choice.end() ? -1 : it->second;
 }

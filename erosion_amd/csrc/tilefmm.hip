@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void k_fmm_p2m(const double* __restrict__ x,
 }
 
 // Local expansions of every target cluster from its far clusters, on the
-// matrix cores.  For a batch of 16 far sources j and one series q, the
+// matrix cores.  For a batch of 16 far sources j and one series, the
 // translation is a real matrix product followed by a complex scaling:
 //   Y[k][j] = sum_n B[k][n] W[n][j],   B[k][n] = binom(n + k, k),
 //   W[n][j] = (-R_j/d_j)^n M_j[n]      (real and imaginary parts apart),
@@ -135,37 +135,36 @@ __global__ __launch_bounds__(256) void k_fmm_p2m(const double* __restrict__ x,
 // so that v_mfma_f64_16x16x4f64 forms Y (k in two tiles of 16, n in eight
 // steps of 4) and the vector ALU the powers and the scaling.  Lane l of a
 // wave holds source j = l % 16 of the batch: W rows n = 4 s + l / 16 (step s)
-// and Y rows k = 16 kt + l / 16 + 4 r (register r).  Grid nc, block 256 =
-// 4 waves taking batches of 16 sources in turn; the waves' sums meet in LDS.
-// bin holds B in the A-operand layout: bin[(kt * 8 + s) * 64 + l] =
-// binom(4 s + l / 16 + 16 kt + l % 16, 16 kt + l % 16).  lc[t][q][k], q < 3:
-// F, S and H = E - G.
+// and Y rows k = 16 kt + l / 16 + 4 r (register r).  Grid (targets, NQ):
+// block (t, q) translates series q of mp[c][NQ][k] into lc[t][NQO][k]; 4
+// waves take batches of 16 sources in turn and meet in LDS.  With G (the
+// operator's series F, S, E), block q = 0 also forms the series
+// -G = -sum_s conj(c_t - c_s) F_s into lc[t][NQ][k] (NQO = NQ + 1).  bin
+// holds B in the A-operand layout: bin[(kt * 8 + s) * 64 + l] =
+// binom(4 s + l / 16 + 16 kt + l % 16, 16 kt + l % 16).
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-//
-// MODE 0: the operator's three series per cluster (mp[c][3][k]), targets t =
-// blockIdx.x.  MODE 1: series q = blockIdx.y of NQ plain Cauchy series per
-// cluster (mp[c][NQ][k], F-type translation alone), lc[t][NQ][k].
-template <int MODE, int NQ>
+template <int NQ, bool G>
 __global__ __launch_bounds__(256) void k_fmm_m2l(const double* __restrict__ cr,
                                                  const double* __restrict__ mp,
                                                  const int* __restrict__ far_ptr,
                                                  const int* __restrict__ far_idx,
                                                  const double* __restrict__ bin,
                                                  double* __restrict__ lc) {
-    constexpr int NOUT = MODE == 0 ? 3 : 1;
-    __shared__ double RED[4][NOUT][kP][2];
-    const int t = blockIdx.x, w = threadIdx.x / 64, l = threadIdx.x % 64;
+    constexpr int NQO = NQ + (G ? 1 : 0);
+    __shared__ double RED[4][2][kP][2];
+    const int t = blockIdx.x, q = blockIdx.y, w = threadIdx.x / 64, l = threadIdx.x % 64;
     const int j = l % 16, h = l / 16;
+    const bool with_g = G && q == 0;
     double A[2][8];
 #pragma unroll
     for (int kt = 0; kt < 2; ++kt)
 #pragma unroll
         for (int st = 0; st < 8; ++st) A[kt][st] = bin[(kt * 8 + st) * 64 + l];
     const double ctx = cr[3 * t], cty = cr[3 * t + 1], Rt = cr[3 * t + 2];
-    cplx LF[8], LS[8], LH[8];
+    cplx LQ[8], LG[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) LF[e] = LS[e] = LH[e] = cmk(0.0, 0.0);
+    for (int e = 0; e < 8; ++e) LQ[e] = LG[e] = cmk(0.0, 0.0);
     const int f0 = far_ptr[t], nf = far_ptr[t + 1] - f0;
     for (int b0 = 16 * w; b0 < nf; b0 += 64) {  // uniform over the wave
         const int i = b0 + j;
@@ -178,63 +177,45 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(const double* __restrict__ cr,
             v = cscale(id, Rt);              // R_t / d
             cmd = cmk(-d.r, d.i);            // conj(c_t - c_s)
         }
-        // f(k) for this lane's rows k = 16 kt + h + 4 r
-        const cplx v2 = cmul(v, v), v4 = cmul(v2, v2), v8 = cmul(v4, v4), v16 = cmul(v8, v8);
-        const cplx vh = h == 0 ? cmk(1.0, 0.0) : h == 1 ? v : h == 2 ? v2 : cmul(v2, v);
-        cplx f[8];
-        f[0] = cmul(cmk(-id.r, -id.i), vh);
-#pragma unroll
-        for (int r = 1; r < 4; ++r) f[r] = cmul(f[r - 1], v4);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) f[4 + r] = cmul(f[r], v16);
-        // u^h and u^4 for this lane's rows n = 4 st + h
+        // W rows n = 4 st + h: (-R_s/d)^n M[n]
         const cplx u2 = cmul(u, u), u4 = cmul(u2, u2);
-        const cplx uh = h == 0 ? cmk(1.0, 0.0) : h == 1 ? u : h == 2 ? u2 : cmul(u2, u);
-        const double* M = mp + static_cast<size_t>(s >= 0 ? s : 0) * NQ * kP * 2;
-        // E first, then F (whose product also enters H), then S: one Y live
+        cplx up = h == 0 ? cmk(1.0, 0.0) : h == 1 ? u : h == 2 ? u2 : cmul(u2, u);
+        const double* M = mp + (static_cast<size_t>(s >= 0 ? s : 0) * NQ + q) * kP * 2;
+        double wr[8], wi[8];
 #pragma unroll
-        for (int qi = 0; qi < NOUT; ++qi) {
-            const int q = MODE == 1 ? static_cast<int>(blockIdx.y) : qi == 0 ? 2 : qi == 1 ? 0 : 1;
-            double wr[8], wi[8];
-            cplx up = uh;
+        for (int st = 0; st < 8; ++st) {
+            const int nn = 4 * st + h;
+            const cplx mm = s >= 0 ? cmk(M[2 * nn], M[2 * nn + 1]) : cmk(0.0, 0.0);
+            const cplx wv = cmul(up, mm);
+            wr[st] = wv.r;
+            wi[st] = wv.i;
+            up = cmul(up, u4);
+        }
+        v4d yr[2], yi[2];
+#pragma unroll
+        for (int kt = 0; kt < 2; ++kt) {
+            yr[kt] = v4d{0.0, 0.0, 0.0, 0.0};
+            yi[kt] = v4d{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
             for (int st = 0; st < 8; ++st) {
-                const int nn = 4 * st + h;
-                const cplx mm = s >= 0 ? cmk(M[2 * (q * kP + nn)], M[2 * (q * kP + nn) + 1])
-                                       : cmk(0.0, 0.0);
-                const cplx wv = cmul(up, mm);
-                wr[st] = wv.r;
-                wi[st] = wv.i;
-                up = cmul(up, u4);
+                yr[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[kt][st], wr[st], yr[kt], 0, 0, 0);
+                yi[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[kt][st], wi[st], yi[kt], 0, 0, 0);
             }
-            v4d yr[2], yi[2];
+        }
+        // f(k) for this lane's rows k = 16 kt + h + 4 r
+        const cplx v2 = cmul(v, v), v4 = cmul(v2, v2), v8 = cmul(v4, v4), v16 = cmul(v8, v8);
+        cplx f = cmul(cmk(-id.r, -id.i), h == 0 ? cmk(1.0, 0.0) : h == 1 ? v : h == 2 ? v2 : cmul(v2, v));
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
 #pragma unroll
             for (int kt = 0; kt < 2; ++kt) {
-                yr[kt] = v4d{0.0, 0.0, 0.0, 0.0};
-                yi[kt] = v4d{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-                for (int st = 0; st < 8; ++st) {
-                    yr[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[kt][st], wr[st], yr[kt], 0, 0, 0);
-                    yi[kt] = __builtin_amdgcn_mfma_f64_16x16x4f64(A[kt][st], wi[st], yi[kt], 0, 0, 0);
-                }
+                const int e = 4 * kt + r;
+                const cplx fk = kt == 0 ? f : cmul(f, v16);
+                const cplx fy = cmul(fk, cmk(yr[kt][r], yi[kt][r]));
+                LQ[e] = cadd(LQ[e], fy);
+                if (with_g) LG[e] = csub(LG[e], cmul(cmd, fy));
             }
-#pragma unroll
-            for (int kt = 0; kt < 2; ++kt)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int e = 4 * kt + r;
-                    const cplx fy = cmul(f[e], cmk(yr[kt][r], yi[kt][r]));
-                    if (MODE == 1) {
-                        LF[e] = cadd(LF[e], fy);
-                    } else if (q == 2) {
-                        LH[e] = cadd(LH[e], fy);
-                    } else if (q == 0) {
-                        LF[e] = cadd(LF[e], fy);
-                        LH[e] = csub(LH[e], cmul(cmd, fy));
-                    } else {
-                        LS[e] = cadd(LS[e], fy);
-                    }
-                }
+            f = cmul(f, v4);
         }
     }
     // sum over the 16 sources of a batch (lanes with the same l / 16)
@@ -242,41 +223,34 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(const double* __restrict__ cr,
     for (int e = 0; e < 8; ++e)
 #pragma unroll
         for (int o = 1; o < 16; o <<= 1) {
-            LF[e].r += __shfl_xor(LF[e].r, o);
-            LF[e].i += __shfl_xor(LF[e].i, o);
-            if constexpr (MODE == 0) {
-                LS[e].r += __shfl_xor(LS[e].r, o);
-                LS[e].i += __shfl_xor(LS[e].i, o);
-                LH[e].r += __shfl_xor(LH[e].r, o);
-                LH[e].i += __shfl_xor(LH[e].i, o);
+            LQ[e].r += __shfl_xor(LQ[e].r, o);
+            LQ[e].i += __shfl_xor(LQ[e].i, o);
+            if (with_g) {
+                LG[e].r += __shfl_xor(LG[e].r, o);
+                LG[e].i += __shfl_xor(LG[e].i, o);
             }
         }
     if (j == 0) {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int k = 16 * (e / 4) + h + 4 * (e % 4);
-            RED[w][0][k][0] = LF[e].r;
-            RED[w][0][k][1] = LF[e].i;
-            if constexpr (MODE == 0) {
-                RED[w][NOUT - 2][k][0] = LS[e].r;
-                RED[w][NOUT - 2][k][1] = LS[e].i;
-                RED[w][NOUT - 1][k][0] = LH[e].r;
-                RED[w][NOUT - 1][k][1] = LH[e].i;
-            }
+            RED[w][0][k][0] = LQ[e].r;
+            RED[w][0][k][1] = LQ[e].i;
+            RED[w][1][k][0] = LG[e].r;
+            RED[w][1][k][1] = LG[e].i;
         }
     }
     __syncthreads();
     const int tid = threadIdx.x;
-    if (tid < NOUT * kP) {
-        const int q = tid / kP, k = tid % kP;
+    if (tid < (with_g ? 2 : 1) * kP) {
+        const int o2 = tid / kP, k = tid % kP;
         double sr = 0.0, si = 0.0;
 #pragma unroll
         for (int ww = 0; ww < 4; ++ww) {
-            sr += RED[ww][q][k][0];
-            si += RED[ww][q][k][1];
+            sr += RED[ww][o2][k][0];
+            si += RED[ww][o2][k][1];
         }
-        const int qo = MODE == 1 ? static_cast<int>(blockIdx.y) : q;
-        const size_t o = (static_cast<size_t>(t) * NQ + qo) * kP + k;
+        const size_t o = (static_cast<size_t>(t) * NQO + (o2 == 0 ? q : NQ)) * kP + k;
         lc[2 * o] = sr;
         lc[2 * o + 1] = si;
     }
@@ -438,7 +412,7 @@ __global__ __launch_bounds__(128) void k_fmm_direct(
 // sums (one thread per point, blocks of 256): wall point p belongs to cluster
 // p / kWallSeg, body point p >= no to cluster nwall + (p - no) / n.  Sums
 // part_u/part_v[j][p] over the cluster's tiles j and writes the total to
-// part_u/part_v[0][p].  lc[c][q][k], q < 3: F, S and H = E - G.
+// part_u/part_v[0][p].  lc[c][q][k], q < 4: F, S, E and -G (H = E - G).
 __global__ __launch_bounds__(kBlk) void k_fmm_l2p(const double* __restrict__ x,
                                                   const double* __restrict__ y,
                                                   const double* __restrict__ cr,
@@ -452,18 +426,22 @@ __global__ __launch_bounds__(kBlk) void k_fmm_l2p(const double* __restrict__ x,
     const double dx = x[p] - cr[3 * c], dy = y[p] - cr[3 * c + 1];
     const double iR = 1.0 / cr[3 * c + 2];
     const cplx z = cmk(dx * iR, dy * iR);
-    const double* L = lc + static_cast<size_t>(c) * 3 * kP * 2;
+    const double* L = lc + static_cast<size_t>(c) * 4 * kP * 2;
+    auto hk = [&](int k) {  // H_k = E_k + (-G)_k
+        return cmk(L[2 * (2 * kP + k)] + L[2 * (3 * kP + k)],
+                   L[2 * (2 * kP + k) + 1] + L[2 * (3 * kP + k) + 1]);
+    };
     // Horner: F and its derivative, S, the derivative of H
     cplx fv = cmk(L[2 * (kP - 1)], L[2 * (kP - 1) + 1]), fd = cmk(0.0, 0.0);
     cplx sv = cmk(L[2 * (2 * kP - 1)], L[2 * (2 * kP - 1) + 1]);
-    cplx hd = cmk(0.0, 0.0), hv = cmk(L[2 * (3 * kP - 1)], L[2 * (3 * kP - 1) + 1]);
+    cplx hd = cmk(0.0, 0.0), hv = hk(kP - 1);
 #pragma unroll 4
     for (int k = kP - 2; k >= 0; --k) {
         fd = cadd(cmul(fd, z), fv);
         fv = cadd(cmul(fv, z), cmk(L[2 * k], L[2 * k + 1]));
         sv = cadd(cmul(sv, z), cmk(L[2 * (kP + k)], L[2 * (kP + k) + 1]));
         hd = cadd(cmul(hd, z), hv);
-        hv = cadd(cmul(hv, z), cmk(L[2 * (2 * kP + k)], L[2 * (2 * kP + k) + 1]));
+        hv = cadd(cmul(hv, z), hk(k));
     }
     // w = H' - conj(z - c_t) F'
     const cplx w = csub(cscale(hd, iR), cmul(cmk(dx, -dy), cscale(fd, iR)));
@@ -632,7 +610,7 @@ void tilefmm_prepare(Context& c) {
         STOKES_HIP(hipStreamSynchronize(s));
     }
     F.mp.ensure(static_cast<size_t>(nc) * 3 * kP * 2);
-    F.lc.ensure(static_cast<size_t>(nc) * 3 * kP * 2);
+    F.lc.ensure(static_cast<size_t>(nc) * 4 * kP * 2);
     F.smp.ensure(std::max(1, m) * static_cast<size_t>(kStressSeries) * kP * 2);
     F.slc.ensure(std::max(1, m) * static_cast<size_t>(kStressSeries) * kP * 2);
     F.nc = nc;
@@ -686,7 +664,8 @@ void direct_launch(Context& c, int tpt, int bt, int w0, int count) {
 
 // The far field of the current density (c.ex, c.ey) into part_u/part_v[0]
 // (*wall_nsplit = *body_nsplit = 1): the direct sums one source tile per
-// block, then the local expansions, which also sum each point's tiles.
+// block beside the expansions, then the local expansions, which also sum
+// each point's tiles.
 void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit) {
     Geometry& g = c.g;
     TileFmm& F = c.fmm;
@@ -695,26 +674,36 @@ void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit) {
     const int np = g.npts, nc = F.nc, n = g.n;
     c.part_u.ensure(static_cast<size_t>(np) * F.max_tiles);
     c.part_v.ensure(static_cast<size_t>(np) * F.max_tiles);
-    if (c.time_kernels) c.t_fmm_expand.start(s);
-    k_fmm_p2m<<<nc, 256, 0, s>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p, F.cr.p, F.cl.p,
-                                 F.mp.p);
-    STOKES_LAUNCH_CHECK();
-    k_fmm_m2l<0, 3><<<nc, 256, 0, s>>>(F.cr.p, F.mp.p, F.far_ptr.p, F.far_idx.p, F.bin.p, F.lc.p);
-    STOKES_LAUNCH_CHECK();
-    if (c.time_kernels) {
-        c.t_fmm_expand.stop(s, static_cast<double>(F.far_pairs));
-        c.t_fmm_direct.start(s);
+    if (!F.side) {
+        STOKES_HIP(hipStreamCreateWithFlags(&F.side, hipStreamNonBlocking));
+        STOKES_HIP(hipEventCreateWithFlags(&F.fork, hipEventDisableTiming));
+        STOKES_HIP(hipEventCreateWithFlags(&F.join, hipEventDisableTiming));
     }
+    // multipoles and translations (matrix cores) on the side stream while
+    // the direct sums (vector ALU) run on the context's stream
+    STOKES_HIP(hipEventRecord(F.fork, s));
+    STOKES_HIP(hipStreamWaitEvent(F.side, F.fork, 0));
+    if (c.time_kernels) c.t_fmm_expand.start(F.side);
+    k_fmm_p2m<<<nc, 256, 0, F.side>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p, F.cr.p,
+                                      F.cl.p, F.mp.p);
+    STOKES_LAUNCH_CHECK();
+    k_fmm_m2l<3, true><<<dim3(nc, 3), 256, 0, F.side>>>(F.cr.p, F.mp.p, F.far_ptr.p,
+                                                        F.far_idx.p, F.bin.p, F.lc.p);
+    STOKES_LAUNCH_CHECK();
+    if (c.time_kernels) c.t_fmm_expand.stop(F.side, static_cast<double>(F.far_pairs));
+    STOKES_HIP(hipEventRecord(F.join, F.side));
+    if (c.time_kernels) c.t_fmm_direct.start(s);
     // bodies: 4 targets per lane above 128 points, else 2; wall segments: 2
     if (n > 128)
         direct_launch(c, 4, round32(cdiv_int(n, 4)), 0, F.work_body);
     else
         direct_launch(c, 2, round32(cdiv_int(n, 2)), 0, F.work_body);
     direct_launch(c, 2, round32(cdiv_int(kWallSeg, 2)), F.work_body, F.work_wall);
+    if (c.time_kernels) c.t_fmm_direct.stop(s, static_cast<double>(F.direct_pairs));
+    STOKES_HIP(hipStreamWaitEvent(s, F.join, 0));
     k_fmm_l2p<<<cdiv_int(np, kBlk), kBlk, 0, s>>>(g.x.p, g.y.p, F.cr.p, F.lc.p, F.tile_ptr.p,
                                                  g.no, n, F.nwall, np, c.part_u.p, c.part_v.p);
     STOKES_LAUNCH_CHECK();
-    if (c.time_kernels) c.t_fmm_direct.stop(s, static_cast<double>(F.direct_pairs));
     *wall_nsplit = 1;
     *body_nsplit = 1;
 }
@@ -733,7 +722,7 @@ void tilefmm_stress(Context& c, double* out) {
                                                        static_cast<size_t>(g.npts), F.cr.p, F.cl.p,
                                                        F.nwall, F.smp.p);
     STOKES_LAUNCH_CHECK();
-    k_fmm_m2l<1, kStressSeries><<<dim3(m, kStressSeries), 256, 0, s>>>(
+    k_fmm_m2l<kStressSeries, false><<<dim3(m, kStressSeries), 256, 0, s>>>(
         F.cr.p + 3 * static_cast<size_t>(F.nwall), F.smp.p, F.sfar_ptr.p, F.sfar_idx.p, F.bin.p,
         F.slc.p);
     STOKES_LAUNCH_CHECK();
