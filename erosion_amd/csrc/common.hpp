@@ -350,6 +350,11 @@ struct Context {
     int pc_nfb = 0, pc_nfw = 0;
     int pc_wall_no = -1, pc_wall_kw = -1;  // key of the kept wall block
     DevBuf<double> pc_phi_b, pc_phi_w, pc_S, pc_Gb, pc_Gw, pc_z, pc_wpart;
+    DevBuf<int> pc_bad;  // per-curve flags: coarse block singular
+    // the (z, y) of the last precond_apply_scaled with y, cleared by anything
+    // else that writes c.ex/c.ey (apply_operator's from_precond checks it)
+    const double* pc_token_x = nullptr;
+    const double* pc_token_y = nullptr;
     // per-kernel timing with HIP events on `stream` (measurement hooks only)
     bool time_kernels = false;
     // replay GMRES iteration chunks as HIP graphs (off while timing kernels)
@@ -389,7 +394,9 @@ void build_near_list(Context& c, int ibary);
 // from_precond: x_dev is the output of precond_apply_scaled(..., y_dev), which
 // has left x_dev's density in point order in c.ex/c.ey, the body constraint
 // rows in y_dev and the wall segments' parts of the rank-one integral in
-// c.scal[0 .. precond_wall_segments).
+// c.scal[0 .. precond_wall_segments).  Honoured only when the context's
+// token says that call was the last to write c.ex/c.ey with this (x, y);
+// otherwise the full set-up runs.
 // compensated: a true-residual evaluation, FP64 far field whose sums carry
 // error-free corrections (k_farfield_comp) and an assembly in double-double
 void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary,

@@ -985,6 +985,7 @@ void bd_condition(Context& c, int ibc, double* rhs_dev) {
 
 void gather_density(Context& c, const double* den_dev) {
     const Geometry& g = c.g;
+    c.pc_token_x = c.pc_token_y = nullptr;
     c.ex.ensure(static_cast<size_t>(g.npts));
     c.ey.ensure(static_cast<size_t>(g.npts));
     k_gather<<<cdiv_int(g.npts, kBlk), kBlk, 0, c.stream>>>(den_dev, g.no, g.n, g.npts, c.ex.p,
@@ -1192,6 +1193,11 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, b
     const size_t N = static_cast<size_t>(np);
     c.scal.ensure(std::max<size_t>(8, static_cast<size_t>(precond_wall_segments(g))));
 
+    // the preconditioner's work is reused only when it was the last to write
+    // c.ex/c.ey and its output is this x with these y rows (the token)
+    from_precond = from_precond && c.pc_token_x == x_dev && c.pc_token_y == y_dev;
+    c.pc_token_x = nullptr;
+    c.pc_token_y = nullptr;
     if (!from_precond) gather_density(c, x_dev);
     bool fp32 = c.ff_fp32_now && !compensated;
     // a compensated evaluation runs its own far field: 4 targets per lane,

@@ -183,7 +183,7 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
     const double* __restrict__ speed, const double* __restrict__ cx,
     const double* __restrict__ cy, const double* __restrict__ phi_w,
     const double* __restrict__ phi_b, int nfw, int nfb, int no, int n, int first_curve,
-    const double* __restrict__ S, double* __restrict__ out) {
+    const double* __restrict__ S, double* __restrict__ out, int* __restrict__ bad) {
     constexpr int MAXNF = (MAXNQ - 3) / 2;
     constexpr int kCh = 32;  // points per LDS chunk
     constexpr int kEPT = (3 * MAXNF * MAXNF + kCoarseBlk - 1) / kCoarseBlk;
@@ -342,6 +342,21 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
     }
     __syncthreads();
 
+    // largest entry: a pivot below 1e-13 of it (or not finite) marks the
+    // block singular, and the caller refuses the preconditioner
+    __shared__ double gmax[kCoarseBlk / 64];
+    __shared__ int singular;
+    {
+        double mx = 0.0;
+        for (int e = tid; e < nq * nq; e += kCoarseBlk) mx = fmax(mx, fabs(G[e]));
+        for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+        if (lane == 0) gmax[wv] = mx;
+        if (tid == 0) singular = 0;
+        __syncthreads();
+    }
+    double scale = 0.0;
+    for (int w2 = 0; w2 < kCoarseWaves; ++w2) scale = fmax(scale, gmax[w2]);
+
     for (int k = 0; k < nq; ++k) {
         if (tid < 64) {
             double best = -1.0;
@@ -373,7 +388,14 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
             }
         }
         __syncthreads();
-        const double piv = G[k * nq + k];
+        double piv = G[k * nq + k];
+        if (!(fabs(piv) > 1e-13 * scale) || !isfinite(piv)) {
+            // block-uniform: every thread reads the same pivot; the
+            // elimination goes on with a unit pivot so that no inf or NaN
+            // arises, and the flag refuses the result
+            if (tid == 0) singular = 1;
+            piv = 1.0;
+        }
         for (int r = tid; r < nq; r += kCoarseBlk) fcol[r] = r == k ? 0.0 : G[r * nq + k];
         __syncthreads();
         for (int col = tid; col < nq; col += kCoarseBlk)
@@ -401,6 +423,7 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
         }
         __syncthreads();
     }
+    if (tid == 0) bad[blockIdx.x] = singular;
     double* o = out + static_cast<size_t>(blockIdx.x) * nq * nq;
     // transposed: k_pc_apply reads column k of row-lanes coalesced
     for (int e = tid; e < nq * nq; e += kCoarseBlk) {
@@ -721,19 +744,35 @@ __global__ __launch_bounds__(kApplyBlk) void k_pc_wall(int nfw, int no, int nseg
 }  // namespace
 
 // Coarse inverses of `blocks` curves starting at first_curve, each nq x nq.
+// Coarse inverses of `blocks` curves starting at first_curve, each nq x nq;
+// bad[i] = 1 where curve first_curve + i's block is singular.
 static void coarse_launch(Context& c, int blocks, int nq, int nfw, int nfb, int first_curve,
-                          double* out) {
+                          double* out, int* bad) {
     const Geometry& g = c.g;
     constexpr int kSmallNQ = 2 * (2 * 16 + 1) + 3;  // 16 modes
     if (nq <= kSmallNQ)
         k_pc_coarse<kSmallNQ><<<blocks, kCoarseBlk, 0, c.stream>>>(
             g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p, g.cx.p, g.cy.p, c.pc_phi_w.p,
-            c.pc_phi_b.p, nfw, nfb, g.no, g.n, first_curve, c.pc_S.p, out);
+            c.pc_phi_b.p, nfw, nfb, g.no, g.n, first_curve, c.pc_S.p, out, bad);
     else
         k_pc_coarse<kMaxNQ><<<blocks, kCoarseBlk, 0, c.stream>>>(
             g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p, g.cx.p, g.cy.p, c.pc_phi_w.p,
-            c.pc_phi_b.p, nfw, nfb, g.no, g.n, first_curve, c.pc_S.p, out);
+            c.pc_phi_b.p, nfw, nfb, g.no, g.n, first_curve, c.pc_S.p, out, bad);
     STOKES_LAUNCH_CHECK();
+}
+
+// Refuses the preconditioner when a coarse block came out singular.
+static void check_coarse(Context& c, int count, int first_curve) {
+    std::vector<int> bad(static_cast<size_t>(count));
+    STOKES_HIP(hipMemcpyAsync(bad.data(), c.pc_bad.p, bad.size() * sizeof(int),
+                              hipMemcpyDeviceToHost, c.stream));
+    STOKES_HIP(hipStreamSynchronize(c.stream));
+    for (int i = 0; i < count; ++i)
+        if (bad[static_cast<size_t>(i)])
+            throw Error("the preconditioner's coarse block of curve " +
+                        std::to_string(first_curve + i) +
+                        " is singular (a pivot below 1e-13 of its largest entry); solve without "
+                        "the preconditioner");
 }
 
 void precond_setup(Context& c) {
@@ -771,7 +810,9 @@ void precond_setup(Context& c) {
             g.x.p, g.y.p, g.mx.p, g.my.p, c.pc_phi_w.p, c.pc_phi_b.p, nfw, nfb, no, n, 0,
             c.pc_S.p);
         STOKES_LAUNCH_CHECK();
-        coarse_launch(c, 1, nqw, nfw, nfb, 0, c.pc_Gw.p);
+        c.pc_bad.ensure(static_cast<size_t>(std::max(m, 1)));
+        coarse_launch(c, 1, nqw, nfw, nfb, 0, c.pc_Gw.p, c.pc_bad.p);
+        check_coarse(c, 1, 0);
         c.pc_wall_no = no;
         c.pc_wall_kw = kw;
     }
@@ -781,7 +822,9 @@ void precond_setup(Context& c) {
             g.x.p, g.y.p, g.mx.p, g.my.p, c.pc_phi_w.p, c.pc_phi_b.p, nfw, nfb, no, n, 1,
             c.pc_S.p);
         STOKES_LAUNCH_CHECK();
-        coarse_launch(c, m, nqb, nfw, nfb, 1, c.pc_Gb.p);
+        c.pc_bad.ensure(static_cast<size_t>(m));
+        coarse_launch(c, m, nqb, nfw, nfb, 1, c.pc_Gb.p, c.pc_bad.p);
+        check_coarse(c, m, 1);
     }
     c.pc_nfb = nfb;
     c.pc_nfw = nfw;
@@ -808,6 +851,11 @@ void precond_apply_scaled(Context& c, double* r, double* z, const GmresState* st
     STOKES_LAUNCH_CHECK();
     // bytes: the vector read and written once
     if (c.time_kernels) c.t_precond.stop(c.stream, 2.0 * 8.0 * static_cast<double>(g.nunk));
+    // z's density now sits in c.ex/c.ey and, with y, y's constraint rows and
+    // the rank-one parts in c.scal: apply_operator(z, y, from_precond) may
+    // reuse them
+    c.pc_token_x = y != nullptr ? z : nullptr;
+    c.pc_token_y = y;
 }
 
 void precond_apply(Context& c, const double* r, double* z) {
