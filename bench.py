@@ -59,9 +59,9 @@ STRESS_FLOP_PER_PAIR = 130.0  # per (target body point, source node) pair: 3 pas
 # pass reads exactly splits x points x 16 B), so the traffic of the shape in
 # use is this plus that.
 FF_FETCH_BYTES = {"bodies100_n256": 10.5e6, "bodies400_n256_eroded50": 45.9e6}
-# HBM bytes per k_fmm_eval launch (FETCH_SIZE x 2.00 + WRITE_SIZE), from
+# HBM bytes per k_fmm_direct launch (FETCH_SIZE x 2.00 + WRITE_SIZE), from
 # rocprofv3 counter passes on the same workload
-FMM_EVAL_BYTES = {}
+FMM_DIRECT_BYTES = {}
 
 
 def log(*a):
@@ -488,10 +488,10 @@ def main() -> int:
         roofline = {
             "bound": "valu",
             "pipe": "VALU FP64 (vector ALU; no matrix instructions in this kernel)",
-            "kernel": "k_fmm_eval (direct sums of the cluster far field, FP64, 20 FLOP per pair)",
+            "kernel": "k_fmm_direct (direct sums of the cluster far field, FP64, 20 FLOP per pair)",
             "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
             "frac": ach / PEAK_FP64_TFLOPS,
-            "traffic": FMM_EVAL_BYTES.get(w.name),
+            "traffic": FMM_DIRECT_BYTES.get(w.name),
             "traffic_unit": "bytes per launch",
         }
     else:

@@ -68,7 +68,7 @@ void stokessolver_(const int64_t* npts, const int64_t* nbods, const int64_t* nou
         Context& c = stokes::ctx();
         c.status = 0;
         c.last_error.clear();
-        // ifmm = 1: the far field may be the body-cluster expansion scheme
+        // ifmm = 1: the far field may be the cluster expansion scheme
         // (the reference's FMM switch, stokesSolver.f:530-536)
         c.ff_fmm = *ifmm != 0 && c.ff_method == 1 ? 1 : 0;
         const int n = static_cast<int>(*npts), m = static_cast<int>(*nbods),
@@ -375,7 +375,7 @@ int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t iba
         STOKES_HIP(hipMemcpyAsync(c.xvec.p, den, nunk * sizeof(double), hipMemcpyHostToDevice,
                                   c.stream));
         c.ff_fp32_now = c.precision == 2;
-        // the direct far field unless the body-cluster far field is forced
+        // the direct far field unless the cluster far field is forced
         // (then its whole application is the far-field time)
         const bool fmm = c.ff_force == stokes::farfield_shape_count() - 1;
         const int fmm_setting = c.ff_fmm;

@@ -296,7 +296,7 @@ struct Context {
     std::map<long long, float> ff_tuned_ms;  // size class -> tuned launch time
     std::map<long long, int> ff32_choice;    // the same for the FP32 far field
     int ff_force = -1;                 // a fixed far-field shape index (tests), -1 tunes
-    // the body-cluster far field may be chosen: ff_method is the caller's
+    // the cluster far field may be chosen: ff_method is the caller's
     // setting (1 allowed, 0 direct sums only), ff_fmm the current call's
     // (stokessolver_'s ifmm = 1 and the setting)
     int ff_method = 1;
@@ -408,7 +408,7 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary,
 void prepare_operator(Context& c, const double* x_dev, bool fp32);
 // far-field shape index in use for the current size (after prepare_operator)
 int farfield_choice(const Context& c, bool fp32);
-// direct launch shapes plus the body-cluster far field (the last index)
+// direct launch shapes plus the cluster far field (the last index)
 int farfield_shape_count();
 // true when the tuned FP32 far field is faster than the FP64 far field in use
 bool farfield_fp32_pays(const Context& c);

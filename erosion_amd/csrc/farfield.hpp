@@ -1,5 +1,5 @@
 // Pair routines of the FP64 all-pairs double layer, shared by the direct far
-// field (operator.hip) and the direct part of the body-cluster far field
+// field (operator.hip) and the direct part of the cluster far field
 // (tilefmm.hip).
 #pragma once
 

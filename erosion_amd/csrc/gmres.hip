@@ -591,7 +591,7 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
     }
     const bool accel = pc || c.precision != 0;
     // mixed precision switches to the FP32 far field only where that is
-    // faster than the FP64 far field in use (the body-cluster far field can
+    // faster than the FP64 far field in use (the cluster far field can
     // be faster than the direct FP32 sum at 1e5 points)
     bool fp32_pays = true;
     if (c.precision == 1) {

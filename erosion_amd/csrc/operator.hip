@@ -871,8 +871,8 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
             }
         }
     }
-    // far-field partial sums of the source splits s = sub mod kAsmSub (the
-    // body-cluster far field leaves nsplit_body partials for body points)
+    // far-field partial sums of the source splits s = sub mod kAsmSub (wall
+    // and body points may have different split counts)
     if (valid) {
         const int ns = i < no ? nsplit : nsplit_body;
         for (int s = sub; s < ns; s += kAsmSub) {
@@ -1034,7 +1034,7 @@ void boundary_values(Context& c, const int* curves_dev, int ncurves, int wall_fi
 // Shapes include the shared-reciprocal FP64 tiles, so they are used only
 // where they measure faster.
 constexpr int kFFDirectShapes = static_cast<int>(sizeof(kFFShapes) / sizeof(kFFShapes[0]));
-constexpr int kFmmIndex = kFFDirectShapes;  // the body-cluster far field
+constexpr int kFmmIndex = kFFDirectShapes;  // the cluster far field
 
 // Tuning results are kept per size class -- points per body, wall points,
 // boundary points in buckets of 2^(1/16) (about 4%) -- and, for FP64,
@@ -1164,7 +1164,7 @@ void prepare_operator(Context& c, const double* x_dev, bool fp32) {
     if (fp32) ff32_sources(c);
     const int choice = tune_farfield(c, fp32);
     if (choice == kFmmIndex) {
-        // lists and part buffers of the body-cluster far field
+        // lists and part buffers of the cluster far field
         int a = 0, b = 0;
         tilefmm_apply(c, &a, &b);
     } else {
@@ -1203,7 +1203,7 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, b
     // a compensated evaluation runs its own far field: 4 targets per lane,
     // per-pair reciprocals, about 2048 blocks
     const int choice = compensated ? -1 : tune_farfield(c, fp32);
-    if (choice == kFmmIndex) fp32 = false;  // the body-cluster far field is FP64
+    if (choice == kFmmIndex) fp32 = false;  // the cluster far field is FP64
     if (fp32) ff32_sources(c);
     int nsplit = 0, nsplit_body = 0;
     if (choice == kFmmIndex) {

@@ -28,10 +28,15 @@
 // size of the rounding of the direct sum.  Other pairs (and the cluster
 // itself) are summed directly.
 //
-// Kernels per operator application: k_fmm_p2m (one block per cluster),
-// k_fmm_m2l (one block per target cluster), k_fmm_eval (target cluster x
-// source-tile split: the direct part, and the local expansion on split 0).
-// Lists are built on the host once per geometry.
+// Kernels per operator application: on a side stream k_fmm_p2m (one block
+// per cluster) and k_fmm_m2l (matrix cores; one block per target cluster and
+// series); beside them k_fmm_direct (one block per target cluster and source
+// tile); after both, k_fmm_l2p (local expansions, and the sum of each
+// point's tiles).  Lists are built on the host once per geometry.
+//
+// The shear stress uses the same clusters: the far bodies' five
+// boundary-value series through k_fmm_p2m_bd, k_fmm_m2l and
+// k_fmm_l2p_strain (tilefmm_stress).
 #include <algorithm>
 #include <cmath>
 #include <vector>

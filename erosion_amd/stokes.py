@@ -163,7 +163,7 @@ FARFIELD_DIRECT, FARFIELD_AUTO = 0, 1
 
 
 def set_farfield_method(method: int) -> None:
-    """FARFIELD_AUTO (default): the direct sums or the body-cluster
+    """FARFIELD_AUTO (default): the direct sums or the cluster
     expansions, whichever the autotuner measures faster; FARFIELD_DIRECT:
     the direct all-pairs sums only."""
     L = lib()

@@ -42,11 +42,12 @@ extern "C" {
  * GMRES stops when the implicit residual estimate is <= tol * ||b||, tol as
  * set by erosion_set_gmres_tol (default 1e-8, the reference's hard-coded value,
  * stokesSolver.f:482).  ibary=1 selects the near-pair barycentric operator of
- * matvec_DLP_fmmbary, ibary=0 the trapezoid operator of matvec_DLP.  The far
- * field is summed directly on the GPU for either value of ifmm; the
- * all-pairs-barycentric operator of ifmm=0, ibary=1 (matvec_DLP_bary) is not
- * provided and that combination is reported as an error (den untouched,
- * nits = 0). */
+ * matvec_DLP_fmmbary, ibary=0 the trapezoid operator of matvec_DLP.  With
+ * ifmm=1 the far field is the cluster expansion or the direct sums, whichever
+ * the autotuner measures faster (erosion_set_farfield_method); with ifmm=0
+ * (and ibary=0) it is summed directly.  The all-pairs-barycentric operator of
+ * ifmm=0, ibary=1 (matvec_DLP_bary) is not provided and that combination is
+ * reported as an error (den untouched, nits = 0). */
 void stokessolver_(const int64_t* npts, const int64_t* nbods, const int64_t* nouter,
                    const int64_t* ifmm, const int64_t* ibary, const int64_t* ibc,
                    const int64_t* maxl, const double* xx, const double* yy, double* den,
@@ -57,7 +58,11 @@ void stokessolver_(const int64_t* npts, const int64_t* nbods, const int64_t* nou
  * (ErosionPackage/fortran_code/stokesSolver.f:3002-3096, deformation_on_boundary
  * :2618-2998), bound at ErosionPackage/src/DensityStress.jl:195-198.
  * ibary=0 (the trapezoid variant deformation_on_boundary_trap, :2305-2614) is
- * not provided and is reported as an error (tau zeroed). */
+ * not provided and is reported as an error (tau zeroed).  The barycentric
+ * rule covers the wall and the bodies near each target body; the other
+ * bodies' boundary-value integrals, to which that rule converges away from a
+ * curve, come from the cluster expansions (32 or more bodies, far-field
+ * method 1). */
 void computeshearstress_(const int64_t* npts, const int64_t* nbods, const int64_t* nouter,
                          const double* xx, const double* yy, const double* den,
                          const int64_t* ibary, double* tau);
@@ -163,14 +168,16 @@ int erosion_kernel_timing(int enable);
 
 /* Totals since timing was enabled, three values per family in the order
  * far field (FP64, direct), near pairs, stress evaluation, Gram-Schmidt,
- * preconditioner, far field (FP32, direct), body-cluster far field (whole
- * application), its direct sums at body targets (k_fmm_eval), its direct sums
- * at wall targets: out[3i] milliseconds, out[3i+1] launches, out[3i+2]
- * algorithmic work (ordered source-target pairs; bytes of the near-pair
- * gather pass, as erosion_time_matvec's out[6]; target-node pairs; bytes of
- * Krylov basis streamed; bytes of vector read and written; ordered
+ * preconditioner, far field (FP32, direct), cluster far field (whole
+ * application), its direct sums (k_fmm_direct), its multipole and local
+ * expansions (k_fmm_p2m + k_fmm_m2l, on a second stream beside the direct
+ * sums): out[3i] milliseconds, out[3i+1] launches, out[3i+2] algorithmic
+ * work (ordered source-target pairs; bytes of the near-pair gather pass, as
+ * erosion_time_matvec's out[6]; target-node pairs of the barycentric rule;
+ * bytes of Krylov basis streamed; bytes of vector read and written; ordered
  * source-target pairs; the Npts (Npts - 1) pairs the application stands for;
- * pairs summed; pairs summed).  `out` holds 27 doubles.  Returns 0 or -1. */
+ * pairs summed; cluster translations).  `out` holds 27 doubles.  Returns 0 or
+ * -1. */
 int erosion_kernel_timers(double* out);
 
 /* Precision of the far field in stokessolver_ (no counterpart in the
@@ -283,12 +290,14 @@ int erosion_farfield_shape_count(void);
 /* Far field of the completed operator (the reference's FMM switch ifmm,
  * stokesSolver.f:530-536, which stokessolver_ still takes): method 1
  * (default) lets the autotuner pick, per boundary size, between the direct
- * all-pairs sums and the body-cluster expansions of tilefmm.hip (multipole
- * and local expansions of 32 terms between bodies at least three times
- * their radii apart, truncation below 5e-16 of the terms; direct sums for
- * the rest and the wall), whichever is faster, on calls with ifmm = 1;
- * method 0 keeps the direct sums.  The last shape index of
- * erosion_set_farfield_shape is the body-cluster far field.  Returns 0 or -1. */
+ * all-pairs sums and the cluster expansions of tilefmm.hip (clusters: the
+ * bodies and the wall's segments of 64 points; multipole and local
+ * expansions of 32 terms between clusters at least three times their radii
+ * apart, truncation below 5e-16 of the terms; direct sums for the rest),
+ * whichever is faster, on calls with ifmm = 1, and sends the far bodies of
+ * the shear stress through the same expansions; method 0 keeps the direct
+ * sums and the barycentric rule everywhere.  The last shape index of
+ * erosion_set_farfield_shape is the cluster far field.  Returns 0 or -1. */
 int erosion_set_farfield_method(int method);
 
 /* HIP graphs for GMRES (no counterpart in the reference).  on = 1: stokessolver_
