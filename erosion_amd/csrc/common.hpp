@@ -417,7 +417,7 @@ bool farfield_fp32_pays(const Context& c);
 // number of source splits whose partial sums it leaves in part_u/part_v
 bool tilefmm_available(const Context& c);
 void tilefmm_prepare(Context& c);
-void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit);
+void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit, bool compensated = false);
 // strain rate (E11, E12, E22 at out[{0,1,2} nbp + q]) of the far bodies'
 // boundary values in c.bd
 void tilefmm_stress(Context& c, double* out);

@@ -110,6 +110,77 @@ __device__ __forceinline__ void ff_tile_shared(const double2* __restrict__ sxy,
     }
 }
 
+// The far field of a true-residual evaluation.  Where bodies carry large,
+// mostly cancelling densities, the running sum of a target's pair terms is
+// many orders above the final value, and rounding in the sum (not in the
+// terms) bounds how small a residual b - A x can be shown to be.  Each lane
+// therefore keeps, per target, the sum and a correction: every pair term
+// t = c rx enters through an error-free product (t + fma(c, rx, -t) = c rx)
+// and an error-free sum (TwoSum, Knuth), the lost low parts gathering in the
+// correction (Ogita, Rump and Oishi, "Accurate sum and dot product", 2005).
+// The pair terms themselves are the plain kernel's (their rounding is a few
+// units in the last place of each term).  Contraction is off, so the
+// error-free transformations are evaluated as written.
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+#pragma clang fp contract(off)
+    s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+}
+
+// (hi, lo) += x
+__device__ __forceinline__ void dd_add(double& hi, double& lo, double x) {
+    double s, e;
+    two_sum(hi, x, s, e);
+    hi = s;
+    lo += e;
+}
+
+// (hi, lo) += c * r
+__device__ __forceinline__ void dd_add_prod(double& hi, double& lo, double c, double r) {
+#pragma clang fp contract(off)
+    const double p = c * r;
+    const double ep = __builtin_fma(c, r, -p);
+    double s, e;
+    two_sum(hi, p, s, e);
+    hi = s;
+    lo += e + ep;
+}
+
+// The pair sums of a true-residual evaluation, TPT targets per lane: each
+// term formed as in the direct far field with its own full-precision
+// reciprocal (a shared one adds a few roundings of every term, and the terms
+// of a body's own rigid rotation are 10^6 times the sum), entering its
+// target's (sum, correction) through dd_add_prod.  DIAG: the tile holds some
+// of the lane's own targets; the zero-distance pair's rho^2 is raised to
+// 1e-100, so its reciprocal stays finite and its term (moment product 0) is 0.
+template <int TPT, bool DIAG>
+__device__ __forceinline__ void ff_tile_pairs_comp(const double2* __restrict__ sxy,
+                                                   const double2* __restrict__ sab,
+                                                   const double* __restrict__ sc,
+                                                   const double* xt, const double* yt, double* u,
+                                                   double* cu, double* v, double* cv) {
+#pragma unroll 2
+    for (int jj = 0; jj < kTile; ++jj) {
+        const double2 p = sxy[jj];
+        const double2 ab = sab[jj];
+        const double cc = sc[jj];
+#pragma unroll
+        for (int q = 0; q < TPT; ++q) {
+            const double rx = xt[q] - p.x, ry = yt[q] - p.y;
+            const double ry2 = ry * ry;
+            double rho2 = __builtin_fma(rx, rx, ry2);
+            if (DIAG) rho2 = __builtin_fmax(rho2, 1.0e-100);
+            const double t = __builtin_fma(ab.x, rx, ab.y * ry);
+            const double w = __builtin_fma(t, rx, cc * ry2);
+            const double inv = rcp64(rho2);
+            const double c = w * (inv * inv);
+            dd_add_prod(u[q], cu[q], c, rx);
+            dd_add_prod(v[q], cv[q], c, ry);
+        }
+    }
+}
+
 // Coordinate of a lane's unused target slots in the FP64 far field: far from
 // every source and from the padding sources at +1e30.
 constexpr double kNoTarget = -1.0e30;

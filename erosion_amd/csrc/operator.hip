@@ -127,43 +127,6 @@ __global__ __launch_bounds__(kBlk) void k_farfield(
 }
 
 // ---------------------------------------------- compensated FP64 far field
-// The far field of a true-residual evaluation.  Where bodies carry large,
-// mostly cancelling densities, the running sum of a target's pair terms is
-// many orders above the final value, and rounding in the sum (not in the
-// terms) bounds how small a residual b - A x can be shown to be.  Each lane
-// therefore keeps, per target, the sum and a correction: every pair term
-// t = c rx enters through an error-free product (t + fma(c, rx, -t) = c rx)
-// and an error-free sum (TwoSum, Knuth), the lost low parts gathering in the
-// correction (Ogita, Rump and Oishi, "Accurate sum and dot product", 2005).
-// The pair terms themselves are the plain kernel's (their rounding is a few
-// units in the last place of each term).  Contraction is off, so the
-// error-free transformations are evaluated as written.
-__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
-#pragma clang fp contract(off)
-    s = a + b;
-    const double bb = s - a;
-    e = (a - (s - bb)) + (b - bb);
-}
-
-// (hi, lo) += x
-__device__ __forceinline__ void dd_add(double& hi, double& lo, double x) {
-    double s, e;
-    two_sum(hi, x, s, e);
-    hi = s;
-    lo += e;
-}
-
-// (hi, lo) += c * r
-__device__ __forceinline__ void dd_add_prod(double& hi, double& lo, double c, double r) {
-#pragma clang fp contract(off)
-    const double p = c * r;
-    const double ep = __builtin_fma(c, r, -p);
-    double s, e;
-    two_sum(hi, p, s, e);
-    hi = s;
-    lo += e + ep;
-}
-
 template <int TPT, bool CHECK>
 __device__ __forceinline__ void ff_tile_comp(const double2* __restrict__ sxy,
                                              const double2* __restrict__ sab,
@@ -1200,16 +1163,20 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, b
     c.pc_token_y = nullptr;
     if (!from_precond) gather_density(c, x_dev);
     bool fp32 = c.ff_fp32_now && !compensated;
-    // a compensated evaluation runs its own far field: 4 targets per lane,
-    // per-pair reciprocals, about 2048 blocks
-    const int choice = compensated ? -1 : tune_farfield(c, fp32);
+    // a compensated evaluation runs the cluster far field with compensated
+    // direct sums where the FP64 far field in use is the cluster one, else
+    // its own direct far field: 4 targets per lane, per-pair reciprocals,
+    // about 2048 blocks
+    int choice = tune_farfield(c, fp32);
+    if (compensated && choice != kFmmIndex) choice = -1;
     if (choice == kFmmIndex) fp32 = false;  // the cluster far field is FP64
     if (fp32) ff32_sources(c);
     int nsplit = 0, nsplit_body = 0;
     if (choice == kFmmIndex) {
-        if (c.time_kernels) c.t_fmm.start(s);
-        tilefmm_apply(c, &nsplit, &nsplit_body);
-        if (c.time_kernels) c.t_fmm.stop(s, static_cast<double>(np) * static_cast<double>(np - 1));
+        const bool timed = c.time_kernels && !compensated;
+        if (timed) c.t_fmm.start(s);
+        tilefmm_apply(c, &nsplit, &nsplit_body, compensated);
+        if (timed) c.t_fmm.stop(s, static_cast<double>(np) * static_cast<double>(np - 1));
         c.ff_splits = nsplit;
         c.ff_tpt = -1;
     }

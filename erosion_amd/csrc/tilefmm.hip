@@ -14,10 +14,14 @@
 //   a_j = m_j eta_j,  s_j = 2 Re(conj(m_j) eta_j).
 // So a cluster s contributes, through the Cauchy sums F(z) = sum a_j/(z-w_j),
 // S(z) = sum s_j/(z-w_j) and E(z) = sum a_j conj(w_j - c_s)/(z-w_j),
-//   u(z) = (1/4) [ F + conj(S) + conj(E' - conj(z - c_s) F') ].
-// Summed over far clusters into local expansions about the target cluster's
-// centre c_t, with G = sum_s conj(c_t - c_s) F_s,
-//   u(z) = (1/4) [ F_L + conj(S_L) + conj(E_L' - conj(z - c_t) F_L' - G_L') ].
+//   u(z) = (1/4) [ F + conj(S + E') - (z - c_s) conj(F') ],
+// and S and E enter only as Q = S + E', one series.  Summed over far
+// clusters into local expansions about the target cluster's centre c_t,
+// with G = sum_s conj(c_t - c_s) F_s,
+//   u(z) = (1/4) [ F_L + conj(Q_L - G_L') - (z - c_t) conj(F_L') ].
+// For a rigid motion of a closed curve F and Q vanish term by term of the
+// expansion, so the large rigid rotation densities of small bodies cancel
+// inside the multipole sums instead of between translated series.
 //
 // Expansions: multipole about c_s scaled by the cluster radius R_s,
 // f(z) = sum_n M_n R_s^n / (z - c_s)^(n+1), M_n = sum_j coef_j ((w_j-c_s)/R_s)^n;
@@ -71,10 +75,19 @@ __device__ __forceinline__ cplx cpow_int(cplx z, int k) {
 }
 
 // Multipole coefficients of every cluster: grid nc, block 256 = 4 waves.
-// Wave w forms the orders k = 8w .. 8w + 7 of F, S and E: lane l sums the
-// cluster's points l, l + 64, ... (its first power by squaring, the next
-// seven by products), then the wave sums its lanes.  cr holds (cx, cy, R)
-// and cl (first point, count) per cluster; mp[c][q][k], q < 3: F, S, E.
+// Wave w forms the orders k = 8w .. 8w + 7 of two series: F, and Q = S + E'
+// (the S and E terms of a far cluster enter the velocity only as
+// conj(S + E'), so they are translated as one series),
+//   F_k = sum_j a_j zeta_j^k,   Q_k = sum_j [ s_j zeta_j^k - (k/R) e_j zeta_j^(k-1) ],
+// zeta_j = (w_j - c)/R, e_j = a_j conj(w_j - c).  For a rigid motion of a
+// closed curve F and Q vanish: the large rigid rotation densities of small
+// bodies cancel inside these sums.  COMP (a true-residual evaluation): each
+// term enters a (sum, correction) pair through error-free products and sums,
+// and the coefficients come out accurate to the terms' own rounding.  Lane l
+// sums the cluster's points l, l + 64, ... (its first power by squaring, the
+// next seven by products), then the wave sums its lanes.  cr holds (cx, cy,
+// R) and cl (first point, count) per cluster; mp[c][q][k], q < 2: F, Q.
+template <bool COMP>
 __global__ __launch_bounds__(256) void k_fmm_p2m(const double* __restrict__ x,
                                                  const double* __restrict__ y,
                                                  const double* __restrict__ mx,
@@ -88,11 +101,14 @@ __global__ __launch_bounds__(256) void k_fmm_p2m(const double* __restrict__ x,
     const int c = blockIdx.x, w = threadIdx.x / 64, l = threadIdx.x % 64;
     const double cx = cr[3 * c], cy = cr[3 * c + 1], iR = 1.0 / cr[3 * c + 2];
     const int off = cl[2 * c], len = cl[2 * c + 1];
-    cplx acc[3][kPer];
+    // (sum, correction) of the real and imaginary parts: [series][order][re, im][hi, lo]
+    double acc[2][kPer][2][2];
 #pragma unroll
-    for (int q = 0; q < 3; ++q)
+    for (int q = 0; q < 2; ++q)
 #pragma unroll
-        for (int i = 0; i < kPer; ++i) acc[q][i] = cmk(0.0, 0.0);
+        for (int i = 0; i < kPer; ++i)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) acc[q][i][h][0] = acc[q][i][h][1] = 0.0;
     for (int j = l; j < len; j += 64) {
         const int p = off + j;
         const double dx = x[p] - cx, dy = y[p] - cy;
@@ -101,33 +117,69 @@ __global__ __launch_bounds__(256) void k_fmm_p2m(const double* __restrict__ x,
         const cplx a = cmk(mxx * exx - myy * eyy, mxx * eyy + myy * exx);
         const double sj = 2.0 * (mxx * exx + myy * eyy);
         const cplx e = cmk(a.r * dx + a.i * dy, a.i * dx - a.r * dy);  // a conj(w - c)
-        cplx zk = cpow_int(z, kPer * w);  // the same count on the whole wave
+        // zeta^(k-1) and zeta^k for k = 8w (the same count on the whole wave)
+        cplx zkm = w == 0 ? cmk(0.0, 0.0) : cpow_int(z, kPer * w - 1);
+        cplx zk = w == 0 ? cmk(1.0, 0.0) : cmul(zkm, z);
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
-            cfma(acc[0][i], a, zk);
-            acc[1][i].r = __builtin_fma(sj, zk.r, acc[1][i].r);
-            acc[1][i].i = __builtin_fma(sj, zk.i, acc[1][i].i);
-            cfma(acc[2][i], e, zk);
+            const cplx ek = cscale(e, -static_cast<double>(kPer * w + i) * iR);  // -(k/R) e
+            if constexpr (!COMP) {
+                acc[0][i][0][0] = __builtin_fma(a.r, zk.r, __builtin_fma(-a.i, zk.i, acc[0][i][0][0]));
+                acc[0][i][1][0] = __builtin_fma(a.r, zk.i, __builtin_fma(a.i, zk.r, acc[0][i][1][0]));
+                acc[1][i][0][0] = __builtin_fma(sj, zk.r, acc[1][i][0][0]);
+                acc[1][i][0][0] = __builtin_fma(ek.r, zkm.r, __builtin_fma(-ek.i, zkm.i, acc[1][i][0][0]));
+                acc[1][i][1][0] = __builtin_fma(sj, zk.i, acc[1][i][1][0]);
+                acc[1][i][1][0] = __builtin_fma(ek.r, zkm.i, __builtin_fma(ek.i, zkm.r, acc[1][i][1][0]));
+                zkm = zk;
+                zk = cmul(zk, z);
+                continue;
+            }
+            // F += a zeta^k
+            dd_add_prod(acc[0][i][0][0], acc[0][i][0][1], a.r, zk.r);
+            dd_add_prod(acc[0][i][0][0], acc[0][i][0][1], -a.i, zk.i);
+            dd_add_prod(acc[0][i][1][0], acc[0][i][1][1], a.r, zk.i);
+            dd_add_prod(acc[0][i][1][0], acc[0][i][1][1], a.i, zk.r);
+            // Q += s zeta^k - (k/R) e zeta^(k-1)
+            dd_add_prod(acc[1][i][0][0], acc[1][i][0][1], sj, zk.r);
+            dd_add_prod(acc[1][i][0][0], acc[1][i][0][1], ek.r, zkm.r);
+            dd_add_prod(acc[1][i][0][0], acc[1][i][0][1], -ek.i, zkm.i);
+            dd_add_prod(acc[1][i][1][0], acc[1][i][1][1], sj, zk.i);
+            dd_add_prod(acc[1][i][1][0], acc[1][i][1][1], ek.r, zkm.i);
+            dd_add_prod(acc[1][i][1][0], acc[1][i][1][1], ek.i, zkm.r);
+            zkm = zk;
             zk = cmul(zk, z);
         }
     }
+    // wave sums of the (sum, correction) pairs: TwoSum of the sums, the
+    // corrections added
 #pragma unroll
-    for (int q = 0; q < 3; ++q)
+    for (int q = 0; q < 2; ++q)
 #pragma unroll
         for (int i = 0; i < kPer; ++i)
 #pragma unroll
-            for (int o = 32; o >= 1; o >>= 1) {
-                acc[q][i].r += __shfl_xor(acc[q][i].r, o);
-                acc[q][i].i += __shfl_xor(acc[q][i].i, o);
+            for (int h = 0; h < 2; ++h) {
+                double hi = acc[q][i][h][0], lo = acc[q][i][h][1];
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) {
+                    if constexpr (COMP) {
+                        double ss, ee;
+                        two_sum(hi, __shfl_xor(hi, o), ss, ee);
+                        lo = lo + __shfl_xor(lo, o) + ee;
+                        hi = ss;
+                    } else {
+                        hi += __shfl_xor(hi, o);
+                    }
+                }
+                acc[q][i][h][0] = COMP ? hi + lo : hi;
             }
 #pragma unroll
-    for (int q = 0; q < 3; ++q)
+    for (int q = 0; q < 2; ++q)
 #pragma unroll
         for (int i = 0; i < kPer; ++i)
             if (l == q * kPer + i) {
-                const size_t o = (static_cast<size_t>(c) * 3 + q) * kP + kPer * w + i;
-                mp[2 * o] = acc[q][i].r;
-                mp[2 * o + 1] = acc[q][i].i;
+                const size_t o = (static_cast<size_t>(c) * 2 + q) * kP + kPer * w + i;
+                mp[2 * o] = acc[q][i][0][0];
+                mp[2 * o + 1] = acc[q][i][1][0];
             }
 }
 
@@ -143,7 +195,7 @@ __global__ __launch_bounds__(256) void k_fmm_p2m(const double* __restrict__ x,
 // and Y rows k = 16 kt + l / 16 + 4 r (register r).  Grid (targets, NQ):
 // block (t, q) translates series q of mp[c][NQ][k] into lc[t][NQO][k]; 4
 // waves take batches of 16 sources in turn and meet in LDS.  With G (the
-// operator's series F, S, E), block q = 0 also forms the series
+// operator's series F, Q), block q = 0 also forms the series
 // -G = -sum_s conj(c_t - c_s) F_s into lc[t][NQ][k] (NQO = NQ + 1).  bin
 // holds B in the A-operand layout: bin[(kt * 8 + s) * 64 + l] =
 // binom(4 s + l / 16 + 16 kt + l % 16, 16 kt + l % 16).
@@ -360,13 +412,15 @@ __global__ __launch_bounds__(kBlk) void k_fmm_l2p_strain(const double* __restric
 // BT, ..., sharing one reciprocal per source).  Writes part_u/part_v[j][point]
 // for the cluster's points.  work holds (cluster, j) per item and tiles
 // (first point, count) per tile.
-template <int TPT>
+// COMP (a true-residual evaluation): each pair term enters through
+// dd_add_prod, and the corrections go to part_cu/part_cv.
+template <int TPT, bool COMP>
 __global__ __launch_bounds__(128) void k_fmm_direct(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ mx,
     const double* __restrict__ my, const double* __restrict__ ex, const double* __restrict__ ey,
     const int* __restrict__ cl, const int* __restrict__ work, const int* __restrict__ tile_ptr,
     const int* __restrict__ tiles, int npts, double* __restrict__ part_u,
-    double* __restrict__ part_v) {
+    double* __restrict__ part_v, double* __restrict__ part_cu, double* __restrict__ part_cv) {
     __shared__ double2 sxy[kTile], sab[kTile];
     __shared__ double sc[kTile];
     const int c = work[2 * blockIdx.x], j = work[2 * blockIdx.x + 1];
@@ -385,7 +439,7 @@ __global__ __launch_bounds__(128) void k_fmm_direct(
             sc[e] = 0.0;
         }
     }
-    double xt[TPT], yt[TPT], u[TPT], v[TPT];
+    double xt[TPT], yt[TPT], u[TPT], v[TPT], cu[TPT], cv[TPT];
     int tp[TPT];
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
@@ -394,67 +448,97 @@ __global__ __launch_bounds__(128) void k_fmm_direct(
         tp[q] = k < len ? off + k : -1;
         xt[q] = tp[q] >= 0 ? x[tp[q]] : kNoTarget;
         yt[q] = tp[q] >= 0 ? y[tp[q]] : kNoTarget;
-        u[q] = v[q] = 0.0;
+        u[q] = v[q] = cu[q] = cv[q] = 0.0;
     }
     __syncthreads();
     // a tile that holds points of the cluster itself holds self pairs (zero
     // distance, zero moment product): the DIAG form keeps their shared
     // product finite
-    if (ts < off + len && ts + tl > off)
-        ff_tile_shared<TPT, true>(sxy, sab, sc, xt, yt, u, v);
-    else
-        ff_tile_shared<TPT, false>(sxy, sab, sc, xt, yt, u, v);
+    const bool diag = ts < off + len && ts + tl > off;
+    if constexpr (COMP) {
+        if (diag)
+            ff_tile_pairs_comp<TPT, true>(sxy, sab, sc, xt, yt, u, cu, v, cv);
+        else
+            ff_tile_pairs_comp<TPT, false>(sxy, sab, sc, xt, yt, u, cu, v, cv);
+    } else {
+        if (diag)
+            ff_tile_shared<TPT, true>(sxy, sab, sc, xt, yt, u, v);
+        else
+            ff_tile_shared<TPT, false>(sxy, sab, sc, xt, yt, u, v);
+    }
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
         if (tp[q] >= 0) {
-            part_u[static_cast<size_t>(j) * npts + tp[q]] = u[q];
-            part_v[static_cast<size_t>(j) * npts + tp[q]] = v[q];
+            const size_t o = static_cast<size_t>(j) * npts + tp[q];
+            part_u[o] = u[q];
+            part_v[o] = v[q];
+            if (COMP) {
+                part_cu[o] = cu[q];
+                part_cv[o] = cv[q];
+            }
         }
     }
 }
 
 // Local expansions at every target point, plus the point's direct partial
 // sums (one thread per point, blocks of 256): wall point p belongs to cluster
-// p / kWallSeg, body point p >= no to cluster nwall + (p - no) / n.  Sums
-// part_u/part_v[j][p] over the cluster's tiles j and writes the total to
-// part_u/part_v[0][p].  lc[c][q][k], q < 4: F, S, E and -G (H = E - G).
+// p / kWallSeg, body point p >= no to cluster nwall + (p - no) / n.
+// lc[c][q][k], q < 3: F, Q and -G, and
+//   u = (1/4) [ F + conj(Q - G') - (z - c_t) conj(F') ].
+// Sums part_u/part_v[j][p] over the cluster's tiles j and writes the total
+// to part_u/part_v[0][p].  COMP: the tiles' (sum, correction) pairs and the
+// local value are added in double-double, the correction left in
+// part_cu/part_cv[0][p].
+template <bool COMP>
 __global__ __launch_bounds__(kBlk) void k_fmm_l2p(const double* __restrict__ x,
                                                   const double* __restrict__ y,
                                                   const double* __restrict__ cr,
                                                   const double* __restrict__ lc,
                                                   const int* __restrict__ tile_ptr, int no, int n,
                                                   int nwall, int npts, double* __restrict__ part_u,
-                                                  double* __restrict__ part_v) {
+                                                  double* __restrict__ part_v,
+                                                  double* __restrict__ part_cu,
+                                                  double* __restrict__ part_cv) {
     const int p = blockIdx.x * kBlk + threadIdx.x;
     if (p >= npts) return;
     const int c = p < no ? p / kWallSeg : nwall + (p - no) / n;
     const double dx = x[p] - cr[3 * c], dy = y[p] - cr[3 * c + 1];
     const double iR = 1.0 / cr[3 * c + 2];
     const cplx z = cmk(dx * iR, dy * iR);
-    const double* L = lc + static_cast<size_t>(c) * 4 * kP * 2;
-    auto hk = [&](int k) {  // H_k = E_k + (-G)_k
-        return cmk(L[2 * (2 * kP + k)] + L[2 * (3 * kP + k)],
-                   L[2 * (2 * kP + k) + 1] + L[2 * (3 * kP + k) + 1]);
-    };
-    // Horner: F and its derivative, S, the derivative of H
+    const double* L = lc + static_cast<size_t>(c) * 3 * kP * 2;
+    // Horner: F and its derivative, Q, the derivative of -G
     cplx fv = cmk(L[2 * (kP - 1)], L[2 * (kP - 1) + 1]), fd = cmk(0.0, 0.0);
-    cplx sv = cmk(L[2 * (2 * kP - 1)], L[2 * (2 * kP - 1) + 1]);
-    cplx hd = cmk(0.0, 0.0), hv = hk(kP - 1);
+    cplx qv = cmk(L[2 * (2 * kP - 1)], L[2 * (2 * kP - 1) + 1]);
+    cplx gv = cmk(L[2 * (3 * kP - 1)], L[2 * (3 * kP - 1) + 1]), gd = cmk(0.0, 0.0);
 #pragma unroll 4
     for (int k = kP - 2; k >= 0; --k) {
         fd = cadd(cmul(fd, z), fv);
         fv = cadd(cmul(fv, z), cmk(L[2 * k], L[2 * k + 1]));
-        sv = cadd(cmul(sv, z), cmk(L[2 * (kP + k)], L[2 * (kP + k) + 1]));
-        hd = cadd(cmul(hd, z), hv);
-        hv = cadd(cmul(hv, z), hk(k));
+        qv = cadd(cmul(qv, z), cmk(L[2 * (kP + k)], L[2 * (kP + k) + 1]));
+        gd = cadd(cmul(gd, z), gv);
+        gv = cadd(cmul(gv, z), cmk(L[2 * (2 * kP + k)], L[2 * (2 * kP + k) + 1]));
     }
-    // w = H' - conj(z - c_t) F'
-    const cplx w = csub(cscale(hd, iR), cmul(cmk(dx, -dy), cscale(fd, iR)));
-    double u = 0.25 * (fv.r + sv.r + w.r), v = 0.25 * (fv.i - sv.i - w.i);
+    const cplx A = cadd(qv, cscale(gd, iR));                    // Q - G'
+    const cplx fdz = cscale(fd, iR);                            // F'
+    const cplx B = cmul(cmk(dx, dy), cmk(fdz.r, -fdz.i));       // (z - c_t) conj(F')
+    double u = 0.25 * (fv.r + A.r - B.r), v = 0.25 * (fv.i - A.i - B.i);
     const int nt = tile_ptr[c + 1] - tile_ptr[c];
-    for (int j = 0; j < nt; ++j) {
-        u += part_u[static_cast<size_t>(j) * npts + p];
-        v += part_v[static_cast<size_t>(j) * npts + p];
+    if constexpr (COMP) {
+        double lu = 0.0, lv = 0.0;
+        for (int j = 0; j < nt; ++j) {
+            const size_t o = static_cast<size_t>(j) * npts + p;
+            dd_add(u, lu, part_u[o]);
+            dd_add(v, lv, part_v[o]);
+            lu += part_cu[o];
+            lv += part_cv[o];
+        }
+        part_cu[p] = lu;
+        part_cv[p] = lv;
+    } else {
+        for (int j = 0; j < nt; ++j) {
+            u += part_u[static_cast<size_t>(j) * npts + p];
+            v += part_v[static_cast<size_t>(j) * npts + p];
+        }
     }
     part_u[p] = u;
     part_v[p] = v;
@@ -614,8 +698,8 @@ void tilefmm_prepare(Context& c) {
         up(F.bin, bin);
         STOKES_HIP(hipStreamSynchronize(s));
     }
-    F.mp.ensure(static_cast<size_t>(nc) * 3 * kP * 2);
-    F.lc.ensure(static_cast<size_t>(nc) * 4 * kP * 2);
+    F.mp.ensure(static_cast<size_t>(nc) * 2 * kP * 2);
+    F.lc.ensure(static_cast<size_t>(nc) * 3 * kP * 2);
     F.smp.ensure(std::max(1, m) * static_cast<size_t>(kStressSeries) * kP * 2);
     F.slc.ensure(std::max(1, m) * static_cast<size_t>(kStressSeries) * kP * 2);
     F.nc = nc;
@@ -649,19 +733,22 @@ namespace {
 inline int round32(int v) { return (v + 31) / 32 * 32; }
 
 // Direct sums of work items [w0, w0 + count) (clusters of len <= TPT bt).
+template <bool COMP>
 void direct_launch(Context& c, int tpt, int bt, int w0, int count) {
     Geometry& g = c.g;
     TileFmm& F = c.fmm;
     if (count <= 0) return;
     const int* work = F.work.p + 2 * static_cast<size_t>(w0);
+    double* pcu = COMP ? c.part_cu.p : nullptr;
+    double* pcv = COMP ? c.part_cv.p : nullptr;
     if (tpt == 4)
-        k_fmm_direct<4><<<count, bt, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
-                                                    F.cl.p, work, F.tile_ptr.p, F.tiles.p, g.npts,
-                                                    c.part_u.p, c.part_v.p);
+        k_fmm_direct<4, COMP><<<count, bt, 0, c.stream>>>(
+            g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p, F.cl.p, work, F.tile_ptr.p, F.tiles.p,
+            g.npts, c.part_u.p, c.part_v.p, pcu, pcv);
     else
-        k_fmm_direct<2><<<count, bt, 0, c.stream>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
-                                                    F.cl.p, work, F.tile_ptr.p, F.tiles.p, g.npts,
-                                                    c.part_u.p, c.part_v.p);
+        k_fmm_direct<2, COMP><<<count, bt, 0, c.stream>>>(
+            g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p, F.cl.p, work, F.tile_ptr.p, F.tiles.p,
+            g.npts, c.part_u.p, c.part_v.p, pcu, pcv);
     STOKES_LAUNCH_CHECK();
 }
 
@@ -670,8 +757,9 @@ void direct_launch(Context& c, int tpt, int bt, int w0, int count) {
 // The far field of the current density (c.ex, c.ey) into part_u/part_v[0]
 // (*wall_nsplit = *body_nsplit = 1): the direct sums one source tile per
 // block beside the expansions, then the local expansions, which also sum
-// each point's tiles.
-void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit) {
+// each point's tiles.  compensated: the direct sums and the final sum carry
+// corrections (part_cu/part_cv[0]) for a true-residual evaluation.
+void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit, bool compensated) {
     Geometry& g = c.g;
     TileFmm& F = c.fmm;
     tilefmm_prepare(c);
@@ -679,6 +767,10 @@ void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit) {
     const int np = g.npts, nc = F.nc, n = g.n;
     c.part_u.ensure(static_cast<size_t>(np) * F.max_tiles);
     c.part_v.ensure(static_cast<size_t>(np) * F.max_tiles);
+    if (compensated) {
+        c.part_cu.ensure(static_cast<size_t>(np) * F.max_tiles);
+        c.part_cv.ensure(static_cast<size_t>(np) * F.max_tiles);
+    }
     if (!F.side) {
         STOKES_HIP(hipStreamCreateWithFlags(&F.side, hipStreamNonBlocking));
         STOKES_HIP(hipEventCreateWithFlags(&F.fork, hipEventDisableTiming));
@@ -686,28 +778,43 @@ void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit) {
     }
     // multipoles and translations (matrix cores) on the side stream while
     // the direct sums (vector ALU) run on the context's stream
+    // timers count the operator's applications, not true-residual ones
+    const bool timed = c.time_kernels && !compensated;
     STOKES_HIP(hipEventRecord(F.fork, s));
     STOKES_HIP(hipStreamWaitEvent(F.side, F.fork, 0));
-    if (c.time_kernels) c.t_fmm_expand.start(F.side);
-    k_fmm_p2m<<<nc, 256, 0, F.side>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p, F.cr.p,
-                                      F.cl.p, F.mp.p);
+    if (timed) c.t_fmm_expand.start(F.side);
+    if (compensated)
+        k_fmm_p2m<true><<<nc, 256, 0, F.side>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
+                                                F.cr.p, F.cl.p, F.mp.p);
+    else
+        k_fmm_p2m<false><<<nc, 256, 0, F.side>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
+                                                 F.cr.p, F.cl.p, F.mp.p);
     STOKES_LAUNCH_CHECK();
-    k_fmm_m2l<3, true><<<dim3(nc, 3), 256, 0, F.side>>>(F.cr.p, F.mp.p, F.far_ptr.p,
+    k_fmm_m2l<2, true><<<dim3(nc, 2), 256, 0, F.side>>>(F.cr.p, F.mp.p, F.far_ptr.p,
                                                         F.far_idx.p, F.bin.p, F.lc.p);
     STOKES_LAUNCH_CHECK();
-    if (c.time_kernels) c.t_fmm_expand.stop(F.side, static_cast<double>(F.far_pairs));
+    if (timed) c.t_fmm_expand.stop(F.side, static_cast<double>(F.far_pairs));
     STOKES_HIP(hipEventRecord(F.join, F.side));
-    if (c.time_kernels) c.t_fmm_direct.start(s);
+    if (timed) c.t_fmm_direct.start(s);
     // bodies: 4 targets per lane above 128 points, else 2; wall segments: 2
+    auto direct = compensated ? direct_launch<true> : direct_launch<false>;
     if (n > 128)
-        direct_launch(c, 4, round32(cdiv_int(n, 4)), 0, F.work_body);
+        direct(c, 4, round32(cdiv_int(n, 4)), 0, F.work_body);
     else
-        direct_launch(c, 2, round32(cdiv_int(n, 2)), 0, F.work_body);
-    direct_launch(c, 2, round32(cdiv_int(kWallSeg, 2)), F.work_body, F.work_wall);
-    if (c.time_kernels) c.t_fmm_direct.stop(s, static_cast<double>(F.direct_pairs));
+        direct(c, 2, round32(cdiv_int(n, 2)), 0, F.work_body);
+    direct(c, 2, round32(cdiv_int(kWallSeg, 2)), F.work_body, F.work_wall);
+    if (timed) c.t_fmm_direct.stop(s, static_cast<double>(F.direct_pairs));
     STOKES_HIP(hipStreamWaitEvent(s, F.join, 0));
-    k_fmm_l2p<<<cdiv_int(np, kBlk), kBlk, 0, s>>>(g.x.p, g.y.p, F.cr.p, F.lc.p, F.tile_ptr.p,
-                                                 g.no, n, F.nwall, np, c.part_u.p, c.part_v.p);
+    if (compensated)
+        k_fmm_l2p<true><<<cdiv_int(np, kBlk), kBlk, 0, s>>>(g.x.p, g.y.p, F.cr.p, F.lc.p,
+                                                           F.tile_ptr.p, g.no, n, F.nwall, np,
+                                                           c.part_u.p, c.part_v.p, c.part_cu.p,
+                                                           c.part_cv.p);
+    else
+        k_fmm_l2p<false><<<cdiv_int(np, kBlk), kBlk, 0, s>>>(g.x.p, g.y.p, F.cr.p, F.lc.p,
+                                                            F.tile_ptr.p, g.no, n, F.nwall, np,
+                                                            c.part_u.p, c.part_v.p, nullptr,
+                                                            nullptr);
     STOKES_LAUNCH_CHECK();
     *wall_nsplit = 1;
     *body_nsplit = 1;

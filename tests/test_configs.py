@@ -250,3 +250,37 @@ def test_stress_far_bodies_by_expansions(name):
     e = relerr(tau_c, tau_d)
     print(f"{name}: shear stress, far bodies by expansions vs barycentric everywhere {e:.2e}")
     assert e <= 1e-10
+
+
+def _compensated_relres(n, m, no, xx, yy, x, method):
+    """The library's compensated relative residual of x: the initial-guess
+    residual of a solve started from x (one iteration at an unreachable
+    tolerance), with the far-field method given."""
+    stokes.set_farfield_method(method)
+    old = stokes.get_gmres_tol()
+    stokes.set_gmres_tol(1e-30)
+    try:
+        stokes.stokessolver(n, m, no, 1, 1, 0, 1, xx, yy, den=x.copy())
+    finally:
+        stokes.set_gmres_tol(old)
+        stokes.set_farfield_method(stokes.FARFIELD_AUTO)
+    return stokes.last_solve_info()["r0_relres"]
+
+
+@pytest.mark.parametrize("name", ["bodies100_n256", "bodies400_n256_eroded50"])
+def test_compensated_residual_by_clusters(name):
+    """The true residual of record through the cluster far field (compensated
+    direct sums, multipoles with error-free sums, F and Q series) against the
+    compensated all-pairs sum and against the oracle's operator, on a
+    converged density whose rigid rotation densities are 10^3..10^6 times the
+    right-hand side.  Bars: 1e-12 between the two library evaluations, 5e-12
+    to the oracle (a 1e-10 tolerance is decided on these values)."""
+    _, n, m, no, xx, yy = _state(name)
+    with solver():
+        x, _, _ = solve(n, m, no, xx, yy)
+        rc = _compensated_relres(n, m, no, xx, yy, x, stokes.FARFIELD_AUTO)
+        rd = _compensated_relres(n, m, no, xx, yy, x, stokes.FARFIELD_DIRECT)
+    ro = relres(oracle_on_library_geometry(n, m, no, xx, yy), x)
+    print(f"{name}: compensated residual by clusters {rc:.6e}, all pairs {rd:.6e}, oracle {ro:.6e}")
+    assert abs(rc - rd) <= 1e-12
+    assert abs(rc - ro) <= 5e-12
