@@ -213,6 +213,9 @@ struct TileFmm {
     // and locals of its five boundary-value series, and (host) the near
     // bodies of each body, itself included, for the barycentric rule
     DevBuf<int> sfar_ptr, sfar_idx;
+    DevBuf<int> nb_ptr, nb_idx;         // near bodies of each cluster (the assembly's stokeslets)
+    DevBuf<double> mlog;                // per cluster the log terms (c_F, c_Q) of its stokeslet
+    DevBuf<double> logd;                // per far pair log(c_t - c_s) (complex)
     DevBuf<double> smp, slc;            // [m][5][32] complex
     std::vector<int> h_snear_ptr, h_snear_idx;
     long long stress_direct_pairs = 0;  // (target point, source node) pairs of the near curves
@@ -417,7 +420,16 @@ bool farfield_fp32_pays(const Context& c);
 // number of source splits whose partial sums it leaves in part_u/part_v
 bool tilefmm_available(const Context& c);
 void tilefmm_prepare(Context& c);
-void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit, bool compensated = false);
+void tilefmm_apply(Context& c, const double* den, int* wall_nsplit, int* body_nsplit,
+                   bool compensated = false);
+// the near bodies of every cluster (CSR, body indices): point p belongs to
+// cluster p / wall_seg on the wall, nwall + (p - no) / n on a body
+struct TileNear {
+    const int* ptr;
+    const int* idx;
+    int nwall, wall_seg;
+};
+TileNear tilefmm_near_bodies(const Context& c);
 // strain rate (E11, E12, E22 at out[{0,1,2} nbp + q]) of the far bodies'
 // boundary values in c.bd
 void tilefmm_stress(Context& c, double* out);

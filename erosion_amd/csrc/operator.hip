@@ -794,17 +794,44 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
     const int* __restrict__ tgt_ptr,
     const int* __restrict__ tgt_ent, const double* __restrict__ corr_u,
     const double* __restrict__ corr_v, int has_near, const double* __restrict__ scal, int nsig,
-    const double* __restrict__ lgtab, int no, int n, int m, int npts, double* __restrict__ out) {
+    const double* __restrict__ lgtab, int no, int n, int m, int npts,
+    const int* __restrict__ nb_ptr, const int* __restrict__ nb_idx, int nwall, int wall_seg,
+    double* __restrict__ out) {
     __shared__ double sc[5][kBlk];
     const int sub = threadIdx.x & (kAsmSub - 1);
     const int i = blockIdx.x * (kBlk / kAsmSub) + threadIdx.x / kAsmSub;
     const bool valid = i < npts;
     const int ii = valid ? i : 0;
     const double xi = x[ii], yi = y[ii];
-    // stokeslets and rotlets of every body, bodies q = sub mod kAsmSub on this lane
+    // stokeslets and rotlets of every body, bodies q = sub mod kAsmSub on this
+    // lane; with the cluster far field (nb_ptr), only the bodies near the
+    // point's cluster (the far ones came with the expansions), their
+    // logarithms formed here
     double us = 0.0, vs = 0.0, lus = 0.0, lvs = 0.0;
     const int base = 2 * no + 2 * n * m;
-    for (int b0 = 0; b0 < m; b0 += kBlk) {
+    if (nb_ptr != nullptr) {
+        const int cc = ii < no ? ii / wall_seg : nwall + (ii - no) / n;
+        for (int t = nb_ptr[cc] + sub; t < nb_ptr[cc + 1]; t += kAsmSub) {
+            const int b = nb_idx[t];
+            const double rx = xi - cx[b], ry = yi - cy[b];
+            const double sto1 = den[base + 3 * b], sto2 = den[base + 3 * b + 1],
+                         rot = den[base + 3 * b + 2];
+            const double rho2 = rx * rx + ry * ry;
+            const double irho2 = rcp64(rho2);
+            const double lg = log(rho2);
+            const double rdots = rx * sto1 + ry * sto2;
+            const double tu = 0.5 / kTwoPi * (-0.5 * lg * sto1 + rdots * irho2 * rx) + rot * ry * irho2;
+            const double tv = 0.5 / kTwoPi * (-0.5 * lg * sto2 + rdots * irho2 * ry) - rot * rx * irho2;
+            if (COMP) {
+                dd_add(us, lus, tu);
+                dd_add(vs, lvs, tv);
+            } else {
+                us += tu;
+                vs += tv;
+            }
+        }
+    }
+    for (int b0 = 0; nb_ptr == nullptr && b0 < m; b0 += kBlk) {  // block-uniform
         __syncthreads();
         const int bb = b0 + threadIdx.x;
         if (bb < m) {
@@ -1051,7 +1078,7 @@ static int tune_farfield(Context& c, bool fp32) {
         if (i == kFmmIndex) {
             launch = [&]() {
                 int a = 0, b = 0;
-                tilefmm_apply(c, &a, &b);
+                tilefmm_apply(c, nullptr, &a, &b);
             };
         } else {
             // the FP32 kernel has no shared form (and no 8-target form)
@@ -1129,13 +1156,15 @@ void prepare_operator(Context& c, const double* x_dev, bool fp32) {
     if (choice == kFmmIndex) {
         // lists and part buffers of the cluster far field
         int a = 0, b = 0;
-        tilefmm_apply(c, &a, &b);
+        tilefmm_apply(c, nullptr, &a, &b);
     } else {
         const FFPlan plan = ff_plan(c.g.npts, kFFShapes[choice], fp32);
         c.part_u.ensure(static_cast<size_t>(c.g.npts) * static_cast<size_t>(plan.nsplit));
         c.part_v.ensure(static_cast<size_t>(c.g.npts) * static_cast<size_t>(plan.nsplit));
+        // the direct far field's assembly reads every body's stokeslet
+        // logarithms from a table (the cluster far field needs none)
+        stokes_logs(c);
     }
-    stokes_logs(c);
 }
 
 int farfield_shape_count() { return kFFDirectShapes + 1; }
@@ -1175,7 +1204,7 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, b
     if (choice == kFmmIndex) {
         const bool timed = c.time_kernels && !compensated;
         if (timed) c.t_fmm.start(s);
-        tilefmm_apply(c, &nsplit, &nsplit_body, compensated);
+        tilefmm_apply(c, x_dev, &nsplit, &nsplit_body, compensated);
         if (timed) c.t_fmm.stop(s, static_cast<double>(np) * static_cast<double>(np - 1));
         c.ff_splits = nsplit;
         c.ff_tpt = -1;
@@ -1229,14 +1258,18 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, b
         STOKES_LAUNCH_CHECK();
     }
     const int nsig = from_precond ? precond_wall_segments(g) : 1;
-    stokes_logs(c);
+    // with the cluster far field the far bodies' stokeslets came with it, and
+    // the assembly sums the near bodies' with logarithms of its own
+    const TileNear tn = choice == kFmmIndex ? tilefmm_near_bodies(c) : TileNear{nullptr, nullptr, 0, 1};
+    if (choice != kFmmIndex) stokes_logs(c);
 #define STOKES_ASM(COMP)                                                                       \
     k_assemble<COMP><<<cdiv_int(np, kBlk / kAsmSub), kBlk, 0, s>>>(                             \
         g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p, c.ex.p, c.ey.p, g.cx.p, g.cy.p, x_dev, \
         c.part_u.p, c.part_v.p, c.part_cu.p, c.part_cv.p, nsplit, nsplit_body, L.tgt_ptr.p,      \
         L.tgt_ent.p,                                                                             \
         L.corr_u.p, L.corr_v.p, has_near ? 1 : 0, c.scal.p, nsig,                                 \
-        g.m > 0 ? c.stk_log.p : nullptr, g.no, g.n, g.m, np, y_dev)
+        g.m > 0 && choice != kFmmIndex ? c.stk_log.p : nullptr, g.no, g.n, g.m, np, tn.ptr,      \
+        tn.idx, tn.nwall, tn.wall_seg, y_dev)
     if (compensated)
         STOKES_ASM(true);
     else

@@ -87,6 +87,16 @@ __device__ __forceinline__ cplx cpow_int(cplx z, int k) {
 // sums the cluster's points l, l + 64, ... (its first power by squaring, the
 // next seven by products), then the wave sums its lanes.  cr holds (cx, cy,
 // R) and cl (first point, count) per cluster; mp[c][q][k], q < 2: F, Q.
+//
+// A body also carries its stokeslet lambda and rotlet xi (stk[3b..3b+2],
+// cluster nwall + b, at its centre): with r = z - c,
+//   (1/4pi) [ -log|r| lambda + (r.lambda) r / |r|^2 ] + xi (r_y, -r_x) / |r|^2
+//   = (1/4) [ -(1/2pi) (lambda log r + conj(conj(lambda) log r))
+//             + (1/2pi) lambda + (1/2pi) conj(lambda) r / conj(r) ] + conj(i xi / r),
+// which is the velocity formula above for F + c_F log(z - c), Q + c_Q
+// log(z - c) + 4 i xi / (z - c), c_F = -lambda/(2pi), c_Q = -conj(lambda)/(2pi),
+// plus the constant lambda / (8pi): mlog[c] = (c_F, c_Q) and Q_0 += 4 i xi.
+// stk == nullptr: no stokeslets (zero).
 template <bool COMP>
 __global__ __launch_bounds__(256) void k_fmm_p2m(const double* __restrict__ x,
                                                  const double* __restrict__ y,
@@ -96,7 +106,9 @@ __global__ __launch_bounds__(256) void k_fmm_p2m(const double* __restrict__ x,
                                                  const double* __restrict__ ey,
                                                  const double* __restrict__ cr,
                                                  const int* __restrict__ cl,
-                                                 double* __restrict__ mp) {
+                                                 const double* __restrict__ stk, int nwall,
+                                                 double* __restrict__ mp,
+                                                 double* __restrict__ mlog) {
     constexpr int kPer = kP / 4;
     const int c = blockIdx.x, w = threadIdx.x / 64, l = threadIdx.x % 64;
     const double cx = cr[3 * c], cy = cr[3 * c + 1], iR = 1.0 / cr[3 * c + 2];
@@ -172,15 +184,29 @@ __global__ __launch_bounds__(256) void k_fmm_p2m(const double* __restrict__ x,
                 }
                 acc[q][i][h][0] = COMP ? hi + lo : hi;
             }
+    double lr = 0.0, li = 0.0, xi = 0.0;  // lambda, xi of a body
+    if (stk != nullptr && c >= nwall) {
+        const int b = c - nwall;
+        lr = stk[3 * b];
+        li = stk[3 * b + 1];
+        xi = stk[3 * b + 2];
+    }
 #pragma unroll
     for (int q = 0; q < 2; ++q)
 #pragma unroll
         for (int i = 0; i < kPer; ++i)
             if (l == q * kPer + i) {
                 const size_t o = (static_cast<size_t>(c) * 2 + q) * kP + kPer * w + i;
+                const bool rot = q == 1 && w == 0 && i == 0;  // Q_0 += 4 i xi
                 mp[2 * o] = acc[q][i][0][0];
-                mp[2 * o + 1] = acc[q][i][1][0];
+                mp[2 * o + 1] = acc[q][i][1][0] + (rot ? 4.0 * xi : 0.0);
             }
+    if (threadIdx.x == 0) {
+        mlog[4 * c] = -lr / kTwoPi;      // c_F = -lambda / (2 pi)
+        mlog[4 * c + 1] = -li / kTwoPi;
+        mlog[4 * c + 2] = -lr / kTwoPi;  // c_Q = -conj(lambda) / (2 pi)
+        mlog[4 * c + 3] = li / kTwoPi;
+    }
 }
 
 // Local expansions of every target cluster from its far clusters, on the
@@ -201,12 +227,21 @@ __global__ __launch_bounds__(256) void k_fmm_p2m(const double* __restrict__ x,
 // binom(4 s + l / 16 + 16 kt + l % 16, 16 kt + l % 16).
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-template <int NQ, bool G>
+//
+// mlog (or nullptr): per source cluster a log term c_q log(z - c_s) of
+// series q < 2, whose local expansion about c_t is
+//   c log(-d) - c sum_{k>=1} (1/k) ((z - c_t)/d)^k,
+// and for series 0 the constant -c_0 (the stokeslet's lambda / (8 pi) in
+// the velocity, times 4).
+// logd[i] holds log(c_t - c_s) of far pair i (in far_idx's order).
+template <int NQ, bool G, bool LOGS>
 __global__ __launch_bounds__(256) void k_fmm_m2l(const double* __restrict__ cr,
                                                  const double* __restrict__ mp,
                                                  const int* __restrict__ far_ptr,
                                                  const int* __restrict__ far_idx,
                                                  const double* __restrict__ bin,
+                                                 const double* __restrict__ mlog,
+                                                 const double* __restrict__ logd,
                                                  double* __restrict__ lc) {
     constexpr int NQO = NQ + (G ? 1 : 0);
     __shared__ double RED[4][2][kP][2];
@@ -220,19 +255,34 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(const double* __restrict__ cr,
         for (int st = 0; st < 8; ++st) A[kt][st] = bin[(kt * 8 + st) * 64 + l];
     const double ctx = cr[3 * t], cty = cr[3 * t + 1], Rt = cr[3 * t + 2];
     cplx LQ[8], LG[8];
+    double ik[8];  // 1/k of this lane's rows k = 16 kt + h + 4 r (0 for k = 0)
 #pragma unroll
-    for (int e = 0; e < 8; ++e) LQ[e] = LG[e] = cmk(0.0, 0.0);
+    for (int e = 0; e < 8; ++e) {
+        LQ[e] = LG[e] = cmk(0.0, 0.0);
+        const int k = 16 * (e / 4) + h + 4 * (e % 4);
+        ik[e] = LOGS && k > 0 ? 1.0 / k : 0.0;
+    }
     const int f0 = far_ptr[t], nf = far_ptr[t + 1] - f0;
     for (int b0 = 16 * w; b0 < nf; b0 += 64) {  // uniform over the wave
         const int i = b0 + j;
         const int s = i < nf ? far_idx[f0 + i] : -1;
         cplx id = cmk(0.0, 0.0), u = cmk(0.0, 0.0), v = cmk(0.0, 0.0), cmd = cmk(0.0, 0.0);
+        cplx d = cmk(0.0, 0.0), lg = cmk(0.0, 0.0);
         if (s >= 0) {
-            const cplx d = cmk(cr[3 * s] - ctx, cr[3 * s + 1] - cty);
+            d = cmk(cr[3 * s] - ctx, cr[3 * s + 1] - cty);
             id = cinv(d);
             u = cscale(id, -cr[3 * s + 2]);  // -R_s / d
             v = cscale(id, Rt);              // R_t / d
             cmd = cmk(-d.r, d.i);            // conj(c_t - c_s)
+            if (LOGS && q < 2) lg = cmk(mlog[4 * s + 2 * q], mlog[4 * s + 2 * q + 1]);
+        }
+        // log term: c d f_k / k for k >= 1 (= -c (R_t/d)^k / k), and at k = 0
+        // c log(-d), less c for series 0
+        cplx lgd = cmk(0.0, 0.0), l0 = cmk(0.0, 0.0);
+        if (LOGS && s >= 0) {
+            lgd = cmul(lg, d);
+            l0 = cmul(lg, cmk(logd[2 * (f0 + i)], logd[2 * (f0 + i) + 1]));
+            if (q == 0) l0 = csub(l0, lg);
         }
         // W rows n = 4 st + h: (-R_s/d)^n M[n]
         const cplx u2 = cmul(u, u), u4 = cmul(u2, u2);
@@ -268,7 +318,11 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(const double* __restrict__ cr,
             for (int kt = 0; kt < 2; ++kt) {
                 const int e = 4 * kt + r;
                 const cplx fk = kt == 0 ? f : cmul(f, v16);
-                const cplx fy = cmul(fk, cmk(yr[kt][r], yi[kt][r]));
+                cplx fy = cmul(fk, cmk(yr[kt][r], yi[kt][r]));
+                if (LOGS) {
+                    const int k = 16 * kt + h + 4 * r;
+                    fy = k > 0 ? cadd(fy, cscale(cmul(lgd, fk), ik[e])) : cadd(fy, l0);
+                }
                 LQ[e] = cadd(LQ[e], fy);
                 if (with_g) LG[e] = csub(LG[e], cmul(cmd, fy));
             }
@@ -607,7 +661,9 @@ void tilefmm_prepare(Context& c) {
         cl[2 * static_cast<size_t>(t) + 1] = C[static_cast<size_t>(t)].len;
     }
     std::vector<int> far_ptr(static_cast<size_t>(nc) + 1, 0), far_idx,
-        tile_ptr(static_cast<size_t>(nc) + 1, 0), tiles;
+        tile_ptr(static_cast<size_t>(nc) + 1, 0), tiles, nb_ptr(static_cast<size_t>(nc) + 1, 0),
+        nb_idx;
+    std::vector<double> logd;
     std::vector<std::pair<int, int>> near;  // (first point, count)
     long long direct = 0;
     for (int t = 0; t < nc; ++t) {
@@ -619,9 +675,13 @@ void tilefmm_prepare(Context& c) {
             const double dx = S.cx - T.cx, dy = S.cy - T.cy;
             if (s != t && S.r + T.r <= kSep * std::sqrt(dx * dx + dy * dy)) {
                 far_idx.push_back(s);
+                // log(c_t - c_s), the branch the same for both series of a pair
+                logd.push_back(0.5 * std::log(dx * dx + dy * dy));
+                logd.push_back(std::atan2(-dy, -dx));
             } else {
                 near.emplace_back(S.start, S.len);
                 nsrc += S.len;
+                if (s >= nwall) nb_idx.push_back(s - nwall);
             }
         }
         // near clusters adjacent in memory (runs of wall segments) merge
@@ -640,6 +700,7 @@ void tilefmm_prepare(Context& c) {
         }
         far_ptr[static_cast<size_t>(t) + 1] = static_cast<int>(far_idx.size());
         tile_ptr[static_cast<size_t>(t) + 1] = static_cast<int>(tiles.size() / 2);
+        nb_ptr[static_cast<size_t>(t) + 1] = static_cast<int>(nb_idx.size());
         direct += static_cast<long long>(T.len) * nsrc;
     }
     // shear stress: far bodies of each body by the same rule (expansions),
@@ -679,6 +740,9 @@ void tilefmm_prepare(Context& c) {
     up(F.tiles, tiles);
     up(F.sfar_ptr, sfar_ptr);
     up(F.sfar_idx, sfar_idx);
+    up(F.nb_ptr, nb_ptr);
+    up(F.nb_idx, nb_idx);
+    up(F.logd, logd);
     STOKES_HIP(hipStreamSynchronize(s));
     if (!F.bin.p) {
         // binom(n + k, k) for n, k < 32 (exact in long double: < 2^63), in
@@ -699,6 +763,7 @@ void tilefmm_prepare(Context& c) {
         STOKES_HIP(hipStreamSynchronize(s));
     }
     F.mp.ensure(static_cast<size_t>(nc) * 2 * kP * 2);
+    F.mlog.ensure(static_cast<size_t>(nc) * 4);
     F.lc.ensure(static_cast<size_t>(nc) * 3 * kP * 2);
     F.smp.ensure(std::max(1, m) * static_cast<size_t>(kStressSeries) * kP * 2);
     F.slc.ensure(std::max(1, m) * static_cast<size_t>(kStressSeries) * kP * 2);
@@ -757,9 +822,13 @@ void direct_launch(Context& c, int tpt, int bt, int w0, int count) {
 // The far field of the current density (c.ex, c.ey) into part_u/part_v[0]
 // (*wall_nsplit = *body_nsplit = 1): the direct sums one source tile per
 // block beside the expansions, then the local expansions, which also sum
-// each point's tiles.  compensated: the direct sums and the final sum carry
-// corrections (part_cu/part_cv[0]) for a true-residual evaluation.
-void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit, bool compensated) {
+// each point's tiles.  The stokeslets and rotlets of each target cluster's
+// far bodies (den's last 3 m entries; den == nullptr: none) enter through
+// the same expansions; k_assemble adds the near bodies' (tilefmm_near_bodies).
+// compensated: the direct sums and the final sum carry corrections
+// (part_cu/part_cv[0]) for a true-residual evaluation.
+void tilefmm_apply(Context& c, const double* den, int* wall_nsplit, int* body_nsplit,
+                   bool compensated) {
     Geometry& g = c.g;
     TileFmm& F = c.fmm;
     tilefmm_prepare(c);
@@ -783,15 +852,18 @@ void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit, bool compensa
     STOKES_HIP(hipEventRecord(F.fork, s));
     STOKES_HIP(hipStreamWaitEvent(F.side, F.fork, 0));
     if (timed) c.t_fmm_expand.start(F.side);
+    const double* stk = den != nullptr ? den + 2 * static_cast<size_t>(g.no) +
+                                             2 * static_cast<size_t>(g.n) * g.m
+                                       : nullptr;
     if (compensated)
         k_fmm_p2m<true><<<nc, 256, 0, F.side>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
-                                                F.cr.p, F.cl.p, F.mp.p);
+                                                F.cr.p, F.cl.p, stk, F.nwall, F.mp.p, F.mlog.p);
     else
         k_fmm_p2m<false><<<nc, 256, 0, F.side>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
-                                                 F.cr.p, F.cl.p, F.mp.p);
+                                                 F.cr.p, F.cl.p, stk, F.nwall, F.mp.p, F.mlog.p);
     STOKES_LAUNCH_CHECK();
-    k_fmm_m2l<2, true><<<dim3(nc, 2), 256, 0, F.side>>>(F.cr.p, F.mp.p, F.far_ptr.p,
-                                                        F.far_idx.p, F.bin.p, F.lc.p);
+    k_fmm_m2l<2, true, true><<<dim3(nc, 2), 256, 0, F.side>>>(
+        F.cr.p, F.mp.p, F.far_ptr.p, F.far_idx.p, F.bin.p, F.mlog.p, F.logd.p, F.lc.p);
     STOKES_LAUNCH_CHECK();
     if (timed) c.t_fmm_expand.stop(F.side, static_cast<double>(F.far_pairs));
     STOKES_HIP(hipEventRecord(F.join, F.side));
@@ -820,6 +892,11 @@ void tilefmm_apply(Context& c, int* wall_nsplit, int* body_nsplit, bool compensa
     *body_nsplit = 1;
 }
 
+TileNear tilefmm_near_bodies(const Context& c) {
+    const TileFmm& F = c.fmm;
+    return TileNear{F.nb_ptr.p, F.nb_idx.p, F.nwall, kWallSeg};
+}
+
 // Strain rate at every body point from the far bodies of the shear stress
 // (the bodies that are far by the clusters' rule), from the boundary values
 // in c.bd: out[{0,1,2} nbp + q] = E11, E12, E22.
@@ -834,9 +911,9 @@ void tilefmm_stress(Context& c, double* out) {
                                                        static_cast<size_t>(g.npts), F.cr.p, F.cl.p,
                                                        F.nwall, F.smp.p);
     STOKES_LAUNCH_CHECK();
-    k_fmm_m2l<kStressSeries, false><<<dim3(m, kStressSeries), 256, 0, s>>>(
+    k_fmm_m2l<kStressSeries, false, false><<<dim3(m, kStressSeries), 256, 0, s>>>(
         F.cr.p + 3 * static_cast<size_t>(F.nwall), F.smp.p, F.sfar_ptr.p, F.sfar_idx.p, F.bin.p,
-        F.slc.p);
+        nullptr, nullptr, F.slc.p);
     STOKES_LAUNCH_CHECK();
     k_fmm_l2p_strain<<<cdiv_int(nbp, kBlk), kBlk, 0, s>>>(g.x.p, g.y.p, F.cr.p, F.slc.p, g.no, n,
                                                          F.nwall, nbp, out);
