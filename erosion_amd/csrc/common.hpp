@@ -227,11 +227,14 @@ struct TileFmm {
 
 // Per-solve GMRES state kept on the device.
 struct GmresState {
-    double bnrm, r0nrm, prod, rho, snormw, vnrm, inv_snormw;
-    int k;          // iterations done
-    int done;       // 1 converged, 2 stopped (maxl / breakdown)
-    int reorth;     // unused (every iteration runs two Gram-Schmidt passes)
-    int nreorth;    // iterations with a second Gram-Schmidt pass
+    double bnrm, r0nrm, prod, rho;
+    double snormw, inv_snormw;  // ||w'|| of the last iteration (Gram-Schmidt) and 1/||w'||
+    double vnrm;
+    double rinv, gk;            // 1/r and g_j of the delayed second pass
+    int k;          // Arnoldi columns completed
+    int done;       // 1 converged, 2 stopped (maxl), 3 breakdown
+    int reorth;     // unused
+    int nreorth;    // unused (every vector gets its second pass)
 };
 
 // GMRES iteration chunks captured as HIP graphs.  `sig` lists every value the
@@ -327,9 +330,10 @@ struct Context {
     int npad = 0;
     int maxl_alloc = 0;
     DevBuf<double> V;       // (maxl+1) x npad
-    DevBuf<double> H;       // (maxl+1) x maxl, column-major
+    DevBuf<double> H;       // (maxl+1) x maxl, column-major: the rotated (R) Hessenberg
+    DevBuf<double> Hraw;    // the same, Arnoldi columns before rotation
     DevBuf<double> q;       // Givens (c, s) pairs
-    DevBuf<double> hbuf, hpart, wpart, npart, ybuf, hcur, bpart;
+    DevBuf<double> hbuf, hpart, hpart2, wpart, npart, ybuf, hcur, bpart;
     DevBuf<double> rhs, xvec, rvec;
     DevBuf<GmresState> st;
     GmresState* h_st = nullptr;  // pinned mirror

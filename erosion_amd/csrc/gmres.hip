@@ -128,56 +128,69 @@ __global__ void k_scale_row(double* __restrict__ v, size_t n, const GmresState* 
         v[e] *= s;
 }
 
-// Partial dots of rows [r0, r0+kRows) of V with w over one chunk:
-// hpart[r][chunk]; k_update sums them over the chunks.
-__global__ __launch_bounds__(kBlk) void k_dots(const double* __restrict__ V, size_t npad,
-                                               const double* __restrict__ w, int nrows,
-                                               size_t chunk, int nchunk,
-                                               const GmresState* __restrict__ st,
-                                               double* __restrict__ hpart) {
+// Gram-Schmidt with delayed reorthogonalisation (DCGS2: Bielich, Langou,
+// Thomas, Swirydowicz, Yamazaki and Boman 2022), two passes over the Krylov
+// basis per iteration instead of CGS2's four.  Iteration j finds V[0..j-1]
+// final (orthonormal Q) and V[j] = u, the previous iteration's vector after
+// one projection, normalised.  With w = A u in V[j+1], one pass forms
+//   a = Q^T u, alpha = u.u, b = Q^T w, beta = u.w, omega = w.w;
+// then r = sqrt(alpha - a.a), q_j = (u - Q a)/r (u's second pass), the
+// Arnoldi column j-1 is completed with a (its last vector was u, now q_j),
+// and, with g = [b; (beta - a.b)/r] = [Q q_j]^T w,
+//   A q_j = (w - [Q q_j] H a)/r  gives column j: (g - H a)/r  (provisional),
+//   w' = (w - [Q q_j] g)/r,  ||w'|| = sqrt(omega - g.g)/r,
+// so the second pass writes q_j into V[j] and w'/||w'|| into V[j+1].  Column
+// j is completed, and its Givens rotation formed, in iteration j + 1: the
+// stop test runs one iteration late (one operator application more per
+// solve).  H_raw keeps the Arnoldi columns, H their rotated (R) form.
+
+// Partial dots of rows [r0, r0+kRows) of V with u and with w over one chunk:
+// hu[r][chunk], hw[r][chunk].
+__global__ __launch_bounds__(kBlk) void k_dots2(const double* __restrict__ V, size_t npad,
+                                                const double* __restrict__ u,
+                                                const double* __restrict__ w, int nrows,
+                                                size_t chunk, int nchunk,
+                                                const GmresState* __restrict__ st,
+                                                double* __restrict__ hu, double* __restrict__ hw) {
     __shared__ double sh[kBlk];
     if (st->done) return;
     const int r0 = blockIdx.x * kRows;
     const int c = blockIdx.y;
     const size_t e0 = static_cast<size_t>(c) * chunk;
     const size_t e1 = std::min(npad, e0 + chunk);
-    double acc[kRows];
+    double au[kRows], aw[kRows];
 #pragma unroll
-    for (int r = 0; r < kRows; ++r) acc[r] = 0.0;
+    for (int r = 0; r < kRows; ++r) au[r] = aw[r] = 0.0;
     for (size_t e = e0 + 2 * threadIdx.x; e < e1; e += 2 * kBlk) {
+        const double2 uv = *reinterpret_cast<const double2*>(u + e);
         const double2 wv = *reinterpret_cast<const double2*>(w + e);
 #pragma unroll
         for (int r = 0; r < kRows; ++r) {
             if (r0 + r < nrows) {
                 const double2 vv =
                     *reinterpret_cast<const double2*>(V + static_cast<size_t>(r0 + r) * npad + e);
-                acc[r] = __builtin_fma(vv.x, wv.x, __builtin_fma(vv.y, wv.y, acc[r]));
+                au[r] = __builtin_fma(vv.x, uv.x, __builtin_fma(vv.y, uv.y, au[r]));
+                aw[r] = __builtin_fma(vv.x, wv.x, __builtin_fma(vv.y, wv.y, aw[r]));
             }
         }
     }
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
-        double s = block_sum(acc[r], sh);
-        if (threadIdx.x == 0 && r0 + r < nrows) hpart[static_cast<size_t>(r0 + r) * nchunk + c] = s;
+        const double su = block_sum(au[r], sh);
+        const double sw = block_sum(aw[r], sh);
+        if (threadIdx.x == 0 && r0 + r < nrows) {
+            hu[static_cast<size_t>(r0 + r) * nchunk + c] = su;
+            hw[static_cast<size_t>(r0 + r) * nchunk + c] = sw;
+        }
     }
 }
 
-// wpart[split][e] = sum_{rows of split} h_r V[r][e].
-// Gram-Schmidt passes (mode 1, 2): h_r is the sum of the chunk partials of
-// k_dots, formed by every block for its own rows, 256 rows at a time in LDS;
-// the blocks of the first element range also store h (hcur) and add it to
-// the Hessenberg column (hsum: pass 1 stores, pass 2 adds).  Final solution
-// (mode 0): h is given.
+// wpart[split][e] = sum_{rows of split} h_r V[r][e] for the final solution
+// x += V y (h = y).
 __global__ __launch_bounds__(kBlk) void k_update(const double* __restrict__ V, size_t npad,
-                                                 const double* __restrict__ h,
-                                                 const double* __restrict__ hpart, int nchunk,
-                                                 int nrows, int rows_per_split,
-                                                 const GmresState* __restrict__ st, int mode,
-                                                 double* __restrict__ hcur,
-                                                 double* __restrict__ hsum,
-                                                 double* __restrict__ wpart) {
+                                                 const double* __restrict__ h, int nrows,
+                                                 int rows_per_split, double* __restrict__ wpart) {
     __shared__ double hs[kBlk];
-    if (mode != 0 && st->done) return;
     const size_t e = (static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x) * 2;
     const bool valid = e < npad;
     const int s = blockIdx.y;
@@ -186,54 +199,10 @@ __global__ __launch_bounds__(kBlk) void k_update(const double* __restrict__ V, s
     for (int t0 = ra; t0 < rb; t0 += kBlk) {
         const int tn = min(kBlk, rb - t0);
         __syncthreads();
-        if (static_cast<int>(threadIdx.x) < tn) {
-            const int r = t0 + threadIdx.x;
-            double v;
-            if (mode == 0) {
-                v = h[r];
-            } else {
-                // the chunk partials in order, loaded 16 at a time (indices
-                // clamped, values past nchunk replaced by 0) so the sum
-                // waits on L2 once per 16 rather than once per partial
-                v = 0.0;
-                const double* hp = hpart + static_cast<size_t>(r) * nchunk;
-                for (int c0 = 0; c0 < nchunk; c0 += 16) {
-                    double t[16];
-#pragma unroll
-                    for (int q = 0; q < 16; ++q) {
-                        const double x = hp[min(c0 + q, nchunk - 1)];
-                        t[q] = c0 + q < nchunk ? x : 0.0;
-                    }
-#pragma unroll
-                    for (int q = 0; q < 16; ++q) v += t[q];
-                }
-                if (blockIdx.x == 0) {
-                    hcur[r] = v;
-                    hsum[r] = mode == 1 ? v : hsum[r] + v;
-                }
-            }
-            hs[threadIdx.x] = v;
-        }
+        if (static_cast<int>(threadIdx.x) < tn) hs[threadIdx.x] = h[t0 + threadIdx.x];
         __syncthreads();
         if (!valid) continue;
-        int q = 0;
-        for (; q + 3 < tn; q += 4) {
-            const int r = t0 + q;
-            const double h0 = hs[q], h1 = hs[q + 1], h2 = hs[q + 2], h3 = hs[q + 3];
-            const double2 v0 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r) * npad + e);
-            const double2 v1 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r + 1) * npad + e);
-            const double2 v2 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r + 2) * npad + e);
-            const double2 v3 = *reinterpret_cast<const double2*>(V + static_cast<size_t>(r + 3) * npad + e);
-            ax = __builtin_fma(h0, v0.x, ax);
-            ay = __builtin_fma(h0, v0.y, ay);
-            ax = __builtin_fma(h1, v1.x, ax);
-            ay = __builtin_fma(h1, v1.y, ay);
-            ax = __builtin_fma(h2, v2.x, ax);
-            ay = __builtin_fma(h2, v2.y, ay);
-            ax = __builtin_fma(h3, v3.x, ax);
-            ay = __builtin_fma(h3, v3.y, ay);
-        }
-        for (; q < tn; ++q) {
+        for (int q = 0; q < tn; ++q) {
             const double hr = hs[q];
             const double2 v0 =
                 *reinterpret_cast<const double2*>(V + static_cast<size_t>(t0 + q) * npad + e);
@@ -245,76 +214,114 @@ __global__ __launch_bounds__(kBlk) void k_update(const double* __restrict__ V, s
         *reinterpret_cast<double2*>(wpart + static_cast<size_t>(s) * npad + e) = make_double2(ax, ay);
 }
 
-// w += sign * sum_s wpart[s]; except after the first pass also the per-block ||w||^2
-__global__ __launch_bounds__(kBlk) void k_combine(double* __restrict__ w, size_t npad,
-                                                  const double* __restrict__ wpart, int nsplit,
-                                                  double sign, const GmresState* __restrict__ st,
-                                                  int mode, double* __restrict__ npart) {
-    __shared__ double sh[kBlk];
-    if (mode != 0 && st->done) return;
+// The same over two coefficient sets at once (one read of V): the second
+// pass of DCGS2, wa[split][e] = sum a_r V[r][e], wb[split][e] = sum b_r V[r][e].
+__global__ __launch_bounds__(kBlk) void k_update2(const double* __restrict__ V, size_t npad,
+                                                  const double* __restrict__ ca,
+                                                  const double* __restrict__ cb, int nrows,
+                                                  int rows_per_split,
+                                                  const GmresState* __restrict__ st,
+                                                  double* __restrict__ wa, double* __restrict__ wb) {
+    __shared__ double ha[kBlk], hb[kBlk];
+    if (st->done) return;
     const size_t e = (static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x) * 2;
-    double acc = 0.0;
-    if (e < npad) {
-        double2 wv = *reinterpret_cast<const double2*>(w + e);
-        // the split partials in order, loaded 8 at a time (as in k_update)
-        double sx = 0.0, sy = 0.0;
-        for (int s0 = 0; s0 < nsplit; s0 += 8) {
-            double2 t[8];
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const double2 p = *reinterpret_cast<const double2*>(
-                    wpart + static_cast<size_t>(min(s0 + q, nsplit - 1)) * npad + e);
-                t[q] = s0 + q < nsplit ? p : make_double2(0.0, 0.0);
-            }
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                sx += t[q].x;
-                sy += t[q].y;
-            }
+    const bool valid = e < npad;
+    const int s = blockIdx.y;
+    const int ra = s * rows_per_split, rb = min(nrows, ra + rows_per_split);
+    double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
+    for (int t0 = ra; t0 < rb; t0 += kBlk) {
+        const int tn = min(kBlk, rb - t0);
+        __syncthreads();
+        if (static_cast<int>(threadIdx.x) < tn) {
+            ha[threadIdx.x] = ca[t0 + threadIdx.x];
+            hb[threadIdx.x] = cb[t0 + threadIdx.x];
         }
-        wv.x = __builtin_fma(sign, sx, wv.x);
-        wv.y = __builtin_fma(sign, sy, wv.y);
-        *reinterpret_cast<double2*>(w + e) = wv;
-        acc = wv.x * wv.x + wv.y * wv.y;
+        __syncthreads();
+        if (!valid) continue;
+#pragma unroll 4
+        for (int q = 0; q < tn; ++q) {
+            const double2 v0 =
+                *reinterpret_cast<const double2*>(V + static_cast<size_t>(t0 + q) * npad + e);
+            ax = __builtin_fma(ha[q], v0.x, ax);
+            ay = __builtin_fma(ha[q], v0.y, ay);
+            bx = __builtin_fma(hb[q], v0.x, bx);
+            by = __builtin_fma(hb[q], v0.y, by);
+        }
     }
-    if (mode == 1) return;  // block-uniform
-    double s = block_sum(acc, sh);
-    if (threadIdx.x == 0) npart[blockIdx.x] = s;
+    if (valid) {
+        *reinterpret_cast<double2*>(wa + static_cast<size_t>(s) * npad + e) = make_double2(ax, ay);
+        *reinterpret_cast<double2*>(wb + static_cast<size_t>(s) * npad + e) = make_double2(bx, by);
+    }
 }
 
+// w += sum_s wpart[s] (the final solution update)
+__global__ __launch_bounds__(kBlk) void k_combine(double* __restrict__ w, size_t npad,
+                                                  const double* __restrict__ wpart, int nsplit) {
+    const size_t e = (static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x) * 2;
+    if (e >= npad) return;
+    double2 wv = *reinterpret_cast<const double2*>(w + e);
+    // the split partials in order, loaded 8 at a time
+    double sx = 0.0, sy = 0.0;
+    for (int s0 = 0; s0 < nsplit; s0 += 8) {
+        double2 t[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const double2 p = *reinterpret_cast<const double2*>(
+                wpart + static_cast<size_t>(min(s0 + q, nsplit - 1)) * npad + e);
+            t[q] = s0 + q < nsplit ? p : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            sx += t[q].x;
+            sy += t[q].y;
+        }
+    }
+    wv.x += sx;
+    wv.y += sy;
+    *reinterpret_cast<double2*>(w + e) = wv;
+}
 
-// Hessenberg column k, Givens rotations (DHEQR, IJOB>1), stop test.
-// The k stored rotations act on (col[i], col[i+1]) in order; the value
-// carried into row i obeys carry_{i+1} = s_i carry_i + c_i h_{i+1}, an affine
-// recurrence evaluated with a block-wide scan over segments.
-constexpr int kGivT = 1024;
-__global__ __launch_bounds__(kGivT) void k_givens(double* __restrict__ H, int ldh,
-                                                  double* __restrict__ q,
-                                                  const double* __restrict__ hsum,
-                                                  const double* __restrict__ npart, int nb,
-                                                  GmresState* __restrict__ st, double tol,
-                                                  int maxl) {
-    __shared__ double sA[kGivT], sB[kGivT];
+// DCGS2's second pass, final step: with sa = sum_s wa[s], sb = sum_s wb[s],
+//   q = (u - sa) / r  into u's row,  w' = (w - sb - q g_j) / r / ||w'|| into w's row.
+__global__ __launch_bounds__(kBlk) void k_combine2(double* __restrict__ u, double* __restrict__ w,
+                                                   size_t npad, const double* __restrict__ wa,
+                                                   const double* __restrict__ wb, int nsplit,
+                                                   const GmresState* __restrict__ st) {
     if (st->done) return;
+    const size_t e = (static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x) * 2;
+    if (e >= npad) return;
+    double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
+    for (int s = 0; s < nsplit; ++s) {
+        const double2 p = *reinterpret_cast<const double2*>(wa + static_cast<size_t>(s) * npad + e);
+        const double2 t = *reinterpret_cast<const double2*>(wb + static_cast<size_t>(s) * npad + e);
+        ax += p.x;
+        ay += p.y;
+        bx += t.x;
+        by += t.y;
+    }
+    const double rinv = st->rinv, gk = st->gk, sc = st->rinv * st->inv_snormw;
+    const double2 uv = *reinterpret_cast<const double2*>(u + e);
+    const double2 wv = *reinterpret_cast<const double2*>(w + e);
+    const double qx = (uv.x - ax) * rinv, qy = (uv.y - ay) * rinv;
+    *reinterpret_cast<double2*>(u + e) = make_double2(qx, qy);
+    *reinterpret_cast<double2*>(w + e) =
+        make_double2((wv.x - bx - qx * gk) * sc, (wv.y - by - qy * gk) * sc);
+}
+
+// Givens update of Hessenberg column k = st->k (DHEQR, IJOB>1) and the stop
+// test: hsum[0..k] the column, sub its subdiagonal entry.  The k stored
+// rotations act on (col[i], col[i+1]) in order; the value carried into row i
+// obeys carry_{i+1} = s_i carry_i + c_i h_{i+1}, an affine recurrence
+// evaluated with a block-wide scan over segments.  Every thread of the
+// (kGivT-thread) block calls it.
+constexpr int kGivT = 1024;
+__device__ void givens_column(double* __restrict__ H, int ldh, double* __restrict__ q,
+                              const double* __restrict__ hsum, double sub,
+                              GmresState* __restrict__ st, double tol, int maxl, double* sA,
+                              double* sB) {
     const int k = st->k;
     double* __restrict__ col = H + static_cast<size_t>(k) * ldh;
     const int tid = threadIdx.x;
-    // ||w|| after the second Gram-Schmidt pass from the per-block partials
-    {
-        double p = 0.0;
-        for (int i = tid; i < nb; i += kGivT) p += npart[i];
-        for (int off = 32; off > 0; off >>= 1) p += __shfl_xor(p, off);
-        if ((tid & 63) == 0) sA[tid >> 6] = p;
-        __syncthreads();
-        if (tid == 0) {
-            double t = 0.0;
-            for (int w = 0; w < kGivT / 64; ++w) t += sA[w];
-            sB[0] = sqrt(t);
-        }
-        __syncthreads();
-    }
-    const double snormw = sB[0];
-    __syncthreads();
     // segment of rotations for this thread
     const int L = (k + kGivT - 1) / kGivT;
     const int i0 = min(k, tid * L), i1 = min(k, i0 + L);
@@ -353,40 +360,142 @@ __global__ __launch_bounds__(kGivT) void k_givens(double* __restrict__ H, int ld
     }
     // the thread owning the last rotation (or thread 0 when k == 0) finishes
     const bool last = (k == 0) ? (tid == 0) : (i0 < i1 && i1 == k);
-    if (!last) return;
-    const double t1 = carry, t2 = snormw;
-    st->snormw = snormw;
-    st->nreorth += 1;
-    double c, s;
-    if (t2 == 0.0) {
-        c = 1.0;
-        s = 0.0;
-    } else if (fabs(t2) >= fabs(t1)) {
-        const double t = t1 / t2;
-        s = -1.0 / sqrt(1.0 + t * t);
-        c = -s * t;
-    } else {
-        const double t = t2 / t1;
-        c = 1.0 / sqrt(1.0 + t * t);
-        s = -c * t;
+    if (last) {
+        const double t1 = carry, t2 = sub;
+        double c, s;
+        if (t2 == 0.0) {
+            c = 1.0;
+            s = 0.0;
+        } else if (fabs(t2) >= fabs(t1)) {
+            const double t = t1 / t2;
+            s = -1.0 / sqrt(1.0 + t * t);
+            c = -s * t;
+        } else {
+            const double t = t2 / t1;
+            c = 1.0 / sqrt(1.0 + t * t);
+            s = -c * t;
+        }
+        q[2 * k] = c;
+        q[2 * k + 1] = s;
+        const double rkk = c * t1 - s * t2;
+        col[k] = rkk;
+        col[k + 1] = t2;
+        st->k = k + 1;
+        if (rkk == 0.0) {  // DHEQR INFO = k: breakdown
+            st->done = 3;
+        } else {
+            st->prod = st->prod * s;
+            st->rho = fabs(st->prod * st->r0nrm);
+            if (st->rho / st->bnrm <= tol)
+                st->done = 1;
+            else if (k + 1 >= maxl)
+                st->done = 2;
+        }
     }
-    q[2 * k] = c;
-    q[2 * k + 1] = s;
-    const double rkk = c * t1 - s * t2;
-    col[k] = rkk;
-    col[k + 1] = t2;
-    st->k = k + 1;
-    if (rkk == 0.0) {  // DHEQR INFO = k: breakdown
-        st->done = 3;
+    __syncthreads();
+}
+
+// DCGS2's step between its passes, one block: the dots from k_dots2's
+// chunk partials (a, alpha into ca; b, beta, omega into cb), r, the completed
+// Arnoldi column j-1 (Givens, stop test), then unless `last` the provisional
+// column j (into H_raw), ||w'||, and the second pass's coefficients (ca = a,
+// cb = g) and scalars (st->rinv, st->gk, st->inv_snormw).
+__global__ __launch_bounds__(kGivT) void k_dcgs(const double* __restrict__ hu,
+                                                const double* __restrict__ hw, int nchunk, int j,
+                                                int last, double* __restrict__ Hraw,
+                                                double* __restrict__ H, int ldh,
+                                                double* __restrict__ q, double* __restrict__ ca,
+                                                double* __restrict__ cb,
+                                                GmresState* __restrict__ st, double tol, int maxl) {
+    __shared__ double sA[kGivT], sB[kGivT];
+    if (st->done) return;
+    const int tid = threadIdx.x;
+    // chunk partials in order, loaded 16 at a time (indices clamped, values
+    // past nchunk replaced by 0) so that each sum waits on L2 once per 16
+    auto chunk_sum = [&](const double* __restrict__ hp) {
+        double v = 0.0;
+        for (int c0 = 0; c0 < nchunk; c0 += 16) {
+            double t[16];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const double x = hp[min(c0 + e, nchunk - 1)];
+                t[e] = c0 + e < nchunk ? x : 0.0;
+            }
+#pragma unroll
+            for (int e = 0; e < 16; ++e) v += t[e];
+        }
+        return v;
+    };
+    for (int r = tid; r <= j; r += kGivT) ca[r] = chunk_sum(hu + static_cast<size_t>(r) * nchunk);
+    if (!last)
+        for (int r = tid; r <= j + 1; r += kGivT) cb[r] = chunk_sum(hw + static_cast<size_t>(r) * nchunk);
+    __syncthreads();
+    double paa = 0.0, pab = 0.0;
+    for (int r = tid; r < j; r += kGivT) {
+        paa = __builtin_fma(ca[r], ca[r], paa);
+        if (!last) pab = __builtin_fma(ca[r], cb[r], pab);
+    }
+    const double saa = block_sum(paa, sA);
+    const double sab = block_sum(pab, sA);
+    const double rr2 = ca[j] - saa;
+    // the last iteration left w' = 0 (its Krylov space is invariant): column
+    // j-1 completes with a zero subdiagonal, and its rotation ends the cycle
+    const bool invariant = j >= 1 && st->snormw == 0.0;
+    if (!(rr2 > 0.0) && !invariant) {  // u lies in the span of Q: breakdown
+        if (tid == 0) st->done = 3;
         return;
     }
-    st->prod = st->prod * s;
-    st->rho = fabs(st->prod * st->r0nrm);
-    st->inv_snormw = 1.0 / t2;
-    if (st->rho / st->bnrm <= tol)
-        st->done = 1;
-    else if (k + 1 >= maxl)
-        st->done = 2;
+    const double rn = rr2 > 0.0 ? sqrt(rr2) : 0.0;
+    if (j >= 1) {
+        // column j-1: its last vector was u, now r q_j + Q a
+        double* colr = Hraw + static_cast<size_t>(j - 1) * ldh;
+        const double est = st->snormw;
+        for (int i = tid; i < j; i += kGivT) colr[i] = __builtin_fma(est, ca[i], colr[i]);
+        if (tid == 0) colr[j] = est * rn;
+        __syncthreads();
+        givens_column(H, ldh, q, colr, est * rn, st, tol, maxl, sA, sB);
+        if (st->done) return;  // block-uniform after givens_column's barrier
+    }
+    if (last) return;
+    // provisional column j: (g - H a) / r, H upper Hessenberg (H[i][c], c >=
+    // i - 1); four neighbouring lanes share a row, each taking every fourth
+    // column, with loads 8 columns at a time
+    const double gj = (cb[j] - sab) / rn;
+    double pgg = 0.0;
+    double* colj = Hraw + static_cast<size_t>(j) * ldh;
+    for (int base = 0; base <= j; base += kGivT / 4) {  // uniform over the block
+        const int i = base + tid / 4, part = tid % 4;
+        double ha = 0.0;
+        if (i <= j) {
+            for (int c0 = max(0, i - 1) + part; c0 < j; c0 += 32) {
+                double hv[8], av[8];
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const int cc = min(c0 + 4 * t, j - 1);
+                    hv[t] = Hraw[static_cast<size_t>(cc) * ldh + i];
+                    av[t] = c0 + 4 * t < j ? ca[cc] : 0.0;
+                }
+#pragma unroll
+                for (int t = 0; t < 8; ++t) ha = __builtin_fma(hv[t], av[t], ha);
+            }
+        }
+        ha += __shfl_xor(ha, 1);
+        ha += __shfl_xor(ha, 2);
+        if (part == 0 && i <= j) {
+            const double gi = i < j ? cb[i] : gj;
+            colj[i] = (gi - ha) / rn;
+            pgg = __builtin_fma(gi, gi, pgg);
+        }
+    }
+    const double sgg = block_sum(pgg, sA);
+    if (tid == 0) {
+        const double ww = cb[j + 1] - sgg;
+        const double est = ww > 0.0 ? sqrt(ww) / rn : 0.0;
+        st->snormw = est;
+        st->inv_snormw = est > 0.0 ? 1.0 / est : 0.0;
+        st->rinv = 1.0 / rn;
+        st->gk = gj;
+    }
 }
 
 // DHELS: rotate (r0nrm, 0, ..., 0), back-substitute the k x k triangle.
@@ -433,24 +542,35 @@ Plan dots_plan(int nrows, size_t npad) {
 
 }  // namespace
 
-// One classical Gram-Schmidt pass over rows [0, nrows) of V against w; every
-// iteration runs two (CGS2), the second removing what rounding left of the
-// first, and the second leaves the per-block partial norms of w in npart.
-static void gs_pass(Context& c, double* w, int nrows, int pass, double* hcur) {
+// One DCGS2 step for iteration j: the dot pass, the step between the passes
+// and (unless `last`) the second pass.  last: no new column -- only column
+// j-1 is completed (after the final iteration of a cycle, V[j] = u).
+static void dcgs_step(Context& c, int j, bool last, int ldh, double tol, int maxl) {
     const size_t npad = static_cast<size_t>(c.npad);
     hipStream_t s = c.stream;
+    double* u = c.V.p + static_cast<size_t>(j) * npad;
+    double* w = last ? u : c.V.p + static_cast<size_t>(j + 1) * npad;
+    const int nrows = last ? j + 1 : j + 2;
     Plan p = dots_plan(nrows, npad);
-    k_dots<<<dim3(p.bx, p.nchunk), kBlk, 0, s>>>(c.V.p, npad, w, nrows, p.chunk, p.nchunk, c.st.p,
-                                                 c.hpart.p);
+    k_dots2<<<dim3(p.bx, p.nchunk), kBlk, 0, s>>>(c.V.p, npad, u, w, nrows, p.chunk, p.nchunk,
+                                                  c.st.p, c.hpart.p, c.hpart2.p);
     STOKES_LAUNCH_CHECK();
+    k_dcgs<<<1, kGivT, 0, s>>>(c.hpart.p, c.hpart2.p, p.nchunk, j, last ? 1 : 0, c.Hraw.p, c.H.p,
+                               ldh, c.q.p, c.hcur.p, c.hbuf.p, c.st.p, tol, maxl);
+    STOKES_LAUNCH_CHECK();
+    if (last) return;
     const int ebx = static_cast<int>(npad / (2 * kBlk));
-    int nsplit = std::min(nrows, std::max(1, cdiv_int(1024, ebx)));
-    int rps = cdiv_int(nrows, nsplit);
-    nsplit = cdiv_int(nrows, rps);
-    k_update<<<dim3(ebx, nsplit), kBlk, 0, s>>>(c.V.p, npad, nullptr, c.hpart.p, p.nchunk, nrows,
-                                                rps, c.st.p, pass, hcur, c.hbuf.p, c.wpart.p);
-    STOKES_LAUNCH_CHECK();
-    k_combine<<<ebx, kBlk, 0, s>>>(w, npad, c.wpart.p, nsplit, -1.0, c.st.p, pass, c.npart.p);
+    int nsplit = 0;
+    const size_t half = c.wpart.cap / 2;
+    if (j > 0) {
+        nsplit = std::min(j, std::max(1, cdiv_int(1024, ebx)));
+        const int rps = cdiv_int(j, nsplit);
+        nsplit = cdiv_int(j, rps);
+        k_update2<<<dim3(ebx, nsplit), kBlk, 0, s>>>(c.V.p, npad, c.hcur.p, c.hbuf.p, j, rps,
+                                                     c.st.p, c.wpart.p, c.wpart.p + half);
+        STOKES_LAUNCH_CHECK();
+    }
+    k_combine2<<<ebx, kBlk, 0, s>>>(u, w, npad, c.wpart.p, c.wpart.p + half, nsplit, c.st.p);
     STOKES_LAUNCH_CHECK();
 }
 
@@ -522,16 +642,20 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
     const int ldh = maxl + 1;
     c.V.ensure(static_cast<size_t>(maxl + 1) * npad);
     c.H.ensure(static_cast<size_t>(ldh) * static_cast<size_t>(maxl));
+    c.Hraw.ensure(static_cast<size_t>(ldh) * static_cast<size_t>(maxl));
     c.q.ensure(2 * static_cast<size_t>(maxl));
-    c.hbuf.ensure(static_cast<size_t>(maxl + 1 + kRows));
+    c.hbuf.ensure(static_cast<size_t>(maxl + 2 + kRows));
     c.ybuf.ensure(static_cast<size_t>(maxl + 1 + kRows));
     {
-        Plan p = dots_plan(maxl + 1, npad);
-        c.hpart.ensure(static_cast<size_t>(p.bx * kRows) * static_cast<size_t>(std::max(p.nchunk, 64)));
+        // dot partials of up to maxl + 2 rows, at most 64 chunks each
+        const size_t rows = static_cast<size_t>(cdiv_int(maxl + 2, kRows) * kRows);
+        c.hpart.ensure(rows * 64);
+        c.hpart2.ensure(rows * 64);
     }
     const int ebx = static_cast<int>(npad / (2 * kBlk));
     c.npart.ensure(static_cast<size_t>(std::max(ebx, 1024)));
-    c.wpart.ensure(static_cast<size_t>(std::max(1, cdiv_int(1024, ebx)) + 1) * npad);
+    // two halves of split partials (DCGS2's two coefficient sets)
+    c.wpart.ensure(2 * static_cast<size_t>(std::max(1, cdiv_int(1024, ebx)) + 1) * npad);
     c.rhs.ensure(npad);
     c.xvec.ensure(npad);
     c.rvec.ensure(npad);
@@ -543,8 +667,7 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
         STOKES_LAUNCH_CHECK();
     }
     if (!c.h_st) STOKES_HIP(hipHostMalloc(reinterpret_cast<void**>(&c.h_st), sizeof(GmresState)));
-    DevBuf<double>& hcur = c.hcur;
-    hcur.ensure(static_cast<size_t>(maxl + 1 + kRows));
+    c.hcur.ensure(static_cast<size_t>(maxl + 2 + kRows));
     DevBuf<double>& bpart = c.bpart;
     bpart.ensure(1024);
 
@@ -647,28 +770,20 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
                     const int kk = k + j;
                     double* vk = c.V.p + static_cast<size_t>(kk) * npad;
                     double* w = c.V.p + static_cast<size_t>(kk + 1) * npad;
+                    // V[kk] is normalised (v_0 before the loop, later ones
+                    // by the previous iteration's second pass)
                     if (pc) {
-                        // v_k = w_k / ||w_k|| happens in the preconditioner's
-                        // first read of it (v_0 is scaled before the loop)
-                        precond_apply_scaled(c, vk, c.pc_z.p, kk > 0 ? c.st.p : nullptr, w);
+                        precond_apply_scaled(c, vk, c.pc_z.p, nullptr, w);
                         apply_operator(c, c.pc_z.p, w, ibary, true);
                     } else {
                         apply_operator(c, vk, w, ibary);
                     }
                     if (c.time_kernels) c.t_gs.start(s);
-                    gs_pass(c, w, kk + 1, 1, hcur.p);
-                    gs_pass(c, w, kk + 1, 2, hcur.p);
-                    k_givens<<<1, kGivT, 0, s>>>(c.H.p, ldh, c.q.p, c.hbuf.p, c.npart.p, ebx,
-                                                 c.st.p, tol_now, maxl_cycle);
-                    STOKES_LAUNCH_CHECK();
+                    dcgs_step(c, kk, false, ldh, tol_now, maxl_cycle);
                     if (c.time_kernels) {
-                        // Krylov rows streamed by the dot and update kernels of both passes
-                        c.t_gs.stop(s, 4.0 * static_cast<double>(kk + 1) *
+                        // Krylov rows streamed by the dot and update passes
+                        c.t_gs.stop(s, 2.0 * static_cast<double>(kk + 1) *
                                            static_cast<double>(npad) * 8.0);
-                    }
-                    if (!pc) {
-                        k_scale_row<<<std::min(1024, ebx * 2), kBlk, 0, s>>>(w, npad, c.st.p, 0);
-                        STOKES_LAUNCH_CHECK();
                     }
                 }
             };
@@ -687,6 +802,12 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
                 break;
             }
         }
+        if (!c.h_st->done) {
+            // the cycle's last column is completed by one more dot pass
+            dcgs_step(c, k, true, ldh, tol_now, maxl_cycle);
+            STOKES_HIP(hipMemcpyAsync(c.h_st, c.st.p, sizeof(GmresState), hipMemcpyDeviceToHost, s));
+            STOKES_HIP(hipStreamSynchronize(s));
+        }
         c.ff_fp32_now = false;
         GmresState fin = *c.h_st;
         if (cycle == 0) c.last_r0_relres = fin.r0nrm / fin.bnrm;
@@ -701,15 +822,13 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
             int nsplit = std::min(iters, std::max(1, cdiv_int(1024, ebx)));
             int rps = cdiv_int(iters, nsplit);
             nsplit = cdiv_int(iters, rps);
-            k_update<<<dim3(ebx, nsplit), kBlk, 0, s>>>(c.V.p, npad, c.ybuf.p, nullptr, 0, iters,
-                                                        rps, c.st.p, 0, nullptr, nullptr,
+            k_update<<<dim3(ebx, nsplit), kBlk, 0, s>>>(c.V.p, npad, c.ybuf.p, iters, rps,
                                                         c.wpart.p);
             STOKES_LAUNCH_CHECK();
             if (pc) {
                 // u = V y, then x += P^{-1} u
                 STOKES_HIP(hipMemsetAsync(c.rvec.p, 0, npad * sizeof(double), s));
-                k_combine<<<ebx, kBlk, 0, s>>>(c.rvec.p, npad, c.wpart.p, nsplit, 1.0, c.st.p, 0,
-                                               c.npart.p);
+                k_combine<<<ebx, kBlk, 0, s>>>(c.rvec.p, npad, c.wpart.p, nsplit);
                 STOKES_LAUNCH_CHECK();
                 if (track_u) {
                     k_add<<<static_cast<int>(npad / kBlk), kBlk, 0, s>>>(c.uvec.p, c.rvec.p, npad);
@@ -719,8 +838,7 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
                 k_add<<<static_cast<int>(npad / kBlk), kBlk, 0, s>>>(c.xvec.p, c.pc_z.p, npad);
                 STOKES_LAUNCH_CHECK();
             } else {
-                k_combine<<<ebx, kBlk, 0, s>>>(c.xvec.p, npad, c.wpart.p, nsplit, 1.0, c.st.p, 0,
-                                               c.npart.p);
+                k_combine<<<ebx, kBlk, 0, s>>>(c.xvec.p, npad, c.wpart.p, nsplit);
                 STOKES_LAUNCH_CHECK();
             }
             status = fin.done == 1 ? 0 : 1;
@@ -741,7 +859,7 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
         double rr = 0.0;
         for (double v : part) rr += v;
         total_iters += iters;
-        nreorth += fin.nreorth;
+        nreorth += fin.k;  // every vector had its (delayed) second pass
         implicit_relres = fin.rho / fin.bnrm;
         true_relres = std::sqrt(rr) / fin.bnrm;
         ++cycles;
