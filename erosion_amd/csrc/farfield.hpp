@@ -60,14 +60,14 @@ __device__ __forceinline__ void ff_tile(const double2* __restrict__ sxy,
 // self pair) has rx = ry = 0, so its moment product w is exactly 0; its rho^2
 // is raised to 1e-100 (every other pair has rho^2 >= 1e-20 and is unchanged),
 // which keeps P, 1/P and 1/rho^4 <= 1e200 finite, and the pair adds 0.
-template <int TPT, bool DIAG>
+template <int TPT, bool DIAG, int NT = kTile>
 __device__ __forceinline__ void ff_tile_shared(const double2* __restrict__ sxy,
                                                const double2* __restrict__ sab,
                                                const double* __restrict__ sc, const double* xt,
                                                const double* yt, double* u, double* v) {
     static_assert(TPT == 2 || TPT == 4 || TPT == 8, "shared reciprocal over 2, 4 or 8 targets");
 #pragma unroll 4
-    for (int jj = 0; jj < kTile; ++jj) {
+    for (int jj = 0; jj < NT; ++jj) {
         const double2 p = sxy[jj];
         const double2 ab = sab[jj];
         const double cc = sc[jj];
@@ -154,14 +154,14 @@ __device__ __forceinline__ void dd_add_prod(double& hi, double& lo, double c, do
 // target's (sum, correction) through dd_add_prod.  DIAG: the tile holds some
 // of the lane's own targets; the zero-distance pair's rho^2 is raised to
 // 1e-100, so its reciprocal stays finite and its term (moment product 0) is 0.
-template <int TPT, bool DIAG>
+template <int TPT, bool DIAG, int NT = kTile>
 __device__ __forceinline__ void ff_tile_pairs_comp(const double2* __restrict__ sxy,
                                                    const double2* __restrict__ sab,
                                                    const double* __restrict__ sc,
                                                    const double* xt, const double* yt, double* u,
                                                    double* cu, double* v, double* cv) {
 #pragma unroll 2
-    for (int jj = 0; jj < kTile; ++jj) {
+    for (int jj = 0; jj < NT; ++jj) {
         const double2 p = sxy[jj];
         const double2 ab = sab[jj];
         const double cc = sc[jj];

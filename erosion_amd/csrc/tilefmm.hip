@@ -59,6 +59,7 @@ constexpr int kP = 32;                 // expansion terms
 constexpr double kSep = 1.0 / 3.0;     // far: R_s + R_t <= kSep |c_s - c_t|
 constexpr int kMaxBodyPts = 512;
 constexpr int kWallSeg = 64;           // wall points per cluster
+constexpr int kDTile = 128;            // sources per direct-sum block
 constexpr int kStressSeries = 5;       // boundary-value series of the shear stress
 
 inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
@@ -475,14 +476,14 @@ __global__ __launch_bounds__(128) void k_fmm_direct(
     const int* __restrict__ cl, const int* __restrict__ work, const int* __restrict__ tile_ptr,
     const int* __restrict__ tiles, int npts, double* __restrict__ part_u,
     double* __restrict__ part_v, double* __restrict__ part_cu, double* __restrict__ part_cv) {
-    __shared__ double2 sxy[kTile], sab[kTile];
-    __shared__ double sc[kTile];
+    __shared__ double2 sxy[kDTile], sab[kDTile];
+    __shared__ double sc[kDTile];
     const int c = work[2 * blockIdx.x], j = work[2 * blockIdx.x + 1];
     const int tid = threadIdx.x, bt = blockDim.x;
     const int off = cl[2 * c], len = cl[2 * c + 1];
     const int ti = tile_ptr[c] + j;
     const int ts = tiles[2 * ti], tl = tiles[2 * ti + 1];
-    for (int e = tid; e < kTile; e += bt) {
+    for (int e = tid; e < kDTile; e += bt) {
         if (e < tl) {
             const int k = ts + e;
             sxy[e] = make_double2(x[k], y[k]);
@@ -511,14 +512,14 @@ __global__ __launch_bounds__(128) void k_fmm_direct(
     const bool diag = ts < off + len && ts + tl > off;
     if constexpr (COMP) {
         if (diag)
-            ff_tile_pairs_comp<TPT, true>(sxy, sab, sc, xt, yt, u, cu, v, cv);
+            ff_tile_pairs_comp<TPT, true, kDTile>(sxy, sab, sc, xt, yt, u, cu, v, cv);
         else
-            ff_tile_pairs_comp<TPT, false>(sxy, sab, sc, xt, yt, u, cu, v, cv);
+            ff_tile_pairs_comp<TPT, false, kDTile>(sxy, sab, sc, xt, yt, u, cu, v, cv);
     } else {
         if (diag)
-            ff_tile_shared<TPT, true>(sxy, sab, sc, xt, yt, u, v);
+            ff_tile_shared<TPT, true, kDTile>(sxy, sab, sc, xt, yt, u, v);
         else
-            ff_tile_shared<TPT, false>(sxy, sab, sc, xt, yt, u, v);
+            ff_tile_shared<TPT, false, kDTile>(sxy, sab, sc, xt, yt, u, v);
     }
 #pragma unroll
     for (int q = 0; q < TPT; ++q) {
@@ -692,9 +693,9 @@ void tilefmm_prepare(Context& c) {
             int en = st + near[i].second;
             size_t j = i + 1;
             while (j < near.size() && near[j].first == en) en += near[j++].second;
-            for (int k0 = st; k0 < en; k0 += kTile) {
+            for (int k0 = st; k0 < en; k0 += kDTile) {
                 tiles.push_back(k0);
-                tiles.push_back(std::min(kTile, en - k0));
+                tiles.push_back(std::min(kDTile, en - k0));
             }
             i = j;
         }
