@@ -239,7 +239,8 @@ def main() -> int:
         stokes.set_precision(stokes.PRECISION_MIXED if mixed else stokes.PRECISION_FP64)
 
     def tol_misses():
-        return stokes.last_solve_info()["tol_misses_total"]
+        i = stokes.last_solve_info()
+        return np.array([i["tol_misses_total"], i["tol_misses_at_floor_total"]])
 
     def run_device(label, warmup, steps, pc=True, mixed=True, warm=2, learned=True,
                    graphs=False, timed_kernels=True):
@@ -270,6 +271,11 @@ def main() -> int:
         sync()
         elapsed = time.perf_counter() - t0
         misses = tol_misses() - misses0
+        miss_rec = None
+        if misses[0]:
+            miss_rec = stokes.last_solve_info()["latest_miss_record"]
+            log(f"[{label}] {misses[0]} solve(s) above the tolerance ({misses[1]} at the FP64 "
+                f"floor of their density); the latest: {miss_rec}")
         if graphs and timed_kernels:
             log(f"[{label}] graph replay: {stokes.graph_stats()}")
             stokes.set_graphs(False)
@@ -281,7 +287,7 @@ def main() -> int:
         configure(False, False)
         return {"elapsed": elapsed, "steps": steps, "timings": timings, "kt": kt,
                 "solve_info": solve_info, "true_relres_max": max(relres), "state": th,
-                "graphs": graphs, "tol_misses": misses}
+                "graphs": graphs, "tol_misses": misses, "miss_record": miss_rec}
 
     def run_plain(label, steps):
         """The reference's iteration through the C-ABI, theta-L on the host."""
@@ -309,6 +315,7 @@ def main() -> int:
         return {"elapsed": elapsed, "steps": steps, "timings": timings, "kt": kt,
                 "solve_info": solve_info, "true_relres_max": timings["true_relres_max"],
                 "state": th, "graphs": False, "tol_misses": tol_misses() - misses0,
+                "miss_record": None,
                 "first_density": np.asarray(th_init.density)}
 
     def roofline_of_run(r):
@@ -346,7 +353,12 @@ def main() -> int:
                 "ms_per_step": 1e3 * r["elapsed"] / r["steps"],
                 "gmres_iters_per_solve": r["timings"].get("gmres_iters", 0) / solves,
                 "true_relres_max": r["true_relres_max"],
-                "solves_above_tol": r["tol_misses"],
+                "solves_above_tol": int(r["tol_misses"][0]),
+                # of those, solves whose density's FP64 floor (the residual a
+                # half-ulp perturbation of it leaves) is at least half the
+                # residual reached, and the latest miss's record
+                "solves_above_tol_at_fp64_floor": int(r["tol_misses"][1]),
+                "latest_miss": r["miss_record"],
                 # operator applications (FP64 and FP32 far field); with graphs
                 # the kernel counts come from one extra step, scaled
                 "matvecs_per_s": mvps}
@@ -400,7 +412,7 @@ def main() -> int:
                               "gmres_iters_per_solve": r["timings"]["gmres_iters"] /
                               max(1, r["timings"]["solves"]),
                               "true_relres_max": r["true_relres_max"],
-                              "solves_above_tol": r["tol_misses"]}
+                              "solves_above_tol": int(r["tol_misses"][0])}
         ablation["sample"] = (f"each row: 1 untimed step, then {args.ablation_steps} timed steps "
                               f"from the same state (device stepper)")
     if sp:

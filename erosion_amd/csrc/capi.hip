@@ -293,6 +293,17 @@ int erosion_last_solve_info(double* info) {
         info[10] = static_cast<double>(c.tol_misses);
         info[11] = static_cast<double>(c.last_cycles);
         info[12] = c.last_r0_relres;
+        const Context::CycleRecord* recs[2] = {&c.last_rec, &c.miss_rec};
+        for (int r = 0; r < 2; ++r) {
+            double* o = info + 13 + r * (2 + 2 * stokes::kCycleRecord);
+            o[0] = static_cast<double>(recs[r]->stop);
+            for (int i = 0; i < stokes::kCycleRecord; ++i) {
+                o[1 + i] = recs[r]->true_relres[i];
+                o[1 + stokes::kCycleRecord + i] = recs[r]->iters[i];
+            }
+            o[1 + 2 * stokes::kCycleRecord] = recs[r]->floor_relres;
+        }
+        info[33] = static_cast<double>(c.tol_misses_at_floor);
         return 0;
     } catch (const std::exception& e) {
         return record_error("erosion_last_solve_info", e.what());

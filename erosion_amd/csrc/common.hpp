@@ -34,6 +34,8 @@ struct Error : std::runtime_error {
 
 constexpr double kPi = 3.14159265358979323846;
 constexpr double kTwoPi = 6.28318530717958647692;
+// GMRES cycles whose true residual and iteration count a solve record keeps
+constexpr int kCycleRecord = 4;
 
 // ------------------------------------------------------- device helpers
 // FP64 reciprocal: hardware estimate refined by two Newton steps.  Used for
@@ -386,6 +388,22 @@ struct Context {
     double last_r0_relres = -1.0;   // ||b - A x0|| / ||b|| of its initial guess
     int last_tol_miss = 0;          // 1: the last solve ended above the tolerance
     long long tol_misses = 0;       // such solves since the process started
+    // of those, the ones whose density's FP64 floor (CycleRecord::floor_relres)
+    // is at least half the residual reached
+    long long tol_misses_at_floor = 0;
+    // per-cycle record of a solve (at most kCycleRecord cycles): why it
+    // stopped, the true relative residual and the iterations of each cycle;
+    // `last` is the latest solve, `miss` the latest one above the tolerance
+    struct CycleRecord {
+        int stop = 0;  // 0 met the tolerance, 1 a restart did not halve the
+                       // true residual, 2 cycle limit, 3 maxl spent, 4 GMRES
+                       // status nonzero (maxl or breakdown inside the cycle)
+        double true_relres[kCycleRecord] = {};
+        double iters[kCycleRecord] = {};
+        // measured only above the tolerance: ||A d|| / ||b|| for a random-sign
+        // half-ulp perturbation d of the returned density (0 otherwise)
+        double floor_relres = 0.0;
+    } last_rec, miss_rec;
 };
 
 Context& ctx();

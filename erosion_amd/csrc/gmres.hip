@@ -81,6 +81,27 @@ __global__ __launch_bounds__(kBlk) void k_axpby_norm(double a, const double* __r
     if (threadIdx.x == 0) npart[blockIdx.x] = s;
 }
 
+// d[i] = +-ulp(x[i]) / 2 with a sign from a hash of i, zero in the padding:
+// the size of the rounding of a density to FP64, whose image A d bounds the
+// residual any FP64 density near x can reach
+__global__ __launch_bounds__(kBlk) void k_half_ulp(const double* __restrict__ x,
+                                                   double* __restrict__ d, int n, int npad) {
+    const int i = blockIdx.x * kBlk + threadIdx.x;
+    if (i >= npad) return;
+    double v = 0.0;
+    if (i < n) {
+        const double a = fabs(x[i]);
+        v = 0.5 * (nextafter(a, __builtin_inf()) - a);
+        // splitmix64 finaliser of the index: one pseudo-random sign per entry
+        unsigned long long h = static_cast<unsigned long long>(i) + 0x9E3779B97F4A7C15ull;
+        h = (h ^ (h >> 30)) * 0xBF58476D1CE4E5B9ull;
+        h = (h ^ (h >> 27)) * 0x94D049BB133111EBull;
+        h ^= h >> 31;
+        if (h & 1ull) v = -v;
+    }
+    d[i] = v;
+}
+
 // x += y
 __global__ __launch_bounds__(kBlk) void k_add(double* __restrict__ x, const double* __restrict__ y,
                                               size_t n) {
@@ -729,6 +750,9 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
     int status = 0, nreorth = 0, cycles = 0;
     double implicit_relres = 0.0, true_relres = 0.0;
     const bool mixed = c.precision != 0;
+    static_assert(kMaxCycles <= kCycleRecord, "the solve record keeps every cycle");
+    Context::CycleRecord rec;
+    double bnrm_last = 1.0;
     for (int cycle = 0;; ++cycle) {
         const int maxl_cycle = maxl - static_cast<int>(total_iters);
         c.ff_fp32_now = false;  // residuals are formed with the FP64 operator
@@ -862,20 +886,57 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
         nreorth += fin.k;  // every vector had its (delayed) second pass
         implicit_relres = fin.rho / fin.bnrm;
         true_relres = std::sqrt(rr) / fin.bnrm;
+        bnrm_last = fin.bnrm;
         ++cycles;
+        rec.true_relres[cycle] = true_relres;
+        rec.iters[cycle] = static_cast<double>(iters);
         // restart while the true residual misses the tolerance and the last
         // restart still reduced it
         const bool stalled = cycle > 0 && true_relres > 0.5 * prev_true;
         prev_true = true_relres;
-        if (status != 0 || true_relres <= tol || stalled || total_iters >= maxl ||
-            cycle + 1 >= kMaxCycles)
-            break;
+        rec.stop = true_relres <= tol     ? 0
+                   : status != 0          ? 4
+                   : stalled              ? 1
+                   : total_iters >= maxl  ? 3
+                   : cycle + 1 >= kMaxCycles ? 2
+                                          : -1;
+        if (rec.stop >= 0) break;
     }
     // a solve that ends above the tolerance is counted where callers see it
     // (erosion_last_solve_info), whatever its status
     c.last_tol_miss = true_relres > tol ? 1 : 0;
     c.tol_misses += c.last_tol_miss;
     c.last_cycles = cycles;
+    // a solve above the tolerance measures the FP64 floor of its density:
+    // ||A d|| / ||b|| for a half-ulp perturbation d of x (one FP64
+    // application to d, whose entries are tiny, so it is formed accurately)
+    rec.floor_relres = 0.0;
+    if (c.last_tol_miss) {
+        c.pc_z.ensure(npad);
+        k_half_ulp<<<static_cast<int>(npad / kBlk), kBlk, 0, s>>>(c.xvec.p, c.pc_z.p, N,
+                                                                  static_cast<int>(npad));
+        STOKES_LAUNCH_CHECK();
+        apply_operator(c, c.pc_z.p, c.rvec.p, ibary);
+        k_zero_tail<<<cdiv_int(static_cast<int>(npad) - N + 1, kBlk), kBlk, 0, s>>>(
+            c.rvec.p, N, static_cast<int>(npad));
+        STOKES_LAUNCH_CHECK();
+        k_axpby_norm<<<nb, kBlk, 0, s>>>(1.0, c.rvec.p, 0.0, nullptr, nullptr, npad, c.npart.p);
+        STOKES_LAUNCH_CHECK();
+        std::vector<double> part(static_cast<size_t>(nb));
+        STOKES_HIP(hipMemcpyAsync(part.data(), c.npart.p, sizeof(double) * nb,
+                                  hipMemcpyDeviceToHost, s));
+        STOKES_HIP(hipStreamSynchronize(s));
+        double dd = 0.0;
+        for (double v : part) dd += v;
+        rec.floor_relres = std::sqrt(dd) / bnrm_last;
+    }
+    c.last_rec = rec;
+    if (c.last_tol_miss) {
+        c.miss_rec = rec;
+        // rounding the density to FP64 alone leaves a residual of this size:
+        // the miss is at the floor of an FP64 density for this boundary
+        if (rec.floor_relres >= 0.5 * true_relres) ++c.tol_misses_at_floor;
+    }
     STOKES_HIP(hipMemcpyAsync(den_io, c.xvec.p, sizeof(double) * N,
                               on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     if (track_u && !pc)  // P = I: u is the density

@@ -139,14 +139,27 @@ def get_gmres_tol() -> float:
 
 
 def last_solve_info() -> dict:
-    info = np.zeros(16)
+    info = np.zeros(40)
     lib().erosion_last_solve_info(_p(info))
+
+    def record(o):
+        return {"stop": STOP_REASONS.get(int(info[o]), str(int(info[o]))),
+                "true_relres": [float(v) for v in info[o + 1:o + 5]],
+                "iters": [int(v) for v in info[o + 5:o + 9]],
+                "fp64_floor_relres": float(info[o + 9])}
+
     return {"iters": int(info[0]), "implicit_relres": float(info[1]),
             "true_relres": float(info[2]), "seconds": float(info[3]), "reorth": int(info[4]),
             "ff_targets_per_lane": int(info[5]), "ff_splits": int(info[6]),
             "preconditioner": int(info[7]), "fp32_iters": int(info[8]),
             "tol_miss": int(info[9]), "tol_misses_total": int(info[10]), "cycles": int(info[11]),
-            "r0_relres": float(info[12])}
+            "r0_relres": float(info[12]), "cycle_record": record(13),
+            "latest_miss_record": record(23), "tol_misses_at_floor_total": int(info[33])}
+
+
+# why a solve stopped (erosion_last_solve_info, info[13] and info[23])
+STOP_REASONS = {0: "met tolerance", 1: "restart did not halve the residual", 2: "cycle limit",
+                3: "maxl spent", 4: "GMRES status nonzero"}
 
 
 GUESS_DENSITY, GUESS_PRECONDITIONED = 0, 1

@@ -118,7 +118,18 @@ const char* erosion_last_error(void);
  * is formed with compensated sums (error-free transformations in the far
  * field and the assembly), and a solve whose true residual misses the
  * tolerance restarts from its density, aiming at half the tolerance.
- * `info` holds at least 13 doubles.  Returns 0. */
+ * info[13] says why the last solve stopped (0 it met the tolerance, 1 a
+ * restart did not halve the true residual, 2 the cycle limit of 4, 3 maxl
+ * iterations spent, 4 a cycle ended with GMRES status 1 or 2),
+ * info[14..17] the true relative residual after each of its cycles and
+ * info[18..21] each cycle's iterations (0 for cycles not run) and info[22],
+ * for a solve above the tolerance, the FP64 floor of its density: ||A d|| /
+ * ||b|| for a half-ulp perturbation d of the returned density (random signs),
+ * the residual that rounding the density to FP64 alone can leave (0 when the
+ * tolerance was met); info[23..32] hold the same 10 values for the latest
+ * solve that ended above the tolerance, and info[33] counts the solves above
+ * the tolerance (of info[10]) whose FP64 floor is at least half the residual
+ * they reached.  `info` holds at least 34 doubles.  Returns 0. */
 int erosion_last_solve_info(double* info);
 
 /* Initial guess of stokessolver_ (no counterpart in the reference, whose

@@ -195,6 +195,38 @@ def test_config5_eroded_state_solves_and_guesses():
     assert relerr(x1, x0) < 1e-7
 
 
+def test_guess_accuracy_ceiling_eroded_state():
+    """What any initial guess can gain on the eroded 400-body state, and where
+    the trained network stands.  u* is the converged preconditioned variable
+    (tol 1e-13); guesses u*(1 + eps g), g ~ N(0, 1) per entry, take GMRES from
+    the starting residual down to 1e-10.  Measured: zeros 214 iterations,
+    u* 6, eps = 1e-2: 190, eps = 1e-4: 174 -- GMRES reduces the residual by
+    about a decade per 21 iterations whatever the guess, so even a guess right
+    to 1e-4 in every entry saves under a fifth of the iterations.  The
+    network, trained on 100-body runs, is 70% off in the bodies' modes 0-1 of
+    u* on this state, and takes as many iterations as zeros.  Bars: the exact
+    guess <= 10 iterations; eps = 1e-4 saves less than 30%; eps = 1e-2 less
+    than 20%; the network needs no fewer iterations than the 1e-2 guess."""
+    from erosion_amd.learned import WEIGHTS, LearnedGuess
+    th, n, m, no, xx, yy = _state("bodies400_n256_eroded50")
+    rng = np.random.default_rng(0)
+    with solver(space=stokes.GUESS_PRECONDITIONED):
+        solve(n, m, no, xx, yy, tol=1e-13)
+        u = stokes.last_preconditioned(n, m, no).copy()
+        its = {"zeros": solve(n, m, no, xx, yy)[1], "exact": solve(n, m, no, xx, yy, den=u)[1]}
+        for eps in (1e-2, 1e-4):
+            g = u * (1.0 + eps * rng.standard_normal(u.size))
+            its[eps] = solve(n, m, no, xx, yy, den=g)[1]
+        g = LearnedGuess(WEIGHTS, nouter=no).predict(th, no, scale=True)
+        its["network"] = solve(n, m, no, xx, yy, den=g)[1]
+    print(f"eroded 400-body state, GMRES iterations by initial guess: {its}")
+    z = its["zeros"]
+    assert its["exact"] <= 10
+    assert its[1e-4] > 0.7 * z
+    assert its[1e-2] > 0.8 * z
+    assert its["network"] >= its[1e-2]
+
+
 def test_preconditioned_extrapolation_saves_iterations():
     """The device stepper's guesses, extrapolated from its last two
     solutions in the preconditioned variable, against zero guesses: four

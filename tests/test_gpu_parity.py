@@ -176,6 +176,41 @@ def test_maxl_exhaustion(golden):
     assert abs(implicit - _err) < 1e-8
 
 
+def test_solve_record_and_fp64_floor(golden):
+    """erosion_last_solve_info's cycle record.  A solve that meets its
+    tolerance says so and measures no floor.  A tolerance of 1e-17 cannot be
+    met by any FP64 density: the solve restarts until a restart stops halving
+    the true residual, is counted above the tolerance, and measures the FP64
+    floor of its density (the residual of a half-ulp perturbation of it),
+    which must be positive, far below 1e-12 for this O(1) density, and within
+    a factor 1000 of the residual reached."""
+    g = golden("tight8_n64")
+    n, m, no, xx, yy = _case(g)
+    stokes.set_preconditioner(stokes.PRECOND_CURVE_FOURIER)
+    try:
+        stokes.set_gmres_tol(1e-10)
+        stokes.stokessolver(n, m, no, 1, 1, 0, 2000, xx, yy)
+        ok = stokes.last_solve_info()
+        stokes.set_gmres_tol(1e-17)
+        stokes.stokessolver(n, m, no, 1, 1, 0, 2000, xx, yy)
+        miss = stokes.last_solve_info()
+    finally:
+        stokes.set_gmres_tol(1e-8)
+        stokes.set_preconditioner(stokes.PRECOND_NONE)
+    rec = ok["cycle_record"]
+    assert rec["stop"] == "met tolerance" and rec["fp64_floor_relres"] == 0.0
+    assert rec["true_relres"][ok["cycles"] - 1] == ok["true_relres"] <= 1e-10
+    assert sum(rec["iters"]) == ok["iters"]
+    rec = miss["cycle_record"]
+    print(f"tol 1e-17: {miss['cycles']} cycles, record {rec}")
+    assert miss["tol_miss"] == 1 and miss["latest_miss_record"] == rec
+    assert miss["tol_misses_total"] >= ok["tol_misses_total"] + 1
+    assert rec["stop"] != "met tolerance"
+    assert sum(rec["iters"]) == miss["iters"]
+    fl = rec["fp64_floor_relres"]
+    assert 0.0 < fl < 1e-12 and miss["true_relres"] < 1000 * fl
+
+
 def test_unsupported_combinations_raise(golden):
     g = golden("tight8_n64")
     n, m, no, xx, yy = _case(g)
