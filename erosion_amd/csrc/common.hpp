@@ -219,6 +219,16 @@ struct TileFmm {
     DevBuf<double> mlog;                // per cluster the log terms (c_F, c_Q) of its stokeslet
     DevBuf<double> logd;                // per far pair log(c_t - c_s) (complex)
     DevBuf<double> smp, slc;            // [m][5][32] complex
+    // groups of nearby clusters (parents, cluster indices nc .. nc + npar - 1
+    // in cr, mp and mlog): their children (CSR), the two-level far lists
+    // (a parent far from a target stands for all its children), and the
+    // binomials binom(k, n) of the multipole shift, [32][32]
+    int npar = 0;
+    DevBuf<int> par_ptr, par_idx;
+    DevBuf<int> far2_ptr, far2_idx;
+    DevBuf<double> logd2;
+    DevBuf<double> binm;
+    long long far2_pairs = 0;
     std::vector<int> h_snear_ptr, h_snear_idx;
     long long stress_direct_pairs = 0;  // (target point, source node) pairs of the near curves
     // the expansions run on a second stream (default priority) beside the

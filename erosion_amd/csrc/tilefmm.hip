@@ -61,6 +61,7 @@ constexpr int kMaxBodyPts = 512;
 constexpr int kWallSeg = 64;           // wall points per cluster
 constexpr int kDTile = 128;            // sources per direct-sum block
 constexpr int kStressSeries = 5;       // boundary-value series of the shear stress
+constexpr int kGroupMax = 8;           // most children of a parent cluster
 
 inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
 
@@ -207,6 +208,104 @@ __global__ __launch_bounds__(256) void k_fmm_p2m(const double* __restrict__ x,
         mlog[4 * c + 1] = -li / kTwoPi;
         mlog[4 * c + 2] = -lr / kTwoPi;  // c_Q = -conj(lambda) / (2 pi)
         mlog[4 * c + 3] = li / kTwoPi;
+    }
+}
+
+// Multipoles of the parents (groups of nearby clusters, cluster indices nc +
+// p) from their children's: grid npar, block 256.  First lane (child, n)
+// scales the child's order-n coefficients of both series and forms tau^n;
+// then lane (g, series, order k) sums children g and g + 4, and the four
+// partials meet in LDS.  A child s (centre c_s, radius
+// R_s) inside its parent (|c_s - c_P| + R_s <= R_P) moves to c_P by
+//   M'_k = sum_{n <= k} binom(k, n) (rho^n M_n) tau^(k-n),
+//   rho = R_s / R_P,  tau = (c_s - c_P) / R_P,
+// each term at most the child's largest coefficient (rho + |tau| <= 1).  Q
+// depends on the centre through E's factor conj(w - c): about c_P it is
+// Q_s + conj(c_s - c_P) F_s', and F_s' has the coefficients c_F (order 0,
+// the derivative of the log term) and -(n / R_s) F_(n-1); with the factor
+// and the rho^n scaling these are conj(c_s - c_P) c_F (n = 0) and
+// -n conj(tau) rho^(n-1) F_(n-1).  A log term c log(z - c_s) is
+// c log(z - c_P) - sum_n [c R_P tau^(n+1) / (n + 1)] R_P^n / (z - c_P)^(n+1):
+// the parent's log coefficient is the sum of its children's and the rest
+// enters M'_n.  binm[k][n] = binom(k, n).
+__global__ __launch_bounds__(256) void k_fmm_m2m(const double* __restrict__ cr,
+                                                const int* __restrict__ par_ptr,
+                                                const int* __restrict__ par_idx,
+                                                const double* __restrict__ binm, int nc,
+                                                double* __restrict__ mp,
+                                                double* __restrict__ mlog) {
+    static_assert(kGroupMax * kP == 256, "a lane per (child, order)");
+    __shared__ double sbin[kP][kP];
+    __shared__ cplx A[kGroupMax][2][kP];
+    __shared__ cplx tp[kGroupMax][kP + 1];
+    __shared__ cplx cl2[kGroupMax][2];   // log coefficients (c_F, c_Q) of each child
+    __shared__ cplx part[4][2][kP];
+    const int p = blockIdx.x, P = nc + p, t = threadIdx.x;
+    const int e0 = par_ptr[p], nch = min(par_ptr[p + 1] - e0, kGroupMax);
+    for (int e = t; e < kP * kP; e += 256) sbin[e / kP][e % kP] = binm[e];
+    const double cpx = cr[3 * P], cpy = cr[3 * P + 1], RP = cr[3 * P + 2];
+    const double iRP = 1.0 / RP;
+    // phase 1: lane (child ci, order n) scales the child's coefficients and
+    // forms tau^n
+    {
+        const int ci = t / kP, n = t % kP;
+        if (ci < nch) {
+            const int s = par_idx[e0 + ci];
+            const cplx delta = cmk(cr[3 * s] - cpx, cr[3 * s + 1] - cpy);
+            const cplx tau = cscale(delta, iRP);
+            const double rho = cr[3 * s + 2] * iRP;
+            const double* M = mp + static_cast<size_t>(s) * 2 * kP * 2;
+            // rho^(n-1) and rho^n by binary powering
+            double rn1 = 1.0;
+            {
+                double bb = rho;
+                for (int k = n - 1; k > 0; k >>= 1) {
+                    if (k & 1) rn1 *= bb;
+                    bb *= bb;
+                }
+            }
+            const double rn = n == 0 ? 1.0 : rn1 * rho;
+            A[ci][0][n] = cscale(cmk(M[2 * n], M[2 * n + 1]), rn);
+            cplx qn = cscale(cmk(M[2 * (kP + n)], M[2 * (kP + n) + 1]), rn);
+            if (n == 0) {
+                const cplx cF = cmk(mlog[4 * s], mlog[4 * s + 1]);
+                qn = cadd(qn, cmul(cmk(delta.r, -delta.i), cF));
+                cl2[ci][0] = cF;
+                cl2[ci][1] = cmk(mlog[4 * s + 2], mlog[4 * s + 3]);
+            } else {
+                const cplx f1 = cmk(M[2 * (n - 1)], M[2 * (n - 1) + 1]);
+                qn = csub(qn, cscale(cmul(cmk(tau.r, -tau.i), f1), static_cast<double>(n) * rn1));
+            }
+            A[ci][1][n] = qn;
+            tp[ci][n] = cpow_int(tau, n);
+            if (n == kP - 1) tp[ci][kP] = cpow_int(tau, kP);
+        }
+    }
+    __syncthreads();
+    // phase 2: lane (part g, series q, order k) sums children g, g + 4
+    {
+        const int g = t / (2 * kP), q = (t / kP) % 2, k = t % kP;
+        cplx acc = cmk(0.0, 0.0);
+        for (int ci = g; ci < nch; ci += 4) {
+            for (int n = 0; n <= k; ++n) cfma(acc, cscale(A[ci][q][n], sbin[k][n]), tp[ci][k - n]);
+            // the log term of the child about the parent's centre
+            acc = csub(acc, cscale(cmul(cl2[ci][q], tp[ci][k + 1]), RP / static_cast<double>(k + 1)));
+        }
+        part[g][q][k] = acc;
+    }
+    __syncthreads();
+    if (t < 2 * kP) {
+        const int q = t / kP, k = t % kP;
+        cplx acc = part[0][q][k];
+        for (int g = 1; g < 4; ++g) acc = cadd(acc, part[g][q][k]);
+        const size_t o = (static_cast<size_t>(P) * 2 + q) * kP + k;
+        mp[2 * o] = acc.r;
+        mp[2 * o + 1] = acc.i;
+    } else if (t < 2 * kP + 4) {
+        const int j = t - 2 * kP;
+        double sum = 0.0;
+        for (int ci = 0; ci < nch; ++ci) sum += (j & 1) ? cl2[ci][j / 2].i : cl2[ci][j / 2].r;
+        mlog[4 * P + j] = sum;
     }
 }
 
@@ -544,7 +643,11 @@ __global__ __launch_bounds__(128) void k_fmm_direct(
 // to part_u/part_v[0][p].  COMP: the tiles' (sum, correction) pairs and the
 // local value are added in double-double, the correction left in
 // part_cu/part_cv[0][p].
-template <bool COMP>
+// STAGED: the locals of the block's clusters (at most kL2pStage, which holds
+// for bodies of 64 points or more) are first copied to LDS, so that the
+// Horner loops read them there.
+constexpr int kL2pStage = 16;
+template <bool COMP, bool STAGED>
 __global__ __launch_bounds__(kBlk) void k_fmm_l2p(const double* __restrict__ x,
                                                   const double* __restrict__ y,
                                                   const double* __restrict__ cr,
@@ -554,13 +657,28 @@ __global__ __launch_bounds__(kBlk) void k_fmm_l2p(const double* __restrict__ x,
                                                   double* __restrict__ part_v,
                                                   double* __restrict__ part_cu,
                                                   double* __restrict__ part_cv) {
-    const int p = blockIdx.x * kBlk + threadIdx.x;
-    if (p >= npts) return;
+    constexpr int kLc = 3 * kP * 2;  // doubles of one cluster's locals
+    __shared__ double sL[STAGED ? kL2pStage * kLc : 1];
+    const int p0 = blockIdx.x * kBlk;
+    const int p = p0 + threadIdx.x;
     const int c = p < no ? p / kWallSeg : nwall + (p - no) / n;
+    const double* L;
+    if constexpr (STAGED) {
+        const int c0 = p0 < no ? p0 / kWallSeg : nwall + (p0 - no) / n;
+        const int pl = min(p0 + kBlk, npts) - 1;
+        const int c1 = pl < no ? pl / kWallSeg : nwall + (pl - no) / n;
+        const int cnt = min(c1 - c0 + 1, kL2pStage) * kLc;
+        const double* src = lc + static_cast<size_t>(c0) * kLc;
+        for (int e = threadIdx.x; e < cnt; e += kBlk) sL[e] = src[e];
+        __syncthreads();
+        L = sL + (c - c0) * kLc;
+    } else {
+        L = lc + static_cast<size_t>(c) * kLc;
+    }
+    if (p >= npts) return;
     const double dx = x[p] - cr[3 * c], dy = y[p] - cr[3 * c + 1];
     const double iR = 1.0 / cr[3 * c + 2];
     const cplx z = cmk(dx * iR, dy * iR);
-    const double* L = lc + static_cast<size_t>(c) * 3 * kP * 2;
     // Horner: F and its derivative, Q, the derivative of -G
     cplx fv = cmk(L[2 * (kP - 1)], L[2 * (kP - 1) + 1]), fd = cmk(0.0, 0.0);
     cplx qv = cmk(L[2 * (2 * kP - 1)], L[2 * (2 * kP - 1) + 1]);
@@ -590,9 +708,23 @@ __global__ __launch_bounds__(kBlk) void k_fmm_l2p(const double* __restrict__ x,
         part_cu[p] = lu;
         part_cv[p] = lv;
     } else {
-        for (int j = 0; j < nt; ++j) {
-            u += part_u[static_cast<size_t>(j) * npts + p];
-            v += part_v[static_cast<size_t>(j) * npts + p];
+        // the tiles' partials in order, loaded 8 at a time (one wait per
+        // batch instead of one per tile)
+        for (int j0 = 0; j0 < nt; j0 += 8) {
+            double tu[8], tv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const size_t o = static_cast<size_t>(min(j0 + q, nt - 1)) * npts + p;
+                tu[q] = part_u[o];
+                tv[q] = part_v[o];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                if (j0 + q < nt) {
+                    u += tu[q];
+                    v += tv[q];
+                }
+            }
         }
     }
     part_u[p] = u;
@@ -634,6 +766,40 @@ Cluster make_cluster(const Geometry& g, int start, int len, bool bbox, double cx
     return Cluster{cx, cy, std::max(std::sqrt(r2), 1e-300), start, len};
 }
 
+// Groups of at most kGroup clusters: the clusters idx[lo, hi) are halved by
+// their centres along the longer side of the centres' bounding box until a
+// part holds kGroup or fewer.
+constexpr int kGroup = kGroupMax;
+void bisect_groups(const std::vector<Cluster>& C, std::vector<int>& idx, int lo, int hi,
+                   std::vector<std::pair<int, int>>& out) {
+    if (hi - lo <= kGroup) {
+        out.emplace_back(lo, hi);
+        return;
+    }
+    double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+    for (int i = lo; i < hi; ++i) {
+        const Cluster& c = C[static_cast<size_t>(idx[static_cast<size_t>(i)])];
+        x0 = std::min(x0, c.cx);
+        x1 = std::max(x1, c.cx);
+        y0 = std::min(y0, c.cy);
+        y1 = std::max(y1, c.cy);
+    }
+    const bool byx = x1 - x0 >= y1 - y0;
+    const int mid = lo + (hi - lo) / 2;
+    std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi, [&](int a, int b) {
+        const Cluster &ca = C[static_cast<size_t>(a)], &cb = C[static_cast<size_t>(b)];
+        const double ka = byx ? ca.cx : ca.cy, kb = byx ? cb.cx : cb.cy;
+        return ka < kb || (ka == kb && a < b);
+    });
+    bisect_groups(C, idx, lo, mid, out);
+    bisect_groups(C, idx, mid, hi, out);
+}
+
+bool far_pair(const Cluster& S, const Cluster& T) {
+    const double dx = S.cx - T.cx, dy = S.cy - T.cy;
+    return S.r + T.r <= kSep * std::sqrt(dx * dx + dy * dy);
+}
+
 }  // namespace
 
 // Clusters, their lists and the expansion buffers of the current geometry
@@ -652,14 +818,81 @@ void tilefmm_prepare(Context& c) {
         C.push_back(make_cluster(g, no + b * n, n, false, g.h_cx[static_cast<size_t>(b)],
                                  g.h_cy[static_cast<size_t>(b)]));
     const int nc = static_cast<int>(C.size());
-    std::vector<double> cr(3 * static_cast<size_t>(nc));
+    // parents: groups of two or more nearby clusters, each enclosing its
+    // children (|c_s - c_P| + R_s <= R_P), centred in the box of the
+    // children's discs
+    std::vector<int> order(static_cast<size_t>(nc));
+    for (int i = 0; i < nc; ++i) order[static_cast<size_t>(i)] = i;
+    std::vector<std::pair<int, int>> groups;
+    bisect_groups(C, order, 0, nc, groups);
+    std::vector<Cluster> PC;                 // parent clusters
+    std::vector<int> gpar(groups.size(), -1);  // parent of each group (-1: none)
+    std::vector<int> par_ptr(1, 0), par_idx;
+    for (size_t gi = 0; gi < groups.size(); ++gi) {
+        const int lo = groups[gi].first, hi = groups[gi].second;
+        if (hi - lo < 2) continue;
+        double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+        for (int i = lo; i < hi; ++i) {
+            const Cluster& c = C[static_cast<size_t>(order[static_cast<size_t>(i)])];
+            x0 = std::min(x0, c.cx - c.r);
+            x1 = std::max(x1, c.cx + c.r);
+            y0 = std::min(y0, c.cy - c.r);
+            y1 = std::max(y1, c.cy + c.r);
+        }
+        Cluster P{0.5 * (x0 + x1), 0.5 * (y0 + y1), 0.0, 0, 0};
+        for (int i = lo; i < hi; ++i) {
+            const int ch = order[static_cast<size_t>(i)];
+            const Cluster& c = C[static_cast<size_t>(ch)];
+            P.r = std::max(P.r, std::hypot(c.cx - P.cx, c.cy - P.cy) + c.r);
+            par_idx.push_back(ch);
+        }
+        // enclosing with a margin of one part in 1e12: the shift's series
+        // needs rho + |tau| <= 1
+        P.r *= 1.0 + 1e-12;
+        par_ptr.push_back(static_cast<int>(par_idx.size()));
+        gpar[gi] = static_cast<int>(PC.size());
+        PC.push_back(P);
+    }
+    const int npar = static_cast<int>(PC.size());
+    std::vector<double> cr(3 * static_cast<size_t>(nc + npar));
     std::vector<int> cl(2 * static_cast<size_t>(nc));
+    for (int t = 0; t < nc + npar; ++t) {
+        const Cluster& c = t < nc ? C[static_cast<size_t>(t)] : PC[static_cast<size_t>(t - nc)];
+        cr[3 * static_cast<size_t>(t)] = c.cx;
+        cr[3 * static_cast<size_t>(t) + 1] = c.cy;
+        cr[3 * static_cast<size_t>(t) + 2] = c.r;
+        if (t < nc) {
+            cl[2 * static_cast<size_t>(t)] = c.start;
+            cl[2 * static_cast<size_t>(t) + 1] = c.len;
+        }
+    }
+    // two-level far lists: a far parent stands for all its children (a
+    // parent far from t implies every child far from t); the others are
+    // taken child by child
+    std::vector<int> far2_ptr(static_cast<size_t>(nc) + 1, 0), far2_idx;
+    std::vector<double> logd2;
+    auto push_far2 = [&](int t, int src, const Cluster& S) {
+        const Cluster& T = C[static_cast<size_t>(t)];
+        const double dx = S.cx - T.cx, dy = S.cy - T.cy;
+        far2_idx.push_back(src);
+        logd2.push_back(0.5 * std::log(dx * dx + dy * dy));
+        logd2.push_back(std::atan2(-dy, -dx));
+    };
     for (int t = 0; t < nc; ++t) {
-        cr[3 * static_cast<size_t>(t)] = C[static_cast<size_t>(t)].cx;
-        cr[3 * static_cast<size_t>(t) + 1] = C[static_cast<size_t>(t)].cy;
-        cr[3 * static_cast<size_t>(t) + 2] = C[static_cast<size_t>(t)].r;
-        cl[2 * static_cast<size_t>(t)] = C[static_cast<size_t>(t)].start;
-        cl[2 * static_cast<size_t>(t) + 1] = C[static_cast<size_t>(t)].len;
+        const Cluster& T = C[static_cast<size_t>(t)];
+        for (size_t gi = 0; gi < groups.size(); ++gi) {
+            const int pi = gpar[gi];
+            if (pi >= 0 && far_pair(PC[static_cast<size_t>(pi)], T)) {
+                push_far2(t, nc + pi, PC[static_cast<size_t>(pi)]);
+                continue;
+            }
+            for (int i = groups[gi].first; i < groups[gi].second; ++i) {
+                const int sidx = order[static_cast<size_t>(i)];
+                if (sidx != t && far_pair(C[static_cast<size_t>(sidx)], T))
+                    push_far2(t, sidx, C[static_cast<size_t>(sidx)]);
+            }
+        }
+        far2_ptr[static_cast<size_t>(t) + 1] = static_cast<int>(far2_idx.size());
     }
     std::vector<int> far_ptr(static_cast<size_t>(nc) + 1, 0), far_idx,
         tile_ptr(static_cast<size_t>(nc) + 1, 0), tiles, nb_ptr(static_cast<size_t>(nc) + 1, 0),
@@ -744,7 +977,25 @@ void tilefmm_prepare(Context& c) {
     up(F.nb_ptr, nb_ptr);
     up(F.nb_idx, nb_idx);
     up(F.logd, logd);
+    up(F.par_ptr, par_ptr);
+    up(F.par_idx, par_idx);
+    up(F.far2_ptr, far2_ptr);
+    up(F.far2_idx, far2_idx);
+    up(F.logd2, logd2);
     STOKES_HIP(hipStreamSynchronize(s));
+    if (!F.binm.p) {
+        // binom(k, n), k, n < 32 (exact in double: < 2^53)
+        std::vector<double> bm(static_cast<size_t>(kP) * kP, 0.0);
+        for (int k = 0; k < kP; ++k) {
+            bm[static_cast<size_t>(k) * kP] = 1.0;
+            for (int n = 1; n <= k; ++n)
+                bm[static_cast<size_t>(k) * kP + n] =
+                    bm[static_cast<size_t>(k - 1) * kP + n - 1] +
+                    (n < k ? bm[static_cast<size_t>(k - 1) * kP + n] : 0.0);
+        }
+        up(F.binm, bm);
+        STOKES_HIP(hipStreamSynchronize(s));
+    }
     if (!F.bin.p) {
         // binom(n + k, k) for n, k < 32 (exact in long double: < 2^63), in
         // the matrix cores' A-operand layout
@@ -763,12 +1014,13 @@ void tilefmm_prepare(Context& c) {
         up(F.bin, bin);
         STOKES_HIP(hipStreamSynchronize(s));
     }
-    F.mp.ensure(static_cast<size_t>(nc) * 2 * kP * 2);
-    F.mlog.ensure(static_cast<size_t>(nc) * 4);
+    F.mp.ensure(static_cast<size_t>(nc + npar) * 2 * kP * 2);
+    F.mlog.ensure(static_cast<size_t>(nc + npar) * 4);
     F.lc.ensure(static_cast<size_t>(nc) * 3 * kP * 2);
     F.smp.ensure(std::max(1, m) * static_cast<size_t>(kStressSeries) * kP * 2);
     F.slc.ensure(std::max(1, m) * static_cast<size_t>(kStressSeries) * kP * 2);
     F.nc = nc;
+    F.npar = npar;
     F.nwall = nwall;
     // one work item per (target cluster, source tile): bodies, then the wall
     std::vector<int> work;
@@ -790,6 +1042,7 @@ void tilefmm_prepare(Context& c) {
     STOKES_HIP(hipStreamSynchronize(s));
     F.max_tiles = max_tiles;
     F.far_pairs = static_cast<long long>(far_idx.size());
+    F.far2_pairs = static_cast<long long>(far2_idx.size());
     F.direct_pairs = direct - static_cast<long long>(g.npts);  // no self pairs
     F.version = g.version;
 }
@@ -863,10 +1116,22 @@ void tilefmm_apply(Context& c, const double* den, int* wall_nsplit, int* body_ns
         k_fmm_p2m<false><<<nc, 256, 0, F.side>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
                                                  F.cr.p, F.cl.p, stk, F.nwall, F.mp.p, F.mlog.p);
     STOKES_LAUNCH_CHECK();
+    // an operator application translates the parents of far groups (their
+    // multipoles formed from the children's); a true-residual evaluation
+    // keeps the one-level lists, whose multipoles carry error-free sums
+    const bool two_level = !compensated && F.npar > 0;
+    if (two_level) {
+        k_fmm_m2m<<<F.npar, 256, 0, F.side>>>(F.cr.p, F.par_ptr.p, F.par_idx.p, F.binm.p, nc,
+                                             F.mp.p, F.mlog.p);
+        STOKES_LAUNCH_CHECK();
+    }
     k_fmm_m2l<2, true, true><<<dim3(nc, 2), 256, 0, F.side>>>(
-        F.cr.p, F.mp.p, F.far_ptr.p, F.far_idx.p, F.bin.p, F.mlog.p, F.logd.p, F.lc.p);
+        F.cr.p, F.mp.p, two_level ? F.far2_ptr.p : F.far_ptr.p,
+        two_level ? F.far2_idx.p : F.far_idx.p, F.bin.p, F.mlog.p,
+        two_level ? F.logd2.p : F.logd.p, F.lc.p);
     STOKES_LAUNCH_CHECK();
-    if (timed) c.t_fmm_expand.stop(F.side, static_cast<double>(F.far_pairs));
+    if (timed)
+        c.t_fmm_expand.stop(F.side, static_cast<double>(two_level ? F.far2_pairs : F.far_pairs));
     STOKES_HIP(hipEventRecord(F.join, F.side));
     if (timed) c.t_fmm_direct.start(s);
     // bodies: 4 targets per lane above 128 points, else 2; wall segments: 2
@@ -878,16 +1143,26 @@ void tilefmm_apply(Context& c, const double* den, int* wall_nsplit, int* body_ns
     direct(c, 2, round32(cdiv_int(kWallSeg, 2)), F.work_body, F.work_wall);
     if (timed) c.t_fmm_direct.stop(s, static_cast<double>(F.direct_pairs));
     STOKES_HIP(hipStreamWaitEvent(s, F.join, 0));
-    if (compensated)
-        k_fmm_l2p<true><<<cdiv_int(np, kBlk), kBlk, 0, s>>>(g.x.p, g.y.p, F.cr.p, F.lc.p,
-                                                           F.tile_ptr.p, g.no, n, F.nwall, np,
-                                                           c.part_u.p, c.part_v.p, c.part_cu.p,
-                                                           c.part_cv.p);
-    else
-        k_fmm_l2p<false><<<cdiv_int(np, kBlk), kBlk, 0, s>>>(g.x.p, g.y.p, F.cr.p, F.lc.p,
-                                                            F.tile_ptr.p, g.no, n, F.nwall, np,
-                                                            c.part_u.p, c.part_v.p, nullptr,
-                                                            nullptr);
+    // a block of 256 points spans at most 5 wall segments and, for bodies of
+    // 64 points or more, 5 bodies: their locals fit the LDS stage
+    static_assert(kBlk / kWallSeg + 1 + kBlk / 64 + 1 <= kL2pStage, "L2P stage too small");
+    const bool staged = n >= 64;
+#define STOKES_L2P(COMP, ST, PCU, PCV)                                                        \
+    k_fmm_l2p<COMP, ST><<<cdiv_int(np, kBlk), kBlk, 0, s>>>(g.x.p, g.y.p, F.cr.p, F.lc.p,   \
+                                                            F.tile_ptr.p, g.no, n, F.nwall, \
+                                                            np, c.part_u.p, c.part_v.p, PCU, PCV)
+    if (compensated) {
+        if (staged)
+            STOKES_L2P(true, true, c.part_cu.p, c.part_cv.p);
+        else
+            STOKES_L2P(true, false, c.part_cu.p, c.part_cv.p);
+    } else {
+        if (staged)
+            STOKES_L2P(false, true, nullptr, nullptr);
+        else
+            STOKES_L2P(false, false, nullptr, nullptr);
+    }
+#undef STOKES_L2P
     STOKES_LAUNCH_CHECK();
     *wall_nsplit = 1;
     *body_nsplit = 1;

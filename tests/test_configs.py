@@ -258,6 +258,47 @@ def test_preconditioned_extrapolation_saves_iterations():
 
 @pytest.mark.parametrize("name", ["bodies100_n256", "bodies200_n512", "bodies60_n128_tight",
                                   "bodies400_n256_eroded50"])
+def test_cluster_operator_matches_direct(name):
+    """One operator application through the cluster far field (two levels:
+    groups of up to 8 clusters translated through their parents where far)
+    against the direct far field, on a random density and on a converged one
+    (whose rigid rotation densities are 10^3..10^6 times the right-hand
+    side).  The truncation is below (1/3)^32 of the terms, so the two agree
+    to rounding: 1e-13 of max |A x| for the random density.  On the converged
+    density the direct sums' own rounding of those large terms dominates
+    (residual 1.0e-9 ||b|| at the eroded state, where the cluster far field,
+    whose multipoles cancel a rigid motion inside their sums, leaves
+    1.1e-10; the one-level scheme measured the same): bars 1e-8 of max |A x|
+    between the two and 2e-10 ||b|| for the cluster far field's FP64
+    residual."""
+    th, n, m, no, xx, yy = _state(name)
+    N = stokes.nunk(n, m, no)
+    eta = np.random.default_rng(21).standard_normal(N)
+    with solver():
+        x, _, _ = solve(n, m, no, xx, yy)
+    fmm = stokes.farfield_shape_count() - 1
+    out = {}
+    try:
+        for label, shape in (("cluster", fmm), ("direct", 0)):
+            stokes.set_farfield_shape(shape)
+            out[label] = (stokes.matvec(n, m, no, xx, yy, eta), stokes.matvec(n, m, no, xx, yy, x))
+    finally:
+        stokes.set_farfield_shape(-1)
+    from erosion_amd.learned import wall_rhs
+    e_rand = relerr(out["cluster"][0], out["direct"][0])
+    bmax = np.max(np.abs(out["direct"][1]))
+    e_conv = float(np.max(np.abs(out["cluster"][1] - out["direct"][1])) / bmax)
+    b = wall_rhs(no, m, n)
+    res = {k: float(np.linalg.norm(b - v[1]) / np.linalg.norm(b)) for k, v in out.items()}
+    print(f"{name}: cluster vs direct operator, random density {e_rand:.2e}, converged density "
+          f"{e_conv:.2e} of max |A x|; FP64 residuals {res}")
+    assert e_rand <= 1e-13
+    assert e_conv <= 1e-8
+    assert res["cluster"] <= 2e-10
+
+
+@pytest.mark.parametrize("name", ["bodies100_n256", "bodies200_n512", "bodies60_n128_tight",
+                                  "bodies400_n256_eroded50"])
 def test_stress_far_bodies_by_expansions(name):
     """The shear stress with the far bodies' boundary-value integrals through
     the cluster expansions (the barycentric rule kept for the wall and the
