@@ -452,6 +452,9 @@ bool farfield_fp32_pays(const Context& c);
 // number of source splits whose partial sums it leaves in part_u/part_v
 bool tilefmm_available(const Context& c);
 void tilefmm_prepare(Context& c);
+// lists, part buffers and the side stream of the cluster far field, without
+// an application (set-up before a captured chunk)
+void tilefmm_ready(Context& c, bool compensated = false);
 void tilefmm_apply(Context& c, const double* den, int* wall_nsplit, int* body_nsplit,
                    bool compensated = false);
 // the near bodies of every cluster (CSR, body indices): point p belongs to

@@ -727,22 +727,26 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
     // operator and, should it still exceed the tolerance, GMRES restarts from
     // it aiming at half the tolerance (iterative refinement; at most
     // kMaxCycles cycles within maxl iterations in total).
-    if (c.use_graphs) {
-        // set-up work an operator application would otherwise launch inside
-        // a captured chunk (and then not repeat on a later replay)
-        prepare_operator(c, c.xvec.p, false);
-        if (c.precision != 0) prepare_operator(c, c.xvec.p, true);
-    }
     const bool accel = pc || c.precision != 0;
     // mixed precision switches to the FP32 far field only where that is
-    // faster than the FP64 far field in use (the cluster far field can
-    // be faster than the direct FP32 sum at 1e5 points)
+    // faster than the FP64 far field in use (the cluster far field can be
+    // faster than the direct FP32 sum at 1e5 points); the FP32 far field's
+    // set-up (source tiles, the stokeslet logarithm table) is made only when
+    // it will run -- its tuning is kept per size class, so this is decided
+    // without it after the first solve of a class
     bool fp32_pays = true;
+    if (c.precision == 1 || c.use_graphs) prepare_operator(c, c.xvec.p, false);
     if (c.precision == 1) {
-        prepare_operator(c, c.xvec.p, false);
-        prepare_operator(c, c.xvec.p, true);
         fp32_pays = farfield_fp32_pays(c);
+        if (fp32_pays) {
+            prepare_operator(c, c.xvec.p, true);
+            fp32_pays = farfield_fp32_pays(c);
+        }
     }
+    // set-up work an operator application would otherwise launch inside a
+    // captured chunk (and then not repeat on a later replay)
+    if (c.use_graphs && (c.precision == 2 || (c.precision == 1 && fp32_pays)))
+        prepare_operator(c, c.xvec.p, true);
     const double tol_cycle = accel ? kPcTolFrac * tol : tol;
     double* V0 = c.V.p;
     long long total_iters = 0, fp32_iters = 0;
@@ -781,7 +785,8 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
         double relres_now = cycle > 0 ? true_relres : 1.0;
         while (k < maxl_cycle) {
             // mixed precision decides per chunk: shorter chunks switch sooner
-            const int chunk = std::min(mixed ? kChunkIters / 2 : kChunkIters, maxl_cycle - k);
+            const int chunk =
+                std::min(mixed && fp32_pays ? kChunkIters / 2 : kChunkIters, maxl_cycle - k);
             // after a mixed cycle missed the tolerance, later cycles stay FP64
             c.ff_fp32_now = c.precision == 2 ||
                             (mixed && fp32_pays && cycle == 0 &&

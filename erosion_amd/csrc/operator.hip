@@ -1155,8 +1155,7 @@ void prepare_operator(Context& c, const double* x_dev, bool fp32) {
     const int choice = tune_farfield(c, fp32);
     if (choice == kFmmIndex) {
         // lists and part buffers of the cluster far field
-        int a = 0, b = 0;
-        tilefmm_apply(c, nullptr, &a, &b);
+        tilefmm_ready(c);
     } else {
         const FFPlan plan = ff_plan(c.g.npts, kFFShapes[choice], fp32);
         c.part_u.ensure(static_cast<size_t>(c.g.npts) * static_cast<size_t>(plan.nsplit));

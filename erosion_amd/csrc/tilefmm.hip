@@ -731,6 +731,22 @@ __global__ __launch_bounds__(kBlk) void k_fmm_l2p(const double* __restrict__ x,
     part_v[p] = v;
 }
 
+// log(c_t - c_s) of every far pair (t, s) of a CSR list, the branch the same
+// for both series of a pair: grid = target clusters, block 256.
+__global__ __launch_bounds__(256) void k_fmm_logd(const double* __restrict__ cr,
+                                                 const int* __restrict__ far_ptr,
+                                                 const int* __restrict__ far_idx,
+                                                 double* __restrict__ logd) {
+    const int t = blockIdx.x;
+    const double ctx = cr[3 * t], cty = cr[3 * t + 1];
+    for (int i = far_ptr[t] + static_cast<int>(threadIdx.x); i < far_ptr[t + 1]; i += 256) {
+        const int s = far_idx[i];
+        const double dx = cr[3 * s] - ctx, dy = cr[3 * s + 1] - cty;
+        logd[2 * static_cast<size_t>(i)] = 0.5 * log(dx * dx + dy * dy);
+        logd[2 * static_cast<size_t>(i) + 1] = atan2(-dy, -dx);
+    }
+}
+
 }  // namespace
 
 bool tilefmm_available(const Context& c) {
@@ -870,14 +886,7 @@ void tilefmm_prepare(Context& c) {
     // parent far from t implies every child far from t); the others are
     // taken child by child
     std::vector<int> far2_ptr(static_cast<size_t>(nc) + 1, 0), far2_idx;
-    std::vector<double> logd2;
-    auto push_far2 = [&](int t, int src, const Cluster& S) {
-        const Cluster& T = C[static_cast<size_t>(t)];
-        const double dx = S.cx - T.cx, dy = S.cy - T.cy;
-        far2_idx.push_back(src);
-        logd2.push_back(0.5 * std::log(dx * dx + dy * dy));
-        logd2.push_back(std::atan2(-dy, -dx));
-    };
+    auto push_far2 = [&](int, int src, const Cluster&) { far2_idx.push_back(src); };
     for (int t = 0; t < nc; ++t) {
         const Cluster& T = C[static_cast<size_t>(t)];
         for (size_t gi = 0; gi < groups.size(); ++gi) {
@@ -897,7 +906,6 @@ void tilefmm_prepare(Context& c) {
     std::vector<int> far_ptr(static_cast<size_t>(nc) + 1, 0), far_idx,
         tile_ptr(static_cast<size_t>(nc) + 1, 0), tiles, nb_ptr(static_cast<size_t>(nc) + 1, 0),
         nb_idx;
-    std::vector<double> logd;
     std::vector<std::pair<int, int>> near;  // (first point, count)
     long long direct = 0;
     for (int t = 0; t < nc; ++t) {
@@ -909,9 +917,6 @@ void tilefmm_prepare(Context& c) {
             const double dx = S.cx - T.cx, dy = S.cy - T.cy;
             if (s != t && S.r + T.r <= kSep * std::sqrt(dx * dx + dy * dy)) {
                 far_idx.push_back(s);
-                // log(c_t - c_s), the branch the same for both series of a pair
-                logd.push_back(0.5 * std::log(dx * dx + dy * dy));
-                logd.push_back(std::atan2(-dy, -dx));
             } else {
                 near.emplace_back(S.start, S.len);
                 nsrc += S.len;
@@ -976,12 +981,17 @@ void tilefmm_prepare(Context& c) {
     up(F.sfar_idx, sfar_idx);
     up(F.nb_ptr, nb_ptr);
     up(F.nb_idx, nb_idx);
-    up(F.logd, logd);
     up(F.par_ptr, par_ptr);
     up(F.par_idx, par_idx);
     up(F.far2_ptr, far2_ptr);
     up(F.far2_idx, far2_idx);
-    up(F.logd2, logd2);
+    // log(c_t - c_s) of every far pair of both lists, on the device
+    F.logd.ensure(2 * std::max<size_t>(far_idx.size(), 1));
+    F.logd2.ensure(2 * std::max<size_t>(far2_idx.size(), 1));
+    k_fmm_logd<<<nc, 256, 0, s>>>(F.cr.p, F.far_ptr.p, F.far_idx.p, F.logd.p);
+    STOKES_LAUNCH_CHECK();
+    k_fmm_logd<<<nc, 256, 0, s>>>(F.cr.p, F.far2_ptr.p, F.far2_idx.p, F.logd2.p);
+    STOKES_LAUNCH_CHECK();
     STOKES_HIP(hipStreamSynchronize(s));
     if (!F.binm.p) {
         // binom(k, n), k, n < 32 (exact in double: < 2^53)
@@ -1081,24 +1091,30 @@ void direct_launch(Context& c, int tpt, int bt, int w0, int count) {
 // the same expansions; k_assemble adds the near bodies' (tilefmm_near_bodies).
 // compensated: the direct sums and the final sum carry corrections
 // (part_cu/part_cv[0]) for a true-residual evaluation.
-void tilefmm_apply(Context& c, const double* den, int* wall_nsplit, int* body_nsplit,
-                   bool compensated) {
-    Geometry& g = c.g;
+void tilefmm_ready(Context& c, bool compensated) {
     TileFmm& F = c.fmm;
     tilefmm_prepare(c);
-    hipStream_t s = c.stream;
-    const int np = g.npts, nc = F.nc, n = g.n;
-    c.part_u.ensure(static_cast<size_t>(np) * F.max_tiles);
-    c.part_v.ensure(static_cast<size_t>(np) * F.max_tiles);
+    const size_t np = static_cast<size_t>(c.g.npts);
+    c.part_u.ensure(np * F.max_tiles);
+    c.part_v.ensure(np * F.max_tiles);
     if (compensated) {
-        c.part_cu.ensure(static_cast<size_t>(np) * F.max_tiles);
-        c.part_cv.ensure(static_cast<size_t>(np) * F.max_tiles);
+        c.part_cu.ensure(np * F.max_tiles);
+        c.part_cv.ensure(np * F.max_tiles);
     }
     if (!F.side) {
         STOKES_HIP(hipStreamCreateWithFlags(&F.side, hipStreamNonBlocking));
         STOKES_HIP(hipEventCreateWithFlags(&F.fork, hipEventDisableTiming));
         STOKES_HIP(hipEventCreateWithFlags(&F.join, hipEventDisableTiming));
     }
+}
+
+void tilefmm_apply(Context& c, const double* den, int* wall_nsplit, int* body_nsplit,
+                   bool compensated) {
+    Geometry& g = c.g;
+    TileFmm& F = c.fmm;
+    tilefmm_ready(c, compensated);
+    hipStream_t s = c.stream;
+    const int np = g.npts, nc = F.nc, n = g.n;
     // multipoles and translations (matrix cores) on the side stream while
     // the direct sums (vector ALU) run on the context's stream
     // timers count the operator's applications, not true-residual ones
