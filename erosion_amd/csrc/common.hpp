@@ -230,6 +230,15 @@ struct TileFmm {
     DevBuf<double> binm;
     long long far2_pairs = 0;
     std::vector<int> h_snear_ptr, h_snear_idx;
+    // the wall in the shear stress: per body 1 if it is close to the wall
+    // (barycentric rule over the whole wall), else the wall comes through
+    // the far segments' expansions and the near segments' direct sums
+    // (swn: CSR of near segment indices per body); scr holds the expansion
+    // sources' (cx, cy, R): bodies, then wall segments
+    std::vector<unsigned char> h_swall;
+    DevBuf<int> swn_ptr, swn_idx;
+    DevBuf<double> scr;
+    bool stress_wall = false;
     long long stress_direct_pairs = 0;  // (target point, source node) pairs of the near curves
     // the expansions run on a second stream (default priority) beside the
     // direct sums: fork and join events on the context's stream

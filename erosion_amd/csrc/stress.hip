@@ -705,25 +705,31 @@ static int near_curve_lists(Context& c, int tiles) {
     std::vector<unsigned char> mask(static_cast<size_t>(m) * (m + 1), 0);
     std::vector<std::vector<int>> blk(static_cast<size_t>(tiles));
     long long nodes = 0;
-    int heaviest = std::max(no, n);
+    bool any_wall = false;
     for (int tb = 0; tb < tiles; ++tb) {
         const int b0 = tb * kBT / n, b1 = (std::min(nbp, (tb + 1) * kBT) - 1) / n;
         std::vector<int>& L = blk[static_cast<size_t>(tb)];
-        L.push_back(0);
+        // the wall where a body of the block is close to it
+        bool wall = false;
+        for (int b = b0; b <= b1; ++b) wall = wall || F.h_swall[static_cast<size_t>(b)];
+        if (wall) L.push_back(0);
+        any_wall = any_wall || wall;
+        const size_t first = L.size();
         for (int b = b0; b <= b1; ++b)
             for (int i = F.h_snear_ptr[static_cast<size_t>(b)]; i < F.h_snear_ptr[static_cast<size_t>(b) + 1]; ++i)
                 if (F.h_snear_idx[static_cast<size_t>(i)] != b) L.push_back(1 + F.h_snear_idx[static_cast<size_t>(i)]);
-        std::sort(L.begin() + 1, L.end());
-        L.erase(std::unique(L.begin() + 1, L.end()), L.end());
-        nodes += no + static_cast<long long>(n) * (static_cast<long long>(L.size()) - 1);
+        std::sort(L.begin() + static_cast<long>(first), L.end());
+        L.erase(std::unique(L.begin() + static_cast<long>(first), L.end()), L.end());
+        nodes += (wall ? no : 0) + static_cast<long long>(n) * static_cast<long long>(L.size() - first);
     }
     for (int b = 0; b < m; ++b) {
-        mask[static_cast<size_t>(b) * (m + 1)] = 1;
+        mask[static_cast<size_t>(b) * (m + 1)] = F.h_swall[static_cast<size_t>(b)];
         for (int i = F.h_snear_ptr[static_cast<size_t>(b)]; i < F.h_snear_ptr[static_cast<size_t>(b) + 1]; ++i)
             mask[static_cast<size_t>(b) * (m + 1) + 1 + F.h_snear_idx[static_cast<size_t>(i)]] = 1;
     }
     // groups: about (nodes of a block) / (heaviest curve), at most 8
     const double per_block = static_cast<double>(nodes) / tiles;
+    const int heaviest = any_wall ? std::max(no, n) : n;
     const int G = std::max(1, std::min(8, static_cast<int>(std::lround(per_block / heaviest))));
     std::vector<int> ptr(1, 0), idx;
     std::vector<long long> load(static_cast<size_t>(G));

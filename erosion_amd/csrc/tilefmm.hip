@@ -62,6 +62,7 @@ constexpr int kWallSeg = 64;           // wall points per cluster
 constexpr int kDTile = 128;            // sources per direct-sum block
 constexpr int kStressSeries = 5;       // boundary-value series of the shear stress
 constexpr int kGroupMax = 8;           // most children of a parent cluster
+constexpr double kWallTrapH = 6.0;     // wall node spacings: trapezoid rule for the stress's wall
 
 inline int cdiv_int(int a, int b) { return (a + b - 1) / b; }
 
@@ -467,23 +468,26 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(const double* __restrict__ cr,
     }
 }
 
-// Multipoles of the five boundary-value series of the shear stress at every
-// body: grid (m, 5), block 256 as k_fmm_p2m (wave w: orders 8w .. 8w + 7).
-// Body b is cluster nwall + b; the series f of point p has the coefficient
-// a = b_f(p) dz(p) / (2 pi i), so that sum_p a / (z - w_p) is the exterior
-// Cauchy integral of b_f by the trapezoid rule -- the value the
-// barycentric exterior rule converges to away from the curve.  bd holds
-// b_f as bd[(2f + {0, 1}) N + p]; mp[b][f][k].
+// Multipoles of the five boundary-value series of the shear stress: grid
+// (m + nwall, 5), block 256 as k_fmm_p2m (wave w: orders 8w .. 8w + 7).
+// Source i < m is body i (cluster nwall + i): the series f of point p has the
+// coefficient a = b_f(p) dz(p) / (2 pi i), so that sum_p a / (z - w_p) is
+// the exterior Cauchy integral of b_f by the trapezoid rule -- the value the
+// barycentric exterior rule converges to away from the curve.  Source m + k
+// is wall segment k (cluster k) with the coefficient -a: the wall's interior
+// Cauchy integral, (1/2 pi i) sum b_f dz / (w - z).  bd holds b_f as
+// bd[(2f + {0, 1}) N + p]; mp[i][f][k].
 __global__ __launch_bounds__(256) void k_fmm_p2m_bd(const double* __restrict__ x,
                                                     const double* __restrict__ y,
                                                     const double* __restrict__ dzr,
                                                     const double* __restrict__ dzi,
                                                     const double* __restrict__ bd, size_t N,
                                                     const double* __restrict__ cr,
-                                                    const int* __restrict__ cl, int nwall,
+                                                    const int* __restrict__ cl, int nwall, int m,
                                                     double* __restrict__ mp) {
     constexpr int kPer = kP / 4;
-    const int b = blockIdx.x, f = blockIdx.y, c = nwall + b;
+    const int b = blockIdx.x, f = blockIdx.y, c = b < m ? nwall + b : b - m;
+    const double sgn = b < m ? 1.0 : -1.0;
     const int w = threadIdx.x / 64, l = threadIdx.x % 64;
     const double cx = cr[3 * c], cy = cr[3 * c + 1], iR = 1.0 / cr[3 * c + 2];
     const int off = cl[2 * c], len = cl[2 * c + 1];
@@ -495,7 +499,7 @@ __global__ __launch_bounds__(256) void k_fmm_p2m_bd(const double* __restrict__ x
         const cplx z = cmk((x[p] - cx) * iR, (y[p] - cy) * iR);
         const cplx bdz = cmul(cmk(bd[(2 * f) * N + p], bd[(2 * f + 1) * N + p]),
                               cmk(dzr[p], dzi[p]));
-        const cplx a = cmk(bdz.i / kTwoPi, -bdz.r / kTwoPi);  // bdz / (2 pi i)
+        const cplx a = cmk(sgn * bdz.i / kTwoPi, -sgn * bdz.r / kTwoPi);  // +-bdz / (2 pi i)
         cplx zk = cpow_int(z, kPer * w);
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
@@ -551,6 +555,7 @@ __global__ __launch_bounds__(kBlk) void k_fmm_l2p_strain(const double* __restric
         du[f] = cscale(d1, iR);
         dd[f] = cscale(d2, 2.0 * iR * iR);
     }
+
     const double u1x = dd[0].r - du[1].r - xt * dd[1].r - yt * dd[2].r + du[3].r;
     const double u1y = -dd[0].i + xt * dd[1].i - du[2].r + yt * dd[2].i - du[3].i;
     const double u2x = -dd[0].i + du[1].i + xt * dd[1].i + yt * dd[2].i + du[4].r;
@@ -558,6 +563,82 @@ __global__ __launch_bounds__(kBlk) void k_fmm_l2p_strain(const double* __restric
     out[q] = u1x;
     out[static_cast<size_t>(nbp) + q] = 0.5 * (u1y + u2x);
     out[2 * static_cast<size_t>(nbp) + q] = u2y;
+}
+
+// The wall segments near a body away from the wall (by the cluster rule),
+// summed directly by the trapezoid rule: grid m, block 256, body b's points
+// on lanes tid, tid + 256 (n <= 512).  Per segment (wnear_ptr, wnear_idx:
+// CSR of segment indices per body) the 64 nodes' positions and coefficients
+// a_f = b_f dz / (2 pi i) are staged in LDS; the wall's interior Cauchy
+// integral C_f = sum a_f / (w - z) has C_f' = sum a_f / (z - w)^2 and
+// C_f'' = -2 sum a_f / (z - w)^3, combined as in k_fmm_l2p_strain and added
+// to out.
+__global__ __launch_bounds__(kBlk) void k_wall_strain_direct(
+    const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ dzr,
+    const double* __restrict__ dzi, const double* __restrict__ bd, size_t N, int no, int n,
+    int nbp, const int* __restrict__ wnear_ptr, const int* __restrict__ wnear_idx,
+    double* __restrict__ out) {
+    __shared__ double sx[kWallSeg], sy[kWallSeg];
+    __shared__ cplx sa[kStressSeries][kWallSeg];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    const int e0 = wnear_ptr[b], e1 = wnear_ptr[b + 1];
+    if (e0 == e1) return;  // block-uniform
+    constexpr int kPts = kMaxBodyPts / kBlk;
+    cplx du[kPts][kStressSeries], dd[kPts][kStressSeries];
+    double xt[kPts], yt[kPts];
+#pragma unroll
+    for (int r = 0; r < kPts; ++r) {
+        const int i = tid + r * kBlk;
+        xt[r] = i < n ? x[no + b * n + i] : 1.0e30;
+        yt[r] = i < n ? y[no + b * n + i] : 1.0e30;
+#pragma unroll
+        for (int f = 0; f < kStressSeries; ++f) du[r][f] = dd[r][f] = cmk(0.0, 0.0);
+    }
+    for (int e = e0; e < e1; ++e) {
+        const int w0 = wnear_idx[e] * kWallSeg, cnt = min(kWallSeg, no - w0);
+        __syncthreads();
+        if (tid < cnt) {
+            const int wp = w0 + tid;
+            sx[tid] = x[wp];
+            sy[tid] = y[wp];
+            const cplx dz = cmk(dzr[wp], dzi[wp]);
+#pragma unroll
+            for (int f = 0; f < kStressSeries; ++f) {
+                const cplx bdz = cmul(cmk(bd[(2 * f) * N + wp], bd[(2 * f + 1) * N + wp]), dz);
+                sa[f][tid] = cmk(bdz.i / kTwoPi, -bdz.r / kTwoPi);  // bdz / (2 pi i)
+            }
+        }
+        __syncthreads();
+        for (int k = 0; k < cnt; ++k) {
+#pragma unroll
+            for (int r = 0; r < kPts; ++r) {
+                const cplx id = cinv(cmk(xt[r] - sx[k], yt[r] - sy[k]));
+                const cplx id2 = cmul(id, id);
+                const cplx id3 = cmul(id2, id);
+#pragma unroll
+                for (int f = 0; f < kStressSeries; ++f) {
+                    du[r][f] = cadd(du[r][f], cmul(sa[f][k], id2));
+                    dd[r][f] = csub(dd[r][f], cscale(cmul(sa[f][k], id3), 2.0));
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < kPts; ++r) {
+        const int i = tid + r * kBlk;
+        if (i >= n) continue;
+        const int q = b * n + i;
+        const double xq = xt[r], yq = yt[r];
+        const cplx* u = du[r];
+        const cplx* d = dd[r];
+        const double u1x = d[0].r - u[1].r - xq * d[1].r - yq * d[2].r + u[3].r;
+        const double u1y = -d[0].i + xq * d[1].i - u[2].r + yq * d[2].i - u[3].i;
+        const double u2x = -d[0].i + u[1].i + xq * d[1].i + yq * d[2].i + u[4].r;
+        const double u2y = -d[0].r + xq * d[1].r + u[2].i + yq * d[2].r - u[4].i;
+        out[q] += u1x;
+        out[static_cast<size_t>(nbp) + q] += 0.5 * (u1y + u2x);
+        out[2 * static_cast<size_t>(nbp) + q] += u2y;
+    }
 }
 
 // Direct sums: one block per work item (cluster c, tile j of c's source
@@ -943,11 +1024,63 @@ void tilefmm_prepare(Context& c) {
         direct += static_cast<long long>(T.len) * nsrc;
     }
     // shear stress: far bodies of each body by the same rule (expansions),
-    // near bodies (itself included) for the barycentric rule
+    // near bodies (itself included) for the barycentric rule.  The wall:
+    // a body whose points all lie kWallTrapH node spacings h or more from
+    // every wall node (h the largest spacing of the node's segment) takes the
+    // wall by the trapezoid rule -- its error decays like exp(-2 pi d / h),
+    // 4e-17 of the terms at 6 h -- far segments through the expansions, near
+    // ones summed directly; a body closer than that takes the whole wall by
+    // the barycentric interior rule.
+    // per segment: its largest node spacing times kWallTrapH, and its box
+    std::vector<double> dseg(static_cast<size_t>(nwall), 0.0);
+    std::vector<double> wbox(4 * static_cast<size_t>(nwall));
+    for (int k = 0; k < nwall; ++k) {
+        double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+        for (int p = k * kWallSeg; p < std::min(no, (k + 1) * kWallSeg); ++p)
+            dseg[static_cast<size_t>(k)] = std::max(dseg[static_cast<size_t>(k)],
+                                                    g.h_speed[static_cast<size_t>(p)]);
+        dseg[static_cast<size_t>(k)] *= kWallTrapH * kTwoPi / no;
+        for (int p = k * kWallSeg; p < std::min(no, (k + 1) * kWallSeg); ++p) {
+            x0 = std::min(x0, g.h_x[static_cast<size_t>(p)]);
+            x1 = std::max(x1, g.h_x[static_cast<size_t>(p)]);
+            y0 = std::min(y0, g.h_y[static_cast<size_t>(p)]);
+            y1 = std::max(y1, g.h_y[static_cast<size_t>(p)]);
+        }
+        wbox[4 * static_cast<size_t>(k)] = x0;
+        wbox[4 * static_cast<size_t>(k) + 1] = x1;
+        wbox[4 * static_cast<size_t>(k) + 2] = y0;
+        wbox[4 * static_cast<size_t>(k) + 3] = y1;
+    }
+    auto wall_close = [&](int b) {
+        const size_t o = static_cast<size_t>(no) + static_cast<size_t>(b) * n;
+        const Cluster& B = C[static_cast<size_t>(nwall + b)];
+        for (int k = 0; k < nwall; ++k) {
+            const Cluster& W = C[static_cast<size_t>(k)];
+            const double dwall = dseg[static_cast<size_t>(k)];
+            if (std::hypot(B.cx - W.cx, B.cy - W.cy) - B.r - W.r >= dwall) continue;
+            const double x0 = wbox[4 * static_cast<size_t>(k)] - dwall,
+                         x1 = wbox[4 * static_cast<size_t>(k) + 1] + dwall,
+                         y0 = wbox[4 * static_cast<size_t>(k) + 2] - dwall,
+                         y1 = wbox[4 * static_cast<size_t>(k) + 3] + dwall;
+            for (int i = 0; i < n; ++i) {
+                const double xb = g.h_x[o + i], yb = g.h_y[o + i];
+                if (xb < x0 || xb > x1 || yb < y0 || yb > y1) continue;
+                for (int p = k * kWallSeg; p < std::min(no, (k + 1) * kWallSeg); ++p) {
+                    const double dx = xb - g.h_x[static_cast<size_t>(p)],
+                                 dy = yb - g.h_y[static_cast<size_t>(p)];
+                    if (dx * dx + dy * dy < dwall * dwall) return true;
+                }
+            }
+        }
+        return false;
+    };
     std::vector<int> sfar_ptr(static_cast<size_t>(m) + 1, 0), sfar_idx;
+    std::vector<int> swn_ptr(static_cast<size_t>(m) + 1, 0), swn_idx;
     F.h_snear_ptr.assign(static_cast<size_t>(m) + 1, 0);
     F.h_snear_idx.clear();
+    F.h_swall.assign(static_cast<size_t>(m), 1);
     long long sdirect = 0;
+    int away = 0;
     for (int b = 0; b < m; ++b) {
         const Cluster& T = C[static_cast<size_t>(nwall + b)];
         for (int e = 0; e < m; ++e) {
@@ -958,12 +1091,33 @@ void tilefmm_prepare(Context& c) {
             else
                 F.h_snear_idx.push_back(e);
         }
+        const bool close = wall_close(b);
+        F.h_swall[static_cast<size_t>(b)] = close ? 1 : 0;
+        if (!close) {
+            ++away;
+            for (int k = 0; k < nwall; ++k) {
+                if (far_pair(C[static_cast<size_t>(k)], T))
+                    sfar_idx.push_back(m + k);
+                else
+                    swn_idx.push_back(k);
+            }
+        }
         sfar_ptr[static_cast<size_t>(b) + 1] = static_cast<int>(sfar_idx.size());
+        swn_ptr[static_cast<size_t>(b) + 1] = static_cast<int>(swn_idx.size());
         F.h_snear_ptr[static_cast<size_t>(b) + 1] = static_cast<int>(F.h_snear_idx.size());
         const long long nnear = F.h_snear_ptr[static_cast<size_t>(b) + 1] - F.h_snear_ptr[static_cast<size_t>(b)];
-        sdirect += static_cast<long long>(n) * (no + static_cast<long long>(n) * (nnear - 1));
+        sdirect += static_cast<long long>(n) * ((close ? no : 0) + static_cast<long long>(n) * (nnear - 1));
     }
     F.stress_direct_pairs = sdirect;
+    F.stress_wall = away > 0;
+    // sources of the stress expansions: bodies, then wall segments
+    std::vector<double> scr(3 * static_cast<size_t>(m + nwall));
+    for (int i = 0; i < m + nwall; ++i) {
+        const Cluster& S = C[static_cast<size_t>(i < m ? nwall + i : i - m)];
+        scr[3 * static_cast<size_t>(i)] = S.cx;
+        scr[3 * static_cast<size_t>(i) + 1] = S.cy;
+        scr[3 * static_cast<size_t>(i) + 2] = S.r;
+    }
     hipStream_t s = c.stream;
     auto up = [&](auto& buf, const auto& vec) {
         buf.ensure(std::max<size_t>(vec.size(), 1));
@@ -979,6 +1133,9 @@ void tilefmm_prepare(Context& c) {
     up(F.tiles, tiles);
     up(F.sfar_ptr, sfar_ptr);
     up(F.sfar_idx, sfar_idx);
+    up(F.swn_ptr, swn_ptr);
+    up(F.swn_idx, swn_idx);
+    up(F.scr, scr);
     up(F.nb_ptr, nb_ptr);
     up(F.nb_idx, nb_idx);
     up(F.par_ptr, par_ptr);
@@ -1027,7 +1184,7 @@ void tilefmm_prepare(Context& c) {
     F.mp.ensure(static_cast<size_t>(nc + npar) * 2 * kP * 2);
     F.mlog.ensure(static_cast<size_t>(nc + npar) * 4);
     F.lc.ensure(static_cast<size_t>(nc) * 3 * kP * 2);
-    F.smp.ensure(std::max(1, m) * static_cast<size_t>(kStressSeries) * kP * 2);
+    F.smp.ensure(std::max(1, m + nwall) * static_cast<size_t>(kStressSeries) * kP * 2);
     F.slc.ensure(std::max(1, m) * static_cast<size_t>(kStressSeries) * kP * 2);
     F.nc = nc;
     F.npar = npar;
@@ -1199,17 +1356,25 @@ void tilefmm_stress(Context& c, double* out) {
     hipStream_t s = c.stream;
     const int m = g.m, n = g.n, nbp = n * m;
     if (m == 0) return;
-    k_fmm_p2m_bd<<<dim3(m, kStressSeries), 256, 0, s>>>(g.x.p, g.y.p, g.dzr.p, g.dzi.p, c.bd.p,
-                                                       static_cast<size_t>(g.npts), F.cr.p, F.cl.p,
-                                                       F.nwall, F.smp.p);
+    // multipoles of the bodies and of the wall segments (the wall's only
+    // where some body is away from the wall)
+    const int nsrc = m + (F.stress_wall ? F.nwall : 0);
+    k_fmm_p2m_bd<<<dim3(nsrc, kStressSeries), 256, 0, s>>>(g.x.p, g.y.p, g.dzr.p, g.dzi.p, c.bd.p,
+                                                          static_cast<size_t>(g.npts), F.cr.p,
+                                                          F.cl.p, F.nwall, m, F.smp.p);
     STOKES_LAUNCH_CHECK();
     k_fmm_m2l<kStressSeries, false, false><<<dim3(m, kStressSeries), 256, 0, s>>>(
-        F.cr.p + 3 * static_cast<size_t>(F.nwall), F.smp.p, F.sfar_ptr.p, F.sfar_idx.p, F.bin.p,
-        nullptr, nullptr, F.slc.p);
+        F.scr.p, F.smp.p, F.sfar_ptr.p, F.sfar_idx.p, F.bin.p, nullptr, nullptr, F.slc.p);
     STOKES_LAUNCH_CHECK();
     k_fmm_l2p_strain<<<cdiv_int(nbp, kBlk), kBlk, 0, s>>>(g.x.p, g.y.p, F.cr.p, F.slc.p, g.no, n,
                                                          F.nwall, nbp, out);
     STOKES_LAUNCH_CHECK();
+    if (F.stress_wall) {
+        k_wall_strain_direct<<<m, kBlk, 0, s>>>(g.x.p, g.y.p, g.dzr.p, g.dzi.p, c.bd.p,
+                                                static_cast<size_t>(g.npts), g.no, n, nbp,
+                                                F.swn_ptr.p, F.swn_idx.p, out);
+        STOKES_LAUNCH_CHECK();
+    }
 }
 
 }  // namespace stokes
