@@ -446,7 +446,7 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
 // summed in two conflict-free stages, all of a chunk's at once.  The inverse coarse
 // matrices are stored transposed, so d = G^{-1} c reads them coalesced with
 // one row per lane.
-constexpr int kVals = 2 * kChunkNF;  // every coefficient of a chunk in one LDS round (~146 KB)
+constexpr int kVals = kChunkNF;  // coefficients per LDS round: half a chunk (~74 KB, two blocks per CU)
 constexpr int kSeg = 16;
 
 // geometry arrays the apply's reductions read (point order)
