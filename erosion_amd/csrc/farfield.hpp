@@ -147,6 +147,30 @@ __device__ __forceinline__ void dd_add_prod(double& hi, double& lo, double c, do
     lo += e + ep;
 }
 
+// One target per lane, each pair with its own reciprocal (estimate and one
+// Newton step); DIAG as in ff_tile_shared.
+template <bool DIAG, int NT = kTile>
+__device__ __forceinline__ void ff_tile_single(const double2* __restrict__ sxy,
+                                               const double2* __restrict__ sab,
+                                               const double* __restrict__ sc, double xt,
+                                               double yt, double& u, double& v) {
+#pragma unroll 4
+    for (int jj = 0; jj < NT; ++jj) {
+        const double2 p = sxy[jj];
+        const double2 ab = sab[jj];
+        const double rx = xt - p.x, ry = yt - p.y;
+        const double ry2 = ry * ry;
+        double r2 = __builtin_fma(rx, rx, ry2);
+        if (DIAG) r2 = __builtin_fmax(r2, 1.0e-100);
+        const double t = __builtin_fma(ab.x, rx, ab.y * ry);
+        const double w = __builtin_fma(t, rx, sc[jj] * ry2);
+        const double inv = rcp64_1(r2);
+        const double c = w * (inv * inv);
+        u = __builtin_fma(c, rx, u);
+        v = __builtin_fma(c, ry, v);
+    }
+}
+
 // The pair sums of a true-residual evaluation, TPT targets per lane: each
 // term formed as in the direct far field with its own full-precision
 // reciprocal (a shared one adds a few roundings of every term, and the terms

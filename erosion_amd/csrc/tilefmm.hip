@@ -695,6 +695,11 @@ __global__ __launch_bounds__(128) void k_fmm_direct(
             ff_tile_pairs_comp<TPT, true, kDTile>(sxy, sab, sc, xt, yt, u, cu, v, cv);
         else
             ff_tile_pairs_comp<TPT, false, kDTile>(sxy, sab, sc, xt, yt, u, cu, v, cv);
+    } else if constexpr (TPT == 1) {
+        if (diag)
+            ff_tile_single<true, kDTile>(sxy, sab, sc, xt[0], yt[0], u[0], v[0]);
+        else
+            ff_tile_single<false, kDTile>(sxy, sab, sc, xt[0], yt[0], u[0], v[0]);
     } else {
         if (diag)
             ff_tile_shared<TPT, true, kDTile>(sxy, sab, sc, xt, yt, u, v);
@@ -1231,6 +1236,10 @@ void direct_launch(Context& c, int tpt, int bt, int w0, int count) {
         k_fmm_direct<4, COMP><<<count, bt, 0, c.stream>>>(
             g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p, F.cl.p, work, F.tile_ptr.p, F.tiles.p,
             g.npts, c.part_u.p, c.part_v.p, pcu, pcv);
+    else if (tpt == 1)
+        k_fmm_direct<1, COMP><<<count, bt, 0, c.stream>>>(
+            g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p, F.cl.p, work, F.tile_ptr.p, F.tiles.p,
+            g.npts, c.part_u.p, c.part_v.p, pcu, pcv);
     else
         k_fmm_direct<2, COMP><<<count, bt, 0, c.stream>>>(
             g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p, F.cl.p, work, F.tile_ptr.p, F.tiles.p,
@@ -1307,13 +1316,14 @@ void tilefmm_apply(Context& c, const double* den, int* wall_nsplit, int* body_ns
         c.t_fmm_expand.stop(F.side, static_cast<double>(two_level ? F.far2_pairs : F.far_pairs));
     STOKES_HIP(hipEventRecord(F.join, F.side));
     if (timed) c.t_fmm_direct.start(s);
-    // bodies: 4 targets per lane above 128 points, else 2; wall segments: 2
+    // bodies: 4 targets per lane above 128 points, else 2; wall segments:
+    // one per lane, a full wave per segment
     auto direct = compensated ? direct_launch<true> : direct_launch<false>;
     if (n > 128)
         direct(c, 4, round32(cdiv_int(n, 4)), 0, F.work_body);
     else
         direct(c, 2, round32(cdiv_int(n, 2)), 0, F.work_body);
-    direct(c, 2, round32(cdiv_int(kWallSeg, 2)), F.work_body, F.work_wall);
+    direct(c, 1, kWallSeg, F.work_body, F.work_wall);
     if (timed) c.t_fmm_direct.stop(s, static_cast<double>(F.direct_pairs));
     STOKES_HIP(hipStreamWaitEvent(s, F.join, 0));
     // a block of 256 points spans at most 5 wall segments and, for bodies of
