@@ -32,6 +32,9 @@ namespace {
 constexpr int kBlk = 256;
 constexpr int kRows = 4;       // Krylov rows per block in the dot kernel
 constexpr int kChunkIters = 16;
+// DCGS2's second pass runs as one kernel from this many blocks of 512
+// elements up (vectors of 2^17 doubles), as split partials below
+constexpr int kFusedUpdateBlocks = 256;
 // preconditioned solves: implicit target of a cycle as a fraction of the
 // tolerance, and the most cycles one solve may run
 constexpr double kPcTolFrac = 0.5;
@@ -273,6 +276,59 @@ __global__ __launch_bounds__(kBlk) void k_update2(const double* __restrict__ V, 
         *reinterpret_cast<double2*>(wa + static_cast<size_t>(s) * npad + e) = make_double2(ax, ay);
         *reinterpret_cast<double2*>(wb + static_cast<size_t>(s) * npad + e) = make_double2(bx, by);
     }
+}
+
+// DCGS2's second pass in one kernel where the vectors are long (npad of
+// 2^17 or more: one block per 512 elements then fills the chip): each thread
+// sums a_r V[r][e] and b_r V[r][e] over every row r < nrows (the rows loaded
+// 8 at a time, the coefficients staged in LDS) and forms q and w' as
+// k_combine2 does, without split partials.
+__global__ __launch_bounds__(kBlk) void k_update_combine2(const double* __restrict__ V,
+                                                          size_t npad,
+                                                          const double* __restrict__ ca,
+                                                          const double* __restrict__ cb, int nrows,
+                                                          const GmresState* __restrict__ st,
+                                                          double* __restrict__ u,
+                                                          double* __restrict__ w) {
+    __shared__ double ha[kBlk], hb[kBlk];
+    if (st->done) return;
+    const size_t e = (static_cast<size_t>(blockIdx.x) * kBlk + threadIdx.x) * 2;
+    const bool valid = e < npad;
+    double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
+    for (int t0 = 0; t0 < nrows; t0 += kBlk) {
+        const int tn = min(kBlk, nrows - t0);
+        __syncthreads();
+        if (static_cast<int>(threadIdx.x) < tn) {
+            ha[threadIdx.x] = ca[t0 + threadIdx.x];
+            hb[threadIdx.x] = cb[t0 + threadIdx.x];
+        }
+        __syncthreads();
+        if (!valid) continue;
+        for (int q0 = 0; q0 < tn; q0 += 8) {
+            double2 vv[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                vv[q] = *reinterpret_cast<const double2*>(
+                    V + static_cast<size_t>(t0 + min(q0 + q, tn - 1)) * npad + e);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                if (q0 + q < tn) {
+                    ax = __builtin_fma(ha[q0 + q], vv[q].x, ax);
+                    ay = __builtin_fma(ha[q0 + q], vv[q].y, ay);
+                    bx = __builtin_fma(hb[q0 + q], vv[q].x, bx);
+                    by = __builtin_fma(hb[q0 + q], vv[q].y, by);
+                }
+            }
+        }
+    }
+    if (!valid) return;
+    const double rinv = st->rinv, gk = st->gk, sc = st->rinv * st->inv_snormw;
+    const double2 uv = *reinterpret_cast<const double2*>(u + e);
+    const double2 wv = *reinterpret_cast<const double2*>(w + e);
+    const double qx = (uv.x - ax) * rinv, qy = (uv.y - ay) * rinv;
+    *reinterpret_cast<double2*>(u + e) = make_double2(qx, qy);
+    *reinterpret_cast<double2*>(w + e) =
+        make_double2((wv.x - bx - qx * gk) * sc, (wv.y - by - qy * gk) * sc);
 }
 
 // w += sum_s wpart[s] (the final solution update)
@@ -581,6 +637,11 @@ static void dcgs_step(Context& c, int j, bool last, int ldh, double tol, int max
     STOKES_LAUNCH_CHECK();
     if (last) return;
     const int ebx = static_cast<int>(npad / (2 * kBlk));
+    if (ebx >= kFusedUpdateBlocks) {
+        k_update_combine2<<<ebx, kBlk, 0, s>>>(c.V.p, npad, c.hcur.p, c.hbuf.p, j, c.st.p, u, w);
+        STOKES_LAUNCH_CHECK();
+        return;
+    }
     int nsplit = 0;
     const size_t half = c.wpart.cap / 2;
     if (j > 0) {
