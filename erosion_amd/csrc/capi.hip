@@ -757,6 +757,54 @@ int erosion_mlp_train(int nin, int nhid, int nlayers, int nout, float* params, i
     }
 }
 
+namespace {
+// 8 independent FP64 FMA chains per lane, `iters` iterations; one double out
+// per thread (out holds gridDim.x * 256)
+__global__ __launch_bounds__(256) void k_fp64_chains(double* __restrict__ out, int iters, double a,
+                                                     double b) {
+    double x[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) x[q] = threadIdx.x * 1e-9 + q * 1e-9;
+    for (int i = 0; i < iters; ++i) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) x[q] = __builtin_fma(x[q], a, b);
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) s += x[q];
+    out[static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x] = s;
+}
+}  // namespace
+
+int erosion_measure_fp64(int64_t blocks, int64_t iters, double* ms) {
+    try {
+        if (blocks < 1 || blocks > (1 << 20) || iters < 1 || iters > (1 << 24) || !ms)
+            throw stokes::Error("invalid argument");
+        Context& c = stokes::ctx();
+        stokes::DevBuf<double> out;
+        out.ensure(static_cast<size_t>(blocks) * 256);
+        hipEvent_t e0, e1;
+        STOKES_HIP(hipEventCreate(&e0));
+        STOKES_HIP(hipEventCreate(&e1));
+        float t = 0.f;
+        for (int rep = 0; rep < 3; ++rep) {
+            STOKES_HIP(hipEventRecord(e0, c.stream));
+            k_fp64_chains<<<static_cast<int>(blocks), 256, 0, c.stream>>>(
+                out.p, static_cast<int>(iters), 0.999999, 1e-7);
+            STOKES_LAUNCH_CHECK();
+            STOKES_HIP(hipEventRecord(e1, c.stream));
+            STOKES_HIP(hipEventSynchronize(e1));
+            STOKES_HIP(hipEventElapsedTime(&t, e0, e1));
+        }
+        STOKES_HIP(hipEventDestroy(e0));
+        STOKES_HIP(hipEventDestroy(e1));
+        *ms = static_cast<double>(t);
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_measure_fp64", e.what());
+    }
+}
+
 int erosion_measure_copy(int64_t bytes, int64_t reps, double* ms) {
     try {
         if (bytes < 1 || reps < 1 || !ms) throw stokes::Error("invalid argument");

@@ -3,6 +3,8 @@
 
 Runs, on one workload (default: 100 bodies x 256 points):
 
+* with --fp64-rate, the FP64 FMA rate the chip sustains
+  (erosion_measure_fp64);
 * device-to-device copies of a known size (erosion_measure_copy), so that a
   profiler's per-dispatch byte counters for the other kernels can be read
   against those of a copy whose traffic is known: 2 x bytes per copy;
@@ -40,8 +42,24 @@ def main(argv=None) -> int:
     ap.add_argument("--no-stress", action="store_true")
     ap.add_argument("--shapes", action="store_true",
                     help="time every far-field launch shape (FP64 and FP32)")
+    ap.add_argument("--fp64-rate", action="store_true",
+                    help="the FP64 FMA rate the chip sustains at 4, 8 and 16 waves per SIMD")
     a = ap.parse_args(argv)
     L = stokes.lib()
+    if a.fp64_rate:
+        L.erosion_measure_fp64.argtypes = [ctypes.c_int64, ctypes.c_int64,
+                                           ctypes.POINTER(ctypes.c_double)]
+        L.erosion_measure_fp64.restype = ctypes.c_int
+        iters = 20000
+        for blocks in (1024, 2048, 4096):  # 4, 8, 16 waves of 256 CUs x 4 SIMDs
+            ms = ctypes.c_double(0.0)
+            if L.erosion_measure_fp64(blocks, iters, ctypes.byref(ms)) != 0:
+                raise stokes.StokesError(stokes.last_error())
+            fma = 8.0 * iters * blocks * 256
+            rate = fma / (ms.value * 1e-3) / 1e12
+            print(f"FP64 FMA chains, {blocks * 4 // 1024} waves per SIMD: {ms.value:.3f} ms, "
+                  f"{rate:.2f} T FMA/s = {2 * rate:.1f} TFLOP/s ({2 * rate / PEAK_FP64_TFLOPS:.3f} "
+                  f"of {PEAK_FP64_TFLOPS})", flush=True)
     if a.copy_mib > 0:
         L.erosion_measure_copy.argtypes = [ctypes.c_int64, ctypes.c_int64,
                                            ctypes.POINTER(ctypes.c_double)]

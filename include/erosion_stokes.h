@@ -230,6 +230,12 @@ int erosion_solve_batch(int64_t ngeom, const int64_t* npts, const int64_t* nbods
  * ms = mean milliseconds per copy.  Returns 0 or -1. */
 int erosion_measure_copy(int64_t bytes, int64_t reps, double* ms);
 
+/* Measurement hook: the FP64 vector rate the chip sustains -- `blocks` blocks
+ * of 256 threads, each thread `iters` iterations of 8 independent FP64 FMA
+ * chains, on the library's stream; ms = milliseconds of the third of three
+ * launches.  FMA per launch: 8 iters blocks 256.  Returns 0 or -1. */
+int erosion_measure_fp64(int64_t blocks, int64_t iters, double* ms);
+
 /* Right preconditioner of stokessolver_ (no counterpart in the reference,
  * whose solveBIE runs GMRES unpreconditioned, stokesSolver.f:490).
  * kind 0: none (default, the reference's iteration).  kind 1: per-curve
