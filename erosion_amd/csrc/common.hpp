@@ -370,6 +370,12 @@ struct Context {
     DevBuf<unsigned char> lst_mask;
     long long stress_lists_version = -1;
     int stress_groups = 0;
+    // the wall in the shear stress of bodies close to it, by node ranges:
+    // their points (wb_tq), the ranges' partial sums of each pass and the
+    // per-target pass results (S0 and u_f; du_f)
+    DevBuf<int> wb_tq;
+    DevBuf<double> wb_part, wb_u, wb_du;
+    int wb_nct = 0;
     DevBuf<double> tgt;     // target coordinates and QoI outputs
     double tol = 1e-8;
     // right preconditioner (precond.hip): kind 0 none (the reference's

@@ -214,6 +214,163 @@ __global__ __launch_bounds__(kBT) void k_bary_eval(
     }
 }
 
+// The wall (curve 0, the barycentric interior rule) at the points of the
+// bodies close to it, split over ranges of kWallRange wall nodes so that a
+// handful of bodies still fill the chip: grid (ceil(nct / 64), ranges), one
+// wave per block, the range's nodes staged in LDS.  tq[i] is the body point
+// of target i.  Pass PASS writes its partial sums part[(range * NV + v) nct +
+// i] (pass 1: S0, N_f, 12 values; pass 2: M_f, 10; pass 3: P_f, 6), with
+// wu = (S0, u_f) [12][nct] from pass 1 and wdu = du_f [10][nct] from pass 2
+// (k_bary_eval's passes for an interior curve, stokesSolver.f:5020-5098).
+constexpr int kWallRange = 256;
+template <int PASS>
+__global__ __launch_bounds__(64) void k_wall_bary_pass(NodeView nv, int no,
+                                                       const int* __restrict__ tq, int nct,
+                                                       const double* __restrict__ wu,
+                                                       const double* __restrict__ wdu,
+                                                       double* __restrict__ part) {
+    constexpr int NV = PASS == 1 ? 12 : PASS == 2 ? 10 : 6;
+    constexpr int NB = PASS == 3 ? 6 : 10;  // boundary-value components read
+    __shared__ double sx[kWallRange], sy[kWallRange], szr[kWallRange], szi[kWallRange];
+    __shared__ double sb[NB][kWallRange];
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const bool valid = i < nct;
+    const int p0 = blockIdx.y * kWallRange, cnt = min(kWallRange, no - p0);
+    for (int k = threadIdx.x; k < cnt; k += 64) {
+        const int p = p0 + k;
+        sx[k] = nv.x[p];
+        sy[k] = nv.y[p];
+        szr[k] = nv.dzr[p];
+        szi[k] = nv.dzi[p];
+#pragma unroll
+        for (int f = 0; f < NB; ++f) sb[f][k] = nv.bd[static_cast<size_t>(f) * nv.N + p];
+    }
+    __syncthreads();
+    const int q = valid ? tq[i] : 0;
+    // lanes past the targets evaluate a far point and store nothing
+    const double xt = valid ? nv.x[no + q] : 1.0e3, yt = valid ? nv.y[no + q] : 1.0e3;
+    cplx u[5], du[3];
+    if (PASS >= 2) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f)
+            u[f] = valid ? cmk(wu[(2 + 2 * f) * static_cast<size_t>(nct) + i],
+                               wu[(3 + 2 * f) * static_cast<size_t>(nct) + i])
+                         : cmk(0.0, 0.0);
+    }
+    if (PASS == 3) {
+#pragma unroll
+        for (int f = 0; f < 3; ++f)
+            du[f] = valid ? cmk(wdu[(2 * f) * static_cast<size_t>(nct) + i],
+                                wdu[(2 * f + 1) * static_cast<size_t>(nct) + i])
+                          : cmk(0.0, 0.0);
+    }
+    double v[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) v[j] = 0.0;
+    for (int k = 0; k < cnt; ++k) {
+        const cplx d = cmk(sx[k] - xt, sy[k] - yt);
+        const cplx dz = cmk(szr[k], szi[k]);
+        if (PASS == 1) {
+            const cplx w = cmul(dz, cinv(d));
+            v[0] += w.r;
+            v[1] += w.i;
+#pragma unroll
+            for (int f = 0; f < 5; ++f) {
+                const cplx b = cmk(sb[2 * f][k], sb[2 * f + 1][k]);
+                v[2 + 2 * f] = __builtin_fma(b.r, w.r, __builtin_fma(-b.i, w.i, v[2 + 2 * f]));
+                v[3 + 2 * f] = __builtin_fma(b.r, w.i, __builtin_fma(b.i, w.r, v[3 + 2 * f]));
+            }
+        } else if (PASS == 2) {
+            const double rho2 = __builtin_fma(d.r, d.r, d.i * d.i);
+            const double ir = rcp64(rho2);
+            const cplx id = cmk(d.r * ir, -d.i * ir);
+            const cplx w2 = cmul(cmul(dz, id), id);
+#pragma unroll
+            for (int f = 0; f < 5; ++f) {
+                const cplx t = csub(cmk(sb[2 * f][k], sb[2 * f + 1][k]), u[f]);
+                v[2 * f] = __builtin_fma(t.r, w2.r, __builtin_fma(-t.i, w2.i, v[2 * f]));
+                v[2 * f + 1] = __builtin_fma(t.r, w2.i, __builtin_fma(t.i, w2.r, v[2 * f + 1]));
+            }
+        } else {
+            const cplx id = cinv(d);
+            const cplx w3 = cmul(cmul(cmul(dz, id), id), id);
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                const cplx t = csub(csub(cmk(sb[2 * f][k], sb[2 * f + 1][k]), u[f]), cmul(du[f], d));
+                v[2 * f] = __builtin_fma(t.r, w3.r, __builtin_fma(-t.i, w3.i, v[2 * f]));
+                v[2 * f + 1] = __builtin_fma(t.r, w3.i, __builtin_fma(t.i, w3.r, v[2 * f + 1]));
+            }
+        }
+    }
+    if (valid) {
+        const size_t o = static_cast<size_t>(blockIdx.y) * NV * static_cast<size_t>(nct) + i;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) part[o + static_cast<size_t>(j) * nct] = v[j];
+    }
+}
+
+// The passes' sums over the node ranges, in range order, one thread per
+// target: STAGE 1 gives wu = (S0, u_f = N_f / S0), STAGE 2 wdu = du_f = M_f /
+// S0, STAGE 3 dd_f = 2 P_f / S0 and adds the nder=2 combination of the wall
+// (E11, E12, E22) to out[{0,1,2} nbp + q].
+template <int STAGE>
+__global__ __launch_bounds__(kBlk) void k_wall_bary_reduce(NodeView nv, int no,
+                                                           const int* __restrict__ tq, int nct,
+                                                           int nrange,
+                                                           const double* __restrict__ part,
+                                                           double* __restrict__ wu,
+                                                           double* __restrict__ wdu, int nbp,
+                                                           double* __restrict__ out) {
+    constexpr int NV = STAGE == 1 ? 12 : STAGE == 2 ? 10 : 6;
+    const int i = blockIdx.x * kBlk + threadIdx.x;
+    if (i >= nct) return;
+    double v[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) v[j] = 0.0;
+    for (int r = 0; r < nrange; ++r) {
+        const size_t o = static_cast<size_t>(r) * NV * static_cast<size_t>(nct) + i;
+#pragma unroll
+        for (int j = 0; j < NV; ++j) v[j] += part[o + static_cast<size_t>(j) * nct];
+    }
+    const size_t n1 = static_cast<size_t>(nct);
+    if (STAGE == 1) {
+        const cplx idv = cinv(cmk(v[0], v[1]));  // interior: 1 / S0
+        wu[i] = v[0];
+        wu[n1 + i] = v[1];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            const cplx uf = cmul(cmk(v[2 + 2 * f], v[3 + 2 * f]), idv);
+            wu[(2 + 2 * f) * n1 + i] = uf.r;
+            wu[(3 + 2 * f) * n1 + i] = uf.i;
+        }
+        return;
+    }
+    const cplx iden = cinv(cmk(wu[i], wu[n1 + i]));
+    if (STAGE == 2) {
+#pragma unroll
+        for (int f = 0; f < 5; ++f) {
+            const cplx d = cmul(cmk(v[2 * f], v[2 * f + 1]), iden);
+            wdu[(2 * f) * n1 + i] = d.r;
+            wdu[(2 * f + 1) * n1 + i] = d.i;
+        }
+        return;
+    }
+    cplx du[5], dd[3];
+#pragma unroll
+    for (int f = 0; f < 5; ++f) du[f] = cmk(wdu[(2 * f) * n1 + i], wdu[(2 * f + 1) * n1 + i]);
+#pragma unroll
+    for (int f = 0; f < 3; ++f) dd[f] = cscale(cmul(cmk(v[2 * f], v[2 * f + 1]), iden), 2.0);
+    const int q = tq[i];
+    const double xt = nv.x[no + q], yt = nv.y[no + q];
+    const double u1x = dd[0].r - du[1].r - xt * dd[1].r - yt * dd[2].r + du[3].r;
+    const double u1y = -dd[0].i + xt * dd[1].i - du[2].r + yt * dd[2].i - du[3].i;
+    const double u2x = -dd[0].i + du[1].i + xt * dd[1].i + yt * dd[2].i + du[4].r;
+    const double u2y = -dd[0].r + xt * dd[1].r + du[2].i + yt * dd[2].r - du[4].i;
+    out[q] += u1x;
+    out[static_cast<size_t>(nbp) + q] += 0.5 * (u1y + u2x);
+    out[2 * static_cast<size_t>(nbp) + q] += u2y;
+}
+
 // The MODE 1 evaluation of k_bary_eval for a few targets: one wave per
 // (target, curve) pair, the lanes striding over the curve's nodes and meeting
 // in cross-lane butterflies after each of the three passes.  A handful of
@@ -693,10 +850,11 @@ static int all_curve_lists(Context& c, int tiles) {
     return G;
 }
 
-// The shear stress's near curves of every target block: the wall and the
-// near bodies (by the cluster rule) of each body in the block, in groups
-// balanced by node count (largest first onto the lightest group), with the
-// per-(body, curve) mask.  Cached per geometry.  Returns the number of groups.
+// The shear stress's near curves of every target block: the near bodies (by
+// the cluster rule) of each body in the block, in groups balanced by node
+// count (largest first onto the lightest group), with the per-(body, curve)
+// mask, and the points of the bodies close to the wall, whose wall goes by
+// node ranges.  Cached per geometry.  Returns the number of groups.
 static int near_curve_lists(Context& c, int tiles) {
     const Geometry& g = c.g;
     if (c.stress_lists_version == g.version) return c.stress_groups;
@@ -705,32 +863,34 @@ static int near_curve_lists(Context& c, int tiles) {
     std::vector<unsigned char> mask(static_cast<size_t>(m) * (m + 1), 0);
     std::vector<std::vector<int>> blk(static_cast<size_t>(tiles));
     long long nodes = 0;
-    bool any_wall = false;
     for (int tb = 0; tb < tiles; ++tb) {
         const int b0 = tb * kBT / n, b1 = (std::min(nbp, (tb + 1) * kBT) - 1) / n;
         std::vector<int>& L = blk[static_cast<size_t>(tb)];
-        // the wall where a body of the block is close to it
-        bool wall = false;
-        for (int b = b0; b <= b1; ++b) wall = wall || F.h_swall[static_cast<size_t>(b)];
-        if (wall) L.push_back(0);
-        any_wall = any_wall || wall;
         const size_t first = L.size();
         for (int b = b0; b <= b1; ++b)
             for (int i = F.h_snear_ptr[static_cast<size_t>(b)]; i < F.h_snear_ptr[static_cast<size_t>(b) + 1]; ++i)
                 if (F.h_snear_idx[static_cast<size_t>(i)] != b) L.push_back(1 + F.h_snear_idx[static_cast<size_t>(i)]);
         std::sort(L.begin() + static_cast<long>(first), L.end());
         L.erase(std::unique(L.begin() + static_cast<long>(first), L.end()), L.end());
-        nodes += (wall ? no : 0) + static_cast<long long>(n) * static_cast<long long>(L.size() - first);
+        nodes += static_cast<long long>(n) * static_cast<long long>(L.size() - first);
     }
+    // the wall of the bodies close to it goes by node ranges (k_wall_bary_pass)
+    std::vector<int> tq;
+    for (int b = 0; b < m; ++b)
+        if (F.h_swall[static_cast<size_t>(b)])
+            for (int k = 0; k < n; ++k) tq.push_back(b * n + k);
+    c.wb_nct = static_cast<int>(tq.size());
+    c.wb_tq.ensure(std::max<size_t>(tq.size(), 1));
+    if (!tq.empty())
+        STOKES_HIP(hipMemcpyAsync(c.wb_tq.p, tq.data(), tq.size() * sizeof(int),
+                                  hipMemcpyHostToDevice, c.stream));
     for (int b = 0; b < m; ++b) {
-        mask[static_cast<size_t>(b) * (m + 1)] = F.h_swall[static_cast<size_t>(b)];
         for (int i = F.h_snear_ptr[static_cast<size_t>(b)]; i < F.h_snear_ptr[static_cast<size_t>(b) + 1]; ++i)
             mask[static_cast<size_t>(b) * (m + 1) + 1 + F.h_snear_idx[static_cast<size_t>(i)]] = 1;
     }
     // groups: about (nodes of a block) / (heaviest curve), at most 8
     const double per_block = static_cast<double>(nodes) / tiles;
-    const int heaviest = any_wall ? std::max(no, n) : n;
-    const int G = std::max(1, std::min(8, static_cast<int>(std::lround(per_block / heaviest))));
+    const int G = std::max(1, std::min(8, static_cast<int>(std::lround(per_block / n))));
     std::vector<int> ptr(1, 0), idx;
     std::vector<long long> load(static_cast<size_t>(G));
     std::vector<std::vector<int>> grp(static_cast<size_t>(G));
@@ -797,6 +957,26 @@ void shear_stress(Context& c, const double* den_dev, double* tau_dev) {
                                                        g.ax.p, g.ay.p, no, n, nullptr, nullptr,
                                                        nbp, c.spart.p);
     STOKES_LAUNCH_CHECK();
+    if (clusters && c.wb_nct > 0) {
+        // the wall at the points of the bodies close to it, added to slot 0
+        const int nct = c.wb_nct, nr = cdiv_int(no, kWallRange);
+        c.wb_part.ensure(static_cast<size_t>(nr) * 12 * nct);
+        c.wb_u.ensure(12 * static_cast<size_t>(nct));
+        c.wb_du.ensure(10 * static_cast<size_t>(nct));
+        const dim3 pg(cdiv_int(nct, 64), nr);
+        const int rb = cdiv_int(nct, kBlk);
+        k_wall_bary_pass<1><<<pg, 64, 0, s>>>(nv, no, c.wb_tq.p, nct, nullptr, nullptr, c.wb_part.p);
+        k_wall_bary_reduce<1><<<rb, kBlk, 0, s>>>(nv, no, c.wb_tq.p, nct, nr, c.wb_part.p,
+                                                   c.wb_u.p, c.wb_du.p, nbp, c.spart.p);
+        k_wall_bary_pass<2><<<pg, 64, 0, s>>>(nv, no, c.wb_tq.p, nct, c.wb_u.p, nullptr, c.wb_part.p);
+        k_wall_bary_reduce<2><<<rb, kBlk, 0, s>>>(nv, no, c.wb_tq.p, nct, nr, c.wb_part.p,
+                                                   c.wb_u.p, c.wb_du.p, nbp, c.spart.p);
+        k_wall_bary_pass<3><<<pg, 64, 0, s>>>(nv, no, c.wb_tq.p, nct, c.wb_u.p, c.wb_du.p,
+                                               c.wb_part.p);
+        k_wall_bary_reduce<3><<<rb, kBlk, 0, s>>>(nv, no, c.wb_tq.p, nct, nr, c.wb_part.p,
+                                                   c.wb_u.p, c.wb_du.p, nbp, c.spart.p);
+        STOKES_LAUNCH_CHECK();
+    }
     if (c.time_kernels) {
         // (target point, source node) pairs of the barycentric rule
         c.t_stress_eval.stop(s, clusters ? static_cast<double>(c.fmm.stress_direct_pairs)
