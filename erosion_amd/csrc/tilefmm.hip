@@ -897,9 +897,11 @@ void bisect_groups(const std::vector<Cluster>& C, std::vector<int>& idx, int lo,
     bisect_groups(C, idx, mid, hi, out);
 }
 
+// The far rule R_s + R_t <= kSep |c_s - c_t|, squared (no square root); every
+// list uses this one predicate, so a pair is near or far the same way in all.
 bool far_pair(const Cluster& S, const Cluster& T) {
-    const double dx = S.cx - T.cx, dy = S.cy - T.cy;
-    return S.r + T.r <= kSep * std::sqrt(dx * dx + dy * dy);
+    const double dx = S.cx - T.cx, dy = S.cy - T.cy, rr = S.r + T.r;
+    return rr * rr <= kSep * kSep * (dx * dx + dy * dy);
 }
 
 }  // namespace
@@ -1000,8 +1002,7 @@ void tilefmm_prepare(Context& c) {
         long long nsrc = 0;
         for (int s = 0; s < nc; ++s) {
             const Cluster& S = C[static_cast<size_t>(s)];
-            const double dx = S.cx - T.cx, dy = S.cy - T.cy;
-            if (s != t && S.r + T.r <= kSep * std::sqrt(dx * dx + dy * dy)) {
+            if (s != t && far_pair(S, T)) {
                 far_idx.push_back(s);
             } else {
                 near.emplace_back(S.start, S.len);
@@ -1090,8 +1091,7 @@ void tilefmm_prepare(Context& c) {
         const Cluster& T = C[static_cast<size_t>(nwall + b)];
         for (int e = 0; e < m; ++e) {
             const Cluster& S = C[static_cast<size_t>(nwall + e)];
-            const double dx = S.cx - T.cx, dy = S.cy - T.cy;
-            if (e != b && S.r + T.r <= kSep * std::sqrt(dx * dx + dy * dy))
+            if (e != b && far_pair(S, T))
                 sfar_idx.push_back(e);
             else
                 F.h_snear_idx.push_back(e);
