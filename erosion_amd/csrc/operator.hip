@@ -1018,6 +1018,23 @@ void boundary_values(Context& c, const int* curves_dev, int ncurves, int wall_fi
     }
 }
 
+// FP32 far-field source tiles of the current geometry (once per geometry).
+// Every FP32 far-field launch reads them, the autotuner's included: they must
+// be formed for the current geometry before the first such launch.
+static void ff32_sources(Context& c) {
+    Geometry& g = c.g;
+    if (c.ff32_version == g.version) return;
+    const int np = g.npts;
+    const int ntiles = cdiv_int(np, kTile);
+    c.ff32_ctr.ensure(2 * static_cast<size_t>(ntiles));
+    c.ff32_xy.ensure(2 * static_cast<size_t>(ntiles) * kTile);
+    k_ff32_prep<<<ntiles, kBlk, 0, c.stream>>>(g.x.p, g.y.p, np,
+                                               reinterpret_cast<double2*>(c.ff32_ctr.p),
+                                               reinterpret_cast<float2*>(c.ff32_xy.p));
+    STOKES_LAUNCH_CHECK();
+    c.ff32_version = g.version;
+}
+
 // Autotuner: the first operator application at a given number of boundary
 // points times every far-field shape (one warm-up launch, then three timed
 // launches each, HIP events on the library stream) and keeps the fastest.
@@ -1068,6 +1085,7 @@ static int tune_farfield(Context& c, bool fp32) {
         }
     const bool fmm = key & 1;
     if (fmm) tilefmm_prepare(c);
+    if (fp32) ff32_sources(c);
     int best = 0;
     float best_ms = 1e30f;
     hipEvent_t e0, e1;
@@ -1117,21 +1135,6 @@ bool farfield_fp32_pays(const Context& c) {
     auto b = c.ff_tuned_ms.find(tune_key(c, false));
     if (a == c.ff32_tuned_ms.end() || b == c.ff_tuned_ms.end()) return true;
     return a->second < b->second;
-}
-
-// FP32 far-field source tiles of the current geometry (once per geometry).
-static void ff32_sources(Context& c) {
-    Geometry& g = c.g;
-    if (c.ff32_version == g.version) return;
-    const int np = g.npts;
-    const int ntiles = cdiv_int(np, kTile);
-    c.ff32_ctr.ensure(2 * static_cast<size_t>(ntiles));
-    c.ff32_xy.ensure(2 * static_cast<size_t>(ntiles) * kTile);
-    k_ff32_prep<<<ntiles, kBlk, 0, c.stream>>>(g.x.p, g.y.p, np,
-                                               reinterpret_cast<double2*>(c.ff32_ctr.p),
-                                               reinterpret_cast<float2*>(c.ff32_xy.p));
-    STOKES_LAUNCH_CHECK();
-    c.ff32_version = g.version;
 }
 
 // Stokeslet logarithms of the current geometry (once per geometry).
@@ -1191,6 +1194,7 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, b
     c.pc_token_y = nullptr;
     if (!from_precond) gather_density(c, x_dev);
     bool fp32 = c.ff_fp32_now && !compensated;
+    if (fp32) ff32_sources(c);
     // a compensated evaluation runs the cluster far field with compensated
     // direct sums where the FP64 far field in use is the cluster one, else
     // its own direct far field: 4 targets per lane, per-pair reciprocals,
@@ -1198,7 +1202,6 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, b
     int choice = tune_farfield(c, fp32);
     if (compensated && choice != kFmmIndex) choice = -1;
     if (choice == kFmmIndex) fp32 = false;  // the cluster far field is FP64
-    if (fp32) ff32_sources(c);
     int nsplit = 0, nsplit_body = 0;
     if (choice == kFmmIndex) {
         const bool timed = c.time_kernels && !compensated;

@@ -125,3 +125,57 @@ def test_precision_mode_validation(st):
         st.set_precision(3)
     st.set_precision(1)
     assert st.get_precision() == 1
+
+
+_FRESH_PROCESS_FP32 = r"""
+import numpy as np
+from erosion_amd import stokes
+from erosion_amd.geometry import WORKLOADS, workload_xy
+
+def run(nbodies, seed):
+    n, m, no, xx, yy = workload_xy(WORKLOADS["bodies100_n256"], nbodies=nbodies)
+    eta = np.random.default_rng(seed).standard_normal(2 * no + 2 * n * m + 3 * m)
+    return n, m, no, xx, yy, eta
+
+# 1. the process's first operator application is an FP32 one
+n, m, no, xx, yy, eta = run(20, 31)
+stokes.set_precision(2)
+v32 = stokes.matvec(n, m, no, xx, yy, eta)
+stokes.set_precision(0)
+v64 = stokes.matvec(n, m, no, xx, yy, eta)
+# 2. the kernel-timing sequence: FP64, then FP32, at a larger geometry than
+#    any FP32 source tiles formed so far
+n, m, no, xx, yy, eta = run(40, 32)
+stokes.time_matvec(n, m, no, xx, yy, eta, reps=1)
+stokes.set_precision(2)
+stokes.time_matvec(n, m, no, xx, yy, eta, reps=1)
+w32 = stokes.matvec(n, m, no, xx, yy, eta)
+stokes.set_precision(0)
+w64 = stokes.matvec(n, m, no, xx, yy, eta)
+for a, b in ((v32, v64), (w32, w64)):
+    err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+    rms = np.sqrt(np.mean((a - b) ** 2) / np.mean(b ** 2))
+    print("RESULT", err, rms, flush=True)
+"""
+
+
+def test_fp32_farfield_first_in_fresh_process():
+    """The FP32 far field's autotuning launches read the FP32 source tiles of
+    the current geometry: an FP32 application that comes first in a process,
+    and one at a geometry larger than any before it, must form them before
+    timing any shape.  Runs in a child process, where no earlier FP32 solve has
+    left source tiles behind."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    env = dict(os.environ, PYTHONPATH=root + os.pathsep + os.environ.get("PYTHONPATH", ""))
+    r = subprocess.run([sys.executable, "-c", _FRESH_PROCESS_FP32], cwd=root, env=env,
+                       capture_output=True, text=True, timeout=180)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    res = [tuple(map(float, ln.split()[1:])) for ln in r.stdout.splitlines()
+           if ln.startswith("RESULT")]
+    assert len(res) == 2
+    for err, rms in res:
+        assert err < 1e-5 and rms < 1e-6, (err, rms)
