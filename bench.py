@@ -194,6 +194,9 @@ def main() -> int:
     ap.add_argument("--body-modes", type=int, default=None,
                     help="accelerated variant: Fourier modes per body in the preconditioner's "
                          "coarse space (default: the library's)")
+    ap.add_argument("--timing-period", type=int, default=16,
+                    help="time the kernels of one GMRES iteration in this many (each HIP event "
+                         "record costs the stream a few microseconds; 1 times every launch)")
     ap.add_argument("--graphs", action="store_true",
                     help="replay GMRES iteration chunks as HIP graphs in the accelerated "
                          "variant; kernel timing then runs on one extra untimed step")
@@ -256,7 +259,7 @@ def main() -> int:
             th = rungekutta2_device(th, params, warm=warm, timings=timings, guess=guess)
             log(f"[{label}] warmup step {i + 1}/{warmup}: bodies={len(th.thlenvec)}")
         sync()
-        stokes.kernel_timing(timed_kernels and not graphs)
+        stokes.kernel_timing(timed_kernels and not graphs, period=args.timing_period)
         misses0 = tol_misses()
         timings, solve_info, relres = {}, [], []
         t0 = time.perf_counter()
@@ -551,6 +554,10 @@ def main() -> int:
                        "network's; the whole RK2 step in the library",
             "dt": w.dt, "fixpdrop": True, "ibc": w.ibc,
             "hip_graphs": bool(args.graphs),
+            # kernel durations: HIP events around every launch outside the
+            # GMRES iterations and around the launches of one iteration in
+            # this many (each standing for that many)
+            "kernel_timing_period": args.timing_period,
             "accelerated": sa,
             "plain_fp64": sp,
             "plain_fp64_sample": (f"{args.plain_steps} step(s) from the same state, no warm-up"

@@ -835,6 +835,7 @@ int erosion_measure_copy(int64_t bytes, int64_t reps, double* ms) {
 
 int erosion_kernel_timing(int enable) {
     try {
+        if (enable < 0) throw stokes::Error("invalid timing period");
         Context& c = stokes::ctx();
         if (enable) {
             for (stokes::KernelTimer* t : {&c.t_farfield, &c.t_near, &c.t_stress_eval, &c.t_gs,
@@ -848,6 +849,8 @@ int erosion_kernel_timing(int enable) {
                 t->harvest();
         }
         c.time_kernels = enable != 0;
+        c.time_period = enable > 1 ? enable : 1;
+        c.time_tick = 0;
         return 0;
     } catch (const std::exception& e) {
         return record_error("erosion_kernel_timing", e.what());

@@ -131,12 +131,22 @@ struct Geometry {
     long long version = 0;
 };
 
+// Weight of the launches timed now: 1, or P while one GMRES iteration in P
+// is timed and stands for the P iterations (one per host thread, as each
+// context is driven by one thread at a time).
+inline thread_local int tl_time_weight = 1;
+
 // Durations of one kernel family, measured with HIP events recorded around
 // each launch on the library stream.  Events come from a pool and are read
 // back only when the pool is full or on harvest(), so timing adds no host
-// synchronisation to the launch sequence.
+// synchronisation to the launch sequence.  Each event record costs the
+// stream a few microseconds, so the GMRES iterations may be sampled: every
+// timed launch carries its weight, ms, launches and work are weighted sums
+// (estimates of the totals over all launches) and ms / launches is the
+// measured mean duration.
 struct KernelTimer {
     std::vector<hipEvent_t> ev;  // pairs: start 2i, stop 2i+1
+    std::vector<int> wt;         // weight of pair i
     size_t used = 0;
     double ms = 0.0;
     long long launches = 0;
@@ -144,6 +154,7 @@ struct KernelTimer {
     void start(hipStream_t s) {
         if (ev.empty()) {
             ev.resize(2 * 512);
+            wt.assign(512, 1);
             for (auto& e : ev) STOKES_HIP(hipEventCreate(&e));
         }
         if (used == ev.size() / 2) harvest();
@@ -151,8 +162,9 @@ struct KernelTimer {
     }
     void stop(hipStream_t s, double work_units) {
         STOKES_HIP(hipEventRecord(ev[2 * used + 1], s));
+        wt[used] = tl_time_weight;
         ++used;
-        work += work_units;
+        work += work_units * tl_time_weight;
     }
     void harvest() {
         if (!used) return;
@@ -160,9 +172,9 @@ struct KernelTimer {
         for (size_t i = 0; i < used; ++i) {
             float t = 0.f;
             STOKES_HIP(hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]));
-            ms += t;
+            ms += static_cast<double>(t) * wt[i];
+            launches += wt[i];
         }
-        launches += static_cast<long long>(used);
         used = 0;
     }
     void reset() {
@@ -389,8 +401,12 @@ struct Context {
     // else that writes c.ex/c.ey (apply_operator's from_precond checks it)
     const double* pc_token_x = nullptr;
     const double* pc_token_y = nullptr;
-    // per-kernel timing with HIP events on `stream` (measurement hooks only)
+    // per-kernel timing with HIP events on `stream` (measurement hooks only);
+    // the GMRES iterations' kernels are timed in one iteration of every
+    // time_period (time_tick counts the iterations)
     bool time_kernels = false;
+    int time_period = 1;
+    long long time_tick = 0;
     // replay GMRES iteration chunks as HIP graphs (off while timing kernels)
     int use_graphs = 0;
     GraphCache graphs;

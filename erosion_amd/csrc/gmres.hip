@@ -856,7 +856,23 @@ long long gmres_solve(Context& c, int ibary, int ibc, int maxl_in, double tol, d
             // iterations k .. k+chunk-1 (each exits early on the device once
             // the state reports convergence)
             auto run_chunk = [&]() {
+                // kernel timing samples one iteration in c.time_period
+                struct Sampling {
+                    Context& c;
+                    const bool on;
+                    explicit Sampling(Context& cc) : c(cc), on(cc.time_kernels) {}
+                    void next() {
+                        const bool now = on && c.time_tick++ % c.time_period == 0;
+                        c.time_kernels = now;
+                        tl_time_weight = now ? c.time_period : 1;
+                    }
+                    ~Sampling() {
+                        c.time_kernels = on;
+                        tl_time_weight = 1;
+                    }
+                } sampling(c);
                 for (int j = 0; j < chunk; ++j) {
+                    sampling.next();
                     const int kk = k + j;
                     double* vk = c.V.p + static_cast<size_t>(kk) * npad;
                     double* w = c.V.p + static_cast<size_t>(kk + 1) * npad;

@@ -1086,6 +1086,14 @@ static int tune_farfield(Context& c, bool fp32) {
     const bool fmm = key & 1;
     if (fmm) tilefmm_prepare(c);
     if (fp32) ff32_sources(c);
+    // the tuning launches are not operator applications: the kernel timers
+    // of the cluster far field's parts stay off while they run
+    struct TimingOff {
+        Context& c;
+        const bool was;
+        explicit TimingOff(Context& cc) : c(cc), was(cc.time_kernels) { c.time_kernels = false; }
+        ~TimingOff() { c.time_kernels = was; }
+    } timing_off(c);
     int best = 0;
     float best_ms = 1e30f;
     hipEvent_t e0, e1;

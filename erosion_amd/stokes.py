@@ -269,9 +269,14 @@ TIMER_FAMILIES = ("farfield", "near", "stress_eval", "gram_schmidt", "precond", 
                   "fmm", "fmm_direct", "fmm_expand")
 
 
-def kernel_timing(enable: bool) -> None:
-    """Start (clearing totals) or stop HIP-event timing of the hot kernels."""
-    if lib().erosion_kernel_timing(1 if enable else 0) != 0:
+def kernel_timing(enable: bool, period: int = 1) -> None:
+    """Start (clearing totals) or stop HIP-event timing of the hot kernels.
+    period > 1: a GMRES iteration's launches are timed in one iteration of
+    every `period`, each standing for `period` (totals stay estimates of all
+    launches, per-launch means are measured)."""
+    if period < 1:
+        raise StokesError("period must be at least 1")
+    if lib().erosion_kernel_timing(int(period) if enable else 0) != 0:
         raise StokesError(f"erosion_kernel_timing: {last_error()}")
 
 

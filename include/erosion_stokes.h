@@ -171,13 +171,19 @@ int erosion_time_matvec(int64_t npts, int64_t nbods, int64_t nouter, int64_t iba
                         const double* xx, const double* yy, const double* den, int64_t reps,
                         double* out);
 
-/* Measurement hook: enable (1) or stop (0) recording HIP events around the
- * far-field, near-pair, stress-evaluation, Gram-Schmidt and preconditioner
- * launches of every later call; enabling clears the totals.  Events are read
- * back lazily, so timing adds no host synchronisation.  Returns 0 or -1. */
+/* Measurement hook: enable (n >= 1) or stop (0) recording HIP events around
+ * the far-field, near-pair, stress-evaluation, Gram-Schmidt and
+ * preconditioner launches of every later call; enabling clears the totals.
+ * With n > 1 the launches of a GMRES iteration are timed in one iteration of
+ * every n, each standing for n (an event record costs the stream a few
+ * microseconds: timing every iteration slows a solve by about a tenth);
+ * launches outside the iterations are always timed.  Events are read back
+ * lazily, so timing adds no host synchronisation.  Returns 0 or -1. */
 int erosion_kernel_timing(int enable);
 
-/* Totals since timing was enabled, three values per family in the order
+/* Totals since timing was enabled (sampled launches weighted by their period,
+ * so that out[3i] / out[3i+1] is the measured mean duration and the totals
+ * estimate all launches), three values per family in the order
  * far field (FP64, direct), near pairs, stress evaluation, Gram-Schmidt,
  * preconditioner, far field (FP32, direct), cluster far field (whole
  * application), its direct sums (k_fmm_direct), its multipole and local

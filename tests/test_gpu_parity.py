@@ -211,6 +211,40 @@ def test_solve_record_and_fp64_floor(golden):
     assert 0.0 < fl < 1e-12 and miss["true_relres"] < 1000 * fl
 
 
+def test_sampled_kernel_timing(golden):
+    """Kernel timing with a period of 4 records one GMRES iteration's launches
+    in four, each standing for four.  The estimated totals match those of
+    timing every launch: launch counts within the last incomplete period (3),
+    work within what the Gram-Schmidt passes' growth over a period allows, and
+    the measured per-launch means agree."""
+    g = golden("bodies10_n128")
+    n, m, no, xx, yy = _case(g)
+    stokes.set_preconditioner(stokes.PRECOND_CURVE_FOURIER)
+    kt = {}
+    try:
+        for period in (1, 4):
+            stokes.kernel_timing(True, period=period)
+            den, its = stokes.stokessolver(n, m, no, 1, 1, 0, 4000, xx, yy)
+            kt[period] = stokes.kernel_timers()
+            stokes.kernel_timing(False)
+        with pytest.raises(stokes.StokesError):
+            stokes.kernel_timing(True, period=0)
+    finally:
+        stokes.kernel_timing(False)
+        stokes.set_preconditioner(stokes.PRECOND_NONE)
+    fams = [f for f, v in kt[1].items() if v["launches"] > 0]
+    print(f"{its} iterations; families {fams}")
+    assert "gram_schmidt" in fams and "precond" in fams and len(fams) >= 3
+    for f in fams:
+        a, b = kt[1][f], kt[4][f]
+        assert abs(b["launches"] - a["launches"]) <= 3, (f, a, b)
+        assert abs(b["work"] - a["work"]) <= 0.1 * a["work"], (f, a, b)
+        ma, mb = a["ms"] / a["launches"], b["ms"] / b["launches"]
+        print(f"{f}: {a['launches']} launches, mean {ma * 1e3:.1f} us; sampled "
+              f"{b['launches']}, mean {mb * 1e3:.1f} us")
+        assert 0.6 < mb / ma < 1.6, (f, ma, mb)
+
+
 def test_unsupported_combinations_raise(golden):
     g = golden("tight8_n64")
     n, m, no, xx, yy = _case(g)
