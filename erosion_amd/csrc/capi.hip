@@ -607,6 +607,19 @@ int erosion_set_farfield_shape(int index) {
 
 int erosion_farfield_shape_count(void) { return stokes::farfield_shape_count(); }
 
+int erosion_set_fmm_one_stream(int on) {
+    try {
+        if (on != 0 && on != 1) return -1;
+        Context& c = stokes::ctx();
+        STOKES_HIP(hipStreamSynchronize(c.stream));
+        if (c.fmm.side) STOKES_HIP(hipStreamSynchronize(c.fmm.side));
+        c.fmm.one_stream = on != 0;
+        return 0;
+    } catch (const std::exception& e) {
+        return record_error("erosion_set_fmm_one_stream", e.what());
+    }
+}
+
 int erosion_set_farfield_method(int method) {
     try {
         if (method != 0 && method != 1) return -1;

@@ -253,9 +253,12 @@ struct TileFmm {
     bool stress_wall = false;
     long long stress_direct_pairs = 0;  // (target point, source node) pairs of the near curves
     // the expansions run on a second stream (default priority) beside the
-    // direct sums: fork and join events on the context's stream
+    // direct sums: fork and join events on the context's stream; one_stream:
+    // on the context's stream, before them (hardware-counter passes, which
+    // serialize dispatches, then see no waits between streams)
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    bool one_stream = false;
 };
 
 // Per-solve GMRES state kept on the device.

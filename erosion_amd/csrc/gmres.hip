@@ -13,7 +13,7 @@
 // streaming kernels (all dots at once, then one fused update): classical
 // Gram-Schmidt with a second pass when ||w|| drops below ||w_in||/sqrt(2)
 // (the Daniel-Gragg-Kaufman-Stewart test).  The Givens update applies the k
-// stored rotations with a block-wide scan of the affine carry recurrence
+// stored rotations with a one-wave scan of the affine carry recurrence
 // c_{i+1} = s_i c_i + c_i' h_{i+1} instead of a k-step serial loop.
 //
 // The host enqueues chunks of iterations and reads the device state once per
@@ -388,85 +388,80 @@ __global__ __launch_bounds__(kBlk) void k_combine2(double* __restrict__ u, doubl
 // Givens update of Hessenberg column k = st->k (DHEQR, IJOB>1) and the stop
 // test: hsum[0..k] the column, sub its subdiagonal entry.  The k stored
 // rotations act on (col[i], col[i+1]) in order; the value carried into row i
-// obeys carry_{i+1} = s_i carry_i + c_i h_{i+1}, an affine recurrence
-// evaluated with a block-wide scan over segments.  Every thread of the
-// (kGivT-thread) block calls it.
-constexpr int kGivT = 1024;
+// obeys carry_{i+1} = s_i carry_i + c_i h_{i+1}, an affine recurrence: each
+// lane of the block's first wave composes the maps of its segment of
+// ceil(k / 64) rotations, a shuffle scan over the wave composes the
+// segments, and each lane then applies its segment from the carry entering
+// it.  Every thread of the (kGivT-thread) block calls it.
+constexpr int kGivT = 256;
 __device__ void givens_column(double* __restrict__ H, int ldh, double* __restrict__ q,
                               const double* __restrict__ hsum, double sub,
-                              GmresState* __restrict__ st, double tol, int maxl, double* sA,
-                              double* sB) {
+                              GmresState* __restrict__ st, double tol, int maxl) {
     const int k = st->k;
     double* __restrict__ col = H + static_cast<size_t>(k) * ldh;
     const int tid = threadIdx.x;
-    // segment of rotations for this thread
-    const int L = (k + kGivT - 1) / kGivT;
-    const int i0 = min(k, tid * L), i1 = min(k, i0 + L);
-    double A = 1.0, B = 0.0;
-    for (int i = i0; i < i1; ++i) {
-        const double c = q[2 * i], s = q[2 * i + 1];
-        A = s * A;
-        B = __builtin_fma(s, B, c * hsum[i + 1]);
-    }
-    sA[tid] = A;
-    sB[tid] = B;
-    __syncthreads();
-    // inclusive scan of maps (composition: later after earlier)
-    for (int off = 1; off < kGivT; off <<= 1) {
-        double a2 = 0, b2 = 0;
-        const bool has = tid >= off;
-        if (has) {
-            a2 = sA[tid - off];
-            b2 = sB[tid - off];
+    if (tid < 64) {
+        // segment of rotations for this lane
+        const int L = (k + 63) / 64;
+        const int i0 = min(k, tid * L), i1 = min(k, i0 + L);
+        double A = 1.0, B = 0.0;
+        for (int i = i0; i < i1; ++i) {
+            const double c = q[2 * i], s = q[2 * i + 1];
+            A = s * A;
+            B = __builtin_fma(s, B, c * hsum[i + 1]);
         }
-        __syncthreads();
-        if (has) {
-            const double a1 = sA[tid], b1 = sB[tid];
-            sA[tid] = a1 * a2;
-            sB[tid] = __builtin_fma(a1, b2, b1);
+        // inclusive scan of the maps x -> A x + B (composition: later after
+        // earlier), then the composite of the earlier lanes applied to h_0
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const double a2 = __shfl_up(A, off), b2 = __shfl_up(B, off);
+            if (tid >= off) {
+                B = __builtin_fma(A, b2, B);
+                A = A * a2;
+            }
         }
-        __syncthreads();
-    }
-    const double h0 = hsum[0];
-    double carry = tid == 0 ? h0 : __builtin_fma(sA[tid - 1], h0, sB[tid - 1]);
-    for (int i = i0; i < i1; ++i) {
-        const double c = q[2 * i], s = q[2 * i + 1];
-        const double hn = hsum[i + 1];
-        col[i] = c * carry - s * hn;
-        carry = __builtin_fma(s, carry, c * hn);
-    }
-    // the thread owning the last rotation (or thread 0 when k == 0) finishes
-    const bool last = (k == 0) ? (tid == 0) : (i0 < i1 && i1 == k);
-    if (last) {
-        const double t1 = carry, t2 = sub;
-        double c, s;
-        if (t2 == 0.0) {
-            c = 1.0;
-            s = 0.0;
-        } else if (fabs(t2) >= fabs(t1)) {
-            const double t = t1 / t2;
-            s = -1.0 / sqrt(1.0 + t * t);
-            c = -s * t;
-        } else {
-            const double t = t2 / t1;
-            c = 1.0 / sqrt(1.0 + t * t);
-            s = -c * t;
+        const double Ap = __shfl_up(A, 1), Bp = __shfl_up(B, 1);
+        const double h0 = hsum[0];
+        double carry = tid == 0 ? h0 : __builtin_fma(Ap, h0, Bp);
+        for (int i = i0; i < i1; ++i) {
+            const double c = q[2 * i], s = q[2 * i + 1];
+            const double hn = hsum[i + 1];
+            col[i] = c * carry - s * hn;
+            carry = __builtin_fma(s, carry, c * hn);
         }
-        q[2 * k] = c;
-        q[2 * k + 1] = s;
-        const double rkk = c * t1 - s * t2;
-        col[k] = rkk;
-        col[k + 1] = t2;
-        st->k = k + 1;
-        if (rkk == 0.0) {  // DHEQR INFO = k: breakdown
-            st->done = 3;
-        } else {
-            st->prod = st->prod * s;
-            st->rho = fabs(st->prod * st->r0nrm);
-            if (st->rho / st->bnrm <= tol)
-                st->done = 1;
-            else if (k + 1 >= maxl)
-                st->done = 2;
+        // the thread owning the last rotation (or thread 0 when k == 0) finishes
+        const bool last = (k == 0) ? (tid == 0) : (i0 < i1 && i1 == k);
+        if (last) {
+            const double t1 = carry, t2 = sub;
+            double c, s;
+            if (t2 == 0.0) {
+                c = 1.0;
+                s = 0.0;
+            } else if (fabs(t2) >= fabs(t1)) {
+                const double t = t1 / t2;
+                s = -1.0 / sqrt(1.0 + t * t);
+                c = -s * t;
+            } else {
+                const double t = t2 / t1;
+                c = 1.0 / sqrt(1.0 + t * t);
+                s = -c * t;
+            }
+            q[2 * k] = c;
+            q[2 * k + 1] = s;
+            const double rkk = c * t1 - s * t2;
+            col[k] = rkk;
+            col[k + 1] = t2;
+            st->k = k + 1;
+            if (rkk == 0.0) {  // DHEQR INFO = k: breakdown
+                st->done = 3;
+            } else {
+                st->prod = st->prod * s;
+                st->rho = fabs(st->prod * st->r0nrm);
+                if (st->rho / st->bnrm <= tol)
+                    st->done = 1;
+                else if (k + 1 >= maxl)
+                    st->done = 2;
+            }
         }
     }
     __syncthreads();
@@ -530,7 +525,7 @@ __global__ __launch_bounds__(kGivT) void k_dcgs(const double* __restrict__ hu,
         for (int i = tid; i < j; i += kGivT) colr[i] = __builtin_fma(est, ca[i], colr[i]);
         if (tid == 0) colr[j] = est * rn;
         __syncthreads();
-        givens_column(H, ldh, q, colr, est * rn, st, tol, maxl, sA, sB);
+        givens_column(H, ldh, q, colr, est * rn, st, tol, maxl);
         if (st->done) return;  // block-uniform after givens_column's barrier
     }
     if (last) return;

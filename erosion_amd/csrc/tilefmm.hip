@@ -1285,17 +1285,20 @@ void tilefmm_apply(Context& c, const double* den, int* wall_nsplit, int* body_ns
     // the direct sums (vector ALU) run on the context's stream
     // timers count the operator's applications, not true-residual ones
     const bool timed = c.time_kernels && !compensated;
-    STOKES_HIP(hipEventRecord(F.fork, s));
-    STOKES_HIP(hipStreamWaitEvent(F.side, F.fork, 0));
-    if (timed) c.t_fmm_expand.start(F.side);
+    hipStream_t side = F.one_stream ? s : F.side;
+    if (!F.one_stream) {
+        STOKES_HIP(hipEventRecord(F.fork, s));
+        STOKES_HIP(hipStreamWaitEvent(side, F.fork, 0));
+    }
+    if (timed) c.t_fmm_expand.start(side);
     const double* stk = den != nullptr ? den + 2 * static_cast<size_t>(g.no) +
                                              2 * static_cast<size_t>(g.n) * g.m
                                        : nullptr;
     if (compensated)
-        k_fmm_p2m<true><<<nc, 256, 0, F.side>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
+        k_fmm_p2m<true><<<nc, 256, 0, side>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
                                                 F.cr.p, F.cl.p, stk, F.nwall, F.mp.p, F.mlog.p);
     else
-        k_fmm_p2m<false><<<nc, 256, 0, F.side>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
+        k_fmm_p2m<false><<<nc, 256, 0, side>>>(g.x.p, g.y.p, g.mx.p, g.my.p, c.ex.p, c.ey.p,
                                                  F.cr.p, F.cl.p, stk, F.nwall, F.mp.p, F.mlog.p);
     STOKES_LAUNCH_CHECK();
     // an operator application translates the parents of far groups (their
@@ -1303,18 +1306,18 @@ void tilefmm_apply(Context& c, const double* den, int* wall_nsplit, int* body_ns
     // keeps the one-level lists, whose multipoles carry error-free sums
     const bool two_level = !compensated && F.npar > 0;
     if (two_level) {
-        k_fmm_m2m<<<F.npar, 256, 0, F.side>>>(F.cr.p, F.par_ptr.p, F.par_idx.p, F.binm.p, nc,
+        k_fmm_m2m<<<F.npar, 256, 0, side>>>(F.cr.p, F.par_ptr.p, F.par_idx.p, F.binm.p, nc,
                                              F.mp.p, F.mlog.p);
         STOKES_LAUNCH_CHECK();
     }
-    k_fmm_m2l<2, true, true><<<dim3(nc, 2), 256, 0, F.side>>>(
+    k_fmm_m2l<2, true, true><<<dim3(nc, 2), 256, 0, side>>>(
         F.cr.p, F.mp.p, two_level ? F.far2_ptr.p : F.far_ptr.p,
         two_level ? F.far2_idx.p : F.far_idx.p, F.bin.p, F.mlog.p,
         two_level ? F.logd2.p : F.logd.p, F.lc.p);
     STOKES_LAUNCH_CHECK();
     if (timed)
-        c.t_fmm_expand.stop(F.side, static_cast<double>(two_level ? F.far2_pairs : F.far_pairs));
-    STOKES_HIP(hipEventRecord(F.join, F.side));
+        c.t_fmm_expand.stop(side, static_cast<double>(two_level ? F.far2_pairs : F.far_pairs));
+    if (!F.one_stream) STOKES_HIP(hipEventRecord(F.join, side));
     if (timed) c.t_fmm_direct.start(s);
     // bodies: 4 targets per lane above 128 points, else 2; wall segments:
     // one per lane, a full wave per segment
@@ -1325,7 +1328,7 @@ void tilefmm_apply(Context& c, const double* den, int* wall_nsplit, int* body_ns
         direct(c, 2, round32(cdiv_int(n, 2)), 0, F.work_body);
     direct(c, 1, kWallSeg, F.work_body, F.work_wall);
     if (timed) c.t_fmm_direct.stop(s, static_cast<double>(F.direct_pairs));
-    STOKES_HIP(hipStreamWaitEvent(s, F.join, 0));
+    if (!F.one_stream) STOKES_HIP(hipStreamWaitEvent(s, F.join, 0));
     // a block of 256 points spans at most 5 wall segments and, for bodies of
     // 64 points or more, 5 bodies: their locals fit the LDS stage
     static_assert(kBlk / kWallSeg + 1 + kBlk / 64 + 1 <= kL2pStage, "L2P stage too small");

@@ -15,7 +15,11 @@ Runs, on one workload (default: 100 bodies x 256 points):
   curve's nodes x 14 doubles, plus 16 bytes written);
 * one shear-stress evaluation (computeshearstress_) under the library's kernel
   timers, reporting the barycentric stress pass's FLOP rate (130 FLOP per
-  target-node pair).
+  target-node pair);
+* with --cluster, a fixed number of applications of the cluster far field
+  (multipoles, translations and direct sums of well-separated bodies and wall
+  segments), with --one-stream its expansions on the library stream ahead of
+  the direct sums (for hardware-counter passes, which serialize dispatches).
 """
 from __future__ import annotations
 
@@ -44,6 +48,12 @@ def main(argv=None) -> int:
                     help="time every far-field launch shape (FP64 and FP32)")
     ap.add_argument("--fp64-rate", action="store_true",
                     help="the FP64 FMA rate the chip sustains at 4, 8 and 16 waves per SIMD")
+    ap.add_argument("--no-direct", action="store_true",
+                    help="skip the all-pairs FP64 and FP32 far field")
+    ap.add_argument("--cluster", action="store_true",
+                    help="time the cluster far field (FP64)")
+    ap.add_argument("--one-stream", action="store_true",
+                    help="with --cluster: expansions on the library stream")
     a = ap.parse_args(argv)
     L = stokes.lib()
     if a.fp64_rate:
@@ -73,8 +83,10 @@ def main(argv=None) -> int:
     n, m, no, xx, yy = workload_xy(WORKLOADS[a.workload])
     print(f"workload {a.workload}: {m} bodies x {n} points, wall {no}", flush=True)
     eta = np.random.default_rng(7).standard_normal(2 * no + 2 * n * m + 3 * m)
-    for mode, name, peak in ((stokes.PRECISION_FP64, "FP64", PEAK_FP64_TFLOPS),
-                             (stokes.PRECISION_FP32_FARFIELD, "FP32", PEAK_FP32_TFLOPS)):
+    direct_modes = () if a.no_direct else (
+        (stokes.PRECISION_FP64, "FP64", PEAK_FP64_TFLOPS),
+        (stokes.PRECISION_FP32_FARFIELD, "FP32", PEAK_FP32_TFLOPS))
+    for mode, name, peak in direct_modes:
         stokes.set_precision(mode)
         r = stokes.time_matvec(n, m, no, xx, yy, eta, reps=a.reps)
         tf = 20.0 * r["pairs"] / (r["farfield_ms"] * 1e-3) / 1e12
@@ -96,6 +108,17 @@ def main(argv=None) -> int:
                       f"{r['splits']} splits", flush=True)
         stokes.set_farfield_shape(-1)
     stokes.set_precision(stokes.PRECISION_FP64)
+    if a.cluster:
+        stokes.set_fmm_one_stream(a.one_stream)
+        stokes.set_farfield_shape(stokes.farfield_shape_count() - 1)
+        try:
+            r = stokes.time_matvec(n, m, no, xx, yy, eta, reps=a.reps)
+        finally:
+            stokes.set_farfield_shape(-1)
+            stokes.set_fmm_one_stream(False)
+        print(f"cluster far field ({'one stream' if a.one_stream else 'two streams'}): "
+              f"{r['farfield_ms']:.4f} ms per application, operator {r['matvec_ms']:.4f} ms",
+              flush=True)
     if not a.no_stress:
         stokes.set_gmres_tol(1e-8)
         stokes.set_preconditioner(stokes.PRECOND_CURVE_FOURIER)

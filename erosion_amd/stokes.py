@@ -335,6 +335,17 @@ def farfield_shape_count() -> int:
     return int(lib().erosion_farfield_shape_count())
 
 
+def set_fmm_one_stream(on: bool) -> None:
+    """Run the cluster far field's expansions on the library stream ahead of
+    its direct sums (a measurement hook for hardware-counter passes; results
+    are unchanged)."""
+    L = lib()
+    L.erosion_set_fmm_one_stream.argtypes = [ctypes.c_int]
+    L.erosion_set_fmm_one_stream.restype = ctypes.c_int
+    if L.erosion_set_fmm_one_stream(1 if on else 0) != 0:
+        raise StokesError(f"erosion_set_fmm_one_stream: {last_error()}")
+
+
 def set_graphs(on: bool) -> None:
     """Capture GMRES iteration chunks as HIP graphs (the second time a chunk
     comes under the same sizes and buffers) and replay them on later solves

@@ -236,6 +236,14 @@ int erosion_solve_batch(int64_t ngeom, const int64_t* npts, const int64_t* nbods
  * ms = mean milliseconds per copy.  Returns 0 or -1. */
 int erosion_measure_copy(int64_t bytes, int64_t reps, double* ms);
 
+/* Measurement hook (no counterpart in the reference): on = 1 runs the
+ * cluster far field's multipole and translation kernels on the library's
+ * stream ahead of its direct sums instead of on a second stream beside them
+ * (same kernels, same results), for hardware-counter passes, which serialize
+ * every dispatch; on = 0 (default) restores the two streams.  Returns 0 or
+ * -1. */
+int erosion_set_fmm_one_stream(int on);
+
 /* Measurement hook: the FP64 vector rate the chip sustains -- `blocks` blocks
  * of 256 threads, each thread `iters` iterations of 8 independent FP64 FMA
  * chains, on the library's stream; ms = milliseconds of the third of three

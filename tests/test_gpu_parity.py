@@ -245,6 +245,26 @@ def test_sampled_kernel_timing(golden):
         assert 0.6 < mb / ma < 1.6, (f, ma, mb)
 
 
+def test_cluster_far_field_one_stream_identical():
+    """The counter-pass hook runs the cluster far field's expansions on the
+    library stream ahead of the direct sums: the same kernels on the same
+    data, so the operator's output is identical bit for bit."""
+    n, m, no, xx, yy = workload_xy(WORKLOADS["bodies100_n256"], nbodies=40)
+    eta = np.random.default_rng(41).standard_normal(2 * no + 2 * n * m + 3 * m)
+    stokes.set_farfield_shape(stokes.farfield_shape_count() - 1)
+    try:
+        v2 = stokes.matvec(n, m, no, xx, yy, eta)
+        stokes.set_fmm_one_stream(True)
+        v1 = stokes.matvec(n, m, no, xx, yy, eta)
+        stokes.set_fmm_one_stream(False)
+        v2b = stokes.matvec(n, m, no, xx, yy, eta)
+    finally:
+        stokes.set_fmm_one_stream(False)
+        stokes.set_farfield_shape(-1)
+    assert np.array_equal(v1, v2) and np.array_equal(v2b, v2)
+    assert stokes.lib().erosion_set_fmm_one_stream(2) == -1
+
+
 def test_unsupported_combinations_raise(golden):
     g = golden("tight8_n64")
     n, m, no, xx, yy = _case(g)
