@@ -197,7 +197,7 @@ def main() -> int:
     ap.add_argument("--body-modes", type=int, default=None,
                     help="accelerated variant: Fourier modes per body in the preconditioner's "
                          "coarse space (default: the library's)")
-    ap.add_argument("--timing-period", type=int, default=16,
+    ap.add_argument("--timing-period", type=int, default=32,
                     help="time the kernels of one GMRES iteration in this many (each HIP event "
                          "record costs the stream a few microseconds; 1 times every launch)")
     ap.add_argument("--graphs", action="store_true",

@@ -169,7 +169,8 @@ __global__ void k_scale_row(double* __restrict__ v, size_t n, const GmresState* 
 // solve).  H_raw keeps the Arnoldi columns, H their rotated (R) form.
 
 // Partial dots of rows [r0, r0+kRows) of V with u and with w over one chunk:
-// hu[r][chunk], hw[r][chunk].
+// hu[chunk][r], hw[chunk][r] (chunk-major, nrows per chunk, so that the sums
+// over chunks read consecutive rows in consecutive lanes).
 __global__ __launch_bounds__(kBlk) void k_dots2(const double* __restrict__ V, size_t npad,
                                                 const double* __restrict__ u,
                                                 const double* __restrict__ w, int nrows,
@@ -203,8 +204,8 @@ __global__ __launch_bounds__(kBlk) void k_dots2(const double* __restrict__ V, si
         const double su = block_sum(au[r], sh);
         const double sw = block_sum(aw[r], sh);
         if (threadIdx.x == 0 && r0 + r < nrows) {
-            hu[static_cast<size_t>(r0 + r) * nchunk + c] = su;
-            hw[static_cast<size_t>(r0 + r) * nchunk + c] = sw;
+            hu[static_cast<size_t>(c) * nrows + r0 + r] = su;
+            hw[static_cast<size_t>(c) * nrows + r0 + r] = sw;
         }
     }
 }
@@ -393,7 +394,7 @@ __global__ __launch_bounds__(kBlk) void k_combine2(double* __restrict__ u, doubl
 // ceil(k / 64) rotations, a shuffle scan over the wave composes the
 // segments, and each lane then applies its segment from the carry entering
 // it.  Every thread of the (kGivT-thread) block calls it.
-constexpr int kGivT = 256;
+constexpr int kGivT = 1024;
 __device__ void givens_column(double* __restrict__ H, int ldh, double* __restrict__ q,
                               const double* __restrict__ hsum, double sub,
                               GmresState* __restrict__ st, double tol, int maxl) {
@@ -479,28 +480,34 @@ __global__ __launch_bounds__(kGivT) void k_dcgs(const double* __restrict__ hu,
                                                 double* __restrict__ q, double* __restrict__ ca,
                                                 double* __restrict__ cb,
                                                 GmresState* __restrict__ st, double tol, int maxl) {
-    __shared__ double sA[kGivT], sB[kGivT];
+    constexpr int kWaves = kGivT / 64;
+    __shared__ double sA[kGivT];
+    __shared__ double sP[kWaves][64];
     if (st->done) return;
-    const int tid = threadIdx.x;
-    // chunk partials in order, loaded 16 at a time (indices clamped, values
-    // past nchunk replaced by 0) so that each sum waits on L2 once per 16
-    auto chunk_sum = [&](const double* __restrict__ hp) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the dots pass's rows: 0..j (u's) and, unless last, 0..j+1 (w's); each
+    // sum over the chunks in chunk order, loaded 16 at a time (indices
+    // clamped, values past nchunk replaced by 0) so that it waits on L2 once
+    // per 16; thread t sums u's row t, then w's rows follow
+    const int nrows = last ? j + 1 : j + 2;
+    const int nsum = (j + 1) + (last ? 0 : j + 2);
+    for (int t = tid; t < nsum; t += kGivT) {
+        const bool wrow = t > j;
+        const int r = wrow ? t - (j + 1) : t;
+        const double* __restrict__ hp = (wrow ? hw : hu) + r;
         double v = 0.0;
         for (int c0 = 0; c0 < nchunk; c0 += 16) {
-            double t[16];
+            double tv[16];
 #pragma unroll
             for (int e = 0; e < 16; ++e) {
-                const double x = hp[min(c0 + e, nchunk - 1)];
-                t[e] = c0 + e < nchunk ? x : 0.0;
+                const double x = hp[static_cast<size_t>(min(c0 + e, nchunk - 1)) * nrows];
+                tv[e] = c0 + e < nchunk ? x : 0.0;
             }
 #pragma unroll
-            for (int e = 0; e < 16; ++e) v += t[e];
+            for (int e = 0; e < 16; ++e) v += tv[e];
         }
-        return v;
-    };
-    for (int r = tid; r <= j; r += kGivT) ca[r] = chunk_sum(hu + static_cast<size_t>(r) * nchunk);
-    if (!last)
-        for (int r = tid; r <= j + 1; r += kGivT) cb[r] = chunk_sum(hw + static_cast<size_t>(r) * nchunk);
+        (wrow ? cb : ca)[r] = v;
+    }
     __syncthreads();
     double paa = 0.0, pab = 0.0;
     for (int r = tid; r < j; r += kGivT) {
@@ -530,34 +537,43 @@ __global__ __launch_bounds__(kGivT) void k_dcgs(const double* __restrict__ hu,
     }
     if (last) return;
     // provisional column j: (g - H a) / r, H upper Hessenberg (H[i][c], c >=
-    // i - 1); four neighbouring lanes share a row, each taking every fourth
-    // column, with loads 8 columns at a time
+    // i - 1).  Rows in tiles of 64 (a lane per row); in each tile wave w takes
+    // the columns w, w + 16, ... of the tile's range, 8 loads at a time, each
+    // a coalesced column segment, and the waves' partials are added in wave
+    // order.
     const double gj = (cb[j] - sab) / rn;
     double pgg = 0.0;
     double* colj = Hraw + static_cast<size_t>(j) * ldh;
-    for (int base = 0; base <= j; base += kGivT / 4) {  // uniform over the block
-        const int i = base + tid / 4, part = tid % 4;
+    for (int i0 = 0; i0 <= j; i0 += 64) {  // uniform over the block
+        const int i = i0 + lane;
+        const int cbeg = max(0, i0 - 1);
         double ha = 0.0;
-        if (i <= j) {
-            for (int c0 = max(0, i - 1) + part; c0 < j; c0 += 32) {
-                double hv[8], av[8];
+        for (int c0 = cbeg + wave; c0 < j; c0 += 8 * kWaves) {
+            double hv[8], av[8];
 #pragma unroll
-                for (int t = 0; t < 8; ++t) {
-                    const int cc = min(c0 + 4 * t, j - 1);
-                    hv[t] = Hraw[static_cast<size_t>(cc) * ldh + i];
-                    av[t] = c0 + 4 * t < j ? ca[cc] : 0.0;
-                }
-#pragma unroll
-                for (int t = 0; t < 8; ++t) ha = __builtin_fma(hv[t], av[t], ha);
+            for (int t = 0; t < 8; ++t) {
+                // entries below the subdiagonal are never written: the
+                // loaded value itself is dropped there, not multiplied by 0
+                const int cc = min(c0 + kWaves * t, j - 1);
+                const bool use = c0 + kWaves * t < j && cc >= i - 1 && i <= j;
+                const double h = Hraw[static_cast<size_t>(cc) * ldh + min(i, j)];
+                hv[t] = use ? h : 0.0;
+                av[t] = use ? ca[cc] : 0.0;
             }
+#pragma unroll
+            for (int t = 0; t < 8; ++t) ha = __builtin_fma(hv[t], av[t], ha);
         }
-        ha += __shfl_xor(ha, 1);
-        ha += __shfl_xor(ha, 2);
-        if (part == 0 && i <= j) {
+        sP[wave][lane] = ha;
+        __syncthreads();
+        if (wave == 0 && i <= j) {
+            double h = 0.0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) h += sP[w][lane];
             const double gi = i < j ? cb[i] : gj;
-            colj[i] = (gi - ha) / rn;
+            colj[i] = (gi - h) / rn;
             pgg = __builtin_fma(gi, gi, pgg);
         }
+        __syncthreads();
     }
     const double sgg = block_sum(pgg, sA);
     if (tid == 0) {
