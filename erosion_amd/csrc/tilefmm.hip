@@ -1319,10 +1319,13 @@ void tilefmm_apply(Context& c, const double* den, int* wall_nsplit, int* body_ns
         c.t_fmm_expand.stop(side, static_cast<double>(two_level ? F.far2_pairs : F.far_pairs));
     if (!F.one_stream) STOKES_HIP(hipEventRecord(F.join, side));
     if (timed) c.t_fmm_direct.start(s);
-    // bodies: 4 targets per lane above 128 points, else 2; wall segments:
-    // one per lane, a full wave per segment
+    // bodies: 2 targets per lane up to 256 points (blocks of at most 128
+    // threads: twice the waves of 4 per lane, for the same shared-reciprocal
+    // cost within 1%), 4 above; wall segments: one per lane, a full wave per
+    // segment
+    static_assert(kMaxBodyPts <= 4 * 128, "4 targets per lane in blocks of at most 128");
     auto direct = compensated ? direct_launch<true> : direct_launch<false>;
-    if (n > 128)
+    if (n > 256)
         direct(c, 4, round32(cdiv_int(n, 4)), 0, F.work_body);
     else
         direct(c, 2, round32(cdiv_int(n, 2)), 0, F.work_body);
