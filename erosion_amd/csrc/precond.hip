@@ -36,8 +36,6 @@ constexpr int kBlk = 256;
 constexpr int kApplyBlk = 256;
 constexpr int kQC = 17;                        // basis functions per pair pass
 constexpr int kMaxModes = 32;                  // K <= 32 per curve
-constexpr int kChunkModes = 16;                // modes per register chunk in k_pc_apply
-constexpr int kChunkNF = 2 * kChunkModes + 1;  // slots per component and chunk
 constexpr int kMaxNF = 2 * kMaxModes + 1;      // functions per component
 constexpr int kMaxNQ = 2 * kMaxNF + 3;         // coarse unknowns per curve
 
@@ -439,24 +437,22 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
 // k_pc_wall sums the wall's partial coefficients, solves its coarse system
 // and expands each segment.
 //
-// Projection: each lane walks its points once per chunk of 16 modes,
-// generating the basis functions by the rotation
-// e^{i(k+1)theta} = e^{ik theta} e^{i theta} (no table loads), and keeps the
-// chunk's coefficient partials in registers; the partials meet in LDS and are
-// summed in two conflict-free stages, all of a chunk's at once.  The inverse coarse
-// matrices are stored transposed, so d = G^{-1} c reads them coalesced with
-// one row per lane.
-constexpr int kVals = kChunkNF;  // coefficients per LDS round: half a chunk (~74 KB, two blocks per CU)
-constexpr int kSeg = 16;
+// Projection: a lane per (mode, slice of the points), the basis functions
+// generated by rotation from the slice's first point (no table loads), four
+// partials per lane meeting in LDS (pc_project).  The inverse coarse matrices
+// are stored transposed, so d = G^{-1} c reads them coalesced with one row per
+// lane.
 
 // geometry arrays the apply's reductions read (point order)
 struct SumGeom {
     const double *x, *y, *px, *py, *speed;
 };
 
+// part: the staged point values (rows 0, 1) and the slices' partials (rows
+// 2-5) of the projection; the coarse solve's partials (row 0)
 struct ApplyLds {
-    double part[kVals][kApplyBlk];
-    double red[kVals][kSeg];
+    double part[6][kApplyBlk];
+    double red[3][kApplyBlk / 64];
     double cc[kMaxNQ], dd[kMaxNQ];
 };
 
@@ -464,84 +460,85 @@ struct ApplyLds {
 // (block-wide; every thread calls it).
 // With scale, r is first multiplied by sc in place (the Krylov vector's
 // normalisation, left to this kernel by GMRES).
+// The points' values are staged in LDS, kApplyBlk at a time.  Lane l takes
+// mode k = l % (K + 1) over the (l / (K + 1))-th of nsub = kApplyBlk / (K + 1)
+// contiguous slices of each staged pass: from e^{i k theta} at the slice's
+// first point (the angle reduced mod 2 pi exactly) it steps by
+// e^{i k 2 pi / len} to each next point, summing cos and sin times both
+// components.  Only the slices' four partials per mode meet in LDS, and they
+// are added in slice order.
 __device__ __forceinline__ void pc_project(ApplyLds& L, int len, int nf, int i0, int i1,
                                            double* r, int xoff, int yoff, bool scale, double sc,
                                            double* cc) {
-    constexpr int kChunkNC = 2 * kChunkNF;
     const int tid = threadIdx.x;
     const int K = (nf - 1) / 2;
-    const double c0 = 1.0 / sqrt(static_cast<double>(len));
-    const double c1 = sqrt(2.0 / static_cast<double>(len));
-    const int nchunks = K > 0 ? (K + kChunkModes - 1) / kChunkModes : 1;
-    for (int ch = 0; ch < nchunks; ++ch) {
-        // chunk ch holds modes kb+1 .. kb+16 in slots 2l-1 (cos), 2l (sin),
-        // l = 1..16, and (chunk 0 only) the mean in slot 0; x slots first
-        const int kb = ch * kChunkModes;
-        double ax[kChunkNF], ay[kChunkNF];
-#pragma unroll
-        for (int v = 0; v < kChunkNF; ++v) ax[v] = ay[v] = 0.0;
-        for (int i = i0 + tid; i < i1; i += kApplyBlk) {
+    const int nmode = K + 1;
+    const int npt = i1 - i0;
+    double* sx = L.part[0];
+    double* sy = L.part[1];
+    double* pp = L.part[2];  // [slice][mode][cos x, sin x, cos y, sin y]: rows 2-5
+    const int nsub = kApplyBlk / nmode;
+    const int k = tid % nmode, sl = tid / nmode;
+    double cx = 0.0, sxs = 0.0, cy = 0.0, sys = 0.0;
+    for (int p0 = 0; p0 < npt; p0 += kApplyBlk) {  // uniform over the block
+        const int pn = min(kApplyBlk, npt - p0);
+        __syncthreads();  // the previous pass's reads are done
+        if (tid < pn) {
+            const int i = i0 + p0 + tid;
             double rx = r[xoff + i], ry = r[yoff + i];
-            if (scale && ch == 0) {
+            if (scale) {
                 rx *= sc;
                 ry *= sc;
                 r[xoff + i] = rx;
                 r[yoff + i] = ry;
             }
-            double s1, c1t;
-            sincospi(2.0 * static_cast<double>(i) / static_cast<double>(len), &s1, &c1t);
-            double ck = 1.0, sk = 0.0;
-            if (ch == 0) {
-                ax[0] = __builtin_fma(c0, rx, ax[0]);
-                ay[0] = __builtin_fma(c0, ry, ay[0]);
-            } else {
-                // start from e^{i kb theta}, the angle reduced mod 2 pi exactly
-                const long long ki = (static_cast<long long>(kb) * i) % len;
-                sincospi(2.0 * static_cast<double>(ki) / static_cast<double>(len), &sk, &ck);
-            }
-#pragma unroll
-            for (int l = 1; l <= kChunkModes; ++l) {
-                // modes above K stay zero (their slots are never read)
-                const double cn = __builtin_fma(ck, c1t, -sk * s1);
-                sk = __builtin_fma(sk, c1t, ck * s1);
-                ck = cn;
-                const double pc = c1 * ck, ps = c1 * sk;
-                ax[2 * l - 1] = __builtin_fma(pc, rx, ax[2 * l - 1]);
-                ax[2 * l] = __builtin_fma(ps, rx, ax[2 * l]);
-                ay[2 * l - 1] = __builtin_fma(pc, ry, ay[2 * l - 1]);
-                ay[2 * l] = __builtin_fma(ps, ry, ay[2 * l]);
-            }
+            sx[tid] = rx;
+            sy[tid] = ry;
         }
-#pragma unroll
-        for (int j = 0; j * kVals < kChunkNC; ++j) {
-#pragma unroll
-            for (int v = 0; v < kVals; ++v) {
-                const int slot = j * kVals + v;
-                if (slot < kChunkNC) L.part[v][tid] = slot < kChunkNF ? ax[slot] : ay[slot - kChunkNF];
-            }
-            __syncthreads();
-            for (int job = tid; job < kVals * kSeg; job += kApplyBlk) {
-                const int v = job / kSeg, sg = job - v * kSeg;
-                double s = 0.0;
-                for (int k = sg; k < kApplyBlk; k += kSeg) s += L.part[v][k];
-                L.red[v][sg] = s;
-            }
-            __syncthreads();
-            if (tid < kVals) {
-                const int slot = j * kVals + tid;
-                const int comp = slot / kChunkNF, pl = slot - comp * kChunkNF;
-                // local slot pl of chunk ch is coefficient 2 kb + pl of the component
-                const int pg = 2 * kb + pl;
-                if (slot < kChunkNC && (pl > 0 || ch == 0) && pg < nf) {
-                    double s = 0.0;
-#pragma unroll
-                    for (int sg = 0; sg < kSeg; ++sg) s += L.red[tid][sg];
-                    cc[comp * nf + pg] = s;
+        __syncthreads();
+        if (sl < nsub) {
+            const int span = (pn + nsub - 1) / nsub;
+            const int a = min(pn, sl * span), b = min(pn, a + span);
+            if (a < b) {
+                const long long ka = (static_cast<long long>(k) * (i0 + p0 + a)) % len;
+                double sk, ck, s1, c1t;
+                sincospi(2.0 * static_cast<double>(ka) / static_cast<double>(len), &sk, &ck);
+                sincospi(2.0 * static_cast<double>(k) / static_cast<double>(len), &s1, &c1t);
+                for (int i = a; i < b; ++i) {
+                    const double vx = sx[i], vy = sy[i];
+                    cx = __builtin_fma(ck, vx, cx);
+                    sxs = __builtin_fma(sk, vx, sxs);
+                    cy = __builtin_fma(ck, vy, cy);
+                    sys = __builtin_fma(sk, vy, sys);
+                    const double cn = __builtin_fma(ck, c1t, -sk * s1);
+                    sk = __builtin_fma(sk, c1t, ck * s1);
+                    ck = cn;
                 }
             }
-            __syncthreads();
         }
     }
+    // slices x modes <= kApplyBlk lanes, four partials each: rows 2-5
+    if (sl < nsub) {
+        double* o = pp + 4 * (sl * nmode + k);
+        o[0] = cx;
+        o[1] = sxs;
+        o[2] = cy;
+        o[3] = sys;
+    }
+    __syncthreads();
+    if (tid < 4 * nmode) {
+        const int kk = tid / 4, q = tid % 4;
+        double v = 0.0;
+        for (int j = 0; j < nsub; ++j) v += pp[4 * (j * nmode + kk) + q];
+        const int comp = q / 2;
+        const bool sn = (q & 1) != 0;
+        if (kk == 0) {
+            if (!sn) cc[comp * nf] = v / sqrt(static_cast<double>(len));
+        } else {
+            cc[comp * nf + (sn ? 2 * kk : 2 * kk - 1)] = v * sqrt(2.0 / static_cast<double>(len));
+        }
+    }
+    __syncthreads();
 }
 
 // d = G^{-1} c with G^{-1} stored transposed (Gt[k nq + j]); dd = d + 2 c on
