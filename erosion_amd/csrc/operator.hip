@@ -774,15 +774,21 @@ __global__ __launch_bounds__(kBlk) void k_stokes_logs(const double* __restrict__
     lg[e] = log(rx * rx + ry * ry);
 }
 
-// kAsmSub lanes per target: the stokeslet/rotlet sum over the bodies and the
+// SUB lanes per target: the stokeslet/rotlet sum over the bodies and the
 // sum of the far field's per-split partials are split across neighbouring
-// lanes and reduced with cross-lane moves, so a launch has kAsmSub npts / 64
+// lanes and reduced with cross-lane moves, so a launch has SUB npts / 64
 // waves to hide the latency of the logarithms and of the partial-sum loads.
+// kAsmSub for the direct far field (every body's stokeslet, up to a hundred
+// split partials per point); kAsmSubNear with the cluster far field, whose
+// points sum a few near bodies and one partial: there a short grid of fatter
+// lanes runs in one round of blocks instead of several rounds of
+// mostly idle lanes waiting on dependent loads.
 constexpr int kAsmSub = 16;
+constexpr int kAsmSubNear = 4;
 // COMP (a true-residual evaluation): the far field's split sums come with
 // their corrections (part_cu, part_cv), and every sum of the assembly is kept
 // as a value and a correction (TwoSum), rounded once at the end.
-template <bool COMP>
+template <bool COMP, int SUB>
 __global__ __launch_bounds__(kBlk) void k_assemble(
     const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ px,
     const double* __restrict__ py, const double* __restrict__ cur,
@@ -797,13 +803,14 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
     const double* __restrict__ lgtab, int no, int n, int m, int npts,
     const int* __restrict__ nb_ptr, const int* __restrict__ nb_idx, int nwall, int wall_seg,
     double* __restrict__ out) {
+    static_assert(SUB >= 1 && SUB <= 64 && (SUB & (SUB - 1)) == 0, "SUB: a power of two up to 64");
     __shared__ double sc[5][kBlk];
-    const int sub = threadIdx.x & (kAsmSub - 1);
-    const int i = blockIdx.x * (kBlk / kAsmSub) + threadIdx.x / kAsmSub;
+    const int sub = threadIdx.x & (SUB - 1);
+    const int i = blockIdx.x * (kBlk / SUB) + threadIdx.x / SUB;
     const bool valid = i < npts;
     const int ii = valid ? i : 0;
     const double xi = x[ii], yi = y[ii];
-    // stokeslets and rotlets of every body, bodies q = sub mod kAsmSub on this
+    // stokeslets and rotlets of every body, bodies q = sub mod SUB on this
     // lane; with the cluster far field (nb_ptr), only the bodies near the
     // point's cluster (the far ones came with the expansions), their
     // logarithms formed here
@@ -811,7 +818,7 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
     const int base = 2 * no + 2 * n * m;
     if (nb_ptr != nullptr) {
         const int cc = ii < no ? ii / wall_seg : nwall + (ii - no) / n;
-        for (int t = nb_ptr[cc] + sub; t < nb_ptr[cc + 1]; t += kAsmSub) {
+        for (int t = nb_ptr[cc] + sub; t < nb_ptr[cc + 1]; t += SUB) {
             const int b = nb_idx[t];
             const double rx = xi - cx[b], ry = yi - cy[b];
             const double sto1 = den[base + 3 * b], sto2 = den[base + 3 * b + 1],
@@ -843,7 +850,7 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
         }
         __syncthreads();
         const int cnt = min(kBlk, m - b0);
-        for (int q = sub; q < cnt; q += kAsmSub) {
+        for (int q = sub; q < cnt; q += SUB) {
             const double rx = xi - sc[0][q], ry = yi - sc[1][q];
             const double sto1 = sc[2][q], sto2 = sc[3][q], rot = sc[4][q];
             const double rho2 = rx * rx + ry * ry;
@@ -861,11 +868,11 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
             }
         }
     }
-    // far-field partial sums of the source splits s = sub mod kAsmSub (wall
+    // far-field partial sums of the source splits s = sub mod SUB (wall
     // and body points may have different split counts)
     if (valid) {
         const int ns = i < no ? nsplit : nsplit_body;
-        for (int s = sub; s < ns; s += kAsmSub) {
+        for (int s = sub; s < ns; s += SUB) {
             const size_t o = static_cast<size_t>(s) * npts + i;
             if (COMP) {
                 dd_add(us, lus, part_u[o]);
@@ -879,7 +886,7 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
         }
     }
 #pragma unroll
-    for (int off = 1; off < kAsmSub; off <<= 1) {
+    for (int off = 1; off < SUB; off <<= 1) {
         if (COMP) {
             // (hi, lo) pairs: TwoSum of the values, corrections added
             double s, e;
@@ -1139,6 +1146,9 @@ static int tune_farfield(Context& c, bool fp32) {
 }
 
 bool farfield_fp32_pays(const Context& c) {
+    // a direct shape fixed by the caller: mixed precision uses the FP32 far
+    // field wherever its residual switch allows, whatever the timings
+    if (c.ff_force >= 0 && c.ff_force != kFmmIndex) return true;
     auto a = c.ff32_tuned_ms.find(tune_key(c, true));
     auto b = c.ff_tuned_ms.find(tune_key(c, false));
     if (a == c.ff32_tuned_ms.end() || b == c.ff_tuned_ms.end()) return true;
@@ -1272,18 +1282,25 @@ void apply_operator(Context& c, const double* x_dev, double* y_dev, int ibary, b
     // the assembly sums the near bodies' with logarithms of its own
     const TileNear tn = choice == kFmmIndex ? tilefmm_near_bodies(c) : TileNear{nullptr, nullptr, 0, 1};
     if (choice != kFmmIndex) stokes_logs(c);
-#define STOKES_ASM(COMP)                                                                       \
-    k_assemble<COMP><<<cdiv_int(np, kBlk / kAsmSub), kBlk, 0, s>>>(                             \
+#define STOKES_ASM(COMP, SUB)                                                                  \
+    k_assemble<COMP, SUB><<<cdiv_int(np, kBlk / SUB), kBlk, 0, s>>>(                            \
         g.x.p, g.y.p, g.px.p, g.py.p, g.cur.p, g.speed.p, c.ex.p, c.ey.p, g.cx.p, g.cy.p, x_dev, \
         c.part_u.p, c.part_v.p, c.part_cu.p, c.part_cv.p, nsplit, nsplit_body, L.tgt_ptr.p,      \
         L.tgt_ent.p,                                                                             \
         L.corr_u.p, L.corr_v.p, has_near ? 1 : 0, c.scal.p, nsig,                                 \
         g.m > 0 && choice != kFmmIndex ? c.stk_log.p : nullptr, g.no, g.n, g.m, np, tn.ptr,      \
         tn.idx, tn.nwall, tn.wall_seg, y_dev)
-    if (compensated)
-        STOKES_ASM(true);
-    else
-        STOKES_ASM(false);
+    if (choice == kFmmIndex) {
+        if (compensated)
+            STOKES_ASM(true, kAsmSubNear);
+        else
+            STOKES_ASM(false, kAsmSubNear);
+    } else {
+        if (compensated)
+            STOKES_ASM(true, kAsmSub);
+        else
+            STOKES_ASM(false, kAsmSub);
+    }
 #undef STOKES_ASM
     STOKES_LAUNCH_CHECK();
 }

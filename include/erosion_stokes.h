@@ -315,7 +315,9 @@ int erosion_spectral(int op, int64_t npts, int64_t rows, const double* in, const
 /* Test and measurement hook: use far-field launch shape `index` (0 ..
  * erosion_farfield_shape_count() - 1: targets per lane, grid size, shared
  * reciprocal) for every size instead of the autotuner's choice; -1 restores
- * the autotuner.  Returns 0, or -1 for an index out of range. */
+ * the autotuner.  With a direct shape fixed, mixed precision uses the FP32 far
+ * field wherever its residual switch allows (no timing decides).  Returns 0,
+ * or -1 for an index out of range. */
 int erosion_set_farfield_shape(int index);
 int erosion_farfield_shape_count(void);
 /* Far field of the completed operator (the reference's FMM switch ifmm,
