@@ -183,9 +183,14 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
     const double* __restrict__ phi_b, int nfw, int nfb, int no, int n, int first_curve,
     const double* __restrict__ S, double* __restrict__ out, int* __restrict__ bad) {
     constexpr int MAXNF = (MAXNQ - 3) / 2;
-    constexpr int kCh = 32;  // points per LDS chunk
-    constexpr int kEPT = (3 * MAXNF * MAXNF + kCoarseBlk - 1) / kCoarseBlk;
-    static_assert(4 * kCh * MAXNF <= MAXNQ * MAXNQ, "chunk buffers fit in G's memory");
+    // per-point functions whose projections give a body's stokeslet/rotlet
+    // columns (six) and constraint rows (three)
+    constexpr int kNX = 9;
+    constexpr int kCh = MAXNQ >= 100 ? 32 : 16;  // points per LDS chunk
+    constexpr int kLdMax = MAXNF > kNX ? MAXNF : kNX;
+    constexpr int kEPT = (3 * MAXNF * MAXNF + kNX * MAXNF + kCoarseBlk - 1) / kCoarseBlk;
+    static_assert(kCh * MAXNF + 4 * kCh * kLdMax <= MAXNQ * MAXNQ,
+                  "chunk buffers fit in G's memory");
     __shared__ double G[MAXNQ * MAXNQ];
     __shared__ double fcol[MAXNQ];
     __shared__ int perm[MAXNQ];
@@ -230,19 +235,34 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
     // f = curvature x speed / len.  Both depend on a + b only: three nf x nf
     // products over the curve's points, staged kCh points at a time in LDS
     // (in G's memory, unused until they are done) with each thread's entries
-    // in registers.
+    // in registers.  A body's stokeslet/rotlet columns and constraint rows are
+    // the projections sum_i phi_p(i) F_f(i) of nine functions of the point
+    // (r = z - c the offset from the body's centre, w = speed / len):
+    //   F_0 = (-log|r|^2 / 2 + rx^2 / |r|^2) / 4 pi, F_1 = F_3 = rx ry / (4 pi |r|^2),
+    //   F_2 = ry / |r|^2 (stokeslet x, y and rotlet in the x rows),
+    //   F_4 = (-log|r|^2 / 2 + ry^2 / |r|^2) / 4 pi, F_5 = -rx / |r|^2 (y rows),
+    //   F_6 = -w, F_7 = -y w, F_8 = x w (the constraint rows),
+    // staged as a fourth slab and summed in the same pass.
+    const int ldm = max(nf, kNX);
     double* Pc = G;            // [kCh][nf]: phi_q at the chunk's points
-    double* Mc = G + kCh * nf;  // [3][kCh][nf]: S_k + L_k phi_q
+    double* Mc = G + kCh * nf;  // [4][kCh][ldm]: S_k + L_k phi_q; F_f
     const int nn = nf * nf;
+    const int ntot = 3 * nn + (wall ? 0 : kNX * nf);
     double acc[kEPT];
-    int poff[kEPT], moff[kEPT];  // entry u: Pc column p, Mc offset k kCh nf + q
+    int poff[kEPT], moff[kEPT];  // entry u: Pc column p, Mc offset k kCh ldm + q
 #pragma unroll
     for (int u = 0; u < kEPT; ++u) {
         acc[u] = 0.0;
-        const int e = min(tid + u * kCoarseBlk, 3 * nn - 1);
-        const int k = e / nn, rem = e - k * nn, p = rem / nf, q = rem - p * nf;
-        poff[u] = p;
-        moff[u] = k * kCh * nf + q;
+        const int e = min(tid + u * kCoarseBlk, ntot - 1);
+        if (e < 3 * nn) {
+            const int k = e / nn, rem = e - k * nn, p = rem / nf, q = rem - p * nf;
+            poff[u] = p;
+            moff[u] = k * kCh * ldm + q;
+        } else {
+            const int f = (e - 3 * nn) / nf, p = e - 3 * nn - f * nf;
+            poff[u] = p;
+            moff[u] = 3 * kCh * ldm + f;
+        }
     }
     for (int i0 = 0; i0 < len; i0 += kCh) {
         __syncthreads();
@@ -261,16 +281,43 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
                                    Sc[static_cast<size_t>(2 * nf + q) * len + pt]);
             }
             Pc[i * nf + q] = ph;
-            Mc[(0 * kCh + i) * nf + q] = m0;
-            Mc[(1 * kCh + i) * nf + q] = m1;
-            Mc[(2 * kCh + i) * nf + q] = m2;
+            Mc[(0 * kCh + i) * ldm + q] = m0;
+            Mc[(1 * kCh + i) * ldm + q] = m1;
+            Mc[(2 * kCh + i) * ldm + q] = m2;
+        }
+        if (!wall) {
+            for (int i = tid; i < kCh; i += kCoarseBlk) {
+                const int pt = i0 + i;
+                double fv[kNX];
+#pragma unroll
+                for (int f = 0; f < kNX; ++f) fv[f] = 0.0;
+                if (pt < len) {
+                    const int gi = cv.beg + pt;
+                    const double qx = x[gi] - cxb, qy = y[gi] - cyb;
+                    const double r2 = __builtin_fma(qx, qx, qy * qy);
+                    const double ir2 = 1.0 / r2;
+                    const double lg = log(r2);
+                    const double w = speed[gi] / dlen;
+                    fv[0] = k4pi * (-0.5 * lg + qx * qx * ir2);
+                    fv[1] = k4pi * qx * qy * ir2;
+                    fv[2] = qy * ir2;
+                    fv[3] = fv[1];
+                    fv[4] = k4pi * (-0.5 * lg + qy * qy * ir2);
+                    fv[5] = -qx * ir2;
+                    fv[6] = -w;
+                    fv[7] = -y[gi] * w;
+                    fv[8] = x[gi] * w;
+                }
+#pragma unroll
+                for (int f = 0; f < kNX; ++f) Mc[(3 * kCh + i) * ldm + f] = fv[f];
+            }
         }
         __syncthreads();
-        // entries past 3 nn repeat the last one; they are never stored
+        // entries past ntot repeat the last one; they are never stored
         const int cnt = min(kCh, len - i0);
         for (int i = 0; i < cnt; ++i) {
             const double* pi = Pc + i * nf;
-            const double* mi = Mc + i * nf;
+            const double* mi = Mc + i * ldm;
 #pragma unroll
             for (int u = 0; u < kEPT; ++u) acc[u] = __builtin_fma(pi[poff[u]], mi[moff[u]], acc[u]);
         }
@@ -288,56 +335,27 @@ __global__ __launch_bounds__(kCoarseBlk) void k_pc_coarse(
                 if (wall) g = __builtin_fma(uv[a * nf + p], uv[2 * nf + b * nf + q], g);
                 G[(a * nf + p) * nq + b * nf + q] = g;
             }
+        } else if (e < ntot) {
+            const int f = (e - 3 * nn) / nf, p = e - 3 * nn - f * nf;
+            const double g = acc[u];
+            if (f < 6) {  // stokeslet x, y, rotlet column cc of row (a, p)
+                const int a = f / 3, cc = f - 3 * a;
+                G[(a * nf + p) * nq + nc + cc] = g;
+            } else if (f == 6) {  // -w: constraint row 0 on x, row 1 on y
+                G[nc * nq + p] = g;
+                G[nc * nq + nf + p] = 0.0;
+                G[(nc + 1) * nq + p] = 0.0;
+                G[(nc + 1) * nq + nf + p] = g;
+            } else {  // -y w on x, x w on y: constraint row 2
+                G[(nc + 2) * nq + (f - 7) * nf + p] = g;
+            }
         }
+    }
+    if (!wall && tid < 9) {  // the constraint unknowns' identity block
+        const int r = tid / 3, col = tid - 3 * r;
+        G[(nc + r) * nq + nc + col] = r == col ? 1.0 : 0.0;
     }
 
-    // the body's stokeslet/rotlet columns, constraint rows and identity
-    // block: one wave per entry, lanes over the curve's points
-    for (int e = wv; e < nq * nq; e += kCoarseWaves) {
-        const int r = e / nq, col = e - r * nq;
-        if (r < nc && col < nc) continue;
-        double s = 0.0;
-        if (r < nc) {
-            // stokeslet / rotlet column of the body's own centre
-            const int a = r / nf, p = r - a * nf, cc = col - nc;
-            const double* pp = phi + static_cast<size_t>(p) * len;
-            for (int i = lane; i < len; i += 64) {
-                const int gi = cv.beg + i;
-                const double qx = x[gi] - cxb, qy = y[gi] - cyb;
-                const double r2 = __builtin_fma(qx, qx, qy * qy);
-                const double ir2 = 1.0 / r2;
-                double t;
-                if (cc == 0)
-                    t = a == 0 ? k4pi * (-0.5 * log(r2) + qx * qx * ir2) : k4pi * qx * qy * ir2;
-                else if (cc == 1)
-                    t = a == 0 ? k4pi * qx * qy * ir2 : k4pi * (-0.5 * log(r2) + qy * qy * ir2);
-                else
-                    t = a == 0 ? qy * ir2 : -qx * ir2;
-                s = __builtin_fma(pp[i], t, s);
-            }
-            s = wave_sum(s);
-        } else if (col < nc) {
-            // constraint row of the body
-            const int cr = r - nc, b = col / nf, q = col - b * nf;
-            const double* pq = phi + static_cast<size_t>(q) * len;
-            for (int i = lane; i < len; i += 64) {
-                const int gi = cv.beg + i;
-                const double w = speed[gi] / dlen;
-                double t;
-                if (cr == 0)
-                    t = b == 0 ? -w : 0.0;
-                else if (cr == 1)
-                    t = b == 1 ? -w : 0.0;
-                else
-                    t = b == 0 ? -y[gi] * w : x[gi] * w;
-                s = __builtin_fma(t, pq[i], s);
-            }
-            s = wave_sum(s);
-        } else {
-            s = r == col ? 1.0 : 0.0;
-        }
-        if (lane == 0) G[e] = s;
-    }
     __syncthreads();
 
     // largest entry: a pivot below 1e-13 of it (or not finite) marks the
