@@ -63,7 +63,7 @@ FF_FETCH_BYTES = {"bodies100_n256": 10.5e6, "bodies400_n256_eroded50": 45.9e6}
 # (k_fmm_direct for body targets and for wall-segment targets: FETCH_SIZE x
 # 2.00, the factor a 256 MiB device copy gives, plus WRITE_SIZE), from
 # rocprofv3 counter passes on the workload's starting state (386 bodies)
-FMM_DIRECT_BYTES = {"bodies400_n256_eroded50": 56.39e6}
+FMM_DIRECT_BYTES = {"bodies400_n256_eroded50": 56.93e6}
 
 
 def log(*a):

@@ -114,6 +114,9 @@ __global__ __launch_bounds__(256) void k_fmm_p2m(const double* __restrict__ x,
                                                  double* __restrict__ mlog) {
     constexpr int kPer = kP / 4;
     const int c = blockIdx.x, w = threadIdx.x / 64, l = threadIdx.x % 64;
+    // issue priority over the direct sums' waves beside it on the same SIMDs:
+    // the expansion chain (P2M, M2M, M2L) sets the application's pace
+    __builtin_amdgcn_s_setprio(3);
     const double cx = cr[3 * c], cy = cr[3 * c + 1], iR = 1.0 / cr[3 * c + 2];
     const int off = cl[2 * c], len = cl[2 * c + 1];
     // (sum, correction) of the real and imaginary parts: [series][order][re, im][hi, lo]
@@ -236,6 +239,7 @@ __global__ __launch_bounds__(256) void k_fmm_m2m(const double* __restrict__ cr,
                                                 double* __restrict__ mp,
                                                 double* __restrict__ mlog) {
     static_assert(kGroupMax * kP == 256, "a lane per (child, order)");
+    __builtin_amdgcn_s_setprio(3);  // as in k_fmm_p2m
     __shared__ double sbin[kP][kP];
     __shared__ cplx A[kGroupMax][2][kP];
     __shared__ cplx tp[kGroupMax][kP + 1];
@@ -345,6 +349,7 @@ __global__ __launch_bounds__(256) void k_fmm_m2l(const double* __restrict__ cr,
                                                  const double* __restrict__ logd,
                                                  double* __restrict__ lc) {
     constexpr int NQO = NQ + (G ? 1 : 0);
+    __builtin_amdgcn_s_setprio(3);  // as in k_fmm_p2m
     __shared__ double RED[4][2][kP][2];
     const int t = blockIdx.x, q = blockIdx.y, w = threadIdx.x / 64, l = threadIdx.x % 64;
     const int j = l % 16, h = l / 16;
