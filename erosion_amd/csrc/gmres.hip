@@ -483,14 +483,21 @@ __global__ __launch_bounds__(kGivT) void k_dcgs(const double* __restrict__ hu,
     constexpr int kWaves = kGivT / 64;
     __shared__ double sA[kGivT];
     __shared__ double sP[kWaves][64];
+    __shared__ double sC[kGivT];  // the summed dots, u's rows then w's
+    // H a for j <= kFastHa: tiles of 64 rows, each column range in one pass
+    constexpr int kHaTiles = 3, kFastHa = 8 * kWaves;
+    static_assert(64 * kHaTiles > kFastHa, "rows 0..kFastHa in kHaTiles tiles");
+    __shared__ double sPT[kHaTiles][kWaves][64];
     if (st->done) return;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     // the dots pass's rows: 0..j (u's) and, unless last, 0..j+1 (w's); each
     // sum over the chunks in chunk order, loaded 16 at a time (indices
     // clamped, values past nchunk replaced by 0) so that it waits on L2 once
-    // per 16; thread t sums u's row t, then w's rows follow
+    // per 16; thread t sums u's row t, then w's rows follow.  Up to kGivT
+    // sums the block reads them back from LDS (sC), beyond that from ca, cb.
     const int nrows = last ? j + 1 : j + 2;
     const int nsum = (j + 1) + (last ? 0 : j + 2);
+    const bool staged = nsum <= kGivT;
     for (int t = tid; t < nsum; t += kGivT) {
         const bool wrow = t > j;
         const int r = wrow ? t - (j + 1) : t;
@@ -507,16 +514,35 @@ __global__ __launch_bounds__(kGivT) void k_dcgs(const double* __restrict__ hu,
             for (int e = 0; e < 16; ++e) v += tv[e];
         }
         (wrow ? cb : ca)[r] = v;
+        if (staged) sC[t] = v;
     }
     __syncthreads();
+    auto a_at = [&](int r) { return staged ? sC[r] : ca[r]; };
+    auto b_at = [&](int r) { return staged ? sC[j + 1 + r] : cb[r]; };
     double paa = 0.0, pab = 0.0;
     for (int r = tid; r < j; r += kGivT) {
-        paa = __builtin_fma(ca[r], ca[r], paa);
-        if (!last) pab = __builtin_fma(ca[r], cb[r], pab);
+        const double ar = a_at(r);
+        paa = __builtin_fma(ar, ar, paa);
+        if (!last) pab = __builtin_fma(ar, b_at(r), pab);
     }
-    const double saa = block_sum(paa, sA);
-    const double sab = block_sum(pab, sA);
-    const double rr2 = ca[j] - saa;
+    // a.a and a.b: one butterfly per wave for both, the waves' pairs through
+    // LDS in wave order
+    for (int off = 32; off > 0; off >>= 1) {
+        paa += __shfl_xor(paa, off);
+        pab += __shfl_xor(pab, off);
+    }
+    if (lane == 0) {
+        sP[0][wave] = paa;
+        sP[1][wave] = pab;
+    }
+    __syncthreads();
+    double saa = 0.0, sab = 0.0;
+    for (int w = 0; w < kWaves; ++w) {
+        saa += sP[0][w];
+        sab += sP[1][w];
+    }
+    __syncthreads();
+    const double rr2 = a_at(j) - saa;
     // the last iteration left w' = 0 (its Krylov space is invariant): column
     // j-1 completes with a zero subdiagonal, and its rotation ends the cycle
     const bool invariant = j >= 1 && st->snormw == 0.0;
@@ -529,7 +555,7 @@ __global__ __launch_bounds__(kGivT) void k_dcgs(const double* __restrict__ hu,
         // column j-1: its last vector was u, now r q_j + Q a
         double* colr = Hraw + static_cast<size_t>(j - 1) * ldh;
         const double est = st->snormw;
-        for (int i = tid; i < j; i += kGivT) colr[i] = __builtin_fma(est, ca[i], colr[i]);
+        for (int i = tid; i < j; i += kGivT) colr[i] = __builtin_fma(est, a_at(i), colr[i]);
         if (tid == 0) colr[j] = est * rn;
         __syncthreads();
         givens_column(H, ldh, q, colr, est * rn, st, tol, maxl);
@@ -541,10 +567,44 @@ __global__ __launch_bounds__(kGivT) void k_dcgs(const double* __restrict__ hu,
     // the columns w, w + 16, ... of the tile's range, 8 loads at a time, each
     // a coalesced column segment, and the waves' partials are added in wave
     // order.
-    const double gj = (cb[j] - sab) / rn;
+    const double gj = (b_at(j) - sab) / rn;
     double pgg = 0.0;
     double* colj = Hraw + static_cast<size_t>(j) * ldh;
-    for (int i0 = 0; i0 <= j; i0 += 64) {  // uniform over the block
+    if (j <= kFastHa) {
+        // every tile's columns in one load per (wave, t): all tiles' loads
+        // issued before any product, one barrier for all tiles; each row's
+        // products and sums as in the loop below, and wave w finishes tile w
+        // (g.g summed per tile, not per lane across tiles)
+#pragma unroll
+        for (int tl = 0; tl < kHaTiles; ++tl) {
+            const int i0 = 64 * tl, i = i0 + lane, c0 = max(0, i0 - 1) + wave;
+            double hv[8], av[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int cc = max(0, min(c0 + kWaves * t, j - 1));
+                const bool use = c0 + kWaves * t < j && cc >= i - 1 && i <= j;
+                const double h = Hraw[static_cast<size_t>(cc) * ldh + min(i, j)];
+                hv[t] = use ? h : 0.0;
+                av[t] = use ? a_at(cc) : 0.0;
+            }
+            double ha = 0.0;
+#pragma unroll
+            for (int t = 0; t < 8; ++t) ha = __builtin_fma(hv[t], av[t], ha);
+            sPT[tl][wave][lane] = ha;
+        }
+        __syncthreads();
+        const int i = 64 * wave + lane;
+        if (wave < kHaTiles && i <= j) {
+            double h = 0.0;
+#pragma unroll
+            for (int w = 0; w < kWaves; ++w) h += sPT[wave][w][lane];
+            const double gi = i < j ? b_at(i) : gj;
+            colj[i] = (gi - h) / rn;
+            pgg = __builtin_fma(gi, gi, pgg);
+        }
+        __syncthreads();
+    }
+    for (int i0 = 0; i0 <= j && j > kFastHa; i0 += 64) {  // uniform over the block
         const int i = i0 + lane;
         const int cbeg = max(0, i0 - 1);
         double ha = 0.0;
@@ -558,7 +618,7 @@ __global__ __launch_bounds__(kGivT) void k_dcgs(const double* __restrict__ hu,
                 const bool use = c0 + kWaves * t < j && cc >= i - 1 && i <= j;
                 const double h = Hraw[static_cast<size_t>(cc) * ldh + min(i, j)];
                 hv[t] = use ? h : 0.0;
-                av[t] = use ? ca[cc] : 0.0;
+                av[t] = use ? a_at(cc) : 0.0;
             }
 #pragma unroll
             for (int t = 0; t < 8; ++t) ha = __builtin_fma(hv[t], av[t], ha);
@@ -569,7 +629,7 @@ __global__ __launch_bounds__(kGivT) void k_dcgs(const double* __restrict__ hu,
             double h = 0.0;
 #pragma unroll
             for (int w = 0; w < kWaves; ++w) h += sP[w][lane];
-            const double gi = i < j ? cb[i] : gj;
+            const double gi = i < j ? b_at(i) : gj;
             colj[i] = (gi - h) / rn;
             pgg = __builtin_fma(gi, gi, pgg);
         }
@@ -577,7 +637,7 @@ __global__ __launch_bounds__(kGivT) void k_dcgs(const double* __restrict__ hu,
     }
     const double sgg = block_sum(pgg, sA);
     if (tid == 0) {
-        const double ww = cb[j + 1] - sgg;
+        const double ww = b_at(j + 1) - sgg;
         const double est = ww > 0.0 ? sqrt(ww) / rn : 0.0;
         st->snormw = est;
         st->inv_snormw = est > 0.0 ? 1.0 / est : 0.0;
