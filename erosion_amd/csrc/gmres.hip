@@ -199,14 +199,33 @@ __global__ __launch_bounds__(kBlk) void k_dots2(const double* __restrict__ V, si
             }
         }
     }
+    // the 2 kRows block sums together: a butterfly per wave for each, the
+    // waves' values through LDS and added in wave order (as block_sum), one
+    // barrier
+    constexpr int kW = kBlk / 64;
+    static_assert(2 * kRows * kW <= kBlk, "sums fit sh");
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
-        const double su = block_sum(au[r], sh);
-        const double sw = block_sum(aw[r], sh);
-        if (threadIdx.x == 0 && r0 + r < nrows) {
-            hu[static_cast<size_t>(c) * nrows + r0 + r] = su;
-            hw[static_cast<size_t>(c) * nrows + r0 + r] = sw;
+        for (int off = 32; off > 0; off >>= 1) {
+            au[r] += __shfl_xor(au[r], off);
+            aw[r] += __shfl_xor(aw[r], off);
         }
+    }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) {
+            sh[(2 * r) * kW + wv] = au[r];
+            sh[(2 * r + 1) * kW + wv] = aw[r];
+        }
+    }
+    __syncthreads();
+    const int t = threadIdx.x, r = t >> 1;
+    if (t < 2 * kRows && r0 + r < nrows) {
+        double v = 0.0;
+#pragma unroll
+        for (int w = 0; w < kW; ++w) v += sh[t * kW + w];
+        ((t & 1) ? hw : hu)[static_cast<size_t>(c) * nrows + r0 + r] = v;
     }
 }
 
