@@ -31,6 +31,7 @@ namespace {
 
 constexpr int kBlk = 256;
 constexpr int kRows = 4;       // Krylov rows per block in the dot kernel
+constexpr int kXcds = 8;       // MI355X: accelerator complex dies, one L2 each
 constexpr int kChunkIters = 16;
 // DCGS2's second pass runs as one kernel from this many blocks of 512
 // elements up (vectors of 2^17 doubles), as split partials below
@@ -170,17 +171,22 @@ __global__ void k_scale_row(double* __restrict__ v, size_t n, const GmresState* 
 
 // Partial dots of rows [r0, r0+kRows) of V with u and with w over one chunk:
 // hu[chunk][r], hw[chunk][r] (chunk-major, nrows per chunk, so that the sums
-// over chunks read consecutive rows in consecutive lanes).
+// over chunks read consecutive rows in consecutive lanes).  A 1-D grid of bx x
+// nchunk blocks (nchunk a multiple of kXcds): workgroups go to the XCDs in
+// turn by linear index, so block b runs on XCD b % kXcds, and the bx row
+// blocks of chunk c are all given to XCD c % kXcds, whose L2 then fetches
+// the chunk of u and w once instead of once per XCD.
 __global__ __launch_bounds__(kBlk) void k_dots2(const double* __restrict__ V, size_t npad,
                                                 const double* __restrict__ u,
                                                 const double* __restrict__ w, int nrows,
-                                                size_t chunk, int nchunk,
+                                                size_t chunk, int bx,
                                                 const GmresState* __restrict__ st,
                                                 double* __restrict__ hu, double* __restrict__ hw) {
     __shared__ double sh[kBlk];
     if (st->done) return;
-    const int r0 = blockIdx.x * kRows;
-    const int c = blockIdx.y;
+    const int xcd = static_cast<int>(blockIdx.x) % kXcds, k = static_cast<int>(blockIdx.x) / kXcds;
+    const int r0 = (k % bx) * kRows;
+    const int c = xcd + kXcds * (k / bx);
     const size_t e0 = static_cast<size_t>(c) * chunk;
     const size_t e1 = std::min(npad, e0 + chunk);
     double au[kRows], aw[kRows];
@@ -701,9 +707,10 @@ Plan dots_plan(int nrows, size_t npad) {
     p.bx = cdiv_int(nrows, kRows);
     int want = std::max(1, cdiv_int(1024, p.bx));
     int maxc = static_cast<int>(std::max<size_t>(1, npad / (2 * kBlk * 4)));
-    p.nchunk = std::min(std::min(want, maxc), 64);
+    // a multiple of kXcds (k_dots2's block mapping); trailing chunks past
+    // npad, if any, are empty and their partials 0
+    p.nchunk = static_cast<int>(round_up(std::min(std::min(want, maxc), 64), kXcds));
     p.chunk = round_up(cdiv_int(static_cast<int>(npad), p.nchunk), 2 * kBlk);
-    p.nchunk = static_cast<int>((npad + p.chunk - 1) / p.chunk);
     return p;
 }
 
@@ -719,7 +726,7 @@ static void dcgs_step(Context& c, int j, bool last, int ldh, double tol, int max
     double* w = last ? u : c.V.p + static_cast<size_t>(j + 1) * npad;
     const int nrows = last ? j + 1 : j + 2;
     Plan p = dots_plan(nrows, npad);
-    k_dots2<<<dim3(p.bx, p.nchunk), kBlk, 0, s>>>(c.V.p, npad, u, w, nrows, p.chunk, p.nchunk,
+    k_dots2<<<p.bx * p.nchunk, kBlk, 0, s>>>(c.V.p, npad, u, w, nrows, p.chunk, p.bx,
                                                   c.st.p, c.hpart.p, c.hpart2.p);
     STOKES_LAUNCH_CHECK();
     k_dcgs<<<1, kGivT, 0, s>>>(c.hpart.p, c.hpart2.p, p.nchunk, j, last ? 1 : 0, c.Hraw.p, c.H.p,
