@@ -942,9 +942,19 @@ __global__ __launch_bounds__(kBlk) void k_assemble(
         v += vs;
     }
     if (i < no) {
-        // the rank-one integral from its nsig unscaled parts, in order
+        // the rank-one integral from its nsig unscaled parts, in order,
+        // loaded 16 at a time (indices clamped, values past nsig read as 0)
         double sig = 0.0;
-        for (int j = 0; j < nsig; ++j) sig += scal[j];
+        for (int j0 = 0; j0 < nsig; j0 += 16) {
+            double tv[16];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const double v = scal[min(j0 + e, nsig - 1)];
+                tv[e] = j0 + e < nsig ? v : 0.0;
+            }
+#pragma unroll
+            for (int e = 0; e < 16; ++e) sig += tv[e];
+        }
         sig = sig * kTwoPi / static_cast<double>(no);
         if (COMP) {
             dd_add(u, lu, sig * px[i]);

@@ -739,8 +739,20 @@ __global__ __launch_bounds__(kApplyBlk) void k_pc_wall(int nfw, int no, int nseg
     const int tid = threadIdx.x;
     const int nc = 2 * nfw;
     if (tid < nc) {
+        // the segments' partials in segment order, loaded 16 at a time
+        // (indices clamped, values past nseg read as 0): one wait on memory
+        // per 16 segments instead of a chain of dependent loads
         double s = 0.0;
-        for (int g = 0; g < nseg; ++g) s += wpart[static_cast<size_t>(g) * kMaxNQ + tid];
+        for (int g0 = 0; g0 < nseg; g0 += 16) {
+            double tv[16];
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const double v = wpart[static_cast<size_t>(min(g0 + e, nseg - 1)) * kMaxNQ + tid];
+                tv[e] = g0 + e < nseg ? v : 0.0;
+            }
+#pragma unroll
+            for (int e = 0; e < 16; ++e) s += tv[e];
+        }
         L.cc[tid] = s;
     }
     __syncthreads();
